@@ -1,0 +1,273 @@
+/*
+ * vct.h -- C ABI of the MI355X voxel cone tracing passes.
+ *
+ * The reference (sfreed141/vct) has no plugin or FFI layer: each pass is a GL
+ * program bound and dispatched from Application::render
+ * (src/Application.cpp:196-1085).  Every entry point below replaces exactly one
+ * of those pass invocations (program bind + uniforms + image/texture bindings +
+ * glDispatchCompute / scene->draw + glMemoryBarrier).  The replaced call site is
+ * cited beside each declaration.
+ *
+ * Conventions
+ *  - All matrices are column-major float[16] (GLM / GLSL layout: m[col*4+row]).
+ *  - A context owns every device buffer.  Host pointers passed in are borrowed
+ *    for the duration of the call only.
+ *  - Every call returns an int status: VCT_OK (0) or a negative VCT_ERR_*.
+ *    Nothing aborts or throws across the ABI; vct_last_error() gives the text.
+ *  - One host thread per context.  All passes are enqueued in order on the
+ *    context's single HIP stream (stream order replaces glMemoryBarrier).
+ *    Only vct_download, vct_get_stats, vct_get_pass_times and vct_sync block.
+ *  - Parameter validation happens before any HIP call, so invalid arguments
+ *    are reported without touching the GPU.
+ */
+#ifndef VCT_H
+#define VCT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VCT_ABI_VERSION 1
+
+/* ---- status codes ------------------------------------------------------ */
+#define VCT_OK 0
+#define VCT_ERR_INVALID (-1)   /* bad argument (null, out of range, size)   */
+#define VCT_ERR_HIP (-2)       /* a HIP runtime call failed                 */
+#define VCT_ERR_STATE (-3)     /* pass called before its inputs exist       */
+#define VCT_ERR_NOMEM (-4)     /* device allocation failed                  */
+#define VCT_ERR_NODEVICE (-5)  /* no usable gfx950 device                   */
+
+/* ---- storage formats of the radiance mip chain ---------------------------- */
+#define VCT_STORE_FP16 0  /* rgba16f every level (the reference's use_rgba16f)   */
+#define VCT_STORE_FP32 1  /* rgba32f every level                                 */
+#define VCT_STORE_FP8 2   /* rgba16f levels 0-2, FP8 E4M3 + per-4^3-brick fp32
+                              scale on levels >= 3                               */
+
+/* ---- light types (src/Scene.h:13-34, shaders/voxelize.frag:29-30) -------- */
+#define VCT_LIGHT_POINT 0u
+#define VCT_LIGHT_DIRECTIONAL 1u
+
+/* ---- what vct_download / vct_upload address ------------------------------ */
+#define VCT_BUF_ALBEDO 0     /* u32 packed RGBA8 [dim^3], x fastest (voxelColor)  */
+#define VCT_BUF_NORMAL 1     /* u32 packed RGBA8 [dim^3] (voxelNormal)            */
+#define VCT_BUF_RADIANCE 2   /* float4 [s^3] of mip `level` (decoded to f32); for
+                                 anisotropic levels >= 1 pass l | (face << 4),
+                                 face 0..5 = +x,-x,+y,-y,+z,-z                  */
+#define VCT_BUF_SHADOWMAP 3  /* float [size^2] depth of light `level`            */
+#define VCT_BUF_GBUFFER 4    /* vct_gbuffer_texel [width*height]                 */
+#define VCT_BUF_INDIRECT 5   /* float4 [width*height]: (indirect.rgb*occlusion,
+                                 occlusion) -- phong.frag:489 debugIndirect view  */
+#define VCT_BUF_SPECULAR 6   /* float4 [width*height]: specular cone rgba          */
+#define VCT_BUF_LITMASK 7    /* u32 [dim^3/32] bit per voxel lit by any light     */
+#define VCT_BUF_RADIANCE_RAW 8 /* raw stored bytes of mip `level` (layout-native) */
+#define VCT_BUF_COMPOSED 9   /* float4 [width*height]: composed indirect term     */
+
+/* Grid description: the VCT resource owner (src/Application.h:107-156). */
+typedef struct vct_grid_desc {
+    int dim;           /* voxelDim: power of two, 4..512                        */
+    int levels;        /* voxelLevels: clamped to log2(dim)+1 like VCT::remake  */
+    float center[3];   /* VCT::center                                           */
+    float vmin[3];     /* VCT::min                                              */
+    float vmax[3];     /* VCT::max                                              */
+    int storage;       /* VCT_STORE_*                                           */
+    int anisotropic;   /* 0: isotropic mips (reference); 1: 6-direction mips    */
+    int device;        /* HIP device ordinal                                    */
+} vct_grid_desc;
+
+/* Triangle soup in object space.  Replaces the VBO/IBO drawn by Mesh::draw
+ * (src/Graphics/Mesh.cpp:268-374) with one actor per object. */
+typedef struct vct_mesh {
+    int num_tris;
+    int num_objects;
+    const float *positions;    /* [num_tris][3][3] object-space vertices          */
+    const float *normals;      /* [num_tris][3][3] object-space vertex normals     */
+    const float *tangents;     /* [num_tris][3]   object-space tangent per triangle
+                                  (Mesh.cpp:174-197 rule with uv (0,0),(1,0),(0,1)) */
+    const uint32_t *albedo;    /* [num_tris] RGBA8 (R in low byte) = diffuseMap texel */
+    const uint32_t *object;    /* [num_tris] actor index in [0, num_objects)       */
+    const float *roughness;    /* [num_objects] roughness map value, or NULL (0.5)  */
+} vct_mesh;
+
+/* One light, replacing the std140 SSBO record of Light::writeSSBO
+ * (src/Scene.h:13-34, src/Scene.cpp:64-76) plus its light-space matrix
+ * (src/Application.cpp:207-210) and shadow-map size. */
+typedef struct vct_light {
+    float position[3];
+    float range;
+    float direction[3];
+    float intensity;
+    float color[3];
+    uint32_t type;          /* VCT_LIGHT_POINT | VCT_LIGHT_DIRECTIONAL          */
+    uint32_t enabled;
+    uint32_t shadow_caster; /* renders a shadow map, shadows voxelize, injects    */
+    uint32_t shadow_size;   /* shadow map is shadow_size^2 fp32                   */
+    uint32_t perspective;   /* 0: ortho light (w == 1), 1: perspective (w-divide)  */
+    float ls[16];           /* light projection * light view                       */
+    float ls_inverse[16];   /* inverse(ls), the lsInverse uniform of
+                               injectRadiance.comp (src/Application.cpp:804-805)   */
+} vct_light;
+
+/* Voxelize pass uniforms (src/Application.cpp:689-733). */
+typedef struct vct_voxelize_params {
+    float mvp_x[16], mvp_y[16], mvp_z[16]; /* dominant-axis projections            */
+    int axis_override;      /* -1 = dominant axis, 0/1/2 forces x/y/z             */
+    int lighting;           /* voxelizeLighting                                    */
+    int atomic_max;         /* voxelizeAtomicMax: must be 1 (the order-independent
+                               mode); the CAS running average is order dependent  */
+    int msaa;               /* 1: 4x MSAA coverage (ConservativeRasterizeMode MSAA)
+                               0: single sample at the pixel centre (OFF)          */
+    float eye[3];
+} vct_voxelize_params;
+
+/* transferVoxels + injectRadiance uniforms (src/Application.cpp:757-837). */
+typedef struct vct_inject_params {
+    float voxel_set_opacity; /* voxelSetOpacity (Application.h:77), 0.5          */
+    uint32_t light_mask;     /* bit i: light i injects (shadow casters only)       */
+} vct_inject_params;
+
+/* Mip chain (src/Application.cpp:877-921, shaders/filterRadiance.comp). */
+typedef struct vct_mip_params {
+    int kernel_mode;   /* 0 = KERNEL_BOX2 (the only live mode)                     */
+    int albedo_chain;  /* reserved: the albedo (voxelColor) chain is not built     */
+} vct_mip_params;
+
+/* Camera for the G-buffer (depth pre-pass, src/Application.cpp:936-965). */
+typedef struct vct_camera {
+    int width, height;
+    float eye[3];
+    float view[16];
+    float projection[16];
+    int cull_back;   /* GL_CULL_FACE with CCW front faces                          */
+} vct_camera;
+
+/* Cone settings: VCTSettings (src/Application.h:28-34). */
+typedef struct vct_cone_settings {
+    int steps;
+    float cone_angle;          /* radians                                          */
+    float bias;
+    float cone_initial_height;
+    float lod_offset;
+} vct_cone_settings;
+
+/* Trace pass uniforms (src/Application.cpp:982-1048, shaders/phong.frag:455-512). */
+typedef struct vct_trace_params {
+    vct_cone_settings diffuse;    /* {16, 60deg, 1.0, 1.0, 0.5}                    */
+    vct_cone_settings specular;   /* {32, 30deg, 1.7, 0.5, 0.1}                    */
+    int enable_reflections;       /* the specular cone                             */
+    int cone_angle_from_roughness;/* vctSpecularConeAngleFromRoughness             */
+    int draw_occlusion;           /* drawOcclusion                                 */
+    float ambient_scale;          /* ambientScale                                  */
+    float reflect_scale;          /* reflectScale                                  */
+    float eye[3];
+} vct_trace_params;
+
+/* One G-buffer texel, 64 bytes (four float4). */
+typedef struct vct_gbuffer_texel {
+    float position[3];  uint32_t albedo;     /* world position, RGBA8 albedo       */
+    float normal[3];    float roughness;     /* interpolated normal (unnormalized) */
+    float tangent[3];   uint32_t covered;    /* TBN column 0; 1 if a surface hit   */
+    float bitangent[3]; uint32_t triangle;   /* TBN column 1; visible triangle id  */
+} vct_gbuffer_texel;
+
+/* VoxelizeInfo (src/Application.h:194-198). */
+typedef struct vct_stats {
+    uint32_t total_voxel_fragments;
+    uint32_t unique_voxels;
+    uint32_t max_fragments_per_voxel;
+    uint32_t lit_voxels;
+} vct_stats;
+
+/* Per-pass GPU milliseconds of the last frame (GLBufferedTimer brackets,
+ * src/Graphics/GLTimer.h:49-87 and Application.cpp:213-1077). */
+typedef struct vct_pass_times {
+    double shadowmap_ms;
+    double voxelize_ms;
+    double radiance_ms;   /* normalize + inject                                    */
+    double fill_ms;
+    double mipmap_ms;
+    double gbuffer_ms;
+    double render_ms;     /* cone trace                                            */
+    double learned_ms;    /* learned gather (when used instead of render)          */
+} vct_pass_times;
+
+typedef struct vct_ctx vct_ctx;
+
+/* ---- library ------------------------------------------------------------ */
+int vct_abi_version(void);
+/* Text of the last error on this context (ctx may be NULL for errors raised
+ * by vct_create).  No reference counterpart: GL reports through LOG_ERROR and
+ * the debug callback (src/Graphics/GLHelper.cpp:45-59). */
+const char *vct_last_error(const vct_ctx *ctx);
+
+/* ---- context lifetime: VCT::VCT + VCT::make (src/Application.h:109-148) --- */
+int vct_create(vct_ctx **out, const vct_grid_desc *desc);
+/* VCT::remake (src/Application.h:118-129): levels clamped to [1, log2(dim)+1]
+ * with a warning in vct_last_error (status stays VCT_OK). */
+int vct_remake(vct_ctx *ctx, int dim, int levels);
+void vct_destroy(vct_ctx *ctx);
+int vct_get_grid(const vct_ctx *ctx, vct_grid_desc *out);
+/* VCT::center/min/max assignment, e.g. the voxelTrackCamera snap
+ * (src/Application.cpp:187-191). */
+int vct_set_extent(vct_ctx *ctx, const float center[3], const float vmin[3],
+                   const float vmax[3]);
+
+/* ---- scene: Scene::addActor / Mesh upload / Scene::addLight --------------- */
+/* Uploads geometry (src/Scene.cpp:31-36 draws every actor; Mesh.cpp:268-374). */
+int vct_set_scene(vct_ctx *ctx, const vct_mesh *mesh);
+/* Per-actor model matrices, the `model` uniform set per actor by Scene::draw
+ * (src/Scene.cpp:33; Transform::getMatrix, src/Transform.h:98-104).  The
+ * normal matrix mat3(transpose(inverse(model))) of voxelize.vert:18 and
+ * phong.vert:37 is derived here once per actor. */
+int vct_set_transforms(vct_ctx *ctx, const float *model, int num_objects);
+/* Light SSBO at binding 3 (src/Scene.cpp:38-76). */
+int vct_set_lights(vct_ctx *ctx, const vct_light *lights, int num_lights);
+
+/* ---- passes (order of Application::render) -------------------------------- */
+/* Shadow map of one light: simple.vert + depth only (src/Application.cpp:212-233). */
+int vct_shadowmap(vct_ctx *ctx, int light_index);
+/* Voxelize, GS path: voxelize.vert/geom/frag drawn by scene->draw
+ * (src/Application.cpp:666-754). Clears albedo/normal first (:686-687). */
+int vct_voxelize(vct_ctx *ctx, const vct_voxelize_params *p);
+/* transferVoxels.comp then injectRadiance.comp (src/Application.cpp:757-837),
+ * fused: lit-mask scatter over shadow texels, then one per-voxel pass that
+ * normalizes, rewrites opacity and writes radiance level 0. */
+int vct_normalize_inject(vct_ctx *ctx, const vct_inject_params *p);
+/* voxelFillHoles.comp + glCopyImageSubData (src/Application.cpp:839-875). */
+int vct_fill_holes(vct_ctx *ctx);
+/* filterRadiance.comp loop over levels (src/Application.cpp:877-921). */
+int vct_mip(vct_ctx *ctx, const vct_mip_params *p);
+/* Depth pre-pass + per-pixel surface attributes consumed by phong.frag
+ * (src/Application.cpp:936-965, shaders/phong.vert:36-59). */
+int vct_gbuffer(vct_ctx *ctx, const vct_camera *cam);
+/* Cone-traced indirect light: traceCone + the 6-cone gather + specular cone
+ * (src/Application.cpp:967-1067, shaders/phong.frag:135-180,455-512). */
+int vct_trace(vct_ctx *ctx, const vct_trace_params *p);
+
+/* ---- readback, timing ----------------------------------------------------- */
+int vct_sync(vct_ctx *ctx);
+/* VoxelizeInfo readback (src/Overlay.cpp:106-110). Blocks. */
+int vct_get_stats(vct_ctx *ctx, vct_stats *out);
+/* Per-pass times of the most recent pass calls (GLBufferedTimer::getTime,
+ * src/Graphics/GLTimer.h:80-87). Blocks. Zero when timing is disabled. */
+int vct_get_pass_times(vct_ctx *ctx, vct_pass_times *out);
+/* 1: bracket every pass with HIP events (default); 0: no events. */
+int vct_set_timing(vct_ctx *ctx, int enable);
+/* Byte size vct_download/vct_upload expect for (what, level); 0 if invalid. */
+size_t vct_buffer_bytes(const vct_ctx *ctx, int what, int level);
+/* Copies a context buffer to host memory in the documented linear layout.
+ * Blocks. No reference counterpart beyond glGetTextureImage
+ * (src/Application.cpp:306-310). */
+int vct_download(vct_ctx *ctx, int what, int level, void *host, size_t bytes);
+/* Overwrites a context buffer from host memory (test and replay hook:
+ * VCT_BUF_ALBEDO, VCT_BUF_NORMAL, VCT_BUF_RADIANCE level 0,
+ * VCT_BUF_SHADOWMAP, VCT_BUF_GBUFFER, VCT_BUF_LITMASK). */
+int vct_upload(vct_ctx *ctx, int what, int level, const void *host, size_t bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VCT_H */

@@ -1,0 +1,85 @@
+/*
+ * vct_app.h -- headless application layer above the pass ABI (vct.h).
+ *
+ * Replaces the part of the reference's Application (src/Application.{h,cpp})
+ * that feeds the voxel cone tracing passes: the Settings/VCTSettings defaults
+ * (Application.h:28-103), the light and voxel matrices (Application.cpp:
+ * 200-210, 689-692), the scene (here: seeded synthetic scenes instead of
+ * Sponza, Application.cpp:93-136) and the per-frame pass order of
+ * Application::render (Application.cpp:196-1085).  It calls the GPU only
+ * through vct.h, as the reference's Application calls GL.
+ */
+#ifndef VCT_APP_H
+#define VCT_APP_H
+
+#include "vct.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct vct_app vct_app;
+
+/* Workload description.  Fields left at 0 (or -1 for the flags) take the
+ * defaults of the chosen scene:
+ *  scene 1: room [-1,1]^3 + 2 seeded boxes (36 tris), 64^3 x 7 levels,
+ *           1 directional light with a 512^2 shadow map, 256x256 camera,
+ *           6 diffuse cones.
+ *  scene 2: 2000 boxes + 64 icospheres (subdivision 4), 351,680 tris, moving
+ *           rigidly every frame, 256^3 x 8 levels with 6-direction mips,
+ *           1 directional + 4 point lights with 1024^2 shadow maps,
+ *           1920x1080 camera, 6 diffuse + 1 specular cone. */
+typedef struct vct_app_config {
+    int scene;          /* 1 or 2                                               */
+    uint64_t seed;      /* 0 = scene index                                      */
+    int dim, levels;    /* voxel grid                                           */
+    int storage;        /* VCT_STORE_*, -1 = default (FP16)                     */
+    int anisotropic;    /* -1 = default                                         */
+    int width, height;  /* camera                                               */
+    int shadow_size;    /* per light                                            */
+    int lighting;       /* voxelizeLighting, -1 = default (1)                   */
+    int msaa;           /* 4x MSAA voxelization, -1 = default (1)               */
+    int reflections;    /* specular cone, -1 = default                          */
+    int fill_holes;     /* voxelFillHoles, -1 = default (0, Application.h:95)   */
+    int device;         /* HIP device; -1 builds the scene and uniforms only
+                           (no GPU context: render calls fail)                */
+} vct_app_config;
+
+int vct_app_default_config(int scene, vct_app_config *out);
+int vct_app_create(vct_app **out, const vct_app_config *cfg);
+void vct_app_destroy(vct_app *app);
+const char *vct_app_last_error(const vct_app *app);
+vct_ctx *vct_app_ctx(vct_app *app);
+int vct_app_get_config(const vct_app *app, vct_app_config *out);
+
+/* Scene sizes and the object-space geometry the app uploaded. */
+int vct_app_scene_sizes(const vct_app *app, int *num_tris, int *num_objects, int *num_lights);
+/* Any pointer may be NULL.  Returns 1 if the scene has per-object roughness. */
+int vct_app_get_mesh(const vct_app *app, float *pos, float *nrm, float *tan, uint32_t *albedo, uint32_t *obj,
+                     float *rough);
+
+/* Everything the passes of frame `frame` receive (uniforms and matrices);
+ * any pointer may be NULL. model: [num_objects][16]; lights: [num_lights]. */
+int vct_app_frame_state(vct_app *app, int frame, float *model, vct_light *lights, vct_voxelize_params *vox,
+                        vct_inject_params *inj, vct_mip_params *mip, vct_camera *cam, vct_trace_params *trace);
+
+/* One frame in the order of Application::render: transforms, shadow maps,
+ * voxelize, normalize+inject, [fill holes], mip, G-buffer, trace. */
+int vct_app_render(vct_app *app, int frame);
+/* The subset of a frame up to and including the mip chain. */
+int vct_app_build_radiance(vct_app *app, int frame);
+
+/* GLM-convention host math (column-major), exported for tests. */
+void vct_math_ortho(float l, float r, float b, float t, float n, float f, float out[16]);
+void vct_math_lookat(const float eye[3], const float center[3], const float up[3], float out[16]);
+void vct_math_perspective(float fovy, float aspect, float n, float f, float out[16]);
+int vct_math_inverse4(const float m[16], float out[16]);
+void vct_math_mul4(const float a[16], const float b[16], float out[16]);
+void vct_math_voxel_projections(const float center[3], const float vmin[3], const float vmax[3], float mvp_x[16],
+                                float mvp_y[16], float mvp_z[16]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VCT_APP_H */

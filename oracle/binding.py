@@ -1,0 +1,311 @@
+"""ctypes binding of the oracle (oracle/_build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product (vct_amd/).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "_build" / "liboracle.so"
+_lib = None
+
+F16 = C.c_float * 16
+F3 = C.c_float * 3
+
+
+class Scene(C.Structure):
+    _fields_ = [("ntri", C.c_int), ("nobj", C.c_int), ("pos", C.c_void_p), ("nrm", C.c_void_p),
+                ("tan", C.c_void_p), ("albedo", C.c_void_p), ("obj", C.c_void_p), ("rough", C.c_void_p),
+                ("model", C.c_void_p), ("nmat", C.c_void_p)]
+
+
+class VoxParams(C.Structure):
+    _fields_ = [("mvp_x", C.c_void_p), ("mvp_y", C.c_void_p), ("mvp_z", C.c_void_p), ("axis_override", C.c_int),
+                ("lighting", C.c_int), ("msaa", C.c_int), ("dim", C.c_int)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("total_voxel_fragments", C.c_uint32), ("unique_voxels", C.c_uint32),
+                ("max_fragments_per_voxel", C.c_uint32), ("lit_voxels", C.c_uint32)]
+
+
+class Cone(C.Structure):
+    _fields_ = [("steps", C.c_int), ("angle", C.c_float), ("bias", C.c_float), ("h0", C.c_float),
+                ("lod_offset", C.c_float)]
+
+
+class TraceParams(C.Structure):
+    _fields_ = [("diffuse", Cone), ("specular", Cone), ("enable_reflections", C.c_int),
+                ("angle_from_roughness", C.c_int), ("draw_occlusion", C.c_int), ("has_roughness", C.c_int),
+                ("ambient_scale", C.c_float), ("reflect_scale", C.c_float), ("eye", F3), ("dim", C.c_int),
+                ("nlevels", C.c_int), ("center", F3), ("vmin", F3), ("vmax", F3), ("anisotropic", C.c_int)]
+
+
+class Camera(C.Structure):
+    _fields_ = [("width", C.c_int), ("height", C.c_int), ("eye", F3), ("view", F16), ("projection", F16),
+                ("cull_back", C.c_int)]
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        L = C.CDLL(str(LIB))
+        vp = C.c_void_p
+        sig = {
+            "or_mul4": (None, [vp, vp, vp]),
+            "or_ortho": (None, [C.c_float] * 6 + [vp]),
+            "or_lookat": (None, [vp, vp, vp, vp]),
+            "or_perspective": (None, [C.c_float] * 4 + [vp]),
+            "or_inverse4": (C.c_int, [vp, vp]),
+            "or_normal_matrix": (None, [vp, vp]),
+            "or_mat4_vec4": (None, [vp, vp, vp]),
+            "or_voxel_axis_projections": (None, [vp] * 6),
+            "or_float_to_half": (C.c_uint16, [C.c_float]),
+            "or_half_to_float": (C.c_float, [C.c_uint16]),
+            "or_float_to_fp8": (C.c_uint8, [C.c_float]),
+            "or_fp8_to_float": (C.c_float, [C.c_uint8]),
+            "or_pack_unorm4x8": (C.c_uint32, [vp]),
+            "or_shadowmap": (None, [C.POINTER(Scene), vp, vp]),
+            "or_voxelize": (None, [C.POINTER(Scene), vp, C.c_int, vp, C.POINTER(VoxParams), vp, vp,
+                                   C.POINTER(Stats)]),
+            "or_normalize_inject": (None, [vp, C.c_int, vp, vp, vp, vp, C.c_int, vp, C.c_uint32, C.c_float,
+                                           C.c_int, vp, vp, C.POINTER(Stats)]),
+            "or_fill_holes": (None, [vp, vp, C.c_int, C.c_int]),
+            "or_mip": (None, [vp, C.c_int, C.c_int, C.c_int]),
+            "or_mip_aniso": (None, [vp, vp, C.c_int, C.c_int, C.c_int]),
+            "or_fp8_quantize_level": (None, [vp, C.c_int]),
+            "or_gbuffer": (None, [C.POINTER(Scene), C.POINTER(Camera), vp]),
+            "or_trace": (None, [vp, C.c_int, vp, vp, C.POINTER(TraceParams), vp, vp]),
+            "or_trace64": (None, [vp, C.c_int, vp, vp, C.POINTER(TraceParams), vp, vp]),
+            "or_trace_cone": (None, [vp, vp, C.c_int, C.c_int, vp, vp, vp, C.POINTER(Cone), vp]),
+            "or_sample": (None, [vp, C.c_int, C.c_int, vp, C.c_float, vp]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray) -> C.c_void_p:
+    return C.c_void_p(a.ctypes.data)
+
+
+# ---------------------------------------------------------------- math
+def ortho(l, r, b, t, n, f):
+    o = np.zeros(16, np.float32)
+    lib().or_ortho(l, r, b, t, n, f, _p(o))
+    return o
+
+
+def lookat(eye, center, up):
+    o = np.zeros(16, np.float32)
+    e, c, u = (np.asarray(v, np.float32) for v in (eye, center, up))
+    lib().or_lookat(_p(e), _p(c), _p(u), _p(o))
+    return o
+
+
+def perspective(fovy, aspect, n, f):
+    o = np.zeros(16, np.float32)
+    lib().or_perspective(fovy, aspect, n, f, _p(o))
+    return o
+
+
+def inverse4(m):
+    m = np.ascontiguousarray(m, np.float32)
+    o = np.zeros(16, np.float32)
+    ok = lib().or_inverse4(_p(m), _p(o))
+    return o if ok else None
+
+
+def mul4(a, b):
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    o = np.zeros(16, np.float32)
+    lib().or_mul4(_p(a), _p(b), _p(o))
+    return o
+
+
+def normal_matrices(model: np.ndarray) -> np.ndarray:
+    model = np.ascontiguousarray(model, np.float32).reshape(-1, 16)
+    out = np.zeros((model.shape[0], 9), np.float32)
+    for i in range(model.shape[0]):
+        lib().or_normal_matrix(_p(model[i]), _p(out[i]))
+    return out
+
+
+def voxel_projections(center, vmin, vmax):
+    c, a, b = (np.asarray(v, np.float32) for v in (center, vmin, vmax))
+    x, y, z = (np.zeros(16, np.float32) for _ in range(3))
+    lib().or_voxel_axis_projections(_p(c), _p(a), _p(b), _p(x), _p(y), _p(z))
+    return x, y, z
+
+
+def half_round(x: np.ndarray) -> np.ndarray:
+    f = np.vectorize(lambda v: lib().or_half_to_float(lib().or_float_to_half(float(v))), otypes=[np.float32])
+    return f(np.asarray(x, np.float32))
+
+
+def fp8_encode(x: float) -> int:
+    return int(lib().or_float_to_fp8(float(x)))
+
+
+def fp8_decode(c: int) -> float:
+    return float(lib().or_fp8_to_float(int(c)))
+
+
+# ---------------------------------------------------------------- scene
+class SceneArrays:
+    """Keeps the numpy arrays alive behind an or_scene."""
+
+    def __init__(self, mesh: dict, model: np.ndarray):
+        self.pos = np.ascontiguousarray(mesh["pos"], np.float32)
+        self.nrm = np.ascontiguousarray(mesh["nrm"], np.float32)
+        self.tan = np.ascontiguousarray(mesh["tan"], np.float32)
+        self.albedo = np.ascontiguousarray(mesh["albedo"], np.uint32)
+        self.obj = np.ascontiguousarray(mesh["obj"], np.uint32)
+        self.rough = None if mesh.get("rough") is None else np.ascontiguousarray(mesh["rough"], np.float32)
+        self.model = np.ascontiguousarray(model, np.float32).reshape(-1, 16)
+        self.nmat = normal_matrices(self.model)
+        self.s = Scene(int(self.albedo.size), int(self.model.shape[0]), _p(self.pos), _p(self.nrm), _p(self.tan),
+                       _p(self.albedo), _p(self.obj), None if self.rough is None else _p(self.rough),
+                       _p(self.model), _p(self.nmat))
+
+
+def _light_array(lights):
+    from vct_amd import _abi as abi  # identical layout to or_light
+    return (abi.Light * len(lights))(*lights)
+
+
+def shadowmap(sc: SceneArrays, light) -> np.ndarray:
+    from vct_amd import _abi as abi
+    n = int(light.shadow_size)
+    out = np.empty((n, n), np.float32)
+    L = abi.Light.from_buffer_copy(light)
+    lib().or_shadowmap(C.byref(sc.s), C.byref(L), _p(out))
+    return out
+
+
+def voxelize(sc: SceneArrays, lights, shadowmaps, vox, dim: int, msaa=None, lighting=None, axis_override=None):
+    mx = np.array(list(vox.mvp_x), np.float32)
+    my = np.array(list(vox.mvp_y), np.float32)
+    mz = np.array(list(vox.mvp_z), np.float32)
+    p = VoxParams(_p(mx), _p(my), _p(mz), vox.axis_override if axis_override is None else axis_override,
+                  vox.lighting if lighting is None else lighting, vox.msaa if msaa is None else msaa, dim)
+    la = _light_array(lights)
+    sms = [np.ascontiguousarray(s, np.float32) if s is not None else None for s in shadowmaps]
+    ptrs = (C.c_void_p * max(1, len(sms)))(*[(s.ctypes.data if s is not None else None) for s in sms])
+    alb = np.zeros(dim ** 3, np.uint32)
+    nrm = np.zeros(dim ** 3, np.uint32)
+    st = Stats()
+    lib().or_voxelize(C.byref(sc.s), C.cast(la, C.c_void_p), len(lights), C.cast(ptrs, C.c_void_p), C.byref(p),
+                      _p(alb), _p(nrm), C.byref(st))
+    return alb.reshape(dim, dim, dim), nrm.reshape(dim, dim, dim), st
+
+
+def normalize_inject(albedo: np.ndarray, dim, center, vmin, vmax, lights, shadowmaps, light_mask=0xffffffff,
+                     opacity=0.5, fp32=False):
+    alb = np.ascontiguousarray(albedo, np.uint32).copy().reshape(-1)
+    c, a, b = (np.asarray(v, np.float32) for v in (center, vmin, vmax))
+    la = _light_array(lights)
+    sms = [np.ascontiguousarray(s, np.float32) if s is not None else None for s in shadowmaps]
+    ptrs = (C.c_void_p * max(1, len(sms)))(*[(s.ctypes.data if s is not None else None) for s in sms])
+    rad = np.zeros((dim ** 3, 4), np.float32)
+    lm = np.zeros((dim ** 3 + 31) // 32, np.uint32)
+    st = Stats()
+    lib().or_normalize_inject(_p(alb), dim, _p(c), _p(a), _p(b), C.cast(la, C.c_void_p), len(lights),
+                              C.cast(ptrs, C.c_void_p), light_mask, opacity, 1 if fp32 else 0, _p(rad), _p(lm),
+                              C.byref(st))
+    return alb.reshape(dim, dim, dim), rad.reshape(dim, dim, dim, 4), lm, st
+
+
+def fill_holes(rad0: np.ndarray, fp32=False) -> np.ndarray:
+    src = np.ascontiguousarray(rad0, np.float32)
+    dim = src.shape[0]
+    dst = np.empty_like(src)
+    lib().or_fill_holes(_p(src), _p(dst), dim, 1 if fp32 else 0)
+    return dst
+
+
+def mip(rad0: np.ndarray, nlevels: int, fp32=False) -> list:
+    dim = rad0.shape[0]
+    levels = [np.ascontiguousarray(rad0, np.float32)]
+    for l in range(1, nlevels):
+        s = max(1, dim >> l)
+        levels.append(np.zeros((s, s, s, 4), np.float32))
+    ptrs = (C.c_void_p * nlevels)(*[a.ctypes.data for a in levels])
+    lib().or_mip(C.cast(ptrs, C.c_void_p), dim, nlevels, 1 if fp32 else 0)
+    return levels
+
+
+def mip_aniso(rad0: np.ndarray, nlevels: int, fp32=False) -> list:
+    """Returns [None, faces_l1, faces_l2, ...] where faces_l = list of 6 volumes."""
+    dim = rad0.shape[0]
+    src = np.ascontiguousarray(rad0, np.float32)
+    faces = [None]
+    rows = [None]
+    for l in range(1, nlevels):
+        s = max(1, dim >> l)
+        fl = [np.zeros((s, s, s, 4), np.float32) for _ in range(6)]
+        faces.append(fl)
+        rows.append((C.c_void_p * 6)(*[a.ctypes.data for a in fl]))
+    top = (C.c_void_p * nlevels)(*([None] + [C.cast(r, C.c_void_p).value for r in rows[1:]]))
+    lib().or_mip_aniso(_p(src), C.cast(top, C.c_void_p), dim, nlevels, 1 if fp32 else 0)
+    return faces
+
+
+def fp8_quantize(level: np.ndarray) -> np.ndarray:
+    a = np.ascontiguousarray(level, np.float32).copy()
+    lib().or_fp8_quantize_level(_p(a), a.shape[0])
+    return a
+
+
+def gbuffer(sc: SceneArrays, cam) -> np.ndarray:
+    c = Camera.from_buffer_copy(cam)
+    out = np.zeros((cam.height, cam.width, 16), np.float32)
+    lib().or_gbuffer(C.byref(sc.s), C.byref(c), _p(out))
+    return out
+
+
+def trace_params(tr, dim, nlevels, center, vmin, vmax, has_roughness, anisotropic=0) -> TraceParams:
+    d, s = tr.diffuse, tr.specular
+    return TraceParams(Cone(d.steps, d.cone_angle, d.bias, d.cone_initial_height, d.lod_offset),
+                       Cone(s.steps, s.cone_angle, s.bias, s.cone_initial_height, s.lod_offset),
+                       tr.enable_reflections, tr.cone_angle_from_roughness, tr.draw_occlusion,
+                       1 if has_roughness else 0, tr.ambient_scale, tr.reflect_scale, F3(*tr.eye), dim, nlevels,
+                       F3(*center), F3(*vmin), F3(*vmax), anisotropic)
+
+
+def _pyramid_ptrs(levels, aniso_faces):
+    lp = (C.c_void_p * len(levels))(*[np.ascontiguousarray(a).ctypes.data for a in levels])
+    if aniso_faces is None:
+        return lp, None, None
+    rows = [None] + [(C.c_void_p * 6)(*[f.ctypes.data for f in aniso_faces[l]]) for l in range(1, len(levels))]
+    top = (C.c_void_p * len(levels))(*([None] + [C.cast(r, C.c_void_p).value for r in rows[1:]]))
+    return lp, top, rows
+
+
+def trace(gbuf: np.ndarray, levels, params: TraceParams, aniso_faces=None, fp64=False):
+    g = np.ascontiguousarray(gbuf, np.float32)
+    npix = g.size // 16
+    lp, top, _keep = _pyramid_ptrs(levels, aniso_faces)
+    dt = np.float64 if fp64 else np.float32
+    ind = np.zeros((npix, 4), dt)
+    spec = np.zeros((npix, 4), dt)
+    fn = lib().or_trace64 if fp64 else lib().or_trace
+    fn(_p(g), npix, C.cast(lp, C.c_void_p), None if top is None else C.cast(top, C.c_void_p), C.byref(params),
+       _p(ind), _p(spec))
+    return ind, spec

@@ -1,0 +1,270 @@
+"""vct_amd -- MI355X-native voxel cone tracing.
+
+The hot path of the reference renderer (sfreed141/vct): dominant-axis
+voxelization, shadow-map radiance injection, the radiance mip chain and the
+cone-traced indirect light, as hand-written gfx950 HIP kernels behind the C ABI
+of include/vct.h (libvct.so), driven by the headless application layer of
+include/vct_app.h (libvct_app.so).
+
+This module is a thin ctypes binding for tests and benchmarks.  It performs no
+computation itself: every pass runs in libvct.so on the GPU, and loading fails
+loudly when the built library is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+from . import _abi as abi
+
+__all__ = ["VCTError", "load_library", "load_app_library", "Context", "App", "abi"]
+
+LIB_DIR = Path(__file__).resolve().parent / "_lib"
+_lib = None
+_app = None
+
+
+class VCTError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+def load_library() -> C.CDLL:
+    """dlopen libvct.so (built by vct_amd/build.py). Raises if it is missing."""
+    global _lib
+    if _lib is None:
+        path = LIB_DIR / "libvct.so"
+        if not path.exists():
+            raise ImportError(f"{path} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+        lib = C.CDLL(str(path), mode=C.RTLD_GLOBAL)
+        abi.bind(lib, abi.VCT_FUNCS)
+        _lib = lib
+    return _lib
+
+
+def load_app_library() -> C.CDLL:
+    global _app
+    if _app is None:
+        load_library()
+        path = LIB_DIR / "libvct_app.so"
+        if not path.exists():
+            raise ImportError(f"{path} is missing: build it first")
+        lib = C.CDLL(str(path))
+        abi.bind(lib, abi.APP_FUNCS)
+        _app = lib
+    return _app
+
+
+def _ptr(a: np.ndarray) -> C.c_void_p:
+    return C.c_void_p(a.ctypes.data)
+
+
+class Context:
+    """Owns one vct_ctx (the VCT resources of src/Application.h:107-156)."""
+
+    def __init__(self, dim=64, levels=7, center=(0, 0, 0), vmin=(-1, -1, -1), vmax=(1, 1, 1),
+                 storage=abi.VCT_STORE_FP16, anisotropic=0, device=0, handle=None, owner=None):
+        self.lib = load_library()
+        self._owner = owner
+        if handle is not None:
+            self.h = C.c_void_p(handle)
+            self._own = False
+            return
+        d = abi.GridDesc(dim, levels, (C.c_float * 3)(*center), (C.c_float * 3)(*vmin), (C.c_float * 3)(*vmax),
+                         storage, anisotropic, device)
+        h = C.c_void_p()
+        rc = self.lib.vct_create(C.byref(h), C.byref(d))
+        if rc:
+            raise VCTError(rc, self.lib.vct_last_error(None).decode())
+        self.h = h
+        self._own = True
+
+    def close(self):
+        if getattr(self, "_own", False) and self.h:
+            self.lib.vct_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, rc: int) -> None:
+        if rc:
+            raise VCTError(rc, (self.lib.vct_last_error(self.h) or b"").decode())
+
+    # --- state ----------------------------------------------------------
+    def grid(self) -> abi.GridDesc:
+        d = abi.GridDesc()
+        self.check(self.lib.vct_get_grid(self.h, C.byref(d)))
+        return d
+
+    def set_scene(self, pos, nrm, tan, albedo, obj, nobj, rough=None):
+        pos = np.ascontiguousarray(pos, np.float32)
+        nrm = np.ascontiguousarray(nrm, np.float32)
+        tan = np.ascontiguousarray(tan, np.float32)
+        albedo = np.ascontiguousarray(albedo, np.uint32)
+        obj = np.ascontiguousarray(obj, np.uint32)
+        rough_a = None if rough is None else np.ascontiguousarray(rough, np.float32)
+        m = abi.Mesh(int(albedo.size), int(nobj), _ptr(pos), _ptr(nrm), _ptr(tan), _ptr(albedo), _ptr(obj),
+                     None if rough_a is None else _ptr(rough_a))
+        self.check(self.lib.vct_set_scene(self.h, C.byref(m)))
+
+    def set_transforms(self, model: np.ndarray):
+        model = np.ascontiguousarray(model, np.float32)
+        self.check(self.lib.vct_set_transforms(self.h, _ptr(model), model.size // 16))
+
+    def set_lights(self, lights):
+        arr = (abi.Light * len(lights))(*lights)
+        self.check(self.lib.vct_set_lights(self.h, arr, len(lights)))
+
+    # --- passes (Application::render order) ---------------------------------
+    def shadowmap(self, i):
+        self.check(self.lib.vct_shadowmap(self.h, i))
+
+    def voxelize(self, p: abi.VoxelizeParams):
+        self.check(self.lib.vct_voxelize(self.h, C.byref(p)))
+
+    def normalize_inject(self, p: abi.InjectParams):
+        self.check(self.lib.vct_normalize_inject(self.h, C.byref(p)))
+
+    def fill_holes(self):
+        self.check(self.lib.vct_fill_holes(self.h))
+
+    def mip(self, p: abi.MipParams | None = None):
+        p = p or abi.MipParams(0, 0)
+        self.check(self.lib.vct_mip(self.h, C.byref(p)))
+
+    def gbuffer(self, cam: abi.Camera):
+        self.check(self.lib.vct_gbuffer(self.h, C.byref(cam)))
+
+    def trace(self, p: abi.TraceParams):
+        self.check(self.lib.vct_trace(self.h, C.byref(p)))
+
+    def sync(self):
+        self.check(self.lib.vct_sync(self.h))
+
+    def set_timing(self, on: bool):
+        self.check(self.lib.vct_set_timing(self.h, 1 if on else 0))
+
+    def stats(self) -> abi.Stats:
+        s = abi.Stats()
+        self.check(self.lib.vct_get_stats(self.h, C.byref(s)))
+        return s
+
+    def pass_times(self) -> abi.PassTimes:
+        t = abi.PassTimes()
+        self.check(self.lib.vct_get_pass_times(self.h, C.byref(t)))
+        return t
+
+    # --- readback ----------------------------------------------------------
+    def nbytes(self, what: int, level: int = 0) -> int:
+        return int(self.lib.vct_buffer_bytes(self.h, what, level))
+
+    def download(self, what: int, level: int = 0, dtype=np.uint8) -> np.ndarray:
+        n = self.nbytes(what, level)
+        if n == 0:
+            raise VCTError(abi.VCT_ERR_INVALID, f"buffer {what} level {level} does not exist")
+        out = np.empty(n, np.uint8)
+        self.check(self.lib.vct_download(self.h, what, level, _ptr(out), n))
+        return out.view(dtype)
+
+    def upload(self, what: int, level: int, data: np.ndarray):
+        data = np.ascontiguousarray(data)
+        self.check(self.lib.vct_upload(self.h, what, level, _ptr(data), data.nbytes))
+
+    def albedo(self) -> np.ndarray:
+        d = self.grid().dim
+        return self.download(abi.VCT_BUF_ALBEDO, 0, np.uint32).reshape(d, d, d)
+
+    def normal(self) -> np.ndarray:
+        d = self.grid().dim
+        return self.download(abi.VCT_BUF_NORMAL, 0, np.uint32).reshape(d, d, d)
+
+    def radiance(self, level: int = 0, face: int = 0) -> np.ndarray:
+        d = self.grid().dim
+        s = max(1, d >> level)
+        return self.download(abi.VCT_BUF_RADIANCE, level | (face << 4), np.float32).reshape(s, s, s, 4)
+
+    def shadow(self, light: int) -> np.ndarray:
+        a = self.download(abi.VCT_BUF_SHADOWMAP, light, np.float32)
+        n = int(round(a.size ** 0.5))
+        return a.reshape(n, n)
+
+    def litmask(self) -> np.ndarray:
+        return self.download(abi.VCT_BUF_LITMASK, 0, np.uint32)
+
+
+class App:
+    """Owns one vct_app (the headless Application of include/vct_app.h)."""
+
+    def __init__(self, scene=1, **overrides):
+        self.lib = load_app_library()
+        cfg = abi.AppConfig()
+        rc = self.lib.vct_app_default_config(scene, C.byref(cfg))
+        if rc:
+            raise VCTError(rc, f"unknown scene {scene}")
+        for k, v in overrides.items():
+            setattr(cfg, k, v)
+        h = C.c_void_p()
+        rc = self.lib.vct_app_create(C.byref(h), C.byref(cfg))
+        if rc:
+            raise VCTError(rc, (self.lib.vct_app_last_error(None) or b"").decode())
+        self.h = h
+        ctx_handle = self.lib.vct_app_ctx(h)
+        self.ctx = Context(handle=ctx_handle, owner=self) if ctx_handle else None
+        self.cfg = abi.AppConfig()
+        self.lib.vct_app_get_config(h, C.byref(self.cfg))
+
+    def close(self):
+        if getattr(self, "h", None):
+            if self.ctx is not None:
+                self.ctx.h = None
+            self.lib.vct_app_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, rc: int):
+        if rc:
+            raise VCTError(rc, (self.lib.vct_app_last_error(self.h) or b"").decode())
+
+    def sizes(self):
+        nt, no, nl = C.c_int(), C.c_int(), C.c_int()
+        self.check(self.lib.vct_app_scene_sizes(self.h, C.byref(nt), C.byref(no), C.byref(nl)))
+        return nt.value, no.value, nl.value
+
+    def mesh(self) -> dict:
+        nt, no, _ = self.sizes()
+        pos = np.empty((nt, 3, 3), np.float32)
+        nrm = np.empty((nt, 3, 3), np.float32)
+        tan = np.empty((nt, 3), np.float32)
+        alb = np.empty(nt, np.uint32)
+        obj = np.empty(nt, np.uint32)
+        rough = np.empty(no, np.float32)
+        has = self.lib.vct_app_get_mesh(self.h, _ptr(pos), _ptr(nrm), _ptr(tan), _ptr(alb), _ptr(obj), _ptr(rough))
+        return dict(pos=pos, nrm=nrm, tan=tan, albedo=alb, obj=obj, nobj=no, rough=rough if has == 1 else None)
+
+    def frame_state(self, frame: int = 0) -> dict:
+        _, no, nl = self.sizes()
+        model = np.empty((no, 16), np.float32)
+        lights = (abi.Light * nl)()
+        vox, inj, mip = abi.VoxelizeParams(), abi.InjectParams(), abi.MipParams()
+        cam, tr = abi.Camera(), abi.TraceParams()
+        self.check(self.lib.vct_app_frame_state(self.h, frame, _ptr(model), lights, C.byref(vox), C.byref(inj),
+                                                C.byref(mip), C.byref(cam), C.byref(tr)))
+        return dict(model=model, lights=list(lights), vox=vox, inj=inj, mip=mip, cam=cam, trace=tr)
+
+    def render(self, frame: int = 0):
+        self.check(self.lib.vct_app_render(self.h, frame))
+
+    def build_radiance(self, frame: int = 0):
+        self.check(self.lib.vct_app_build_radiance(self.h, frame))

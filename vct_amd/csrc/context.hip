@@ -1,0 +1,835 @@
+// context.hip -- the C ABI of include/vct.h: context lifetime, scene/light
+// upload, pass dispatch on one HIP stream, timing, readback.
+//
+// The reference keeps its state in GL objects owned by VCT (voxel textures,
+// src/Application.h:107-156), Application (shadow FBO, timers) and function
+// statics (warpmap, fill-holes temporaries).  Here one vct_ctx owns every
+// device buffer; all passes are enqueued on the context's stream, whose order
+// replaces the reference's glMemoryBarrier calls.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "vct_internal.h"
+
+using namespace vctk;
+
+enum PassId { P_SHADOW = 0, P_VOXELIZE, P_RADIANCE, P_FILL, P_MIP, P_GBUFFER, P_TRACE, P_LEARNED, P_COUNT };
+
+struct TimedPair {
+    int pass;
+    hipEvent_t a, b;
+};
+
+struct vct_ctx {
+    vct_grid_desc desc{};
+    hipStream_t stream = nullptr;
+    std::string err;
+    // scene
+    int ntri = 0, nobj = 0, has_rough = 0, have_transforms = 0;
+    float *d_pos = nullptr, *d_nrm = nullptr, *d_tan = nullptr, *d_rough = nullptr;
+    uint32_t *d_albedo = nullptr, *d_obj = nullptr;
+    float *d_model = nullptr, *d_nmat = nullptr;
+    // lights
+    std::vector<vct_light> lights;
+    std::vector<float *> d_shadow;
+    std::vector<DevLight> hlights;
+    DevLight *d_lights = nullptr;
+    // voxel grids
+    uint32_t *d_grid_albedo = nullptr, *d_grid_normal = nullptr, *d_litmask = nullptr;
+    char *d_pyr = nullptr;
+    float *d_scales = nullptr;
+    size_t pyr_bytes = 0, scale_floats = 0;
+    PyrDesc pyr{};
+    LevelInfo *d_lv = nullptr;
+    void *d_fill_tmp = nullptr;
+    vct_stats *d_stats = nullptr;
+    bool have_voxels = false, have_radiance = false, have_mips = false;
+    // raster scratch
+    RasterScratch rs{};
+    // camera images
+    int gw = 0, gh = 0;
+    vct_gbuffer_texel *d_gbuf = nullptr;
+    uint64_t *d_vis = nullptr;
+    float4 *d_ind = nullptr, *d_spec = nullptr;
+    bool have_gbuf = false;
+    // staging for downloads
+    float4 *d_stage = nullptr;
+    size_t stage_bytes = 0;
+    // timing
+    bool timing = true;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<TimedPair> pending;
+    vct_pass_times last{};
+};
+
+static thread_local std::string g_err;
+
+static int fail(vct_ctx *c, int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    else g_err = buf;
+    return code;
+}
+
+#define HIPCHK(c, expr)                                                                                        \
+    do {                                                                                                       \
+        hipError_t _e = (expr);                                                                                \
+        if (_e != hipSuccess) return fail((c), VCT_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(_e)); \
+    } while (0)
+
+template <class T>
+static hipError_t dalloc(T **p, size_t bytes) {
+    *p = nullptr;
+    if (bytes == 0) return hipSuccess;
+    return hipMalloc((void **)p, bytes);
+}
+template <class T>
+static void dfree(T *&p) {
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+static bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+static int ilog2(int v) {
+    int l = 0;
+    while ((1 << (l + 1)) <= v) ++l;
+    return l;
+}
+static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+// ---------------------------------------------------------------- timing
+static hipEvent_t take_event(vct_ctx *c) {
+    if (!c->ev_pool.empty()) {
+        hipEvent_t e = c->ev_pool.back();
+        c->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+struct PassTimer {
+    vct_ctx *c;
+    int pass;
+    hipEvent_t a = nullptr;
+    PassTimer(vct_ctx *c_, int p) : c(c_), pass(p) {
+        if (c->timing) {
+            a = take_event(c);
+            if (a) (void)hipEventRecord(a, c->stream);
+        }
+    }
+    void stop() {
+        if (!a) return;
+        hipEvent_t b = take_event(c);
+        if (b && hipEventRecord(b, c->stream) == hipSuccess) c->pending.push_back({pass, a, b});
+        else c->ev_pool.push_back(a);
+        a = nullptr;
+    }
+    ~PassTimer() { stop(); }
+};
+
+// ---------------------------------------------------------------- pyramid
+static void free_grids(vct_ctx *c) {
+    dfree(c->d_grid_albedo);
+    dfree(c->d_grid_normal);
+    dfree(c->d_litmask);
+    dfree(c->d_pyr);
+    dfree(c->d_scales);
+    dfree(c->d_lv);
+    dfree(c->d_fill_tmp);
+    c->have_voxels = c->have_radiance = c->have_mips = false;
+}
+
+static int alloc_grids(vct_ctx *c) {
+    const int dim = c->desc.dim;
+    const size_t nv = (size_t)dim * dim * dim;
+    HIPCHK(c, dalloc(&c->d_grid_albedo, nv * 4));
+    HIPCHK(c, dalloc(&c->d_grid_normal, nv * 4));
+    HIPCHK(c, dalloc(&c->d_litmask, (nv + 31) / 32 * 4));
+    HIPCHK(c, hipMemsetAsync(c->d_grid_albedo, 0, nv * 4, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_grid_normal, 0, nv * 4, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_litmask, 0, (nv + 31) / 32 * 4, c->stream));
+    PyrDesc &P = c->pyr;
+    memset(&P, 0, sizeof(P));
+    P.dim = dim;
+    P.nlevels = c->desc.levels;
+    P.storage = c->desc.storage;
+    P.aniso = c->desc.anisotropic;
+    size_t off = 0, soff = 0;
+    size_t base_off[VCT_MAX_LEVELS][6], stage_off[VCT_MAX_LEVELS][6], scale_off[VCT_MAX_LEVELS][6];
+    for (int l = 0; l < P.nlevels; ++l) {
+        LevelInfo &L = P.lv[l];
+        L.size = std::max(1, dim >> l);
+        L.nb = (L.size + 3) / 4;
+        L.fmt = P.storage == VCT_STORE_FP32 ? 1 : ((P.storage == VCT_STORE_FP8 && l >= 3) ? 2 : 0);
+        L.nfaces = (P.aniso && l >= 1) ? 6 : 1;
+        const size_t nvox = (size_t)L.nb * L.nb * L.nb * 64;
+        const size_t bpv = L.fmt == 1 ? 16 : (L.fmt == 2 ? 4 : 8);
+        for (int f = 0; f < L.nfaces; ++f) {
+            base_off[l][f] = off;
+            off += align256(nvox * bpv);
+            if (L.fmt == 2) {
+                stage_off[l][f] = off;
+                off += align256(nvox * 8);
+                scale_off[l][f] = soff;
+                soff += (size_t)L.nb * L.nb * L.nb;
+            } else {
+                stage_off[l][f] = base_off[l][f];
+                scale_off[l][f] = 0;
+            }
+        }
+    }
+    c->pyr_bytes = off;
+    c->scale_floats = soff;
+    HIPCHK(c, dalloc(&c->d_pyr, off));
+    HIPCHK(c, hipMemsetAsync(c->d_pyr, 0, off, c->stream));
+    if (soff) {
+        HIPCHK(c, dalloc(&c->d_scales, soff * 4));
+        HIPCHK(c, hipMemsetAsync(c->d_scales, 0, soff * 4, c->stream));
+    }
+    for (int l = 0; l < P.nlevels; ++l) {
+        LevelInfo &L = P.lv[l];
+        for (int f = 0; f < 6; ++f) {
+            const int ff = f < L.nfaces ? f : 0;
+            L.base[f] = c->d_pyr + base_off[l][ff];
+            L.stage[f] = c->d_pyr + stage_off[l][ff];
+            L.scale[f] = c->d_scales ? c->d_scales + scale_off[l][ff] : nullptr;
+        }
+    }
+    HIPCHK(c, dalloc(&c->d_lv, sizeof(LevelInfo) * P.nlevels));
+    HIPCHK(c, hipMemcpyAsync(c->d_lv, P.lv, sizeof(LevelInfo) * P.nlevels, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    P.d_lv = c->d_lv;
+    return VCT_OK;
+}
+
+static int validate_desc(const vct_grid_desc *d, vct_grid_desc *out, std::string *warn) {
+    if (!d) return fail(nullptr, VCT_ERR_INVALID, "vct_create: desc is NULL");
+    if (!is_pow2(d->dim) || d->dim < 8 || d->dim > 512)
+        return fail(nullptr, VCT_ERR_INVALID, "voxel dim %d must be a power of two in [8, 512]", d->dim);
+    if (d->storage < VCT_STORE_FP16 || d->storage > VCT_STORE_FP8)
+        return fail(nullptr, VCT_ERR_INVALID, "unknown storage format %d", d->storage);
+    for (int k = 0; k < 3; ++k)
+        if (!(d->vmax[k] > d->vmin[k]))
+            return fail(nullptr, VCT_ERR_INVALID, "grid extent max[%d] must exceed min[%d]", k, k);
+    *out = *d;
+    // VCT::remake: voxelLevels = clamp(levels, 0, log2(dim) + 1) with a warning
+    const int maxl = std::min(ilog2(d->dim) + 1, VCT_MAX_LEVELS);
+    int lv = d->levels < 1 ? 1 : (d->levels > maxl ? maxl : d->levels);
+    if (lv != d->levels && warn) {
+        char b[160];
+        snprintf(b, sizeof(b), "Attempted remaking VCT with invalid number of levels, clamped %d to %d", d->levels,
+                 lv);
+        *warn = b;
+    }
+    out->levels = lv;
+    out->anisotropic = d->anisotropic ? 1 : 0;
+    return VCT_OK;
+}
+
+// ---------------------------------------------------------------- ABI
+extern "C" {
+
+int vct_abi_version(void) { return VCT_ABI_VERSION; }
+
+const char *vct_last_error(const vct_ctx *c) { return c ? c->err.c_str() : g_err.c_str(); }
+
+int vct_create(vct_ctx **out, const vct_grid_desc *desc) {
+    if (!out) return fail(nullptr, VCT_ERR_INVALID, "vct_create: out is NULL");
+    *out = nullptr;
+    vct_grid_desc d;
+    std::string warn;
+    int rc = validate_desc(desc, &d, &warn);
+    if (rc) return rc;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return fail(nullptr, VCT_ERR_NODEVICE, "no HIP device available");
+    if (d.device < 0 || d.device >= ndev)
+        return fail(nullptr, VCT_ERR_INVALID, "device %d out of range (have %d)", d.device, ndev);
+    if (hipSetDevice(d.device) != hipSuccess) return fail(nullptr, VCT_ERR_NODEVICE, "hipSetDevice failed");
+    vct_ctx *c = new vct_ctx();
+    c->desc = d;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return fail(nullptr, VCT_ERR_HIP, "hipStreamCreate failed");
+    }
+    if (hipMalloc((void **)&c->d_stats, sizeof(vct_stats)) != hipSuccess ||
+        hipMemset(c->d_stats, 0, sizeof(vct_stats)) != hipSuccess) {
+        vct_destroy(c);
+        return fail(nullptr, VCT_ERR_NOMEM, "stats allocation failed");
+    }
+    rc = alloc_grids(c);
+    if (rc) {
+        g_err = c->err;
+        vct_destroy(c);
+        return rc;
+    }
+    c->err = warn;
+    *out = c;
+    return VCT_OK;
+}
+
+void vct_destroy(vct_ctx *c) {
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    free_grids(c);
+    dfree(c->d_pos);
+    dfree(c->d_nrm);
+    dfree(c->d_tan);
+    dfree(c->d_rough);
+    dfree(c->d_albedo);
+    dfree(c->d_obj);
+    dfree(c->d_model);
+    dfree(c->d_nmat);
+    for (auto &p : c->d_shadow) dfree(p);
+    dfree(c->d_lights);
+    dfree(c->rs.rec);
+    dfree(c->rs.attr);
+    dfree(c->rs.counts);
+    dfree(c->rs.offs);
+    dfree(c->rs.chunk);
+    if (c->rs.scan_tmp) (void)hipFree(c->rs.scan_tmp);
+    dfree(c->d_gbuf);
+    dfree(c->d_vis);
+    dfree(c->d_ind);
+    dfree(c->d_spec);
+    dfree(c->d_stage);
+    dfree(c->d_stats);
+    for (auto &tp : c->pending) {
+        (void)hipEventDestroy(tp.a);
+        (void)hipEventDestroy(tp.b);
+    }
+    for (auto e : c->ev_pool) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int vct_remake(vct_ctx *c, int dim, int levels) {
+    if (!c) return fail(nullptr, VCT_ERR_INVALID, "ctx is NULL");
+    vct_grid_desc d = c->desc, nd;
+    d.dim = dim;
+    d.levels = levels;
+    std::string warn;
+    int rc = validate_desc(&d, &nd, &warn);
+    if (rc) {
+        c->err = g_err;
+        return rc;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    free_grids(c);
+    c->desc = nd;
+    rc = alloc_grids(c);
+    if (rc) return rc;
+    c->err = warn;
+    return VCT_OK;
+}
+
+int vct_get_grid(const vct_ctx *c, vct_grid_desc *out) {
+    if (!c || !out) return VCT_ERR_INVALID;
+    *out = c->desc;
+    return VCT_OK;
+}
+
+int vct_set_extent(vct_ctx *c, const float center[3], const float vmin[3], const float vmax[3]) {
+    if (!c || !center || !vmin || !vmax) return fail(c, VCT_ERR_INVALID, "vct_set_extent: NULL argument");
+    for (int k = 0; k < 3; ++k)
+        if (!(vmax[k] > vmin[k])) return fail(c, VCT_ERR_INVALID, "extent max[%d] must exceed min[%d]", k, k);
+    for (int k = 0; k < 3; ++k) {
+        c->desc.center[k] = center[k];
+        c->desc.vmin[k] = vmin[k];
+        c->desc.vmax[k] = vmax[k];
+    }
+    return VCT_OK;
+}
+
+static int ensure_raster(vct_ctx *c, size_t slots) {
+    if (slots <= c->rs.cap_slots) return VCT_OK;
+    dfree(c->rs.rec);
+    dfree(c->rs.attr);
+    dfree(c->rs.counts);
+    dfree(c->rs.offs);
+    if (c->rs.scan_tmp) (void)hipFree(c->rs.scan_tmp);
+    c->rs.scan_tmp = nullptr;
+    HIPCHK(c, dalloc(&c->rs.rec, slots * 4 * sizeof(int4)));
+    HIPCHK(c, dalloc(&c->rs.attr, slots * 8 * sizeof(float4)));
+    HIPCHK(c, dalloc(&c->rs.counts, (slots + 1) * sizeof(uint64_t)));
+    HIPCHK(c, dalloc(&c->rs.offs, (slots + 1) * sizeof(uint64_t)));
+    c->rs.scan_tmp_bytes = raster_scan_tmp_bytes(slots);
+    HIPCHK(c, hipMalloc(&c->rs.scan_tmp, std::max<size_t>(c->rs.scan_tmp_bytes, 256)));
+    if (!c->rs.chunk) {
+        c->rs.chunk_cap = (size_t)1 << 20;  // chunk starts for 64M items; beyond that a full search is used
+        HIPCHK(c, dalloc(&c->rs.chunk, c->rs.chunk_cap * sizeof(uint32_t)));
+    }
+    c->rs.cap_slots = slots;
+    return VCT_OK;
+}
+
+int vct_set_scene(vct_ctx *c, const vct_mesh *m) {
+    if (!c || !m) return fail(c, VCT_ERR_INVALID, "vct_set_scene: NULL argument");
+    if (m->num_tris < 0 || m->num_objects <= 0)
+        return fail(c, VCT_ERR_INVALID, "vct_set_scene: num_tris %d / num_objects %d invalid", m->num_tris,
+                    m->num_objects);
+    if (m->num_tris > (1 << 28)) return fail(c, VCT_ERR_INVALID, "vct_set_scene: too many triangles");
+    if (m->num_tris > 0 && (!m->positions || !m->normals || !m->tangents || !m->albedo || !m->object))
+        return fail(c, VCT_ERR_INVALID, "vct_set_scene: NULL geometry array");
+    for (int t = 0; t < m->num_tris; ++t)
+        if (m->object[t] >= (uint32_t)m->num_objects)
+            return fail(c, VCT_ERR_INVALID, "vct_set_scene: triangle %d references object %u >= %d", t,
+                        m->object[t], m->num_objects);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    dfree(c->d_pos);
+    dfree(c->d_nrm);
+    dfree(c->d_tan);
+    dfree(c->d_albedo);
+    dfree(c->d_obj);
+    dfree(c->d_rough);
+    dfree(c->d_model);
+    dfree(c->d_nmat);
+    const size_t T = (size_t)m->num_tris;
+    HIPCHK(c, dalloc(&c->d_pos, T * 9 * 4));
+    HIPCHK(c, dalloc(&c->d_nrm, T * 9 * 4));
+    HIPCHK(c, dalloc(&c->d_tan, T * 3 * 4));
+    HIPCHK(c, dalloc(&c->d_albedo, T * 4));
+    HIPCHK(c, dalloc(&c->d_obj, T * 4));
+    HIPCHK(c, dalloc(&c->d_model, (size_t)m->num_objects * 16 * 4));
+    HIPCHK(c, dalloc(&c->d_nmat, (size_t)m->num_objects * 9 * 4));
+    if (T) {
+        HIPCHK(c, hipMemcpy(c->d_pos, m->positions, T * 9 * 4, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(c->d_nrm, m->normals, T * 9 * 4, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(c->d_tan, m->tangents, T * 3 * 4, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(c->d_albedo, m->albedo, T * 4, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(c->d_obj, m->object, T * 4, hipMemcpyHostToDevice));
+    }
+    c->has_rough = m->roughness != nullptr;
+    if (c->has_rough) {
+        HIPCHK(c, dalloc(&c->d_rough, (size_t)m->num_objects * 4));
+        HIPCHK(c, hipMemcpy(c->d_rough, m->roughness, (size_t)m->num_objects * 4, hipMemcpyHostToDevice));
+    }
+    c->ntri = m->num_tris;
+    c->nobj = m->num_objects;
+    c->have_transforms = 0;
+    int rc = ensure_raster(c, std::max<size_t>(T * 2, 64));
+    if (rc) return rc;
+    // identity transforms until the caller sets its own
+    std::vector<float> id((size_t)c->nobj * 16, 0.0f);
+    for (int o = 0; o < c->nobj; ++o) id[(size_t)o * 16 + 0] = id[(size_t)o * 16 + 5] = id[(size_t)o * 16 + 10] =
+        id[(size_t)o * 16 + 15] = 1.0f;
+    return vct_set_transforms(c, id.data(), c->nobj);
+}
+
+// mat3(transpose(inverse(model))) as cofactor(A) / det(A) of the upper-left
+// 3x3 block (voxelize.vert:18, phong.vert:37); column-major out.
+static void normal_matrix(const float *m, float *out) {
+#define A(r, cc) m[(cc)*4 + (r)]
+    float c00 = A(1, 1) * A(2, 2) - A(1, 2) * A(2, 1);
+    float c01 = A(1, 2) * A(2, 0) - A(1, 0) * A(2, 2);
+    float c02 = A(1, 0) * A(2, 1) - A(1, 1) * A(2, 0);
+    float c10 = A(0, 2) * A(2, 1) - A(0, 1) * A(2, 2);
+    float c11 = A(0, 0) * A(2, 2) - A(0, 2) * A(2, 0);
+    float c12 = A(0, 1) * A(2, 0) - A(0, 0) * A(2, 1);
+    float c20 = A(0, 1) * A(1, 2) - A(0, 2) * A(1, 1);
+    float c21 = A(0, 2) * A(1, 0) - A(0, 0) * A(1, 2);
+    float c22 = A(0, 0) * A(1, 1) - A(0, 1) * A(1, 0);
+    float det = (A(0, 0) * c00 + A(0, 1) * c01) + A(0, 2) * c02;
+#undef A
+    out[0] = c00 / det;
+    out[1] = c10 / det;
+    out[2] = c20 / det;
+    out[3] = c01 / det;
+    out[4] = c11 / det;
+    out[5] = c21 / det;
+    out[6] = c02 / det;
+    out[7] = c12 / det;
+    out[8] = c22 / det;
+}
+
+int vct_set_transforms(vct_ctx *c, const float *model, int n) {
+    if (!c || !model) return fail(c, VCT_ERR_INVALID, "vct_set_transforms: NULL argument");
+    if (n != c->nobj) return fail(c, VCT_ERR_INVALID, "vct_set_transforms: %d matrices for %d objects", n, c->nobj);
+    std::vector<float> nm((size_t)n * 9);
+    for (int o = 0; o < n; ++o) normal_matrix(model + (size_t)o * 16, nm.data() + (size_t)o * 9);
+    // synchronous copies: the host arrays are borrowed for the call only
+    HIPCHK(c, hipMemcpyAsync(c->d_model, model, (size_t)n * 64, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_nmat, nm.data(), (size_t)n * 36, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->have_transforms = 1;
+    return VCT_OK;
+}
+
+int vct_set_lights(vct_ctx *c, const vct_light *lights, int n) {
+    if (!c || (n > 0 && !lights)) return fail(c, VCT_ERR_INVALID, "vct_set_lights: NULL argument");
+    if (n < 0 || n > 32) return fail(c, VCT_ERR_INVALID, "vct_set_lights: %d lights (max 32)", n);
+    for (int i = 0; i < n; ++i) {
+        if (lights[i].type > 1u) return fail(c, VCT_ERR_INVALID, "light %d: unknown type %u", i, lights[i].type);
+        if (lights[i].shadow_caster && (lights[i].shadow_size < 16 || lights[i].shadow_size > 8192))
+            return fail(c, VCT_ERR_INVALID, "light %d: shadow_size %u out of [16, 8192]", i, lights[i].shadow_size);
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    // keep shadow maps whose size is unchanged (rebuilt per frame by vct_shadowmap)
+    std::vector<float *> maps(n, nullptr);
+    for (int i = 0; i < n; ++i) {
+        const bool want = lights[i].shadow_caster != 0;
+        if (want && i < (int)c->d_shadow.size() && c->d_shadow[i] && i < (int)c->lights.size() &&
+            c->lights[i].shadow_caster && c->lights[i].shadow_size == lights[i].shadow_size) {
+            maps[i] = c->d_shadow[i];
+            c->d_shadow[i] = nullptr;
+        } else if (want) {
+            const size_t N = lights[i].shadow_size;
+            HIPCHK(c, dalloc(&maps[i], N * N * 4));
+            if (launch_fill_u32((uint32_t *)maps[i], 0x3f800000u, N * N, c->stream) != hipSuccess)
+                return fail(c, VCT_ERR_HIP, "shadow map clear failed");
+        }
+    }
+    for (auto &p : c->d_shadow) dfree(p);
+    c->d_shadow = maps;
+    c->lights.assign(lights, lights + n);
+    c->hlights.resize(n);
+    for (int i = 0; i < n; ++i) {
+        DevLight &d = c->hlights[i];
+        const vct_light &L = lights[i];
+        memcpy(d.position, L.position, 12);
+        d.range = L.range;
+        memcpy(d.direction, L.direction, 12);
+        d.intensity = L.intensity;
+        memcpy(d.color, L.color, 12);
+        d.type = L.type;
+        d.enabled = L.enabled;
+        d.shadow_caster = L.shadow_caster;
+        d.shadow_size = L.shadow_size;
+        d.perspective = L.perspective;
+        memcpy(d.ls, L.ls, 64);
+        memcpy(d.ls_inverse, L.ls_inverse, 64);
+        d.shadowmap = maps[i];
+    }
+    dfree(c->d_lights);
+    if (n) {
+        HIPCHK(c, dalloc(&c->d_lights, sizeof(DevLight) * n));
+        HIPCHK(c, hipMemcpyAsync(c->d_lights, c->hlights.data(), sizeof(DevLight) * n, hipMemcpyHostToDevice,
+                                 c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return VCT_OK;
+}
+
+static DevScene dev_scene(const vct_ctx *c) {
+    DevScene s;
+    s.ntri = c->ntri;
+    s.nobj = c->nobj;
+    s.pos = c->d_pos;
+    s.nrm = c->d_nrm;
+    s.tan = c->d_tan;
+    s.albedo = c->d_albedo;
+    s.obj = c->d_obj;
+    s.rough = c->has_rough ? c->d_rough : nullptr;
+    s.model = c->d_model;
+    s.nmat = c->d_nmat;
+    return s;
+}
+
+#define LAUNCHCHK(c, expr)                                                                               \
+    do {                                                                                                 \
+        hipError_t _e = (expr);                                                                          \
+        if (_e != hipSuccess) return fail((c), VCT_ERR_HIP, "%s: %s", #expr, hipGetErrorString(_e));   \
+    } while (0)
+
+int vct_shadowmap(vct_ctx *c, int li) {
+    if (!c) return fail(nullptr, VCT_ERR_INVALID, "ctx is NULL");
+    if (li < 0 || li >= (int)c->lights.size()) return fail(c, VCT_ERR_INVALID, "light index %d out of range", li);
+    if (!c->lights[li].shadow_caster) return fail(c, VCT_ERR_INVALID, "light %d casts no shadow", li);
+    if (!c->d_model) return fail(c, VCT_ERR_STATE, "vct_shadowmap: no scene");
+    PassTimer pt(c, P_SHADOW);
+    LAUNCHCHK(c, launch_shadowmap(dev_scene(c), &c->hlights[li], c->d_shadow[li], c->rs, c->stream));
+    return VCT_OK;
+}
+
+int vct_voxelize(vct_ctx *c, const vct_voxelize_params *p) {
+    if (!c || !p) return fail(c, VCT_ERR_INVALID, "vct_voxelize: NULL argument");
+    if (!p->atomic_max)
+        return fail(c, VCT_ERR_INVALID,
+                    "vct_voxelize: only the atomicMax accumulation (voxelizeAtomicMax) is supported; the CAS "
+                    "running average of voxelize.frag:127-139 is order dependent");
+    if (p->axis_override < -1 || p->axis_override > 2)
+        return fail(c, VCT_ERR_INVALID, "axis_override %d out of [-1, 2]", p->axis_override);
+    if (!c->d_model) return fail(c, VCT_ERR_STATE, "vct_voxelize: no scene");
+    PassTimer pt(c, P_VOXELIZE);
+    LAUNCHCHK(c, launch_voxelize(dev_scene(c), c->d_lights, (int)c->lights.size(), *p, c->desc.dim,
+                                 c->d_grid_albedo, c->d_grid_normal, c->d_stats, c->rs, c->stream));
+    c->have_voxels = true;
+    c->have_mips = false;
+    return VCT_OK;
+}
+
+int vct_normalize_inject(vct_ctx *c, const vct_inject_params *p) {
+    if (!c || !p) return fail(c, VCT_ERR_INVALID, "vct_normalize_inject: NULL argument");
+    if (!(p->voxel_set_opacity >= 0.0f && p->voxel_set_opacity <= 1.0f))
+        return fail(c, VCT_ERR_INVALID, "voxel_set_opacity %f out of [0, 1]", (double)p->voxel_set_opacity);
+    if (!c->have_voxels) return fail(c, VCT_ERR_STATE, "vct_normalize_inject: voxelize first");
+    PassTimer pt(c, P_RADIANCE);
+    Extent ext;
+    memcpy(ext.center, c->desc.center, 12);
+    memcpy(ext.vmin, c->desc.vmin, 12);
+    memcpy(ext.vmax, c->desc.vmax, 12);
+    LAUNCHCHK(c, launch_inject_scatter(c->hlights.data(), c->d_lights, (int)c->lights.size(), p->light_mask,
+                                       c->desc.dim, ext, c->d_litmask, c->stream));
+    LAUNCHCHK(c, launch_normalize_inject(c->d_grid_albedo, c->d_litmask, c->desc.dim, p->voxel_set_opacity, c->pyr,
+                                         c->d_stats, c->stream));
+    c->have_radiance = true;
+    c->have_mips = false;
+    return VCT_OK;
+}
+
+int vct_fill_holes(vct_ctx *c) {
+    if (!c) return fail(nullptr, VCT_ERR_INVALID, "ctx is NULL");
+    if (!c->have_radiance) return fail(c, VCT_ERR_STATE, "vct_fill_holes: inject first");
+    const LevelInfo &L0 = c->pyr.lv[0];
+    if (!c->d_fill_tmp)
+        HIPCHK(c, dalloc((char **)&c->d_fill_tmp, (size_t)L0.nb * L0.nb * L0.nb * 64 * (L0.fmt == 1 ? 16 : 8)));
+    PassTimer pt(c, P_FILL);
+    LAUNCHCHK(c, launch_fill_holes(c->pyr, c->d_fill_tmp, c->stream));
+    c->have_mips = false;
+    return VCT_OK;
+}
+
+int vct_mip(vct_ctx *c, const vct_mip_params *p) {
+    if (!c || !p) return fail(c, VCT_ERR_INVALID, "vct_mip: NULL argument");
+    if (p->kernel_mode != 0)
+        return fail(c, VCT_ERR_INVALID, "vct_mip: kernel_mode %d unsupported (only KERNEL_BOX2 = 0 is live)",
+                    p->kernel_mode);
+    if (!c->have_radiance) return fail(c, VCT_ERR_STATE, "vct_mip: inject first");
+    PassTimer pt(c, P_MIP);
+    LAUNCHCHK(c, launch_mip(c->pyr, c->stream));
+    c->have_mips = true;
+    return VCT_OK;
+}
+
+static int ensure_camera(vct_ctx *c, int w, int h) {
+    if (w == c->gw && h == c->gh && c->d_gbuf) return VCT_OK;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    dfree(c->d_gbuf);
+    dfree(c->d_vis);
+    dfree(c->d_ind);
+    dfree(c->d_spec);
+    const size_t np = (size_t)w * h;
+    HIPCHK(c, dalloc(&c->d_gbuf, np * sizeof(vct_gbuffer_texel)));
+    HIPCHK(c, dalloc(&c->d_vis, np * 8));
+    HIPCHK(c, dalloc(&c->d_ind, np * 16));
+    HIPCHK(c, dalloc(&c->d_spec, np * 16));
+    HIPCHK(c, hipMemset(c->d_gbuf, 0, np * sizeof(vct_gbuffer_texel)));
+    HIPCHK(c, hipMemset(c->d_ind, 0, np * 16));
+    HIPCHK(c, hipMemset(c->d_spec, 0, np * 16));
+    c->gw = w;
+    c->gh = h;
+    c->have_gbuf = false;
+    return VCT_OK;
+}
+
+int vct_gbuffer(vct_ctx *c, const vct_camera *cam) {
+    if (!c || !cam) return fail(c, VCT_ERR_INVALID, "vct_gbuffer: NULL argument");
+    if (cam->width < 1 || cam->height < 1 || cam->width > 8192 || cam->height > 8192)
+        return fail(c, VCT_ERR_INVALID, "camera size %dx%d out of [1, 8192]", cam->width, cam->height);
+    if (!c->d_model) return fail(c, VCT_ERR_STATE, "vct_gbuffer: no scene");
+    int rc = ensure_camera(c, cam->width, cam->height);
+    if (rc) return rc;
+    PassTimer pt(c, P_GBUFFER);
+    LAUNCHCHK(c, launch_gbuffer(dev_scene(c), *cam, c->d_vis, c->d_gbuf, c->rs, c->stream));
+    c->have_gbuf = true;
+    return VCT_OK;
+}
+
+int vct_trace(vct_ctx *c, const vct_trace_params *p) {
+    if (!c || !p) return fail(c, VCT_ERR_INVALID, "vct_trace: NULL argument");
+    if (!c->have_gbuf) return fail(c, VCT_ERR_STATE, "vct_trace: no G-buffer (call vct_gbuffer or upload one)");
+    if (!c->have_mips) return fail(c, VCT_ERR_STATE, "vct_trace: mip chain not built");
+    if (p->diffuse.steps < 0 || p->diffuse.steps > 4096 || p->specular.steps < 0 || p->specular.steps > 4096)
+        return fail(c, VCT_ERR_INVALID, "cone steps out of [0, 4096]");
+    TraceLaunch t;
+    t.gbuf = c->d_gbuf;
+    t.width = c->gw;
+    t.height = c->gh;
+    t.out_ind = c->d_ind;
+    t.out_spec = c->d_spec;
+    t.enable_reflections = p->enable_reflections;
+    t.angle_from_roughness = p->cone_angle_from_roughness;
+    t.draw_occlusion = p->draw_occlusion;
+    t.has_roughness = c->has_rough;
+    t.ambient_scale = p->ambient_scale;
+    t.reflect_scale = p->reflect_scale;
+    memcpy(t.eye, p->eye, 12);
+    t.d_steps = p->diffuse.steps;
+    t.d_tan_half = tanf(p->diffuse.cone_angle / 2.0f);
+    t.d_bias = p->diffuse.bias;
+    t.d_h0 = p->diffuse.cone_initial_height;
+    t.d_lod_offset = p->diffuse.lod_offset;
+    t.s_steps = p->specular.steps;
+    t.s_angle = p->specular.cone_angle;
+    t.s_bias = p->specular.bias;
+    t.s_h0 = p->specular.cone_initial_height;
+    t.s_lod_offset = p->specular.lod_offset;
+    memcpy(t.ext.center, c->desc.center, 12);
+    memcpy(t.ext.vmin, c->desc.vmin, 12);
+    memcpy(t.ext.vmax, c->desc.vmax, 12);
+    PassTimer pt(c, P_TRACE);
+    LAUNCHCHK(c, launch_trace(c->pyr, t, c->stream));
+    return VCT_OK;
+}
+
+int vct_sync(vct_ctx *c) {
+    if (!c) return fail(nullptr, VCT_ERR_INVALID, "ctx is NULL");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return VCT_OK;
+}
+
+int vct_get_stats(vct_ctx *c, vct_stats *out) {
+    if (!c || !out) return fail(c, VCT_ERR_INVALID, "vct_get_stats: NULL argument");
+    HIPCHK(c, hipMemcpyAsync(out, c->d_stats, sizeof(vct_stats), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return VCT_OK;
+}
+
+int vct_set_timing(vct_ctx *c, int enable) {
+    if (!c) return fail(nullptr, VCT_ERR_INVALID, "ctx is NULL");
+    c->timing = enable != 0;
+    return VCT_OK;
+}
+
+int vct_get_pass_times(vct_ctx *c, vct_pass_times *out) {
+    if (!c || !out) return fail(c, VCT_ERR_INVALID, "vct_get_pass_times: NULL argument");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double ms[P_COUNT] = {0};
+    for (auto &tp : c->pending) {
+        float v = 0.0f;
+        if (hipEventElapsedTime(&v, tp.a, tp.b) == hipSuccess) ms[tp.pass] += v;
+        c->ev_pool.push_back(tp.a);
+        c->ev_pool.push_back(tp.b);
+    }
+    c->pending.clear();
+    out->shadowmap_ms = ms[P_SHADOW];
+    out->voxelize_ms = ms[P_VOXELIZE];
+    out->radiance_ms = ms[P_RADIANCE];
+    out->fill_ms = ms[P_FILL];
+    out->mipmap_ms = ms[P_MIP];
+    out->gbuffer_ms = ms[P_GBUFFER];
+    out->render_ms = ms[P_TRACE];
+    out->learned_ms = ms[P_LEARNED];
+    return VCT_OK;
+}
+
+static int decode_level(const vct_ctx *c, int level, int *l, int *face) {
+    *l = level & 15;
+    *face = level >> 4;
+    if (*l < 0 || *l >= c->pyr.nlevels) return 0;
+    if (*face < 0 || *face >= c->pyr.lv[*l].nfaces) return 0;
+    return 1;
+}
+
+size_t vct_buffer_bytes(const vct_ctx *c, int what, int level) {
+    if (!c) return 0;
+    const size_t dim = (size_t)c->desc.dim, nv = dim * dim * dim;
+    int l, f;
+    switch (what) {
+    case VCT_BUF_ALBEDO:
+    case VCT_BUF_NORMAL: return nv * 4;
+    case VCT_BUF_LITMASK: return (nv + 31) / 32 * 4;
+    case VCT_BUF_RADIANCE:
+        if (!decode_level(c, level, &l, &f)) return 0;
+        return (size_t)c->pyr.lv[l].size * c->pyr.lv[l].size * c->pyr.lv[l].size * 16;
+    case VCT_BUF_RADIANCE_RAW: {
+        if (!decode_level(c, level, &l, &f)) return 0;
+        const LevelInfo &L = c->pyr.lv[l];
+        return (size_t)L.nb * L.nb * L.nb * 64 * (L.fmt == 1 ? 16 : (L.fmt == 2 ? 4 : 8));
+    }
+    case VCT_BUF_SHADOWMAP:
+        if (level < 0 || level >= (int)c->lights.size() || !c->lights[level].shadow_caster) return 0;
+        return (size_t)c->lights[level].shadow_size * c->lights[level].shadow_size * 4;
+    case VCT_BUF_GBUFFER: return (size_t)c->gw * c->gh * sizeof(vct_gbuffer_texel);
+    case VCT_BUF_INDIRECT:
+    case VCT_BUF_SPECULAR: return (size_t)c->gw * c->gh * 16;
+    default: return 0;
+    }
+}
+
+static int ensure_stage(vct_ctx *c, size_t bytes) {
+    if (bytes <= c->stage_bytes) return VCT_OK;
+    dfree(c->d_stage);
+    HIPCHK(c, dalloc(&c->d_stage, bytes));
+    c->stage_bytes = bytes;
+    return VCT_OK;
+}
+
+int vct_download(vct_ctx *c, int what, int level, void *host, size_t bytes) {
+    if (!c || !host) return fail(c, VCT_ERR_INVALID, "vct_download: NULL argument");
+    const size_t need = vct_buffer_bytes(c, what, level);
+    if (need == 0) return fail(c, VCT_ERR_INVALID, "vct_download: buffer %d level %d does not exist", what, level);
+    if (bytes != need) return fail(c, VCT_ERR_INVALID, "vct_download: %zu bytes given, %zu needed", bytes, need);
+    const void *src = nullptr;
+    int l, f;
+    switch (what) {
+    case VCT_BUF_ALBEDO: src = c->d_grid_albedo; break;
+    case VCT_BUF_NORMAL: src = c->d_grid_normal; break;
+    case VCT_BUF_LITMASK: src = c->d_litmask; break;
+    case VCT_BUF_SHADOWMAP: src = c->d_shadow[level]; break;
+    case VCT_BUF_GBUFFER: src = c->d_gbuf; break;
+    case VCT_BUF_INDIRECT: src = c->d_ind; break;
+    case VCT_BUF_SPECULAR: src = c->d_spec; break;
+    case VCT_BUF_RADIANCE: {
+        decode_level(c, level, &l, &f);
+        int rc = ensure_stage(c, need);
+        if (rc) return rc;
+        LAUNCHCHK(c, launch_pyr_to_linear(c->pyr, l, f, c->d_stage, c->stream));
+        src = c->d_stage;
+        break;
+    }
+    case VCT_BUF_RADIANCE_RAW:
+        decode_level(c, level, &l, &f);
+        src = c->pyr.lv[l].base[f];
+        break;
+    default: return fail(c, VCT_ERR_INVALID, "vct_download: unknown buffer %d", what);
+    }
+    HIPCHK(c, hipMemcpyAsync(host, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return VCT_OK;
+}
+
+int vct_upload(vct_ctx *c, int what, int level, const void *host, size_t bytes) {
+    if (!c || !host) return fail(c, VCT_ERR_INVALID, "vct_upload: NULL argument");
+    const size_t need = vct_buffer_bytes(c, what, level);
+    if (need == 0) return fail(c, VCT_ERR_INVALID, "vct_upload: buffer %d level %d does not exist", what, level);
+    if (bytes != need) return fail(c, VCT_ERR_INVALID, "vct_upload: %zu bytes given, %zu needed", bytes, need);
+    void *dst = nullptr;
+    switch (what) {
+    case VCT_BUF_ALBEDO: dst = c->d_grid_albedo; c->have_voxels = true; break;
+    case VCT_BUF_NORMAL: dst = c->d_grid_normal; break;
+    case VCT_BUF_LITMASK: dst = c->d_litmask; break;
+    case VCT_BUF_SHADOWMAP: dst = c->d_shadow[level]; break;
+    case VCT_BUF_GBUFFER: dst = c->d_gbuf; c->have_gbuf = true; break;
+    case VCT_BUF_RADIANCE: {
+        if (level != 0) return fail(c, VCT_ERR_INVALID, "vct_upload: only radiance level 0 can be uploaded");
+        int rc = ensure_stage(c, need);
+        if (rc) return rc;
+        HIPCHK(c, hipMemcpyAsync(c->d_stage, host, bytes, hipMemcpyHostToDevice, c->stream));
+        LAUNCHCHK(c, launch_linear_to_pyr0(c->pyr, c->d_stage, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        c->have_radiance = true;
+        c->have_mips = false;
+        return VCT_OK;
+    }
+    default: return fail(c, VCT_ERR_INVALID, "vct_upload: buffer %d cannot be uploaded", what);
+    }
+    HIPCHK(c, hipMemcpyAsync(dst, host, bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return VCT_OK;
+}
+
+}  // extern "C"

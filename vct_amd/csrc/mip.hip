@@ -1,0 +1,222 @@
+// mip.hip -- the radiance mip chain (filterRadiance.comp KERNEL_BOX2,
+// src/Application.cpp:877-921) fused over three levels per launch.
+//
+// The reference dispatches one compute pass per level with a full barrier in
+// between, re-reading each level from memory.  Here one workgroup loads an
+// 8^3 region of the source level (two 4^3 bricks per axis, contiguous 512-byte
+// brick runs) into LDS and produces the 4^3, 2^3 and 1^3 blocks of the next
+// three levels from LDS, so level 0 -- the only large one -- is read exactly
+// once.  Each level is rounded to its storage format before the next one is
+// filtered from it, exactly as a per-level pass over stored texels would be,
+// and the eight taps are summed in the reference's offset order, so the chain
+// is bit-identical to filtering level by level.
+//
+// Anisotropic chains keep six directional volumes per level >= 1 (+x,-x,+y,
+// -y,+z,-z): each is the mean of the four front-to-back composites
+// near + (1 - near.a) * far of the child pairs along that axis.
+#pragma clang fp contract(off)
+
+#include "vct_device.h"
+
+namespace vctk {
+
+struct MipLevelArgs {
+    char *stage[6];
+    int nb, size;
+};
+
+struct MipArgs {
+    MipLevelArgs src;
+    MipLevelArgs out[3];
+    int nout;
+    int src_faces;  // 1, or 6 when the source level is anisotropic
+    int fp32;       // stage format: 1 = fp32x4, 0 = fp16x4
+};
+
+__device__ __forceinline__ float4 ld_stage(const char *base, int fp32, uint32_t a) {
+    if (fp32) return ((const float4 *)base)[a];
+    const uint2 v = ((const uint2 *)base)[a];
+    return make_float4(h2f((uint16_t)(v.x & 0xffff)), h2f((uint16_t)(v.x >> 16)), h2f((uint16_t)(v.y & 0xffff)),
+                       h2f((uint16_t)(v.y >> 16)));
+}
+__device__ __forceinline__ void st_stage(char *base, int fp32, uint32_t a, float4 c) {
+    if (fp32) {
+        ((float4 *)base)[a] = c;
+        return;
+    }
+    ((uint2 *)base)[a] = make_uint2((uint32_t)f2h(c.x) | ((uint32_t)f2h(c.y) << 16),
+                                    (uint32_t)f2h(c.z) | ((uint32_t)f2h(c.w) << 16));
+}
+__device__ __forceinline__ float4 round_stage(float4 v, int fp32) {
+    if (fp32) return v;
+    return make_float4(q16(v.x), q16(v.y), q16(v.z), q16(v.w));
+}
+
+template <bool ANISO>
+__global__ __launch_bounds__(256) void k_mip_chain(MipArgs a) {
+    constexpr int NF = ANISO ? 6 : 1;
+    __shared__ float4 bufA[NF][512];
+    __shared__ float4 bufB[NF][64];
+    const int tid = threadIdx.x;
+    const int s = a.src.size;
+    const int R = s < 8 ? s : 8;
+    const int nreg = s / R;
+    const int rx = blockIdx.x % nreg, ry = (blockIdx.x / nreg) % nreg, rz = blockIdx.x / (nreg * nreg);
+    const int ox = rx * R, oy = ry * R, oz = rz * R;
+    const int R3 = R * R * R;
+    for (int f = 0; f < a.src_faces; ++f) {
+        for (int i = tid; i < R3; i += 256) {
+            int x, y, z;
+            if (R == 8) {  // brick-major walk: 8 bricks of 64 contiguous voxels
+                const int j = i >> 6, v = i & 63;
+                x = ox + (j & 1) * 4 + (v & 3);
+                y = oy + ((j >> 1) & 1) * 4 + ((v >> 2) & 3);
+                z = oz + (j >> 2) * 4 + (v >> 4);
+            } else {
+                x = i % R;
+                y = (i / R) % R;
+                z = i / (R * R);
+            }
+            const float4 val = ld_stage(a.src.stage[f], a.fp32, brick_addr(x, y, z, a.src.nb));
+            bufA[f][((z - oz) * R + (y - oy)) * R + (x - ox)] = val;
+        }
+    }
+    __syncthreads();
+    int cur = R, faces = a.src_faces;
+    float4 *in = &bufA[0][0];
+    int in_stride = 512;
+    for (int k = 0; k < a.nout; ++k) {
+        const int d = cur >> 1;
+        if (d < 1) break;
+        // ping-pong: outputs of step k go to bufB (k even) or bufA (k odd)
+        float4 *out = (k & 1) ? &bufA[0][0] : &bufB[0][0];
+        const int out_stride = (k & 1) ? 512 : 64;
+        const MipLevelArgs &ol = a.out[k];
+        const int sh = k + 1;
+        const int gx = ox >> sh, gy = oy >> sh, gz = oz >> sh;
+        const int d3 = d * d * d;
+        for (int i = tid; i < d3 * NF; i += 256) {
+            const int f = i / d3;
+            const int vi = i - f * d3;
+            const int x = vi % d, y = (vi / d) % d, z = vi / (d * d);
+            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (!ANISO) {
+                // BOX2 offsets in filterRadiance.comp order: (0,0,0),(0,0,1),(0,1,0),(0,1,1),(1,0,0),...
+#pragma unroll
+                for (int o = 0; o < 8; ++o) {
+                    const int dx = (o >> 2) & 1, dy = (o >> 1) & 1, dz = o & 1;
+                    const float4 t = in[((2 * z + dz) * cur + (2 * y + dy)) * cur + (2 * x + dx)];
+                    v.x = v.x + t.x;
+                    v.y = v.y + t.y;
+                    v.z = v.z + t.z;
+                    v.w = v.w + t.w;
+                }
+                v.x = v.x * 0.125f;
+                v.y = v.y * 0.125f;
+                v.z = v.z * 0.125f;
+                v.w = v.w * 0.125f;
+            } else {
+                const int ax = f >> 1, neg = f & 1;
+                const int sf = faces == 1 ? 0 : f;
+                const float4 *src = in + (size_t)sf * in_stride;
+                const int oa = ax == 0 ? 1 : 0, ob = ax == 2 ? 1 : 2;
+#pragma unroll
+                for (int pa = 0; pa < 2; ++pa)
+#pragma unroll
+                    for (int pb = 0; pb < 2; ++pb) {
+                        int cn[3] = {2 * x, 2 * y, 2 * z}, cf[3] = {2 * x, 2 * y, 2 * z};
+                        cn[oa] += pa;
+                        cf[oa] += pa;
+                        cn[ob] += pb;
+                        cf[ob] += pb;
+                        cn[ax] += neg ? 1 : 0;
+                        cf[ax] += neg ? 0 : 1;
+                        const float4 pn = src[(cn[2] * cur + cn[1]) * cur + cn[0]];
+                        const float4 pf = src[(cf[2] * cur + cf[1]) * cur + cf[0]];
+                        const float t = 1.0f - pn.w;
+                        v.x = v.x + (pn.x + t * pf.x);
+                        v.y = v.y + (pn.y + t * pf.y);
+                        v.z = v.z + (pn.z + t * pf.z);
+                        v.w = v.w + (pn.w + t * pf.w);
+                    }
+                v.x = v.x * 0.25f;
+                v.y = v.y * 0.25f;
+                v.z = v.z * 0.25f;
+                v.w = v.w * 0.25f;
+            }
+            v = round_stage(v, a.fp32);
+            out[(size_t)f * out_stride + vi] = v;
+            st_stage(ol.stage[f], a.fp32, brick_addr(gx + x, gy + y, gz + z, ol.nb), v);
+        }
+        __syncthreads();
+        in = out;
+        in_stride = out_stride;
+        cur = d;
+        faces = NF;
+    }
+}
+
+// FP8 E4M3 encode of one level: one 64-thread wavefront per 4^3 brick.
+__global__ __launch_bounds__(64) void k_fp8_encode(const char *stage, char *base, float *scale, int nb, int size) {
+    const int b = blockIdx.x;
+    const int v = threadIdx.x;
+    const int bx = b % nb, by = (b / nb) % nb, bz = b / (nb * nb);
+    const int x = bx * 4 + (v & 3), y = by * 4 + ((v >> 2) & 3), z = bz * 4 + (v >> 4);
+    const bool valid = x < size && y < size && z < size;
+    const uint32_t a = ((uint32_t)b << 6) | (uint32_t)v;
+    float4 c = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (valid) c = ld_stage(stage, 0, a);
+    float m = fmaxf(fmaxf(__builtin_fabsf(c.x), __builtin_fabsf(c.y)), fmaxf(__builtin_fabsf(c.z), __builtin_fabsf(c.w)));
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+    const float sc = m > 0.0f ? m / 448.0f : 1.0f;
+    if (v == 0) scale[b] = sc;
+    const uint32_t code = f2fp8(c.x / sc) | (f2fp8(c.y / sc) << 8) | (f2fp8(c.z / sc) << 16) | (f2fp8(c.w / sc) << 24);
+    ((uint32_t *)base)[a] = code;
+}
+
+hipError_t launch_mip(const PyrDesc &pyr, hipStream_t s) {
+    const int fp32 = pyr.storage == VCT_STORE_FP32 ? 1 : 0;
+    int src = 0;
+    while (src + 1 < pyr.nlevels) {
+        const LevelInfo &S = pyr.lv[src];
+        const int R = S.size < 8 ? S.size : 8;
+        if (R < 2) break;
+        const int lg = R == 8 ? 3 : (R == 4 ? 2 : 1);
+        int nout = pyr.nlevels - 1 - src;
+        if (nout > lg) nout = lg;
+        MipArgs a;
+        for (int f = 0; f < 6; ++f) a.src.stage[f] = S.stage[f < S.nfaces ? f : 0];
+        a.src.nb = S.nb;
+        a.src.size = S.size;
+        for (int k = 0; k < 3; ++k) {
+            const LevelInfo &O = pyr.lv[k < nout ? src + 1 + k : src + 1];
+            for (int f = 0; f < 6; ++f) a.out[k].stage[f] = O.stage[f < O.nfaces ? f : 0];
+            a.out[k].nb = O.nb;
+            a.out[k].size = O.size;
+        }
+        a.nout = nout;
+        a.src_faces = S.nfaces;
+        a.fp32 = fp32;
+        const int nreg = S.size / R;
+        dim3 grid((unsigned)(nreg * nreg * nreg));
+        if (pyr.aniso) hipLaunchKernelGGL(k_mip_chain<true>, grid, dim3(256), 0, s, a);
+        else hipLaunchKernelGGL(k_mip_chain<false>, grid, dim3(256), 0, s, a);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        src += nout;
+    }
+    if (pyr.storage == VCT_STORE_FP8) {
+        for (int l = 3; l < pyr.nlevels; ++l) {
+            const LevelInfo &L = pyr.lv[l];
+            for (int f = 0; f < L.nfaces; ++f) {
+                hipLaunchKernelGGL(k_fp8_encode, dim3((unsigned)(L.nb * L.nb * L.nb)), dim3(64), 0, s, L.stage[f],
+                                   L.base[f], L.scale[f], L.nb, L.size);
+                hipError_t e = hipGetLastError();
+                if (e != hipSuccess) return e;
+            }
+        }
+    }
+    return hipSuccess;
+}
+
+}  // namespace vctk

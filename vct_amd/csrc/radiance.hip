@@ -1,0 +1,268 @@
+// radiance.hip -- normalize + inject (transferVoxels.comp, injectRadiance.comp)
+// and voxelFillHoles.comp, plus the level <-> linear conversions used by
+// download/upload.
+//
+// The reference injects by scattering one store per shadow-map texel
+// (injectRadiance.comp:152-214, radiance[v] := packUnorm4x8(voxelColor[v])).
+// Every texel that lands in voxel v stores the same value, so the scatter is
+// idempotent and equals a per-voxel "is lit" bit.  Here the scatter only sets
+// that bit (a 1-bit-per-voxel mask, 2 MiB at 256^3) and one coalesced
+// per-voxel pass then does transferVoxels' work (statistics, opacity rewrite,
+// radiance (0,0,0,a)) and the injection together, writing radiance level 0
+// once in the storage format.
+#pragma clang fp contract(off)
+
+#include "vct_device.h"
+
+namespace vctk {
+
+struct InjectLight {
+    float ls_inverse[16];
+    const float *shadowmap;
+    int N, perspective;
+};
+
+__global__ __launch_bounds__(256) void k_inject_scatter(InjectLight L, int dim, Extent ext, uint32_t *litmask) {
+    const int i = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int j = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (i >= L.N || j >= L.N) return;
+    const float tcx = (float)i / (float)L.N, tcy = (float)j / (float)L.N;
+    const float d = shadow_bilinear(L.shadowmap, L.N, tcx, tcy, 0, 0);
+    float ndc4[4] = {tcx * 2.0f - 1.0f, tcy * 2.0f - 1.0f, d * 2.0f - 1.0f, 1.0f}, w[4];
+    mat4_vec4(L.ls_inverse, ndc4, w);
+    if (L.perspective) {
+        w[0] = w[0] / w[3];
+        w[1] = w[1] / w[3];
+        w[2] = w[2] / w[3];
+    }
+    int idx[3];
+    for (int k = 0; k < 3; ++k) {
+        const float lin = ((w[k] - ext.center[k]) - ext.vmin[k]) / (ext.vmax[k] - ext.vmin[k]);
+        idx[k] = (int)((float)dim * lin);
+    }
+    if (idx[0] < 0 || idx[1] < 0 || idx[2] < 0 || idx[0] >= dim || idx[1] >= dim || idx[2] >= dim) return;
+    const size_t vi = ((size_t)idx[2] * dim + idx[1]) * dim + idx[0];
+    const uint32_t bit = 1u << (vi & 31);
+    uint32_t *word = &litmask[vi >> 5];
+    if (!(__atomic_load_n(word, __ATOMIC_RELAXED) & bit)) atomicOr(word, bit);  // skip set bits
+}
+
+hipError_t launch_inject_scatter(const DevLight *lights, const DevLight *dlights_dev, int nlights,
+                                 uint32_t light_mask, int dim, Extent ext, uint32_t *litmask, hipStream_t s) {
+    (void)dlights_dev;
+    const size_t nv = (size_t)dim * dim * dim;
+    hipError_t e = launch_fill_u32(litmask, 0u, (nv + 31) / 32, s);
+    if (e != hipSuccess) return e;
+    for (int li = 0; li < nlights; ++li) {
+        const DevLight &L = lights[li];
+        if (!((light_mask >> li) & 1u) || !L.shadow_caster || !L.shadowmap) continue;
+        InjectLight il;
+        for (int k = 0; k < 16; ++k) il.ls_inverse[k] = L.ls_inverse[k];
+        il.shadowmap = L.shadowmap;
+        il.N = (int)L.shadow_size;
+        il.perspective = (int)L.perspective;
+        dim3 grid((il.N + 15) / 16, (il.N + 15) / 16);
+        hipLaunchKernelGGL(k_inject_scatter, grid, dim3(256), 0, s, il, dim, ext, litmask);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+// One thread per 4-voxel x-row of a brick: 16 threads per brick, a wavefront
+// covers 4 consecutive bricks, so albedo reads are 64-byte runs and radiance
+// writes are whole 512-byte (fp16) bricks.
+template <int FMT>
+__global__ __launch_bounds__(256) void k_normalize_inject(uint32_t *albedo, const uint32_t *__restrict__ litmask,
+                                                          int dim, int nb, float opacity, char *rad0,
+                                                          vct_stats *stats) {
+    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t nrows = (size_t)nb * nb * nb * 16;
+    uint32_t uniq = 0, maxf = 0, lit_count = 0;
+    if (g < nrows) {
+        const size_t b = g >> 4;
+        const int q = (int)(g & 15);
+        const int bx = (int)(b % nb), by = (int)((b / nb) % nb), bz = (int)(b / ((size_t)nb * nb));
+        const int x0 = bx * 4, y = by * 4 + (q & 3), z = bz * 4 + (q >> 2);
+        const size_t li = ((size_t)z * dim + y) * dim + x0;
+        uint4 a = *(const uint4 *)&albedo[li];
+        const uint32_t lw = litmask[li >> 5] >> (li & 31);
+        uint32_t w[4] = {a.x, a.y, a.z, a.w};
+        float r[4][4];
+        bool changed = false;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const uint32_t a32 = w[v];
+            const float ca = unorm8(a32, 3);
+            if (ca > 0.0f) {
+                uniq += 1;
+                const uint32_t mf = (uint32_t)(255.0f * ca);
+                maxf = mf > maxf ? mf : maxf;
+                const float c0 = unorm8(a32, 0), c1 = unorm8(a32, 1), c2 = unorm8(a32, 2);
+                const float na = opacity > 0.0f ? opacity : ca;
+                const uint32_t nw = pack_unorm4x8(c0, c1, c2, na);
+                if (nw != a32) changed = true;
+                w[v] = nw;
+                const bool lit = (lw >> v) & 1u;
+                lit_count += lit ? 1u : 0u;
+                r[v][0] = lit ? c0 : 0.0f;
+                r[v][1] = lit ? c1 : 0.0f;
+                r[v][2] = lit ? c2 : 0.0f;
+                r[v][3] = na;
+            } else {
+                r[v][0] = r[v][1] = r[v][2] = r[v][3] = 0.0f;
+            }
+        }
+        if (changed) *(uint4 *)&albedo[li] = make_uint4(w[0], w[1], w[2], w[3]);
+        const size_t dst = (b << 6) + (size_t)q * 4;  // brick_addr(x0, y, z) for this row
+        if (FMT == 1) {
+            float4 *o = (float4 *)rad0 + dst;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) o[v] = make_float4(r[v][0], r[v][1], r[v][2], r[v][3]);
+        } else {
+            uint32_t h[8];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                h[v * 2 + 0] = (uint32_t)f2h(r[v][0]) | ((uint32_t)f2h(r[v][1]) << 16);
+                h[v * 2 + 1] = (uint32_t)f2h(r[v][2]) | ((uint32_t)f2h(r[v][3]) << 16);
+            }
+            uint4 *o = (uint4 *)((uint2 *)rad0 + dst);
+            o[0] = make_uint4(h[0], h[1], h[2], h[3]);
+            o[1] = make_uint4(h[4], h[5], h[6], h[7]);
+        }
+    }
+    // wavefront reductions, one atomic per wavefront per statistic
+    for (int off = 32; off > 0; off >>= 1) {
+        uniq += __shfl_xor(uniq, off, 64);
+        lit_count += __shfl_xor(lit_count, off, 64);
+        uint32_t o = __shfl_xor(maxf, off, 64);
+        maxf = o > maxf ? o : maxf;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (uniq) atomicAdd(&stats->unique_voxels, uniq);
+        if (lit_count) atomicAdd(&stats->lit_voxels, lit_count);
+        if (maxf) atomicMax(&stats->max_fragments_per_voxel, maxf);
+    }
+}
+
+hipError_t launch_normalize_inject(uint32_t *albedo, const uint32_t *litmask, int dim, float opacity,
+                                   const PyrDesc &pyr, vct_stats *stats, hipStream_t s) {
+    // transferVoxels clears radiance and the stats it owns (Application.cpp:581, :763)
+    hipError_t e = launch_fill_u32(&stats->unique_voxels, 0u, 3, s);  // unique, max, lit
+    if (e != hipSuccess) return e;
+    const LevelInfo &L0 = pyr.lv[0];
+    const size_t nrows = (size_t)L0.nb * L0.nb * L0.nb * 16;
+    dim3 grid((unsigned)((nrows + 255) / 256));
+    if (L0.fmt == 1)
+        hipLaunchKernelGGL(k_normalize_inject<1>, grid, dim3(256), 0, s, albedo, litmask, dim, L0.nb, opacity,
+                           L0.stage[0], stats);
+    else
+        hipLaunchKernelGGL(k_normalize_inject<0>, grid, dim3(256), 0, s, albedo, litmask, dim, L0.nb, opacity,
+                           L0.stage[0], stats);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- fill holes
+__device__ __forceinline__ float4 load_vox(const char *base, int fmt, int nb, int x, int y, int z) {
+    const uint32_t a = brick_addr(x, y, z, nb);
+    if (fmt == 1) return ((const float4 *)base)[a];
+    const uint2 v = ((const uint2 *)base)[a];
+    return make_float4(h2f((uint16_t)(v.x & 0xffff)), h2f((uint16_t)(v.x >> 16)), h2f((uint16_t)(v.y & 0xffff)),
+                       h2f((uint16_t)(v.y >> 16)));
+}
+__device__ __forceinline__ void store_vox(char *base, int fmt, int nb, int x, int y, int z, float4 c) {
+    const uint32_t a = brick_addr(x, y, z, nb);
+    if (fmt == 1) {
+        ((float4 *)base)[a] = c;
+        return;
+    }
+    ((uint2 *)base)[a] = make_uint2((uint32_t)f2h(c.x) | ((uint32_t)f2h(c.y) << 16),
+                                    (uint32_t)f2h(c.z) | ((uint32_t)f2h(c.w) << 16));
+}
+
+__global__ __launch_bounds__(256) void k_fill_holes(const char *src, char *dst, int fmt, int nb, int dim) {
+    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t n = (size_t)dim * dim * dim;
+    if (g >= n) return;
+    const int x = (int)(g % dim), y = (int)((g / dim) % dim), z = (int)(g / ((size_t)dim * dim));
+    float4 cur = load_vox(src, fmt, nb, x, y, z);
+    if (cur.w == 0.0f) {
+        float count = 0.0f;
+        for (int i = -1; i <= 1; ++i)
+            for (int j = -1; j <= 1; ++j)
+                for (int k = -1; k <= 1; ++k) {
+                    const int xx = x + i, yy = y + j, zz = z + k;
+                    if (xx < 0 || yy < 0 || zz < 0 || xx >= dim || yy >= dim || zz >= dim) continue;
+                    const float4 v = load_vox(src, fmt, nb, xx, yy, zz);
+                    if (v.w > 0.0f) {
+                        cur.x = cur.x + v.x;
+                        cur.y = cur.y + v.y;
+                        cur.z = cur.z + v.z;
+                        cur.w = cur.w + v.w;
+                        count += 1.0f;
+                    }
+                }
+        if (count > 0.0f) {
+            cur.x = cur.x / count;
+            cur.y = cur.y / count;
+            cur.z = cur.z / count;
+            cur.w = cur.w / count;
+        }
+    }
+    store_vox(dst, fmt, nb, x, y, z, cur);
+}
+
+hipError_t launch_fill_holes(const PyrDesc &pyr, void *tmp, hipStream_t s) {
+    const LevelInfo &L0 = pyr.lv[0];
+    const size_t n = (size_t)pyr.dim * pyr.dim * pyr.dim;
+    hipLaunchKernelGGL(k_fill_holes, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, L0.stage[0], (char *)tmp,
+                       L0.fmt, L0.nb, pyr.dim);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const size_t bytes = (size_t)L0.nb * L0.nb * L0.nb * 64 * (L0.fmt == 1 ? 16 : 8);
+    return hipMemcpyAsync(L0.stage[0], tmp, bytes, hipMemcpyDeviceToDevice, s);  // glCopyImageSubData
+}
+
+// ---------------------------------------------------------------- conversions
+__global__ void k_pyr_to_linear(const char *base, const float *scale, int fmt, int nb, int size, float4 *out) {
+    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t n = (size_t)size * size * size;
+    if (g >= n) return;
+    const int x = (int)(g % size), y = (int)((g / size) % size), z = (int)(g / ((size_t)size * size));
+    const uint32_t a = brick_addr(x, y, z, nb);
+    float4 v;
+    if (fmt == 2) {
+        const uint32_t c = ((const uint32_t *)base)[a];
+        const float sc = scale[a >> 6];
+        v = make_float4(fp82f(c & 0xff) * sc, fp82f((c >> 8) & 0xff) * sc, fp82f((c >> 16) & 0xff) * sc,
+                        fp82f(c >> 24) * sc);
+    } else {
+        v = load_vox(base, fmt, nb, x, y, z);
+    }
+    out[g] = v;
+}
+
+hipError_t launch_pyr_to_linear(const PyrDesc &pyr, int level, int face, float4 *out, hipStream_t s) {
+    const LevelInfo &L = pyr.lv[level];
+    const size_t n = (size_t)L.size * L.size * L.size;
+    hipLaunchKernelGGL(k_pyr_to_linear, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, L.base[face],
+                       L.scale[face], L.fmt, L.nb, L.size, out);
+    return hipGetLastError();
+}
+
+__global__ void k_linear_to_pyr0(char *base, int fmt, int nb, int size, const float4 *in) {
+    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t n = (size_t)size * size * size;
+    if (g >= n) return;
+    const int x = (int)(g % size), y = (int)((g / size) % size), z = (int)(g / ((size_t)size * size));
+    store_vox(base, fmt, nb, x, y, z, in[g]);
+}
+
+hipError_t launch_linear_to_pyr0(const PyrDesc &pyr, const float4 *in, hipStream_t s) {
+    const LevelInfo &L = pyr.lv[0];
+    const size_t n = (size_t)L.size * L.size * L.size;
+    hipLaunchKernelGGL(k_linear_to_pyr0, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, L.stage[0], L.fmt,
+                       L.nb, L.size, in);
+    return hipGetLastError();
+}
+
+}  // namespace vctk

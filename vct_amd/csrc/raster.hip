@@ -1,0 +1,284 @@
+// raster.hip -- shadow-map depth pass and the camera G-buffer (depth pre-pass
+// + per-pixel surface attributes) on the binned rasterizer.
+//
+// Shadow map: simple.vert draws every actor with projection*view of the light
+// (src/Application.cpp:212-233); only depth matters for cone tracing.
+// G-buffer: the depth pre-pass (src/Application.cpp:936-965) plus what
+// phong.vert hands phong.frag (shaders/phong.vert:36-59): world position,
+// normal, and the TBN columns.  Visibility is an order-independent 64-bit
+// atomicMin of (depth bits, ~triangle id): the nearest fragment wins and, at
+// equal depth, the triangle drawn last wins, as glDepthFunc(GL_EQUAL) shading
+// after the pre-pass does.
+#pragma clang fp contract(off)
+
+#include "raster_common.h"
+
+namespace vctk {
+
+__device__ __forceinline__ void world_vertex(const DevScene &sc, int t, int v, float w[3]) {
+    const float *M = sc.model + (size_t)sc.obj[t] * 16;
+    const float *p = sc.pos + (size_t)t * 9 + v * 3;
+    float p4[4] = {p[0], p[1], p[2], 1.0f}, o[4];
+    mat4_vec4(M, p4, o);
+    w[0] = o[0];
+    w[1] = o[1];
+    w[2] = o[2];
+}
+
+// ---------------------------------------------------------------- shadow map
+struct LightMat {
+    float ls[16];
+};
+
+__global__ __launch_bounds__(256) void k_shadow_setup(DevScene sc, LightMat L, int N, int4 *rec, float4 *attr,
+                                                      uint64_t *counts) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= sc.ntri) return;
+    float clip[3][4];
+    bool behind = false;
+    for (int v = 0; v < 3; ++v) {
+        float w[3];
+        world_vertex(sc, t, v, w);
+        float w4[4] = {w[0], w[1], w[2], 1.0f};
+        mat4_vec4(L.ls, w4, clip[v]);
+        if (clip[v][2] < -clip[v][3] || clip[v][3] <= 0.0f) behind = true;
+    }
+    uint64_t cnt = 0;
+    if (!behind) {
+        float half = (float)N * 0.5f;
+        float win[3][2], zw[3];
+        for (int v = 0; v < 3; ++v) {
+            float xd = clip[v][0] / clip[v][3], yd = clip[v][1] / clip[v][3], zd = clip[v][2] / clip[v][3];
+            win[v][0] = xd * half + half;
+            win[v][1] = yd * half + half;
+            zw[v] = zd * 0.5f + 0.5f;
+        }
+        int4 r[4];
+        // back faces culled: GL_CULL_FACE stays enabled from the previous
+        // frame's passes when the reference draws its shadow map
+        // (Application.cpp:737,942,979)
+        cnt = tri_setup(win, N, N, 1, t, r);
+        if (cnt) {
+            for (int i = 0; i < 4; ++i) rec[(size_t)t * 4 + i] = r[i];
+            attr[t] = make_float4(zw[0], zw[1], zw[2], 0.0f);
+        }
+    }
+    counts[t] = cnt;
+}
+
+struct ShadowFrag {
+    const float4 *attr;
+    uint32_t *depth;
+    int N;
+    __device__ void item(int slot, int px, int py, const Edges &e) {
+        const int64_t cx = (int64_t)px * 256 + 128, cy = (int64_t)py * 256 + 128;
+        if (!edge_inside(e, cx, cy)) return;
+        float l[3];
+        edge_bary(e, cx, cy, l);
+        float4 a = attr[slot];
+        float z = interp3(l, a.x, a.y, a.z);
+        z = z + 0.0f;
+        if (!(z >= 0.0f && z <= 1.0f)) return;
+        atomicMin(&depth[(size_t)py * N + px], __float_as_uint(z));
+    }
+    __device__ void finish() {}
+};
+
+hipError_t launch_shadowmap(const DevScene &sc, const DevLight *L, float *depth, RasterScratch &rs,
+                            hipStream_t s) {
+    const int N = (int)L->shadow_size;
+    hipError_t e = launch_fill_u32((uint32_t *)depth, 0x3f800000u, (size_t)N * N, s);  // clear depth 1.0
+    if (e != hipSuccess) return e;
+    if (sc.ntri == 0) return hipSuccess;
+    LightMat lm;
+    for (int i = 0; i < 16; ++i) lm.ls[i] = L->ls[i];
+    hipLaunchKernelGGL(k_shadow_setup, dim3((sc.ntri + 255) / 256), dim3(256), 0, s, sc, lm, N, rs.rec, rs.attr,
+                       rs.counts);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = raster_scan(rs, (size_t)sc.ntri, s)) != hipSuccess) return e;
+    ShadowFrag f{rs.attr, (uint32_t *)depth, N};
+    hipLaunchKernelGGL(k_raster_items<ShadowFrag>, dim3(kRasterBlocks), dim3(256), 0, s, rs.rec, rs.offs, rs.chunk,
+                       rs.chunk_cap, sc.ntri, f);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- G-buffer
+struct CamMats {
+    float pv[16];
+    int W, H, cull;
+};
+
+__global__ __launch_bounds__(256) void k_gbuf_setup(DevScene sc, CamMats cm, int4 *rec, float4 *attr,
+                                                    uint64_t *counts) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= sc.ntri) return;
+    float clip[3][4];
+    for (int v = 0; v < 3; ++v) {
+        float w[3];
+        world_vertex(sc, t, v, w);
+        float w4[4] = {w[0], w[1], w[2], 1.0f};
+        mat4_vec4(cm.pv, w4, clip[v]);
+    }
+    // Sutherland-Hodgman against the near plane z >= -w
+    float poly[4][4];
+    int n = 0;
+    float d[3];
+    for (int i = 0; i < 3; ++i) d[i] = clip[i][2] + clip[i][3];
+    for (int i = 0; i < 3; ++i) {
+        int j = (i + 1) % 3;
+        bool ii = d[i] >= 0.0f, jj = d[j] >= 0.0f;
+        if (ii) {
+            for (int k = 0; k < 4; ++k) poly[n][k] = clip[i][k];
+            n++;
+        }
+        if (ii != jj) {
+            float tt = d[i] / (d[i] - d[j]);
+            for (int k = 0; k < 4; ++k) poly[n][k] = clip[i][k] + tt * (clip[j][k] - clip[i][k]);
+            n++;
+        }
+    }
+    const float hw = (float)cm.W * 0.5f, hh = (float)cm.H * 0.5f;
+    for (int sub = 0; sub < 2; ++sub) {
+        const size_t slot = (size_t)t * 2 + sub;
+        uint64_t cnt = 0;
+        if (sub + 2 < n) {  // fan (0,1,2) needs 3 vertices, (0,2,3) needs 4
+            const int vi[3] = {0, sub + 1, sub + 2};
+            float win[3][2], zw[3];
+            bool bad = false;
+            for (int v = 0; v < 3; ++v) {
+                const float *c = poly[vi[v]];
+                float w = c[3];
+                if (!(w > 0.0f)) bad = true;
+                float xd = c[0] / w, yd = c[1] / w, zd = c[2] / w;
+                win[v][0] = xd * hw + hw;
+                win[v][1] = yd * hh + hh;
+                zw[v] = zd * 0.5f + 0.5f;
+            }
+            if (!bad) {
+                int4 r[4];
+                cnt = tri_setup(win, cm.W, cm.H, cm.cull, t, r);
+                if (cnt) {
+                    for (int i = 0; i < 4; ++i) rec[slot * 4 + i] = r[i];
+                    attr[slot] = make_float4(zw[0], zw[1], zw[2], 0.0f);
+                }
+            }
+        }
+        counts[slot] = cnt;
+    }
+}
+
+struct GbufFrag {
+    const float4 *attr;
+    unsigned long long *vis;
+    int W;
+    __device__ void item(int slot, int px, int py, const Edges &e) {
+        const int64_t cx = (int64_t)px * 256 + 128, cy = (int64_t)py * 256 + 128;
+        if (!edge_inside(e, cx, cy)) return;
+        float l[3];
+        edge_bary(e, cx, cy, l);
+        float4 a = attr[slot];
+        float z = interp3(l, a.x, a.y, a.z);
+        z = z + 0.0f;
+        if (!(z >= 0.0f && z <= 1.0f)) return;
+        const uint32_t tri = (uint32_t)e.user;
+        unsigned long long key = ((unsigned long long)__float_as_uint(z) << 32) | (unsigned long long)(0xffffffffu - tri);
+        atomicMin(&vis[(size_t)py * W + px], key);
+    }
+    __device__ void finish() {}
+};
+
+__global__ __launch_bounds__(256) void k_gbuf_resolve(DevScene sc, CamMats cm, const unsigned long long *vis,
+                                                      vct_gbuffer_texel *out) {
+    const int px = blockIdx.x * 16 + (threadIdx.x & 15), py = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (px >= cm.W || py >= cm.H) return;
+    const size_t pi = (size_t)py * cm.W + px;
+    float4 *o = (float4 *)&out[pi];
+    const unsigned long long key = vis[pi];
+    if (key == ~0ull) {
+        float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+        o[0] = z;
+        o[1] = z;
+        o[2] = z;
+        o[3] = z;
+        return;
+    }
+    const int t = (int)(0xffffffffu - (uint32_t)(key & 0xffffffffull));
+    const float xn = (2.0f * ((float)px + 0.5f)) / (float)cm.W - 1.0f;
+    const float yn = (2.0f * ((float)py + 0.5f)) / (float)cm.H - 1.0f;
+    float Wp[3][3], clip[3][4];
+    for (int v = 0; v < 3; ++v) {
+        world_vertex(sc, t, v, Wp[v]);
+        float w4[4] = {Wp[v][0], Wp[v][1], Wp[v][2], 1.0f};
+        mat4_vec4(cm.pv, w4, clip[v]);
+    }
+    float ax[3], ay[3];
+    for (int v = 0; v < 3; ++v) {
+        ax[v] = clip[v][0] - xn * clip[v][3];
+        ay[v] = clip[v][1] - yn * clip[v][3];
+    }
+    float b[3];
+    b[0] = ax[1] * ay[2] - ay[1] * ax[2];
+    b[1] = ax[2] * ay[0] - ay[2] * ax[0];
+    b[2] = ax[0] * ay[1] - ay[0] * ax[1];
+    const float sum = (b[0] + b[1]) + b[2];
+    for (int v = 0; v < 3; ++v) b[v] = b[v] / sum;
+    const float *Nm = sc.nmat + (size_t)sc.obj[t] * 9;
+    float Tw[3];
+    mat3_vec3(Nm, sc.tan + (size_t)t * 3, Tw);
+    float Nv[3][3], Tv[3][3], Bv[3][3];
+    for (int v = 0; v < 3; ++v) {
+        mat3_vec3(Nm, sc.nrm + (size_t)t * 9 + v * 3, Nv[v]);
+        float dtn = dot3(Tw, Nv[v]);
+        float tt[3] = {Tw[0] - dtn * Nv[v][0], Tw[1] - dtn * Nv[v][1], Tw[2] - dtn * Nv[v][2]};
+        nrm3(tt, Tv[v]);
+        cross3(Nv[v], Tv[v], Bv[v]);
+    }
+    float P[3], N[3], T[3], B[3];
+    for (int k = 0; k < 3; ++k) {
+        P[k] = interp3(b, Wp[0][k], Wp[1][k], Wp[2][k]);
+        N[k] = interp3(b, Nv[0][k], Nv[1][k], Nv[2][k]);
+        T[k] = interp3(b, Tv[0][k], Tv[1][k], Tv[2][k]);
+        B[k] = interp3(b, Bv[0][k], Bv[1][k], Bv[2][k]);
+    }
+    const uint32_t alb = sc.albedo[t];
+    const float rough = sc.rough ? sc.rough[sc.obj[t]] : 0.5f;
+    o[0] = make_float4(P[0], P[1], P[2], __uint_as_float(alb));
+    o[1] = make_float4(N[0], N[1], N[2], rough);
+    o[2] = make_float4(T[0], T[1], T[2], __uint_as_float(1u));
+    o[3] = make_float4(B[0], B[1], B[2], __uint_as_float((uint32_t)t));
+}
+
+hipError_t launch_gbuffer(const DevScene &sc, const vct_camera &cam, uint64_t *vis, vct_gbuffer_texel *gbuf,
+                          RasterScratch &rs, hipStream_t s) {
+    const size_t np = (size_t)cam.width * cam.height;
+    hipError_t e = launch_fill_u64(vis, ~0ull, np, s);
+    if (e != hipSuccess) return e;
+    CamMats cm;
+    // pv = projection * view, the matrix product phong.vert applies before model
+    for (int c = 0; c < 4; ++c)
+        for (int r = 0; r < 4; ++r) {
+            float acc = cam.projection[0 * 4 + r] * cam.view[c * 4 + 0];
+            acc = acc + cam.projection[1 * 4 + r] * cam.view[c * 4 + 1];
+            acc = acc + cam.projection[2 * 4 + r] * cam.view[c * 4 + 2];
+            acc = acc + cam.projection[3 * 4 + r] * cam.view[c * 4 + 3];
+            cm.pv[c * 4 + r] = acc;
+        }
+    cm.W = cam.width;
+    cm.H = cam.height;
+    cm.cull = cam.cull_back;
+    if (sc.ntri > 0) {
+        hipLaunchKernelGGL(k_gbuf_setup, dim3((sc.ntri + 255) / 256), dim3(256), 0, s, sc, cm, rs.rec, rs.attr,
+                           rs.counts);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        if ((e = raster_scan(rs, (size_t)sc.ntri * 2, s)) != hipSuccess) return e;
+        GbufFrag f{rs.attr, (unsigned long long *)vis, cam.width};
+        hipLaunchKernelGGL(k_raster_items<GbufFrag>, dim3(kRasterBlocks), dim3(256), 0, s, rs.rec, rs.offs,
+                           rs.chunk, rs.chunk_cap, sc.ntri * 2, f);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    dim3 grid((cam.width + 15) / 16, (cam.height + 15) / 16);
+    hipLaunchKernelGGL(k_gbuf_resolve, grid, dim3(256), 0, s, sc, cm, (const unsigned long long *)vis, gbuf);
+    return hipGetLastError();
+}
+
+}  // namespace vctk

@@ -1,0 +1,173 @@
+// raster_common.h -- the binned triangle rasterizer shared by the shadow-map,
+// voxelize and G-buffer passes.
+//
+// Rules (the GL 4.5 rasterization rules as the oracle restates them):
+// window coordinates snapped to 1/256 px (round to nearest even), integer edge
+// functions, top-left fill rule, zero-area triangles dropped, samples at the
+// pixel centre or the standard 4x pattern, varyings at the pixel centre with
+// lambda_i = E_i(centre) / area.
+//
+// Work distribution: a setup kernel writes one 64-byte record per triangle
+// slot plus the number of bbox pixels it may cover; an exclusive scan turns
+// the counts into item offsets; each wavefront then takes 64 consecutive
+// (slot, pixel) items, so a triangle of any size is spread over as many
+// wavefronts as it needs and tiny triangles are packed 64 to a wavefront.
+#pragma once
+
+#include "vct_device.h"
+
+namespace vctk {
+
+__device__ __forceinline__ float clamp_win(float v) {
+    return v < -524288.0f ? -524288.0f : (v > 524288.0f ? 524288.0f : v);
+}
+
+// Record layout (4 x int4):
+//  r0 = (A0, B0, A1, B1)
+//  r1 = (A2, B2, x0 | y0 << 16, bw | bias_bits << 16)
+//  r2 = (C0.lo, C0.hi, C1.lo, C1.hi)
+//  r3 = (C2.lo, C2.hi, float area, user)
+// Returns the number of bbox pixels (0 = no fragments).
+__device__ __forceinline__ uint64_t tri_setup(const float win[3][2], int W, int H, int cull, int user,
+                                              int4 *rec) {
+    int64_t X[3], Y[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        X[i] = (int64_t)__builtin_rintf(clamp_win(win[i][0]) * 256.0f);
+        Y[i] = (int64_t)__builtin_rintf(clamp_win(win[i][1]) * 256.0f);
+    }
+    int64_t A[3], B[3], C[3];
+    A[0] = Y[1] - Y[2];
+    B[0] = X[2] - X[1];
+    C[0] = X[1] * Y[2] - X[2] * Y[1];
+    A[1] = Y[2] - Y[0];
+    B[1] = X[0] - X[2];
+    C[1] = X[2] * Y[0] - X[0] * Y[2];
+    A[2] = Y[0] - Y[1];
+    B[2] = X[1] - X[0];
+    C[2] = X[0] * Y[1] - X[1] * Y[0];
+    int64_t area = C[0] + C[1] + C[2];
+    if (area == 0) return 0;
+    if (area < 0) {
+        if (cull) return 0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            A[i] = -A[i];
+            B[i] = -B[i];
+            C[i] = -C[i];
+        }
+        area = -area;
+    }
+    uint32_t bias_bits = 0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        bool tl = (A[i] > 0) || (A[i] == 0 && B[i] < 0);
+        if (!tl) bias_bits |= 1u << i;
+    }
+    int64_t mnx = X[0], mxx = X[0], mny = Y[0], mxy = Y[0];
+#pragma unroll
+    for (int i = 1; i < 3; ++i) {
+        mnx = X[i] < mnx ? X[i] : mnx;
+        mxx = X[i] > mxx ? X[i] : mxx;
+        mny = Y[i] < mny ? Y[i] : mny;
+        mxy = Y[i] > mxy ? Y[i] : mxy;
+    }
+    int64_t x0 = mnx >> 8, x1 = mxx >> 8, y0 = mny >> 8, y1 = mxy >> 8;
+    if (x0 < 0) x0 = 0;
+    if (y0 < 0) y0 = 0;
+    if (x1 > W - 1) x1 = W - 1;
+    if (y1 > H - 1) y1 = H - 1;
+    if (x0 > x1 || y0 > y1) return 0;
+    uint32_t bw = (uint32_t)(x1 - x0 + 1), bh = (uint32_t)(y1 - y0 + 1);
+    rec[0] = make_int4((int)A[0], (int)B[0], (int)A[1], (int)B[1]);
+    rec[1] = make_int4((int)A[2], (int)B[2], (int)((uint32_t)x0 | ((uint32_t)y0 << 16)),
+                       (int)(bw | (bias_bits << 16)));
+    rec[2] = make_int4((int)(uint32_t)(uint64_t)C[0], (int)(uint32_t)((uint64_t)C[0] >> 32),
+                       (int)(uint32_t)(uint64_t)C[1], (int)(uint32_t)((uint64_t)C[1] >> 32));
+    rec[3] = make_int4((int)(uint32_t)(uint64_t)C[2], (int)(uint32_t)((uint64_t)C[2] >> 32),
+                       __float_as_int((float)area), user);
+    return (uint64_t)bw * bh;
+}
+
+struct Edges {
+    int64_t A[3], B[3], C[3], bias[3];
+    float area;
+    int x0, y0, bw, user;
+};
+
+__device__ __forceinline__ void load_edges(const int4 *__restrict__ recs, size_t slot, Edges &e) {
+    int4 r0 = recs[slot * 4 + 0], r1 = recs[slot * 4 + 1], r2 = recs[slot * 4 + 2], r3 = recs[slot * 4 + 3];
+    e.A[0] = r0.x;
+    e.B[0] = r0.y;
+    e.A[1] = r0.z;
+    e.B[1] = r0.w;
+    e.A[2] = r1.x;
+    e.B[2] = r1.y;
+    e.x0 = r1.z & 0xffff;
+    e.y0 = (int)((uint32_t)r1.z >> 16);
+    e.bw = r1.w & 0xffff;
+    uint32_t bb = ((uint32_t)r1.w >> 16) & 7u;
+    e.bias[0] = (bb & 1u) ? -1 : 0;
+    e.bias[1] = (bb & 2u) ? -1 : 0;
+    e.bias[2] = (bb & 4u) ? -1 : 0;
+    e.C[0] = (int64_t)(((uint64_t)(uint32_t)r2.y << 32) | (uint32_t)r2.x);
+    e.C[1] = (int64_t)(((uint64_t)(uint32_t)r2.w << 32) | (uint32_t)r2.z);
+    e.C[2] = (int64_t)(((uint64_t)(uint32_t)r3.y << 32) | (uint32_t)r3.x);
+    e.area = __int_as_float(r3.z);
+    e.user = r3.w;
+}
+
+__device__ __forceinline__ bool edge_inside(const Edges &e, int64_t sx, int64_t sy) {
+    bool in = true;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) in = in && (e.A[i] * sx + e.B[i] * sy + e.C[i] + e.bias[i] >= 0);
+    return in;
+}
+
+__device__ __forceinline__ void edge_bary(const Edges &e, int64_t sx, int64_t sy, float l[3]) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) l[i] = (float)(e.A[i] * sx + e.B[i] * sy + e.C[i]) / e.area;
+}
+
+// 4x MSAA sample offsets in 1/256 px: (0.375,0.125) (0.875,0.375) (0.125,0.625) (0.625,0.875)
+__device__ __forceinline__ int msaa_x(int k) { return k == 0 ? 96 : (k == 1 ? 224 : (k == 2 ? 32 : 160)); }
+__device__ __forceinline__ int msaa_y(int k) { return k == 0 ? 32 : (k == 1 ? 96 : (k == 2 ? 160 : 224)); }
+
+// Walk every (slot, pixel) item.  F provides item(slot, px, py, edges) and
+// finish().  chunk[c] is the slot holding item 64*c for c < chunk_cap.
+template <class F>
+__global__ __launch_bounds__(256) void k_raster_items(const int4 *__restrict__ recs,
+                                                      const uint64_t *__restrict__ offs,
+                                                      const uint32_t *__restrict__ chunk, size_t chunk_cap,
+                                                      int nslots, F f) {
+    const uint64_t total = offs[nslots];
+    const uint64_t nchunks = (total + 63) >> 6;
+    const int lane = threadIdx.x & 63;
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    for (uint64_t c = wave; c < nchunks; c += nwaves) {
+        const uint64_t k = (c << 6) + (uint64_t)lane;
+        int lo = c < chunk_cap ? (int)chunk[c] : 0;
+        int hi = (c + 1 < nchunks && c + 1 < chunk_cap) ? (int)chunk[c + 1] : nslots - 1;
+        if (k < total) {
+            while (lo < hi) {
+                int mid = (lo + hi + 1) >> 1;
+                if (offs[mid] <= k) lo = mid;
+                else hi = mid - 1;
+            }
+            const uint64_t local = k - offs[lo];
+            Edges e;
+            load_edges(recs, (size_t)lo, e);
+            const uint32_t lu = (uint32_t)local;
+            const int px = e.x0 + (int)(lu % (uint32_t)e.bw);
+            const int py = e.y0 + (int)(lu / (uint32_t)e.bw);
+            f.item(lo, px, py, e);
+        }
+    }
+    f.finish();
+}
+
+// Grid for k_raster_items: enough resident wavefronts to fill 256 CUs.
+constexpr int kRasterBlocks = 2048;
+
+}  // namespace vctk

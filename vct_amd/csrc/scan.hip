@@ -1,0 +1,68 @@
+// scan.hip -- item offsets for the binned rasterizer, chunk starts, fills.
+#include <hipcub/hipcub.hpp>
+
+#include "vct_device.h"
+
+namespace vctk {
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+__global__ void k_fill_u64(uint64_t *p, uint64_t v, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+static unsigned fill_blocks(size_t n) {
+    size_t b = (n + 255) / 256;
+    return (unsigned)(b > 4096 ? 4096 : (b == 0 ? 1 : b));
+}
+
+hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_u32, dim3(fill_blocks(n)), dim3(256), 0, s, p, v, n);
+    return hipGetLastError();
+}
+hipError_t launch_fill_u64(uint64_t *p, uint64_t v, size_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_u64, dim3(fill_blocks(n)), dim3(256), 0, s, p, v, n);
+    return hipGetLastError();
+}
+
+// chunk[c] = largest slot t with offs[t] <= 64*c  (the slot holding item 64c)
+__global__ void k_chunk_starts(const uint64_t *__restrict__ offs, int nslots, uint32_t *chunk, size_t cap) {
+    const uint64_t total = offs[nslots];
+    uint64_t nchunks = (total + 63) >> 6;
+    if (nchunks > cap) nchunks = cap;
+    for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < nchunks;
+         c += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t k = c << 6;
+        int lo = 0, hi = nslots - 1;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (offs[mid] <= k) lo = mid;
+            else hi = mid - 1;
+        }
+        chunk[c] = (uint32_t)lo;
+    }
+}
+
+size_t raster_scan_tmp_bytes(size_t nslots) {
+    size_t bytes = 0;
+    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (const uint64_t *)nullptr, (uint64_t *)nullptr,
+                                     (int)(nslots + 1));
+    return bytes;
+}
+
+hipError_t raster_scan(RasterScratch &rs, size_t nslots, hipStream_t s) {
+    // counts[nslots] is kept at 0 so offs[nslots] is the total
+    hipError_t e = launch_fill_u64(rs.counts + nslots, 0, 1, s);
+    if (e != hipSuccess) return e;
+    size_t bytes = rs.scan_tmp_bytes;
+    e = hipcub::DeviceScan::ExclusiveSum(rs.scan_tmp, bytes, rs.counts, rs.offs, (int)(nslots + 1), s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_chunk_starts, dim3(1024), dim3(256), 0, s, rs.offs, (int)nslots, rs.chunk,
+                       rs.chunk_cap);
+    return hipGetLastError();
+}
+
+}  // namespace vctk

@@ -1,0 +1,142 @@
+// vct_device.h -- device-side helpers shared by the gfx950 kernels.
+//
+// Float arithmetic on the bit-exact passes (raster, voxelize, inject, mip) is
+// written in one fixed evaluation order and compiled without FP contraction
+// (each .hip file that includes this header for those passes sets
+// `#pragma clang fp contract(off)`), so a pass's integer outputs and storage
+// rounding are reproducible bit for bit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <stdint.h>
+
+#include "vct_internal.h"
+
+namespace vctk {
+
+// ---- small vector math (fixed order, no fused multiply-add) ---------------
+__device__ __forceinline__ float dot3(const float a[3], const float b[3]) {
+    return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
+}
+__device__ __forceinline__ void nrm3(const float v[3], float o[3]) {
+    float l = __builtin_sqrtf(dot3(v, v));
+    o[0] = v[0] / l;
+    o[1] = v[1] / l;
+    o[2] = v[2] / l;
+}
+__device__ __forceinline__ void cross3(const float a[3], const float b[3], float o[3]) {
+    float t0 = a[1] * b[2] - a[2] * b[1];
+    float t1 = a[2] * b[0] - a[0] * b[2];
+    float t2 = a[0] * b[1] - a[1] * b[0];
+    o[0] = t0;
+    o[1] = t1;
+    o[2] = t2;
+}
+// column-major mat4 * vec4
+__device__ __forceinline__ void mat4_vec4(const float *m, const float v[4], float o[4]) {
+    float t[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        float s = m[0 * 4 + r] * v[0];
+        s = s + m[1 * 4 + r] * v[1];
+        s = s + m[2 * 4 + r] * v[2];
+        s = s + m[3 * 4 + r] * v[3];
+        t[r] = s;
+    }
+    o[0] = t[0];
+    o[1] = t[1];
+    o[2] = t[2];
+    o[3] = t[3];
+}
+__device__ __forceinline__ void mat3_vec3(const float *m, const float v[3], float o[3]) {
+    float t[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) t[r] = (m[0 * 3 + r] * v[0] + m[1 * 3 + r] * v[1]) + m[2 * 3 + r] * v[2];
+    o[0] = t[0];
+    o[1] = t[1];
+    o[2] = t[2];
+}
+__device__ __forceinline__ float interp3(const float l[3], float a0, float a1, float a2) {
+    return (l[0] * a0 + l[1] * a1) + l[2] * a2;
+}
+__device__ __forceinline__ float clampf(float x, float a, float b) { return x < a ? a : (x > b ? b : x); }
+
+// ---- storage formats -------------------------------------------------------
+__device__ __forceinline__ uint32_t pack_unorm4x8(float r, float g, float b, float a) {
+    float c[4] = {r, g, b, a};
+    uint32_t o = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        float v = clampf(c[i], 0.0f, 1.0f);
+        uint32_t q = (uint32_t)__builtin_rintf(v * 255.0f);
+        o |= (q & 0xffu) << (8 * i);
+    }
+    return o;
+}
+__device__ __forceinline__ float unorm8(uint32_t v, int i) { return (float)((v >> (8 * i)) & 0xffu) / 255.0f; }
+
+__device__ __forceinline__ uint16_t f2h(float f) { return __half_as_ushort(__float2half_rn(f)); }
+__device__ __forceinline__ float h2f(uint16_t h) { return __half2float(__ushort_as_half(h)); }
+__device__ __forceinline__ float q16(float f) { return h2f(f2h(f)); }
+
+// OCP FP8 E4M3: bias 7, max 448, 0x7f = NaN; round to nearest even, saturate.
+__device__ __forceinline__ uint32_t f2fp8(float f) {
+    if (f != f) return 0x7fu;
+    uint32_t sign = (__float_as_uint(f) >> 24) & 0x80u;
+    float a = __builtin_fabsf(f);
+    if (a >= 448.0f) return sign | 0x7eu;
+    if (a < 0.015625f) return sign | (uint32_t)__builtin_rintf(a * 512.0f);
+    uint32_t bits = __float_as_uint(a);
+    int e = (int)((bits >> 23) & 0xff) - 127;  // a = 1.m * 2^e, e >= -6
+    float mant = __uint_as_float((bits & 0x7fffffu) | 0x3f800000u);  // [1,2)
+    float q = __builtin_rintf((mant - 1.0f) * 8.0f);
+    if (q >= 8.0f) {
+        q = 0.0f;
+        e += 1;
+    }
+    int code = ((e + 7) << 3) | (int)q;
+    if (code > 0x7e) code = 0x7e;
+    return sign | (uint32_t)code;
+}
+__device__ __forceinline__ float fp82f(uint32_t v) {
+    uint32_t e = (v >> 3) & 15u, m = v & 7u;
+    float mag;
+    if (e == 0) mag = (float)m * 0.001953125f;  // m * 2^-9
+    else mag = __uint_as_float(((e - 7 + 127) << 23) | (m << 20));
+    return (v & 0x80u) ? -mag : mag;
+}
+
+// ---- brick layout of the radiance pyramid ---------------------------------
+// Every level is stored as 4x4x4 bricks, bricks in x-fastest order, voxels in
+// a brick x-fastest.  A level of edge s has nb = ceil(s/4) bricks per axis.
+__device__ __forceinline__ uint32_t brick_addr(int x, int y, int z, int nb) {
+    return ((uint32_t)(((z >> 2) * nb + (y >> 2)) * nb + (x >> 2)) << 6) | ((uint32_t)(z & 3) << 4) |
+           ((uint32_t)(y & 3) << 2) | (uint32_t)(x & 3);
+}
+
+// GL LINEAR sample of an fp32 depth texture, CLAMP_TO_BORDER with border 1.0
+// (src/Application.cpp:46-53), with a textureOffset texel offset.
+__device__ __forceinline__ float shadow_bilinear(const float *__restrict__ sm, int N, float s, float t, int ox,
+                                                 int oy) {
+    float u = s * (float)N;
+    u = u + (float)ox;
+    u = u - 0.5f;
+    float v = t * (float)N;
+    v = v + (float)oy;
+    v = v - 0.5f;
+    const float fu = __builtin_floorf(u), fv = __builtin_floorf(v);
+    const int i0 = (int)fu, j0 = (int)fv;
+    const float a = u - fu, b = v - fv;
+    auto tex = [&](int i, int j) -> float {
+        return (i >= 0 && i < N && j >= 0 && j < N) ? sm[(size_t)j * N + i] : 1.0f;
+    };
+    const float t00 = tex(i0, j0), t10 = tex(i0 + 1, j0), t01 = tex(i0, j0 + 1), t11 = tex(i0 + 1, j0 + 1);
+    const float w00 = (1.0f - a) * (1.0f - b), w10 = a * (1.0f - b), w01 = (1.0f - a) * b, w11 = a * b;
+    return ((w00 * t00 + w10 * t10) + w01 * t01) + w11 * t11;
+}
+
+// ---- wave helpers ----------------------------------------------------------
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+}  // namespace vctk

@@ -1,0 +1,134 @@
+// vct_internal.h -- structures shared between the host context (context.cpp)
+// and the gfx950 kernels, plus the launch wrappers each .hip file exports.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../../include/vct.h"
+
+#define VCT_MAX_LEVELS 10
+
+namespace vctk {
+
+// Light as the kernels see it: the ABI record plus its shadow map pointer.
+struct DevLight {
+    float position[3];
+    float range;
+    float direction[3];
+    float intensity;
+    float color[3];
+    uint32_t type;
+    uint32_t enabled, shadow_caster, shadow_size, perspective;
+    float ls[16];
+    float ls_inverse[16];
+    const float *shadowmap;  // device, shadow_size^2 fp32, row 0 at t = 0
+};
+
+// Radiance pyramid.  Every level is stored in 4x4x4 bricks (vct_device.h
+// brick_addr).  Level l has edge `size`, `nb` bricks per axis and `nfaces`
+// volumes (1, or 6 for anisotropic levels >= 1: +x,-x,+y,-y,+z,-z).
+// fmt: 0 = fp16x4 (8 B/voxel), 1 = fp32x4 (16 B), 2 = fp8 E4M3 x4 (4 B) with
+// one fp32 scale per brick.  `stage` is where the mip chain reads and writes
+// the level: the storage itself for fp16/fp32, an fp16 copy for fp8 levels
+// (the chain is filtered from fp16 values and fp8 levels are encoded after).
+struct LevelInfo {
+    char *base[6];
+    char *stage[6];
+    float *scale[6];
+    int size, nb, fmt, nfaces;
+};
+
+struct PyrDesc {
+    int dim, nlevels, storage, aniso;
+    LevelInfo lv[VCT_MAX_LEVELS];
+    const LevelInfo *d_lv;  // device copy of lv[0..nlevels)
+};
+
+// Grid extent as the kernels use it (VCT::center/min/max).
+struct Extent {
+    float center[3], vmin[3], vmax[3];
+};
+
+// Scene geometry on the device (object space, triangle soup).
+struct DevScene {
+    int ntri, nobj;
+    const float *pos;       // [ntri][9]
+    const float *nrm;       // [ntri][9]
+    const float *tan;       // [ntri][3]
+    const uint32_t *albedo; // [ntri]
+    const uint32_t *obj;    // [ntri]
+    const float *rough;     // [nobj] or null
+    const float *model;     // [nobj][16]
+    const float *nmat;      // [nobj][9]
+};
+
+// Scratch for the binned rasterizer: per-slot setup records (4 x int4), an
+// attribute block of `attr_f4` float4 per slot, per-slot fragment-candidate
+// counts, their exclusive scan (nslots + 1 entries, last = total) and the
+// first slot of every 64-item chunk.
+struct RasterScratch {
+    int4 *rec;
+    float4 *attr;
+    uint64_t *counts;
+    uint64_t *offs;
+    uint32_t *chunk;
+    size_t chunk_cap;
+    void *scan_tmp;
+    size_t scan_tmp_bytes;
+    size_t cap_slots;
+};
+
+struct TraceLaunch {
+    const vct_gbuffer_texel *gbuf;
+    int width, height;
+    float4 *out_ind, *out_spec;
+    int enable_reflections, angle_from_roughness, draw_occlusion, has_roughness;
+    float ambient_scale, reflect_scale;
+    float eye[3];
+    // diffuse / specular cone settings; tan(angle/2) precomputed on the host
+    int d_steps;
+    float d_tan_half, d_bias, d_h0, d_lod_offset;
+    int s_steps;
+    float s_angle, s_bias, s_h0, s_lod_offset;
+    Extent ext;
+};
+
+// ---- launch wrappers (return hipError_t of the last launch) ----------------
+hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
+hipError_t launch_fill_u64(uint64_t *p, uint64_t v, size_t n, hipStream_t s);
+
+// scan.hip: counts[0..n) -> offs[0..n] exclusive (offs[n] = total) and chunk starts
+hipError_t raster_scan(RasterScratch &rs, size_t nslots, hipStream_t s);
+size_t raster_scan_tmp_bytes(size_t nslots);
+
+// raster.hip
+hipError_t launch_shadowmap(const DevScene &sc, const DevLight *dlight_host, float *depth, RasterScratch &rs,
+                            hipStream_t s);
+hipError_t launch_gbuffer(const DevScene &sc, const vct_camera &cam, uint64_t *vis, vct_gbuffer_texel *gbuf,
+                          RasterScratch &rs, hipStream_t s);
+
+// voxelize.hip
+hipError_t launch_voxelize(const DevScene &sc, const DevLight *dlights, int nlights, const vct_voxelize_params &p,
+                           int dim, uint32_t *albedo, uint32_t *normal, vct_stats *stats, RasterScratch &rs,
+                           hipStream_t s);
+
+// radiance.hip
+hipError_t launch_inject_scatter(const DevLight *dlights_host, const DevLight *dlights_dev, int nlights,
+                                 uint32_t light_mask, int dim, Extent ext, uint32_t *litmask, hipStream_t s);
+hipError_t launch_normalize_inject(uint32_t *albedo, const uint32_t *litmask, int dim, float opacity,
+                                   const PyrDesc &pyr, vct_stats *stats, hipStream_t s);
+hipError_t launch_fill_holes(const PyrDesc &pyr, void *tmp, hipStream_t s);
+
+// mip.hip
+hipError_t launch_mip(const PyrDesc &pyr, hipStream_t s);
+
+// trace.hip
+hipError_t launch_trace(const PyrDesc &pyr, const TraceLaunch &t, hipStream_t s);
+
+// conversions for download (radiance level -> linear float4)
+hipError_t launch_pyr_to_linear(const PyrDesc &pyr, int level, int face, float4 *out, hipStream_t s);
+hipError_t launch_linear_to_pyr0(const PyrDesc &pyr, const float4 *in, hipStream_t s);
+
+}  // namespace vctk

@@ -1,0 +1,256 @@
+// voxelize.hip -- the dominant-axis voxelizer (GS path) on the binned
+// rasterizer.
+//
+// Per triangle (setup kernel): voxelize.vert (world position, normal matrix),
+// voxelize.geom (dominant axis from the summed vertex normals, mvp_axis),
+// snap + edge setup.  Per (triangle, pixel) item: 4x MSAA coverage, varyings
+// at the pixel centre, the optional per-fragment lighting with 5-tap PCF per
+// shadow-casting light, getVoxelPosition's swizzle, and the two packed RGBA8
+// imageAtomicMax stores (shaders/voxelize.frag:186-279).  Integer atomicMax on
+// the packed word makes the result independent of fragment order.
+#pragma clang fp contract(off)
+
+#include "raster_common.h"
+
+namespace vctk {
+
+struct VoxMats {
+    float mvp[3][16];
+    int axis_override, dim;
+};
+
+// attribute block per triangle: 8 float4
+//  f[0..8]  ndc xyz of v0,v1,v2        f[9..17]  world positions
+//  f[18..26] world normals             f[27] albedo bits  f[28] axis | zclip<<4
+__global__ __launch_bounds__(256) void k_vox_setup(DevScene sc, VoxMats vm, int4 *rec, float4 *attr,
+                                                   uint64_t *counts) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= sc.ntri) return;
+    const int o = (int)sc.obj[t];
+    const float *M = sc.model + (size_t)o * 16;
+    const float *Nm = sc.nmat + (size_t)o * 9;
+    float W[3][3], Nw[3][3];
+    for (int v = 0; v < 3; ++v) {
+        const float *p = sc.pos + (size_t)t * 9 + v * 3;
+        float p4[4] = {p[0], p[1], p[2], 1.0f}, w4[4];
+        mat4_vec4(M, p4, w4);
+        W[v][0] = w4[0];
+        W[v][1] = w4[1];
+        W[v][2] = w4[2];
+        mat3_vec3(Nm, sc.nrm + (size_t)t * 9 + v * 3, Nw[v]);
+    }
+    float fs[3], fn[3];
+    for (int k = 0; k < 3; ++k) fs[k] = (Nw[0][k] + Nw[1][k]) + Nw[2][k];
+    nrm3(fs, fn);
+    const float ax = __builtin_fabsf(fn[0]), ay = __builtin_fabsf(fn[1]), az = __builtin_fabsf(fn[2]);
+    int axis;
+    if (ax > ay && ax > az) axis = 0;
+    else if (ay > ax && ay > az) axis = 1;
+    else axis = 2;
+    if (vm.axis_override >= 0 && vm.axis_override <= 2) axis = vm.axis_override;
+    const float *mvp = vm.mvp[axis];
+    const float half = (float)vm.dim * 0.5f;
+    float ndc[3][3], win[3][2];
+    int zclip = 0;
+    for (int v = 0; v < 3; ++v) {
+        float w4[4] = {W[v][0], W[v][1], W[v][2], 1.0f}, c[4];
+        mat4_vec4(mvp, w4, c);
+        for (int k = 0; k < 3; ++k) ndc[v][k] = c[k] / c[3];
+        if (ndc[v][2] < -1.0f || ndc[v][2] > 1.0f) zclip = 1;
+        win[v][0] = ndc[v][0] * half + half;
+        win[v][1] = ndc[v][1] * half + half;
+    }
+    int4 r[4];
+    const uint64_t cnt = tri_setup(win, vm.dim, vm.dim, 0, t, r);
+    if (cnt) {
+        for (int i = 0; i < 4; ++i) rec[(size_t)t * 4 + i] = r[i];
+        float f[32];
+        for (int v = 0; v < 3; ++v)
+            for (int k = 0; k < 3; ++k) {
+                f[v * 3 + k] = ndc[v][k];
+                f[9 + v * 3 + k] = W[v][k];
+                f[18 + v * 3 + k] = Nw[v][k];
+            }
+        f[27] = __uint_as_float(sc.albedo[t]);
+        f[28] = __uint_as_float((uint32_t)axis | ((uint32_t)zclip << 4));
+        f[29] = f[30] = f[31] = 0.0f;
+        float4 *a = attr + (size_t)t * 8;
+        for (int i = 0; i < 8; ++i) a[i] = make_float4(f[i * 4], f[i * 4 + 1], f[i * 4 + 2], f[i * 4 + 3]);
+    }
+    counts[t] = cnt;
+}
+
+// calcShadowFactor (shaders/voxelize.frag:160-184)
+__device__ __forceinline__ float calc_shadow_factor(const float lsp[4], const float *sm, int N) {
+    float sh[3];
+    for (int k = 0; k < 3; ++k) sh[k] = (lsp[k] / lsp[3] + 1.0f) * 0.5f;
+    const float frag = sh[2] - 0.01f;
+    if (frag > 1.0f) return 0.0f;
+    float f = 0.0f;
+    if (frag > shadow_bilinear(sm, N, sh[0], sh[1], 0, 0)) f += 1.0f;
+    if (frag > shadow_bilinear(sm, N, sh[0], sh[1], 1, 0)) f += 1.0f;
+    if (frag > shadow_bilinear(sm, N, sh[0], sh[1], 0, 1)) f += 1.0f;
+    if (frag > shadow_bilinear(sm, N, sh[0], sh[1], -1, 0)) f += 1.0f;
+    if (frag > shadow_bilinear(sm, N, sh[0], sh[1], 0, -1)) f += 1.0f;
+    f = f / 5.0f;
+    return f;
+}
+
+struct VoxFrag {
+    const float4 *attr;
+    const DevLight *lights;
+    int nlights, lighting, msaa, dim;
+    uint32_t *albedo, *normal;
+    uint32_t *stat_frags;
+    uint32_t frags;
+
+    __device__ void item(int slot, int px, int py, const Edges &e) {
+        const float4 *a = attr + (size_t)slot * 8;
+        const float4 a7 = a[7];
+        const uint32_t flags = __float_as_uint(a7.x);
+        const int axis = (int)(flags & 3u);
+        const bool zclip = (flags >> 4) & 1u;
+        const float4 a0 = a[0], a1 = a[1], a2 = a[2];
+        // ndc z of the three vertices: f[2], f[5], f[8]
+        const float nz0 = a0.z, nz1 = a1.y, nz2 = a2.x;
+        bool cov = false;
+        const int ns = msaa ? 4 : 1;
+        for (int k = 0; k < ns && !cov; ++k) {
+            const int64_t sx = (int64_t)px * 256 + (msaa ? msaa_x(k) : 128);
+            const int64_t sy = (int64_t)py * 256 + (msaa ? msaa_y(k) : 128);
+            if (!edge_inside(e, sx, sy)) continue;
+            if (zclip) {
+                float ls[3];
+                edge_bary(e, sx, sy, ls);
+                const float z = interp3(ls, nz0, nz1, nz2);
+                if (!(z >= -1.0f && z <= 1.0f)) continue;
+            }
+            cov = true;
+        }
+        if (!cov) return;
+        const int64_t cx = (int64_t)px * 256 + 128, cy = (int64_t)py * 256 + 128;
+        float l[3];
+        edge_bary(e, cx, cy, l);
+        const float4 a3 = a[3], a4 = a[4], a5 = a[5], a6 = a[6];
+        float f[28];
+        f[0] = a0.x; f[1] = a0.y; f[2] = a0.z; f[3] = a0.w;
+        f[4] = a1.x; f[5] = a1.y; f[6] = a1.z; f[7] = a1.w;
+        f[8] = a2.x; f[9] = a2.y; f[10] = a2.z; f[11] = a2.w;
+        f[12] = a3.x; f[13] = a3.y; f[14] = a3.z; f[15] = a3.w;
+        f[16] = a4.x; f[17] = a4.y; f[18] = a4.z; f[19] = a4.w;
+        f[20] = a5.x; f[21] = a5.y; f[22] = a5.z; f[23] = a5.w;
+        f[24] = a6.x; f[25] = a6.y; f[26] = a6.z; f[27] = a6.w;
+        float pos[3], wp[3], nr[3];
+        for (int k = 0; k < 3; ++k) {
+            pos[k] = interp3(l, f[k], f[3 + k], f[6 + k]);
+            wp[k] = interp3(l, f[9 + k], f[12 + k], f[15 + k]);
+            nr[k] = interp3(l, f[18 + k], f[21 + k], f[24 + k]);
+        }
+        frags += 1;
+        const uint32_t alb = __float_as_uint(f[27]);
+        float color[3] = {unorm8(alb, 0), unorm8(alb, 1), unorm8(alb, 2)};
+        float nn[3];
+        nrm3(nr, nn);
+        const float ne0 = (nn[0] + 1.0f) * 0.5f, ne1 = (nn[1] + 1.0f) * 0.5f, ne2 = (nn[2] + 1.0f) * 0.5f;
+        if (lighting) {
+            float fin[3] = {0.0f, 0.0f, 0.0f};
+            for (int i = 0; i < nlights; ++i) {
+                const DevLight &L = lights[i];
+                if (!L.enabled) continue;
+                float lit[3];
+                if (L.type == 0u) {
+                    float d[3] = {L.position[0] - wp[0], L.position[1] - wp[1], L.position[2] - wp[2]};
+                    float dn[3];
+                    nrm3(d, dn);
+                    const float ndl = __builtin_fmaxf(0.0f, dot3(nn, dn));
+                    for (int k = 0; k < 3; ++k) lit[k] = ((color[k] * L.intensity) * L.color[k]) * ndl;
+                    const float dist = __builtin_sqrtf(dot3(d, d));
+                    const float e0 = 0.75f * L.range, e1 = L.range;
+                    const float tt = clampf((dist - e0) / (e1 - e0), 0.0f, 1.0f);
+                    const float ss = (tt * tt) * (3.0f - 2.0f * tt);
+                    const float att = 1.0f - ss;
+                    for (int k = 0; k < 3; ++k) lit[k] = lit[k] * att;
+                } else {
+                    float nd[3] = {-L.direction[0], -L.direction[1], -L.direction[2]}, ndn[3];
+                    nrm3(nd, ndn);
+                    const float ndl = __builtin_fmaxf(0.0f, dot3(nn, ndn));
+                    for (int k = 0; k < 3; ++k) lit[k] = ((color[k] * L.color[k]) * L.intensity) * ndl;
+                }
+                if (L.shadow_caster && L.shadowmap) {
+                    float w4[4] = {wp[0], wp[1], wp[2], 1.0f}, lsp[4];
+                    mat4_vec4(L.ls, w4, lsp);
+                    const float sf = 1.0f - calc_shadow_factor(lsp, L.shadowmap, (int)L.shadow_size);
+                    for (int k = 0; k < 3; ++k) lit[k] = lit[k] * sf;
+                }
+                for (int k = 0; k < 3; ++k) fin[k] = fin[k] + lit[k];
+            }
+            for (int k = 0; k < 3; ++k) color[k] = clampf(fin[k], 0.0f, 1.0f);
+        }
+        float u[3];
+        for (int k = 0; k < 3; ++k) u[k] = (pos[k] + 1.0f) * 0.5f;
+        float un[3];
+        if (axis == 0) {
+            un[0] = 1.0f - u[2];
+            un[1] = u[1];
+            un[2] = u[0];
+        } else if (axis == 1) {
+            un[0] = u[0];
+            un[1] = 1.0f - u[2];
+            un[2] = u[1];
+        } else {
+            un[0] = u[0];
+            un[1] = u[1];
+            un[2] = u[2];
+        }
+        un[2] = 1.0f - un[2];
+        int idx[3];
+        for (int k = 0; k < 3; ++k) idx[k] = (int)((float)dim * un[k]);
+        if (idx[0] < 0 || idx[1] < 0 || idx[2] < 0 || idx[0] >= dim || idx[1] >= dim || idx[2] >= dim) return;
+        const size_t vi = ((size_t)idx[2] * dim + idx[1]) * dim + idx[0];
+        atomicMax(&albedo[vi], pack_unorm4x8(color[0], color[1], color[2], 1.0f));
+        atomicMax(&normal[vi], pack_unorm4x8(ne0, ne1, ne2, 1.0f));
+    }
+    __device__ void finish() {
+        uint32_t v = frags;
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if ((threadIdx.x & 63) == 0 && v) atomicAdd(stat_frags, v);
+    }
+};
+
+hipError_t launch_voxelize(const DevScene &sc, const DevLight *dlights, int nlights, const vct_voxelize_params &p,
+                           int dim, uint32_t *albedo, uint32_t *normal, vct_stats *stats, RasterScratch &rs,
+                           hipStream_t s) {
+    const size_t nv = (size_t)dim * dim * dim;
+    hipError_t e = launch_fill_u32(albedo, 0u, nv, s);  // glClearTexImage (Application.cpp:686-687)
+    if (e != hipSuccess) return e;
+    if ((e = launch_fill_u32(normal, 0u, nv, s)) != hipSuccess) return e;
+    if ((e = launch_fill_u32(&stats->total_voxel_fragments, 0u, 1, s)) != hipSuccess) return e;
+    if (sc.ntri == 0) return hipSuccess;
+    VoxMats vm;
+    for (int i = 0; i < 16; ++i) {
+        vm.mvp[0][i] = p.mvp_x[i];
+        vm.mvp[1][i] = p.mvp_y[i];
+        vm.mvp[2][i] = p.mvp_z[i];
+    }
+    vm.axis_override = p.axis_override;
+    vm.dim = dim;
+    hipLaunchKernelGGL(k_vox_setup, dim3((sc.ntri + 255) / 256), dim3(256), 0, s, sc, vm, rs.rec, rs.attr, rs.counts);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = raster_scan(rs, (size_t)sc.ntri, s)) != hipSuccess) return e;
+    VoxFrag f;
+    f.attr = rs.attr;
+    f.lights = dlights;
+    f.nlights = nlights;
+    f.lighting = p.lighting;
+    f.msaa = p.msaa;
+    f.dim = dim;
+    f.albedo = albedo;
+    f.normal = normal;
+    f.stat_frags = &stats->total_voxel_fragments;
+    f.frags = 0;
+    hipLaunchKernelGGL(k_raster_items<VoxFrag>, dim3(kRasterBlocks), dim3(256), 0, s, rs.rec, rs.offs, rs.chunk,
+                       rs.chunk_cap, sc.ntri, f);
+    return hipGetLastError();
+}
+
+}  // namespace vctk

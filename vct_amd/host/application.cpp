@@ -1,0 +1,622 @@
+// application.cpp -- headless application layer (include/vct_app.h).
+//
+// Builds the seeded synthetic scenes, owns the per-frame uniforms and runs the
+// pass order of Application::render (src/Application.cpp:196-1085) through
+// the C ABI of include/vct.h.  The scenes replace the reference's Sponza actor
+// (Application.cpp:93-136): triangle soups with per-triangle RGBA8 albedo
+// (the 1x1 diffuse texture a material would bind), face or smooth normals and
+// per-triangle tangents from the Mesh.cpp:174-197 rule with uv (0,0), (1,0),
+// (0,1), i.e. tangent = normalize(v1 - v0).
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <map>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "glmath.h"
+#include "vct_app.h"
+
+using namespace vcthost;
+
+namespace {
+
+// SplitMix64 stream; floats from the top 24 bits.
+struct Rng {
+    uint64_t s;
+    explicit Rng(uint64_t seed) : s(seed) {}
+    uint64_t next() {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    double uni() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }
+    double uni(double a, double b) { return a + (b - a) * uni(); }
+    double normal() {
+        double u1 = uni(), u2 = uni();
+        if (u1 < 1e-300) u1 = 1e-300;
+        return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+    }
+};
+
+struct Mesh {
+    std::vector<float> pos, nrm, tan, rough;
+    std::vector<uint32_t> albedo, obj;
+    int nobj = 0;
+    int ntri() const { return (int)albedo.size(); }
+};
+
+uint32_t random_albedo(Rng &r) {
+    uint32_t v = 0;
+    for (int k = 0; k < 3; ++k) {
+        double u = r.uni(0.2, 0.9);
+        uint32_t q = (uint32_t)std::lrint(u * 255.0);
+        v |= (q & 0xffu) << (8 * k);
+    }
+    return v | 0xff000000u;
+}
+
+void push_tri(Mesh &m, const double p[3][3], const double n[3][3], uint32_t albedo, uint32_t obj) {
+    for (int v = 0; v < 3; ++v)
+        for (int k = 0; k < 3; ++k) {
+            m.pos.push_back((float)p[v][k]);
+            m.nrm.push_back((float)n[v][k]);
+        }
+    float e[3] = {(float)p[1][0] - (float)p[0][0], (float)p[1][1] - (float)p[0][1], (float)p[1][2] - (float)p[0][2]};
+    float t[3];
+    nrm3(e, t);
+    m.tan.insert(m.tan.end(), t, t + 3);
+    m.albedo.push_back(albedo);
+    m.obj.push_back(obj);
+}
+
+// Axis-aligned box centred on the object origin; outward (or inward for the
+// room) face normals, counter-clockwise as seen from the side the normal
+// points to.
+void add_box(Mesh &m, const double h[3], bool inward, uint32_t obj, Rng &r) {
+    for (int a = 0; a < 3; ++a)
+        for (int sgn = -1; sgn <= 1; sgn += 2) {
+            const int u = (a + 1) % 3, v = (a + 2) % 3;
+            double corner[4][3];
+            const double cu[4] = {-1, 1, 1, -1}, cv[4] = {-1, -1, 1, 1};
+            for (int i = 0; i < 4; ++i) {
+                corner[i][a] = sgn * h[a];
+                corner[i][u] = cu[i] * h[u];
+                corner[i][v] = cv[i] * h[v];
+            }
+            bool flip = (sgn < 0) != inward;
+            double nrm[3] = {0, 0, 0};
+            nrm[a] = inward ? -sgn : sgn;
+            const int q[2][3] = {{0, 1, 2}, {0, 2, 3}};
+            for (int t = 0; t < 2; ++t) {
+                double p[3][3], n[3][3];
+                for (int k = 0; k < 3; ++k) {
+                    int idx = q[t][flip ? (k == 0 ? 0 : 3 - k) : k];
+                    std::memcpy(p[k], corner[idx], sizeof(p[k]));
+                    std::memcpy(n[k], nrm, sizeof(n[k]));
+                }
+                push_tri(m, p, n, random_albedo(r), obj);
+            }
+        }
+}
+
+void add_icosphere(Mesh &m, double radius, int subdiv, uint32_t obj, Rng &r) {
+    const double t = (1.0 + std::sqrt(5.0)) / 2.0;
+    std::vector<std::array<double, 3>> V = {{-1, t, 0}, {1, t, 0},  {-1, -t, 0}, {1, -t, 0},
+                                            {0, -1, t}, {0, 1, t},  {0, -1, -t}, {0, 1, -t},
+                                            {t, 0, -1}, {t, 0, 1},  {-t, 0, -1}, {-t, 0, 1}};
+    for (auto &v : V) {
+        double l = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        for (auto &c : v) c /= l;
+    }
+    std::vector<std::array<int, 3>> F = {{0, 11, 5}, {0, 5, 1},  {0, 1, 7},   {0, 7, 10}, {0, 10, 11},
+                                         {1, 5, 9},  {5, 11, 4}, {11, 10, 2}, {10, 7, 6}, {7, 1, 8},
+                                         {3, 9, 4},  {3, 4, 2},  {3, 2, 6},   {3, 6, 8},  {3, 8, 9},
+                                         {4, 9, 5},  {2, 4, 11}, {6, 2, 10},  {8, 6, 7},  {9, 8, 1}};
+    for (int s = 0; s < subdiv; ++s) {
+        std::vector<std::array<int, 3>> F2;
+        F2.reserve(F.size() * 4);
+        std::map<std::pair<int, int>, int> mid;
+        auto midpoint = [&](int a, int b) {
+            auto key = std::make_pair(std::min(a, b), std::max(a, b));
+            auto it = mid.find(key);
+            if (it != mid.end()) return it->second;
+            std::array<double, 3> p;
+            for (int k = 0; k < 3; ++k) p[k] = (V[a][k] + V[b][k]) * 0.5;
+            double l = std::sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+            for (auto &c : p) c /= l;
+            V.push_back(p);
+            mid[key] = (int)V.size() - 1;
+            return (int)V.size() - 1;
+        };
+        for (auto &f : F) {
+            int ab = midpoint(f[0], f[1]), bc = midpoint(f[1], f[2]), ca = midpoint(f[2], f[0]);
+            F2.push_back({f[0], ab, ca});
+            F2.push_back({f[1], bc, ab});
+            F2.push_back({f[2], ca, bc});
+            F2.push_back({ab, bc, ca});
+        }
+        F.swap(F2);
+    }
+    for (auto &f : F) {
+        double p[3][3], n[3][3];
+        for (int k = 0; k < 3; ++k)
+            for (int c = 0; c < 3; ++c) {
+                n[k][c] = V[f[k]][c];
+                p[k][c] = V[f[k]][c] * radius;
+            }
+        // make the winding counter-clockwise seen from outside
+        double e1[3], e2[3], cr[3];
+        for (int c = 0; c < 3; ++c) {
+            e1[c] = p[1][c] - p[0][c];
+            e2[c] = p[2][c] - p[0][c];
+        }
+        cr[0] = e1[1] * e2[2] - e1[2] * e2[1];
+        cr[1] = e1[2] * e2[0] - e1[0] * e2[2];
+        cr[2] = e1[0] * e2[1] - e1[1] * e2[0];
+        double cen = cr[0] * (p[0][0] + p[1][0] + p[2][0]) + cr[1] * (p[0][1] + p[1][1] + p[2][1]) +
+                     cr[2] * (p[0][2] + p[1][2] + p[2][2]);
+        if (cen < 0) {
+            for (int c = 0; c < 3; ++c) {
+                std::swap(p[1][c], p[2][c]);
+                std::swap(n[1][c], n[2][c]);
+            }
+        }
+        push_tri(m, p, n, random_albedo(r), obj);
+    }
+}
+
+// Rigid transform T(c) * R(axis, angle) as a column-major float matrix.
+void rigid(const double c[3], const double axis[3], double angle, float out[16]) {
+    const double s = std::sin(angle), co = std::cos(angle), ic = 1.0 - co;
+    const double x = axis[0], y = axis[1], z = axis[2];
+    double R[3][3] = {{co + x * x * ic, x * y * ic - z * s, x * z * ic + y * s},
+                      {y * x * ic + z * s, co + y * y * ic, y * z * ic - x * s},
+                      {z * x * ic - y * s, z * y * ic + x * s, co + z * z * ic}};
+    identity(out);
+    for (int row = 0; row < 3; ++row)
+        for (int col = 0; col < 3; ++col) out[col * 4 + row] = (float)R[row][col];
+    out[12] = (float)c[0];
+    out[13] = (float)c[1];
+    out[14] = (float)c[2];
+}
+
+double reflect_into(double p, double B) {
+    double u = std::fmod(p + B, 4.0 * B);
+    if (u < 0) u += 4.0 * B;
+    return u < 2.0 * B ? u - B : 3.0 * B - u;
+}
+
+struct ObjMotion {
+    double c0[3], vel[3], axis[3], omega;
+};
+
+}  // namespace
+
+struct vct_app {
+    vct_app_config cfg{};
+    vct_ctx *ctx = nullptr;
+    std::string err;
+    Mesh mesh;
+    std::vector<ObjMotion> motion;
+    bool dynamic = false;
+    std::vector<vct_light> lights;
+    float extent_min[3], extent_max[3], center[3];
+    vct_camera cam{};
+    vct_trace_params trace{};
+    std::vector<float> model;  // scratch
+};
+
+static int app_fail(vct_app *a, int code, const std::string &msg) {
+    if (a) a->err = msg;
+    return code;
+}
+
+static void make_dir_light(Rng &r, double bound, int sm, vct_light &L) {
+    std::memset(&L, 0, sizeof(L));
+    const double az = r.uni(0.0, 6.283185307179586), el = r.uni(0.5235987755982988, 1.3089969389957472);
+    const float dir[3] = {(float)(-std::cos(el) * std::cos(az)), (float)(-std::sin(el)),
+                          (float)(-std::cos(el) * std::sin(az))};
+    const float dist = (float)(bound * 1.7);
+    const float pos[3] = {-dir[0] * dist, -dir[1] * dist, -dir[2] * dist};
+    std::memcpy(L.position, pos, 12);
+    std::memcpy(L.direction, dir, 12);
+    L.color[0] = L.color[1] = L.color[2] = 1.0f;
+    L.intensity = 1.0f;
+    L.range = 5.0f;
+    L.type = VCT_LIGHT_DIRECTIONAL;
+    L.enabled = 1;
+    L.shadow_caster = 1;
+    L.shadow_size = (uint32_t)sm;
+    L.perspective = 0;
+    // lp = ortho(-b, b, -b, b, 0, far); lv = lookAt(pos, pos + dir, up)  (Application.cpp:207-210)
+    float lp[16], lv[16];
+    const float b = (float)bound;
+    ortho(-b, b, -b, b, 0.0f, 2.0f * dist, lp);
+    const float tgt[3] = {pos[0] + dir[0], pos[1] + dir[1], pos[2] + dir[2]};
+    const float up[3] = {0.0f, 1.0f, 0.0f};
+    lookat(pos, tgt, up, lv);
+    mul4(lp, lv, L.ls);
+    inverse4(L.ls, L.ls_inverse);
+}
+
+static void make_point_light(Rng &r, int sm, vct_light &L) {
+    std::memset(&L, 0, sizeof(L));
+    const float pos[3] = {(float)r.uni(-0.8, 0.8), (float)r.uni(-0.8, 0.8), (float)r.uni(-0.8, 0.8)};
+    std::memcpy(L.position, pos, 12);
+    L.direction[2] = -1.0f;
+    for (int k = 0; k < 3; ++k) L.color[k] = (float)r.uni(0.5, 1.0);
+    L.intensity = 1.0f;
+    L.range = (float)r.uni(0.8, 1.6);
+    L.type = VCT_LIGHT_POINT;
+    L.enabled = 1;
+    L.shadow_caster = 1;
+    L.shadow_size = (uint32_t)sm;
+    L.perspective = 1;
+    // one 120-degree perspective shadow map aimed at the scene centre
+    float lp[16], lv[16];
+    perspective(2.0943951f, 1.0f, 0.02f, 4.0f, lp);
+    float tgt[3] = {0.0f, 0.0f, 0.0f};
+    const float d2 = (pos[0] * pos[0] + pos[1] * pos[1]) + pos[2] * pos[2];
+    if (d2 < 0.0025f) tgt[1] = pos[1] - 1.0f;
+    float up[3] = {0.0f, 1.0f, 0.0f};
+    const float f[3] = {tgt[0] - pos[0], tgt[1] - pos[1], tgt[2] - pos[2]};
+    if (std::fabs(f[1]) > 0.99f * std::sqrt((f[0] * f[0] + f[1] * f[1]) + f[2] * f[2])) {
+        up[1] = 0.0f;
+        up[2] = 1.0f;
+    }
+    lookat(pos, tgt, up, lv);
+    mul4(lp, lv, L.ls);
+    inverse4(L.ls, L.ls_inverse);
+}
+
+static bool inside_any_box(const Mesh &m, const std::vector<std::array<double, 6>> &boxes, const double p[3]) {
+    (void)m;
+    for (auto &b : boxes) {
+        bool in = true;
+        for (int k = 0; k < 3; ++k)
+            if (std::fabs(p[k] - b[k]) > b[3 + k] + 0.05) in = false;
+        if (in) return true;
+    }
+    return false;
+}
+
+static void make_camera(Rng &r, const std::vector<std::array<double, 6>> &boxes, const Mesh &m, double range, int W,
+                        int H, float znear, vct_camera &cam) {
+    double eye[3];
+    for (int tries = 0; tries < 1000; ++tries) {
+        for (int k = 0; k < 3; ++k) eye[k] = r.uni(-range, range);
+        if (!inside_any_box(m, boxes, eye)) break;
+    }
+    const double yaw = r.uni(0.0, 6.283185307179586), pitch = r.uni(-0.3, 0.3);
+    const float e[3] = {(float)eye[0], (float)eye[1], (float)eye[2]};
+    const float front[3] = {(float)(std::cos(pitch) * std::cos(yaw)), (float)std::sin(pitch),
+                            (float)(std::cos(pitch) * std::sin(yaw))};
+    const float tgt[3] = {e[0] + front[0], e[1] + front[1], e[2] + front[2]};
+    const float up[3] = {0.0f, 1.0f, 0.0f};
+    std::memset(&cam, 0, sizeof(cam));
+    cam.width = W;
+    cam.height = H;
+    std::memcpy(cam.eye, e, 12);
+    lookat(e, tgt, up, cam.view);
+    // explicit 60-degree vertical field of view (the reference passes 45 as
+    // radians to glm::perspective, src/Camera.h:18 + Application.cpp:200)
+    perspective(1.0471975512f, (float)W / (float)H, znear, 10.0f, cam.projection);
+    cam.cull_back = 1;
+}
+
+extern "C" {
+
+int vct_app_default_config(int scene, vct_app_config *o) {
+    if (!o || (scene != 1 && scene != 2)) return VCT_ERR_INVALID;
+    std::memset(o, 0, sizeof(*o));
+    o->scene = scene;
+    o->seed = (uint64_t)scene;
+    o->storage = VCT_STORE_FP16;
+    o->lighting = 1;
+    o->msaa = 1;
+    o->fill_holes = 0;
+    o->device = 0;
+    if (scene == 1) {
+        o->dim = 64;
+        o->levels = 7;
+        o->anisotropic = 0;
+        o->width = o->height = 256;
+        o->shadow_size = 512;
+        o->reflections = 0;
+    } else {
+        o->dim = 256;
+        o->levels = 8;
+        o->anisotropic = 1;
+        o->width = 1920;
+        o->height = 1080;
+        o->shadow_size = 1024;
+        o->reflections = 1;
+    }
+    return VCT_OK;
+}
+
+static void fill_defaults(vct_app_config &c) {
+    vct_app_config d;
+    vct_app_default_config(c.scene, &d);
+    if (!c.seed) c.seed = d.seed;
+    if (c.dim <= 0) c.dim = d.dim;
+    if (c.levels <= 0) c.levels = d.levels;
+    if (c.storage < 0) c.storage = d.storage;
+    if (c.anisotropic < 0) c.anisotropic = d.anisotropic;
+    if (c.width <= 0) c.width = d.width;
+    if (c.height <= 0) c.height = d.height;
+    if (c.shadow_size <= 0) c.shadow_size = d.shadow_size;
+    if (c.lighting < 0) c.lighting = d.lighting;
+    if (c.msaa < 0) c.msaa = d.msaa;
+    if (c.reflections < 0) c.reflections = d.reflections;
+    if (c.fill_holes < 0) c.fill_holes = d.fill_holes;
+}
+
+int vct_app_create(vct_app **out, const vct_app_config *cfg_in) {
+    if (!out || !cfg_in) return VCT_ERR_INVALID;
+    *out = nullptr;
+    if (cfg_in->scene != 1 && cfg_in->scene != 2) return VCT_ERR_INVALID;
+    vct_app *a = new vct_app();
+    a->cfg = *cfg_in;
+    fill_defaults(a->cfg);
+    const vct_app_config &c = a->cfg;
+    Rng r(c.seed * 0x9E3779B97F4A7C15ull + 12345);
+    std::vector<std::array<double, 6>> boxes;
+    std::vector<double> centers;
+    double bound;
+    if (c.scene == 1) {
+        // room [-1,1]^3 (12 inward triangles) + 2 boxes (centres U[-0.6,0.6]^3,
+        // half-extents U[0.1,0.3])
+        const double room[3] = {1, 1, 1};
+        add_box(a->mesh, room, true, 0, r);
+        centers.insert(centers.end(), {0, 0, 0});
+        for (int b = 0; b < 2; ++b) {
+            double cen[3], h[3];
+            for (int k = 0; k < 3; ++k) cen[k] = r.uni(-0.6, 0.6);
+            for (int k = 0; k < 3; ++k) h[k] = r.uni(0.1, 0.3);
+            add_box(a->mesh, h, false, (uint32_t)(b + 1), r);
+            centers.insert(centers.end(), {cen[0], cen[1], cen[2]});
+            boxes.push_back({cen[0], cen[1], cen[2], h[0], h[1], h[2]});
+        }
+        a->mesh.nobj = 3;
+        // voxel extent +-(1 + 1/(dim-1)): the room walls fall on voxel centres
+        const float e = (float)(1.0 + 1.0 / (double)(c.dim - 1));
+        for (int k = 0; k < 3; ++k) {
+            a->extent_min[k] = -e;
+            a->extent_max[k] = e;
+            a->center[k] = 0.0f;
+        }
+        bound = 1.8;
+    } else {
+        // 2000 boxes (half-extents lognormal(mu=-3, sigma=0.5)), 64 icospheres
+        // (subdivision 4, radius U[0.02,0.1]); each object moves rigidly
+        const int nbox = 2000, nsph = 64;
+        for (int b = 0; b < nbox; ++b) {
+            double h[3];
+            for (int k = 0; k < 3; ++k) h[k] = std::exp(-3.0 + 0.5 * r.normal());
+            add_box(a->mesh, h, false, (uint32_t)b, r);
+            double cen[3];
+            for (int k = 0; k < 3; ++k) cen[k] = r.uni(-0.9, 0.9);
+            centers.insert(centers.end(), {cen[0], cen[1], cen[2]});
+            boxes.push_back({cen[0], cen[1], cen[2], h[0], h[1], h[2]});
+        }
+        for (int s = 0; s < nsph; ++s) {
+            double rad = r.uni(0.02, 0.1);
+            add_icosphere(a->mesh, rad, 4, (uint32_t)(nbox + s), r);
+            double cen[3];
+            for (int k = 0; k < 3; ++k) cen[k] = r.uni(-0.9, 0.9);
+            centers.insert(centers.end(), {cen[0], cen[1], cen[2]});
+            boxes.push_back({cen[0], cen[1], cen[2], rad, rad, rad});
+        }
+        a->mesh.nobj = nbox + nsph;
+        a->mesh.rough.resize(a->mesh.nobj);
+        for (auto &v : a->mesh.rough) v = (float)r.uni(0.1, 0.8);
+        for (int k = 0; k < 3; ++k) {
+            a->extent_min[k] = -1.0f;
+            a->extent_max[k] = 1.0f;
+            a->center[k] = 0.0f;
+        }
+        bound = 1.8;
+        a->dynamic = true;
+    }
+    // per-object motion (static scenes keep omega = vel = 0)
+    a->motion.resize(a->mesh.nobj);
+    for (int o = 0; o < a->mesh.nobj; ++o) {
+        ObjMotion &mo = a->motion[o];
+        for (int k = 0; k < 3; ++k) mo.c0[k] = centers[(size_t)o * 3 + k];
+        double ax[3] = {r.normal(), r.normal(), r.normal()};
+        double l = std::sqrt(ax[0] * ax[0] + ax[1] * ax[1] + ax[2] * ax[2]);
+        if (l < 1e-12) {
+            ax[0] = 1;
+            l = 1;
+        }
+        for (int k = 0; k < 3; ++k) mo.axis[k] = ax[k] / l;
+        if (a->dynamic) {
+            mo.omega = r.uni(0.0, 5.0 * 3.141592653589793 / 180.0);
+            for (int k = 0; k < 3; ++k) mo.vel[k] = r.uni(-0.01, 0.01);
+        } else {
+            mo.omega = 0.0;
+            for (int k = 0; k < 3; ++k) mo.vel[k] = 0.0;
+        }
+    }
+    // lights
+    if (c.scene == 1) {
+        a->lights.resize(1);
+        make_dir_light(r, bound, c.shadow_size, a->lights[0]);
+    } else {
+        a->lights.resize(5);
+        make_dir_light(r, bound, c.shadow_size, a->lights[0]);
+        for (int i = 1; i < 5; ++i) make_point_light(r, c.shadow_size, a->lights[i]);
+    }
+    make_camera(r, boxes, a->mesh, c.scene == 1 ? 0.5 : 0.6, c.width, c.height, c.scene == 1 ? 0.05f : 0.02f,
+                a->cam);
+    // trace settings: VCTSettings defaults (Application.h:96-98)
+    vct_trace_params &t = a->trace;
+    std::memset(&t, 0, sizeof(t));
+    t.diffuse = {16, 1.0471975512f, 1.0f, 1.0f, 0.5f};
+    t.specular = {32, 0.5235987756f, 1.7f, 0.5f, 0.1f};
+    t.enable_reflections = c.reflections;
+    t.cone_angle_from_roughness = 1;
+    t.draw_occlusion = 1;
+    t.ambient_scale = 1.0f;
+    t.reflect_scale = 1.0f;
+    std::memcpy(t.eye, a->cam.eye, 12);
+    // GPU context
+    vct_grid_desc d;
+    std::memset(&d, 0, sizeof(d));
+    d.dim = c.dim;
+    d.levels = c.levels;
+    std::memcpy(d.center, a->center, 12);
+    std::memcpy(d.vmin, a->extent_min, 12);
+    std::memcpy(d.vmax, a->extent_max, 12);
+    d.storage = c.storage;
+    d.anisotropic = c.anisotropic;
+    d.device = c.device;
+    a->model.resize((size_t)a->mesh.nobj * 16);
+    if (c.device < 0) {  // scene and uniforms only, no GPU context
+        *out = a;
+        return VCT_OK;
+    }
+    int rc = vct_create(&a->ctx, &d);
+    if (rc) {
+        delete a;
+        return rc;
+    }
+    vct_mesh m;
+    m.num_tris = a->mesh.ntri();
+    m.num_objects = a->mesh.nobj;
+    m.positions = a->mesh.pos.data();
+    m.normals = a->mesh.nrm.data();
+    m.tangents = a->mesh.tan.data();
+    m.albedo = a->mesh.albedo.data();
+    m.object = a->mesh.obj.data();
+    m.roughness = a->mesh.rough.empty() ? nullptr : a->mesh.rough.data();
+    if ((rc = vct_set_scene(a->ctx, &m)) || (rc = vct_set_lights(a->ctx, a->lights.data(), (int)a->lights.size()))) {
+        vct_app_destroy(a);
+        return rc;
+    }
+    *out = a;
+    return VCT_OK;
+}
+
+void vct_app_destroy(vct_app *a) {
+    if (!a) return;
+    if (a->ctx) vct_destroy(a->ctx);
+    delete a;
+}
+
+const char *vct_app_last_error(const vct_app *a) {
+    if (!a) return vct_last_error(nullptr);
+    if (!a->err.empty()) return a->err.c_str();
+    return vct_last_error(a->ctx);
+}
+
+vct_ctx *vct_app_ctx(vct_app *a) { return a ? a->ctx : nullptr; }
+
+int vct_app_get_config(const vct_app *a, vct_app_config *out) {
+    if (!a || !out) return VCT_ERR_INVALID;
+    *out = a->cfg;
+    return VCT_OK;
+}
+
+int vct_app_scene_sizes(const vct_app *a, int *nt, int *no, int *nl) {
+    if (!a) return VCT_ERR_INVALID;
+    if (nt) *nt = a->mesh.ntri();
+    if (no) *no = a->mesh.nobj;
+    if (nl) *nl = (int)a->lights.size();
+    return VCT_OK;
+}
+
+int vct_app_get_mesh(const vct_app *a, float *pos, float *nrm, float *tan, uint32_t *albedo, uint32_t *obj,
+                     float *rough) {
+    if (!a) return VCT_ERR_INVALID;
+    const Mesh &m = a->mesh;
+    if (pos) std::memcpy(pos, m.pos.data(), m.pos.size() * 4);
+    if (nrm) std::memcpy(nrm, m.nrm.data(), m.nrm.size() * 4);
+    if (tan) std::memcpy(tan, m.tan.data(), m.tan.size() * 4);
+    if (albedo) std::memcpy(albedo, m.albedo.data(), m.albedo.size() * 4);
+    if (obj) std::memcpy(obj, m.obj.data(), m.obj.size() * 4);
+    if (rough && !m.rough.empty()) std::memcpy(rough, m.rough.data(), m.rough.size() * 4);
+    return m.rough.empty() ? 0 : 1;
+}
+
+static void frame_models(const vct_app *a, int frame, float *out) {
+    for (int o = 0; o < a->mesh.nobj; ++o) {
+        const ObjMotion &mo = a->motion[o];
+        double c[3];
+        for (int k = 0; k < 3; ++k) c[k] = a->dynamic ? reflect_into(mo.c0[k] + mo.vel[k] * frame, 0.95) : mo.c0[k];
+        rigid(c, mo.axis, mo.omega * frame, out + (size_t)o * 16);
+    }
+}
+
+int vct_app_frame_state(vct_app *a, int frame, float *model, vct_light *lights, vct_voxelize_params *vox,
+                        vct_inject_params *inj, vct_mip_params *mip, vct_camera *cam, vct_trace_params *trace) {
+    if (!a) return VCT_ERR_INVALID;
+    if (model) frame_models(a, frame, model);
+    if (lights) std::memcpy(lights, a->lights.data(), a->lights.size() * sizeof(vct_light));
+    if (vox) {
+        std::memset(vox, 0, sizeof(*vox));
+        voxel_projections(a->center, a->extent_min, a->extent_max, vox->mvp_x, vox->mvp_y, vox->mvp_z);
+        vox->axis_override = -1;
+        vox->lighting = a->cfg.lighting;
+        vox->atomic_max = 1;
+        vox->msaa = a->cfg.msaa;
+        std::memcpy(vox->eye, a->cam.eye, 12);
+    }
+    if (inj) {
+        inj->voxel_set_opacity = 0.5f;
+        inj->light_mask = 0xffffffffu;
+    }
+    if (mip) {
+        mip->kernel_mode = 0;
+        mip->albedo_chain = 0;
+    }
+    if (cam) *cam = a->cam;
+    if (trace) *trace = a->trace;
+    return VCT_OK;
+}
+
+int vct_app_build_radiance(vct_app *a, int frame) {
+    if (!a) return VCT_ERR_INVALID;
+    if (!a->ctx) return app_fail(a, VCT_ERR_STATE, "vct_app_build_radiance: app was created without a device");
+    vct_voxelize_params vox;
+    vct_inject_params inj;
+    vct_mip_params mip;
+    vct_app_frame_state(a, frame, a->model.data(), nullptr, &vox, &inj, &mip, nullptr, nullptr);
+    int rc;
+    if ((rc = vct_set_transforms(a->ctx, a->model.data(), a->mesh.nobj))) return rc;
+    for (size_t i = 0; i < a->lights.size(); ++i)
+        if (a->lights[i].shadow_caster && (rc = vct_shadowmap(a->ctx, (int)i))) return rc;
+    if ((rc = vct_voxelize(a->ctx, &vox))) return rc;
+    if ((rc = vct_normalize_inject(a->ctx, &inj))) return rc;
+    if (a->cfg.fill_holes && (rc = vct_fill_holes(a->ctx))) return rc;
+    if ((rc = vct_mip(a->ctx, &mip))) return rc;
+    return VCT_OK;
+}
+
+int vct_app_render(vct_app *a, int frame) {
+    if (!a) return VCT_ERR_INVALID;
+    int rc = vct_app_build_radiance(a, frame);
+    if (rc) return rc;
+    if ((rc = vct_gbuffer(a->ctx, &a->cam))) return rc;
+    return vct_trace(a->ctx, &a->trace);
+}
+
+void vct_math_ortho(float l, float r, float b, float t, float n, float f, float out[16]) { ortho(l, r, b, t, n, f, out); }
+void vct_math_lookat(const float eye[3], const float center[3], const float up[3], float out[16]) {
+    lookat(eye, center, up, out);
+}
+void vct_math_perspective(float fovy, float aspect, float n, float f, float out[16]) {
+    perspective(fovy, aspect, n, f, out);
+}
+int vct_math_inverse4(const float m[16], float out[16]) { return inverse4(m, out) ? 1 : 0; }
+void vct_math_mul4(const float a[16], const float b[16], float out[16]) { mul4(a, b, out); }
+void vct_math_voxel_projections(const float c[3], const float mn[3], const float mx[3], float mvp_x[16],
+                                float mvp_y[16], float mvp_z[16]) {
+    voxel_projections(c, mn, mx, mvp_x, mvp_y, mvp_z);
+}
+
+}  // extern "C"
