@@ -1,16 +1,19 @@
 """Benchmark: per-frame voxelize + inject + mip + cone trace on one MI355X.
 
-Workload: the full-size dynamic scene (2000 boxes + 64 subdivision-4
-icospheres = 351,680 triangles moving rigidly every frame, 256^3 grid with 8
-levels of 6-direction fp16 mips, 1 directional + 4 point lights with 1024^2
-shadow maps, 6 diffuse + 1 specular cone at 1920x1080).  A step is one frame:
-every object transform changes, and the scene is re-voxelized, re-injected,
-re-filtered and re-traced.
+Workload (default, --scene 2): the full-size dynamic scene -- 2000 boxes + 64
+subdivision-4 icospheres = 351,680 triangles, every object moving rigidly
+each frame; 256^3 grid with 8 levels of 6-direction fp16 mips; 1 directional +
+4 point lights with 1024^2 shadow maps; 6 diffuse + 1 specular cone per pixel
+at 1920x1080.  A step is one frame: all transforms change and the scene is
+re-voxelized, re-injected, re-filtered and re-traced.
 
-Prints one JSON line.  `value` is ms per frame of the four metric passes
-(voxelize + normalize/inject + mip + trace), from HIP events on the context
-stream; the shadow-map and G-buffer passes the frame also runs are reported
-beside it.
+One JSON line.  `value` = ms per frame of the four metric passes (voxelize +
+normalize/inject + mip + trace) from HIP events recorded on the context
+stream around each pass; the shadow-map and G-buffer passes each frame also
+needs are reported beside it (`passes`, `frame_ms_all_passes`).
+`roofline` is for the dominant kernel (the cone trace): algorithmic bytes per
+launch / its average event-timed duration.  `cpu_baseline` times the oracle
+(the CPU restatement, one thread) on a bounded sample of the same frame.
 """
 from __future__ import annotations
 
@@ -25,6 +28,76 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 PEAK_HBM_GBS = 8000.0
+METRIC = "ms/frame voxelize+inject+mip+trace @256^3,1080p; GB/s vs peak; learned vs plain"
+
+
+def level_sizes(dim: int, levels: int):
+    return [max(1, dim >> l) for l in range(levels)]
+
+
+def algorithmic_bytes(cfg, ntri: int, nlights_sm: int, sm: int) -> dict:
+    """Compulsory HBM bytes per launch of each pass (the per-unit figures are
+    listed in DESIGN.md)."""
+    dim, L = cfg.dim, cfg.levels
+    nv = dim ** 3
+    bpv = {0: 8, 1: 16, 2: 8}[cfg.storage]  # fp8 mode keeps levels 0-2 in fp16
+    sizes = level_sizes(dim, L)
+    faces = 6 if cfg.anisotropic else 1
+    upper = sum(s ** 3 for s in sizes[1:])
+    pyramid = nv * bpv + faces * upper * bpv
+    npix = cfg.width * cfg.height
+    out_px = 16 * (2 if cfg.reflections else 1)
+    return {
+        # triangles (positions + normals + albedo + object id) read once, albedo + normal grids written once
+        "voxelize": ntri * (72 + 4 + 4) + 2 * 4 * nv,
+        # albedo read, lit mask written + read, shadow maps read, radiance level 0 written
+        "radiance": 4 * nv + 2 * nv + nlights_sm * sm * sm * 4 + bpv * nv,
+        # level 0 read, every level >= 1 (all faces) written
+        "mip": bpv * nv + faces * upper * bpv,
+        # G-buffer read, outputs written, the whole pyramid read once
+        "trace": npix * 64 + npix * out_px + pyramid,
+    }
+
+
+def cpu_baseline(app, frame: int, pixel_stride: int = 64) -> dict:
+    """Times the oracle (single thread) on one frame: shadow maps, voxelize,
+    normalize+inject and the mip chain in full, the trace on every
+    `pixel_stride`-th G-buffer pixel; reports the frame's
+    voxelize+inject+mip+trace time with the trace scaled to all pixels."""
+    os.environ["OMP_NUM_THREADS"] = "1"
+    import numpy as np
+
+    from oracle import binding as ob
+
+    cfg = app.cfg
+    mesh, fs = app.mesh(), app.frame_state(frame)
+    g = app.ctx.grid()
+    sc = ob.SceneArrays(mesh, fs["model"])
+    L = fs["lights"]
+    sms = [ob.shadowmap(sc, l) if l.shadow_caster else None for l in L]
+    t0 = time.perf_counter()
+    alb, _, _ = ob.voxelize(sc, L, sms, fs["vox"], cfg.dim)
+    t1 = time.perf_counter()
+    _, rad0, _, _ = ob.normalize_inject(alb, cfg.dim, tuple(g.center), tuple(g.vmin), tuple(g.vmax), L, sms)
+    t2 = time.perf_counter()
+    levels = ob.mip(rad0, cfg.levels)
+    faces = ob.mip_aniso(rad0, cfg.levels) if cfg.anisotropic else None
+    t3 = time.perf_counter()
+    gb = ob.gbuffer(sc, fs["cam"]).reshape(-1, 16)
+    sample = np.ascontiguousarray(gb[::pixel_stride])
+    tp = ob.trace_params(fs["trace"], cfg.dim, cfg.levels, tuple(g.center), tuple(g.vmin), tuple(g.vmax),
+                         mesh["rough"] is not None, cfg.anisotropic)
+    t4 = time.perf_counter()
+    ob.trace(sample, levels, tp, aniso_faces=faces)
+    t5 = time.perf_counter()
+    trace_ms = (t5 - t4) * 1e3 * (gb.shape[0] / sample.shape[0])
+    total = (t1 - t0) * 1e3 + (t2 - t1) * 1e3 + (t3 - t2) * 1e3 + trace_ms
+    return {"value": round(total, 1), "unit": "ms/frame", "cores": 1, "kind": "port",
+            "sample": f"oracle (CPU restatement, 1 thread) on frame {frame}: voxelize, normalize+inject and the "
+                      f"mip chain in full, the cone trace on 1/{pixel_stride} of the {gb.shape[0]} pixels "
+                      f"scaled to the full frame (not llvmpipe: no GL context on this host)",
+            "passes_ms": {"voxelize": round((t1 - t0) * 1e3, 1), "radiance": round((t2 - t1) * 1e3, 1),
+                          "mipmap": round((t3 - t2) * 1e3, 1), "render": round(trace_ms, 1)}}
 
 
 def main() -> int:
@@ -33,12 +106,12 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--scene", type=int, default=2)
-    ap.add_argument("--storage", type=int, default=-1)
+    ap.add_argument("--storage", type=int, default=-1, help="0 fp16, 1 fp32, 2 fp8 (levels >= 3)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU design is described in DESIGN.md)", file=sys.stderr)
+              "(how the path would shard is described in DESIGN.md)", file=sys.stderr)
         return 2
 
     import vct_amd
@@ -47,6 +120,10 @@ def main() -> int:
     ctx = app.ctx
     cfg = app.cfg
     ntri, nobj, nlights = app.sizes()
+    fs = app.frame_state(0)
+    n_sm = sum(1 for L in fs["lights"] if L.shadow_caster)
+    print(f"bench: scene {args.scene}, {ntri} tris, warmup {args.warmup}, steps {args.steps}", file=sys.stderr,
+          flush=True)
     for f in range(args.warmup):
         app.render(f)
     ctx.sync()
@@ -58,19 +135,19 @@ def main() -> int:
     wall_ms = (time.perf_counter() - t0) * 1e3 / args.steps
     pt = ctx.pass_times()
     k = float(args.steps)
-    passes = {
-        "shadowmap_ms": pt.shadowmap_ms / k,
-        "voxelize_ms": pt.voxelize_ms / k,
-        "radiance_ms": pt.radiance_ms / k,
-        "mipmap_ms": pt.mipmap_ms / k,
-        "gbuffer_ms": pt.gbuffer_ms / k,
-        "render_ms": pt.render_ms / k,
-    }
+    passes = {"shadowmap_ms": pt.shadowmap_ms / k, "voxelize_ms": pt.voxelize_ms / k,
+              "radiance_ms": pt.radiance_ms / k, "mipmap_ms": pt.mipmap_ms / k, "gbuffer_ms": pt.gbuffer_ms / k,
+              "render_ms": pt.render_ms / k}
     metric_ms = passes["voxelize_ms"] + passes["radiance_ms"] + passes["mipmap_ms"] + passes["render_ms"]
     npix = cfg.width * cfg.height
-    cones = npix * (6 + (1 if cfg.reflections else 0))
+    ncones = 6 + (1 if cfg.reflections else 0)
+    ab = algorithmic_bytes(cfg, ntri, n_sm, cfg.shadow_size)
+    per_pass_gbs = {name: round(ab[name] / (passes[key] * 1e-3) / 1e9, 1)
+                    for name, key in (("voxelize", "voxelize_ms"), ("radiance", "radiance_ms"),
+                                      ("mip", "mipmap_ms"), ("trace", "render_ms")) if passes[key] > 0}
+    trace_gbs = ab["trace"] / (passes["render_ms"] * 1e-3) / 1e9
     out = {
-        "metric": "ms/frame voxelize+inject+mip+trace @256^3,1080p; GB/s vs peak; learned vs plain",
+        "metric": METRIC,
         "value": round(metric_ms, 4),
         "unit": "ms/frame",
         "n_gpus": 1,
@@ -80,16 +157,27 @@ def main() -> int:
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 accumulate, fp16 storage" if cfg.storage == 0 else ("fp32" if cfg.storage == 1 else "fp8"),
+        "dtype": {0: "fp16 storage, fp32 accumulate", 1: "fp32", 2: "fp8 e4m3 (levels>=3) + fp16, fp32 accumulate"}[
+            cfg.storage],
         "data": "synthetic",
-        "config": {"workload": f"scene {args.scene}: {ntri} tris, {nobj} objects, {nlights} lights, "
-                               f"{cfg.dim}^3 x {cfg.levels} levels, {'6-dir' if cfg.anisotropic else 'iso'} mips, "
-                               f"{cfg.width}x{cfg.height}, {6 + (1 if cfg.reflections else 0)} cones/px",
+        "config": {"workload": f"scene {args.scene}: {ntri} tris in {nobj} moving objects, {nlights} lights with "
+                               f"{cfg.shadow_size}^2 shadow maps, {cfg.dim}^3 x {cfg.levels} levels "
+                               f"{'6-direction' if cfg.anisotropic else 'isotropic'} mips, "
+                               f"{cfg.width}x{cfg.height}, {ncones} cones/px",
                    "parallelism": "single GPU"},
         "passes": {k2: round(v, 4) for k2, v in passes.items()},
         "frame_ms_all_passes": round(sum(passes.values()), 4),
-        "cones_per_s": round(cones / (passes["render_ms"] * 1e-3), 1) if passes["render_ms"] > 0 else None,
+        "cones_per_s": round(npix * ncones / (passes["render_ms"] * 1e-3), 1) if passes["render_ms"] > 0 else None,
+        "pass_algorithmic_gbs": per_pass_gbs,
+        "roofline": {"kernel": "k_trace", "bound": "hbm", "achieved": round(trace_gbs, 1), "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": round(trace_gbs / PEAK_HBM_GBS, 4), "traffic": None,
+                     "algorithmic_bytes": ab["trace"]},
     }
+    if not args.no_cpu_baseline:
+        try:
+            out["cpu_baseline"] = cpu_baseline(app, args.warmup + args.steps)
+        except Exception as e:  # the baseline is reported, never required
+            out["cpu_baseline"] = {"value": None, "error": repr(e)}
     print(json.dumps(out), flush=True)
     app.close()
     return 0

@@ -61,9 +61,14 @@ extern "C" {
 #define VCT_BUF_INDIRECT 5   /* float4 [width*height]: (indirect.rgb*occlusion,
                                  occlusion) -- phong.frag:489 debugIndirect view  */
 #define VCT_BUF_SPECULAR 6   /* float4 [width*height]: specular cone rgba          */
-#define VCT_BUF_LITMASK 7    /* u32 [dim^3/32] bit per voxel lit by any light     */
+#define VCT_BUF_LITMASK 7    /* u8 [dim^3]: 1 where a voxel is lit by any light   */
 #define VCT_BUF_RADIANCE_RAW 8 /* raw stored bytes of mip `level` (layout-native) */
-#define VCT_BUF_COMPOSED 9   /* float4 [width*height]: composed indirect term     */
+#define VCT_BUF_LABELS 10    /* float4 [npoints]: 64-cone reference (rgb*occ, occ)  */
+#define VCT_BUF_FEATURES 11  /* u16 (bf16) [npoints][160]: learned-gather inputs   */
+#define VCT_BUF_REFERENCE 12 /* float4 [width*height]: 64-cone reference gather    */
+#define VCT_BUF_WEIGHTS 13   /* float [VCT_MLP_PARAMS + 1]: FP32 weights, out scale */
+#define VCT_BUF_GRADIENT 14  /* float [VCT_MLP_PARAMS + 1]: last step's minibatch
+                                 gradient (sum of the workgroup partials) + loss   */
 
 /* Grid description: the VCT resource owner (src/Application.h:107-156). */
 typedef struct vct_grid_desc {
@@ -263,8 +268,46 @@ size_t vct_buffer_bytes(const vct_ctx *ctx, int what, int level);
 int vct_download(vct_ctx *ctx, int what, int level, void *host, size_t bytes);
 /* Overwrites a context buffer from host memory (test and replay hook:
  * VCT_BUF_ALBEDO, VCT_BUF_NORMAL, VCT_BUF_RADIANCE level 0,
- * VCT_BUF_SHADOWMAP, VCT_BUF_GBUFFER, VCT_BUF_LITMASK). */
+ * VCT_BUF_SHADOWMAP, VCT_BUF_GBUFFER, VCT_BUF_LITMASK, VCT_BUF_WEIGHTS). */
 int vct_upload(vct_ctx *ctx, int what, int level, const void *host, size_t bytes);
+
+/* ---- learned diffuse gather ------------------------------------------------
+ * No reference counterpart: a small MLP stands in for the 6-cone diffuse loop
+ * of phong.frag:480-486, and the plain 6-cone tracer remains the per-8x8-tile
+ * fallback.  Network 129 -> 64 -> 64 -> 64 -> 3 (ReLU), BF16 MFMA operands,
+ * FP32 accumulation.  Inputs: 27 trilinear samples of mip level 2 on a 3x3x3
+ * stencil one level-2 voxel apart, the gather axis TBN*(0,1,0), and
+ * sin/cos(pi f p) of the texture-space position for f = 1, 2, 4.  Output: the
+ * gather's rgb * occlusion.  Labels: the FP64-accumulated 64-cone reference
+ * (64 cosine-weighted 20-degree cones around the same axis, quarter-diameter
+ * steps).  Needs an isotropic pyramid with at least 3 levels. */
+#define VCT_MLP_PARAMS 16835
+
+typedef struct vct_learned_params {
+    int batch;        /* rows per Adam step, a multiple of 64 (4096)              */
+    int steps;        /* Adam steps (20000)                                       */
+    float lr, beta1, beta2, eps; /* 1e-3, 0.9, 0.999, 1e-8                        */
+    uint32_t seed;    /* minibatch sampling and weight initialisation             */
+    int reset;        /* 1: He-uniform weights and zero Adam state before training */
+} vct_learned_params;
+
+/* Training surface points (G-buffer texels, host memory). */
+int vct_learned_set_points(vct_ctx *ctx, const vct_gbuffer_texel *points, int npoints);
+/* 64-cone labels and network inputs of the points on the current mip chain. */
+int vct_learned_prepare(vct_ctx *ctx);
+/* Trains on the prepared points; loss (host, [steps]) may be NULL. Blocks.
+ * Adam's step count t runs on across calls until reset or a VCT_BUF_WEIGHTS
+ * upload (which also zeroes the moments); row j of step t's minibatch is
+ * point h(seed ^ h(t * 0x9e3779b9 + j)) % npoints, h the 32-bit mixer
+ * x ^= x>>16; x *= 0x7feb352d; x ^= x>>15; x *= 0x846ca68b; x ^= x>>16. */
+int vct_learned_train(vct_ctx *ctx, const vct_learned_params *p, float *loss);
+/* Learned gather over the current G-buffer into VCT_BUF_INDIRECT: (rgb, 1)
+ * from the network, or the 6-cone tracer's (rgb*occ, occ) on tiles whose
+ * predicted-luminance variance exceeds variance_threshold.  fallback_tiles
+ * (may be NULL; non-NULL blocks) receives the number of such tiles. */
+int vct_trace_learned(vct_ctx *ctx, const vct_trace_params *p, float variance_threshold, int *fallback_tiles);
+/* The 64-cone reference of the current G-buffer into VCT_BUF_REFERENCE. */
+int vct_reference_gather(vct_ctx *ctx);
 
 #ifdef __cplusplus
 }

@@ -63,6 +63,12 @@ int vct_app_get_mesh(const vct_app *app, float *pos, float *nrm, float *tan, uin
 int vct_app_frame_state(vct_app *app, int frame, float *model, vct_light *lights, vct_voxelize_params *vox,
                         vct_inject_params *inj, vct_mip_params *mip, vct_camera *cam, vct_trace_params *trace);
 
+/* `n` seeded surface points of frame `frame` as G-buffer texels (training
+ * points of the learned gather): triangles picked with probability
+ * proportional to their world-space area, uniform barycentrics, attributes
+ * interpolated exactly as the G-buffer resolve interpolates them. */
+int vct_app_surface_points(vct_app *app, int frame, int n, uint64_t seed, vct_gbuffer_texel *out);
+
 /* One frame in the order of Application::render: transforms, shadow maps,
  * voxelize, normalize+inject, [fill holes], mip, G-buffer, trace. */
 int vct_app_render(vct_app *app, int frame);

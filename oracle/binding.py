@@ -224,7 +224,7 @@ def normalize_inject(albedo: np.ndarray, dim, center, vmin, vmax, lights, shadow
     sms = [np.ascontiguousarray(s, np.float32) if s is not None else None for s in shadowmaps]
     ptrs = (C.c_void_p * max(1, len(sms)))(*[(s.ctypes.data if s is not None else None) for s in sms])
     rad = np.zeros((dim ** 3, 4), np.float32)
-    lm = np.zeros((dim ** 3 + 31) // 32, np.uint32)
+    lm = np.zeros(dim ** 3, np.uint8)
     st = Stats()
     lib().or_normalize_inject(_p(alb), dim, _p(c), _p(a), _p(b), C.cast(la, C.c_void_p), len(lights),
                               C.cast(ptrs, C.c_void_p), light_mask, opacity, 1 if fp32 else 0, _p(rad), _p(lm),

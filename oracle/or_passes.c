@@ -373,9 +373,9 @@ static float q16(float x) { return or_half_to_float(or_float_to_half(x)); }
 void or_normalize_inject(uint32_t *albedo, int dim, const float center[3], const float vmin[3],
                          const float vmax[3], const or_light *lights, int nlights,
                          const float *const *shadowmaps, uint32_t light_mask, float opacity,
-                         int storage_fp32, float *rad0, uint32_t *litmask, or_stats *stats) {
+                         int storage_fp32, float *rad0, uint8_t *litmask, or_stats *stats) {
     size_t nv = (size_t)dim * dim * dim;
-    memset(litmask, 0, (nv + 31) / 32 * sizeof(uint32_t));
+    memset(litmask, 0, nv);
     /* injectRadiance.comp:152-168 per shadow texel, as a lit-mask scatter */
     for (int li = 0; li < nlights; ++li) {
         const or_light *L = &lights[li];
@@ -400,7 +400,7 @@ void or_normalize_inject(uint32_t *albedo, int dim, const float center[3], const
                 if (idx[0] < 0 || idx[1] < 0 || idx[2] < 0 || idx[0] >= dim || idx[1] >= dim || idx[2] >= dim)
                     continue;
                 size_t vi = ((size_t)idx[2] * dim + idx[1]) * dim + idx[0];
-                litmask[vi >> 5] |= 1u << (vi & 31);
+                litmask[vi] = 1;
             }
     }
     /* transferVoxels.comp:37-62 (+ normalizeVoxels.comp:101-105 for rgba16f
@@ -416,7 +416,7 @@ void or_normalize_inject(uint32_t *albedo, int dim, const float center[3], const
             if (mf > stats->max_fragments_per_voxel) stats->max_fragments_per_voxel = mf;
             if (opacity > 0.0f) c[3] = opacity;
             albedo[vi] = or_pack_unorm4x8(c);
-            int lit = (litmask[vi >> 5] >> (vi & 31)) & 1u;
+            int lit = litmask[vi] != 0;
             if (lit) stats->lit_voxels += 1;
             r[0] = lit ? c[0] : 0.0f;
             r[1] = lit ? c[1] : 0.0f;

@@ -111,12 +111,12 @@ void or_voxelize(const or_scene *sc, const or_light *lights, int nlights,
  * (shaders/transferVoxels.comp:29-70, normalizeVoxels.comp:89-106,
  * injectRadiance.comp:40-103).  albedo is updated in place (alpha rewrite),
  * rad0 receives float4 radiance already rounded to the storage precision
- * (storage_fp32 = 1: no rounding; else fp16). litmask [dim^3/32] is written. */
+ * (storage_fp32 = 1: no rounding; else fp16). litmask [dim^3] (1 = lit) is written. */
 void or_normalize_inject(uint32_t *albedo, int dim, const float center[3],
                          const float vmin[3], const float vmax[3],
                          const or_light *lights, int nlights, const float *const *shadowmaps,
                          uint32_t light_mask, float opacity, int storage_fp32,
-                         float *rad0, uint32_t *litmask, or_stats *stats);
+                         float *rad0, uint8_t *litmask, or_stats *stats);
 
 /* voxelFillHoles.comp (shaders/voxelFillHoles.comp:8-36) on a float4 grid whose
  * values are quantised by `quant` (0 = fp16 rounding, 1 = none). */
@@ -184,6 +184,15 @@ void or_trace_cone(float *const *levels, float *const *const *aniso, int dim, in
 /* Trilinear LINEAR_MIPMAP_LINEAR sample with border 0 and lambda clamp. */
 void or_sample(float *const *levels, int dim, int nlevels, const float p[3], float lambda,
                float out[4]);
+
+/* FP64 64-cone reference of the diffuse gather (labels of the learned gather;
+ * see or_label.c for the exact definition).  out: [npix][4] = (rgb *
+ * occlusion, occlusion).  Isotropic pyramids only. */
+void or_label_dirs(double dirs[64][3], double w[64]);
+void or_label_cone64(float *const *levels, int dim, int nlevels, const double pos[3], const double nrm[3],
+                     const double dir[3], double tan_half, double out[4]);
+void or_label64(const or_gbuf_texel *g, int npix, float *const *levels, int dim, int nlevels,
+                const float center[3], const float vmin[3], const float vmax[3], double *out);
 
 #ifdef __cplusplus
 }

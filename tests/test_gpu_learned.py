@@ -1,0 +1,209 @@
+"""GPU parity of the learned diffuse gather (scene 1, 64^3, isotropic).
+
+  labels     k_label64 (FP64 accumulation) vs the oracle's or_label64
+  features   k_features vs a numpy restatement of the input definition
+  gradients  one k_mlp_train minibatch vs the FP64 reference network with BF16
+             rounding at the same operand points (mlp_ref, bf16=True)
+  training   the loss falls; the trained network beats the plain 6-cone
+             gather against the 64-cone reference
+  inference  k_learned output vs the reference network on the same inputs;
+             fallback tiles carry the plain tracer's result
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from gpu_util import make_app, rel_err
+from oracle import binding as ob
+from oracle import mlp_ref
+import vct_amd
+from vct_amd import _abi as abi
+
+pytestmark = pytest.mark.gpu
+
+NPTS = 4096
+
+
+@pytest.fixture(scope="module")
+def prepared():
+    app = make_app(1)
+    app.build_radiance(0)
+    ctx = app.ctx
+    pts = app.surface_points(NPTS, 0, seed=5)
+    ctx.learned_set_points(pts)
+    ctx.learned_prepare()
+    g = ctx.grid()
+    levels = [ctx.radiance(l) for l in range(g.levels)]
+    yield dict(app=app, ctx=ctx, pts=pts, levels=levels, g=g)
+    app.close()
+
+
+def _mix_hash(x):
+    x = np.uint32(x)
+    x ^= x >> np.uint32(16)
+    x = np.uint32((int(x) * 0x7feb352d) & 0xFFFFFFFF)
+    x ^= x >> np.uint32(15)
+    x = np.uint32((int(x) * 0x846ca68b) & 0xFFFFFFFF)
+    x ^= x >> np.uint32(16)
+    return int(x)
+
+
+def _picks(seed, step, batch, npts):
+    return np.array([_mix_hash(seed ^ _mix_hash((step * 0x9e3779b9 + r) & 0xFFFFFFFF)) % npts
+                     for r in range(batch)])
+
+
+def test_labels_match_fp64_oracle(prepared):
+    """The GPU samples the pyramid in FP32 and accumulates in FP64, the oracle
+    does both in FP64: a sample can move across the 0.999 exit or the volume
+    border, so the bar is a mean relative error, not bit equality."""
+    ctx, pts, levels, g = prepared["ctx"], prepared["pts"], prepared["levels"], prepared["g"]
+    lab = ctx.download(abi.VCT_BUF_LABELS, 0, np.float32).reshape(-1, 4)
+    L = ob.lib()
+    L.or_label64.restype = None
+    L.or_label64.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int] + [C.c_void_p] * 4
+    sub = np.ascontiguousarray(pts[:256])
+    lv = [np.ascontiguousarray(a, np.float32) for a in levels]
+    lp = (C.c_void_p * len(lv))(*[a.ctypes.data for a in lv])
+    out = np.zeros((256, 4))
+    c, mn, mx = (np.asarray(list(v), np.float32) for v in (g.center, g.vmin, g.vmax))
+    L.or_label64(sub.ctypes.data, 256, C.cast(lp, C.c_void_p), g.dim, g.levels, c.ctypes.data, mn.ctypes.data,
+                 mx.ctypes.data, out.ctypes.data)
+    assert out[:, :3].sum() > 0
+    assert rel_err(lab[:256, :3], out[:, :3]) < 1e-4
+    assert np.abs(lab[:256, 3] - out[:, 3]).max() < 1e-3
+
+
+def _features_ref(pts, levels, g):
+    """numpy restatement of the 160-wide input row (before BF16 rounding)."""
+    c, mn, mx = (np.asarray(list(v), np.float64) for v in (g.center, g.vmin, g.vmax))
+    p = (pts[:, 0:3] - c - mn) / (mx - mn)
+    s2 = g.dim >> 2
+    lv2 = levels[2].astype(np.float64)
+    X = np.zeros((len(pts), 160))
+
+    def lin(q):
+        u = q * s2 - 0.5
+        i0 = np.floor(u).astype(int)
+        f = u - i0
+        acc = np.zeros((len(q), 4))
+        for cc in range(8):
+            d = np.array([cc & 1, (cc >> 1) & 1, cc >> 2])
+            idx = i0 + d
+            w = np.prod(np.where(d == 1, f, 1 - f), axis=1)
+            ok = np.all((idx >= 0) & (idx < s2), axis=1)
+            iv = np.clip(idx, 0, s2 - 1)
+            acc += (w * ok)[:, None] * lv2[iv[:, 2], iv[:, 1], iv[:, 0]]
+        return acc
+
+    for i in range(27):
+        o = np.array([i % 3 - 1, (i // 3) % 3 - 1, i // 9 - 1]) / s2
+        X[:, i * 4:i * 4 + 4] = lin(p + o)
+    b = pts[:, 12:15].astype(np.float64)
+    X[:, 108:111] = b / np.linalg.norm(b, axis=1, keepdims=True)
+    k = 111
+    for cc in range(3):
+        for f in (1, 2, 4):
+            X[:, k] = np.sin(np.pi * f * p[:, cc])
+            X[:, k + 1] = np.cos(np.pi * f * p[:, cc])
+            k += 2
+    return X
+
+
+def _bf16_to_f(u16):
+    return (u16.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+
+
+def test_features_match_definition(prepared):
+    ctx, pts, levels, g = prepared["ctx"], prepared["pts"], prepared["levels"], prepared["g"]
+    X = _bf16_to_f(ctx.download(abi.VCT_BUF_FEATURES, 0, np.uint16).reshape(-1, 160))
+    ref = _features_ref(pts[:512].astype(np.float64), levels, g)
+    # BF16 keeps 8 significant bits; the GPU computes in FP32 before rounding
+    err = np.abs(X[:512] - ref)
+    assert np.all(err <= np.abs(ref) * 2 ** -7 + 1e-6), err.max()
+    assert np.all(X[:, 129:] == 0)
+
+
+def _out_scale(lab):
+    return max(np.abs(lab[:, :3].astype(np.float64)).sum() / (3.0 * lab.shape[0]), 1e-6)
+
+
+def test_gradient_matches_reference(prepared):
+    """One training step from uploaded weights: the summed per-workgroup
+    gradient equals the reference network's with BF16 rounding at the MFMA
+    operands (FP32 vs FP64 sums and rare BF16 rounding flips apart)."""
+    ctx = prepared["ctx"]
+    lab4 = ctx.download(abi.VCT_BUF_LABELS, 0, np.float32).reshape(-1, 4)
+    scale = np.float32(_out_scale(lab4))
+    params = mlp_ref.init_params(7)
+    rng = np.random.default_rng(8)
+    params = [(W, rng.normal(0, 0.05, b.shape)) for W, b in params]
+    w = np.append(mlp_ref.flatten(params), scale).astype(np.float32)
+    ctx.set_weights(w)
+    params = mlp_ref.unflatten(w[:-1].astype(np.float64))  # the FP32 values the GPU holds
+    ctx.learned_train(steps=1, batch=256, lr=1e-3, seed=11, reset=False)
+    grad = ctx.download(abi.VCT_BUF_GRADIENT, 0, np.float32)
+    X = _bf16_to_f(ctx.download(abi.VCT_BUF_FEATURES, 0, np.uint16).reshape(-1, 160))[:, :129]
+    pick = _picks(11, 1, 256, NPTS)
+    t = (lab4[pick, :3] / scale).astype(np.float64)  # FP32 division, as the kernel does
+    loss, grads = mlp_ref.loss_and_grads(params, X[pick], t, bf16=True)
+    gref = mlp_ref.flatten([(gW, gb) for gW, gb in grads])
+    assert abs(grad[-1] - loss) <= 1e-4 * loss + 1e-12
+    assert rel_err(grad[:-1], gref) < 1e-3, rel_err(grad[:-1], gref)
+    # and the Adam step that followed: the first step moves each weight by
+    # lr * g / (|g| + eps) with the GPU's own gradient
+    w1 = ctx.weights()
+    g = grad[:-1].astype(np.float64)
+    want = w[:-1].astype(np.float64) - 1e-3 * g / (np.abs(g) + 1e-8)
+    assert np.abs(w1[:-1] - want).max() < 1e-6
+
+
+def test_training_reduces_loss_and_beats_plain_gather(prepared):
+    app, ctx = prepared["app"], prepared["ctx"]
+    loss = ctx.learned_train(steps=600, batch=1024, lr=2e-3, seed=3, reset=True)
+    assert np.isfinite(loss).all()
+    assert loss[-50:].mean() < 0.5 * loss[:10].mean()
+    # learned vs plain 6-cone gather, both judged by the 64-cone reference
+    fs = app.frame_state(0)
+    ctx.gbuffer(fs["cam"])
+    tr = fs["trace"]
+    tr.enable_reflections = 0
+    ctx.trace(tr)
+    plain = ctx.download(abi.VCT_BUF_INDIRECT, 0, np.float32).reshape(-1, 4)
+    ctx.reference_gather()
+    ref = ctx.download(abi.VCT_BUF_REFERENCE, 0, np.float32).reshape(-1, 4)
+    ctx.trace_learned(tr, 1e9)  # no fallback
+    learned = ctx.download(abi.VCT_BUF_INDIRECT, 0, np.float32).reshape(-1, 4)
+    e_plain = rel_err(plain[:, :3], ref[:, :3])
+    e_learned = rel_err(learned[:, :3], ref[:, :3])
+    print(f"mean relative error vs 64-cone reference: plain {e_plain:.4f}, learned {e_learned:.4f}")
+    assert e_learned < e_plain
+
+
+def test_inference_matches_reference_network_and_fallback(prepared):
+    app, ctx = prepared["app"], prepared["ctx"]
+    fs = app.frame_state(0)
+    ctx.gbuffer(fs["cam"])
+    cam = fs["cam"]
+    gb = ctx.download(abi.VCT_BUF_GBUFFER, 0, np.float32).reshape(-1, 16)
+    tr = fs["trace"]
+    tr.enable_reflections = 0
+    ctx.trace(tr)
+    plain = ctx.download(abi.VCT_BUF_INDIRECT, 0, np.float32).reshape(-1, 4)
+    nfb = ctx.trace_learned(tr, 1e9)
+    assert nfb == 0
+    out = ctx.download(abi.VCT_BUF_INDIRECT, 0, np.float32).reshape(-1, 4)
+    w = ctx.weights()
+    params = mlp_ref.unflatten(w[:-1].astype(np.float64))
+    covered = gb[:, 11].view(np.uint32) == 1
+    idx = np.flatnonzero(covered)[::97]
+    X = _features_ref(gb[idx].astype(np.float64), prepared["levels"], prepared["g"])
+    Xb = mlp_ref.bf16_round(X)[:, :129]
+    y, _ = mlp_ref.forward(params, Xb, bf16=True)
+    assert rel_err(out[idx, :3], y * w[-1]) < 5e-3
+    # a threshold of -1 sends every tile to the plain tracer
+    nfb = ctx.trace_learned(tr, -1.0)
+    assert nfb == ((cam.width + 7) // 8) * ((cam.height + 7) // 8)
+    out = ctx.download(abi.VCT_BUF_INDIRECT, 0, np.float32).reshape(-1, 4)
+    assert np.array_equal(out.view(np.uint32), plain.view(np.uint32))

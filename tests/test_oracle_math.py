@@ -1,0 +1,122 @@
+"""Pins the GLM-convention matrices of the oracle and of the host application
+layer by closed forms, and checks the two produce identical bits (the parity
+tests feed both the same uniforms)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import vct_amd
+
+app_lib = vct_amd.load_app_library()
+
+
+def host(fn, *args):
+    out = np.zeros(16, np.float32)
+    getattr(app_lib, fn)(*args, C.c_void_p(out.ctypes.data))
+    return out
+
+
+def v3(*a):
+    return np.asarray(a, np.float32)
+
+
+def M(m):  # column-major flat -> row-major 4x4 for numpy algebra
+    return np.asarray(m, np.float64).reshape(4, 4).T
+
+
+def test_ortho_maps_box_corners_to_clip_cube(oracle):
+    l, r, b, t, n, f = -2.0, 3.0, -1.0, 4.0, 0.5, 7.0
+    m = M(oracle.ortho(l, r, b, t, n, f))
+    for x, ex in ((l, -1), (r, 1)):
+        for y, ey in ((b, -1), (t, 1)):
+            for z, ez in ((-n, -1), (-f, 1)):  # RH: the view looks down -z
+                c = m @ np.array([x, y, z, 1.0])
+                assert np.allclose(c, [ex, ey, ez, 1], atol=1e-6)
+    assert np.array_equal(oracle.ortho(l, r, b, t, n, f), host("vct_math_ortho", l, r, b, t, n, f))
+
+
+def test_lookat_eye_to_origin_target_on_minus_z(oracle):
+    rng = np.random.default_rng(0)
+    for _ in range(50):
+        eye, tgt = rng.uniform(-3, 3, 3).astype(np.float32), rng.uniform(-3, 3, 3).astype(np.float32)
+        up = v3(0, 1, 0)
+        m = M(oracle.lookat(eye, tgt, up))
+        assert np.allclose(m @ np.r_[eye, 1], [0, 0, 0, 1], atol=1e-5)
+        d = m @ np.r_[tgt, 1]
+        dist = np.linalg.norm(tgt.astype(np.float64) - eye)
+        assert np.allclose(d[:3], [0, 0, -dist], atol=1e-4)
+        assert np.allclose(m[:3, :3] @ m[:3, :3].T, np.eye(3), atol=1e-5)  # rotation
+        ho = np.zeros(16, np.float32)
+        app_lib.vct_math_lookat(eye.ctypes.data, tgt.ctypes.data, up.ctypes.data, ho.ctypes.data)
+        assert np.array_equal(ho, oracle.lookat(eye, tgt, up))
+
+
+def test_perspective_near_far_planes(oracle):
+    fovy, aspect, n, f = 1.0471975512, 16 / 9, 0.05, 10.0
+    m = M(oracle.perspective(fovy, aspect, n, f))
+    for z, ez in ((-n, -1.0), (-f, 1.0)):
+        c = m @ np.array([0, 0, z, 1.0])
+        assert abs(c[2] / c[3] - ez) < 1e-5
+    top = n * np.tan(fovy / 2)
+    c = m @ np.array([0, top, -n, 1.0])
+    assert abs(c[1] / c[3] - 1) < 1e-5
+    c = m @ np.array([top * aspect, 0, -n, 1.0])
+    assert abs(c[0] / c[3] - 1) < 1e-5
+    assert np.array_equal(oracle.perspective(fovy, aspect, n, f), host("vct_math_perspective", fovy, aspect, n, f))
+
+
+def test_inverse_closed_form_and_host_bits(oracle):
+    rng = np.random.default_rng(1)
+    for _ in range(100):
+        m = rng.normal(size=16).astype(np.float32)
+        inv = oracle.inverse4(m)
+        assert np.allclose(M(m) @ M(inv), np.eye(4), atol=1e-3)
+        h = np.zeros(16, np.float32)
+        assert app_lib.vct_math_inverse4(m.ctypes.data, h.ctypes.data) == 1
+        assert np.array_equal(h, inv)
+    assert oracle.inverse4(np.zeros(16, np.float32)) is None
+
+
+def test_normal_matrix_of_rotation_and_scale(oracle):
+    # rotation: the normal matrix is the rotation itself
+    a = 0.7
+    R = np.array([[np.cos(a), -np.sin(a), 0], [np.sin(a), np.cos(a), 0], [0, 0, 1]])
+    m = np.eye(4)
+    m[:3, :3] = R
+    m[:3, 3] = [0.3, -2, 5]
+    nm = oracle.normal_matrices(m.T.reshape(1, 16).astype(np.float32))[0].reshape(3, 3).T
+    assert np.allclose(nm, R, atol=1e-6)
+    # scale: inverse-transpose of diag(s) is diag(1/s)
+    s = np.diag([2.0, 0.5, 4.0, 1.0])
+    nm = oracle.normal_matrices(s.T.reshape(1, 16).astype(np.float32))[0].reshape(3, 3).T
+    assert np.allclose(nm, np.diag([0.5, 2.0, 0.25]), atol=1e-7)
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_axis_projection_plus_swizzle_is_linear_voxel_position(oracle, axis):
+    """mvp_axis (Application.cpp:689-692) followed by getVoxelPosition's
+    NDC->unit swizzle (voxelize.frag:79-98) equals voxelLinearPosition
+    (p - center - min) / (max - min) for every axis."""
+    rng = np.random.default_rng(2 + axis)
+    for center, half in (((0, 0, 0), 1.0), ((0.3, -0.2, 0.1), 1.5)):
+        c = np.asarray(center, np.float32)
+        mn, mx = v3(-half, -half, -half), v3(half, half, half)
+        mats = oracle.voxel_projections(c, mn, mx)
+        hx, hy, hz = (np.zeros(16, np.float32) for _ in range(3))
+        app_lib.vct_math_voxel_projections(c.ctypes.data, mn.ctypes.data, mx.ctypes.data, hx.ctypes.data,
+                                           hy.ctypes.data, hz.ctypes.data)
+        for a, b in zip(mats, (hx, hy, hz)):
+            assert np.array_equal(a, b)
+        mvp = M(mats[axis])
+        for _ in range(50):
+            p = c + rng.uniform(-half, half, 3)
+            ndc = (mvp @ np.r_[p, 1])[:3]
+            u = (ndc + 1) * 0.5
+            if axis == 0:
+                u = np.array([1 - u[2], u[1], u[0]])
+            elif axis == 1:
+                u = np.array([u[0], 1 - u[2], u[1]])
+            u[2] = 1 - u[2]
+            expect = (p - c - mn) / (mx - mn)
+            assert np.allclose(u, expect, atol=2e-6), (axis, p, u, expect)

@@ -196,7 +196,35 @@ class Context:
         return a.reshape(n, n)
 
     def litmask(self) -> np.ndarray:
-        return self.download(abi.VCT_BUF_LITMASK, 0, np.uint32)
+        return self.download(abi.VCT_BUF_LITMASK, 0, np.uint8)
+
+    # --- learned diffuse gather ------------------------------------------------
+    def learned_set_points(self, points: np.ndarray):
+        points = np.ascontiguousarray(points, np.float32).reshape(-1, 16)
+        self.check(self.lib.vct_learned_set_points(self.h, _ptr(points), points.shape[0]))
+
+    def learned_prepare(self):
+        self.check(self.lib.vct_learned_prepare(self.h))
+
+    def learned_train(self, steps=20000, batch=4096, lr=1e-3, seed=1, reset=False) -> np.ndarray:
+        p = abi.LearnedParams(batch, steps, lr, 0.9, 0.999, 1e-8, seed, 1 if reset else 0)
+        loss = np.zeros(max(steps, 1), np.float32)
+        self.check(self.lib.vct_learned_train(self.h, C.byref(p), _ptr(loss)))
+        return loss[:steps]
+
+    def trace_learned(self, p: abi.TraceParams, threshold: float, count_fallback=True) -> int:
+        n = C.c_int(0)
+        self.check(self.lib.vct_trace_learned(self.h, C.byref(p), threshold, C.byref(n) if count_fallback else None))
+        return n.value
+
+    def reference_gather(self):
+        self.check(self.lib.vct_reference_gather(self.h))
+
+    def weights(self) -> np.ndarray:
+        return self.download(abi.VCT_BUF_WEIGHTS, 0, np.float32)
+
+    def set_weights(self, w: np.ndarray):
+        self.upload(abi.VCT_BUF_WEIGHTS, 0, np.ascontiguousarray(w, np.float32))
 
 
 class App:
@@ -262,6 +290,11 @@ class App:
         self.check(self.lib.vct_app_frame_state(self.h, frame, _ptr(model), lights, C.byref(vox), C.byref(inj),
                                                 C.byref(mip), C.byref(cam), C.byref(tr)))
         return dict(model=model, lights=list(lights), vox=vox, inj=inj, mip=mip, cam=cam, trace=tr)
+
+    def surface_points(self, n: int, frame: int = 0, seed: int = 1) -> np.ndarray:
+        out = np.zeros((n, 16), np.float32)
+        self.check(self.lib.vct_app_surface_points(self.h, frame, n, seed, _ptr(out)))
+        return out
 
     def render(self, frame: int = 0):
         self.check(self.lib.vct_app_render(self.h, frame))

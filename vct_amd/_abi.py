@@ -30,6 +30,13 @@ VCT_BUF_INDIRECT = 5
 VCT_BUF_SPECULAR = 6
 VCT_BUF_LITMASK = 7
 VCT_BUF_RADIANCE_RAW = 8
+VCT_BUF_LABELS = 10
+VCT_BUF_FEATURES = 11
+VCT_BUF_REFERENCE = 12
+VCT_BUF_WEIGHTS = 13
+VCT_BUF_GRADIENT = 14
+VCT_MLP_PARAMS = 16835
+MLP_IN = 160
 
 F16 = C.c_float * 16
 
@@ -99,6 +106,11 @@ class AppConfig(C.Structure):
                 ("fill_holes", C.c_int), ("device", C.c_int)]
 
 
+class LearnedParams(C.Structure):
+    _fields_ = [("batch", C.c_int), ("steps", C.c_int), ("lr", C.c_float), ("beta1", C.c_float),
+                ("beta2", C.c_float), ("eps", C.c_float), ("seed", C.c_uint32), ("reset", C.c_int)]
+
+
 GBUFFER_TEXEL_BYTES = 64
 
 # (name, restype, argtypes) of every function include/vct.h declares
@@ -127,6 +139,11 @@ VCT_FUNCS = [
     ("vct_buffer_bytes", C.c_size_t, [C.c_void_p, C.c_int, C.c_int]),
     ("vct_download", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_size_t]),
     ("vct_upload", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_size_t]),
+    ("vct_learned_set_points", C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
+    ("vct_learned_prepare", C.c_int, [C.c_void_p]),
+    ("vct_learned_train", C.c_int, [C.c_void_p, C.POINTER(LearnedParams), C.c_void_p]),
+    ("vct_trace_learned", C.c_int, [C.c_void_p, C.POINTER(TraceParams), C.c_float, C.POINTER(C.c_int)]),
+    ("vct_reference_gather", C.c_int, [C.c_void_p]),
 ]
 
 APP_FUNCS = [
@@ -143,6 +160,7 @@ APP_FUNCS = [
                                       C.POINTER(Camera), C.POINTER(TraceParams)]),
     ("vct_app_render", C.c_int, [C.c_void_p, C.c_int]),
     ("vct_app_build_radiance", C.c_int, [C.c_void_p, C.c_int]),
+    ("vct_app_surface_points", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_uint64, C.c_void_p]),
     ("vct_math_ortho", None, [C.c_float] * 6 + [C.c_void_p]),
     ("vct_math_lookat", None, [C.c_void_p] * 4),
     ("vct_math_perspective", None, [C.c_float] * 4 + [C.c_void_p]),

@@ -16,6 +16,9 @@
 #include <string>
 #include <vector>
 
+#include <random>
+
+#include "learned.h"
 #include "vct_internal.h"
 
 using namespace vctk;
@@ -42,7 +45,9 @@ struct vct_ctx {
     std::vector<DevLight> hlights;
     DevLight *d_lights = nullptr;
     // voxel grids
-    uint32_t *d_grid_albedo = nullptr, *d_grid_normal = nullptr, *d_litmask = nullptr;
+    uint32_t *d_grid_albedo = nullptr, *d_grid_normal = nullptr;
+    uint8_t *d_litmask = nullptr;
+    uint32_t *d_partials = nullptr;  // per-workgroup statistics partials
     char *d_pyr = nullptr;
     float *d_scales = nullptr;
     size_t pyr_bytes = 0, scale_floats = 0;
@@ -62,6 +67,19 @@ struct vct_ctx {
     // staging for downloads
     float4 *d_stage = nullptr;
     size_t stage_bytes = 0;
+    // learned gather
+    vct_gbuffer_texel *d_points = nullptr;
+    int npoints = 0;
+    float4 *d_labels = nullptr;
+    uint16_t *d_X = nullptr;
+    bool have_prepared = false, have_weights = false;
+    MlpBuffers mlp;
+    float *d_loss = nullptr;
+    int loss_cap = 0, last_train_blocks = 0;
+    int adam_t = 0;  // Adam steps taken since the weights were initialised or uploaded
+    uint32_t *d_tiles = nullptr, *d_tile_count = nullptr;
+    int tiles_cap = 0;
+    float4 *d_reference = nullptr;
     // timing
     bool timing = true;
     std::vector<hipEvent_t> ev_pool;
@@ -70,6 +88,7 @@ struct vct_ctx {
 };
 
 static thread_local std::string g_err;
+static int ensure_mlp(vct_ctx *c);
 
 static int fail(vct_ctx *c, int code, const char *fmt, ...) {
     char buf[512];
@@ -157,10 +176,10 @@ static int alloc_grids(vct_ctx *c) {
     const size_t nv = (size_t)dim * dim * dim;
     HIPCHK(c, dalloc(&c->d_grid_albedo, nv * 4));
     HIPCHK(c, dalloc(&c->d_grid_normal, nv * 4));
-    HIPCHK(c, dalloc(&c->d_litmask, (nv + 31) / 32 * 4));
+    HIPCHK(c, dalloc(&c->d_litmask, nv));
     HIPCHK(c, hipMemsetAsync(c->d_grid_albedo, 0, nv * 4, c->stream));
     HIPCHK(c, hipMemsetAsync(c->d_grid_normal, 0, nv * 4, c->stream));
-    HIPCHK(c, hipMemsetAsync(c->d_litmask, 0, (nv + 31) / 32 * 4, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_litmask, 0, nv, c->stream));
     PyrDesc &P = c->pyr;
     memset(&P, 0, sizeof(P));
     P.dim = dim;
@@ -265,7 +284,8 @@ int vct_create(vct_ctx **out, const vct_grid_desc *desc) {
         delete c;
         return fail(nullptr, VCT_ERR_HIP, "hipStreamCreate failed");
     }
-    if (hipMalloc((void **)&c->d_stats, sizeof(vct_stats)) != hipSuccess ||
+    if (hipMalloc((void **)&c->d_partials, kStatPartials * 4 * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc((void **)&c->d_stats, sizeof(vct_stats)) != hipSuccess ||
         hipMemset(c->d_stats, 0, sizeof(vct_stats)) != hipSuccess) {
         vct_destroy(c);
         return fail(nullptr, VCT_ERR_NOMEM, "stats allocation failed");
@@ -306,7 +326,26 @@ void vct_destroy(vct_ctx *c) {
     dfree(c->d_ind);
     dfree(c->d_spec);
     dfree(c->d_stage);
+    dfree(c->d_points);
+    dfree(c->d_labels);
+    dfree(c->d_X);
+    dfree(c->mlp.master);
+    dfree(c->mlp.m1);
+    dfree(c->mlp.m2);
+    dfree(c->mlp.w1);
+    dfree(c->mlp.w2);
+    dfree(c->mlp.w3);
+    dfree(c->mlp.w4);
+    dfree(c->mlp.w2t);
+    dfree(c->mlp.w3t);
+    dfree(c->mlp.w4t);
+    dfree(c->mlp.partials);
+    dfree(c->d_loss);
+    dfree(c->d_tiles);
+    dfree(c->d_tile_count);
+    dfree(c->d_reference);
     dfree(c->d_stats);
+    dfree(c->d_partials);
     for (auto &tp : c->pending) {
         (void)hipEventDestroy(tp.a);
         (void)hipEventDestroy(tp.b);
@@ -565,7 +604,7 @@ int vct_voxelize(vct_ctx *c, const vct_voxelize_params *p) {
     if (!c->d_model) return fail(c, VCT_ERR_STATE, "vct_voxelize: no scene");
     PassTimer pt(c, P_VOXELIZE);
     LAUNCHCHK(c, launch_voxelize(dev_scene(c), c->d_lights, (int)c->lights.size(), *p, c->desc.dim,
-                                 c->d_grid_albedo, c->d_grid_normal, c->d_stats, c->rs, c->stream));
+                                 c->d_grid_albedo, c->d_grid_normal, c->d_stats, c->d_partials, c->rs, c->stream));
     c->have_voxels = true;
     c->have_mips = false;
     return VCT_OK;
@@ -584,7 +623,7 @@ int vct_normalize_inject(vct_ctx *c, const vct_inject_params *p) {
     LAUNCHCHK(c, launch_inject_scatter(c->hlights.data(), c->d_lights, (int)c->lights.size(), p->light_mask,
                                        c->desc.dim, ext, c->d_litmask, c->stream));
     LAUNCHCHK(c, launch_normalize_inject(c->d_grid_albedo, c->d_litmask, c->desc.dim, p->voxel_set_opacity, c->pyr,
-                                         c->d_stats, c->stream));
+                                         c->d_stats, c->d_partials, c->stream));
     c->have_radiance = true;
     c->have_mips = false;
     return VCT_OK;
@@ -741,7 +780,7 @@ size_t vct_buffer_bytes(const vct_ctx *c, int what, int level) {
     switch (what) {
     case VCT_BUF_ALBEDO:
     case VCT_BUF_NORMAL: return nv * 4;
-    case VCT_BUF_LITMASK: return (nv + 31) / 32 * 4;
+    case VCT_BUF_LITMASK: return nv;
     case VCT_BUF_RADIANCE:
         if (!decode_level(c, level, &l, &f)) return 0;
         return (size_t)c->pyr.lv[l].size * c->pyr.lv[l].size * c->pyr.lv[l].size * 16;
@@ -755,7 +794,12 @@ size_t vct_buffer_bytes(const vct_ctx *c, int what, int level) {
         return (size_t)c->lights[level].shadow_size * c->lights[level].shadow_size * 4;
     case VCT_BUF_GBUFFER: return (size_t)c->gw * c->gh * sizeof(vct_gbuffer_texel);
     case VCT_BUF_INDIRECT:
-    case VCT_BUF_SPECULAR: return (size_t)c->gw * c->gh * 16;
+    case VCT_BUF_SPECULAR:
+    case VCT_BUF_REFERENCE: return (size_t)c->gw * c->gh * 16;
+    case VCT_BUF_LABELS: return (size_t)c->npoints * 16;
+    case VCT_BUF_FEATURES: return (size_t)c->npoints * kMlpIn * 2;
+    case VCT_BUF_WEIGHTS:
+    case VCT_BUF_GRADIENT: return (size_t)(kMlpParams + 1) * 4;
     default: return 0;
     }
 }
@@ -783,6 +827,33 @@ int vct_download(vct_ctx *c, int what, int level, void *host, size_t bytes) {
     case VCT_BUF_GBUFFER: src = c->d_gbuf; break;
     case VCT_BUF_INDIRECT: src = c->d_ind; break;
     case VCT_BUF_SPECULAR: src = c->d_spec; break;
+    case VCT_BUF_LABELS: src = c->d_labels; break;
+    case VCT_BUF_FEATURES: src = c->d_X; break;
+    case VCT_BUF_REFERENCE:
+        if (!c->d_reference) return fail(c, VCT_ERR_STATE, "vct_download: no reference gather computed");
+        src = c->d_reference;
+        break;
+    case VCT_BUF_GRADIENT: {
+        if (!c->mlp.partials || c->last_train_blocks <= 0)
+            return fail(c, VCT_ERR_STATE, "vct_download: no training step has run");
+        std::vector<float> part((size_t)c->last_train_blocks * kMlpPartialStride);
+        HIPCHK(c, hipMemcpyAsync(part.data(), c->mlp.partials, part.size() * 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        float *o = (float *)host;
+        for (int p = 0; p <= kMlpParams; ++p) {
+            double s = 0.0;
+            for (int b = 0; b < c->last_train_blocks; ++b) s += part[(size_t)b * kMlpPartialStride + p];
+            o[p] = (float)s;
+        }
+        return VCT_OK;
+    }
+    case VCT_BUF_WEIGHTS: {
+        if (!c->mlp.master) return fail(c, VCT_ERR_STATE, "vct_download: no learned weights");
+        HIPCHK(c, hipMemcpyAsync(host, c->mlp.master, (size_t)kMlpParams * 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        ((float *)host)[kMlpParams] = c->mlp.out_scale;
+        return VCT_OK;
+    }
     case VCT_BUF_RADIANCE: {
         decode_level(c, level, &l, &f);
         int rc = ensure_stage(c, need);
@@ -809,6 +880,21 @@ int vct_upload(vct_ctx *c, int what, int level, const void *host, size_t bytes) 
     if (bytes != need) return fail(c, VCT_ERR_INVALID, "vct_upload: %zu bytes given, %zu needed", bytes, need);
     void *dst = nullptr;
     switch (what) {
+    case VCT_BUF_WEIGHTS: {
+        int rc = ensure_mlp(c);
+        if (rc) return rc;
+        const float *w = (const float *)host;
+        if (!(w[kMlpParams] > 0.0f)) return fail(c, VCT_ERR_INVALID, "vct_upload: output scale must be > 0");
+        HIPCHK(c, hipMemcpyAsync(c->mlp.master, w, (size_t)kMlpParams * 4, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemsetAsync(c->mlp.m1, 0, (size_t)kMlpParams * 4, c->stream));
+        HIPCHK(c, hipMemsetAsync(c->mlp.m2, 0, (size_t)kMlpParams * 4, c->stream));
+        c->adam_t = 0;
+        LAUNCHCHK(c, launch_emit_operands(c->mlp, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        c->mlp.out_scale = w[kMlpParams];
+        c->have_weights = true;
+        return VCT_OK;
+    }
     case VCT_BUF_ALBEDO: dst = c->d_grid_albedo; c->have_voxels = true; break;
     case VCT_BUF_NORMAL: dst = c->d_grid_normal; break;
     case VCT_BUF_LITMASK: dst = c->d_litmask; break;
@@ -829,6 +915,196 @@ int vct_upload(vct_ctx *c, int what, int level, const void *host, size_t bytes) 
     }
     HIPCHK(c, hipMemcpyAsync(dst, host, bytes, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return VCT_OK;
+}
+
+
+// ---------------------------------------------------------------- learned gather
+static int ensure_mlp(vct_ctx *c) {
+    if (c->mlp.master) return VCT_OK;
+    MlpBuffers &B = c->mlp;
+    HIPCHK(c, dalloc(&B.master, (size_t)kMlpParams * 4));
+    HIPCHK(c, dalloc(&B.m1, (size_t)kMlpParams * 4));
+    HIPCHK(c, dalloc(&B.m2, (size_t)kMlpParams * 4));
+    HIPCHK(c, dalloc(&B.w1, (size_t)64 * kMlpIn * 2));
+    HIPCHK(c, dalloc(&B.w2, 64 * 64 * 2));
+    HIPCHK(c, dalloc(&B.w3, 64 * 64 * 2));
+    HIPCHK(c, dalloc(&B.w4, 16 * 64 * 2));
+    HIPCHK(c, dalloc(&B.w2t, 64 * 64 * 2));
+    HIPCHK(c, dalloc(&B.w3t, 64 * 64 * 2));
+    HIPCHK(c, dalloc(&B.w4t, 64 * 32 * 2));
+    HIPCHK(c, hipMemset(B.w1, 0, (size_t)64 * kMlpIn * 2));
+    HIPCHK(c, hipMemset(B.w4, 0, 16 * 64 * 2));
+    HIPCHK(c, hipMemset(B.w4t, 0, 64 * 32 * 2));
+    HIPCHK(c, hipMemset(B.master, 0, (size_t)kMlpParams * 4));
+    return VCT_OK;
+}
+
+int vct_learned_set_points(vct_ctx *c, const vct_gbuffer_texel *pts, int n) {
+    if (!c || !pts) return fail(c, VCT_ERR_INVALID, "vct_learned_set_points: NULL argument");
+    if (n < 64 || n > (1 << 24)) return fail(c, VCT_ERR_INVALID, "vct_learned_set_points: %d points (64..2^24)", n);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    dfree(c->d_points);
+    dfree(c->d_labels);
+    dfree(c->d_X);
+    HIPCHK(c, dalloc(&c->d_points, (size_t)n * sizeof(vct_gbuffer_texel)));
+    HIPCHK(c, dalloc(&c->d_labels, (size_t)n * 16));
+    HIPCHK(c, dalloc(&c->d_X, (size_t)n * kMlpIn * 2));
+    HIPCHK(c, hipMemcpy(c->d_points, pts, (size_t)n * sizeof(vct_gbuffer_texel), hipMemcpyHostToDevice));
+    c->npoints = n;
+    c->have_prepared = false;
+    return VCT_OK;
+}
+
+static Extent ctx_extent(const vct_ctx *c) {
+    Extent e;
+    memcpy(e.center, c->desc.center, 12);
+    memcpy(e.vmin, c->desc.vmin, 12);
+    memcpy(e.vmax, c->desc.vmax, 12);
+    return e;
+}
+
+static int learned_ok(vct_ctx *c) {
+    if (c->pyr.aniso) return fail(c, VCT_ERR_INVALID, "the learned gather needs an isotropic mip chain");
+    if (c->pyr.nlevels < 3) return fail(c, VCT_ERR_INVALID, "the learned gather needs mip level 2");
+    if (!c->have_mips) return fail(c, VCT_ERR_STATE, "the mip chain is not built");
+    return VCT_OK;
+}
+
+int vct_learned_prepare(vct_ctx *c) {
+    if (!c) return fail(nullptr, VCT_ERR_INVALID, "ctx is NULL");
+    int rc = learned_ok(c);
+    if (rc) return rc;
+    if (!c->d_points) return fail(c, VCT_ERR_STATE, "vct_learned_prepare: no points");
+    const Extent e = ctx_extent(c);
+    LAUNCHCHK(c, launch_label64(c->pyr, c->d_points, c->npoints, e, c->d_labels, c->stream));
+    LAUNCHCHK(c, launch_features(c->pyr, c->d_points, c->npoints, e, c->d_X, c->stream));
+    // output scale: the mean label magnitude, so the network fits O(1) targets
+    std::vector<float> lab((size_t)c->npoints * 4);
+    HIPCHK(c, hipMemcpyAsync(lab.data(), c->d_labels, lab.size() * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double s = 0.0;
+    for (int i = 0; i < c->npoints; ++i) s += fabs(lab[i * 4]) + fabs(lab[i * 4 + 1]) + fabs(lab[i * 4 + 2]);
+    const double mean = s / (3.0 * c->npoints);
+    c->mlp.out_scale = (float)(mean > 1e-6 ? mean : 1e-6);
+    c->have_prepared = true;
+    return VCT_OK;
+}
+
+int vct_learned_train(vct_ctx *c, const vct_learned_params *p, float *loss) {
+    if (!c || !p) return fail(c, VCT_ERR_INVALID, "vct_learned_train: NULL argument");
+    if (p->batch < 64 || p->batch % 64 || p->batch > 65536)
+        return fail(c, VCT_ERR_INVALID, "batch %d must be a multiple of 64 in [64, 65536]", p->batch);
+    if (p->steps < 0 || p->steps > 1000000) return fail(c, VCT_ERR_INVALID, "steps %d out of range", p->steps);
+    if (!(p->lr > 0.0f)) return fail(c, VCT_ERR_INVALID, "learning rate must be > 0");
+    if (!c->have_prepared) return fail(c, VCT_ERR_STATE, "vct_learned_train: call vct_learned_prepare first");
+    int rc = ensure_mlp(c);
+    if (rc) return rc;
+    MlpBuffers &B = c->mlp;
+    const int nblk = p->batch / 64;
+    if (nblk > B.max_blocks) {
+        dfree(B.partials);
+        HIPCHK(c, dalloc(&B.partials, (size_t)nblk * kMlpPartialStride * 4));
+        B.max_blocks = nblk;
+    }
+    if (p->steps > c->loss_cap) {
+        dfree(c->d_loss);
+        HIPCHK(c, dalloc(&c->d_loss, (size_t)p->steps * 4));
+        c->loss_cap = p->steps;
+    }
+    if (p->reset || !c->have_weights) {
+        // He-uniform initialisation U(-sqrt(6/fan_in), sqrt(6/fan_in)), zero biases
+        std::vector<float> w(kMlpParams, 0.0f);
+        std::mt19937_64 rng(p->seed);
+        const int fin[4] = {129, 64, 64, 64}, fout[4] = {64, 64, 64, 3};
+        size_t o = 0;
+        for (int l = 0; l < 4; ++l) {
+            std::uniform_real_distribution<double> U(-sqrt(6.0 / fin[l]), sqrt(6.0 / fin[l]));
+            for (int i = 0; i < fin[l] * fout[l]; ++i) w[o++] = (float)U(rng);
+            o += fout[l];
+        }
+        HIPCHK(c, hipMemcpy(B.master, w.data(), (size_t)kMlpParams * 4, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemset(B.m1, 0, (size_t)kMlpParams * 4));
+        HIPCHK(c, hipMemset(B.m2, 0, (size_t)kMlpParams * 4));
+        LAUNCHCHK(c, launch_emit_operands(B, c->stream));
+        c->have_weights = true;
+        c->adam_t = 0;
+    }
+    PassTimer pt(c, P_LEARNED);
+    if (p->steps > 0) c->last_train_blocks = nblk;
+    for (int s = 0; s < p->steps; ++s)
+        LAUNCHCHK(c, launch_mlp_train_step(B, c->d_X, c->d_labels, c->npoints, p->batch, p->seed, ++c->adam_t, p->lr,
+                                           p->beta1, p->beta2, p->eps, c->d_loss + s, c->stream));
+    pt.stop();
+    if (loss && p->steps) HIPCHK(c, hipMemcpyAsync(loss, c->d_loss, (size_t)p->steps * 4, hipMemcpyDeviceToHost,
+                                                   c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return VCT_OK;
+}
+
+static TraceLaunch make_trace_launch(vct_ctx *c, const vct_trace_params *p) {
+    TraceLaunch t;
+    t.gbuf = c->d_gbuf;
+    t.width = c->gw;
+    t.height = c->gh;
+    t.out_ind = c->d_ind;
+    t.out_spec = c->d_spec;
+    t.enable_reflections = p->enable_reflections;
+    t.angle_from_roughness = p->cone_angle_from_roughness;
+    t.draw_occlusion = p->draw_occlusion;
+    t.has_roughness = c->has_rough;
+    t.ambient_scale = p->ambient_scale;
+    t.reflect_scale = p->reflect_scale;
+    memcpy(t.eye, p->eye, 12);
+    t.d_steps = p->diffuse.steps;
+    t.d_tan_half = tanf(p->diffuse.cone_angle / 2.0f);
+    t.d_bias = p->diffuse.bias;
+    t.d_h0 = p->diffuse.cone_initial_height;
+    t.d_lod_offset = p->diffuse.lod_offset;
+    t.s_steps = p->specular.steps;
+    t.s_angle = p->specular.cone_angle;
+    t.s_bias = p->specular.bias;
+    t.s_h0 = p->specular.cone_initial_height;
+    t.s_lod_offset = p->specular.lod_offset;
+    t.ext = ctx_extent(c);
+    return t;
+}
+
+int vct_trace_learned(vct_ctx *c, const vct_trace_params *p, float thr, int *fallback) {
+    if (!c || !p) return fail(c, VCT_ERR_INVALID, "vct_trace_learned: NULL argument");
+    int rc = learned_ok(c);
+    if (rc) return rc;
+    if (!c->have_gbuf) return fail(c, VCT_ERR_STATE, "vct_trace_learned: no G-buffer");
+    if (!c->have_weights) return fail(c, VCT_ERR_STATE, "vct_trace_learned: no trained weights");
+    const int tiles = ((c->gw + 7) / 8) * ((c->gh + 7) / 8);
+    if (tiles > c->tiles_cap) {
+        dfree(c->d_tiles);
+        HIPCHK(c, dalloc(&c->d_tiles, (size_t)tiles * 4));
+        c->tiles_cap = tiles;
+    }
+    if (!c->d_tile_count) HIPCHK(c, dalloc(&c->d_tile_count, 4));
+    TraceLaunch t = make_trace_launch(c, p);
+    PassTimer pt(c, P_LEARNED);
+    LAUNCHCHK(c, launch_learned(c->mlp, c->pyr, c->d_gbuf, c->gw, c->gh, t.ext, thr, c->d_ind, c->d_tiles,
+                                c->d_tile_count, c->stream));
+    LAUNCHCHK(c, launch_trace_tiles(c->pyr, t, c->d_tiles, c->d_tile_count, tiles, c->stream));
+    pt.stop();
+    if (fallback) {
+        uint32_t n = 0;
+        HIPCHK(c, hipMemcpyAsync(&n, c->d_tile_count, 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        *fallback = (int)n;
+    }
+    return VCT_OK;
+}
+
+int vct_reference_gather(vct_ctx *c) {
+    if (!c) return fail(nullptr, VCT_ERR_INVALID, "ctx is NULL");
+    int rc = learned_ok(c);
+    if (rc) return rc;
+    if (!c->have_gbuf) return fail(c, VCT_ERR_STATE, "vct_reference_gather: no G-buffer");
+    if (!c->d_reference) HIPCHK(c, dalloc(&c->d_reference, (size_t)c->gw * c->gh * 16));
+    LAUNCHCHK(c, launch_label64(c->pyr, c->d_gbuf, c->gw * c->gh, ctx_extent(c), c->d_reference, c->stream));
     return VCT_OK;
 }
 

@@ -1,0 +1,41 @@
+// learned.h -- shapes and device buffers of the learned diffuse gather.
+#pragma once
+
+#include "vct_internal.h"
+
+namespace vctk {
+
+// 129 -> 64 -> 64 -> 64 -> 3; flat FP32 parameter vector in the order
+// W1 (64x129, row-major out x in), b1, W2 (64x64), b2, W3 (64x64), b3,
+// W4 (3x64), b4.
+constexpr int kMlpIn = 160;  // 129 inputs padded to a multiple of 32 (MFMA K)
+constexpr int kMlpW1 = 64 * 129;
+constexpr int kMlpW2 = 64 * 64;
+constexpr int kMlpW3 = 64 * 64;
+constexpr int kMlpW4 = 3 * 64;
+constexpr int kMlpParams = kMlpW1 + 64 + kMlpW2 + 64 + kMlpW3 + 64 + kMlpW4 + 3;  // 16835
+constexpr int kMlpPartialStride = 16896;  // >= kMlpParams + 1 (loss), 256-aligned
+
+struct MlpBuffers {
+    float *master = nullptr, *m1 = nullptr, *m2 = nullptr;  // FP32 weights and Adam moments
+    uint16_t *w1 = nullptr, *w2 = nullptr, *w3 = nullptr, *w4 = nullptr;  // BF16 operands
+    uint16_t *w2t = nullptr, *w3t = nullptr, *w4t = nullptr;              // transposed BF16 operands
+    float *partials = nullptr;  // [max_blocks][kMlpPartialStride]
+    int max_blocks = 0;
+    float out_scale = 1.0f;
+};
+
+hipError_t launch_label64(const PyrDesc &pyr, const vct_gbuffer_texel *pts, int npts, Extent ext, float4 *out,
+                          hipStream_t s);
+hipError_t launch_features(const PyrDesc &pyr, const vct_gbuffer_texel *pts, int npts, Extent ext, uint16_t *X,
+                           hipStream_t s);
+hipError_t launch_mlp_train_step(MlpBuffers &B, const uint16_t *X, const float4 *labels, int npts, int batch,
+                                 uint32_t seed, int step, float lr, float b1, float b2, float eps, float *loss_out,
+                                 hipStream_t s);
+size_t mlp_train_smem_bytes();
+hipError_t launch_learned(const MlpBuffers &B, const PyrDesc &pyr, const vct_gbuffer_texel *gbuf, int width,
+                          int height, Extent ext, float threshold, float4 *out, uint32_t *tile_list,
+                          uint32_t *tile_count, hipStream_t s);
+hipError_t launch_emit_operands(MlpBuffers &B, hipStream_t s);
+
+}  // namespace vctk

@@ -1,0 +1,643 @@
+// learned.hip -- the learned diffuse gather: a 129 -> 64 -> 64 -> 64 -> 3 MLP
+// (ReLU hidden layers, BF16 operands on MFMA, FP32 accumulation) that stands
+// in for the 6-cone diffuse loop of phong.frag:480-486, trained on the GPU on
+// labels the pipeline generates itself.
+//
+//   k_label64      FP64-accumulated 64-cone reference of the gather: one
+//                  wavefront per surface point, one lane per cone.
+//   k_features     the network input of a point: 27 trilinear samples of mip
+//                  level 2 on a 3x3x3 stencil (one level-2 voxel apart), the
+//                  gather axis TBN*(0,1,0) and an 18-term sin/cos encoding of
+//                  the texture-space position; padded to 160 BF16 values.
+//   k_mlp_train    forward + backward of one 64-row minibatch tile per
+//                  workgroup on v_mfma_f32_16x16x32_bf16; per-workgroup
+//                  partial gradients.
+//   k_adam         sums the partials, Adam update of the FP32 master weights,
+//                  re-emits the padded BF16 (and transposed) operand copies.
+//   k_learned      inference: one wavefront per 8x8 pixel tile; flags tiles
+//                  whose predicted-luminance variance exceeds a threshold for
+//                  the plain 6-cone tracer (trace.hip, tile-list mode).
+//
+// MFMA 16x16x32 BF16 operand layout (lane l of a wavefront): A holds row
+// l%16, k = 8*(l/16) .. +7; B holds column l%16, the same k; the FP32 result
+// holds rows 4*(l/16) .. +3 of column l%16.
+#include <hip/hip_runtime.h>
+
+#include "vct_device.h"
+#include "learned.h"
+
+namespace vctk {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t f2bf(float f) {  // round to nearest even
+    const uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);  // quiet NaN
+    return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// ---------------------------------------------------------------- sampling
+// GL LINEAR inside one isotropic level (border 0), values decoded to float.
+__device__ __forceinline__ float4 level_linear(const LevelInfo &L, int s, const float p[3]) {
+    const float u[3] = {p[0] * (float)s - 0.5f, p[1] * (float)s - 0.5f, p[2] * (float)s - 0.5f};
+    int i0[3];
+    float f[3];
+    for (int k = 0; k < 3; ++k) {
+        const float fl = floorf(u[k]);
+        i0[k] = (int)fl;
+        f[k] = u[k] - fl;
+    }
+    const int nb = (s + 3) >> 2;
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int c = 0; c < 8; ++c) {
+        const int dx = c & 1, dy = (c >> 1) & 1, dz = c >> 2;
+        const int x = i0[0] + dx, y = i0[1] + dy, z = i0[2] + dz;
+        if ((unsigned)x >= (unsigned)s || (unsigned)y >= (unsigned)s || (unsigned)z >= (unsigned)s) continue;
+        const float w = ((dx ? f[0] : 1.0f - f[0]) * (dy ? f[1] : 1.0f - f[1])) * (dz ? f[2] : 1.0f - f[2]);
+        const uint32_t a = brick_addr(x, y, z, nb);
+        float4 t;
+        if (L.fmt == 1) {
+            t = ((const float4 *)L.base[0])[a];
+        } else if (L.fmt == 2) {
+            const uint32_t cc = ((const uint32_t *)L.base[0])[a];
+            const float sc = L.scale[0][a >> 6];
+            t = make_float4(fp82f(cc & 0xff) * sc, fp82f((cc >> 8) & 0xff) * sc, fp82f((cc >> 16) & 0xff) * sc,
+                            fp82f(cc >> 24) * sc);
+        } else {
+            const uint2 v = ((const uint2 *)L.base[0])[a];
+            t = make_float4(h2f((uint16_t)(v.x & 0xffff)), h2f((uint16_t)(v.x >> 16)), h2f((uint16_t)(v.y & 0xffff)),
+                            h2f((uint16_t)(v.y >> 16)));
+        }
+        acc.x += w * t.x;
+        acc.y += w * t.y;
+        acc.z += w * t.z;
+        acc.w += w * t.w;
+    }
+    return acc;
+}
+
+__device__ __forceinline__ float4 sample_iso(const LevelInfo *__restrict__ lv, int dim, int nlevels, const float p[3],
+                                             float lam) {
+    const int q = nlevels - 1;
+    if (!(lam < (float)q)) {
+        const int s = (dim >> q) > 0 ? (dim >> q) : 1;
+        return level_linear(lv[q], s, p);
+    }
+    if (lam < 0.0f) lam = 0.0f;
+    int d1 = (int)floorf(lam);
+    d1 = d1 > q - 1 ? q - 1 : d1;
+    const float fr = lam - (float)d1;
+    const int s1 = (dim >> d1) > 0 ? (dim >> d1) : 1, s2 = (dim >> (d1 + 1)) > 0 ? (dim >> (d1 + 1)) : 1;
+    const float4 a = level_linear(lv[d1], s1, p), b = level_linear(lv[d1 + 1], s2, p);
+    return make_float4((1.0f - fr) * a.x + fr * b.x, (1.0f - fr) * a.y + fr * b.y, (1.0f - fr) * a.z + fr * b.z,
+                       (1.0f - fr) * a.w + fr * b.w);
+}
+
+// ---------------------------------------------------------------- labels
+// 64 directions: Fibonacci set uniform in cos(theta) around local +y;
+// weights y_i / 32 (sum 1).  Same definition as the oracle's or_label.c.
+__device__ __forceinline__ void label_dir(int i, double d[3], double *w) {
+    const double ga = 3.14159265358979323846 * (3.0 - 2.2360679774997896964);
+    const double y = 1.0 - (i + 0.5) / 64.0;
+    const double r = sqrt(1.0 - y * y);
+    const double phi = ga * i;
+    d[0] = r * cos(phi);
+    d[1] = y;
+    d[2] = r * sin(phi);
+    *w = y / 32.0;
+}
+
+__global__ __launch_bounds__(64) void k_label64(const vct_gbuffer_texel *__restrict__ pts, int npts,
+                                                const LevelInfo *__restrict__ lv, int dim, int nlevels, Extent ext,
+                                                float4 *out) {
+    const int lane = threadIdx.x;
+    for (int pi = blockIdx.x; pi < npts; pi += gridDim.x) {
+        const float4 *g = (const float4 *)&pts[pi];
+        const float4 g0 = g[0], g1 = g[1], g2 = g[2], g3 = g[3];
+        if (__float_as_uint(g2.w) == 0u) {
+            if (lane == 0) out[pi] = make_float4(0.f, 0.f, 0.f, 0.f);
+            continue;
+        }
+        const float gp[3] = {g0.x, g0.y, g0.z};
+        double pos[3], N[3];
+        double n2 = (double)g1.x * g1.x + (double)g1.y * g1.y + (double)g1.z * g1.z;
+        n2 = sqrt(n2);
+        N[0] = g1.x / n2;
+        N[1] = g1.y / n2;
+        N[2] = g1.z / n2;
+        for (int k = 0; k < 3; ++k)
+            pos[k] = ((double)gp[k] - ext.center[k] - ext.vmin[k]) / ((double)ext.vmax[k] - ext.vmin[k]);
+        double dl[3], w;
+        label_dir(lane, dl, &w);
+        double d[3] = {g2.x * dl[0] + g3.x * dl[1] + g1.x * dl[2], g2.y * dl[0] + g3.y * dl[1] + g1.y * dl[2],
+                       g2.z * dl[0] + g3.z * dl[1] + g1.z * dl[2]};
+        const double dn = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        for (int k = 0; k < 3; ++k) d[k] /= dn;
+        const double tan_half = 0.17632698070846498;  // tan(10 degrees)
+        const double scale = 1.0 / dim;
+        double start[3];
+        for (int k = 0; k < 3; ++k) start[k] = pos[k] + N[k] * scale;
+        double c0 = 0, c1 = 0, c2 = 0, a = 0, h = 1.0;
+        for (int i = 0; i < 256 && a < 0.999; ++i) {
+            const double r = h * tan_half;
+            const double lod = log2(fmax(1.0, 2.0 * r)) + 0.5;
+            float sp[3];
+            bool outside = false;
+            double spd[3];
+            for (int k = 0; k < 3; ++k) {
+                spd[k] = start[k] + h * d[k] * scale;
+                if (spd[k] < 0.0 || spd[k] > 1.0) outside = true;
+                sp[k] = (float)spd[k];
+            }
+            if (outside) break;
+            // trilinear sampling in FP32; transmittance and radiance accumulate in FP64
+            const float4 s = sample_iso(lv, dim, nlevels, sp, (float)lod);
+            double sa = s.w < 0.f ? 0.0 : (s.w > 1.f ? 1.0 : (double)s.w);
+            const double ah = 1.0 - sqrt(1.0 - sa);
+            const double tr = 1.0 - a;
+            c0 += 0.5 * s.x * tr;
+            c1 += 0.5 * s.y * tr;
+            c2 += 0.5 * s.z * tr;
+            a += tr * ah;
+            h += 0.5 * r;
+        }
+        double v[4] = {w * c0, w * c1, w * c2, w * a};
+        for (int off = 32; off > 0; off >>= 1)
+            for (int q = 0; q < 4; ++q) v[q] += __shfl_xor(v[q], off, 64);
+        if (lane == 0) {
+            const double occ = 1.0 - (v[3] < 0 ? 0.0 : (v[3] > 1 ? 1.0 : v[3]));
+            out[pi] = make_float4((float)(v[0] * occ), (float)(v[1] * occ), (float)(v[2] * occ), (float)occ);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- features
+// texel -> 160 BF16 inputs (129 used).  Layout:
+//   [0, 108)   27 stencil samples x RGBA, stencil index (iz+1)*9 + (iy+1)*3 + (ix+1)
+//   [108, 111) gather axis normalize(B)
+//   [111, 129) for c in x,y,z: for f in 1,2,4: sin(pi f p_c), cos(pi f p_c)
+//   [129, 160) zero
+__device__ __forceinline__ void make_features(const float4 g0, const float4 g3, const LevelInfo *__restrict__ lv,
+                                              int dim, const Extent &ext, uint16_t *row) {
+    float p[3];
+    const float gp[3] = {g0.x, g0.y, g0.z};
+    for (int k = 0; k < 3; ++k) p[k] = ((gp[k] - ext.center[k]) - ext.vmin[k]) / (ext.vmax[k] - ext.vmin[k]);
+    const int s2 = (dim >> 2) > 0 ? (dim >> 2) : 1;
+    const float step = 1.0f / (float)s2;
+    const LevelInfo &L2 = lv[2];
+    for (int i = 0; i < 27; ++i) {
+        const int ix = i % 3 - 1, iy = (i / 3) % 3 - 1, iz = i / 9 - 1;
+        const float q[3] = {p[0] + ix * step, p[1] + iy * step, p[2] + iz * step};
+        const float4 v = level_linear(L2, s2, q);
+        row[i * 4 + 0] = f2bf(v.x);
+        row[i * 4 + 1] = f2bf(v.y);
+        row[i * 4 + 2] = f2bf(v.z);
+        row[i * 4 + 3] = f2bf(v.w);
+    }
+    float b[3] = {g3.x, g3.y, g3.z};
+    const float bl = sqrtf(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+    for (int k = 0; k < 3; ++k) row[108 + k] = f2bf(bl > 0.f ? b[k] / bl : 0.f);
+    int o = 111;
+    for (int c = 0; c < 3; ++c)
+        for (int f = 1; f <= 4; f <<= 1) {
+            const float ang = 3.14159265358979f * (float)f * p[c];
+            row[o++] = f2bf(sinf(ang));
+            row[o++] = f2bf(cosf(ang));
+        }
+    for (; o < kMlpIn; ++o) row[o] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_features(const vct_gbuffer_texel *__restrict__ pts, int npts,
+                                                  const LevelInfo *__restrict__ lv, int dim, Extent ext,
+                                                  uint16_t *X) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npts) return;
+    const float4 *g = (const float4 *)&pts[i];
+    make_features(g[0], g[3], lv, dim, ext, X + (size_t)i * kMlpIn);
+}
+
+// ---------------------------------------------------------------- MFMA tiles
+// acc (16x16 tile at rows m0, cols n0) += A[m][k] * B[n][k] over k in [0, K),
+// A row-major with leading dimension lda, B "n-major" (each column's k-vector
+// contiguous) with leading dimension ldb; both BF16; K a multiple of 32.
+__device__ __forceinline__ f32x4 mfma_tile(const uint16_t *A, int lda, const uint16_t *B, int ldb, int m0, int n0,
+                                           int K, f32x4 acc) {
+    const int lane = threadIdx.x & 63;
+    const int r = lane & 15, kq = (lane >> 4) * 8;
+    for (int k0 = 0; k0 < K; k0 += 32) {
+        const bf16x8 a = *(const bf16x8 *)(A + (size_t)(m0 + r) * lda + k0 + kq);
+        const bf16x8 b = *(const bf16x8 *)(B + (size_t)(n0 + r) * ldb + k0 + kq);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+    }
+    return acc;
+}
+
+// ---------------------------------------------------------------- training
+struct MlpDev {
+    const float *master;  // [kMlpParams] FP32 (W1, b1, W2, b2, W3, b3, W4, b4; W row-major out x in)
+    const uint16_t *w1;   // [64][160]  BF16, zero padded
+    const uint16_t *w2;   // [64][64]
+    const uint16_t *w3;   // [64][64]
+    const uint16_t *w4;   // [16][64]   (rows 3..15 zero)
+    const uint16_t *w2t;  // [64][64]   transposed: [in][out]
+    const uint16_t *w3t;  // [64][64]
+    const uint16_t *w4t;  // [64][32]   [in][out], out 3..31 zero
+    float out_scale;      // network output * out_scale = predicted rgb * occlusion
+};
+
+__device__ __forceinline__ const float *bias_ptr(const float *master, int layer) {
+    // offsets of b1..b4 in the flat parameter vector
+    const int off[4] = {kMlpW1, kMlpW1 + 64 + kMlpW2, kMlpW1 + 64 + kMlpW2 + 64 + kMlpW3,
+                        kMlpW1 + 64 + kMlpW2 + 64 + kMlpW3 + 64 + kMlpW4};
+    return master + off[layer];
+}
+
+__device__ __forceinline__ uint32_t mix_hash(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+// Shared memory of one training workgroup (64 rows, 4 wavefronts).
+struct TrainSmem {
+    uint16_t x[64 * kMlpIn];      // inputs, row-major
+    uint16_t xt[kMlpIn * 64];     // inputs transposed [k][m]
+    uint16_t a[3][64 * 64];       // hidden activations (post-ReLU), row-major
+    uint16_t at[3][64 * 64];      // transposed
+    float y[64 * 16];             // outputs (fp32, 3 used)
+    float dz[64 * 64];            // current layer's gradient, fp32 [m][n]
+    uint16_t dzb[64 * 64];        // BF16 row-major [m][n]
+    uint16_t dzt[64 * 64];        // BF16 transposed [n][m]
+    float red[4];
+};
+
+// Writes a 16x16 FP32 tile into LDS as bias + ReLU -> BF16 row-major and
+// transposed activations.
+__device__ __forceinline__ void store_act(f32x4 acc, int m0, int n0, const float *bias, bool relu, uint16_t *a,
+                                         uint16_t *at, int ld) {
+    const int lane = threadIdx.x & 63;
+    const int n = n0 + (lane & 15), mb = m0 + (lane >> 4) * 4;
+    for (int i = 0; i < 4; ++i) {
+        float v = acc[i] + bias[n];
+        if (relu) v = v > 0.f ? v : 0.f;
+        const uint16_t h = f2bf(v);
+        a[(mb + i) * ld + n] = h;
+        at[n * 64 + mb + i] = h;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_mlp_train(MlpDev W, const uint16_t *__restrict__ X,
+                                                   const float4 *__restrict__ labels, int npts, int batch,
+                                                   uint32_t seed, int step, float *partials) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    TrainSmem &S = *reinterpret_cast<TrainSmem *>(smem_raw);
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int row0 = blockIdx.x * 64;
+    // gather the minibatch rows: row j of the global batch uses sample hash(seed, step, j) % npts
+    for (int i = tid; i < 64 * (kMlpIn / 8); i += 256) {
+        const int m = i / (kMlpIn / 8), c8 = i % (kMlpIn / 8);
+        const uint32_t pick = mix_hash(seed ^ mix_hash((uint32_t)step * 0x9e3779b9u + (uint32_t)(row0 + m))) %
+                              (uint32_t)npts;
+        const uint4 v = *(const uint4 *)(X + (size_t)pick * kMlpIn + c8 * 8);
+        *(uint4 *)(S.x + m * kMlpIn + c8 * 8) = v;
+        const uint16_t *h = (const uint16_t *)&v;
+        for (int j = 0; j < 8; ++j) S.xt[(c8 * 8 + j) * 64 + m] = h[j];
+    }
+    __syncthreads();
+    // forward: each wavefront computes a 16-row stripe of every layer
+    const int m0 = wave * 16;
+    for (int nt = 0; nt < 4; ++nt) {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        acc = mfma_tile(S.x, kMlpIn, W.w1, kMlpIn, m0, nt * 16, kMlpIn, acc);
+        store_act(acc, m0, nt * 16, bias_ptr(W.master, 0), true, S.a[0], S.at[0], 64);
+    }
+    __syncthreads();
+    for (int nt = 0; nt < 4; ++nt) {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        acc = mfma_tile(S.a[0], 64, W.w2, 64, m0, nt * 16, 64, acc);
+        store_act(acc, m0, nt * 16, bias_ptr(W.master, 1), true, S.a[1], S.at[1], 64);
+    }
+    __syncthreads();
+    for (int nt = 0; nt < 4; ++nt) {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        acc = mfma_tile(S.a[1], 64, W.w3, 64, m0, nt * 16, 64, acc);
+        store_act(acc, m0, nt * 16, bias_ptr(W.master, 2), true, S.a[2], S.at[2], 64);
+    }
+    __syncthreads();
+    {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        acc = mfma_tile(S.a[2], 64, W.w4, 64, m0, 0, 64, acc);
+        const int n = lane & 15, mb = m0 + (lane >> 4) * 4;
+        const float *b4 = bias_ptr(W.master, 3);
+        for (int i = 0; i < 4; ++i) S.y[(mb + i) * 16 + n] = acc[i] + (n < 3 ? b4[n] : 0.f);
+    }
+    __syncthreads();
+    // output gradient of the mean squared error over batch x 3 outputs
+    const float inv_n = 2.0f / ((float)batch * 3.0f);
+    float lsum = 0.f;
+    for (int i = tid; i < 64 * 64; i += 256) {
+        const int m = i >> 6, n = i & 63;
+        float d = 0.f;
+        if (n < 3) {
+            const uint32_t pick = mix_hash(seed ^ mix_hash((uint32_t)step * 0x9e3779b9u + (uint32_t)(row0 + m))) %
+                                  (uint32_t)npts;
+            const float4 t = labels[pick];
+            const float tv = (n == 0 ? t.x : (n == 1 ? t.y : t.z)) / W.out_scale;
+            const float e = S.y[m * 16 + n] - tv;
+            lsum += e * e;
+            d = inv_n * e;
+        }
+        S.dz[i] = d;
+        S.dzb[i] = f2bf(d);
+        S.dzt[n * 64 + m] = S.dzb[i];
+    }
+    for (int off = 32; off > 0; off >>= 1) lsum += __shfl_xor(lsum, off, 64);
+    if (lane == 0) S.red[wave] = lsum;
+    __syncthreads();
+    float *P = partials + (size_t)blockIdx.x * kMlpPartialStride;
+    if (tid == 0) P[kMlpParams] = (S.red[0] + S.red[1] + S.red[2] + S.red[3]) / ((float)batch * 3.0f);
+    // backward, layers 4 .. 1.  dW[n][k] = sum_m dz[m][n] a[m][k]: MFMA with
+    // rows n (from dz transposed), columns k (activations transposed), K = m.
+    const int woff[4] = {0, kMlpW1 + 64, kMlpW1 + 64 + kMlpW2 + 64, kMlpW1 + 64 + kMlpW2 + 64 + kMlpW3 + 64};
+    const int fan_in[4] = {129, 64, 64, 64};
+    const int fan_out[4] = {64, 64, 64, 3};
+    for (int layer = 3; layer >= 0; --layer) {
+        const uint16_t *actT = layer == 0 ? S.xt : S.at[layer - 1];
+        const int K_in = layer == 0 ? kMlpIn : 64;
+        const int nrow_tiles = layer == 3 ? 1 : 4;  // output units / 16
+        const int ncol_tiles = K_in / 16;
+        for (int t = wave; t < nrow_tiles * ncol_tiles; t += 4) {
+            const int rt = t / ncol_tiles, ct = t % ncol_tiles;
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+            acc = mfma_tile(S.dzt, 64, actT, 64, rt * 16, ct * 16, 64, acc);
+            const int k = ct * 16 + (lane & 15), nb = rt * 16 + (lane >> 4) * 4;
+            for (int i = 0; i < 4; ++i) {
+                const int n = nb + i;
+                if (n < fan_out[layer] && k < fan_in[layer]) P[woff[layer] + n * fan_in[layer] + k] = acc[i];
+            }
+        }
+        // bias gradient: sum over the 64 rows of the fp32 dz
+        for (int n = tid; n < fan_out[layer]; n += 256) {
+            float s = 0.f;
+            for (int m = 0; m < 64; ++m) s += S.dz[m * 64 + n];
+            P[woff[layer] + fan_out[layer] * fan_in[layer] + n] = s;
+        }
+        if (layer == 0) break;
+        __syncthreads();
+        // da[m][k] = sum_n dz[m][n] W[n][k]: rows m (dz row-major), columns k
+        // (W transposed, [k][n]); then dz_prev = da * (a > 0)
+        const uint16_t *wt = layer == 3 ? W.w4t : (layer == 2 ? W.w3t : W.w2t);
+        const int Kn = layer == 3 ? 32 : 64;
+        f32x4 da[4];
+        for (int ct = 0; ct < 4; ++ct) {
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+            da[ct] = mfma_tile(S.dzb, 64, wt, Kn, m0, ct * 16, Kn, acc);
+        }
+        __syncthreads();  // everyone has read dzb before it is overwritten
+        const uint16_t *aprev = S.a[layer - 1];
+        for (int ct = 0; ct < 4; ++ct) {
+            const int n = ct * 16 + (lane & 15), mb = m0 + (lane >> 4) * 4;
+            for (int i = 0; i < 4; ++i) {
+                const int m = mb + i;
+                const float g = bf2f(aprev[m * 64 + n]) > 0.f ? da[ct][i] : 0.f;
+                S.dz[m * 64 + n] = g;
+                const uint16_t h = f2bf(g);
+                S.dzb[m * 64 + n] = h;
+                S.dzt[n * 64 + m] = h;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Sums the per-workgroup partial gradients and applies Adam to the FP32
+// master weights; re-emits the BF16 operand copies.
+__global__ __launch_bounds__(256) void k_adam(float *master, float *m1, float *m2, const float *partials,
+                                              int nparts, float lr, float b1, float b2, float eps, int step,
+                                              uint16_t *w1, uint16_t *w2, uint16_t *w3, uint16_t *w4, uint16_t *w2t,
+                                              uint16_t *w3t, uint16_t *w4t, float *loss_out) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p == kMlpParams && loss_out) {
+        float l = 0.f;
+        for (int i = 0; i < nparts; ++i) l += partials[(size_t)i * kMlpPartialStride + kMlpParams];
+        loss_out[0] = l;
+    }
+    if (p >= kMlpParams) return;
+    float g = 0.f;
+    for (int i = 0; i < nparts; ++i) g += partials[(size_t)i * kMlpPartialStride + p];
+    const float m = b1 * m1[p] + (1.f - b1) * g;
+    const float v = b2 * m2[p] + (1.f - b2) * g * g;
+    m1[p] = m;
+    m2[p] = v;
+    const float mh = m / (1.f - powf(b1, (float)step));
+    const float vh = v / (1.f - powf(b2, (float)step));
+    const float w = master[p] - lr * mh / (sqrtf(vh) + eps);
+    master[p] = w;
+    const uint16_t h = f2bf(w);
+    // scatter into the operand copies
+    if (p < kMlpW1) {
+        const int n = p / 129, k = p % 129;
+        w1[n * kMlpIn + k] = h;
+    } else if (p >= kMlpW1 + 64 && p < kMlpW1 + 64 + kMlpW2) {
+        const int q = p - (kMlpW1 + 64), n = q / 64, k = q % 64;
+        w2[n * 64 + k] = h;
+        w2t[k * 64 + n] = h;
+    } else if (p >= kMlpW1 + 64 + kMlpW2 + 64 && p < kMlpW1 + 64 + kMlpW2 + 64 + kMlpW3) {
+        const int q = p - (kMlpW1 + 64 + kMlpW2 + 64), n = q / 64, k = q % 64;
+        w3[n * 64 + k] = h;
+        w3t[k * 64 + n] = h;
+    } else if (p >= kMlpW1 + 64 + kMlpW2 + 64 + kMlpW3 + 64 && p < kMlpW1 + 64 + kMlpW2 + 64 + kMlpW3 + 64 + kMlpW4) {
+        const int q = p - (kMlpW1 + 64 + kMlpW2 + 64 + kMlpW3 + 64), n = q / 64, k = q % 64;
+        w4[n * 64 + k] = h;
+        w4t[k * 32 + n] = h;
+    }
+}
+
+// ---------------------------------------------------------------- inference
+struct InferSmem {
+    uint16_t x[64 * kMlpIn];  // inputs, then hidden activations (ping)
+    uint16_t b[64 * 64];      // hidden activations (pong)
+    float y[64 * 4];
+};
+
+__global__ __launch_bounds__(64) void k_learned(MlpDev W, const vct_gbuffer_texel *__restrict__ gbuf, int width,
+                                                int height, const LevelInfo *__restrict__ lv, int dim, Extent ext,
+                                                float threshold, float4 *out, uint32_t *tile_list,
+                                                uint32_t *tile_count) {
+    __shared__ __align__(16) InferSmem S;
+    const int lane = threadIdx.x;
+    const int tiles_x = (width + 7) / 8;
+    const int tile = blockIdx.x;
+    const int px = (tile % tiles_x) * 8 + (lane & 7), py = (tile / tiles_x) * 8 + (lane >> 3);
+    const bool inside = px < width && py < height;
+    bool covered = false;
+    if (inside) {
+        const float4 *g = (const float4 *)&gbuf[(size_t)py * width + px];
+        const float4 g0 = g[0], g2 = g[2], g3 = g[3];
+        covered = __float_as_uint(g2.w) != 0u;
+        if (covered) make_features(g0, g3, lv, dim, ext, S.x + lane * kMlpIn);
+    }
+    if (!covered)
+        for (int k = 0; k < kMlpIn; ++k) S.x[lane * kMlpIn + k] = 0;
+    __syncthreads();
+    // layer 1: x -> b
+    for (int mt = 0; mt < 4; ++mt)
+        for (int nt = 0; nt < 4; ++nt) {
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+            acc = mfma_tile(S.x, kMlpIn, W.w1, kMlpIn, mt * 16, nt * 16, kMlpIn, acc);
+            const int n = nt * 16 + (lane & 15), mb = mt * 16 + (lane >> 4) * 4;
+            const float *b1 = bias_ptr(W.master, 0);
+            for (int i = 0; i < 4; ++i) {
+                const float v = acc[i] + b1[n];
+                S.b[(mb + i) * 64 + n] = f2bf(v > 0.f ? v : 0.f);
+            }
+        }
+    __syncthreads();
+    // layer 2: b -> x (as a 64-wide activation buffer), layer 3: x -> b
+    for (int layer = 1; layer <= 2; ++layer) {
+        const uint16_t *src = layer == 1 ? S.b : S.x;
+        uint16_t *dst = layer == 1 ? S.x : S.b;
+        const uint16_t *w = layer == 1 ? W.w2 : W.w3;
+        const float *bias = bias_ptr(W.master, layer);
+        f32x4 res[4][4];
+        for (int mt = 0; mt < 4; ++mt)
+            for (int nt = 0; nt < 4; ++nt) {
+                f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+                res[mt][nt] = mfma_tile(src, 64, w, 64, mt * 16, nt * 16, 64, acc);
+            }
+        __syncthreads();
+        for (int mt = 0; mt < 4; ++mt)
+            for (int nt = 0; nt < 4; ++nt) {
+                const int n = nt * 16 + (lane & 15), mb = mt * 16 + (lane >> 4) * 4;
+                for (int i = 0; i < 4; ++i) {
+                    const float v = res[mt][nt][i] + bias[n];
+                    dst[(mb + i) * 64 + n] = f2bf(v > 0.f ? v : 0.f);
+                }
+            }
+        __syncthreads();
+    }
+    // layer 4: b -> y
+    for (int mt = 0; mt < 4; ++mt) {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        acc = mfma_tile(S.b, 64, W.w4, 64, mt * 16, 0, 64, acc);
+        const int n = lane & 15, mb = mt * 16 + (lane >> 4) * 4;
+        const float *b4 = bias_ptr(W.master, 3);
+        if (n < 3)
+            for (int i = 0; i < 4; ++i) S.y[(mb + i) * 4 + n] = (acc[i] + b4[n]) * W.out_scale;
+    }
+    __syncthreads();
+    const float r = covered ? S.y[lane * 4 + 0] : 0.f, gch = covered ? S.y[lane * 4 + 1] : 0.f,
+                bch = covered ? S.y[lane * 4 + 2] : 0.f;
+    // predicted-luminance variance of the tile's covered pixels
+    const float lum = 0.2126f * r + 0.7152f * gch + 0.0722f * bch;
+    float s1 = covered ? lum : 0.f, s2 = covered ? lum * lum : 0.f, cnt = covered ? 1.f : 0.f;
+    for (int off = 32; off > 0; off >>= 1) {
+        s1 += __shfl_xor(s1, off, 64);
+        s2 += __shfl_xor(s2, off, 64);
+        cnt += __shfl_xor(cnt, off, 64);
+    }
+    const float mean = cnt > 0.f ? s1 / cnt : 0.f;
+    const float var = cnt > 0.f ? s2 / cnt - mean * mean : 0.f;
+    if (inside) out[(size_t)py * width + px] = make_float4(r, gch, bch, covered ? 1.0f : 0.0f);
+    if (lane == 0 && var > threshold) {
+        const uint32_t slot = atomicAdd(tile_count, 1u);
+        tile_list[slot] = (uint32_t)tile;
+    }
+}
+
+// ---------------------------------------------------------------- launchers
+hipError_t launch_label64(const PyrDesc &pyr, const vct_gbuffer_texel *pts, int npts, Extent ext, float4 *out,
+                          hipStream_t s) {
+    const int blocks = npts < 65536 ? (npts > 0 ? npts : 1) : 65536;
+    hipLaunchKernelGGL(k_label64, dim3(blocks), dim3(64), 0, s, pts, npts, pyr.d_lv, pyr.dim, pyr.nlevels, ext, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_features(const PyrDesc &pyr, const vct_gbuffer_texel *pts, int npts, Extent ext, uint16_t *X,
+                           hipStream_t s) {
+    hipLaunchKernelGGL(k_features, dim3((npts + 255) / 256), dim3(256), 0, s, pts, npts, pyr.d_lv, pyr.dim, ext, X);
+    return hipGetLastError();
+}
+
+static MlpDev mlp_dev(const MlpBuffers &B) {
+    MlpDev d;
+    d.master = B.master;
+    d.w1 = B.w1;
+    d.w2 = B.w2;
+    d.w3 = B.w3;
+    d.w4 = B.w4;
+    d.w2t = B.w2t;
+    d.w3t = B.w3t;
+    d.w4t = B.w4t;
+    d.out_scale = B.out_scale;
+    return d;
+}
+
+hipError_t launch_mlp_train_step(MlpBuffers &B, const uint16_t *X, const float4 *labels, int npts, int batch,
+                                 uint32_t seed, int step, float lr, float b1, float b2, float eps, float *loss_out,
+                                 hipStream_t s) {
+    const int nblk = batch / 64;
+    static bool smem_attr = false;
+    if (!smem_attr) {  // the training workgroup keeps its tile, activations and gradients in LDS (> 64 KiB)
+        hipError_t e = hipFuncSetAttribute((const void *)k_mlp_train, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)sizeof(TrainSmem));
+        if (e != hipSuccess) return e;
+        smem_attr = true;
+    }
+    hipLaunchKernelGGL(k_mlp_train, dim3(nblk), dim3(256), sizeof(TrainSmem), s, mlp_dev(B), X, labels, npts, batch,
+                       seed, step, B.partials);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_adam, dim3((kMlpParams + 1 + 255) / 256), dim3(256), 0, s, B.master, B.m1, B.m2, B.partials,
+                       nblk, lr, b1, b2, eps, step, B.w1, B.w2, B.w3, B.w4, B.w2t, B.w3t, B.w4t, loss_out);
+    return hipGetLastError();
+}
+
+size_t mlp_train_smem_bytes() { return sizeof(TrainSmem); }
+
+hipError_t launch_learned(const MlpBuffers &B, const PyrDesc &pyr, const vct_gbuffer_texel *gbuf, int width,
+                          int height, Extent ext, float threshold, float4 *out, uint32_t *tile_list,
+                          uint32_t *tile_count, hipStream_t s) {
+    const int tiles = ((width + 7) / 8) * ((height + 7) / 8);
+    hipError_t e = launch_fill_u32(tile_count, 0u, 1, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_learned, dim3(tiles), dim3(64), 0, s, mlp_dev(B), gbuf, width, height, pyr.d_lv, pyr.dim,
+                       ext, threshold, out, tile_list, tile_count);
+    return hipGetLastError();
+}
+
+// Operand copies from the FP32 master vector (used after set_weights).
+__global__ void k_emit_operands(const float *master, uint16_t *w1, uint16_t *w2, uint16_t *w3, uint16_t *w4,
+                                uint16_t *w2t, uint16_t *w3t, uint16_t *w4t) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= kMlpParams) return;
+    const uint16_t h = f2bf(master[p]);
+    if (p < kMlpW1) {
+        w1[(p / 129) * kMlpIn + p % 129] = h;
+    } else if (p >= kMlpW1 + 64 && p < kMlpW1 + 64 + kMlpW2) {
+        const int q = p - (kMlpW1 + 64);
+        w2[q] = h;
+        w2t[(q % 64) * 64 + q / 64] = h;
+    } else if (p >= kMlpW1 + 64 + kMlpW2 + 64 && p < kMlpW1 + 64 + kMlpW2 + 64 + kMlpW3) {
+        const int q = p - (kMlpW1 + 64 + kMlpW2 + 64);
+        w3[q] = h;
+        w3t[(q % 64) * 64 + q / 64] = h;
+    } else if (p >= kMlpW1 + 64 + kMlpW2 + 64 + kMlpW3 + 64 && p < kMlpW1 + 64 + kMlpW2 + 64 + kMlpW3 + 64 + kMlpW4) {
+        const int q = p - (kMlpW1 + 64 + kMlpW2 + 64 + kMlpW3 + 64);
+        w4[q] = h;
+        w4t[(q % 64) * 32 + q / 64] = h;
+    }
+}
+
+hipError_t launch_emit_operands(MlpBuffers &B, hipStream_t s) {
+    hipLaunchKernelGGL(k_emit_operands, dim3((kMlpParams + 255) / 256), dim3(256), 0, s, B.master, B.w1, B.w2, B.w3,
+                       B.w4, B.w2t, B.w3t, B.w4t);
+    return hipGetLastError();
+}
+
+}  // namespace vctk

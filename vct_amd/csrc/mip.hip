@@ -52,10 +52,14 @@ __device__ __forceinline__ float4 round_stage(float4 v, int fp32) {
     return make_float4(q16(v.x), q16(v.y), q16(v.z), q16(v.w));
 }
 
-template <bool ANISO>
+// ANISO: outputs have six faces; SRC6: the source level has six faces (any
+// anisotropic level >= 1).  LDS: the 8^3 source region (per face), then the
+// ping-pong buffers of the three output steps (4^3, 2^3, 1^3 per face).
+template <bool ANISO, bool SRC6>
 __global__ __launch_bounds__(256) void k_mip_chain(MipArgs a) {
     constexpr int NF = ANISO ? 6 : 1;
-    __shared__ float4 bufA[NF][512];
+    constexpr int SF = SRC6 ? 6 : 1;
+    __shared__ float4 bufA[SF][512];
     __shared__ float4 bufB[NF][64];
     const int tid = threadIdx.x;
     const int s = a.src.size;
@@ -64,7 +68,7 @@ __global__ __launch_bounds__(256) void k_mip_chain(MipArgs a) {
     const int rx = blockIdx.x % nreg, ry = (blockIdx.x / nreg) % nreg, rz = blockIdx.x / (nreg * nreg);
     const int ox = rx * R, oy = ry * R, oz = rz * R;
     const int R3 = R * R * R;
-    for (int f = 0; f < a.src_faces; ++f) {
+    for (int f = 0; f < SF; ++f) {  // the launcher picks SRC6 from the source level's face count
         for (int i = tid; i < R3; i += 256) {
             int x, y, z;
             if (R == 8) {  // brick-major walk: 8 bricks of 64 contiguous voxels
@@ -82,15 +86,16 @@ __global__ __launch_bounds__(256) void k_mip_chain(MipArgs a) {
         }
     }
     __syncthreads();
-    int cur = R, faces = a.src_faces;
+    int cur = R, faces = SF;
     float4 *in = &bufA[0][0];
     int in_stride = 512;
     for (int k = 0; k < a.nout; ++k) {
         const int d = cur >> 1;
         if (d < 1) break;
-        // ping-pong: outputs of step k go to bufB (k even) or bufA (k odd)
+        // ping-pong: outputs of step k go to bufB (k even, <= 4^3 per face) or
+        // bufA (k odd, <= 2^3 per face)
         float4 *out = (k & 1) ? &bufA[0][0] : &bufB[0][0];
-        const int out_stride = (k & 1) ? 512 : 64;
+        const int out_stride = (k & 1) ? 8 : 64;
         const MipLevelArgs &ol = a.out[k];
         const int sh = k + 1;
         const int gx = ox >> sh, gy = oy >> sh, gz = oz >> sh;
@@ -199,8 +204,9 @@ hipError_t launch_mip(const PyrDesc &pyr, hipStream_t s) {
         a.fp32 = fp32;
         const int nreg = S.size / R;
         dim3 grid((unsigned)(nreg * nreg * nreg));
-        if (pyr.aniso) hipLaunchKernelGGL(k_mip_chain<true>, grid, dim3(256), 0, s, a);
-        else hipLaunchKernelGGL(k_mip_chain<false>, grid, dim3(256), 0, s, a);
+        if (pyr.aniso && S.nfaces == 6) hipLaunchKernelGGL((k_mip_chain<true, true>), grid, dim3(256), 0, s, a);
+        else if (pyr.aniso) hipLaunchKernelGGL((k_mip_chain<true, false>), grid, dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((k_mip_chain<false, false>), grid, dim3(256), 0, s, a);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
         src += nout;

@@ -5,11 +5,11 @@
 // The reference injects by scattering one store per shadow-map texel
 // (injectRadiance.comp:152-214, radiance[v] := packUnorm4x8(voxelColor[v])).
 // Every texel that lands in voxel v stores the same value, so the scatter is
-// idempotent and equals a per-voxel "is lit" bit.  Here the scatter only sets
-// that bit (a 1-bit-per-voxel mask, 2 MiB at 256^3) and one coalesced
-// per-voxel pass then does transferVoxels' work (statistics, opacity rewrite,
-// radiance (0,0,0,a)) and the injection together, writing radiance level 0
-// once in the storage format.
+// idempotent and equals a per-voxel "is lit" flag.  Here the scatter only
+// stores that flag (one byte per voxel, plain stores: racing writers all write
+// 1, so no atomics are needed) and one coalesced per-voxel pass then does
+// transferVoxels' work (statistics, opacity rewrite, radiance (0,0,0,a)) and
+// the injection together, writing radiance level 0 once in the storage format.
 #pragma clang fp contract(off)
 
 #include "vct_device.h"
@@ -22,7 +22,7 @@ struct InjectLight {
     int N, perspective;
 };
 
-__global__ __launch_bounds__(256) void k_inject_scatter(InjectLight L, int dim, Extent ext, uint32_t *litmask) {
+__global__ __launch_bounds__(256) void k_inject_scatter(InjectLight L, int dim, Extent ext, uint8_t *litmask) {
     const int i = blockIdx.x * 16 + (threadIdx.x & 15);
     const int j = blockIdx.y * 16 + (threadIdx.x >> 4);
     if (i >= L.N || j >= L.N) return;
@@ -41,17 +41,14 @@ __global__ __launch_bounds__(256) void k_inject_scatter(InjectLight L, int dim, 
         idx[k] = (int)((float)dim * lin);
     }
     if (idx[0] < 0 || idx[1] < 0 || idx[2] < 0 || idx[0] >= dim || idx[1] >= dim || idx[2] >= dim) return;
-    const size_t vi = ((size_t)idx[2] * dim + idx[1]) * dim + idx[0];
-    const uint32_t bit = 1u << (vi & 31);
-    uint32_t *word = &litmask[vi >> 5];
-    if (!(__atomic_load_n(word, __ATOMIC_RELAXED) & bit)) atomicOr(word, bit);  // skip set bits
+    litmask[((size_t)idx[2] * dim + idx[1]) * dim + idx[0]] = 1;
 }
 
 hipError_t launch_inject_scatter(const DevLight *lights, const DevLight *dlights_dev, int nlights,
-                                 uint32_t light_mask, int dim, Extent ext, uint32_t *litmask, hipStream_t s) {
+                                 uint32_t light_mask, int dim, Extent ext, uint8_t *litmask, hipStream_t s) {
     (void)dlights_dev;
     const size_t nv = (size_t)dim * dim * dim;
-    hipError_t e = launch_fill_u32(litmask, 0u, (nv + 31) / 32, s);
+    hipError_t e = launch_fill_u32((uint32_t *)litmask, 0u, nv / 4, s);
     if (e != hipSuccess) return e;
     for (int li = 0; li < nlights; ++li) {
         const DevLight &L = lights[li];
@@ -68,24 +65,25 @@ hipError_t launch_inject_scatter(const DevLight *lights, const DevLight *dlights
     return hipSuccess;
 }
 
-// One thread per 4-voxel x-row of a brick: 16 threads per brick, a wavefront
+// One thread per 4-voxel x-row of a brick (16 rows per brick): a wavefront
 // covers 4 consecutive bricks, so albedo reads are 64-byte runs and radiance
-// writes are whole 512-byte (fp16) bricks.
+// writes are whole 512-byte (fp16) bricks.  The grid strides over the rows
+// with a fixed number of workgroups, each of which reduces its statistics in
+// LDS and writes one partial.
 template <int FMT>
-__global__ __launch_bounds__(256) void k_normalize_inject(uint32_t *albedo, const uint32_t *__restrict__ litmask,
+__global__ __launch_bounds__(256) void k_normalize_inject(uint32_t *albedo, const uint8_t *__restrict__ litmask,
                                                           int dim, int nb, float opacity, char *rad0,
-                                                          vct_stats *stats) {
-    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+                                                          uint32_t *partials) {
     const size_t nrows = (size_t)nb * nb * nb * 16;
     uint32_t uniq = 0, maxf = 0, lit_count = 0;
-    if (g < nrows) {
+    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < nrows; g += (size_t)gridDim.x * blockDim.x) {
         const size_t b = g >> 4;
         const int q = (int)(g & 15);
         const int bx = (int)(b % nb), by = (int)((b / nb) % nb), bz = (int)(b / ((size_t)nb * nb));
         const int x0 = bx * 4, y = by * 4 + (q & 3), z = bz * 4 + (q >> 2);
         const size_t li = ((size_t)z * dim + y) * dim + x0;
-        uint4 a = *(const uint4 *)&albedo[li];
-        const uint32_t lw = litmask[li >> 5] >> (li & 31);
+        const uint4 a = *(const uint4 *)&albedo[li];
+        const uint32_t lw = *(const uint32_t *)&litmask[li];  // 4 flag bytes
         uint32_t w[4] = {a.x, a.y, a.z, a.w};
         float r[4][4];
         bool changed = false;
@@ -102,7 +100,7 @@ __global__ __launch_bounds__(256) void k_normalize_inject(uint32_t *albedo, cons
                 const uint32_t nw = pack_unorm4x8(c0, c1, c2, na);
                 if (nw != a32) changed = true;
                 w[v] = nw;
-                const bool lit = (lw >> v) & 1u;
+                const bool lit = ((lw >> (8 * v)) & 0xffu) != 0u;
                 lit_count += lit ? 1u : 0u;
                 r[v][0] = lit ? c0 : 0.0f;
                 r[v][1] = lit ? c1 : 0.0f;
@@ -130,35 +128,27 @@ __global__ __launch_bounds__(256) void k_normalize_inject(uint32_t *albedo, cons
             o[1] = make_uint4(h[4], h[5], h[6], h[7]);
         }
     }
-    // wavefront reductions, one atomic per wavefront per statistic
-    for (int off = 32; off > 0; off >>= 1) {
-        uniq += __shfl_xor(uniq, off, 64);
-        lit_count += __shfl_xor(lit_count, off, 64);
-        uint32_t o = __shfl_xor(maxf, off, 64);
-        maxf = o > maxf ? o : maxf;
-    }
-    if ((threadIdx.x & 63) == 0) {
-        if (uniq) atomicAdd(&stats->unique_voxels, uniq);
-        if (lit_count) atomicAdd(&stats->lit_voxels, lit_count);
-        if (maxf) atomicMax(&stats->max_fragments_per_voxel, maxf);
-    }
+    block_partial(uniq, maxf, lit_count, 0u, partials);
 }
 
-hipError_t launch_normalize_inject(uint32_t *albedo, const uint32_t *litmask, int dim, float opacity,
-                                   const PyrDesc &pyr, vct_stats *stats, hipStream_t s) {
-    // transferVoxels clears radiance and the stats it owns (Application.cpp:581, :763)
-    hipError_t e = launch_fill_u32(&stats->unique_voxels, 0u, 3, s);  // unique, max, lit
-    if (e != hipSuccess) return e;
+hipError_t launch_normalize_inject(uint32_t *albedo, const uint8_t *litmask, int dim, float opacity,
+                                   const PyrDesc &pyr, vct_stats *stats, uint32_t *partials, hipStream_t s) {
     const LevelInfo &L0 = pyr.lv[0];
     const size_t nrows = (size_t)L0.nb * L0.nb * L0.nb * 16;
-    dim3 grid((unsigned)((nrows + 255) / 256));
+    size_t blocks = (nrows + 255) / 256;
+    if (blocks > (size_t)kStatPartials) blocks = kStatPartials;
+    dim3 grid((unsigned)blocks);
     if (L0.fmt == 1)
         hipLaunchKernelGGL(k_normalize_inject<1>, grid, dim3(256), 0, s, albedo, litmask, dim, L0.nb, opacity,
-                           L0.stage[0], stats);
+                           L0.stage[0], partials);
     else
         hipLaunchKernelGGL(k_normalize_inject<0>, grid, dim3(256), 0, s, albedo, litmask, dim, L0.nb, opacity,
-                           L0.stage[0], stats);
-    return hipGetLastError();
+                           L0.stage[0], partials);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    // uniqueVoxels, maxFragmentsPerVoxel (transferVoxels.comp:39-45) and the lit count
+    return launch_reduce_stats(partials, (int)blocks, &stats->unique_voxels, &stats->max_fragments_per_voxel,
+                               &stats->lit_voxels, s);
 }
 
 // ---------------------------------------------------------------- fill holes

@@ -14,6 +14,40 @@ __global__ void k_fill_u64(uint64_t *p, uint64_t v, size_t n) {
         p[i] = v;
 }
 
+__global__ __launch_bounds__(256) void k_reduce_stats(const uint32_t *partials, int n, uint32_t *sum0, uint32_t *max1,
+                                                      uint32_t *sum2) {
+    uint32_t a = 0, b = 0, c = 0;
+    for (int i = threadIdx.x; i < n; i += 256) {
+        a += partials[i * 4 + 0];
+        b = partials[i * 4 + 1] > b ? partials[i * 4 + 1] : b;
+        c += partials[i * 4 + 2];
+    }
+    __shared__ uint32_t red[3][256];
+    red[0][threadIdx.x] = a;
+    red[1][threadIdx.x] = b;
+    red[2][threadIdx.x] = c;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + s];
+            red[1][threadIdx.x] = red[1][threadIdx.x + s] > red[1][threadIdx.x] ? red[1][threadIdx.x + s] : red[1][threadIdx.x];
+            red[2][threadIdx.x] += red[2][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (sum0) *sum0 = red[0][0];
+        if (max1) *max1 = red[1][0];
+        if (sum2) *sum2 = red[2][0];
+    }
+}
+
+hipError_t launch_reduce_stats(const uint32_t *partials, int n, uint32_t *sum0, uint32_t *max1, uint32_t *sum2,
+                               hipStream_t s) {
+    hipLaunchKernelGGL(k_reduce_stats, dim3(1), dim3(256), 0, s, partials, n, sum0, max1, sum2);
+    return hipGetLastError();
+}
+
 static unsigned fill_blocks(size_t n) {
     size_t b = (n + 255) / 256;
     return (unsigned)(b > 4096 ? 4096 : (b == 0 ? 1 : b));

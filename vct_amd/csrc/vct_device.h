@@ -139,4 +139,40 @@ __device__ __forceinline__ float shadow_bilinear(const float *__restrict__ sm, i
 // ---- wave helpers ----------------------------------------------------------
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
+// Reduces (sum a, max b, sum c, sum d) over a 256-thread workgroup and writes
+// them to partials[blockIdx.x * 4 ..].  Every thread of the block must call it.
+__device__ __forceinline__ void block_partial(uint32_t a, uint32_t b, uint32_t c, uint32_t d, uint32_t *partials) {
+    __shared__ uint32_t red[4][4];
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off, 64);
+        const uint32_t ob = __shfl_xor(b, off, 64);
+        b = ob > b ? ob : b;
+        c += __shfl_xor(c, off, 64);
+        d += __shfl_xor(d, off, 64);
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[w][0] = a;
+        red[w][1] = b;
+        red[w][2] = c;
+        red[w][3] = d;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = (blockDim.x + 63) >> 6;
+        uint32_t s0 = 0, m1 = 0, s2 = 0, s3 = 0;
+        for (int i = 0; i < nw; ++i) {
+            s0 += red[i][0];
+            m1 = red[i][1] > m1 ? red[i][1] : m1;
+            s2 += red[i][2];
+            s3 += red[i][3];
+        }
+        uint32_t *o = partials + (size_t)blockIdx.x * 4;
+        o[0] = s0;
+        o[1] = m1;
+        o[2] = s2;
+        o[3] = s3;
+    }
+}
+
 }  // namespace vctk

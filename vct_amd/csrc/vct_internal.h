@@ -95,9 +95,16 @@ struct TraceLaunch {
     Extent ext;
 };
 
+// Statistics are reduced per workgroup into `partials` (kStatPartials slots of
+// 4 words) and summed by one small kernel: no contended same-address atomics.
+constexpr int kStatPartials = 2048;
+
 // ---- launch wrappers (return hipError_t of the last launch) ----------------
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
 hipError_t launch_fill_u64(uint64_t *p, uint64_t v, size_t n, hipStream_t s);
+// sums/maxes partials[0..n) (4 words each: sum, max, sum, sum) into the named stats fields
+hipError_t launch_reduce_stats(const uint32_t *partials, int n, uint32_t *sum0, uint32_t *max1, uint32_t *sum2,
+                               hipStream_t s);
 
 // scan.hip: counts[0..n) -> offs[0..n] exclusive (offs[n] = total) and chunk starts
 hipError_t raster_scan(RasterScratch &rs, size_t nslots, hipStream_t s);
@@ -111,14 +118,14 @@ hipError_t launch_gbuffer(const DevScene &sc, const vct_camera &cam, uint64_t *v
 
 // voxelize.hip
 hipError_t launch_voxelize(const DevScene &sc, const DevLight *dlights, int nlights, const vct_voxelize_params &p,
-                           int dim, uint32_t *albedo, uint32_t *normal, vct_stats *stats, RasterScratch &rs,
-                           hipStream_t s);
+                           int dim, uint32_t *albedo, uint32_t *normal, vct_stats *stats, uint32_t *partials,
+                           RasterScratch &rs, hipStream_t s);
 
 // radiance.hip
 hipError_t launch_inject_scatter(const DevLight *dlights_host, const DevLight *dlights_dev, int nlights,
-                                 uint32_t light_mask, int dim, Extent ext, uint32_t *litmask, hipStream_t s);
-hipError_t launch_normalize_inject(uint32_t *albedo, const uint32_t *litmask, int dim, float opacity,
-                                   const PyrDesc &pyr, vct_stats *stats, hipStream_t s);
+                                 uint32_t light_mask, int dim, Extent ext, uint8_t *litmask, hipStream_t s);
+hipError_t launch_normalize_inject(uint32_t *albedo, const uint8_t *litmask, int dim, float opacity,
+                                   const PyrDesc &pyr, vct_stats *stats, uint32_t *partials, hipStream_t s);
 hipError_t launch_fill_holes(const PyrDesc &pyr, void *tmp, hipStream_t s);
 
 // mip.hip
@@ -126,6 +133,8 @@ hipError_t launch_mip(const PyrDesc &pyr, hipStream_t s);
 
 // trace.hip
 hipError_t launch_trace(const PyrDesc &pyr, const TraceLaunch &t, hipStream_t s);
+hipError_t launch_trace_tiles(const PyrDesc &pyr, const TraceLaunch &t, const uint32_t *list, const uint32_t *count,
+                              int max_tiles, hipStream_t s);
 
 // conversions for download (radiance level -> linear float4)
 hipError_t launch_pyr_to_linear(const PyrDesc &pyr, int level, int face, float4 *out, hipStream_t s);

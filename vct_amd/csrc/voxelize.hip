@@ -101,7 +101,7 @@ struct VoxFrag {
     const DevLight *lights;
     int nlights, lighting, msaa, dim;
     uint32_t *albedo, *normal;
-    uint32_t *stat_frags;
+    uint32_t *partials;
     uint32_t frags;
 
     __device__ void item(int slot, int px, int py, const Edges &e) {
@@ -210,22 +210,17 @@ struct VoxFrag {
         atomicMax(&albedo[vi], pack_unorm4x8(color[0], color[1], color[2], 1.0f));
         atomicMax(&normal[vi], pack_unorm4x8(ne0, ne1, ne2, 1.0f));
     }
-    __device__ void finish() {
-        uint32_t v = frags;
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-        if ((threadIdx.x & 63) == 0 && v) atomicAdd(stat_frags, v);
-    }
+    __device__ void finish() { block_partial(frags, 0u, 0u, 0u, partials); }
 };
 
 hipError_t launch_voxelize(const DevScene &sc, const DevLight *dlights, int nlights, const vct_voxelize_params &p,
-                           int dim, uint32_t *albedo, uint32_t *normal, vct_stats *stats, RasterScratch &rs,
-                           hipStream_t s) {
+                           int dim, uint32_t *albedo, uint32_t *normal, vct_stats *stats, uint32_t *partials,
+                           RasterScratch &rs, hipStream_t s) {
     const size_t nv = (size_t)dim * dim * dim;
     hipError_t e = launch_fill_u32(albedo, 0u, nv, s);  // glClearTexImage (Application.cpp:686-687)
     if (e != hipSuccess) return e;
     if ((e = launch_fill_u32(normal, 0u, nv, s)) != hipSuccess) return e;
-    if ((e = launch_fill_u32(&stats->total_voxel_fragments, 0u, 1, s)) != hipSuccess) return e;
-    if (sc.ntri == 0) return hipSuccess;
+    if (sc.ntri == 0) return launch_fill_u32(&stats->total_voxel_fragments, 0u, 1, s);
     VoxMats vm;
     for (int i = 0; i < 16; ++i) {
         vm.mvp[0][i] = p.mvp_x[i];
@@ -246,11 +241,14 @@ hipError_t launch_voxelize(const DevScene &sc, const DevLight *dlights, int nlig
     f.dim = dim;
     f.albedo = albedo;
     f.normal = normal;
-    f.stat_frags = &stats->total_voxel_fragments;
+    f.partials = partials;
     f.frags = 0;
+    static_assert(kRasterBlocks <= kStatPartials, "one statistics partial per rasterizer workgroup");
     hipLaunchKernelGGL(k_raster_items<VoxFrag>, dim3(kRasterBlocks), dim3(256), 0, s, rs.rec, rs.offs, rs.chunk,
                        rs.chunk_cap, sc.ntri, f);
-    return hipGetLastError();
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    // totalVoxelFragments (voxelize.frag:195)
+    return launch_reduce_stats(partials, kRasterBlocks, &stats->total_voxel_fragments, nullptr, nullptr, s);
 }
 
 }  // namespace vctk

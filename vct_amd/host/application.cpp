@@ -7,6 +7,7 @@
 // (the 1x1 diffuse texture a material would bind), face or smooth normals and
 // per-triangle tangents from the Mesh.cpp:174-197 rule with uv (0,0), (1,0),
 // (0,1), i.e. tangent = normalize(v1 - v0).
+#include <algorithm>
 #include <array>
 #include <cmath>
 #include <cstdint>
@@ -576,6 +577,94 @@ int vct_app_frame_state(vct_app *a, int frame, float *model, vct_light *lights, 
     }
     if (cam) *cam = a->cam;
     if (trace) *trace = a->trace;
+    return VCT_OK;
+}
+
+// normal matrix (cofactor / determinant of the upper 3x3), as the pass ABI derives it
+static void normal_matrix3(const float *m, float *out) {
+#define A(r, cc) m[(cc)*4 + (r)]
+    float c00 = A(1, 1) * A(2, 2) - A(1, 2) * A(2, 1);
+    float c01 = A(1, 2) * A(2, 0) - A(1, 0) * A(2, 2);
+    float c02 = A(1, 0) * A(2, 1) - A(1, 1) * A(2, 0);
+    float c10 = A(0, 2) * A(2, 1) - A(0, 1) * A(2, 2);
+    float c11 = A(0, 0) * A(2, 2) - A(0, 2) * A(2, 0);
+    float c12 = A(0, 1) * A(2, 0) - A(0, 0) * A(2, 1);
+    float c20 = A(0, 1) * A(1, 2) - A(0, 2) * A(1, 1);
+    float c21 = A(0, 2) * A(1, 0) - A(0, 0) * A(1, 2);
+    float c22 = A(0, 0) * A(1, 1) - A(0, 1) * A(1, 0);
+    float det = (A(0, 0) * c00 + A(0, 1) * c01) + A(0, 2) * c02;
+#undef A
+    const float v[9] = {c00, c10, c20, c01, c11, c21, c02, c12, c22};
+    for (int i = 0; i < 9; ++i) out[i] = v[i] / det;
+}
+
+int vct_app_surface_points(vct_app *a, int frame, int n, uint64_t seed, vct_gbuffer_texel *out) {
+    if (!a || !out || n <= 0) return VCT_ERR_INVALID;
+    const Mesh &m = a->mesh;
+    const int T = m.ntri();
+    if (T == 0) return VCT_ERR_INVALID;
+    std::vector<float> model((size_t)m.nobj * 16), nmat((size_t)m.nobj * 9);
+    frame_models(a, frame, model.data());
+    for (int o = 0; o < m.nobj; ++o) normal_matrix3(model.data() + (size_t)o * 16, nmat.data() + (size_t)o * 9);
+    auto world = [&](int t, int v, float w[3]) {
+        const float *M = model.data() + (size_t)m.obj[t] * 16;
+        const float *p = m.pos.data() + (size_t)t * 9 + v * 3;
+        for (int r = 0; r < 3; ++r) {
+            float s = M[0 * 4 + r] * p[0];
+            s = s + M[1 * 4 + r] * p[1];
+            s = s + M[2 * 4 + r] * p[2];
+            s = s + M[3 * 4 + r] * 1.0f;
+            w[r] = s;
+        }
+    };
+    auto m3 = [](const float *N, const float *v, float o[3]) {
+        for (int r = 0; r < 3; ++r) o[r] = (N[0 * 3 + r] * v[0] + N[1 * 3 + r] * v[1]) + N[2 * 3 + r] * v[2];
+    };
+    std::vector<double> cdf((size_t)T);
+    double acc = 0.0;
+    for (int t = 0; t < T; ++t) {
+        float W[3][3];
+        for (int v = 0; v < 3; ++v) world(t, v, W[v]);
+        double e1[3], e2[3];
+        for (int k = 0; k < 3; ++k) {
+            e1[k] = W[1][k] - W[0][k];
+            e2[k] = W[2][k] - W[0][k];
+        }
+        const double cx = e1[1] * e2[2] - e1[2] * e2[1], cy = e1[2] * e2[0] - e1[0] * e2[2],
+                     cz = e1[0] * e2[1] - e1[1] * e2[0];
+        acc += 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
+        cdf[(size_t)t] = acc;
+    }
+    Rng r(seed * 0x9E3779B97F4A7C15ull + 777);
+    for (int i = 0; i < n; ++i) {
+        const double u = r.uni() * acc;
+        int t = (int)(std::lower_bound(cdf.begin(), cdf.end(), u) - cdf.begin());
+        if (t >= T) t = T - 1;
+        double s1 = std::sqrt(r.uni()), s2 = r.uni();
+        float b[3] = {(float)(1.0 - s1), (float)(s1 * (1.0 - s2)), (float)(s1 * s2)};
+        float W[3][3], Nv[3][3], Tv[3][3], Bv[3][3], Tw[3];
+        const float *Nm = nmat.data() + (size_t)m.obj[t] * 9;
+        m3(Nm, m.tan.data() + (size_t)t * 3, Tw);
+        for (int v = 0; v < 3; ++v) {
+            world(t, v, W[v]);
+            m3(Nm, m.nrm.data() + (size_t)t * 9 + v * 3, Nv[v]);
+            const float dtn = (Tw[0] * Nv[v][0] + Tw[1] * Nv[v][1]) + Tw[2] * Nv[v][2];
+            float tt[3] = {Tw[0] - dtn * Nv[v][0], Tw[1] - dtn * Nv[v][1], Tw[2] - dtn * Nv[v][2]};
+            nrm3(tt, Tv[v]);
+            cross3(Nv[v], Tv[v], Bv[v]);
+        }
+        vct_gbuffer_texel &g = out[i];
+        for (int k = 0; k < 3; ++k) {
+            g.position[k] = (b[0] * W[0][k] + b[1] * W[1][k]) + b[2] * W[2][k];
+            g.normal[k] = (b[0] * Nv[0][k] + b[1] * Nv[1][k]) + b[2] * Nv[2][k];
+            g.tangent[k] = (b[0] * Tv[0][k] + b[1] * Tv[1][k]) + b[2] * Tv[2][k];
+            g.bitangent[k] = (b[0] * Bv[0][k] + b[1] * Bv[1][k]) + b[2] * Bv[2][k];
+        }
+        g.albedo = m.albedo[(size_t)t];
+        g.roughness = m.rough.empty() ? 0.5f : m.rough[m.obj[t]];
+        g.covered = 1;
+        g.triangle = (uint32_t)t;
+    }
     return VCT_OK;
 }
 
