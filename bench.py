@@ -36,8 +36,8 @@ def level_sizes(dim: int, levels: int):
 
 
 def algorithmic_bytes(cfg, ntri: int, nlights_sm: int, sm: int) -> dict:
-    """Compulsory HBM bytes per launch of each pass (the per-unit figures are
-    listed in DESIGN.md)."""
+    """Compulsory HBM bytes per launch of each pass: every input read once and
+    every output written once, nothing re-read."""
     dim, L = cfg.dim, cfg.levels
     nv = dim ** 3
     bpv = {0: 8, 1: 16, 2: 8}[cfg.storage]  # fp8 mode keeps levels 0-2 in fp16
@@ -100,6 +100,111 @@ def cpu_baseline(app, frame: int, pixel_stride: int = 64) -> dict:
                           "mipmap": round((t3 - t2) * 1e3, 1), "render": round(trace_ms, 1)}}
 
 
+def learned_leg(args) -> dict:
+    """Learned vs plain diffuse gather on the small scene (64^3, 7 levels,
+    fp16 isotropic mips) at 1920x1080: train the network on the GPU on
+    2^20 surface points (Adam, lr 1e-3, batch 4096), then time the plain
+    6-cone gather and the learned gather (with its per-8x8-tile fallback) on
+    the same G-buffer.  Errors (mean relative, rgb) are measured against the
+    64-cone FP64-accumulated reference on a 480x270 G-buffer of the same
+    camera, where that reference takes a fraction of a second."""
+    import numpy as np
+
+    from vct_amd import _abi as abi
+
+    app = vct_amd_app(1, width=1920, height=1080)
+    ctx = app.ctx
+    t0 = time.perf_counter()
+    app.build_radiance(0)
+    pts = app.surface_points(args.learned_points, 0, seed=7)
+    ctx.learned_set_points(pts)
+    ctx.learned_prepare()
+    ctx.sync()
+    t1 = time.perf_counter()
+    loss = ctx.learned_train(steps=args.learned_steps, batch=4096, lr=1e-3, seed=1, reset=True)
+    ctx.sync()
+    t2 = time.perf_counter()
+    fs = app.frame_state(0)
+    tr = fs["trace"]
+    tr.enable_reflections = 0
+    cam = fs["cam"]
+
+    def errors(threshold):
+        small = abi.Camera.from_buffer_copy(cam)
+        small.width, small.height = 480, 270
+        ctx.gbuffer(small)
+        ctx.trace(tr)
+        plain = ctx.download(abi.VCT_BUF_INDIRECT, 0, np.float32).reshape(-1, 4)[:, :3].astype(np.float64)
+        ctx.reference_gather()
+        ref = ctx.download(abi.VCT_BUF_REFERENCE, 0, np.float32).reshape(-1, 4)[:, :3].astype(np.float64)
+        ctx.trace_learned(tr, threshold)
+        lrn = ctx.download(abi.VCT_BUF_INDIRECT, 0, np.float32).reshape(-1, 4)[:, :3].astype(np.float64)
+        den = np.abs(ref).sum()
+        return float(np.abs(plain - ref).sum() / den), float(np.abs(lrn - ref).sum() / den)
+
+    e_plain, e_learned = errors(1e30)
+    _, e_learned_fb = errors(args.learned_threshold)
+    ctx.gbuffer(cam)
+    reps = 10
+    for _ in range(2):
+        ctx.trace(tr)
+        ctx.trace_learned(tr, args.learned_threshold, count_fallback=False)
+    ctx.sync()
+    ctx.pass_times()
+    for _ in range(reps):
+        ctx.trace(tr)
+    ctx.sync()
+    plain_ms = ctx.pass_times().render_ms / reps
+    for _ in range(reps):
+        ctx.trace_learned(tr, 1e30, count_fallback=False)
+    ctx.sync()
+    learned_ms = ctx.pass_times().learned_ms / reps
+    for _ in range(reps):
+        ctx.trace_learned(tr, args.learned_threshold, count_fallback=False)
+    ctx.sync()
+    learned_fb_ms = ctx.pass_times().learned_ms / reps
+    nfb = ctx.trace_learned(tr, args.learned_threshold)
+    tiles = ((cam.width + 7) // 8) * ((cam.height + 7) // 8)
+    app.close()
+    return {
+        "workload": "64^3 x 7 levels fp16 isotropic, 1920x1080 diffuse gather; network 129-64-64-64-3 BF16 MFMA",
+        "train": {"points": args.learned_points, "steps": args.learned_steps, "batch": 4096, "lr": 1e-3,
+                  "label_and_feature_s": round(t1 - t0, 3), "train_s": round(t2 - t1, 3),
+                  "loss_first": float(loss[:10].mean()) if len(loss) else None,
+                  "loss_last": float(loss[-100:].mean()) if len(loss) else None},
+        "plain_ms": round(plain_ms, 4),
+        "learned_ms": round(learned_ms, 4),
+        "learned_with_fallback_ms": round(learned_fb_ms, 4),
+        "speedup_vs_plain": round(plain_ms / learned_ms, 3) if learned_ms > 0 else None,
+        "fallback_threshold": args.learned_threshold,
+        "fallback_tiles": nfb,
+        "tiles": tiles,
+        "mean_rel_err_vs_64cone": {"plain_6cone": round(e_plain, 5), "learned": round(e_learned, 5),
+                                   "learned_with_fallback": round(e_learned_fb, 5)},
+    }
+
+
+def vct_amd_app(scene, **kw):
+    import vct_amd
+
+    return vct_amd.App(scene, device=0, **kw)
+
+
+def copy_calibration(nbytes: int) -> int:
+    """A plain copy kernel that reads and writes `nbytes` each: the yardstick
+    that turns the FETCH_SIZE / WRITE_SIZE counters into bytes."""
+    import torch
+
+    n = nbytes // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 3}), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -108,10 +213,19 @@ def main() -> int:
     ap.add_argument("--scene", type=int, default=2)
     ap.add_argument("--storage", type=int, default=-1, help="0 fp16, 1 fp32, 2 fp8 (levels >= 3)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-plain gather leg")
+    ap.add_argument("--learned-steps", type=int, default=20000)
+    ap.add_argument("--learned-points", type=int, default=1 << 20)
+    ap.add_argument("--learned-threshold", type=float, default=1e-4,
+                    help="predicted-luminance variance above which an 8x8 tile falls back to the 6-cone tracer")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="only run a plain device-to-device copy of this many bytes (counter calibration)")
     args = ap.parse_args()
+    if args.copy_bytes:
+        return copy_calibration(args.copy_bytes)
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(how the path would shard is described in DESIGN.md)", file=sys.stderr)
+              "(the pipeline runs on one GPU per context)", file=sys.stderr)
         return 2
 
     import vct_amd
@@ -173,13 +287,17 @@ def main() -> int:
                      "unit": "GB/s", "frac": round(trace_gbs / PEAK_HBM_GBS, 4), "traffic": None,
                      "algorithmic_bytes": ab["trace"]},
     }
+    app.close()
+    if not args.no_learned:
+        out["learned_vs_plain"] = learned_leg(args)
     if not args.no_cpu_baseline:
+        app = vct_amd_app(args.scene, storage=args.storage)
         try:
             out["cpu_baseline"] = cpu_baseline(app, args.warmup + args.steps)
         except Exception as e:  # the baseline is reported, never required
             out["cpu_baseline"] = {"value": None, "error": repr(e)}
+        app.close()
     print(json.dumps(out), flush=True)
-    app.close()
     return 0
 
 

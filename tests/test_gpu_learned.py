@@ -164,7 +164,7 @@ def test_training_reduces_loss_and_beats_plain_gather(prepared):
     loss = ctx.learned_train(steps=600, batch=1024, lr=2e-3, seed=3, reset=True)
     assert np.isfinite(loss).all()
     assert loss[-50:].mean() < 0.5 * loss[:10].mean()
-    # learned vs plain 6-cone gather, both judged by the 64-cone reference
+    # learned vs plain 6-cone gather, both measured against the 64-cone reference
     fs = app.frame_state(0)
     ctx.gbuffer(fs["cam"])
     tr = fs["trace"]

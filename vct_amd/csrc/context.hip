@@ -186,47 +186,45 @@ static int alloc_grids(vct_ctx *c) {
     P.nlevels = c->desc.levels;
     P.storage = c->desc.storage;
     P.aniso = c->desc.anisotropic;
+    // one allocation: per level, the faces' bricks interleaved (face f of brick
+    // b at element 64 * (b * nfaces + f)), then the fp16-pair stage copy of
+    // fp8 levels; every offset 256-byte aligned
     size_t off = 0, soff = 0;
-    size_t base_off[VCT_MAX_LEVELS][6], stage_off[VCT_MAX_LEVELS][6], scale_off[VCT_MAX_LEVELS][6];
     for (int l = 0; l < P.nlevels; ++l) {
         LevelInfo &L = P.lv[l];
         L.size = std::max(1, dim >> l);
         L.nb = (L.size + 3) / 4;
         L.fmt = P.storage == VCT_STORE_FP32 ? 1 : ((P.storage == VCT_STORE_FP8 && l >= 3) ? 2 : 0);
         L.nfaces = (P.aniso && l >= 1) ? 6 : 1;
-        const size_t nvox = (size_t)L.nb * L.nb * L.nb * 64;
-        const size_t bpv = L.fmt == 1 ? 16 : (L.fmt == 2 ? 4 : 8);
-        for (int f = 0; f < L.nfaces; ++f) {
-            base_off[l][f] = off;
-            off += align256(nvox * bpv);
-            if (L.fmt == 2) {
-                stage_off[l][f] = off;
-                off += align256(nvox * 8);
-                scale_off[l][f] = soff;
-                soff += (size_t)L.nb * L.nb * L.nb;
-            } else {
-                stage_off[l][f] = base_off[l][f];
-                scale_off[l][f] = 0;
-            }
+        L.sx = 64u * (uint32_t)L.nfaces;
+        L.sy = L.sx * (uint32_t)L.nb;
+        L.sz = L.sy * (uint32_t)L.nb;
+        const size_t nelem = (size_t)L.sz * L.nb;
+        L.base = (uint32_t)off;
+        off += align256(nelem * (L.fmt == 2 ? 4 : 16));
+        if (L.fmt == 2) {
+            L.stage = (uint32_t)off;
+            off += align256(nelem * 16);
+            L.scale = (uint32_t)soff;
+            soff += nelem / 64;
+        } else {
+            L.stage = L.base;
+            L.scale = 0;
         }
+        if (off > 0xffffff00ull)
+            return fail(c, VCT_ERR_INVALID, "radiance pyramid of %zu bytes exceeds the 32-bit offset range", off);
     }
     c->pyr_bytes = off;
     c->scale_floats = soff;
     HIPCHK(c, dalloc(&c->d_pyr, off));
-    HIPCHK(c, hipMemsetAsync(c->d_pyr, 0, off, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_pyr, 0, off, c->stream));  // the fp16-pair border halves stay 0
     if (soff) {
         HIPCHK(c, dalloc(&c->d_scales, soff * 4));
         HIPCHK(c, hipMemsetAsync(c->d_scales, 0, soff * 4, c->stream));
     }
-    for (int l = 0; l < P.nlevels; ++l) {
-        LevelInfo &L = P.lv[l];
-        for (int f = 0; f < 6; ++f) {
-            const int ff = f < L.nfaces ? f : 0;
-            L.base[f] = c->d_pyr + base_off[l][ff];
-            L.stage[f] = c->d_pyr + stage_off[l][ff];
-            L.scale[f] = c->d_scales ? c->d_scales + scale_off[l][ff] : nullptr;
-        }
-    }
+    P.pyr = c->d_pyr;
+    P.bytes = (uint32_t)off;
+    P.scales = c->d_scales;
     HIPCHK(c, dalloc(&c->d_lv, sizeof(LevelInfo) * P.nlevels));
     HIPCHK(c, hipMemcpyAsync(c->d_lv, P.lv, sizeof(LevelInfo) * P.nlevels, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -633,8 +631,11 @@ int vct_fill_holes(vct_ctx *c) {
     if (!c) return fail(nullptr, VCT_ERR_INVALID, "ctx is NULL");
     if (!c->have_radiance) return fail(c, VCT_ERR_STATE, "vct_fill_holes: inject first");
     const LevelInfo &L0 = c->pyr.lv[0];
-    if (!c->d_fill_tmp)
-        HIPCHK(c, dalloc((char **)&c->d_fill_tmp, (size_t)L0.nb * L0.nb * L0.nb * 64 * (L0.fmt == 1 ? 16 : 8)));
+    if (!c->d_fill_tmp) {  // level 0's layout; its fp16-pair border halves stay 0
+        const size_t bytes = (size_t)L0.sz * L0.nb * 16;
+        HIPCHK(c, dalloc((char **)&c->d_fill_tmp, bytes));
+        HIPCHK(c, hipMemsetAsync(c->d_fill_tmp, 0, bytes, c->stream));
+    }
     PassTimer pt(c, P_FILL);
     LAUNCHCHK(c, launch_fill_holes(c->pyr, c->d_fill_tmp, c->stream));
     c->have_mips = false;
@@ -787,7 +788,7 @@ size_t vct_buffer_bytes(const vct_ctx *c, int what, int level) {
     case VCT_BUF_RADIANCE_RAW: {
         if (!decode_level(c, level, &l, &f)) return 0;
         const LevelInfo &L = c->pyr.lv[l];
-        return (size_t)L.nb * L.nb * L.nb * 64 * (L.fmt == 1 ? 16 : (L.fmt == 2 ? 4 : 8));
+        return (size_t)L.sz * L.nb * (L.fmt == 2 ? 4 : 16);
     }
     case VCT_BUF_SHADOWMAP:
         if (level < 0 || level >= (int)c->lights.size() || !c->lights[level].shadow_caster) return 0;
@@ -864,7 +865,7 @@ int vct_download(vct_ctx *c, int what, int level, void *host, size_t bytes) {
     }
     case VCT_BUF_RADIANCE_RAW:
         decode_level(c, level, &l, &f);
-        src = c->pyr.lv[l].base[f];
+        src = c->pyr.pyr + c->pyr.lv[l].base;
         break;
     default: return fail(c, VCT_ERR_INVALID, "vct_download: unknown buffer %d", what);
     }

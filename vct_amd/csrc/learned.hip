@@ -39,58 +39,22 @@ __device__ __forceinline__ uint16_t f2bf(float f) {  // round to nearest even
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
 // ---------------------------------------------------------------- sampling
-// GL LINEAR inside one isotropic level (border 0), values decoded to float.
-__device__ __forceinline__ float4 level_linear(const LevelInfo &L, int s, const float p[3]) {
-    const float u[3] = {p[0] * (float)s - 0.5f, p[1] * (float)s - 0.5f, p[2] * (float)s - 0.5f};
-    int i0[3];
-    float f[3];
-    for (int k = 0; k < 3; ++k) {
-        const float fl = floorf(u[k]);
-        i0[k] = (int)fl;
-        f[k] = u[k] - fl;
-    }
-    const int nb = (s + 3) >> 2;
-    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-    for (int c = 0; c < 8; ++c) {
-        const int dx = c & 1, dy = (c >> 1) & 1, dz = c >> 2;
-        const int x = i0[0] + dx, y = i0[1] + dy, z = i0[2] + dz;
-        if ((unsigned)x >= (unsigned)s || (unsigned)y >= (unsigned)s || (unsigned)z >= (unsigned)s) continue;
-        const float w = ((dx ? f[0] : 1.0f - f[0]) * (dy ? f[1] : 1.0f - f[1])) * (dz ? f[2] : 1.0f - f[2]);
-        const uint32_t a = brick_addr(x, y, z, nb);
-        float4 t;
-        if (L.fmt == 1) {
-            t = ((const float4 *)L.base[0])[a];
-        } else if (L.fmt == 2) {
-            const uint32_t cc = ((const uint32_t *)L.base[0])[a];
-            const float sc = L.scale[0][a >> 6];
-            t = make_float4(fp82f(cc & 0xff) * sc, fp82f((cc >> 8) & 0xff) * sc, fp82f((cc >> 16) & 0xff) * sc,
-                            fp82f(cc >> 24) * sc);
-        } else {
-            const uint2 v = ((const uint2 *)L.base[0])[a];
-            t = make_float4(h2f((uint16_t)(v.x & 0xffff)), h2f((uint16_t)(v.x >> 16)), h2f((uint16_t)(v.y & 0xffff)),
-                            h2f((uint16_t)(v.y >> 16)));
-        }
-        acc.x += w * t.x;
-        acc.y += w * t.y;
-        acc.z += w * t.z;
-        acc.w += w * t.w;
-    }
-    return acc;
-}
+struct PyrView {
+    const char *pyr;
+    const float *scales;
+    const LevelInfo *lv;
+    int dim, nlevels;
+};
 
-__device__ __forceinline__ float4 sample_iso(const LevelInfo *__restrict__ lv, int dim, int nlevels, const float p[3],
-                                             float lam) {
-    const int q = nlevels - 1;
-    if (!(lam < (float)q)) {
-        const int s = (dim >> q) > 0 ? (dim >> q) : 1;
-        return level_linear(lv[q], s, p);
-    }
+// textureLod with LINEAR_MIPMAP_LINEAR on an isotropic pyramid (border 0).
+__device__ __forceinline__ float4 sample_iso(const PyrView &v, const float p[3], float lam) {
+    const int q = v.nlevels - 1;
+    if (!(lam < (float)q)) return level_linear(v.pyr, v.scales, v.lv[q], p);
     if (lam < 0.0f) lam = 0.0f;
     int d1 = (int)floorf(lam);
     d1 = d1 > q - 1 ? q - 1 : d1;
     const float fr = lam - (float)d1;
-    const int s1 = (dim >> d1) > 0 ? (dim >> d1) : 1, s2 = (dim >> (d1 + 1)) > 0 ? (dim >> (d1 + 1)) : 1;
-    const float4 a = level_linear(lv[d1], s1, p), b = level_linear(lv[d1 + 1], s2, p);
+    const float4 a = level_linear(v.pyr, v.scales, v.lv[d1], p), b = level_linear(v.pyr, v.scales, v.lv[d1 + 1], p);
     return make_float4((1.0f - fr) * a.x + fr * b.x, (1.0f - fr) * a.y + fr * b.y, (1.0f - fr) * a.z + fr * b.z,
                        (1.0f - fr) * a.w + fr * b.w);
 }
@@ -109,9 +73,9 @@ __device__ __forceinline__ void label_dir(int i, double d[3], double *w) {
     *w = y / 32.0;
 }
 
-__global__ __launch_bounds__(64) void k_label64(const vct_gbuffer_texel *__restrict__ pts, int npts,
-                                                const LevelInfo *__restrict__ lv, int dim, int nlevels, Extent ext,
-                                                float4 *out) {
+__global__ __launch_bounds__(64) void k_label64(const vct_gbuffer_texel *__restrict__ pts, int npts, PyrView pv,
+                                                Extent ext, float4 *out) {
+    const int dim = pv.dim;
     const int lane = threadIdx.x;
     for (int pi = blockIdx.x; pi < npts; pi += gridDim.x) {
         const float4 *g = (const float4 *)&pts[pi];
@@ -153,7 +117,7 @@ __global__ __launch_bounds__(64) void k_label64(const vct_gbuffer_texel *__restr
             }
             if (outside) break;
             // trilinear sampling in FP32; transmittance and radiance accumulate in FP64
-            const float4 s = sample_iso(lv, dim, nlevels, sp, (float)lod);
+            const float4 s = sample_iso(pv, sp, (float)lod);
             double sa = s.w < 0.f ? 0.0 : (s.w > 1.f ? 1.0 : (double)s.w);
             const double ah = 1.0 - sqrt(1.0 - sa);
             const double tr = 1.0 - a;
@@ -179,18 +143,19 @@ __global__ __launch_bounds__(64) void k_label64(const vct_gbuffer_texel *__restr
 //   [108, 111) gather axis normalize(B)
 //   [111, 129) for c in x,y,z: for f in 1,2,4: sin(pi f p_c), cos(pi f p_c)
 //   [129, 160) zero
-__device__ __forceinline__ void make_features(const float4 g0, const float4 g3, const LevelInfo *__restrict__ lv,
-                                              int dim, const Extent &ext, uint16_t *row) {
+__device__ __forceinline__ void make_features(const float4 g0, const float4 g3, const PyrView &pv, const Extent &ext,
+                                              uint16_t *row) {
+    const int dim = pv.dim;
     float p[3];
     const float gp[3] = {g0.x, g0.y, g0.z};
     for (int k = 0; k < 3; ++k) p[k] = ((gp[k] - ext.center[k]) - ext.vmin[k]) / (ext.vmax[k] - ext.vmin[k]);
     const int s2 = (dim >> 2) > 0 ? (dim >> 2) : 1;
     const float step = 1.0f / (float)s2;
-    const LevelInfo &L2 = lv[2];
+    const LevelInfo &L2 = pv.lv[2];
     for (int i = 0; i < 27; ++i) {
         const int ix = i % 3 - 1, iy = (i / 3) % 3 - 1, iz = i / 9 - 1;
         const float q[3] = {p[0] + ix * step, p[1] + iy * step, p[2] + iz * step};
-        const float4 v = level_linear(L2, s2, q);
+        const float4 v = level_linear(pv.pyr, pv.scales, L2, q);
         row[i * 4 + 0] = f2bf(v.x);
         row[i * 4 + 1] = f2bf(v.y);
         row[i * 4 + 2] = f2bf(v.z);
@@ -209,13 +174,12 @@ __device__ __forceinline__ void make_features(const float4 g0, const float4 g3, 
     for (; o < kMlpIn; ++o) row[o] = 0;
 }
 
-__global__ __launch_bounds__(256) void k_features(const vct_gbuffer_texel *__restrict__ pts, int npts,
-                                                  const LevelInfo *__restrict__ lv, int dim, Extent ext,
-                                                  uint16_t *X) {
+__global__ __launch_bounds__(256) void k_features(const vct_gbuffer_texel *__restrict__ pts, int npts, PyrView pv,
+                                                  Extent ext, uint16_t *X) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= npts) return;
     const float4 *g = (const float4 *)&pts[i];
-    make_features(g[0], g[3], lv, dim, ext, X + (size_t)i * kMlpIn);
+    make_features(g[0], g[3], pv, ext, X + (size_t)i * kMlpIn);
 }
 
 // ---------------------------------------------------------------- MFMA tiles
@@ -466,7 +430,7 @@ struct InferSmem {
 };
 
 __global__ __launch_bounds__(64) void k_learned(MlpDev W, const vct_gbuffer_texel *__restrict__ gbuf, int width,
-                                                int height, const LevelInfo *__restrict__ lv, int dim, Extent ext,
+                                                int height, PyrView pv, Extent ext,
                                                 float threshold, float4 *out, uint32_t *tile_list,
                                                 uint32_t *tile_count) {
     __shared__ __align__(16) InferSmem S;
@@ -480,7 +444,7 @@ __global__ __launch_bounds__(64) void k_learned(MlpDev W, const vct_gbuffer_texe
         const float4 *g = (const float4 *)&gbuf[(size_t)py * width + px];
         const float4 g0 = g[0], g2 = g[2], g3 = g[3];
         covered = __float_as_uint(g2.w) != 0u;
-        if (covered) make_features(g0, g3, lv, dim, ext, S.x + lane * kMlpIn);
+        if (covered) make_features(g0, g3, pv, ext, S.x + lane * kMlpIn);
     }
     if (!covered)
         for (int k = 0; k < kMlpIn; ++k) S.x[lane * kMlpIn + k] = 0;
@@ -551,16 +515,26 @@ __global__ __launch_bounds__(64) void k_learned(MlpDev W, const vct_gbuffer_texe
 }
 
 // ---------------------------------------------------------------- launchers
+static PyrView pyr_view(const PyrDesc &pyr) {
+    PyrView v;
+    v.pyr = pyr.pyr;
+    v.scales = pyr.scales;
+    v.lv = pyr.d_lv;
+    v.dim = pyr.dim;
+    v.nlevels = pyr.nlevels;
+    return v;
+}
+
 hipError_t launch_label64(const PyrDesc &pyr, const vct_gbuffer_texel *pts, int npts, Extent ext, float4 *out,
                           hipStream_t s) {
     const int blocks = npts < 65536 ? (npts > 0 ? npts : 1) : 65536;
-    hipLaunchKernelGGL(k_label64, dim3(blocks), dim3(64), 0, s, pts, npts, pyr.d_lv, pyr.dim, pyr.nlevels, ext, out);
+    hipLaunchKernelGGL(k_label64, dim3(blocks), dim3(64), 0, s, pts, npts, pyr_view(pyr), ext, out);
     return hipGetLastError();
 }
 
 hipError_t launch_features(const PyrDesc &pyr, const vct_gbuffer_texel *pts, int npts, Extent ext, uint16_t *X,
                            hipStream_t s) {
-    hipLaunchKernelGGL(k_features, dim3((npts + 255) / 256), dim3(256), 0, s, pts, npts, pyr.d_lv, pyr.dim, ext, X);
+    hipLaunchKernelGGL(k_features, dim3((npts + 255) / 256), dim3(256), 0, s, pts, npts, pyr_view(pyr), ext, X);
     return hipGetLastError();
 }
 
@@ -606,8 +580,8 @@ hipError_t launch_learned(const MlpBuffers &B, const PyrDesc &pyr, const vct_gbu
     const int tiles = ((width + 7) / 8) * ((height + 7) / 8);
     hipError_t e = launch_fill_u32(tile_count, 0u, 1, s);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_learned, dim3(tiles), dim3(64), 0, s, mlp_dev(B), gbuf, width, height, pyr.d_lv, pyr.dim,
-                       ext, threshold, out, tile_list, tile_count);
+    hipLaunchKernelGGL(k_learned, dim3(tiles), dim3(64), 0, s, mlp_dev(B), gbuf, width, height, pyr_view(pyr), ext,
+                       threshold, out, tile_list, tile_count);
     return hipGetLastError();
 }
 

@@ -20,33 +20,14 @@
 
 namespace vctk {
 
-struct MipLevelArgs {
-    char *stage[6];
-    int nb, size;
-};
-
 struct MipArgs {
-    MipLevelArgs src;
-    MipLevelArgs out[3];
+    char *pyr;
+    LevelInfo src;
+    LevelInfo out[3];
     int nout;
-    int src_faces;  // 1, or 6 when the source level is anisotropic
-    int fp32;       // stage format: 1 = fp32x4, 0 = fp16x4
+    int fp32;  // stage format: 1 = fp32x4, 0 = fp16 pairs
 };
 
-__device__ __forceinline__ float4 ld_stage(const char *base, int fp32, uint32_t a) {
-    if (fp32) return ((const float4 *)base)[a];
-    const uint2 v = ((const uint2 *)base)[a];
-    return make_float4(h2f((uint16_t)(v.x & 0xffff)), h2f((uint16_t)(v.x >> 16)), h2f((uint16_t)(v.y & 0xffff)),
-                       h2f((uint16_t)(v.y >> 16)));
-}
-__device__ __forceinline__ void st_stage(char *base, int fp32, uint32_t a, float4 c) {
-    if (fp32) {
-        ((float4 *)base)[a] = c;
-        return;
-    }
-    ((uint2 *)base)[a] = make_uint2((uint32_t)f2h(c.x) | ((uint32_t)f2h(c.y) << 16),
-                                    (uint32_t)f2h(c.z) | ((uint32_t)f2h(c.w) << 16));
-}
 __device__ __forceinline__ float4 round_stage(float4 v, int fp32) {
     if (fp32) return v;
     return make_float4(q16(v.x), q16(v.y), q16(v.z), q16(v.w));
@@ -69,20 +50,33 @@ __global__ __launch_bounds__(256) void k_mip_chain(MipArgs a) {
     const int ox = rx * R, oy = ry * R, oz = rz * R;
     const int R3 = R * R * R;
     for (int f = 0; f < SF; ++f) {  // the launcher picks SRC6 from the source level's face count
-        for (int i = tid; i < R3; i += 256) {
-            int x, y, z;
-            if (R == 8) {  // brick-major walk: 8 bricks of 64 contiguous voxels
-                const int j = i >> 6, v = i & 63;
-                x = ox + (j & 1) * 4 + (v & 3);
-                y = oy + ((j >> 1) & 1) * 4 + ((v >> 2) & 3);
-                z = oz + (j >> 2) * 4 + (v >> 4);
-            } else {
-                x = i % R;
-                y = (i / R) % R;
-                z = i / (R * R);
+        if (R == 8 && !a.fp32) {
+            // fp16 pairs: the elements of even x hold texels x and x+1; walk
+            // brick by brick (32 such elements per brick)
+            const int i = tid;  // 256 threads, 256 pair elements
+            const int j = i >> 5, v = i & 31;
+            const int x = ox + (j & 1) * 4 + (v & 1) * 2;
+            const int y = oy + ((j >> 1) & 1) * 4 + ((v >> 1) & 3);
+            const int z = oz + (j >> 2) * 4 + (v >> 3);
+            const uint4 e = *(const uint4 *)(a.pyr + a.src.stage + (size_t)elem_index(a.src, x, y, z, f) * 16);
+            float4 *d = &bufA[f][((z - oz) * R + (y - oy)) * R + (x - ox)];
+            d[0] = unpack_h4(e.x, e.y);
+            d[1] = unpack_h4(e.z, e.w);
+        } else {
+            for (int i = tid; i < R3; i += 256) {
+                int x, y, z;
+                if (R == 8) {  // brick-major walk: 8 bricks of 64 contiguous voxels
+                    const int j = i >> 6, v = i & 63;
+                    x = ox + (j & 1) * 4 + (v & 3);
+                    y = oy + ((j >> 1) & 1) * 4 + ((v >> 2) & 3);
+                    z = oz + (j >> 2) * 4 + (v >> 4);
+                } else {
+                    x = i % R;
+                    y = (i / R) % R;
+                    z = i / (R * R);
+                }
+                bufA[f][((z - oz) * R + (y - oy)) * R + (x - ox)] = load_stage_texel(a.pyr, a.src, x, y, z, f);
             }
-            const float4 val = ld_stage(a.src.stage[f], a.fp32, brick_addr(x, y, z, a.src.nb));
-            bufA[f][((z - oz) * R + (y - oy)) * R + (x - ox)] = val;
         }
     }
     __syncthreads();
@@ -96,7 +90,7 @@ __global__ __launch_bounds__(256) void k_mip_chain(MipArgs a) {
         // bufA (k odd, <= 2^3 per face)
         float4 *out = (k & 1) ? &bufA[0][0] : &bufB[0][0];
         const int out_stride = (k & 1) ? 8 : 64;
-        const MipLevelArgs &ol = a.out[k];
+        const LevelInfo &ol = a.out[k];
         const int sh = k + 1;
         const int gx = ox >> sh, gy = oy >> sh, gz = oz >> sh;
         const int d3 = d * d * d;
@@ -151,7 +145,7 @@ __global__ __launch_bounds__(256) void k_mip_chain(MipArgs a) {
             }
             v = round_stage(v, a.fp32);
             out[(size_t)f * out_stride + vi] = v;
-            st_stage(ol.stage[f], a.fp32, brick_addr(gx + x, gy + y, gz + z, ol.nb), v);
+            store_stage_texel(a.pyr, ol, gx + x, gy + y, gz + z, NF == 6 ? f : 0, v);
         }
         __syncthreads();
         in = out;
@@ -161,22 +155,25 @@ __global__ __launch_bounds__(256) void k_mip_chain(MipArgs a) {
     }
 }
 
-// FP8 E4M3 encode of one level: one 64-thread wavefront per 4^3 brick.
-__global__ __launch_bounds__(64) void k_fp8_encode(const char *stage, char *base, float *scale, int nb, int size) {
-    const int b = blockIdx.x;
+// FP8 E4M3 encode of one level: one 64-thread wavefront per (brick, face);
+// the scale is the largest magnitude in the brick over 448 (1 if all zero).
+__global__ __launch_bounds__(64) void k_fp8_encode(char *pyr, float *scales, LevelInfo L) {
+    const int bf = blockIdx.x;  // brick * nfaces + face
+    const int b = bf / L.nfaces, f = bf % L.nfaces;
     const int v = threadIdx.x;
+    const int nb = L.nb;
     const int bx = b % nb, by = (b / nb) % nb, bz = b / (nb * nb);
     const int x = bx * 4 + (v & 3), y = by * 4 + ((v >> 2) & 3), z = bz * 4 + (v >> 4);
-    const bool valid = x < size && y < size && z < size;
-    const uint32_t a = ((uint32_t)b << 6) | (uint32_t)v;
+    const bool valid = x < L.size && y < L.size && z < L.size;
     float4 c = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (valid) c = ld_stage(stage, 0, a);
+    if (valid) c = load_stage_texel(pyr, L, x, y, z, f);
     float m = fmaxf(fmaxf(__builtin_fabsf(c.x), __builtin_fabsf(c.y)), fmaxf(__builtin_fabsf(c.z), __builtin_fabsf(c.w)));
     for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
     const float sc = m > 0.0f ? m / 448.0f : 1.0f;
-    if (v == 0) scale[b] = sc;
+    const uint32_t e = (uint32_t)bf * 64u + (uint32_t)v;  // = elem_index(L, x, y, z, f)
+    if (v == 0) scales[L.scale + (uint32_t)bf] = sc;
     const uint32_t code = f2fp8(c.x / sc) | (f2fp8(c.y / sc) << 8) | (f2fp8(c.z / sc) << 16) | (f2fp8(c.w / sc) << 24);
-    ((uint32_t *)base)[a] = code;
+    *(uint32_t *)(pyr + L.base + (size_t)e * 4) = code;
 }
 
 hipError_t launch_mip(const PyrDesc &pyr, hipStream_t s) {
@@ -190,17 +187,10 @@ hipError_t launch_mip(const PyrDesc &pyr, hipStream_t s) {
         int nout = pyr.nlevels - 1 - src;
         if (nout > lg) nout = lg;
         MipArgs a;
-        for (int f = 0; f < 6; ++f) a.src.stage[f] = S.stage[f < S.nfaces ? f : 0];
-        a.src.nb = S.nb;
-        a.src.size = S.size;
-        for (int k = 0; k < 3; ++k) {
-            const LevelInfo &O = pyr.lv[k < nout ? src + 1 + k : src + 1];
-            for (int f = 0; f < 6; ++f) a.out[k].stage[f] = O.stage[f < O.nfaces ? f : 0];
-            a.out[k].nb = O.nb;
-            a.out[k].size = O.size;
-        }
+        a.pyr = pyr.pyr;
+        a.src = S;
+        for (int k = 0; k < 3; ++k) a.out[k] = pyr.lv[k < nout ? src + 1 + k : src + 1];
         a.nout = nout;
-        a.src_faces = S.nfaces;
         a.fp32 = fp32;
         const int nreg = S.size / R;
         dim3 grid((unsigned)(nreg * nreg * nreg));
@@ -214,12 +204,10 @@ hipError_t launch_mip(const PyrDesc &pyr, hipStream_t s) {
     if (pyr.storage == VCT_STORE_FP8) {
         for (int l = 3; l < pyr.nlevels; ++l) {
             const LevelInfo &L = pyr.lv[l];
-            for (int f = 0; f < L.nfaces; ++f) {
-                hipLaunchKernelGGL(k_fp8_encode, dim3((unsigned)(L.nb * L.nb * L.nb)), dim3(64), 0, s, L.stage[f],
-                                   L.base[f], L.scale[f], L.nb, L.size);
-                hipError_t e = hipGetLastError();
-                if (e != hipSuccess) return e;
-            }
+            hipLaunchKernelGGL(k_fp8_encode, dim3((unsigned)(L.nb * L.nb * L.nb * L.nfaces)), dim3(64), 0, s, pyr.pyr,
+                               pyr.scales, L);
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return e;
         }
     }
     return hipSuccess;

@@ -72,8 +72,9 @@ hipError_t launch_inject_scatter(const DevLight *lights, const DevLight *dlights
 // LDS and writes one partial.
 template <int FMT>
 __global__ __launch_bounds__(256) void k_normalize_inject(uint32_t *albedo, const uint8_t *__restrict__ litmask,
-                                                          int dim, int nb, float opacity, char *rad0,
+                                                          int dim, LevelInfo L0, float opacity, char *pyr,
                                                           uint32_t *partials) {
+    const int nb = L0.nb;
     const size_t nrows = (size_t)nb * nb * nb * 16;
     uint32_t uniq = 0, maxf = 0, lit_count = 0;
     for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < nrows; g += (size_t)gridDim.x * blockDim.x) {
@@ -111,21 +112,25 @@ __global__ __launch_bounds__(256) void k_normalize_inject(uint32_t *albedo, cons
             }
         }
         if (changed) *(uint4 *)&albedo[li] = make_uint4(w[0], w[1], w[2], w[3]);
-        const size_t dst = (b << 6) + (size_t)q * 4;  // brick_addr(x0, y, z) for this row
+        // the row's four elements are contiguous (level 0 is isotropic: one face)
+        char *row = pyr + L0.stage + (size_t)elem_index(L0, x0, y, z, 0) * 16;
         if (FMT == 1) {
-            float4 *o = (float4 *)rad0 + dst;
+            float4 *o = (float4 *)row;
 #pragma unroll
             for (int v = 0; v < 4; ++v) o[v] = make_float4(r[v][0], r[v][1], r[v][2], r[v][3]);
         } else {
-            uint32_t h[8];
+            // fp16 pairs: elements x0 .. x0+2 complete, the first half of x0+3
+            // (its second half is the next row's texel x0+4, written by that
+            // row's thread, or the border), and the second half of x0-1
+            uint2 h[4];
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                h[v * 2 + 0] = (uint32_t)f2h(r[v][0]) | ((uint32_t)f2h(r[v][1]) << 16);
-                h[v * 2 + 1] = (uint32_t)f2h(r[v][2]) | ((uint32_t)f2h(r[v][3]) << 16);
-            }
-            uint4 *o = (uint4 *)((uint2 *)rad0 + dst);
-            o[0] = make_uint4(h[0], h[1], h[2], h[3]);
-            o[1] = make_uint4(h[4], h[5], h[6], h[7]);
+            for (int v = 0; v < 4; ++v) h[v] = pack_h4(make_float4(r[v][0], r[v][1], r[v][2], r[v][3]));
+            uint4 *o = (uint4 *)row;
+            o[0] = make_uint4(h[0].x, h[0].y, h[1].x, h[1].y);
+            o[1] = make_uint4(h[1].x, h[1].y, h[2].x, h[2].y);
+            o[2] = make_uint4(h[2].x, h[2].y, h[3].x, h[3].y);
+            *(uint2 *)(o + 3) = h[3];
+            if (x0 > 0) *(uint2 *)(pyr + L0.stage + (size_t)elem_index(L0, x0 - 1, y, z, 0) * 16 + 8) = h[0];
         }
     }
     block_partial(uniq, maxf, lit_count, 0u, partials);
@@ -139,11 +144,11 @@ hipError_t launch_normalize_inject(uint32_t *albedo, const uint8_t *litmask, int
     if (blocks > (size_t)kStatPartials) blocks = kStatPartials;
     dim3 grid((unsigned)blocks);
     if (L0.fmt == 1)
-        hipLaunchKernelGGL(k_normalize_inject<1>, grid, dim3(256), 0, s, albedo, litmask, dim, L0.nb, opacity,
-                           L0.stage[0], partials);
+        hipLaunchKernelGGL(k_normalize_inject<1>, grid, dim3(256), 0, s, albedo, litmask, dim, L0, opacity, pyr.pyr,
+                           partials);
     else
-        hipLaunchKernelGGL(k_normalize_inject<0>, grid, dim3(256), 0, s, albedo, litmask, dim, L0.nb, opacity,
-                           L0.stage[0], partials);
+        hipLaunchKernelGGL(k_normalize_inject<0>, grid, dim3(256), 0, s, albedo, litmask, dim, L0, opacity, pyr.pyr,
+                           partials);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     // uniqueVoxels, maxFragmentsPerVoxel (transferVoxels.comp:39-45) and the lit count
@@ -152,29 +157,15 @@ hipError_t launch_normalize_inject(uint32_t *albedo, const uint8_t *litmask, int
 }
 
 // ---------------------------------------------------------------- fill holes
-__device__ __forceinline__ float4 load_vox(const char *base, int fmt, int nb, int x, int y, int z) {
-    const uint32_t a = brick_addr(x, y, z, nb);
-    if (fmt == 1) return ((const float4 *)base)[a];
-    const uint2 v = ((const uint2 *)base)[a];
-    return make_float4(h2f((uint16_t)(v.x & 0xffff)), h2f((uint16_t)(v.x >> 16)), h2f((uint16_t)(v.y & 0xffff)),
-                       h2f((uint16_t)(v.y >> 16)));
-}
-__device__ __forceinline__ void store_vox(char *base, int fmt, int nb, int x, int y, int z, float4 c) {
-    const uint32_t a = brick_addr(x, y, z, nb);
-    if (fmt == 1) {
-        ((float4 *)base)[a] = c;
-        return;
-    }
-    ((uint2 *)base)[a] = make_uint2((uint32_t)f2h(c.x) | ((uint32_t)f2h(c.y) << 16),
-                                    (uint32_t)f2h(c.z) | ((uint32_t)f2h(c.w) << 16));
-}
-
-__global__ __launch_bounds__(256) void k_fill_holes(const char *src, char *dst, int fmt, int nb, int dim) {
+// voxelFillHoles.comp:8-36 on level 0: src is the level in the pyramid, dst a
+// scratch copy with the same layout (copied back afterwards, as the reference
+// copies its temporary with glCopyImageSubData).
+__global__ __launch_bounds__(256) void k_fill_holes(const char *pyr, char *tmp, LevelInfo L0, int dim) {
     const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t n = (size_t)dim * dim * dim;
     if (g >= n) return;
     const int x = (int)(g % dim), y = (int)((g / dim) % dim), z = (int)(g / ((size_t)dim * dim));
-    float4 cur = load_vox(src, fmt, nb, x, y, z);
+    float4 cur = load_stage_texel(pyr, L0, x, y, z, 0);
     if (cur.w == 0.0f) {
         float count = 0.0f;
         for (int i = -1; i <= 1; ++i)
@@ -182,7 +173,7 @@ __global__ __launch_bounds__(256) void k_fill_holes(const char *src, char *dst, 
                 for (int k = -1; k <= 1; ++k) {
                     const int xx = x + i, yy = y + j, zz = z + k;
                     if (xx < 0 || yy < 0 || zz < 0 || xx >= dim || yy >= dim || zz >= dim) continue;
-                    const float4 v = load_vox(src, fmt, nb, xx, yy, zz);
+                    const float4 v = load_stage_texel(pyr, L0, xx, yy, zz, 0);
                     if (v.w > 0.0f) {
                         cur.x = cur.x + v.x;
                         cur.y = cur.y + v.y;
@@ -198,60 +189,53 @@ __global__ __launch_bounds__(256) void k_fill_holes(const char *src, char *dst, 
             cur.w = cur.w / count;
         }
     }
-    store_vox(dst, fmt, nb, x, y, z, cur);
+    LevelInfo T = L0;
+    T.stage = 0;
+    store_stage_texel(tmp, T, x, y, z, 0, cur);
 }
 
 hipError_t launch_fill_holes(const PyrDesc &pyr, void *tmp, hipStream_t s) {
     const LevelInfo &L0 = pyr.lv[0];
     const size_t n = (size_t)pyr.dim * pyr.dim * pyr.dim;
-    hipLaunchKernelGGL(k_fill_holes, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, L0.stage[0], (char *)tmp,
-                       L0.fmt, L0.nb, pyr.dim);
+    hipLaunchKernelGGL(k_fill_holes, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, pyr.pyr, (char *)tmp, L0,
+                       pyr.dim);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    const size_t bytes = (size_t)L0.nb * L0.nb * L0.nb * 64 * (L0.fmt == 1 ? 16 : 8);
-    return hipMemcpyAsync(L0.stage[0], tmp, bytes, hipMemcpyDeviceToDevice, s);  // glCopyImageSubData
+    const size_t bytes = (size_t)L0.sz * L0.nb * 16;
+    return hipMemcpyAsync(pyr.pyr + L0.stage, tmp, bytes, hipMemcpyDeviceToDevice, s);  // glCopyImageSubData
 }
 
 // ---------------------------------------------------------------- conversions
-__global__ void k_pyr_to_linear(const char *base, const float *scale, int fmt, int nb, int size, float4 *out) {
+__global__ void k_pyr_to_linear(const char *pyr, const float *scales, LevelInfo L, int face, float4 *out) {
     const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int size = L.size;
     const size_t n = (size_t)size * size * size;
     if (g >= n) return;
     const int x = (int)(g % size), y = (int)((g / size) % size), z = (int)(g / ((size_t)size * size));
-    const uint32_t a = brick_addr(x, y, z, nb);
-    float4 v;
-    if (fmt == 2) {
-        const uint32_t c = ((const uint32_t *)base)[a];
-        const float sc = scale[a >> 6];
-        v = make_float4(fp82f(c & 0xff) * sc, fp82f((c >> 8) & 0xff) * sc, fp82f((c >> 16) & 0xff) * sc,
-                        fp82f(c >> 24) * sc);
-    } else {
-        v = load_vox(base, fmt, nb, x, y, z);
-    }
-    out[g] = v;
+    out[g] = load_texel(pyr, scales, L, x, y, z, face);
 }
 
 hipError_t launch_pyr_to_linear(const PyrDesc &pyr, int level, int face, float4 *out, hipStream_t s) {
     const LevelInfo &L = pyr.lv[level];
     const size_t n = (size_t)L.size * L.size * L.size;
-    hipLaunchKernelGGL(k_pyr_to_linear, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, L.base[face],
-                       L.scale[face], L.fmt, L.nb, L.size, out);
+    hipLaunchKernelGGL(k_pyr_to_linear, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, pyr.pyr, pyr.scales, L,
+                       face < L.nfaces ? face : 0, out);
     return hipGetLastError();
 }
 
-__global__ void k_linear_to_pyr0(char *base, int fmt, int nb, int size, const float4 *in) {
+__global__ void k_linear_to_pyr0(char *pyr, LevelInfo L, const float4 *in) {
     const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int size = L.size;
     const size_t n = (size_t)size * size * size;
     if (g >= n) return;
     const int x = (int)(g % size), y = (int)((g / size) % size), z = (int)(g / ((size_t)size * size));
-    store_vox(base, fmt, nb, x, y, z, in[g]);
+    store_stage_texel(pyr, L, x, y, z, 0, in[g]);
 }
 
 hipError_t launch_linear_to_pyr0(const PyrDesc &pyr, const float4 *in, hipStream_t s) {
     const LevelInfo &L = pyr.lv[0];
     const size_t n = (size_t)L.size * L.size * L.size;
-    hipLaunchKernelGGL(k_linear_to_pyr0, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, L.stage[0], L.fmt,
-                       L.nb, L.size, in);
+    hipLaunchKernelGGL(k_linear_to_pyr0, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, pyr.pyr, L, in);
     return hipGetLastError();
 }
 

@@ -107,12 +107,101 @@ __device__ __forceinline__ float fp82f(uint32_t v) {
     return (v & 0x80u) ? -mag : mag;
 }
 
-// ---- brick layout of the radiance pyramid ---------------------------------
-// Every level is stored as 4x4x4 bricks, bricks in x-fastest order, voxels in
-// a brick x-fastest.  A level of edge s has nb = ceil(s/4) bricks per axis.
-__device__ __forceinline__ uint32_t brick_addr(int x, int y, int z, int nb) {
-    return ((uint32_t)(((z >> 2) * nb + (y >> 2)) * nb + (x >> 2)) << 6) | ((uint32_t)(z & 3) << 4) |
-           ((uint32_t)(y & 3) << 2) | (uint32_t)(x & 3);
+// ---- radiance pyramid addressing (layout in vct_internal.h) ----------------
+__device__ __forceinline__ uint32_t elem_index(const LevelInfo &L, int x, int y, int z, int f) {
+    return (((uint32_t)(x >> 2) * L.sx + (uint32_t)(x & 3)) + ((uint32_t)(y >> 2) * L.sy + (uint32_t)(y & 3) * 4u)) +
+           (((uint32_t)(z >> 2) * L.sz + (uint32_t)(z & 3) * 16u) + (uint32_t)f * 64u);
+}
+__device__ __forceinline__ uint2 pack_h4(float4 c) {
+    return make_uint2((uint32_t)f2h(c.x) | ((uint32_t)f2h(c.y) << 16), (uint32_t)f2h(c.z) | ((uint32_t)f2h(c.w) << 16));
+}
+__device__ __forceinline__ float4 unpack_h4(uint32_t a, uint32_t b) {
+    return make_float4(h2f((uint16_t)(a & 0xffffu)), h2f((uint16_t)(a >> 16)), h2f((uint16_t)(b & 0xffffu)),
+                       h2f((uint16_t)(b >> 16)));
+}
+__device__ __forceinline__ float4 decode_fp8x4(uint32_t c, float sc) {
+    return make_float4(fp82f(c & 0xffu) * sc, fp82f((c >> 8) & 0xffu) * sc, fp82f((c >> 16) & 0xffu) * sc,
+                       fp82f(c >> 24) * sc);
+}
+
+// Texel (x,y,z) of face f in a level's stage (fp32, or fp16 pairs).
+__device__ __forceinline__ float4 load_stage_texel(const char *pyr, const LevelInfo &L, int x, int y, int z, int f) {
+    const char *p = pyr + L.stage + (size_t)elem_index(L, x, y, z, f) * 16;
+    if (L.fmt == 1) return *(const float4 *)p;
+    const uint2 v = *(const uint2 *)p;
+    return unpack_h4(v.x, v.y);
+}
+// Writes texel (x,y,z) of face f into a level's stage: the fp32 element, or
+// both fp16 halves that hold it (the first half of its own element and the
+// second half of the element of x - 1).
+__device__ __forceinline__ void store_stage_texel(char *pyr, const LevelInfo &L, int x, int y, int z, int f,
+                                                  float4 c) {
+    char *p = pyr + L.stage + (size_t)elem_index(L, x, y, z, f) * 16;
+    if (L.fmt == 1) {
+        *(float4 *)p = c;
+        return;
+    }
+    const uint2 h = pack_h4(c);
+    *(uint2 *)p = h;
+    if (x > 0) *(uint2 *)(pyr + L.stage + (size_t)elem_index(L, x - 1, y, z, f) * 16 + 8) = h;
+}
+// Texel (x,y,z) of face f in a level's storage, decoded to float.
+__device__ __forceinline__ float4 load_texel(const char *pyr, const float *scales, const LevelInfo &L, int x, int y,
+                                             int z, int f) {
+    const uint32_t e = elem_index(L, x, y, z, f);
+    if (L.fmt == 2) return decode_fp8x4(*(const uint32_t *)(pyr + L.base + (size_t)e * 4), scales[L.scale + (e >> 6)]);
+    const char *p = pyr + L.base + (size_t)e * 16;
+    if (L.fmt == 1) return *(const float4 *)p;
+    const uint2 v = *(const uint2 *)p;
+    return unpack_h4(v.x, v.y);
+}
+
+// GL LINEAR inside face 0 of one level (border 0), FP32 weights and sums.
+// p in [0,1]^3 texture coordinates; s = the level's edge.
+__device__ __forceinline__ float4 level_linear(const char *pyr, const float *scales, const LevelInfo &L, const float p[3]) {
+    const int s = L.size;
+    int i0[3];
+    float f[3];
+    for (int k = 0; k < 3; ++k) {
+        const float u = p[k] * (float)s - 0.5f;
+        const float fl = __builtin_floorf(u);
+        i0[k] = (int)fl;
+        f[k] = u - fl;
+    }
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (L.fmt == 0) {  // fp16 pairs: one load per (y, z) corner pair
+        const bool xin = i0[0] >= 0 && i0[0] < s;  // i0 = s - 1 reads the zero border half
+        const bool xneg = i0[0] == -1;
+        if (!xin && !xneg) return acc;
+        const int cx = xneg ? 0 : i0[0];
+        const float wlo = xneg ? f[0] : 1.0f - f[0], whi = xneg ? 0.0f : f[0];
+        for (int c = 0; c < 4; ++c) {
+            const int dy = c & 1, dz = c >> 1;
+            const int y = i0[1] + dy, z = i0[2] + dz;
+            if ((unsigned)y >= (unsigned)s || (unsigned)z >= (unsigned)s) continue;
+            const float wyz = (dy ? f[1] : 1.0f - f[1]) * (dz ? f[2] : 1.0f - f[2]);
+            const uint4 v = *(const uint4 *)(pyr + L.base + (size_t)elem_index(L, cx, y, z, 0) * 16);
+            const float4 lo = unpack_h4(v.x, v.y), hi = unpack_h4(v.z, v.w);
+            const float a = wlo * wyz, b = whi * wyz;
+            acc.x += a * lo.x + b * hi.x;
+            acc.y += a * lo.y + b * hi.y;
+            acc.z += a * lo.z + b * hi.z;
+            acc.w += a * lo.w + b * hi.w;
+        }
+        return acc;
+    }
+    for (int c = 0; c < 8; ++c) {
+        const int dx = c & 1, dy = (c >> 1) & 1, dz = c >> 2;
+        const int x = i0[0] + dx, y = i0[1] + dy, z = i0[2] + dz;
+        if ((unsigned)x >= (unsigned)s || (unsigned)y >= (unsigned)s || (unsigned)z >= (unsigned)s) continue;
+        const float w = ((dx ? f[0] : 1.0f - f[0]) * (dy ? f[1] : 1.0f - f[1])) * (dz ? f[2] : 1.0f - f[2]);
+        const float4 t = load_texel(pyr, scales, L, x, y, z, 0);
+        acc.x += w * t.x;
+        acc.y += w * t.y;
+        acc.z += w * t.z;
+        acc.w += w * t.w;
+    }
+    return acc;
 }
 
 // GL LINEAR sample of an fp32 depth texture, CLAMP_TO_BORDER with border 1.0

@@ -26,22 +26,36 @@ struct DevLight {
     const float *shadowmap;  // device, shadow_size^2 fp32, row 0 at t = 0
 };
 
-// Radiance pyramid.  Every level is stored in 4x4x4 bricks (vct_device.h
-// brick_addr).  Level l has edge `size`, `nb` bricks per axis and `nfaces`
-// volumes (1, or 6 for anisotropic levels >= 1: +x,-x,+y,-y,+z,-z).
-// fmt: 0 = fp16x4 (8 B/voxel), 1 = fp32x4 (16 B), 2 = fp8 E4M3 x4 (4 B) with
-// one fp32 scale per brick.  `stage` is where the mip chain reads and writes
-// the level: the storage itself for fp16/fp32, an fp16 copy for fp8 levels
-// (the chain is filtered from fp16 values and fp8 levels are encoded after).
+// Radiance pyramid: one device allocation, addressed by 32-bit byte offsets
+// (so a sampler needs a single buffer descriptor for every level and face).
+// Level l has edge `size`, nb = ceil(size/4) bricks per axis and `nfaces`
+// volumes (1, or 6 for anisotropic levels >= 1: +x,-x,+y,-y,+z,-z).  A brick
+// holds 4x4x4 elements x-fastest and the faces of one brick are adjacent, so
+// the element of texel (x, y, z) in face f is
+//   (x>>2)*sx + (x&3) + (y>>2)*sy + (y&3)*4 + (z>>2)*sz + (z&3)*16 + 64*f
+// with sx = 64*nfaces, sy = nb*sx, sz = nb*sy (vct_device.h elem_index).
+// Element formats (fmt):
+//   0  fp16 pair, 16 B: texel (x,y,z) and texel (x+1,y,z) as fp16x4 each, so
+//      one aligned 16-byte load feeds both x-taps of a trilinear sample; the
+//      second half of the last texel in x is the border (0) and is never
+//      written.
+//   1  fp32x4, 16 B, one texel.
+//   2  fp8 E4M3 x4, 4 B, one texel, times the fp32 scale of its (brick, face):
+//      scales[scale + element / 64].
+// `stage` is where the mip chain reads and writes the level: `base` for fmt 0
+// and 1, an fp16-pair copy for fmt 2 (filtered in fp16, encoded after).
 struct LevelInfo {
-    char *base[6];
-    char *stage[6];
-    float *scale[6];
+    uint32_t base, stage;  // byte offsets into the pyramid allocation
+    uint32_t scale;        // float offset into the scale array (fmt 2)
+    uint32_t sx, sy, sz;   // brick strides in elements along x, y, z
     int size, nb, fmt, nfaces;
 };
 
 struct PyrDesc {
     int dim, nlevels, storage, aniso;
+    char *pyr;       // the allocation (all levels, faces and fp8 stage copies)
+    uint32_t bytes;  // its size (< 4 GiB)
+    float *scales;   // per-(brick, face) fp8 scales, or null
     LevelInfo lv[VCT_MAX_LEVELS];
     const LevelInfo *d_lv;  // device copy of lv[0..nlevels)
 };
