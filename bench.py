@@ -121,7 +121,8 @@ def learned_leg(args) -> dict:
     ctx.learned_prepare()
     ctx.sync()
     t1 = time.perf_counter()
-    loss = ctx.learned_train(steps=args.learned_steps, batch=4096, lr=1e-3, seed=1, reset=True)
+    loss = ctx.learned_train(steps=args.learned_steps, batch=4096, lr=1e-3, seed=1, reset=True,
+                             lr_final=args.learned_lr_final)
     ctx.sync()
     t2 = time.perf_counter()
     fs = app.frame_state(0)
@@ -169,6 +170,7 @@ def learned_leg(args) -> dict:
     return {
         "workload": "64^3 x 7 levels fp16 isotropic, 1920x1080 diffuse gather; network 129-64-64-64-3 BF16 MFMA",
         "train": {"points": args.learned_points, "steps": args.learned_steps, "batch": 4096, "lr": 1e-3,
+                  "lr_final": args.learned_lr_final,
                   "label_and_feature_s": round(t1 - t0, 3), "train_s": round(t2 - t1, 3),
                   "loss_first": float(loss[:10].mean()) if len(loss) else None,
                   "loss_last": float(loss[-100:].mean()) if len(loss) else None},
@@ -214,6 +216,9 @@ def main() -> int:
     ap.add_argument("--storage", type=int, default=-1, help="0 fp16, 1 fp32, 2 fp8 (levels >= 3)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-plain gather leg")
+    ap.add_argument("--only-learned", action="store_true", help="run only the learned-vs-plain gather leg")
+    ap.add_argument("--learned-lr-final", type=float, default=1e-5,
+                    help="learning rate at the last step (cosine decay from 1e-3)")
     ap.add_argument("--learned-steps", type=int, default=20000)
     ap.add_argument("--learned-points", type=int, default=1 << 20)
     ap.add_argument("--learned-threshold", type=float, default=1e-4,
@@ -223,6 +228,9 @@ def main() -> int:
     args = ap.parse_args()
     if args.copy_bytes:
         return copy_calibration(args.copy_bytes)
+    if args.only_learned:
+        print(json.dumps({"learned_vs_plain": learned_leg(args)}), flush=True)
+        return 0
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the pipeline runs on one GPU per context)", file=sys.stderr)

@@ -289,6 +289,9 @@ typedef struct vct_learned_params {
     float lr, beta1, beta2, eps; /* 1e-3, 0.9, 0.999, 1e-8                        */
     uint32_t seed;    /* minibatch sampling and weight initialisation             */
     int reset;        /* 1: He-uniform weights and zero Adam state before training */
+    float lr_final;   /* learning rate of the call's last step: the rate follows a
+                         cosine from lr to lr_final over the call's steps (0 or
+                         lr_final == lr: constant lr)                             */
 } vct_learned_params;
 
 /* Training surface points (G-buffer texels, host memory). */

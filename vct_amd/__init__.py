@@ -206,8 +206,8 @@ class Context:
     def learned_prepare(self):
         self.check(self.lib.vct_learned_prepare(self.h))
 
-    def learned_train(self, steps=20000, batch=4096, lr=1e-3, seed=1, reset=False) -> np.ndarray:
-        p = abi.LearnedParams(batch, steps, lr, 0.9, 0.999, 1e-8, seed, 1 if reset else 0)
+    def learned_train(self, steps=20000, batch=4096, lr=1e-3, seed=1, reset=False, lr_final=0.0) -> np.ndarray:
+        p = abi.LearnedParams(batch, steps, lr, 0.9, 0.999, 1e-8, seed, 1 if reset else 0, lr_final)
         loss = np.zeros(max(steps, 1), np.float32)
         self.check(self.lib.vct_learned_train(self.h, C.byref(p), _ptr(loss)))
         return loss[:steps]

@@ -108,7 +108,8 @@ class AppConfig(C.Structure):
 
 class LearnedParams(C.Structure):
     _fields_ = [("batch", C.c_int), ("steps", C.c_int), ("lr", C.c_float), ("beta1", C.c_float),
-                ("beta2", C.c_float), ("eps", C.c_float), ("seed", C.c_uint32), ("reset", C.c_int)]
+                ("beta2", C.c_float), ("eps", C.c_float), ("seed", C.c_uint32), ("reset", C.c_int),
+                ("lr_final", C.c_float)]
 
 
 GBUFFER_TEXEL_BYTES = 64

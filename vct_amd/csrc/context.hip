@@ -42,6 +42,7 @@ struct vct_ctx {
     // lights
     std::vector<vct_light> lights;
     std::vector<float *> d_shadow;
+    std::vector<float4 *> d_shadowq;  // footprint layout of each map (DevLight::shadowquad)
     std::vector<DevLight> hlights;
     DevLight *d_lights = nullptr;
     // voxel grids
@@ -312,6 +313,7 @@ void vct_destroy(vct_ctx *c) {
     dfree(c->d_model);
     dfree(c->d_nmat);
     for (auto &p : c->d_shadow) dfree(p);
+    for (auto &p : c->d_shadowq) dfree(p);
     dfree(c->d_lights);
     dfree(c->rs.rec);
     dfree(c->rs.attr);
@@ -516,21 +518,28 @@ int vct_set_lights(vct_ctx *c, const vct_light *lights, int n) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     // keep shadow maps whose size is unchanged (rebuilt per frame by vct_shadowmap)
     std::vector<float *> maps(n, nullptr);
+    std::vector<float4 *> quads(n, nullptr);
     for (int i = 0; i < n; ++i) {
         const bool want = lights[i].shadow_caster != 0;
         if (want && i < (int)c->d_shadow.size() && c->d_shadow[i] && i < (int)c->lights.size() &&
             c->lights[i].shadow_caster && c->lights[i].shadow_size == lights[i].shadow_size) {
             maps[i] = c->d_shadow[i];
+            quads[i] = c->d_shadowq[i];
             c->d_shadow[i] = nullptr;
+            c->d_shadowq[i] = nullptr;
         } else if (want) {
             const size_t N = lights[i].shadow_size;
             HIPCHK(c, dalloc(&maps[i], N * N * 4));
-            if (launch_fill_u32((uint32_t *)maps[i], 0x3f800000u, N * N, c->stream) != hipSuccess)
+            HIPCHK(c, dalloc(&quads[i], (N + 1) * (N + 1) * 16));
+            if (launch_fill_u32((uint32_t *)maps[i], 0x3f800000u, N * N, c->stream) != hipSuccess ||
+                launch_shadow_quads(maps[i], (int)N, quads[i], c->stream) != hipSuccess)
                 return fail(c, VCT_ERR_HIP, "shadow map clear failed");
         }
     }
     for (auto &p : c->d_shadow) dfree(p);
+    for (auto &p : c->d_shadowq) dfree(p);
     c->d_shadow = maps;
+    c->d_shadowq = quads;
     c->lights.assign(lights, lights + n);
     c->hlights.resize(n);
     for (int i = 0; i < n; ++i) {
@@ -549,6 +558,7 @@ int vct_set_lights(vct_ctx *c, const vct_light *lights, int n) {
         memcpy(d.ls, L.ls, 64);
         memcpy(d.ls_inverse, L.ls_inverse, 64);
         d.shadowmap = maps[i];
+        d.shadowquad = quads[i];
     }
     dfree(c->d_lights);
     if (n) {
@@ -899,7 +909,13 @@ int vct_upload(vct_ctx *c, int what, int level, const void *host, size_t bytes) 
     case VCT_BUF_ALBEDO: dst = c->d_grid_albedo; c->have_voxels = true; break;
     case VCT_BUF_NORMAL: dst = c->d_grid_normal; break;
     case VCT_BUF_LITMASK: dst = c->d_litmask; break;
-    case VCT_BUF_SHADOWMAP: dst = c->d_shadow[level]; break;
+    case VCT_BUF_SHADOWMAP: {
+        HIPCHK(c, hipMemcpyAsync(c->d_shadow[level], host, bytes, hipMemcpyHostToDevice, c->stream));
+        LAUNCHCHK(c, launch_shadow_quads(c->d_shadow[level], (int)c->lights[level].shadow_size, c->d_shadowq[level],
+                                         c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        return VCT_OK;
+    }
     case VCT_BUF_GBUFFER: dst = c->d_gbuf; c->have_gbuf = true; break;
     case VCT_BUF_RADIANCE: {
         if (level != 0) return fail(c, VCT_ERR_INVALID, "vct_upload: only radiance level 0 can be uploaded");
@@ -998,6 +1014,7 @@ int vct_learned_train(vct_ctx *c, const vct_learned_params *p, float *loss) {
         return fail(c, VCT_ERR_INVALID, "batch %d must be a multiple of 64 in [64, 65536]", p->batch);
     if (p->steps < 0 || p->steps > 1000000) return fail(c, VCT_ERR_INVALID, "steps %d out of range", p->steps);
     if (!(p->lr > 0.0f)) return fail(c, VCT_ERR_INVALID, "learning rate must be > 0");
+    if (!(p->lr_final >= 0.0f)) return fail(c, VCT_ERR_INVALID, "final learning rate must be >= 0");
     if (!c->have_prepared) return fail(c, VCT_ERR_STATE, "vct_learned_train: call vct_learned_prepare first");
     int rc = ensure_mlp(c);
     if (rc) return rc;
@@ -1033,9 +1050,14 @@ int vct_learned_train(vct_ctx *c, const vct_learned_params *p, float *loss) {
     }
     PassTimer pt(c, P_LEARNED);
     if (p->steps > 0) c->last_train_blocks = nblk;
-    for (int s = 0; s < p->steps; ++s)
-        LAUNCHCHK(c, launch_mlp_train_step(B, c->d_X, c->d_labels, c->npoints, p->batch, p->seed, ++c->adam_t, p->lr,
+    const bool cosine = p->lr_final > 0.0f && p->lr_final != p->lr && p->steps > 1;
+    for (int s = 0; s < p->steps; ++s) {
+        const float lr = cosine ? p->lr_final + 0.5f * (p->lr - p->lr_final) *
+                                                     (1.0f + cosf(3.14159265f * (float)s / (float)(p->steps - 1)))
+                                : p->lr;
+        LAUNCHCHK(c, launch_mlp_train_step(B, c->d_X, c->d_labels, c->npoints, p->batch, p->seed, ++c->adam_t, lr,
                                            p->beta1, p->beta2, p->eps, c->d_loss + s, c->stream));
+    }
     pt.stop();
     if (loss && p->steps) HIPCHK(c, hipMemcpyAsync(loss, c->d_loss, (size_t)p->steps * 4, hipMemcpyDeviceToHost,
                                                    c->stream));

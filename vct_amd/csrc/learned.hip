@@ -31,11 +31,13 @@ namespace vctk {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ uint16_t f2bf(float f) {  // round to nearest even
-    const uint32_t u = __float_as_uint(f);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);  // quiet NaN
-    return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+// FP32 -> BF16, round to nearest even (NaN stays NaN): v_cvt_pk_bf16_f32
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t bf2pack(float a, float b) {
+    const bf16x2 v = {(__bf16)a, (__bf16)b};
+    return __builtin_bit_cast(uint32_t, v);
 }
+__device__ __forceinline__ uint16_t f2bf(float f) { return (uint16_t)(bf2pack(f, 0.0f) & 0xffffu); }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
 // ---------------------------------------------------------------- sampling
@@ -143,43 +145,129 @@ __global__ __launch_bounds__(64) void k_label64(const vct_gbuffer_texel *__restr
 //   [108, 111) gather axis normalize(B)
 //   [111, 129) for c in x,y,z: for f in 1,2,4: sin(pi f p_c), cos(pi f p_c)
 //   [129, 160) zero
+// The 27 samples sit one level-2 texel apart, so they share one set of
+// trilinear weights and together read the 4x4x4 texel block around the
+// point: the block is read once (16 rows of 4 texels) and interpolated
+// separably, x, then y, then z.
+
+
+// Texels x0 .. x0+3 of row (y, z) of face 0 of an isotropic level, 0 outside
+// (x0 >= -2).
+template <int FMT>
+__device__ __forceinline__ void row4(const PyrView &pv, const LevelInfo &L, int x0, int y, int z, float4 t[4]) {
+    const int s = L.size;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) t[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if ((unsigned)y >= (unsigned)s || (unsigned)z >= (unsigned)s) return;
+    if (FMT == 0) {  // fp16 pairs (x, x+1); the pair of the last texel ends in the zero border
+        const char *base = pv.pyr + L.base;
+        if (x0 >= 0) {
+            const uint4 v = *(const uint4 *)(base + (size_t)elem_index(L, x0, y, z, 0) * 16);
+            t[0] = unpack_h4(v.x, v.y);
+            t[1] = unpack_h4(v.z, v.w);
+        } else if (x0 == -1) {
+            const uint2 v = *(const uint2 *)(base + (size_t)elem_index(L, 0, y, z, 0) * 16);
+            t[1] = unpack_h4(v.x, v.y);
+        }
+        const int xb = x0 + 2;
+        if (xb >= 0 && xb < s) {
+            const uint4 v = *(const uint4 *)(base + (size_t)elem_index(L, xb, y, z, 0) * 16);
+            t[2] = unpack_h4(v.x, v.y);
+            t[3] = unpack_h4(v.z, v.w);
+        }
+        return;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int x = x0 + k;
+        if ((unsigned)x < (unsigned)s) t[k] = load_texel(pv.pyr, pv.scales, L, x, y, z, 0);
+    }
+}
+
+__device__ __forceinline__ float4 lerp4(float4 a, float4 b, float f) {
+    const float g = 1.0f - f;
+    return make_float4(a.x * g + b.x * f, a.y * g + b.y * f, a.z * g + b.z * f, a.w * g + b.w * f);
+}
+
+// Writes the 160 inputs of one G-buffer texel (g0 = position, g3 = B) as 80
+// packed BF16 pairs to row32 (global or LDS).  FMT: level 2 stored as fp16
+// pairs (0) or any other element format (1).
+template <int FMT>
 __device__ __forceinline__ void make_features(const float4 g0, const float4 g3, const PyrView &pv, const Extent &ext,
-                                              uint16_t *row) {
-    const int dim = pv.dim;
+                                              uint32_t *row32) {
     float p[3];
     const float gp[3] = {g0.x, g0.y, g0.z};
     for (int k = 0; k < 3; ++k) p[k] = ((gp[k] - ext.center[k]) - ext.vmin[k]) / (ext.vmax[k] - ext.vmin[k]);
-    const int s2 = (dim >> 2) > 0 ? (dim >> 2) : 1;
-    const float step = 1.0f / (float)s2;
     const LevelInfo &L2 = pv.lv[2];
-    for (int i = 0; i < 27; ++i) {
-        const int ix = i % 3 - 1, iy = (i / 3) % 3 - 1, iz = i / 9 - 1;
-        const float q[3] = {p[0] + ix * step, p[1] + iy * step, p[2] + iz * step};
-        const float4 v = level_linear(pv.pyr, pv.scales, L2, q);
-        row[i * 4 + 0] = f2bf(v.x);
-        row[i * 4 + 1] = f2bf(v.y);
-        row[i * 4 + 2] = f2bf(v.z);
-        row[i * 4 + 3] = f2bf(v.w);
+    const int s2 = L2.size;
+    int i0[3];
+    float f[3];
+    for (int k = 0; k < 3; ++k) {
+        const float u = p[k] * (float)s2 - 0.5f;
+        const float fl = floorf(u);
+        i0[k] = (int)fl;
+        f[k] = u - fl;
     }
-    float b[3] = {g3.x, g3.y, g3.z};
-    const float bl = sqrtf(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
-    for (int k = 0; k < 3; ++k) row[108 + k] = f2bf(bl > 0.f ? b[k] / bl : 0.f);
-    int o = 111;
-    for (int c = 0; c < 3; ++c)
-        for (int f = 1; f <= 4; f <<= 1) {
-            const float ang = 3.14159265358979f * (float)f * p[c];
-            row[o++] = f2bf(sinf(ang));
-            row[o++] = f2bf(cosf(ang));
+    float4 yprev[9], ycur[9];  // y-interpolated values of the previous / current z slice, [oy][ox]
+    for (int zs = 0; zs < 4; ++zs) {
+        for (int ys = 0; ys < 4; ++ys) {
+            float4 t[4];
+            row4<FMT>(pv, L2, i0[0] - 1, i0[1] - 1 + ys, i0[2] - 1 + zs, t);
+#pragma unroll
+            for (int ox = 0; ox < 3; ++ox) {
+                const float4 xv = lerp4(t[ox], t[ox + 1], f[0]);
+                // ycur[oy] = (1 - fy) * row oy + fy * row oy+1
+                if (ys <= 2) {
+                    const float g = 1.0f - f[1];
+                    ycur[ys * 3 + ox] = make_float4(xv.x * g, xv.y * g, xv.z * g, xv.w * g);
+                }
+                if (ys >= 1) {
+                    float4 &yv = ycur[(ys - 1) * 3 + ox];
+                    yv.x += xv.x * f[1];
+                    yv.y += xv.y * f[1];
+                    yv.z += xv.z * f[1];
+                    yv.w += xv.w * f[1];
+                }
+            }
         }
-    for (; o < kMlpIn; ++o) row[o] = 0;
+        if (zs >= 1) {  // stencils with iz = zs - 2 from slices zs - 1 and zs
+            const int base = (zs - 1) * 9;
+#pragma unroll
+            for (int j = 0; j < 9; ++j) {
+                const float4 v = lerp4(yprev[j], ycur[j], f[2]);
+                row32[(base + j) * 2 + 0] = bf2pack(v.x, v.y);
+                row32[(base + j) * 2 + 1] = bf2pack(v.z, v.w);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 9; ++j) yprev[j] = ycur[j];
+    }
+    const float b[3] = {g3.x, g3.y, g3.z};
+    const float bl = sqrtf(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+    float e[22];  // inputs 108 .. 129, then zero padding to an even count
+    for (int k = 0; k < 3; ++k) e[k] = bl > 0.f ? b[k] / bl : 0.f;
+    int o = 3;
+    // sin/cos(pi f p) = v_sin/v_cos of f p / 2 revolutions: absolute error ~1e-6, far below the BF16 step
+    for (int c = 0; c < 3; ++c)
+        for (int fq = 1; fq <= 4; fq <<= 1) {
+            const float rev = 0.5f * (float)fq * p[c];
+            e[o++] = __builtin_amdgcn_sinf(rev);
+            e[o++] = __builtin_amdgcn_cosf(rev);
+        }
+    e[21] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 11; ++k) row32[54 + k] = bf2pack(e[2 * k], e[2 * k + 1]);
+#pragma unroll
+    for (int k = 65; k < kMlpIn / 2; ++k) row32[k] = 0u;
 }
 
+template <int FMT>
 __global__ __launch_bounds__(256) void k_features(const vct_gbuffer_texel *__restrict__ pts, int npts, PyrView pv,
                                                   Extent ext, uint16_t *X) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= npts) return;
     const float4 *g = (const float4 *)&pts[i];
-    make_features(g[0], g[3], pv, ext, X + (size_t)i * kMlpIn);
+    make_features<FMT>(g[0], g[3], pv, ext, (uint32_t *)(X + (size_t)i * kMlpIn));
 }
 
 // ---------------------------------------------------------------- MFMA tiles
@@ -423,16 +511,64 @@ __global__ __launch_bounds__(256) void k_adam(float *master, float *m1, float *m
 }
 
 // ---------------------------------------------------------------- inference
+// One wavefront per 8x8 tile (64 pixels = four 16-pixel MFMA tiles).  Each
+// lane builds its pixel's inputs in LDS.  The layers run transposed,
+// D^T = W X^T: the weights are the A operand (loaded once per 16 output units
+// and K step, from L1/L2, and applied to all four pixel tiles) and the pixels
+// the B operand, so a lane's four results are four consecutive units of one
+// pixel and leave as one 8-byte LDS store.  Row strides of 168 and 72 BF16 (84
+// and 36 dwords) make a tile's 16 row reads hit disjoint LDS banks.
+constexpr int kXs = 168;  // input row stride (>= kMlpIn)
+constexpr int kAs = 72;   // hidden activation row stride (>= 64)
+
 struct InferSmem {
-    uint16_t x[64 * kMlpIn];  // inputs, then hidden activations (ping)
-    uint16_t b[64 * 64];      // hidden activations (pong)
+    uint16_t x[64 * kXs];  // inputs, then hidden activations (stride kAs)
     float y[64 * 4];
 };
 
+// acc[nt][mt] += W[16 nt .. +15][K] * X[16 mt .. +15][K]^T over K = 32 * nks
+template <int NT>
+__device__ __forceinline__ void mlp_layer_t(f32x4 (&acc)[NT][4], const uint16_t *W, int ldw, const uint16_t *X,
+                                            int ldx, int nks) {
+    const int lane = threadIdx.x & 63;
+    const int r = lane & 15, kq = (lane >> 4) * 8;
+    for (int ks = 0; ks < nks; ++ks) {
+        bf16x8 b[4];
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) b[mt] = *(const bf16x8 *)(X + (mt * 16 + r) * ldx + ks * 32 + kq);
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+            const bf16x8 a = *(const bf16x8 *)(W + (size_t)(nt * 16 + r) * ldw + ks * 32 + kq);
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt)
+                acc[nt][mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[mt], acc[nt][mt], 0, 0, 0);
+        }
+    }
+}
+
+// bias + ReLU -> BF16 hidden activations [pixel][unit], stride kAs
+__device__ __forceinline__ void store_hidden_t(const f32x4 (&acc)[4][4], const float *bias, uint16_t *out) {
+    const int lane = threadIdx.x & 63;
+    const int q = (lane >> 4) * 4;
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+        const int n0 = nt * 16 + q;
+        const float b0 = bias[n0], b1 = bias[n0 + 1], b2 = bias[n0 + 2], b3 = bias[n0 + 3];
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) {
+            const int m = mt * 16 + (lane & 15);
+            const f32x4 v = acc[nt][mt];
+            const uint2 h = make_uint2(bf2pack(fmaxf(v[0] + b0, 0.f), fmaxf(v[1] + b1, 0.f)),
+                                       bf2pack(fmaxf(v[2] + b2, 0.f), fmaxf(v[3] + b3, 0.f)));
+            *(uint2 *)(out + m * kAs + n0) = h;
+        }
+    }
+}
+
+template <int FMT>
 __global__ __launch_bounds__(64) void k_learned(MlpDev W, const vct_gbuffer_texel *__restrict__ gbuf, int width,
-                                                int height, PyrView pv, Extent ext,
-                                                float threshold, float4 *out, uint32_t *tile_list,
-                                                uint32_t *tile_count) {
+                                                int height, PyrView pv, Extent ext, float threshold, float4 *out,
+                                                uint32_t *tile_list, uint32_t *tile_count) {
     __shared__ __align__(16) InferSmem S;
     const int lane = threadIdx.x;
     const int tiles_x = (width + 7) / 8;
@@ -440,59 +576,47 @@ __global__ __launch_bounds__(64) void k_learned(MlpDev W, const vct_gbuffer_texe
     const int px = (tile % tiles_x) * 8 + (lane & 7), py = (tile / tiles_x) * 8 + (lane >> 3);
     const bool inside = px < width && py < height;
     bool covered = false;
+    uint32_t *row32 = (uint32_t *)(S.x + lane * kXs);
     if (inside) {
         const float4 *g = (const float4 *)&gbuf[(size_t)py * width + px];
         const float4 g0 = g[0], g2 = g[2], g3 = g[3];
         covered = __float_as_uint(g2.w) != 0u;
-        if (covered) make_features(g0, g3, pv, ext, S.x + lane * kMlpIn);
+        if (covered) make_features<FMT>(g0, g3, pv, ext, row32);
     }
     if (!covered)
-        for (int k = 0; k < kMlpIn; ++k) S.x[lane * kMlpIn + k] = 0;
+        for (int k = 0; k < kMlpIn / 2; ++k) row32[k] = 0u;
     __syncthreads();
-    // layer 1: x -> b
-    for (int mt = 0; mt < 4; ++mt)
-        for (int nt = 0; nt < 4; ++nt) {
-            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-            acc = mfma_tile(S.x, kMlpIn, W.w1, kMlpIn, mt * 16, nt * 16, kMlpIn, acc);
-            const int n = nt * 16 + (lane & 15), mb = mt * 16 + (lane >> 4) * 4;
-            const float *b1 = bias_ptr(W.master, 0);
-            for (int i = 0; i < 4; ++i) {
-                const float v = acc[i] + b1[n];
-                S.b[(mb + i) * 64 + n] = f2bf(v > 0.f ? v : 0.f);
-            }
-        }
+    f32x4 acc[4][4];
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) acc[nt][mt] = f32x4{0.f, 0.f, 0.f, 0.f};
+    mlp_layer_t<4>(acc, W.w1, kMlpIn, S.x, kXs, kMlpIn / 32);
+    __syncthreads();  // every lane's input reads are done before the activations overwrite them
+    store_hidden_t(acc, bias_ptr(W.master, 0), S.x);
     __syncthreads();
-    // layer 2: b -> x (as a 64-wide activation buffer), layer 3: x -> b
     for (int layer = 1; layer <= 2; ++layer) {
-        const uint16_t *src = layer == 1 ? S.b : S.x;
-        uint16_t *dst = layer == 1 ? S.x : S.b;
-        const uint16_t *w = layer == 1 ? W.w2 : W.w3;
-        const float *bias = bias_ptr(W.master, layer);
-        f32x4 res[4][4];
-        for (int mt = 0; mt < 4; ++mt)
-            for (int nt = 0; nt < 4; ++nt) {
-                f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-                res[mt][nt] = mfma_tile(src, 64, w, 64, mt * 16, nt * 16, 64, acc);
-            }
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt) acc[nt][mt] = f32x4{0.f, 0.f, 0.f, 0.f};
+        mlp_layer_t<4>(acc, layer == 1 ? W.w2 : W.w3, 64, S.x, kAs, 2);
         __syncthreads();
-        for (int mt = 0; mt < 4; ++mt)
-            for (int nt = 0; nt < 4; ++nt) {
-                const int n = nt * 16 + (lane & 15), mb = mt * 16 + (lane >> 4) * 4;
-                for (int i = 0; i < 4; ++i) {
-                    const float v = res[mt][nt][i] + bias[n];
-                    dst[(mb + i) * 64 + n] = f2bf(v > 0.f ? v : 0.f);
-                }
-            }
+        store_hidden_t(acc, bias_ptr(W.master, layer), S.x);
         __syncthreads();
     }
-    // layer 4: b -> y
-    for (int mt = 0; mt < 4; ++mt) {
-        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-        acc = mfma_tile(S.b, 64, W.w4, 64, mt * 16, 0, 64, acc);
-        const int n = lane & 15, mb = mt * 16 + (lane >> 4) * 4;
+    f32x4 o[1][4];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) o[0][mt] = f32x4{0.f, 0.f, 0.f, 0.f};
+    mlp_layer_t<1>(o, W.w4, 64, S.x, kAs, 2);
+    if (lane < 16) {  // units 0..3 of pixels 16 mt + lane; units 0..2 are the outputs
         const float *b4 = bias_ptr(W.master, 3);
-        if (n < 3)
-            for (int i = 0; i < 4; ++i) S.y[(mb + i) * 4 + n] = (acc[i] + b4[n]) * W.out_scale;
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) {
+            const int m = mt * 16 + lane;
+#pragma unroll
+            for (int n = 0; n < 3; ++n) S.y[m * 4 + n] = (o[0][mt][n] + b4[n]) * W.out_scale;
+        }
     }
     __syncthreads();
     const float r = covered ? S.y[lane * 4 + 0] : 0.f, gch = covered ? S.y[lane * 4 + 1] : 0.f,
@@ -534,7 +658,10 @@ hipError_t launch_label64(const PyrDesc &pyr, const vct_gbuffer_texel *pts, int 
 
 hipError_t launch_features(const PyrDesc &pyr, const vct_gbuffer_texel *pts, int npts, Extent ext, uint16_t *X,
                            hipStream_t s) {
-    hipLaunchKernelGGL(k_features, dim3((npts + 255) / 256), dim3(256), 0, s, pts, npts, pyr_view(pyr), ext, X);
+    if (pyr.lv[2].fmt == 0)
+        hipLaunchKernelGGL(k_features<0>, dim3((npts + 255) / 256), dim3(256), 0, s, pts, npts, pyr_view(pyr), ext, X);
+    else
+        hipLaunchKernelGGL(k_features<1>, dim3((npts + 255) / 256), dim3(256), 0, s, pts, npts, pyr_view(pyr), ext, X);
     return hipGetLastError();
 }
 
@@ -580,8 +707,12 @@ hipError_t launch_learned(const MlpBuffers &B, const PyrDesc &pyr, const vct_gbu
     const int tiles = ((width + 7) / 8) * ((height + 7) / 8);
     hipError_t e = launch_fill_u32(tile_count, 0u, 1, s);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_learned, dim3(tiles), dim3(64), 0, s, mlp_dev(B), gbuf, width, height, pyr_view(pyr), ext,
-                       threshold, out, tile_list, tile_count);
+    if (pyr.lv[2].fmt == 0)
+        hipLaunchKernelGGL(k_learned<0>, dim3(tiles), dim3(64), 0, s, mlp_dev(B), gbuf, width, height, pyr_view(pyr),
+                           ext, threshold, out, tile_list, tile_count);
+    else
+        hipLaunchKernelGGL(k_learned<1>, dim3(tiles), dim3(64), 0, s, mlp_dev(B), gbuf, width, height, pyr_view(pyr),
+                           ext, threshold, out, tile_list, tile_count);
     return hipGetLastError();
 }
 

@@ -71,10 +71,9 @@ struct ShadowFrag {
     uint32_t *depth;
     int N;
     __device__ void item(int slot, int px, int py, const Edges &e) {
-        const int64_t cx = (int64_t)px * 256 + 128, cy = (int64_t)py * 256 + 128;
-        if (!edge_inside(e, cx, cy)) return;
+        if (!edge_inside(e, px, py, 128, 128)) return;
         float l[3];
-        edge_bary(e, cx, cy, l);
+        edge_bary(e, px, py, 128, 128, l);
         float4 a = attr[slot];
         float z = interp3(l, a.x, a.y, a.z);
         z = z + 0.0f;
@@ -99,6 +98,25 @@ hipError_t launch_shadowmap(const DevScene &sc, const DevLight *L, float *depth,
     ShadowFrag f{rs.attr, (uint32_t *)depth, N};
     hipLaunchKernelGGL(k_raster_items<ShadowFrag>, dim3(kRasterBlocks), dim3(256), 0, s, rs.rec, rs.offs, rs.chunk,
                        rs.chunk_cap, sc.ntri, f);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    return launch_shadow_quads(depth, N, const_cast<float4 *>(L->shadowquad), s);
+}
+
+__global__ __launch_bounds__(256) void k_shadow_quads(const float *__restrict__ sm, int N, float4 *q) {
+    const int n1 = N + 1;
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n1 * n1) return;
+    const int i = idx % n1 - 1, j = idx / n1 - 1;
+    auto tex = [&](int x, int y) -> float {
+        return (x >= 0 && x < N && y >= 0 && y < N) ? sm[(size_t)y * N + x] : 1.0f;
+    };
+    q[idx] = make_float4(tex(i, j), tex(i + 1, j), tex(i, j + 1), tex(i + 1, j + 1));
+}
+
+hipError_t launch_shadow_quads(const float *depth, int N, float4 *quads, hipStream_t s) {
+    if (!quads) return hipSuccess;
+    const int n = (N + 1) * (N + 1);
+    hipLaunchKernelGGL(k_shadow_quads, dim3((n + 255) / 256), dim3(256), 0, s, depth, N, quads);
     return hipGetLastError();
 }
 
@@ -172,10 +190,9 @@ struct GbufFrag {
     unsigned long long *vis;
     int W;
     __device__ void item(int slot, int px, int py, const Edges &e) {
-        const int64_t cx = (int64_t)px * 256 + 128, cy = (int64_t)py * 256 + 128;
-        if (!edge_inside(e, cx, cy)) return;
+        if (!edge_inside(e, px, py, 128, 128)) return;
         float l[3];
-        edge_bary(e, cx, cy, l);
+        edge_bary(e, px, py, 128, 128, l);
         float4 a = attr[slot];
         float z = interp3(l, a.x, a.y, a.z);
         z = z + 0.0f;

@@ -24,9 +24,13 @@ __device__ __forceinline__ float clamp_win(float v) {
 
 // Record layout (4 x int4):
 //  r0 = (A0, B0, A1, B1)
-//  r1 = (A2, B2, x0 | y0 << 16, bw | bias_bits << 16)
-//  r2 = (C0.lo, C0.hi, C1.lo, C1.hi)
-//  r3 = (C2.lo, C2.hi, float area, user)
+//  r1 = (A2, B2, x0 | y0 << 16, bw | bias_bits << 16 | small << 19)
+//  r2 = (C0.lo, C0.hi, C1.lo, C1.hi)      or, small: (E0_0, E0_1, E0_2, 0)
+//  r3 = (C2.lo, C2.hi, float area, user)  or, small: (0, 0, float area, user)
+// "small" triangles (bbox <= 63 px per side, |A|, |B| < 2^14) keep each edge
+// function relative to the bbox corner (x0, y0): E_i = E0_i + A_i dx + B_i dy
+// with dx, dy < 2^14 in 1/256 px, every term below 2^29 -- the same integers as
+// the 64-bit form, evaluated with 24-bit multiply-adds.
 // Returns the number of bbox pixels (0 = no fragments).
 __device__ __forceinline__ uint64_t tri_setup(const float win[3][2], int W, int H, int cull, int user,
                                               int4 *rec) {
@@ -79,20 +83,38 @@ __device__ __forceinline__ uint64_t tri_setup(const float win[3][2], int W, int 
     if (y1 > H - 1) y1 = H - 1;
     if (x0 > x1 || y0 > y1) return 0;
     uint32_t bw = (uint32_t)(x1 - x0 + 1), bh = (uint32_t)(y1 - y0 + 1);
+    int64_t E0[3];
+    bool small = bw <= 63 && bh <= 63;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        E0[i] = A[i] * (x0 << 8) + B[i] * (y0 << 8) + C[i];
+        const int64_t lim = (int64_t)1 << 14;
+        small = small && A[i] > -lim && A[i] < lim && B[i] > -lim && B[i] < lim && E0[i] > -((int64_t)1 << 29) &&
+                E0[i] < ((int64_t)1 << 29);
+    }
     rec[0] = make_int4((int)A[0], (int)B[0], (int)A[1], (int)B[1]);
     rec[1] = make_int4((int)A[2], (int)B[2], (int)((uint32_t)x0 | ((uint32_t)y0 << 16)),
-                       (int)(bw | (bias_bits << 16)));
-    rec[2] = make_int4((int)(uint32_t)(uint64_t)C[0], (int)(uint32_t)((uint64_t)C[0] >> 32),
-                       (int)(uint32_t)(uint64_t)C[1], (int)(uint32_t)((uint64_t)C[1] >> 32));
-    rec[3] = make_int4((int)(uint32_t)(uint64_t)C[2], (int)(uint32_t)((uint64_t)C[2] >> 32),
-                       __float_as_int((float)area), user);
+                       (int)(bw | (bias_bits << 16) | ((small ? 1u : 0u) << 19)));
+    if (small) {
+        rec[2] = make_int4((int)E0[0], (int)E0[1], (int)E0[2], 0);
+        rec[3] = make_int4(0, 0, __float_as_int((float)area), user);
+    } else {
+        rec[2] = make_int4((int)(uint32_t)(uint64_t)C[0], (int)(uint32_t)((uint64_t)C[0] >> 32),
+                           (int)(uint32_t)(uint64_t)C[1], (int)(uint32_t)((uint64_t)C[1] >> 32));
+        rec[3] = make_int4((int)(uint32_t)(uint64_t)C[2], (int)(uint32_t)((uint64_t)C[2] >> 32),
+                           __float_as_int((float)area), user);
+    }
     return (uint64_t)bw * bh;
 }
 
 struct Edges {
-    int64_t A[3], B[3], C[3], bias[3];
+    int32_t A[3], B[3];  // |A|, |B| <= 2^28 for every window-clamped triangle
+    int64_t C[3];        // large triangles
+    int32_t E0[3];       // small triangles: edge values at the bbox corner
+    int32_t bias[3];
     float area;
     int x0, y0, bw, user;
+    bool small;
 };
 
 __device__ __forceinline__ void load_edges(const int4 *__restrict__ recs, size_t slot, Edges &e) {
@@ -106,10 +128,14 @@ __device__ __forceinline__ void load_edges(const int4 *__restrict__ recs, size_t
     e.x0 = r1.z & 0xffff;
     e.y0 = (int)((uint32_t)r1.z >> 16);
     e.bw = r1.w & 0xffff;
-    uint32_t bb = ((uint32_t)r1.w >> 16) & 7u;
+    const uint32_t bb = ((uint32_t)r1.w >> 16) & 7u;
+    e.small = (((uint32_t)r1.w >> 19) & 1u) != 0u;
     e.bias[0] = (bb & 1u) ? -1 : 0;
     e.bias[1] = (bb & 2u) ? -1 : 0;
     e.bias[2] = (bb & 4u) ? -1 : 0;
+    e.E0[0] = r2.x;
+    e.E0[1] = r2.y;
+    e.E0[2] = r2.z;
     e.C[0] = (int64_t)(((uint64_t)(uint32_t)r2.y << 32) | (uint32_t)r2.x);
     e.C[1] = (int64_t)(((uint64_t)(uint32_t)r2.w << 32) | (uint32_t)r2.z);
     e.C[2] = (int64_t)(((uint64_t)(uint32_t)r3.y << 32) | (uint32_t)r3.x);
@@ -117,16 +143,39 @@ __device__ __forceinline__ void load_edges(const int4 *__restrict__ recs, size_t
     e.user = r3.w;
 }
 
-__device__ __forceinline__ bool edge_inside(const Edges &e, int64_t sx, int64_t sy) {
+// Edge i at sample (px * 256 + ox, py * 256 + oy), ox, oy in [0, 256).
+__device__ __forceinline__ int64_t edge_value(const Edges &e, int i, int px, int py, int ox, int oy) {
+    if (e.small) {
+        const int dx = ((px - e.x0) << 8) + ox, dy = ((py - e.y0) << 8) + oy;
+        return (int64_t)(e.E0[i] + __mul24(e.A[i], dx) + __mul24(e.B[i], dy));
+    }
+    const int64_t sx = (int64_t)px * 256 + ox, sy = (int64_t)py * 256 + oy;
+    return (int64_t)e.A[i] * sx + (int64_t)e.B[i] * sy + e.C[i];
+}
+
+__device__ __forceinline__ bool edge_inside(const Edges &e, int px, int py, int ox, int oy) {
+    if (e.small) {
+        const int dx = ((px - e.x0) << 8) + ox, dy = ((py - e.y0) << 8) + oy;
+        bool in = true;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) in = in && (e.E0[i] + __mul24(e.A[i], dx) + __mul24(e.B[i], dy) + e.bias[i] >= 0);
+        return in;
+    }
     bool in = true;
 #pragma unroll
-    for (int i = 0; i < 3; ++i) in = in && (e.A[i] * sx + e.B[i] * sy + e.C[i] + e.bias[i] >= 0);
+    for (int i = 0; i < 3; ++i) in = in && (edge_value(e, i, px, py, ox, oy) + e.bias[i] >= 0);
     return in;
 }
 
-__device__ __forceinline__ void edge_bary(const Edges &e, int64_t sx, int64_t sy, float l[3]) {
+__device__ __forceinline__ void edge_bary(const Edges &e, int px, int py, int ox, int oy, float l[3]) {
+    if (e.small) {
+        const int dx = ((px - e.x0) << 8) + ox, dy = ((py - e.y0) << 8) + oy;
 #pragma unroll
-    for (int i = 0; i < 3; ++i) l[i] = (float)(e.A[i] * sx + e.B[i] * sy + e.C[i]) / e.area;
+        for (int i = 0; i < 3; ++i) l[i] = (float)(e.E0[i] + __mul24(e.A[i], dx) + __mul24(e.B[i], dy)) / e.area;
+        return;
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) l[i] = (float)edge_value(e, i, px, py, ox, oy) / e.area;
 }
 
 // 4x MSAA sample offsets in 1/256 px: (0.375,0.125) (0.875,0.375) (0.125,0.625) (0.625,0.875)

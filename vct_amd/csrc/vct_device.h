@@ -225,6 +225,25 @@ __device__ __forceinline__ float shadow_bilinear(const float *__restrict__ sm, i
     return ((w00 * t00 + w10 * t10) + w01 * t01) + w11 * t11;
 }
 
+// shadow_bilinear on the footprint layout of DevLight::shadowquad: the same
+// coordinates, weights and sum order, the four texels from one aligned load.
+__device__ __forceinline__ float shadow_bilinear_quad(const float4 *__restrict__ q, int N, float s, float t, int ox,
+                                                      int oy) {
+    float u = s * (float)N;
+    u = u + (float)ox;
+    u = u - 0.5f;
+    float v = t * (float)N;
+    v = v + (float)oy;
+    v = v - 0.5f;
+    const float fu = __builtin_floorf(u), fv = __builtin_floorf(v);
+    const int i0 = (int)fu, j0 = (int)fv;
+    const float a = u - fu, b = v - fv;
+    float4 T = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+    if (i0 >= -1 && i0 <= N - 1 && j0 >= -1 && j0 <= N - 1) T = q[(size_t)(j0 + 1) * (N + 1) + (i0 + 1)];
+    const float w00 = (1.0f - a) * (1.0f - b), w10 = a * (1.0f - b), w01 = (1.0f - a) * b, w11 = a * b;
+    return ((w00 * T.x + w10 * T.y) + w01 * T.z) + w11 * T.w;
+}
+
 // ---- wave helpers ----------------------------------------------------------
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 

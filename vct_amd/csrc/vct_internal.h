@@ -24,6 +24,10 @@ struct DevLight {
     float ls[16];
     float ls_inverse[16];
     const float *shadowmap;  // device, shadow_size^2 fp32, row 0 at t = 0
+    // the same map as bilinear footprints: element (i, j), i, j in [-1, N-1],
+    // at (j + 1) * (N + 1) + (i + 1) holds texels (i, j), (i+1, j), (i, j+1),
+    // (i+1, j+1), border texels 1.0 -- one aligned 16-byte load per PCF tap
+    const float4 *shadowquad;
 };
 
 // Radiance pyramid: one device allocation, addressed by 32-bit byte offsets
@@ -127,6 +131,8 @@ size_t raster_scan_tmp_bytes(size_t nslots);
 // raster.hip
 hipError_t launch_shadowmap(const DevScene &sc, const DevLight *dlight_host, float *depth, RasterScratch &rs,
                             hipStream_t s);
+// rebuilds DevLight::shadowquad from the depth map
+hipError_t launch_shadow_quads(const float *depth, int N, float4 *quads, hipStream_t s);
 hipError_t launch_gbuffer(const DevScene &sc, const vct_camera &cam, uint64_t *vis, vct_gbuffer_texel *gbuf,
                           RasterScratch &rs, hipStream_t s);
 

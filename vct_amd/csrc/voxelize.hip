@@ -81,17 +81,17 @@ __global__ __launch_bounds__(256) void k_vox_setup(DevScene sc, VoxMats vm, int4
 }
 
 // calcShadowFactor (shaders/voxelize.frag:160-184)
-__device__ __forceinline__ float calc_shadow_factor(const float lsp[4], const float *sm, int N) {
+__device__ __forceinline__ float calc_shadow_factor(const float lsp[4], const float4 *sq, int N) {
     float sh[3];
     for (int k = 0; k < 3; ++k) sh[k] = (lsp[k] / lsp[3] + 1.0f) * 0.5f;
     const float frag = sh[2] - 0.01f;
     if (frag > 1.0f) return 0.0f;
     float f = 0.0f;
-    if (frag > shadow_bilinear(sm, N, sh[0], sh[1], 0, 0)) f += 1.0f;
-    if (frag > shadow_bilinear(sm, N, sh[0], sh[1], 1, 0)) f += 1.0f;
-    if (frag > shadow_bilinear(sm, N, sh[0], sh[1], 0, 1)) f += 1.0f;
-    if (frag > shadow_bilinear(sm, N, sh[0], sh[1], -1, 0)) f += 1.0f;
-    if (frag > shadow_bilinear(sm, N, sh[0], sh[1], 0, -1)) f += 1.0f;
+    if (frag > shadow_bilinear_quad(sq, N, sh[0], sh[1], 0, 0)) f += 1.0f;
+    if (frag > shadow_bilinear_quad(sq, N, sh[0], sh[1], 1, 0)) f += 1.0f;
+    if (frag > shadow_bilinear_quad(sq, N, sh[0], sh[1], 0, 1)) f += 1.0f;
+    if (frag > shadow_bilinear_quad(sq, N, sh[0], sh[1], -1, 0)) f += 1.0f;
+    if (frag > shadow_bilinear_quad(sq, N, sh[0], sh[1], 0, -1)) f += 1.0f;
     f = f / 5.0f;
     return f;
 }
@@ -116,21 +116,19 @@ struct VoxFrag {
         bool cov = false;
         const int ns = msaa ? 4 : 1;
         for (int k = 0; k < ns && !cov; ++k) {
-            const int64_t sx = (int64_t)px * 256 + (msaa ? msaa_x(k) : 128);
-            const int64_t sy = (int64_t)py * 256 + (msaa ? msaa_y(k) : 128);
-            if (!edge_inside(e, sx, sy)) continue;
+            const int ox = msaa ? msaa_x(k) : 128, oy = msaa ? msaa_y(k) : 128;
+            if (!edge_inside(e, px, py, ox, oy)) continue;
             if (zclip) {
                 float ls[3];
-                edge_bary(e, sx, sy, ls);
+                edge_bary(e, px, py, ox, oy, ls);
                 const float z = interp3(ls, nz0, nz1, nz2);
                 if (!(z >= -1.0f && z <= 1.0f)) continue;
             }
             cov = true;
         }
         if (!cov) return;
-        const int64_t cx = (int64_t)px * 256 + 128, cy = (int64_t)py * 256 + 128;
         float l[3];
-        edge_bary(e, cx, cy, l);
+        edge_bary(e, px, py, 128, 128, l);
         const float4 a3 = a[3], a4 = a[4], a5 = a[5], a6 = a[6];
         float f[28];
         f[0] = a0.x; f[1] = a0.y; f[2] = a0.z; f[3] = a0.w;
@@ -176,10 +174,10 @@ struct VoxFrag {
                     const float ndl = __builtin_fmaxf(0.0f, dot3(nn, ndn));
                     for (int k = 0; k < 3; ++k) lit[k] = ((color[k] * L.color[k]) * L.intensity) * ndl;
                 }
-                if (L.shadow_caster && L.shadowmap) {
+                if (L.shadow_caster && L.shadowquad) {
                     float w4[4] = {wp[0], wp[1], wp[2], 1.0f}, lsp[4];
                     mat4_vec4(L.ls, w4, lsp);
-                    const float sf = 1.0f - calc_shadow_factor(lsp, L.shadowmap, (int)L.shadow_size);
+                    const float sf = 1.0f - calc_shadow_factor(lsp, L.shadowquad, (int)L.shadow_size);
                     for (int k = 0; k < 3; ++k) lit[k] = lit[k] * sf;
                 }
                 for (int k = 0; k < 3; ++k) fin[k] = fin[k] + lit[k];

@@ -5,8 +5,11 @@
 // the C ABI of include/vct.h.  The scenes replace the reference's Sponza actor
 // (Application.cpp:93-136): triangle soups with per-triangle RGBA8 albedo
 // (the 1x1 diffuse texture a material would bind), face or smooth normals and
-// per-triangle tangents from the Mesh.cpp:174-197 rule with uv (0,0), (1,0),
-// (0,1), i.e. tangent = normalize(v1 - v0).
+// per-triangle tangents.  The tangent is what the Mesh.cpp:174-197 rule gives
+// for the uv layout the scene implies: box faces carry one uv square per face
+// (u along the face's first in-plane axis), so both triangles of a face share
+// the tangent of that axis; sphere triangles use the canonical in-plane axis
+// normalize((0,1,0) x n) (normalize((1,0,0) x n) near the poles).
 #include <algorithm>
 #include <array>
 #include <cmath>
@@ -59,16 +62,15 @@ uint32_t random_albedo(Rng &r) {
     return v | 0xff000000u;
 }
 
-void push_tri(Mesh &m, const double p[3][3], const double n[3][3], uint32_t albedo, uint32_t obj) {
+void push_tri(Mesh &m, const double p[3][3], const double n[3][3], const double tangent[3], uint32_t albedo,
+              uint32_t obj) {
     for (int v = 0; v < 3; ++v)
         for (int k = 0; k < 3; ++k) {
             m.pos.push_back((float)p[v][k]);
             m.nrm.push_back((float)n[v][k]);
         }
-    float e[3] = {(float)p[1][0] - (float)p[0][0], (float)p[1][1] - (float)p[0][1], (float)p[1][2] - (float)p[0][2]};
-    float t[3];
-    nrm3(e, t);
-    m.tan.insert(m.tan.end(), t, t + 3);
+    const double l = std::sqrt(tangent[0] * tangent[0] + tangent[1] * tangent[1] + tangent[2] * tangent[2]);
+    for (int k = 0; k < 3; ++k) m.tan.push_back((float)(tangent[k] / l));
     m.albedo.push_back(albedo);
     m.obj.push_back(obj);
 }
@@ -90,6 +92,8 @@ void add_box(Mesh &m, const double h[3], bool inward, uint32_t obj, Rng &r) {
             bool flip = (sgn < 0) != inward;
             double nrm[3] = {0, 0, 0};
             nrm[a] = inward ? -sgn : sgn;
+            double tng[3] = {0, 0, 0};  // uv (cu+1)/2, (cv+1)/2: dp/du along axis u
+            tng[u] = 1.0;
             const int q[2][3] = {{0, 1, 2}, {0, 2, 3}};
             for (int t = 0; t < 2; ++t) {
                 double p[3][3], n[3][3];
@@ -98,7 +102,7 @@ void add_box(Mesh &m, const double h[3], bool inward, uint32_t obj, Rng &r) {
                     std::memcpy(p[k], corner[idx], sizeof(p[k]));
                     std::memcpy(n[k], nrm, sizeof(n[k]));
                 }
-                push_tri(m, p, n, random_albedo(r), obj);
+                push_tri(m, p, n, tng, random_albedo(r), obj);
             }
         }
 }
@@ -165,7 +169,23 @@ void add_icosphere(Mesh &m, double radius, int subdiv, uint32_t obj, Rng &r) {
                 std::swap(n[1][c], n[2][c]);
             }
         }
-        push_tri(m, p, n, random_albedo(r), obj);
+        double fn[3] = {0, 0, 0};
+        for (int k = 0; k < 3; ++k)
+            for (int c = 0; c < 3; ++c) fn[c] += n[k][c];
+        const double fl = std::sqrt(fn[0] * fn[0] + fn[1] * fn[1] + fn[2] * fn[2]);
+        for (auto &c : fn) c /= fl;
+        // (0,1,0) x n, or (1,0,0) x n within ~25 degrees of the poles
+        double tng[3];
+        if (std::fabs(fn[1]) < 0.9) {
+            tng[0] = fn[2];
+            tng[1] = 0.0;
+            tng[2] = -fn[0];
+        } else {
+            tng[0] = 0.0;
+            tng[1] = -fn[2];
+            tng[2] = fn[1];
+        }
+        push_tri(m, p, n, tng, random_albedo(r), obj);
     }
 }
 
