@@ -145,6 +145,25 @@ def learned_leg(args) -> dict:
 
     e_plain, e_learned = errors(1e30)
     _, e_learned_fb = errors(args.learned_threshold)
+    view = None
+    if args.learned_view_fraction > 0:
+        # view-adaptive variant: part of the training points are texels of
+        # this camera's own 1080p G-buffer (a seeded subset), the rest
+        # area-uniform surface points as above
+        ctx.gbuffer(cam)
+        gb = ctx.download(abi.VCT_BUF_GBUFFER, 0, np.float32).reshape(-1, 16)
+        cov = np.flatnonzero(gb[:, 11].view(np.uint32) == 1)
+        rng = np.random.default_rng(11)
+        nv = int(args.learned_points * args.learned_view_fraction)
+        mix = np.concatenate([pts[: args.learned_points - nv], gb[rng.choice(cov, nv, replace=len(cov) < nv)]])
+        ctx.learned_set_points(np.ascontiguousarray(mix))
+        ctx.learned_prepare()
+        ctx.learned_train(steps=args.learned_steps, batch=4096, lr=1e-3, seed=1, reset=True,
+                          lr_final=args.learned_lr_final)
+        _, ev = errors(1e30)
+        _, ev_fb = errors(args.learned_threshold)
+        view = {"view_fraction": args.learned_view_fraction, "mean_rel_err_vs_64cone": round(ev, 5),
+                "with_fallback": round(ev_fb, 5)}
     ctx.gbuffer(cam)
     reps = 10
     for _ in range(2):
@@ -183,6 +202,7 @@ def learned_leg(args) -> dict:
         "tiles": tiles,
         "mean_rel_err_vs_64cone": {"plain_6cone": round(e_plain, 5), "learned": round(e_learned, 5),
                                    "learned_with_fallback": round(e_learned_fb, 5)},
+        "view_adaptive_training": view,
     }
 
 
@@ -217,8 +237,10 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-plain gather leg")
     ap.add_argument("--only-learned", action="store_true", help="run only the learned-vs-plain gather leg")
-    ap.add_argument("--learned-lr-final", type=float, default=1e-5,
-                    help="learning rate at the last step (cosine decay from 1e-3)")
+    ap.add_argument("--learned-lr-final", type=float, default=1e-3,
+                    help="learning rate at the last step (cosine decay from 1e-3; 1e-3 = constant)")
+    ap.add_argument("--learned-view-fraction", type=float, default=0.5,
+                    help="also train a variant with this fraction of points from the camera's G-buffer (0: skip)")
     ap.add_argument("--learned-steps", type=int, default=20000)
     ap.add_argument("--learned-points", type=int, default=1 << 20)
     ap.add_argument("--learned-threshold", type=float, default=1e-4,

@@ -202,8 +202,9 @@ def test_inference_matches_reference_network_and_fallback(prepared):
     Xb = mlp_ref.bf16_round(X)[:, :129]
     y, _ = mlp_ref.forward(params, Xb, bf16=True)
     assert rel_err(out[idx, :3], y * w[-1]) < 5e-3
-    # a threshold of -1 sends every tile to the plain tracer
+    # a threshold of -1 sends every tile to the plain tracer (the same gather
+    # code inlined into the learned kernel: equal up to FP contraction choices)
     nfb = ctx.trace_learned(tr, -1.0)
     assert nfb == ((cam.width + 7) // 8) * ((cam.height + 7) // 8)
     out = ctx.download(abi.VCT_BUF_INDIRECT, 0, np.float32).reshape(-1, 4)
-    assert np.array_equal(out.view(np.uint32), plain.view(np.uint32))
+    assert np.abs(out - plain).max() <= 1e-6 * max(np.abs(plain).max(), 1e-30)

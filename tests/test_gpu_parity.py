@@ -99,6 +99,7 @@ def test_trace_within_tolerance(scene1):
     ref, _ = ob.trace(cap["gbuf"], cap["levels"], tp)
     got = cap["indirect"]
     e = rel_err(got[:, :3], ref[:, :3])
+    print(f"indirect image mean relative error vs the FP32 oracle: {e:.3e}")
     assert e < 1e-3, f"indirect image mean relative error {e}"
     assert rel_err(got[:, 3], ref[:, 3]) < 1e-3
     assert ref[:, :3].sum() > 0  # the scene is lit: the comparison is not vacuous

@@ -64,9 +64,11 @@ def test_scene2_gbuffer_and_trace(scene2_small):
     tp = ob.trace_params(fs["trace"], g.dim, g.levels, tuple(g.center), tuple(g.vmin), tuple(g.vmax),
                          mesh["rough"] is not None, 1)
     ind, spec = ob.trace(cap["gbuf"], levels, tp, aniso_faces=cap["faces"])
-    assert rel_err(cap["indirect"][:, :3], ind[:, :3]) < 1e-3
+    ei, es = rel_err(cap["indirect"][:, :3], ind[:, :3]), rel_err(cap["specular"], spec)
+    print(f"scene 2 mean relative error vs the FP32 oracle: indirect {ei:.3e}, specular {es:.3e}")
+    assert ei < 1e-3
     assert rel_err(cap["indirect"][:, 3], ind[:, 3]) < 1e-3
-    assert rel_err(cap["specular"], spec) < 1e-3
+    assert es < 1e-3
     assert ind[:, :3].sum() > 0 and spec[:, :3].sum() > 0
 
 

@@ -78,8 +78,7 @@ struct vct_ctx {
     float *d_loss = nullptr;
     int loss_cap = 0, last_train_blocks = 0;
     int adam_t = 0;  // Adam steps taken since the weights were initialised or uploaded
-    uint32_t *d_tiles = nullptr, *d_tile_count = nullptr;
-    int tiles_cap = 0;
+    uint32_t *d_tile_count = nullptr;  // fallback tiles of the last learned gather
     float4 *d_reference = nullptr;
     // timing
     bool timing = true;
@@ -341,7 +340,6 @@ void vct_destroy(vct_ctx *c) {
     dfree(c->mlp.w4t);
     dfree(c->mlp.partials);
     dfree(c->d_loss);
-    dfree(c->d_tiles);
     dfree(c->d_tile_count);
     dfree(c->d_reference);
     dfree(c->d_stats);
@@ -831,8 +829,15 @@ int vct_download(vct_ctx *c, int what, int level, void *host, size_t bytes) {
     const void *src = nullptr;
     int l, f;
     switch (what) {
-    case VCT_BUF_ALBEDO: src = c->d_grid_albedo; break;
-    case VCT_BUF_NORMAL: src = c->d_grid_normal; break;
+    case VCT_BUF_ALBEDO:
+    case VCT_BUF_NORMAL: {  // bricked on the device, linear x-fastest for the caller
+        int rc = ensure_stage(c, need);
+        if (rc) return rc;
+        LAUNCHCHK(c, launch_grid_convert(what == VCT_BUF_ALBEDO ? c->d_grid_albedo : c->d_grid_normal,
+                                         (uint32_t *)c->d_stage, c->desc.dim, true, c->stream));
+        src = c->d_stage;
+        break;
+    }
     case VCT_BUF_LITMASK: src = c->d_litmask; break;
     case VCT_BUF_SHADOWMAP: src = c->d_shadow[level]; break;
     case VCT_BUF_GBUFFER: src = c->d_gbuf; break;
@@ -906,8 +911,18 @@ int vct_upload(vct_ctx *c, int what, int level, const void *host, size_t bytes) 
         c->have_weights = true;
         return VCT_OK;
     }
-    case VCT_BUF_ALBEDO: dst = c->d_grid_albedo; c->have_voxels = true; break;
-    case VCT_BUF_NORMAL: dst = c->d_grid_normal; break;
+    case VCT_BUF_ALBEDO:
+    case VCT_BUF_NORMAL: {
+        int rc = ensure_stage(c, need);
+        if (rc) return rc;
+        HIPCHK(c, hipMemcpyAsync(c->d_stage, host, bytes, hipMemcpyHostToDevice, c->stream));
+        LAUNCHCHK(c, launch_grid_convert((const uint32_t *)c->d_stage,
+                                         what == VCT_BUF_ALBEDO ? c->d_grid_albedo : c->d_grid_normal, c->desc.dim,
+                                         false, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (what == VCT_BUF_ALBEDO) c->have_voxels = true;
+        return VCT_OK;
+    }
     case VCT_BUF_LITMASK: dst = c->d_litmask; break;
     case VCT_BUF_SHADOWMAP: {
         HIPCHK(c, hipMemcpyAsync(c->d_shadow[level], host, bytes, hipMemcpyHostToDevice, c->stream));
@@ -1099,18 +1114,11 @@ int vct_trace_learned(vct_ctx *c, const vct_trace_params *p, float thr, int *fal
     if (rc) return rc;
     if (!c->have_gbuf) return fail(c, VCT_ERR_STATE, "vct_trace_learned: no G-buffer");
     if (!c->have_weights) return fail(c, VCT_ERR_STATE, "vct_trace_learned: no trained weights");
-    const int tiles = ((c->gw + 7) / 8) * ((c->gh + 7) / 8);
-    if (tiles > c->tiles_cap) {
-        dfree(c->d_tiles);
-        HIPCHK(c, dalloc(&c->d_tiles, (size_t)tiles * 4));
-        c->tiles_cap = tiles;
-    }
     if (!c->d_tile_count) HIPCHK(c, dalloc(&c->d_tile_count, 4));
     TraceLaunch t = make_trace_launch(c, p);
+    t.enable_reflections = 0;
     PassTimer pt(c, P_LEARNED);
-    LAUNCHCHK(c, launch_learned(c->mlp, c->pyr, c->d_gbuf, c->gw, c->gh, t.ext, thr, c->d_ind, c->d_tiles,
-                                c->d_tile_count, c->stream));
-    LAUNCHCHK(c, launch_trace_tiles(c->pyr, t, c->d_tiles, c->d_tile_count, tiles, c->stream));
+    LAUNCHCHK(c, launch_learned(c->mlp, c->pyr, t, thr, c->d_tile_count, c->stream));
     pt.stop();
     if (fallback) {
         uint32_t n = 0;

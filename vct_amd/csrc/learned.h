@@ -33,9 +33,11 @@ hipError_t launch_mlp_train_step(MlpBuffers &B, const uint16_t *X, const float4 
                                  uint32_t seed, int step, float lr, float b1, float b2, float eps, float *loss_out,
                                  hipStream_t s);
 size_t mlp_train_smem_bytes();
-hipError_t launch_learned(const MlpBuffers &B, const PyrDesc &pyr, const vct_gbuffer_texel *gbuf, int width,
-                          int height, Extent ext, float threshold, float4 *out, uint32_t *tile_list,
-                          uint32_t *tile_count, hipStream_t s);
+// Learned diffuse gather over the G-buffer of t into t.out_ind; tiles whose
+// predicted-luminance variance exceeds threshold are traced with the 6-cone
+// gather instead and counted in *fallback_count.
+hipError_t launch_learned(const MlpBuffers &B, const PyrDesc &pyr, const TraceLaunch &t, float threshold,
+                          uint32_t *fallback_count, hipStream_t s);
 hipError_t launch_emit_operands(MlpBuffers &B, hipStream_t s);
 
 }  // namespace vctk

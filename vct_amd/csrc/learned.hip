@@ -23,8 +23,8 @@
 // holds rows 4*(l/16) .. +3 of column l%16.
 #include <hip/hip_runtime.h>
 
-#include "vct_device.h"
 #include "learned.h"
+#include "trace_core.h"
 
 namespace vctk {
 
@@ -565,11 +565,14 @@ __device__ __forceinline__ void store_hidden_t(const f32x4 (&acc)[4][4], const f
     }
 }
 
-template <int FMT>
-__global__ __launch_bounds__(64) void k_learned(MlpDev W, const vct_gbuffer_texel *__restrict__ gbuf, int width,
-                                                int height, PyrView pv, Extent ext, float threshold, float4 *out,
-                                                uint32_t *tile_list, uint32_t *tile_count) {
+template <int MODE>
+__global__ __launch_bounds__(64) void k_learned(MlpDev W, TraceLaunch t, PyrArgs pa, PyrView pv, float threshold,
+                                                uint32_t *fallback_count) {
+    constexpr int FMT = MODE == 1 ? 1 : 0;  // level 2 is fp32 in fp32 storage, fp16 pairs otherwise
     __shared__ __align__(16) InferSmem S;
+    const vct_gbuffer_texel *__restrict__ gbuf = t.gbuf;
+    const int width = t.width, height = t.height;
+    const Extent &ext = t.ext;
     const int lane = threadIdx.x;
     const int tiles_x = (width + 7) / 8;
     const int tile = blockIdx.x;
@@ -631,10 +634,12 @@ __global__ __launch_bounds__(64) void k_learned(MlpDev W, const vct_gbuffer_texe
     }
     const float mean = cnt > 0.f ? s1 / cnt : 0.f;
     const float var = cnt > 0.f ? s2 / cnt - mean * mean : 0.f;
-    if (inside) out[(size_t)py * width + px] = make_float4(r, gch, bch, covered ? 1.0f : 0.0f);
-    if (lane == 0 && var > threshold) {
-        const uint32_t slot = atomicAdd(tile_count, 1u);
-        tile_list[slot] = (uint32_t)tile;
+    // the butterfly sums are identical on every lane, so the choice is wave-uniform
+    if (var > threshold) {
+        if (lane == 0) atomicAdd(fallback_count, 1u);
+        if (inside) trace_pixel<MODE, false>(t, pa, pyr_rsrc(pa), px, py, false);
+    } else if (inside) {
+        t.out_ind[(size_t)py * width + px] = make_float4(r, gch, bch, covered ? 1.0f : 0.0f);
     }
 }
 
@@ -701,18 +706,21 @@ hipError_t launch_mlp_train_step(MlpBuffers &B, const uint16_t *X, const float4 
 
 size_t mlp_train_smem_bytes() { return sizeof(TrainSmem); }
 
-hipError_t launch_learned(const MlpBuffers &B, const PyrDesc &pyr, const vct_gbuffer_texel *gbuf, int width,
-                          int height, Extent ext, float threshold, float4 *out, uint32_t *tile_list,
-                          uint32_t *tile_count, hipStream_t s) {
-    const int tiles = ((width + 7) / 8) * ((height + 7) / 8);
-    hipError_t e = launch_fill_u32(tile_count, 0u, 1, s);
+hipError_t launch_learned(const MlpBuffers &B, const PyrDesc &pyr, const TraceLaunch &t, float threshold,
+                          uint32_t *fallback_count, hipStream_t s) {
+    const int tiles = ((t.width + 7) / 8) * ((t.height + 7) / 8);
+    hipError_t e = launch_fill_u32(fallback_count, 0u, 1, s);
     if (e != hipSuccess) return e;
-    if (pyr.lv[2].fmt == 0)
-        hipLaunchKernelGGL(k_learned<0>, dim3(tiles), dim3(64), 0, s, mlp_dev(B), gbuf, width, height, pyr_view(pyr),
-                           ext, threshold, out, tile_list, tile_count);
+    const int mode = pyr.storage == VCT_STORE_FP32 ? 1 : (pyr.storage == VCT_STORE_FP8 ? 2 : 0);
+    const MlpDev W = mlp_dev(B);
+    const PyrArgs pa = pyr_args(pyr);
+    const PyrView pv = pyr_view(pyr);
+    if (mode == 0)
+        hipLaunchKernelGGL(k_learned<0>, dim3(tiles), dim3(64), 0, s, W, t, pa, pv, threshold, fallback_count);
+    else if (mode == 1)
+        hipLaunchKernelGGL(k_learned<1>, dim3(tiles), dim3(64), 0, s, W, t, pa, pv, threshold, fallback_count);
     else
-        hipLaunchKernelGGL(k_learned<1>, dim3(tiles), dim3(64), 0, s, mlp_dev(B), gbuf, width, height, pyr_view(pyr),
-                           ext, threshold, out, tile_list, tile_count);
+        hipLaunchKernelGGL(k_learned<2>, dim3(tiles), dim3(64), 0, s, W, t, pa, pv, threshold, fallback_count);
     return hipGetLastError();
 }
 

@@ -82,8 +82,9 @@ __global__ __launch_bounds__(256) void k_normalize_inject(uint32_t *albedo, cons
         const int q = (int)(g & 15);
         const int bx = (int)(b % nb), by = (int)((b / nb) % nb), bz = (int)(b / ((size_t)nb * nb));
         const int x0 = bx * 4, y = by * 4 + (q & 3), z = bz * 4 + (q >> 2);
-        const size_t li = ((size_t)z * dim + y) * dim + x0;
-        const uint4 a = *(const uint4 *)&albedo[li];
+        const size_t li = ((size_t)z * dim + y) * dim + x0;  // the lit mask is linear
+        const size_t gi = (b << 6) + (size_t)q * 4;           // the albedo grid is bricked (grid_index)
+        const uint4 a = *(const uint4 *)&albedo[gi];
         const uint32_t lw = *(const uint32_t *)&litmask[li];  // 4 flag bytes
         uint32_t w[4] = {a.x, a.y, a.z, a.w};
         float r[4][4];
@@ -111,7 +112,7 @@ __global__ __launch_bounds__(256) void k_normalize_inject(uint32_t *albedo, cons
                 r[v][0] = r[v][1] = r[v][2] = r[v][3] = 0.0f;
             }
         }
-        if (changed) *(uint4 *)&albedo[li] = make_uint4(w[0], w[1], w[2], w[3]);
+        if (changed) *(uint4 *)&albedo[gi] = make_uint4(w[0], w[1], w[2], w[3]);
         // the row's four elements are contiguous (level 0 is isotropic: one face)
         char *row = pyr + L0.stage + (size_t)elem_index(L0, x0, y, z, 0) * 16;
         if (FMT == 1) {
@@ -236,6 +237,24 @@ hipError_t launch_linear_to_pyr0(const PyrDesc &pyr, const float4 *in, hipStream
     const LevelInfo &L = pyr.lv[0];
     const size_t n = (size_t)L.size * L.size * L.size;
     hipLaunchKernelGGL(k_linear_to_pyr0, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, pyr.pyr, L, in);
+    return hipGetLastError();
+}
+
+// bricked voxel grid <-> linear x-fastest (downloads and uploads)
+__global__ void k_grid_convert(const uint32_t *src, uint32_t *dst, int dim, int to_linear) {
+    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t n = (size_t)dim * dim * dim;
+    if (g >= n) return;
+    const int x = (int)(g % dim), y = (int)((g / dim) % dim), z = (int)(g / ((size_t)dim * dim));
+    const uint32_t b = grid_index(x, y, z, dim >> 2);
+    if (to_linear) dst[g] = src[b];
+    else dst[b] = src[g];
+}
+
+hipError_t launch_grid_convert(const uint32_t *src, uint32_t *dst, int dim, bool to_linear, hipStream_t s) {
+    const size_t n = (size_t)dim * dim * dim;
+    hipLaunchKernelGGL(k_grid_convert, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, dst, dim,
+                       to_linear ? 1 : 0);
     return hipGetLastError();
 }
 

@@ -107,6 +107,15 @@ __device__ __forceinline__ float fp82f(uint32_t v) {
     return (v & 0x80u) ? -mag : mag;
 }
 
+// ---- voxel grid addressing ---------------------------------------------------
+// The albedo and normal grids (dim^3 packed RGBA8) are stored in 4x4x4 bricks
+// of 256 bytes, bricks and the voxels inside a brick x-fastest (nb = dim / 4):
+// the voxels one fragment neighbourhood touches share a few cache lines.
+__device__ __forceinline__ uint32_t grid_index(int x, int y, int z, int nb) {
+    return ((uint32_t)(((z >> 2) * nb + (y >> 2)) * nb + (x >> 2)) << 6) | ((uint32_t)(z & 3) << 4) |
+           ((uint32_t)(y & 3) << 2) | (uint32_t)(x & 3);
+}
+
 // ---- radiance pyramid addressing (layout in vct_internal.h) ----------------
 __device__ __forceinline__ uint32_t elem_index(const LevelInfo &L, int x, int y, int z, int f) {
     return (((uint32_t)(x >> 2) * L.sx + (uint32_t)(x & 3)) + ((uint32_t)(y >> 2) * L.sy + (uint32_t)(y & 3) * 4u)) +

@@ -6,7 +6,7 @@
 #include <stddef.h>
 #include <stdint.h>
 
-#include "../../include/vct.h"
+#include "vct.h"
 
 #define VCT_MAX_LEVELS 10
 
@@ -153,8 +153,9 @@ hipError_t launch_mip(const PyrDesc &pyr, hipStream_t s);
 
 // trace.hip
 hipError_t launch_trace(const PyrDesc &pyr, const TraceLaunch &t, hipStream_t s);
-hipError_t launch_trace_tiles(const PyrDesc &pyr, const TraceLaunch &t, const uint32_t *list, const uint32_t *count,
-                              int max_tiles, hipStream_t s);
+
+// bricked albedo/normal grid <-> linear x-fastest
+hipError_t launch_grid_convert(const uint32_t *src, uint32_t *dst, int dim, bool to_linear, hipStream_t s);
 
 // conversions for download (radiance level -> linear float4)
 hipError_t launch_pyr_to_linear(const PyrDesc &pyr, int level, int face, float4 *out, hipStream_t s);

@@ -204,7 +204,7 @@ struct VoxFrag {
         int idx[3];
         for (int k = 0; k < 3; ++k) idx[k] = (int)((float)dim * un[k]);
         if (idx[0] < 0 || idx[1] < 0 || idx[2] < 0 || idx[0] >= dim || idx[1] >= dim || idx[2] >= dim) return;
-        const size_t vi = ((size_t)idx[2] * dim + idx[1]) * dim + idx[0];
+        const uint32_t vi = grid_index(idx[0], idx[1], idx[2], dim >> 2);
         atomicMax(&albedo[vi], pack_unorm4x8(color[0], color[1], color[2], 1.0f));
         atomicMax(&normal[vi], pack_unorm4x8(ne0, ne1, ne2, 1.0f));
     }
