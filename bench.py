@@ -9,8 +9,9 @@ re-voxelized, re-injected, re-filtered and re-traced.
 
 One JSON line.  `value` = ms per frame of the four metric passes (voxelize +
 normalize/inject + mip + trace) from HIP events recorded on the context
-stream around each pass; the shadow-map and G-buffer passes each frame also
-needs are reported beside it (`passes`, `frame_ms_all_passes`).
+stream around each pass; the shadow-map, G-buffer and final composition
+(direct light + indirect + post-process) passes each frame also needs are
+reported beside it (`passes`, `frame_ms_all_passes`).
 `roofline` is for the dominant kernel (the cone trace): algorithmic bytes per
 launch / its average event-timed duration.  `cpu_baseline` times the oracle
 (the CPU restatement, one thread) on a bounded sample of the same frame.
@@ -281,7 +282,7 @@ def main() -> int:
     k = float(args.steps)
     passes = {"shadowmap_ms": pt.shadowmap_ms / k, "voxelize_ms": pt.voxelize_ms / k,
               "radiance_ms": pt.radiance_ms / k, "mipmap_ms": pt.mipmap_ms / k, "gbuffer_ms": pt.gbuffer_ms / k,
-              "render_ms": pt.render_ms / k}
+              "render_ms": pt.render_ms / k, "compose_ms": pt.compose_ms / k}
     metric_ms = passes["voxelize_ms"] + passes["radiance_ms"] + passes["mipmap_ms"] + passes["render_ms"]
     npix = cfg.width * cfg.height
     ncones = 6 + (1 if cfg.reflections else 0)

@@ -69,6 +69,8 @@ extern "C" {
 #define VCT_BUF_WEIGHTS 13   /* float [VCT_MLP_PARAMS + 1]: FP32 weights, out scale */
 #define VCT_BUF_GRADIENT 14  /* float [VCT_MLP_PARAMS + 1]: last step's minibatch
                                  gradient (sum of the workgroup partials) + loss   */
+#define VCT_BUF_COLOR 15     /* float4 [height][width]: the composed image (rgb, 1)
+                                before framebuffer quantization; 0 where uncovered */
 
 /* Grid description: the VCT resource owner (src/Application.h:107-156). */
 typedef struct vct_grid_desc {
@@ -196,6 +198,7 @@ typedef struct vct_pass_times {
     double gbuffer_ms;
     double render_ms;     /* cone trace                                            */
     double learned_ms;    /* learned gather (when used instead of render)          */
+    double compose_ms;    /* direct light + composition                            */
 } vct_pass_times;
 
 typedef struct vct_ctx vct_ctx;
@@ -250,6 +253,30 @@ int vct_gbuffer(vct_ctx *ctx, const vct_camera *cam);
 /* Cone-traced indirect light: traceCone + the 6-cone gather + specular cone
  * (src/Application.cpp:967-1067, shaders/phong.frag:135-180,455-512). */
 int vct_trace(vct_ctx *ctx, const vct_trace_params *p);
+
+/* Final image of the phong pass (shaders/phong.frag:183-344,455-520):
+ * direct light of every enabled light (Cook-Torrance with GGX, Schlick-GGX and
+ * Schlick Fresnel, metallic 0, roughness from the object or 0.5; or
+ * Blinn-Phong), each shadow caster's light scaled by 1 - its 5-tap PCF
+ * shadow factor, plus (indirect + reflection * reflect_scale) * ambient_scale
+ * * albedo from the last vct_trace, times the occlusion when that trace drew
+ * it, then Reinhard tone mapping and gamma 2.2.  Needs a vct_trace of the
+ * current G-buffer (with reflections when enable_reflections is set).  The
+ * reference shades every shadow caster with the main light's map
+ * (phong.frag:331); here each light uses its own. */
+typedef struct vct_compose_params {
+    int cook_torrance;        /* settings.cooktorrance (1); 0: Blinn-Phong      */
+    int enable_diffuse;       /* enableDiffuse (1)                              */
+    int enable_specular;      /* enableSpecular (1)                             */
+    int enable_indirect;      /* enableIndirect (1)                             */
+    int enable_reflections;   /* enableReflections (1)                          */
+    int postprocess;          /* enablePostprocess (1)                          */
+    float ambient_scale;      /* ambientScale (1)                               */
+    float reflect_scale;      /* reflectScale (1)                               */
+    float shininess;          /* material.shininess (Blinn-Phong only)          */
+    float eye[3];             /* camera position                                */
+} vct_compose_params;
+int vct_compose(vct_ctx *ctx, const vct_compose_params *p);
 
 /* ---- readback, timing ----------------------------------------------------- */
 int vct_sync(vct_ctx *ctx);

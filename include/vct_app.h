@@ -69,8 +69,13 @@ int vct_app_frame_state(vct_app *app, int frame, float *model, vct_light *lights
  * interpolated exactly as the G-buffer resolve interpolates them. */
 int vct_app_surface_points(vct_app *app, int frame, int n, uint64_t seed, vct_gbuffer_texel *out);
 
+/* The composition uniforms of frame `frame` (Settings defaults of
+ * src/Application.h:60-71: Cook-Torrance, diffuse, specular, indirect,
+ * reflections as configured, post-processing, ambient and reflect scale 1). */
+int vct_app_compose_params(vct_app *app, int frame, vct_compose_params *out);
+
 /* One frame in the order of Application::render: transforms, shadow maps,
- * voxelize, normalize+inject, [fill holes], mip, G-buffer, trace. */
+ * voxelize, normalize+inject, [fill holes], mip, G-buffer, trace, compose. */
 int vct_app_render(vct_app *app, int frame);
 /* The subset of a frame up to and including the mip chain. */
 int vct_app_build_radiance(vct_app *app, int frame);

@@ -91,6 +91,8 @@ def lib() -> C.CDLL:
             "or_trace64": (None, [vp, C.c_int, vp, vp, C.POINTER(TraceParams), vp, vp]),
             "or_trace_cone": (None, [vp, vp, C.c_int, C.c_int, vp, vp, vp, C.POINTER(Cone), vp]),
             "or_sample": (None, [vp, C.c_int, C.c_int, vp, C.c_float, vp]),
+            "or_calc_shadow_factor": (C.c_float, [vp, vp, C.c_int]),
+            "or_compose": (None, [vp, C.c_int, vp, vp, vp, C.c_int, vp, C.c_int, C.c_int, vp, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -309,3 +311,30 @@ def trace(gbuf: np.ndarray, levels, params: TraceParams, aniso_faces=None, fp64=
     fn(_p(g), npix, C.cast(lp, C.c_void_p), None if top is None else C.cast(top, C.c_void_p), C.byref(params),
        _p(ind), _p(spec))
     return ind, spec
+
+
+def shadow_factor(lsp, shadowmap: np.ndarray) -> float:
+    lp = np.ascontiguousarray(lsp, np.float32)
+    sm = np.ascontiguousarray(shadowmap, np.float32)
+    return float(lib().or_calc_shadow_factor(_p(lp), _p(sm), sm.shape[0]))
+
+
+def compose(gbuf: np.ndarray, indirect, specular, lights, shadowmaps, params, has_roughness=False,
+            occluded=True) -> np.ndarray:
+    """params: a vct_amd._abi.ComposeParams (same layout as or_compose_params).
+    indirect/specular: float4 per pixel (or None when not used)."""
+    from vct_amd import _abi as abi
+    g = np.ascontiguousarray(gbuf, np.float32)
+    npix = g.size // 16
+    ind = np.zeros((npix, 4), np.float32) if indirect is None else np.ascontiguousarray(indirect, np.float32)
+    spe = np.zeros((npix, 4), np.float32) if specular is None else np.ascontiguousarray(specular, np.float32)
+    assert ind.size == npix * 4 and spe.size == npix * 4
+    la = _light_array(lights) if lights else None
+    sms = [None if s is None else np.ascontiguousarray(s, np.float32) for s in (shadowmaps or [])]
+    sms += [None] * (len(lights) - len(sms))
+    ptrs = (C.c_void_p * max(1, len(sms)))(*[(s.ctypes.data if s is not None else None) for s in sms])
+    P = abi.ComposeParams.from_buffer_copy(params)
+    out = np.zeros((npix, 4), np.float32)
+    lib().or_compose(_p(g), npix, _p(ind), _p(spe), None if la is None else C.cast(la, C.c_void_p), len(lights),
+                     C.cast(ptrs, C.c_void_p), 1 if has_roughness else 0, 1 if occluded else 0, C.byref(P), _p(out))
+    return out

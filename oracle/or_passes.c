@@ -158,8 +158,9 @@ static float shadow_bilinear(const float *sm, int N, float s, float t, int ox, i
     return ((w00 * t00 + w10 * t10) + w01 * t01) + w11 * t11;
 }
 
-/* calcShadowFactor (shaders/voxelize.frag:160-184) */
-static float calc_shadow_factor(const float lsp[4], const float *sm, int N) {
+/* calcShadowFactor (shaders/voxelize.frag:160-184; the same function is
+ * phong.frag:183-207) */
+float or_calc_shadow_factor(const float lsp[4], const float *sm, int N) {
     float sh[3];
     for (int k = 0; k < 3; ++k) sh[k] = (lsp[k] / lsp[3] + 1.0f) * 0.5f;
     float frag = sh[2] - 0.01f;
@@ -258,7 +259,7 @@ static void light_fragment(const or_light *lights, int nlights, const float *con
         if (L->shadow_caster && shadowmaps && shadowmaps[i]) {
             float w4[4] = {wpos[0], wpos[1], wpos[2], 1.0f}, lsp[4];
             or_mat4_vec4(L->ls, w4, lsp);
-            float sf = 1.0f - calc_shadow_factor(lsp, shadowmaps[i], (int)L->shadow_size);
+            float sf = 1.0f - or_calc_shadow_factor(lsp, shadowmaps[i], (int)L->shadow_size);
             for (int k = 0; k < 3; ++k) lit[k] = lit[k] * sf;
         }
         for (int k = 0; k < 3; ++k) fin[k] = fin[k] + lit[k];

@@ -30,6 +30,9 @@
  *   or_half / or_fp8                                        numpy float16; OCP
  *       E4M3 code table round trip
  *   or_gbuffer                                              closed form plane hits
+ *   or_compose                                              closed forms: N=L=V
+ *       Cook-Torrance and Blinn-Phong values, point-light range cut and
+ *       smoothstep, shadow on/off, Reinhard+gamma, indirect/occlusion algebra
  *   The reference ships no tests, golden vectors or fixtures for any of these,
  *   so none is pinned by reference data; each is pinned by the checks above.
  */
@@ -193,6 +196,26 @@ void or_label_cone64(float *const *levels, int dim, int nlevels, const double po
                      const double dir[3], double tan_half, double out[4]);
 void or_label64(const or_gbuf_texel *g, int npix, float *const *levels, int dim, int nlevels,
                 const float center[3], const float vmin[3], const float vmax[3], double *out);
+
+/* calcShadowFactor (shaders/voxelize.frag:160-184 = phong.frag:183-207):
+ * fraction of the 5 PCF taps in shadow. lsp: light clip-space position. */
+float or_calc_shadow_factor(const float lsp[4], const float *sm, int N);
+
+/* Final composition of phong.frag (shaders/phong.frag:209-344, 486-518):
+ * per-light Cook-Torrance (metallic 0) or Blinn-Phong direct light with
+ * per-light PCF shadows, combined with the traced indirect (as or_trace
+ * writes it) and reflection, then Reinhard + gamma 2.2.  out: float4 per
+ * pixel (rgb, 1); uncovered pixels are 0.  shadowmaps[l] belongs to
+ * lights[l] (NULL for non-casters).  Layout equals vct_compose_params. */
+typedef struct or_compose_params {
+    int cook_torrance, enable_diffuse, enable_specular, enable_indirect;
+    int enable_reflections, postprocess;
+    float ambient_scale, reflect_scale, shininess;
+    float eye[3];
+} or_compose_params;
+void or_compose(const or_gbuf_texel *g, int npix, const float *indirect, const float *specular,
+                const or_light *lights, int nlights, const float *const *shadowmaps, int has_roughness,
+                int occluded, const or_compose_params *p, float *out);
 
 #ifdef __cplusplus
 }

@@ -52,7 +52,34 @@ def run_frame_capture(app, frame=0, fill_holes=False):
     out["indirect"] = ctx.download(abi.VCT_BUF_INDIRECT, 0, np.float32).reshape(-1, 4)
     if fs["trace"].enable_reflections:
         out["specular"] = ctx.download(abi.VCT_BUF_SPECULAR, 0, np.float32).reshape(-1, 4)
+    cp = app.compose_params(frame)
+    ctx.compose(cp)
+    out["compose_params"] = cp
+    out["color"] = ctx.color()
     return out
+
+
+def compose_variant(base: abi.ComposeParams, **kw) -> abi.ComposeParams:
+    p = abi.ComposeParams.from_buffer_copy(base)
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def compose_reference(app, cap, params):
+    """The oracle's composition on the G-buffer, trace output and shadow maps
+    the GPU produced (so only the compose pass itself is compared)."""
+    from oracle import binding as ob
+    fs = cap["fs"]
+    return ob.compose(cap["gbuf"], cap["indirect"], cap.get("specular"), fs["lights"], cap["shadow"], params,
+                      has_roughness=app.mesh()["rough"] is not None, occluded=bool(fs["trace"].draw_occlusion))
+
+
+def assert_compose_close(got, ref, what):
+    # identical operation order; powf and sqrtf may round differently by an ulp
+    # between the GPU math library and the C library, so the bar is 1e-5 relative
+    bad = ~np.isclose(got, ref, rtol=1e-5, atol=1e-6)
+    assert not bad.any(), f"{what}: {int(bad.sum())} channels differ, max abs {np.abs(got - ref).max():.3e}"
 
 
 def rel_err(a: np.ndarray, b: np.ndarray) -> float:

@@ -2,14 +2,16 @@
 
 Bars (north-star): voxel occupancy, the packed RGBA8 albedo/normal
 accumulators and the statistics bit-exact; shadow depth, radiance level 0 and
-the fp16 mip chain bit-exact (identical operation order; the north-star
+the fp16 mip chain bit-exact (identical operation order; the required
 tolerance would be 1e-3 max-abs); the indirect-lighting image within 1e-3 mean
-relative error of the FP32 oracle.
+relative error of the FP32 oracle; the composed image within 1e-5 relative per
+channel (same operation order, powf/sqrtf of two math libraries).
 """
 import numpy as np
 import pytest
 
-from gpu_util import make_app, rel_err, run_frame_capture
+from gpu_util import (assert_compose_close, compose_reference, compose_variant, make_app, rel_err,
+                      run_frame_capture)
 from oracle import binding as ob
 
 pytestmark = pytest.mark.gpu
@@ -103,3 +105,20 @@ def test_trace_within_tolerance(scene1):
     assert e < 1e-3, f"indirect image mean relative error {e}"
     assert rel_err(got[:, 3], ref[:, 3]) < 1e-3
     assert ref[:, :3].sum() > 0  # the scene is lit: the comparison is not vacuous
+
+
+@pytest.mark.parametrize("variant", [dict(), dict(cook_torrance=0), dict(postprocess=0),
+                                     dict(enable_indirect=0), dict(enable_diffuse=0, enable_specular=0)],
+                         ids=["app", "blinn_phong", "no_post", "no_indirect", "indirect_only"])
+def test_compose_matches_oracle(scene1, variant):
+    app, cap = scene1
+    p = compose_variant(cap["compose_params"], **variant)
+    if variant:
+        app.ctx.compose(p)
+        got = app.ctx.color()
+    else:
+        got = cap["color"]
+    ref = compose_reference(app, cap, p)
+    assert_compose_close(got, ref, f"compose {variant}")
+    cov = cap["gbuf"][..., 11].view(np.uint32).reshape(-1) != 0
+    assert ref[cov, :3].sum() > 0 and np.all(ref[~cov] == 0)

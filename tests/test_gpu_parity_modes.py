@@ -1,10 +1,12 @@
 """GPU parity beyond the small scene: the dynamic scene at reduced resolution
 (5 shadow-casting lights including perspective point-light maps, 6-direction
-mips, the specular cone), the FP32 and FP8 storage formats, and fill-holes."""
+mips, the specular cone, the composed image with point lights and reflections),
+the FP32 and FP8 storage formats, and fill-holes."""
 import numpy as np
 import pytest
 
-from gpu_util import make_app, rel_err, run_frame_capture
+from gpu_util import (assert_compose_close, compose_reference, compose_variant, make_app, rel_err,
+                      run_frame_capture)
 from oracle import binding as ob
 
 pytestmark = pytest.mark.gpu
@@ -70,6 +72,22 @@ def test_scene2_gbuffer_and_trace(scene2_small):
     assert rel_err(cap["indirect"][:, 3], ind[:, 3]) < 1e-3
     assert es < 1e-3
     assert ind[:, :3].sum() > 0 and spec[:, :3].sum() > 0
+
+
+@pytest.mark.parametrize("variant", [dict(), dict(cook_torrance=0), dict(enable_reflections=0)],
+                         ids=["app", "blinn_phong", "no_reflections"])
+def test_scene2_compose(scene2_small, variant):
+    app, cap = scene2_small
+    p = compose_variant(cap["compose_params"], **variant)
+    assert cap["compose_params"].enable_reflections == 1
+    if variant:
+        app.ctx.compose(p)
+        got = app.ctx.color()
+    else:
+        got = cap["color"]
+    ref = compose_reference(app, cap, p)
+    assert_compose_close(got, ref, f"scene 2 compose {variant}")
+    assert ref[:, :3].sum() > 0
 
 
 @pytest.mark.parametrize("storage", [1, 2])

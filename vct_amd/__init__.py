@@ -145,6 +145,13 @@ class Context:
     def trace(self, p: abi.TraceParams):
         self.check(self.lib.vct_trace(self.h, C.byref(p)))
 
+    def compose(self, p: abi.ComposeParams):
+        self.check(self.lib.vct_compose(self.h, C.byref(p)))
+
+    def color(self) -> np.ndarray:
+        """Final image of the last vct_compose, float4 per pixel."""
+        return self.download(abi.VCT_BUF_COLOR, 0, np.float32).reshape(-1, 4)
+
     def sync(self):
         self.check(self.lib.vct_sync(self.h))
 
@@ -290,6 +297,11 @@ class App:
         self.check(self.lib.vct_app_frame_state(self.h, frame, _ptr(model), lights, C.byref(vox), C.byref(inj),
                                                 C.byref(mip), C.byref(cam), C.byref(tr)))
         return dict(model=model, lights=list(lights), vox=vox, inj=inj, mip=mip, cam=cam, trace=tr)
+
+    def compose_params(self, frame: int = 0) -> abi.ComposeParams:
+        p = abi.ComposeParams()
+        self.check(self.lib.vct_app_compose_params(self.h, frame, C.byref(p)))
+        return p
 
     def surface_points(self, n: int, frame: int = 0, seed: int = 1) -> np.ndarray:
         out = np.zeros((n, 16), np.float32)

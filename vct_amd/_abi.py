@@ -35,6 +35,7 @@ VCT_BUF_FEATURES = 11
 VCT_BUF_REFERENCE = 12
 VCT_BUF_WEIGHTS = 13
 VCT_BUF_GRADIENT = 14
+VCT_BUF_COLOR = 15
 VCT_MLP_PARAMS = 16835
 MLP_IN = 160
 
@@ -96,7 +97,14 @@ class Stats(C.Structure):
 class PassTimes(C.Structure):
     _fields_ = [("shadowmap_ms", C.c_double), ("voxelize_ms", C.c_double), ("radiance_ms", C.c_double),
                 ("fill_ms", C.c_double), ("mipmap_ms", C.c_double), ("gbuffer_ms", C.c_double),
-                ("render_ms", C.c_double), ("learned_ms", C.c_double)]
+                ("render_ms", C.c_double), ("learned_ms", C.c_double), ("compose_ms", C.c_double)]
+
+
+class ComposeParams(C.Structure):
+    _fields_ = [("cook_torrance", C.c_int), ("enable_diffuse", C.c_int), ("enable_specular", C.c_int),
+                ("enable_indirect", C.c_int), ("enable_reflections", C.c_int), ("postprocess", C.c_int),
+                ("ambient_scale", C.c_float), ("reflect_scale", C.c_float), ("shininess", C.c_float),
+                ("eye", C.c_float * 3)]
 
 
 class AppConfig(C.Structure):
@@ -133,6 +141,7 @@ VCT_FUNCS = [
     ("vct_mip", C.c_int, [C.c_void_p, C.POINTER(MipParams)]),
     ("vct_gbuffer", C.c_int, [C.c_void_p, C.POINTER(Camera)]),
     ("vct_trace", C.c_int, [C.c_void_p, C.POINTER(TraceParams)]),
+    ("vct_compose", C.c_int, [C.c_void_p, C.POINTER(ComposeParams)]),
     ("vct_sync", C.c_int, [C.c_void_p]),
     ("vct_get_stats", C.c_int, [C.c_void_p, C.POINTER(Stats)]),
     ("vct_get_pass_times", C.c_int, [C.c_void_p, C.POINTER(PassTimes)]),
@@ -160,6 +169,7 @@ APP_FUNCS = [
                                       C.POINTER(VoxelizeParams), C.POINTER(InjectParams), C.POINTER(MipParams),
                                       C.POINTER(Camera), C.POINTER(TraceParams)]),
     ("vct_app_render", C.c_int, [C.c_void_p, C.c_int]),
+    ("vct_app_compose_params", C.c_int, [C.c_void_p, C.c_int, C.POINTER(ComposeParams)]),
     ("vct_app_build_radiance", C.c_int, [C.c_void_p, C.c_int]),
     ("vct_app_surface_points", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_uint64, C.c_void_p]),
     ("vct_math_ortho", None, [C.c_float] * 6 + [C.c_void_p]),

@@ -23,7 +23,7 @@
 
 using namespace vctk;
 
-enum PassId { P_SHADOW = 0, P_VOXELIZE, P_RADIANCE, P_FILL, P_MIP, P_GBUFFER, P_TRACE, P_LEARNED, P_COUNT };
+enum PassId { P_SHADOW = 0, P_VOXELIZE, P_RADIANCE, P_FILL, P_MIP, P_GBUFFER, P_TRACE, P_LEARNED, P_COMPOSE, P_COUNT };
 
 struct TimedPair {
     int pass;
@@ -63,7 +63,9 @@ struct vct_ctx {
     int gw = 0, gh = 0;
     vct_gbuffer_texel *d_gbuf = nullptr;
     uint64_t *d_vis = nullptr;
-    float4 *d_ind = nullptr, *d_spec = nullptr;
+    float4 *d_ind = nullptr, *d_spec = nullptr, *d_color = nullptr;
+    // what the last vct_trace of the current G-buffer left in d_ind / d_spec
+    bool have_trace = false, trace_occluded = false, trace_spec = false;
     bool have_gbuf = false;
     // staging for downloads
     float4 *d_stage = nullptr;
@@ -324,6 +326,7 @@ void vct_destroy(vct_ctx *c) {
     dfree(c->d_vis);
     dfree(c->d_ind);
     dfree(c->d_spec);
+    dfree(c->d_color);
     dfree(c->d_stage);
     dfree(c->d_points);
     dfree(c->d_labels);
@@ -669,11 +672,14 @@ static int ensure_camera(vct_ctx *c, int w, int h) {
     dfree(c->d_vis);
     dfree(c->d_ind);
     dfree(c->d_spec);
+    dfree(c->d_color);
     const size_t np = (size_t)w * h;
     HIPCHK(c, dalloc(&c->d_gbuf, np * sizeof(vct_gbuffer_texel)));
     HIPCHK(c, dalloc(&c->d_vis, np * 8));
     HIPCHK(c, dalloc(&c->d_ind, np * 16));
     HIPCHK(c, dalloc(&c->d_spec, np * 16));
+    HIPCHK(c, dalloc(&c->d_color, np * 16));
+    HIPCHK(c, hipMemset(c->d_color, 0, np * 16));
     HIPCHK(c, hipMemset(c->d_gbuf, 0, np * sizeof(vct_gbuffer_texel)));
     HIPCHK(c, hipMemset(c->d_ind, 0, np * 16));
     HIPCHK(c, hipMemset(c->d_spec, 0, np * 16));
@@ -730,6 +736,23 @@ int vct_trace(vct_ctx *c, const vct_trace_params *p) {
     memcpy(t.ext.vmax, c->desc.vmax, 12);
     PassTimer pt(c, P_TRACE);
     LAUNCHCHK(c, launch_trace(c->pyr, t, c->stream));
+    c->have_trace = true;
+    c->trace_occluded = p->draw_occlusion != 0;
+    c->trace_spec = p->enable_reflections != 0;
+    return VCT_OK;
+}
+
+int vct_compose(vct_ctx *c, const vct_compose_params *p) {
+    if (!c || !p) return fail(c, VCT_ERR_INVALID, "vct_compose: NULL argument");
+    if (!c->have_gbuf) return fail(c, VCT_ERR_STATE, "vct_compose: no G-buffer");
+    if (p->enable_indirect && !c->have_trace)
+        return fail(c, VCT_ERR_STATE, "vct_compose: enable_indirect needs a vct_trace of the current G-buffer");
+    if (p->enable_indirect && p->enable_reflections && !c->trace_spec)
+        return fail(c, VCT_ERR_STATE, "vct_compose: enable_reflections needs a vct_trace with reflections");
+    PassTimer pt(c, P_COMPOSE);
+    LAUNCHCHK(c, launch_compose(c->d_gbuf, c->d_ind, c->d_spec, c->d_color, c->gw * c->gh,
+                                c->lights.empty() ? nullptr : c->d_lights, (int)c->lights.size(), c->has_rough ? 1 : 0,
+                                c->trace_occluded ? 1 : 0, *p, c->stream));
     return VCT_OK;
 }
 
@@ -771,6 +794,7 @@ int vct_get_pass_times(vct_ctx *c, vct_pass_times *out) {
     out->gbuffer_ms = ms[P_GBUFFER];
     out->render_ms = ms[P_TRACE];
     out->learned_ms = ms[P_LEARNED];
+    out->compose_ms = ms[P_COMPOSE];
     return VCT_OK;
 }
 
@@ -804,6 +828,7 @@ size_t vct_buffer_bytes(const vct_ctx *c, int what, int level) {
     case VCT_BUF_GBUFFER: return (size_t)c->gw * c->gh * sizeof(vct_gbuffer_texel);
     case VCT_BUF_INDIRECT:
     case VCT_BUF_SPECULAR:
+    case VCT_BUF_COLOR:
     case VCT_BUF_REFERENCE: return (size_t)c->gw * c->gh * 16;
     case VCT_BUF_LABELS: return (size_t)c->npoints * 16;
     case VCT_BUF_FEATURES: return (size_t)c->npoints * kMlpIn * 2;
@@ -843,6 +868,7 @@ int vct_download(vct_ctx *c, int what, int level, void *host, size_t bytes) {
     case VCT_BUF_GBUFFER: src = c->d_gbuf; break;
     case VCT_BUF_INDIRECT: src = c->d_ind; break;
     case VCT_BUF_SPECULAR: src = c->d_spec; break;
+    case VCT_BUF_COLOR: src = c->d_color; break;
     case VCT_BUF_LABELS: src = c->d_labels; break;
     case VCT_BUF_FEATURES: src = c->d_X; break;
     case VCT_BUF_REFERENCE:

@@ -216,6 +216,88 @@ __global__ __launch_bounds__(256) void k_raster_items(const int4 *__restrict__ r
     f.finish();
 }
 
+// The same walk for passes whose per-fragment work is heavy (voxelize with
+// lighting): F provides cover(slot, px, py, edges, l[3]) -> bool, shade(slot,
+// l) and finish().  Covered fragments are appended to a per-wavefront queue in
+// LDS and shaded 64 at a time, so the shading runs with every lane busy
+// instead of with the lanes of uncovered bbox pixels idle.  The queue holds up
+// to 127 entries (at most 63 left over plus one chunk of 64).
+template <class F>
+__global__ __launch_bounds__(256) void k_raster_queued(const int4 *__restrict__ recs,
+                                                       const uint64_t *__restrict__ offs,
+                                                       const uint32_t *__restrict__ chunk, size_t chunk_cap,
+                                                       int nslots, F f) {
+    __shared__ int q_slot[4][128];
+    __shared__ float q_l[4][3][128];
+    const uint64_t total = offs[nslots];
+    const uint64_t nchunks = (total + 63) >> 6;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    int count = 0;  // wave-uniform queue length
+    for (uint64_t c = wave; c < nchunks; c += nwaves) {
+        const uint64_t k = (c << 6) + (uint64_t)lane;
+        int lo = c < chunk_cap ? (int)chunk[c] : 0;
+        int hi = (c + 1 < nchunks && c + 1 < chunk_cap) ? (int)chunk[c + 1] : nslots - 1;
+        bool cov = false;
+        float l[3];
+        if (k < total) {
+            while (lo < hi) {
+                int mid = (lo + hi + 1) >> 1;
+                if (offs[mid] <= k) lo = mid;
+                else hi = mid - 1;
+            }
+            const uint64_t local = k - offs[lo];
+            Edges e;
+            load_edges(recs, (size_t)lo, e);
+            const uint32_t lu = (uint32_t)local;
+            const int px = e.x0 + (int)(lu % (uint32_t)e.bw);
+            const int py = e.y0 + (int)(lu / (uint32_t)e.bw);
+            cov = f.cover(lo, px, py, e, l);
+        }
+        const uint64_t m = __ballot(cov);
+        if (cov) {
+            const int pos = count + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+            q_slot[w][pos] = lo;
+            q_l[w][0][pos] = l[0];
+            q_l[w][1][pos] = l[1];
+            q_l[w][2][pos] = l[2];
+        }
+        count += __popcll(m);
+        __builtin_amdgcn_wave_barrier();
+        if (count >= 64) {
+            const int s0 = q_slot[w][lane];
+            const float ls[3] = {q_l[w][0][lane], q_l[w][1][lane], q_l[w][2][lane]};
+            // move the entries 64 .. count-1 to the front
+            const int rest = count - 64;
+            int ms = 0;
+            float ml0 = 0.f, ml1 = 0.f, ml2 = 0.f;
+            if (lane < rest) {
+                ms = q_slot[w][64 + lane];
+                ml0 = q_l[w][0][64 + lane];
+                ml1 = q_l[w][1][64 + lane];
+                ml2 = q_l[w][2][64 + lane];
+            }
+            __builtin_amdgcn_wave_barrier();
+            if (lane < rest) {
+                q_slot[w][lane] = ms;
+                q_l[w][0][lane] = ml0;
+                q_l[w][1][lane] = ml1;
+                q_l[w][2][lane] = ml2;
+            }
+            __builtin_amdgcn_wave_barrier();
+            count = rest;
+            f.shade(s0, ls);
+        }
+    }
+    if (lane < count) {
+        const float ls[3] = {q_l[w][0][lane], q_l[w][1][lane], q_l[w][2][lane]};
+        f.shade(q_slot[w][lane], ls);
+    }
+    f.finish();
+}
+
 // Grid for k_raster_items: enough resident wavefronts to fill 256 CUs.
 constexpr int kRasterBlocks = 2048;
 

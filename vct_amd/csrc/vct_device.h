@@ -253,6 +253,24 @@ __device__ __forceinline__ float shadow_bilinear_quad(const float4 *__restrict__
     return ((w00 * T.x + w10 * T.y) + w01 * T.z) + w11 * T.w;
 }
 
+// calcShadowFactor (shaders/voxelize.frag:159-180, phong.frag:183-207): the
+// fraction of the five PCF taps (offsets (0,0), (1,0), (0,1), (-1,0), (0,-1))
+// that occlude a light-space position; 0 beyond the far plane.
+__device__ __forceinline__ float calc_shadow_factor(const float lsp[4], const float4 *sq, int N) {
+    float sh[3];
+    for (int k = 0; k < 3; ++k) sh[k] = (lsp[k] / lsp[3] + 1.0f) * 0.5f;
+    const float frag = sh[2] - 0.01f;
+    if (frag > 1.0f) return 0.0f;
+    float f = 0.0f;
+    if (frag > shadow_bilinear_quad(sq, N, sh[0], sh[1], 0, 0)) f += 1.0f;
+    if (frag > shadow_bilinear_quad(sq, N, sh[0], sh[1], 1, 0)) f += 1.0f;
+    if (frag > shadow_bilinear_quad(sq, N, sh[0], sh[1], 0, 1)) f += 1.0f;
+    if (frag > shadow_bilinear_quad(sq, N, sh[0], sh[1], -1, 0)) f += 1.0f;
+    if (frag > shadow_bilinear_quad(sq, N, sh[0], sh[1], 0, -1)) f += 1.0f;
+    f = f / 5.0f;
+    return f;
+}
+
 // ---- wave helpers ----------------------------------------------------------
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 

@@ -154,6 +154,11 @@ hipError_t launch_mip(const PyrDesc &pyr, hipStream_t s);
 // trace.hip
 hipError_t launch_trace(const PyrDesc &pyr, const TraceLaunch &t, hipStream_t s);
 
+// compose.hip
+hipError_t launch_compose(const vct_gbuffer_texel *gbuf, const float4 *ind, const float4 *spec, float4 *out, int npix,
+                          const DevLight *lights, int nlights, int has_rough, int occluded,
+                          const vct_compose_params &p, hipStream_t s);
+
 // bricked albedo/normal grid <-> linear x-fastest
 hipError_t launch_grid_convert(const uint32_t *src, uint32_t *dst, int dim, bool to_linear, hipStream_t s);
 

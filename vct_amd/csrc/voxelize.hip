@@ -81,21 +81,6 @@ __global__ __launch_bounds__(256) void k_vox_setup(DevScene sc, VoxMats vm, int4
 }
 
 // calcShadowFactor (shaders/voxelize.frag:160-184)
-__device__ __forceinline__ float calc_shadow_factor(const float lsp[4], const float4 *sq, int N) {
-    float sh[3];
-    for (int k = 0; k < 3; ++k) sh[k] = (lsp[k] / lsp[3] + 1.0f) * 0.5f;
-    const float frag = sh[2] - 0.01f;
-    if (frag > 1.0f) return 0.0f;
-    float f = 0.0f;
-    if (frag > shadow_bilinear_quad(sq, N, sh[0], sh[1], 0, 0)) f += 1.0f;
-    if (frag > shadow_bilinear_quad(sq, N, sh[0], sh[1], 1, 0)) f += 1.0f;
-    if (frag > shadow_bilinear_quad(sq, N, sh[0], sh[1], 0, 1)) f += 1.0f;
-    if (frag > shadow_bilinear_quad(sq, N, sh[0], sh[1], -1, 0)) f += 1.0f;
-    if (frag > shadow_bilinear_quad(sq, N, sh[0], sh[1], 0, -1)) f += 1.0f;
-    f = f / 5.0f;
-    return f;
-}
-
 struct VoxFrag {
     const float4 *attr;
     const DevLight *lights;
@@ -104,15 +89,14 @@ struct VoxFrag {
     uint32_t *partials;
     uint32_t frags;
 
-    __device__ void item(int slot, int px, int py, const Edges &e) {
+    // Coverage of one (triangle, pixel) item: 4x MSAA with the per-sample
+    // near/far clip; on coverage, the pixel-centre barycentrics in l.
+    __device__ bool cover(int slot, int px, int py, const Edges &e, float l[3]) const {
         const float4 *a = attr + (size_t)slot * 8;
-        const float4 a7 = a[7];
-        const uint32_t flags = __float_as_uint(a7.x);
-        const int axis = (int)(flags & 3u);
+        const uint32_t flags = __float_as_uint(a[7].x);
         const bool zclip = (flags >> 4) & 1u;
-        const float4 a0 = a[0], a1 = a[1], a2 = a[2];
         // ndc z of the three vertices: f[2], f[5], f[8]
-        const float nz0 = a0.z, nz1 = a1.y, nz2 = a2.x;
+        const float nz0 = a[0].z, nz1 = a[1].y, nz2 = a[2].x;
         bool cov = false;
         const int ns = msaa ? 4 : 1;
         for (int k = 0; k < ns && !cov; ++k) {
@@ -126,9 +110,16 @@ struct VoxFrag {
             }
             cov = true;
         }
-        if (!cov) return;
-        float l[3];
-        edge_bary(e, px, py, 128, 128, l);
+        if (cov) edge_bary(e, px, py, 128, 128, l);
+        return cov;
+    }
+
+    // One covered fragment: varyings, lighting and the two atomicMax stores.
+    __device__ void shade(int slot, const float l[3]) {
+        const float4 *a = attr + (size_t)slot * 8;
+        const uint32_t flags = __float_as_uint(a[7].x);
+        const int axis = (int)(flags & 3u);
+        const float4 a0 = a[0], a1 = a[1], a2 = a[2];
         const float4 a3 = a[3], a4 = a[4], a5 = a[5], a6 = a[6];
         float f[28];
         f[0] = a0.x; f[1] = a0.y; f[2] = a0.z; f[3] = a0.w;
@@ -242,7 +233,7 @@ hipError_t launch_voxelize(const DevScene &sc, const DevLight *dlights, int nlig
     f.partials = partials;
     f.frags = 0;
     static_assert(kRasterBlocks <= kStatPartials, "one statistics partial per rasterizer workgroup");
-    hipLaunchKernelGGL(k_raster_items<VoxFrag>, dim3(kRasterBlocks), dim3(256), 0, s, rs.rec, rs.offs, rs.chunk,
+    hipLaunchKernelGGL(k_raster_queued<VoxFrag>, dim3(kRasterBlocks), dim3(256), 0, s, rs.rec, rs.offs, rs.chunk,
                        rs.chunk_cap, sc.ntri, f);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     // totalVoxelFragments (voxelize.frag:195)

@@ -706,12 +706,34 @@ int vct_app_build_radiance(vct_app *a, int frame) {
     return VCT_OK;
 }
 
+int vct_app_compose_params(vct_app *a, int frame, vct_compose_params *out) {
+    (void)frame;  // the camera is static in every scene
+    if (!a || !out) return VCT_ERR_INVALID;
+    vct_compose_params p;
+    std::memset(&p, 0, sizeof(p));
+    p.cook_torrance = 1;
+    p.enable_diffuse = 1;
+    p.enable_specular = 1;
+    p.enable_indirect = 1;
+    p.enable_reflections = a->trace.enable_reflections;
+    p.postprocess = 1;
+    p.ambient_scale = 1.0f;
+    p.reflect_scale = 1.0f;
+    p.shininess = 32.0f;  // Material() default (src/Graphics/Mesh.h:29)
+    std::memcpy(p.eye, a->cam.eye, 12);
+    *out = p;
+    return VCT_OK;
+}
+
 int vct_app_render(vct_app *a, int frame) {
     if (!a) return VCT_ERR_INVALID;
     int rc = vct_app_build_radiance(a, frame);
     if (rc) return rc;
     if ((rc = vct_gbuffer(a->ctx, &a->cam))) return rc;
-    return vct_trace(a->ctx, &a->trace);
+    if ((rc = vct_trace(a->ctx, &a->trace))) return rc;
+    vct_compose_params cp;
+    vct_app_compose_params(a, frame, &cp);
+    return vct_compose(a->ctx, &cp);
 }
 
 void vct_math_ortho(float l, float r, float b, float t, float n, float f, float out[16]) { ortho(l, r, b, t, n, f, out); }
