@@ -46,7 +46,13 @@ __global__ __launch_bounds__(256) void k_mip_chain(MipArgs a) {
     const int s = a.src.size;
     const int R = s < 8 ? s : 8;
     const int nreg = s / R;
-    const int rx = blockIdx.x % nreg, ry = (blockIdx.x / nreg) % nreg, rz = blockIdx.x / (nreg * nreg);
+    // XCD-aware order: workgroup b runs on XCD b % 8, so give each XCD a
+    // contiguous run of regions; x-neighbours then share an L2 and run at
+    // about the same time, which merges the two halves of the pair elements
+    // on a region's x boundary in one L2 line before it is written back.
+    const int nblk = nreg * nreg * nreg;
+    const int b = (nblk & 7) == 0 ? (int)(blockIdx.x & 7u) * (nblk >> 3) + (int)(blockIdx.x >> 3) : (int)blockIdx.x;
+    const int rx = b % nreg, ry = (b / nreg) % nreg, rz = b / (nreg * nreg);
     const int ox = rx * R, oy = ry * R, oz = rz * R;
     const int R3 = R * R * R;
     for (int f = 0; f < SF; ++f) {  // the launcher picks SRC6 from the source level's face count

@@ -42,6 +42,16 @@ __device__ __forceinline__ uint4 bload16(__amdgpu_buffer_rsrc_t r, uint32_t off)
 __device__ __forceinline__ uint32_t bload4(__amdgpu_buffer_rsrc_t r, uint32_t off) {
     return __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0);
 }
+// Scalar load of 16 bytes at a wave-uniform offset: through the constant
+// address space the load becomes s_load_dwordx4 into SGPRs (scalar data cache,
+// not the vector memory pipeline).  The pyramid is not written while a trace
+// runs.  The offset is one the vector path would load (clamped texel indices),
+// so it is inside the allocation.
+typedef __attribute__((address_space(4))) const char cchar;
+__device__ __forceinline__ uint4 sload16(const char *pyr, uint32_t off) {
+    const u32x4 v = *(const __attribute__((address_space(4))) u32x4 *)((cchar *)pyr + off);
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
 
 __device__ __forceinline__ void axis_terms(float p, int s, uint32_t mul, uint32_t sub, uint32_t t[2], float w[2]) {
     const float u = p * (float)s - 0.5f;
@@ -84,9 +94,9 @@ __device__ __forceinline__ void acc_f4(float4 &acc, uint4 v, float w) {
 // dir^2-weighted mix of the three faces that face the cone (fe: their element
 // offsets, 64 * face).  L is wave-uniform.
 template <int MODE>
-__device__ __forceinline__ void level_sample(float4 &acc, __amdgpu_buffer_rsrc_t rs, const float *__restrict__ scales,
-                                             const LevelInfo &L, const float p[3], float lw, const uint32_t fe[3],
-                                             const float d2[3]) {
+__device__ __forceinline__ void level_sample(float4 &acc, const char *pyr, __amdgpu_buffer_rsrc_t rs,
+                                             const float *__restrict__ scales, const LevelInfo &L, const float p[3],
+                                             float lw, const uint32_t fe[3], const float d2[3]) {
     const int s = L.size;
     const bool aniso = L.nfaces == 6;
     uint32_t ty[2], tz[2];
@@ -149,6 +159,38 @@ __device__ __forceinline__ void level_sample(float4 &acc, __amdgpu_buffer_rsrc_t
         wa[c] = wlo * wyz;
         wb[c] = whi * wyz;
     }
+    // At 1080p a wavefront's 8x8 pixels span about a voxel, so at the coarser
+    // levels all its lanes usually need the same 2x2x2 texels (and faces).  The
+    // four corner elements are then fetched once for the wavefront with scalar
+    // loads and every lane weights them itself: the same operations on the same
+    // values as the per-lane path, without its vector memory traffic.  Corners
+    // 0 and 3 together fix all four (the element encoding of a texel index is
+    // unique, and corner 3 carries the clamped upper indices).
+    const uint32_t fk = aniso ? (fe[0] | (fe[1] << 10) | (fe[2] << 20)) : 0u;
+    const bool same = __builtin_amdgcn_readfirstlane(off[0]) == off[0] &&
+                      __builtin_amdgcn_readfirstlane(off[3]) == off[3] && __builtin_amdgcn_readfirstlane(fk) == fk;
+    if (__ballot(same) == __ballot(true)) {
+        uint32_t so[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) so[c] = __builtin_amdgcn_readfirstlane(off[c]);
+        if (!aniso) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) acc_pair(acc, sload16(pyr, so[c]), wa[c], wb[c]);
+            return;
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const uint32_t fo = __builtin_amdgcn_readfirstlane(fe[a]) * 16u;
+            float4 t = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) acc_pair(t, sload16(pyr, so[c] + fo), wa[c], wb[c]);
+            acc.x = fmaf(t.x, d2[a], acc.x);
+            acc.y = fmaf(t.y, d2[a], acc.y);
+            acc.z = fmaf(t.z, d2[a], acc.z);
+            acc.w = fmaf(t.w, d2[a], acc.w);
+        }
+        return;
+    }
     if (!aniso) {
         uint4 v[4];
 #pragma unroll
@@ -210,11 +252,11 @@ __device__ __forceinline__ float4 sample_class(const PyrArgs &pa, __amdgpu_buffe
         return unpack_h4(v.x, v.y);
     }
     if (D >= q) {
-        level_sample<MODE>(acc, rs, pa.scales, pa.lv[q], p, 1.0f, fe, d2);
+        level_sample<MODE>(acc, pa.pyr, rs, pa.scales, pa.lv[q], p, 1.0f, fe, d2);
         return acc;
     }
-    level_sample<MODE>(acc, rs, pa.scales, pa.lv[D], p, 1.0f - fr, fe, d2);
-    level_sample<MODE>(acc, rs, pa.scales, pa.lv[D + 1], p, fr, fe, d2);
+    level_sample<MODE>(acc, pa.pyr, rs, pa.scales, pa.lv[D], p, 1.0f - fr, fe, d2);
+    level_sample<MODE>(acc, pa.pyr, rs, pa.scales, pa.lv[D + 1], p, fr, fe, d2);
     return acc;
 }
 
