@@ -152,35 +152,51 @@ __global__ __launch_bounds__(64) void k_label64(const vct_gbuffer_texel *__restr
 
 
 // Texels x0 .. x0+3 of row (y, z) of face 0 of an isotropic level, 0 outside
-// (x0 >= -2).
-template <int FMT>
-__device__ __forceinline__ void row4(const PyrView &pv, const LevelInfo &L, int x0, int y, int z, float4 t[4]) {
+// (x0 >= -2), as float4 (any element format).
+__device__ __forceinline__ void row4_f32(const PyrView &pv, const LevelInfo &L, int x0, int y, int z, float4 t[4]) {
     const int s = L.size;
 #pragma unroll
     for (int k = 0; k < 4; ++k) t[k] = make_float4(0.f, 0.f, 0.f, 0.f);
     if ((unsigned)y >= (unsigned)s || (unsigned)z >= (unsigned)s) return;
-    if (FMT == 0) {  // fp16 pairs (x, x+1); the pair of the last texel ends in the zero border
-        const char *base = pv.pyr + L.base;
-        if (x0 >= 0) {
-            const uint4 v = *(const uint4 *)(base + (size_t)elem_index(L, x0, y, z, 0) * 16);
-            t[0] = unpack_h4(v.x, v.y);
-            t[1] = unpack_h4(v.z, v.w);
-        } else if (x0 == -1) {
-            const uint2 v = *(const uint2 *)(base + (size_t)elem_index(L, 0, y, z, 0) * 16);
-            t[1] = unpack_h4(v.x, v.y);
-        }
-        const int xb = x0 + 2;
-        if (xb >= 0 && xb < s) {
-            const uint4 v = *(const uint4 *)(base + (size_t)elem_index(L, xb, y, z, 0) * 16);
-            t[2] = unpack_h4(v.x, v.y);
-            t[3] = unpack_h4(v.z, v.w);
-        }
-        return;
-    }
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const int x = x0 + k;
         if ((unsigned)x < (unsigned)s) t[k] = load_texel(pv.pyr, pv.scales, L, x, y, z, 0);
+    }
+}
+
+// The same row of an fp16-pair level, kept as fp16: t[k] = (rg, ba) halves.
+// The pair of the last texel ends in the zero border.
+typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+struct h4 {
+    h2 rg, ba;
+};
+__device__ __forceinline__ h4 as_h4(uint32_t a, uint32_t b) {
+    h4 r;
+    r.rg = __builtin_bit_cast(h2, a);
+    r.ba = __builtin_bit_cast(h2, b);
+    return r;
+}
+__device__ __forceinline__ void row4_h(const PyrView &pv, const LevelInfo &L, int x0, int y, int z, h4 t[4]) {
+    const int s = L.size;
+    const h2 z2 = {(_Float16)0.0f, (_Float16)0.0f};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) t[k].rg = t[k].ba = z2;
+    if ((unsigned)y >= (unsigned)s || (unsigned)z >= (unsigned)s) return;
+    const char *base = pv.pyr + L.base;
+    if (x0 >= 0) {
+        const uint4 v = *(const uint4 *)(base + (size_t)elem_index(L, x0, y, z, 0) * 16);
+        t[0] = as_h4(v.x, v.y);
+        t[1] = as_h4(v.z, v.w);
+    } else if (x0 == -1) {
+        const uint2 v = *(const uint2 *)(base + (size_t)elem_index(L, 0, y, z, 0) * 16);
+        t[1] = as_h4(v.x, v.y);
+    }
+    const int xb = x0 + 2;
+    if (xb >= 0 && xb < s) {
+        const uint4 v = *(const uint4 *)(base + (size_t)elem_index(L, xb, y, z, 0) * 16);
+        t[2] = as_h4(v.x, v.y);
+        t[3] = as_h4(v.z, v.w);
     }
 }
 
@@ -191,7 +207,10 @@ __device__ __forceinline__ float4 lerp4(float4 a, float4 b, float f) {
 
 // Writes the 160 inputs of one G-buffer texel (g0 = position, g3 = B) as 80
 // packed BF16 pairs to row32 (global or LDS).  FMT: level 2 stored as fp16
-// pairs (0) or any other element format (1).
+// pairs (0) or any other element format (1).  For fp16 levels the x and y
+// passes run on packed fp16 (v_pk_fma_f16: two channels per instruction, on
+// the loaded halves as they are; 11 significant bits against the 8 the BF16
+// inputs keep) and the z pass in FP32 (v_fma_mix_f32 on the fp16 halves).
 template <int FMT>
 __device__ __forceinline__ void make_features(const float4 g0, const float4 g3, const PyrView &pv, const Extent &ext,
                                               uint32_t *row32) {
@@ -208,39 +227,81 @@ __device__ __forceinline__ void make_features(const float4 g0, const float4 g3, 
         i0[k] = (int)fl;
         f[k] = u - fl;
     }
-    float4 yprev[9], ycur[9];  // y-interpolated values of the previous / current z slice, [oy][ox]
-    for (int zs = 0; zs < 4; ++zs) {
-        for (int ys = 0; ys < 4; ++ys) {
-            float4 t[4];
-            row4<FMT>(pv, L2, i0[0] - 1, i0[1] - 1 + ys, i0[2] - 1 + zs, t);
+    if (FMT == 0) {
+        const _Float16 fx = (_Float16)f[0], gx = (_Float16)(1.0f - f[0]);
+        const _Float16 fy = (_Float16)f[1], gy = (_Float16)(1.0f - f[1]);
+        const h2 fx2 = {fx, fx}, gx2 = {gx, gx}, fy2 = {fy, fy}, gy2 = {gy, gy};
+        const float fz = f[2], gz = 1.0f - f[2];
+        h4 yprev[9], ycur[9];  // y-interpolated values of the previous / current z slice, [oy][ox]
+        for (int zs = 0; zs < 4; ++zs) {
+            for (int ys = 0; ys < 4; ++ys) {
+                h4 t[4];
+                row4_h(pv, L2, i0[0] - 1, i0[1] - 1 + ys, i0[2] - 1 + zs, t);
 #pragma unroll
-            for (int ox = 0; ox < 3; ++ox) {
-                const float4 xv = lerp4(t[ox], t[ox + 1], f[0]);
-                // ycur[oy] = (1 - fy) * row oy + fy * row oy+1
-                if (ys <= 2) {
-                    const float g = 1.0f - f[1];
-                    ycur[ys * 3 + ox] = make_float4(xv.x * g, xv.y * g, xv.z * g, xv.w * g);
-                }
-                if (ys >= 1) {
-                    float4 &yv = ycur[(ys - 1) * 3 + ox];
-                    yv.x += xv.x * f[1];
-                    yv.y += xv.y * f[1];
-                    yv.z += xv.z * f[1];
-                    yv.w += xv.w * f[1];
+                for (int ox = 0; ox < 3; ++ox) {
+                    h4 xv;
+                    xv.rg = __builtin_elementwise_fma(t[ox + 1].rg, fx2, t[ox].rg * gx2);
+                    xv.ba = __builtin_elementwise_fma(t[ox + 1].ba, fx2, t[ox].ba * gx2);
+                    if (ys <= 2) {
+                        ycur[ys * 3 + ox].rg = xv.rg * gy2;
+                        ycur[ys * 3 + ox].ba = xv.ba * gy2;
+                    }
+                    if (ys >= 1) {
+                        h4 &yv = ycur[(ys - 1) * 3 + ox];
+                        yv.rg = __builtin_elementwise_fma(xv.rg, fy2, yv.rg);
+                        yv.ba = __builtin_elementwise_fma(xv.ba, fy2, yv.ba);
+                    }
                 }
             }
-        }
-        if (zs >= 1) {  // stencils with iz = zs - 2 from slices zs - 1 and zs
-            const int base = (zs - 1) * 9;
+            if (zs >= 1) {  // stencils with iz = zs - 2 from slices zs - 1 and zs
+                const int base = (zs - 1) * 9;
 #pragma unroll
-            for (int j = 0; j < 9; ++j) {
-                const float4 v = lerp4(yprev[j], ycur[j], f[2]);
-                row32[(base + j) * 2 + 0] = bf2pack(v.x, v.y);
-                row32[(base + j) * 2 + 1] = bf2pack(v.z, v.w);
+                for (int j = 0; j < 9; ++j) {
+                    const float vx = fmaf((float)ycur[j].rg.x, fz, (float)yprev[j].rg.x * gz);
+                    const float vy = fmaf((float)ycur[j].rg.y, fz, (float)yprev[j].rg.y * gz);
+                    const float vz = fmaf((float)ycur[j].ba.x, fz, (float)yprev[j].ba.x * gz);
+                    const float vw = fmaf((float)ycur[j].ba.y, fz, (float)yprev[j].ba.y * gz);
+                    row32[(base + j) * 2 + 0] = bf2pack(vx, vy);
+                    row32[(base + j) * 2 + 1] = bf2pack(vz, vw);
+                }
             }
-        }
 #pragma unroll
-        for (int j = 0; j < 9; ++j) yprev[j] = ycur[j];
+            for (int j = 0; j < 9; ++j) yprev[j] = ycur[j];
+        }
+    } else {
+        float4 yprev[9], ycur[9];
+        for (int zs = 0; zs < 4; ++zs) {
+            for (int ys = 0; ys < 4; ++ys) {
+                float4 t[4];
+                row4_f32(pv, L2, i0[0] - 1, i0[1] - 1 + ys, i0[2] - 1 + zs, t);
+#pragma unroll
+                for (int ox = 0; ox < 3; ++ox) {
+                    const float4 xv = lerp4(t[ox], t[ox + 1], f[0]);
+                    if (ys <= 2) {
+                        const float g = 1.0f - f[1];
+                        ycur[ys * 3 + ox] = make_float4(xv.x * g, xv.y * g, xv.z * g, xv.w * g);
+                    }
+                    if (ys >= 1) {
+                        float4 &yv = ycur[(ys - 1) * 3 + ox];
+                        yv.x += xv.x * f[1];
+                        yv.y += xv.y * f[1];
+                        yv.z += xv.z * f[1];
+                        yv.w += xv.w * f[1];
+                    }
+                }
+            }
+            if (zs >= 1) {
+                const int base = (zs - 1) * 9;
+#pragma unroll
+                for (int j = 0; j < 9; ++j) {
+                    const float4 v = lerp4(yprev[j], ycur[j], f[2]);
+                    row32[(base + j) * 2 + 0] = bf2pack(v.x, v.y);
+                    row32[(base + j) * 2 + 1] = bf2pack(v.z, v.w);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 9; ++j) yprev[j] = ycur[j];
+        }
     }
     const float b[3] = {g3.x, g3.y, g3.z};
     const float bl = sqrtf(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
