@@ -205,15 +205,123 @@ __device__ __forceinline__ float4 lerp4(float4 a, float4 b, float f) {
     return make_float4(a.x * g + b.x * f, a.y * g + b.y * f, a.z * g + b.z * f, a.w * g + b.w * f);
 }
 
+// The fp16-pair feature pass in pieces, so that inference can hand each
+// 32-input chunk to the network as soon as it is complete.  For fp16 levels the
+// x and y passes run on packed fp16 (v_pk_fma_f16: two channels per
+// instruction, on the loaded halves as they are; 11 significant bits against
+// the 8 the BF16 inputs keep) and the z pass in FP32.
+struct FeatState {
+    float p[3];     // texture-space position
+    int i0[3];      // first texel of the 4x4x4 level-2 block is i0 - 1
+    h2 fx, gx, fy, gy;
+    float fz, gz;
+};
+__device__ __forceinline__ void feat_setup(const float4 g0, const PyrView &pv, const Extent &ext, FeatState &F) {
+    const float gp[3] = {g0.x, g0.y, g0.z};
+    for (int k = 0; k < 3; ++k) F.p[k] = ((gp[k] - ext.center[k]) - ext.vmin[k]) / (ext.vmax[k] - ext.vmin[k]);
+    const int s2 = pv.lv[2].size;
+    float f[3];
+    for (int k = 0; k < 3; ++k) {
+        const float u = F.p[k] * (float)s2 - 0.5f;
+        const float fl = floorf(u);
+        F.i0[k] = (int)fl;
+        f[k] = u - fl;
+    }
+    const _Float16 fx = (_Float16)f[0], gx = (_Float16)(1.0f - f[0]);
+    const _Float16 fy = (_Float16)f[1], gy = (_Float16)(1.0f - f[1]);
+    F.fx = h2{fx, fx};
+    F.gx = h2{gx, gx};
+    F.fy = h2{fy, fy};
+    F.gy = h2{gy, gy};
+    F.fz = f[2];
+    F.gz = 1.0f - f[2];
+}
+// z slice zs of the block, interpolated in x and y: ycur[oy * 3 + ox]
+__device__ __forceinline__ void feat_slice(const PyrView &pv, const FeatState &F, int zs, h4 ycur[9]) {
+    const LevelInfo &L2 = pv.lv[2];
+#pragma unroll
+    for (int ys = 0; ys < 4; ++ys) {
+        h4 t[4];
+        row4_h(pv, L2, F.i0[0] - 1, F.i0[1] - 1 + ys, F.i0[2] - 1 + zs, t);
+#pragma unroll
+        for (int ox = 0; ox < 3; ++ox) {
+            h4 xv;
+            xv.rg = __builtin_elementwise_fma(t[ox + 1].rg, F.fx, t[ox].rg * F.gx);
+            xv.ba = __builtin_elementwise_fma(t[ox + 1].ba, F.fx, t[ox].ba * F.gx);
+            if (ys <= 2) {
+                ycur[ys * 3 + ox].rg = xv.rg * F.gy;
+                ycur[ys * 3 + ox].ba = xv.ba * F.gy;
+            }
+            if (ys >= 1) {
+                h4 &yv = ycur[(ys - 1) * 3 + ox];
+                yv.rg = __builtin_elementwise_fma(xv.rg, F.fy, yv.rg);
+                yv.ba = __builtin_elementwise_fma(xv.ba, F.fy, yv.ba);
+            }
+        }
+    }
+}
+// the nine stencil samples between two slices as 18 BF16x2 dwords (RG, BA)
+__device__ __forceinline__ void feat_zpass(const FeatState &F, const h4 yprev[9], const h4 ycur[9], uint32_t d[18]) {
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+        // fmaf(a, b, +0) is the rounded product a * b (both factors >= 0), written
+        // as an FMA so that it is one v_fma_mix_f32 on the fp16 half
+        const float vx = fmaf((float)ycur[j].rg.x, F.fz, fmaf((float)yprev[j].rg.x, F.gz, 0.0f));
+        const float vy = fmaf((float)ycur[j].rg.y, F.fz, fmaf((float)yprev[j].rg.y, F.gz, 0.0f));
+        const float vz = fmaf((float)ycur[j].ba.x, F.fz, fmaf((float)yprev[j].ba.x, F.gz, 0.0f));
+        const float vw = fmaf((float)ycur[j].ba.y, F.fz, fmaf((float)yprev[j].ba.y, F.gz, 0.0f));
+        d[2 * j] = bf2pack(vx, vy);
+        d[2 * j + 1] = bf2pack(vz, vw);
+    }
+}
+// inputs 108 .. 129 (gather axis, position encoding, one zero) as 11 dwords
+__device__ __forceinline__ void feat_tail(const float p[3], const float4 g3, uint32_t d[11]) {
+    const float b[3] = {g3.x, g3.y, g3.z};
+    const float bl = sqrtf(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+    float e[22];
+    for (int k = 0; k < 3; ++k) e[k] = bl > 0.f ? b[k] / bl : 0.f;
+    int o = 3;
+    // sin/cos(pi f p) = v_sin/v_cos of f p / 2 revolutions: absolute error ~1e-6, far below the BF16 step
+    for (int c = 0; c < 3; ++c)
+        for (int fq = 1; fq <= 4; fq <<= 1) {
+            const float rev = 0.5f * (float)fq * p[c];
+            e[o++] = __builtin_amdgcn_sinf(rev);
+            e[o++] = __builtin_amdgcn_cosf(rev);
+        }
+    e[21] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 11; ++k) d[k] = bf2pack(e[2 * k], e[2 * k + 1]);
+}
+
 // Writes the 160 inputs of one G-buffer texel (g0 = position, g3 = B) as 80
 // packed BF16 pairs to row32 (global or LDS).  FMT: level 2 stored as fp16
-// pairs (0) or any other element format (1).  For fp16 levels the x and y
-// passes run on packed fp16 (v_pk_fma_f16: two channels per instruction, on
-// the loaded halves as they are; 11 significant bits against the 8 the BF16
-// inputs keep) and the z pass in FP32 (v_fma_mix_f32 on the fp16 halves).
+// pairs (0) or any other element format (1, all passes in FP32).
 template <int FMT>
 __device__ __forceinline__ void make_features(const float4 g0, const float4 g3, const PyrView &pv, const Extent &ext,
                                               uint32_t *row32) {
+    if (FMT == 0) {
+        FeatState F;
+        feat_setup(g0, pv, ext, F);
+        h4 yprev[9], ycur[9];
+        uint32_t d[18];
+        for (int zs = 0; zs < 4; ++zs) {
+            feat_slice(pv, F, zs, ycur);
+            if (zs >= 1) {  // stencils with iz = zs - 2 from slices zs - 1 and zs
+                feat_zpass(F, yprev, ycur, d);
+#pragma unroll
+                for (int k = 0; k < 18; ++k) row32[(zs - 1) * 18 + k] = d[k];
+            }
+#pragma unroll
+            for (int j = 0; j < 9; ++j) yprev[j] = ycur[j];
+        }
+        uint32_t e[11];
+        feat_tail(F.p, g3, e);
+#pragma unroll
+        for (int k = 0; k < 11; ++k) row32[54 + k] = e[k];
+#pragma unroll
+        for (int k = 65; k < kMlpIn / 2; ++k) row32[k] = 0u;
+        return;
+    }
     float p[3];
     const float gp[3] = {g0.x, g0.y, g0.z};
     for (int k = 0; k < 3; ++k) p[k] = ((gp[k] - ext.center[k]) - ext.vmin[k]) / (ext.vmax[k] - ext.vmin[k]);
@@ -227,97 +335,43 @@ __device__ __forceinline__ void make_features(const float4 g0, const float4 g3, 
         i0[k] = (int)fl;
         f[k] = u - fl;
     }
-    if (FMT == 0) {
-        const _Float16 fx = (_Float16)f[0], gx = (_Float16)(1.0f - f[0]);
-        const _Float16 fy = (_Float16)f[1], gy = (_Float16)(1.0f - f[1]);
-        const h2 fx2 = {fx, fx}, gx2 = {gx, gx}, fy2 = {fy, fy}, gy2 = {gy, gy};
-        const float fz = f[2], gz = 1.0f - f[2];
-        h4 yprev[9], ycur[9];  // y-interpolated values of the previous / current z slice, [oy][ox]
-        for (int zs = 0; zs < 4; ++zs) {
-            for (int ys = 0; ys < 4; ++ys) {
-                h4 t[4];
-                row4_h(pv, L2, i0[0] - 1, i0[1] - 1 + ys, i0[2] - 1 + zs, t);
+    float4 yprev[9], ycur[9];
+    for (int zs = 0; zs < 4; ++zs) {
+        for (int ys = 0; ys < 4; ++ys) {
+            float4 t[4];
+            row4_f32(pv, L2, i0[0] - 1, i0[1] - 1 + ys, i0[2] - 1 + zs, t);
 #pragma unroll
-                for (int ox = 0; ox < 3; ++ox) {
-                    h4 xv;
-                    xv.rg = __builtin_elementwise_fma(t[ox + 1].rg, fx2, t[ox].rg * gx2);
-                    xv.ba = __builtin_elementwise_fma(t[ox + 1].ba, fx2, t[ox].ba * gx2);
-                    if (ys <= 2) {
-                        ycur[ys * 3 + ox].rg = xv.rg * gy2;
-                        ycur[ys * 3 + ox].ba = xv.ba * gy2;
-                    }
-                    if (ys >= 1) {
-                        h4 &yv = ycur[(ys - 1) * 3 + ox];
-                        yv.rg = __builtin_elementwise_fma(xv.rg, fy2, yv.rg);
-                        yv.ba = __builtin_elementwise_fma(xv.ba, fy2, yv.ba);
-                    }
+            for (int ox = 0; ox < 3; ++ox) {
+                const float4 xv = lerp4(t[ox], t[ox + 1], f[0]);
+                if (ys <= 2) {
+                    const float g = 1.0f - f[1];
+                    ycur[ys * 3 + ox] = make_float4(xv.x * g, xv.y * g, xv.z * g, xv.w * g);
+                }
+                if (ys >= 1) {
+                    float4 &yv = ycur[(ys - 1) * 3 + ox];
+                    yv.x += xv.x * f[1];
+                    yv.y += xv.y * f[1];
+                    yv.z += xv.z * f[1];
+                    yv.w += xv.w * f[1];
                 }
             }
-            if (zs >= 1) {  // stencils with iz = zs - 2 from slices zs - 1 and zs
-                const int base = (zs - 1) * 9;
-#pragma unroll
-                for (int j = 0; j < 9; ++j) {
-                    const float vx = fmaf((float)ycur[j].rg.x, fz, (float)yprev[j].rg.x * gz);
-                    const float vy = fmaf((float)ycur[j].rg.y, fz, (float)yprev[j].rg.y * gz);
-                    const float vz = fmaf((float)ycur[j].ba.x, fz, (float)yprev[j].ba.x * gz);
-                    const float vw = fmaf((float)ycur[j].ba.y, fz, (float)yprev[j].ba.y * gz);
-                    row32[(base + j) * 2 + 0] = bf2pack(vx, vy);
-                    row32[(base + j) * 2 + 1] = bf2pack(vz, vw);
-                }
-            }
-#pragma unroll
-            for (int j = 0; j < 9; ++j) yprev[j] = ycur[j];
         }
-    } else {
-        float4 yprev[9], ycur[9];
-        for (int zs = 0; zs < 4; ++zs) {
-            for (int ys = 0; ys < 4; ++ys) {
-                float4 t[4];
-                row4_f32(pv, L2, i0[0] - 1, i0[1] - 1 + ys, i0[2] - 1 + zs, t);
+        if (zs >= 1) {
+            const int base = (zs - 1) * 9;
 #pragma unroll
-                for (int ox = 0; ox < 3; ++ox) {
-                    const float4 xv = lerp4(t[ox], t[ox + 1], f[0]);
-                    if (ys <= 2) {
-                        const float g = 1.0f - f[1];
-                        ycur[ys * 3 + ox] = make_float4(xv.x * g, xv.y * g, xv.z * g, xv.w * g);
-                    }
-                    if (ys >= 1) {
-                        float4 &yv = ycur[(ys - 1) * 3 + ox];
-                        yv.x += xv.x * f[1];
-                        yv.y += xv.y * f[1];
-                        yv.z += xv.z * f[1];
-                        yv.w += xv.w * f[1];
-                    }
-                }
+            for (int j = 0; j < 9; ++j) {
+                const float4 v = lerp4(yprev[j], ycur[j], f[2]);
+                row32[(base + j) * 2 + 0] = bf2pack(v.x, v.y);
+                row32[(base + j) * 2 + 1] = bf2pack(v.z, v.w);
             }
-            if (zs >= 1) {
-                const int base = (zs - 1) * 9;
-#pragma unroll
-                for (int j = 0; j < 9; ++j) {
-                    const float4 v = lerp4(yprev[j], ycur[j], f[2]);
-                    row32[(base + j) * 2 + 0] = bf2pack(v.x, v.y);
-                    row32[(base + j) * 2 + 1] = bf2pack(v.z, v.w);
-                }
-            }
-#pragma unroll
-            for (int j = 0; j < 9; ++j) yprev[j] = ycur[j];
         }
+#pragma unroll
+        for (int j = 0; j < 9; ++j) yprev[j] = ycur[j];
     }
-    const float b[3] = {g3.x, g3.y, g3.z};
-    const float bl = sqrtf(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
-    float e[22];  // inputs 108 .. 129, then zero padding to an even count
-    for (int k = 0; k < 3; ++k) e[k] = bl > 0.f ? b[k] / bl : 0.f;
-    int o = 3;
-    // sin/cos(pi f p) = v_sin/v_cos of f p / 2 revolutions: absolute error ~1e-6, far below the BF16 step
-    for (int c = 0; c < 3; ++c)
-        for (int fq = 1; fq <= 4; fq <<= 1) {
-            const float rev = 0.5f * (float)fq * p[c];
-            e[o++] = __builtin_amdgcn_sinf(rev);
-            e[o++] = __builtin_amdgcn_cosf(rev);
-        }
-    e[21] = 0.0f;
+    uint32_t e[11];
+    feat_tail(p, g3, e);
 #pragma unroll
-    for (int k = 0; k < 11; ++k) row32[54 + k] = bf2pack(e[2 * k], e[2 * k + 1]);
+    for (int k = 0; k < 11; ++k) row32[54 + k] = e[k];
 #pragma unroll
     for (int k = 65; k < kMlpIn / 2; ++k) row32[k] = 0u;
 }
@@ -607,6 +661,46 @@ __device__ __forceinline__ void mlp_layer_t(f32x4 (&acc)[NT][4], const uint16_t 
     }
 }
 
+// acc[nt][mt] := the bias of rows 16 nt + 4 (lane / 16) .. +3 (the rows this
+// lane's part of the 16x16 FP32 tile holds), so the layer's MFMAs add the
+// products onto it.
+__device__ __forceinline__ void init_bias_t(f32x4 (&acc)[4][4], const float *bias) {
+    const int q = ((threadIdx.x & 63) >> 4) * 4;
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+        const int n0 = nt * 16 + q;
+        const f32x4 b = f32x4{bias[n0], bias[n0 + 1], bias[n0 + 2], bias[n0 + 3]};
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) acc[nt][mt] = b;
+    }
+}
+// ReLU -> BF16 hidden activations [pixel][unit], stride kAs, from accumulators
+// that already hold the bias.  ReLU is applied to the rounded BF16 values as a
+// signed 16-bit max with 0 (v_pk_max_i16): a negative BF16 value (or -0) has
+// its sign bit set, so it becomes +0, a positive one is kept; the same result
+// as rounding max(x, 0).
+__device__ __forceinline__ uint32_t relu_bf2(float a, float b) {
+    uint32_t r;
+    // kept as one instruction on the packed pair (the generic vector max makes
+    // the compiler unpack the pair again)
+    asm("v_pk_max_i16 %0, %1, 0" : "=v"(r) : "v"(bf2pack(a, b)));
+    return r;
+}
+__device__ __forceinline__ void store_relu_t(const f32x4 (&acc)[4][4], uint16_t *out) {
+    const int lane = threadIdx.x & 63;
+    const int q = (lane >> 4) * 4;
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+        const int n0 = nt * 16 + q;
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) {
+            const int m = mt * 16 + (lane & 15);
+            const f32x4 v = acc[nt][mt];
+            *(uint2 *)(out + m * kAs + n0) = make_uint2(relu_bf2(v[0], v[1]), relu_bf2(v[2], v[3]));
+        }
+    }
+}
+
 // bias + ReLU -> BF16 hidden activations [pixel][unit], stride kAs
 __device__ __forceinline__ void store_hidden_t(const f32x4 (&acc)[4][4], const float *bias, uint16_t *out) {
     const int lane = threadIdx.x & 63;
@@ -704,6 +798,160 @@ __global__ __launch_bounds__(64) void k_learned(MlpDev W, TraceLaunch t, PyrArgs
     }
 }
 
+// Streamed inference for fp16-pair level 2 (fp16 and fp8 storage).  The 160
+// inputs never sit in LDS whole: each 32-input chunk goes to a small buffer as
+// soon as the feature pass has produced it and layer 1 takes its MFMA k-step
+// on it right away; dwords that run past a chunk wait in registers until the
+// buffer is free.  The chunk buffer is then reused for the hidden
+// activations.  About 10 KiB of LDS per wavefront instead of 22 KiB: more
+// wavefronts per CU to hide the latency of the feature loads and of the
+// layer-to-layer LDS round trips.  The inputs, weights and arithmetic are
+// those of k_learned.
+constexpr int kCs = 40;  // chunk row stride in BF16 (32 inputs + padding)
+struct StreamSmem {
+    union {
+        uint16_t chunk[64 * kCs];
+        uint16_t hid[64 * kAs];
+    };
+    float y[64 * 4];
+};
+
+// acc[nt][mt] += W[16 nt .. +15][kw .. kw+31] * X[16 mt .. +15][0 .. 31]^T
+__device__ __forceinline__ void mlp_kstep(f32x4 (&acc)[4][4], const uint16_t *W, int ldw, int kw, const uint16_t *X,
+                                          int ldx) {
+    const int lane = threadIdx.x & 63;
+    const int r = lane & 15, kq = (lane >> 4) * 8;
+    bf16x8 b[4];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) b[mt] = *(const bf16x8 *)(X + (mt * 16 + r) * ldx + kq);
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+        const bf16x8 a = *(const bf16x8 *)(W + (size_t)(nt * 16 + r) * ldw + kw + kq);
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt)
+            acc[nt][mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[mt], acc[nt][mt], 0, 0, 0);
+    }
+}
+
+// dwords src[0 .. n) of this lane's row to chunk positions [o, o + n)
+template <int N>
+__device__ __forceinline__ void chunk_put(uint32_t *row, int o, const uint32_t *src) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) row[o + k] = src[k];
+}
+
+template <int MODE>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_learned_stream(MlpDev W, TraceLaunch t, PyrArgs pa, PyrView pv,
+                                                       float threshold, uint32_t *fallback_count) {
+    __shared__ __align__(16) StreamSmem S;
+    const vct_gbuffer_texel *__restrict__ gbuf = t.gbuf;
+    const int width = t.width, height = t.height;
+    const int lane = threadIdx.x;
+    const int tiles_x = (width + 7) / 8;
+    const int tile = blockIdx.x;
+    const int px = (tile % tiles_x) * 8 + (lane & 7), py = (tile / tiles_x) * 8 + (lane >> 3);
+    const bool inside = px < width && py < height;
+    bool covered = false;
+    float4 g0 = make_float4(0.f, 0.f, 0.f, 0.f), g3 = g0;
+    if (inside) {
+        const float4 *g = (const float4 *)&gbuf[(size_t)py * width + px];
+        g0 = g[0];
+        g3 = g[3];
+        covered = __float_as_uint(g[2].w) != 0u;
+    }
+    uint32_t *row = (uint32_t *)(S.chunk + lane * kCs);
+    f32x4 acc[4][4];
+    init_bias_t(acc, bias_ptr(W.master, 0));
+    FeatState F;
+    if (covered) feat_setup(g0, pv, t.ext, F);
+    h4 yprev[9], ycur[9];
+    uint32_t d[18];
+    // input dwords 0 .. 53 come from the z passes (18 per slice pair), 54 .. 64
+    // from the tail, 65 .. 79 are zero; chunk c holds dwords 16 c .. 16 c + 15
+    for (int zs = 0; zs < 4; ++zs) {
+        if (covered) {
+            feat_slice(pv, F, zs, ycur);
+            if (zs >= 1) feat_zpass(F, yprev, ycur, d);
+#pragma unroll
+            for (int j = 0; j < 9; ++j) yprev[j] = ycur[j];
+        } else {
+#pragma unroll
+            for (int k = 0; k < 18; ++k) d[k] = 0u;
+        }
+        if (zs == 0) continue;
+        // slice pair c = zs - 1 gives dwords 18 c .. 18 c + 17.  Chunk c already
+        // holds its first 2 c dwords (carried from the previous pair); d[0 .. 16 - 2c)
+        // complete it and d[16 - 2c .. 18) are the first 2 c + 2 dwords of chunk c + 1.
+        const int c = zs - 1;
+        if (c == 0) chunk_put<16>(row, 0, d);
+        if (c == 1) chunk_put<14>(row, 2, d);
+        if (c == 2) chunk_put<12>(row, 4, d);
+        __syncthreads();
+        mlp_kstep(acc, W.w1, kMlpIn, c * 32, S.chunk, kCs);
+        __syncthreads();
+        if (c == 0) chunk_put<2>(row, 0, d + 16);
+        if (c == 1) chunk_put<4>(row, 0, d + 14);
+        if (c == 2) chunk_put<6>(row, 0, d + 12);
+    }
+    uint32_t e[11];
+    if (covered) {
+        feat_tail(F.p, g3, e);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 11; ++k) e[k] = 0u;
+    }
+    chunk_put<10>(row, 6, e);  // dwords 54 .. 63 complete chunk 3
+    __syncthreads();
+    mlp_kstep(acc, W.w1, kMlpIn, 96, S.chunk, kCs);
+    __syncthreads();
+    row[0] = e[10];  // dword 64, then the zero padding: chunk 4
+#pragma unroll
+    for (int k = 1; k < 16; ++k) row[k] = 0u;
+    __syncthreads();
+    mlp_kstep(acc, W.w1, kMlpIn, 128, S.chunk, kCs);
+    __syncthreads();  // every lane's chunk reads are done before the activations overwrite them
+    store_relu_t(acc, S.hid);
+    __syncthreads();
+    for (int layer = 1; layer <= 2; ++layer) {
+        init_bias_t(acc, bias_ptr(W.master, layer));
+        mlp_layer_t<4>(acc, layer == 1 ? W.w2 : W.w3, 64, S.hid, kAs, 2);
+        __syncthreads();
+        store_relu_t(acc, S.hid);
+        __syncthreads();
+    }
+    f32x4 o[1][4];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) o[0][mt] = f32x4{0.f, 0.f, 0.f, 0.f};
+    mlp_layer_t<1>(o, W.w4, 64, S.hid, kAs, 2);
+    if (lane < 16) {
+        const float *b4 = bias_ptr(W.master, 3);
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) {
+            const int m = mt * 16 + lane;
+#pragma unroll
+            for (int n = 0; n < 3; ++n) S.y[m * 4 + n] = (o[0][mt][n] + b4[n]) * W.out_scale;
+        }
+    }
+    __syncthreads();
+    const float r = covered ? S.y[lane * 4 + 0] : 0.f, gch = covered ? S.y[lane * 4 + 1] : 0.f,
+                bch = covered ? S.y[lane * 4 + 2] : 0.f;
+    const float lum = 0.2126f * r + 0.7152f * gch + 0.0722f * bch;
+    float s1 = covered ? lum : 0.f, s2 = covered ? lum * lum : 0.f, cnt = covered ? 1.f : 0.f;
+    for (int off = 32; off > 0; off >>= 1) {
+        s1 += __shfl_xor(s1, off, 64);
+        s2 += __shfl_xor(s2, off, 64);
+        cnt += __shfl_xor(cnt, off, 64);
+    }
+    const float mean = cnt > 0.f ? s1 / cnt : 0.f;
+    const float var = cnt > 0.f ? s2 / cnt - mean * mean : 0.f;
+    if (var > threshold) {
+        if (lane == 0) atomicAdd(fallback_count, 1u);
+        if (inside) trace_pixel<MODE, false>(t, pa, pyr_rsrc(pa), px, py, false);
+    } else if (inside) {
+        t.out_ind[(size_t)py * width + px] = make_float4(r, gch, bch, covered ? 1.0f : 0.0f);
+    }
+}
+
 // ---------------------------------------------------------------- launchers
 static PyrView pyr_view(const PyrDesc &pyr) {
     PyrView v;
@@ -777,11 +1025,13 @@ hipError_t launch_learned(const MlpBuffers &B, const PyrDesc &pyr, const TraceLa
     const PyrArgs pa = pyr_args(pyr);
     const PyrView pv = pyr_view(pyr);
     if (mode == 0)
-        hipLaunchKernelGGL(k_learned<0>, dim3(tiles), dim3(64), 0, s, W, t, pa, pv, threshold, fallback_count);
+        hipLaunchKernelGGL(k_learned_stream<0>, dim3(tiles), dim3(64), 0, s, W, t, pa, pv, threshold,
+                           fallback_count);
     else if (mode == 1)
         hipLaunchKernelGGL(k_learned<1>, dim3(tiles), dim3(64), 0, s, W, t, pa, pv, threshold, fallback_count);
     else
-        hipLaunchKernelGGL(k_learned<2>, dim3(tiles), dim3(64), 0, s, W, t, pa, pv, threshold, fallback_count);
+        hipLaunchKernelGGL(k_learned_stream<2>, dim3(tiles), dim3(64), 0, s, W, t, pa, pv, threshold,
+                           fallback_count);
     return hipGetLastError();
 }
 
