@@ -133,6 +133,15 @@ typedef struct vct_voxelize_params {
 typedef struct vct_inject_params {
     float voxel_set_opacity; /* voxelSetOpacity (Application.h:77), 0.5          */
     uint32_t light_mask;     /* bit i: light i injects (shadow casters only)       */
+    /* temporalFilterRadiance / temporalDecay (Application.h:75-76; off, 0.8):
+     * radiance level 0 is not cleared and every voxel not lit this frame
+     * becomes mix(previous, (0,0,0,a), 1 - decay) (transferVoxels.comp:55-62,
+     * Application.cpp:762-764); lit voxels are overwritten with their albedo as
+     * without the filter (injectRadiance.comp:99-101: the filtered branch
+     * belongs to radianceLighting, which is off).  The previous value is the
+     * stored level 0 of the last frame (fp16 or fp32, per the storage format). */
+    int temporal_filter;
+    float temporal_decay;
 } vct_inject_params;
 
 /* Mip chain (src/Application.cpp:877-921, shaders/filterRadiance.comp). */

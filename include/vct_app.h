@@ -43,6 +43,11 @@ typedef struct vct_app_config {
     int fill_holes;     /* voxelFillHoles, -1 = default (0, Application.h:95)   */
     int device;         /* HIP device; -1 builds the scene and uniforms only
                            (no GPU context: render calls fail)                */
+    int temporal_filter; /* temporalFilterRadiance (Application.h:75), 0       */
+    float temporal_decay;/* temporalDecay (Application.h:76), 0.8              */
+    int track_camera;    /* voxelTrackCamera (Application.h:86), 0: the grid
+                            centre follows the eye in steps of
+                            2^levels * (max - min) / dim (Application.cpp:187-191) */
 } vct_app_config;
 
 int vct_app_default_config(int scene, vct_app_config *out);

@@ -82,6 +82,8 @@ def lib() -> C.CDLL:
                                    C.POINTER(Stats)]),
             "or_normalize_inject": (None, [vp, C.c_int, vp, vp, vp, vp, C.c_int, vp, C.c_uint32, C.c_float,
                                            C.c_int, vp, vp, C.POINTER(Stats)]),
+            "or_normalize_inject_temporal": (None, [vp, C.c_int, vp, vp, vp, vp, C.c_int, vp, C.c_uint32, C.c_float,
+                                                    C.c_float, C.c_int, vp, vp, vp, C.POINTER(Stats)]),
             "or_fill_holes": (None, [vp, vp, C.c_int, C.c_int]),
             "or_mip": (None, [vp, C.c_int, C.c_int, C.c_int]),
             "or_mip_aniso": (None, [vp, vp, C.c_int, C.c_int, C.c_int]),
@@ -219,7 +221,9 @@ def voxelize(sc: SceneArrays, lights, shadowmaps, vox, dim: int, msaa=None, ligh
 
 
 def normalize_inject(albedo: np.ndarray, dim, center, vmin, vmax, lights, shadowmaps, light_mask=0xffffffff,
-                     opacity=0.5, fp32=False):
+                     opacity=0.5, fp32=False, prev=None, decay=0.8):
+    """prev (float4 per voxel, last frame's stored level 0) selects the
+    temporal filter with the given decay."""
     alb = np.ascontiguousarray(albedo, np.uint32).copy().reshape(-1)
     c, a, b = (np.asarray(v, np.float32) for v in (center, vmin, vmax))
     la = _light_array(lights)
@@ -228,9 +232,16 @@ def normalize_inject(albedo: np.ndarray, dim, center, vmin, vmax, lights, shadow
     rad = np.zeros((dim ** 3, 4), np.float32)
     lm = np.zeros(dim ** 3, np.uint8)
     st = Stats()
-    lib().or_normalize_inject(_p(alb), dim, _p(c), _p(a), _p(b), C.cast(la, C.c_void_p), len(lights),
-                              C.cast(ptrs, C.c_void_p), light_mask, opacity, 1 if fp32 else 0, _p(rad), _p(lm),
-                              C.byref(st))
+    if prev is None:
+        lib().or_normalize_inject(_p(alb), dim, _p(c), _p(a), _p(b), C.cast(la, C.c_void_p), len(lights),
+                                  C.cast(ptrs, C.c_void_p), light_mask, opacity, 1 if fp32 else 0, _p(rad), _p(lm),
+                                  C.byref(st))
+    else:
+        pv = np.ascontiguousarray(prev, np.float32).reshape(-1, 4)
+        assert pv.shape[0] == dim ** 3
+        lib().or_normalize_inject_temporal(_p(alb), dim, _p(c), _p(a), _p(b), C.cast(la, C.c_void_p), len(lights),
+                                           C.cast(ptrs, C.c_void_p), light_mask, opacity, decay,
+                                           1 if fp32 else 0, _p(pv), _p(rad), _p(lm), C.byref(st))
     return alb.reshape(dim, dim, dim), rad.reshape(dim, dim, dim, 4), lm, st
 
 

@@ -371,10 +371,9 @@ void or_voxelize(const or_scene *sc, const or_light *lights, int nlights, const 
 /* normalize (transferVoxels) + inject                                       */
 static float q16(float x) { return or_half_to_float(or_float_to_half(x)); }
 
-void or_normalize_inject(uint32_t *albedo, int dim, const float center[3], const float vmin[3],
-                         const float vmax[3], const or_light *lights, int nlights,
-                         const float *const *shadowmaps, uint32_t light_mask, float opacity,
-                         int storage_fp32, float *rad0, uint8_t *litmask, or_stats *stats) {
+static void inject_litmask(int dim, const float center[3], const float vmin[3], const float vmax[3],
+                           const or_light *lights, int nlights, const float *const *shadowmaps,
+                           uint32_t light_mask, uint8_t *litmask) {
     size_t nv = (size_t)dim * dim * dim;
     memset(litmask, 0, nv);
     /* injectRadiance.comp:152-168 per shadow texel, as a lit-mask scatter */
@@ -404,6 +403,14 @@ void or_normalize_inject(uint32_t *albedo, int dim, const float center[3], const
                 litmask[vi] = 1;
             }
     }
+}
+
+void or_normalize_inject(uint32_t *albedo, int dim, const float center[3], const float vmin[3],
+                         const float vmax[3], const or_light *lights, int nlights,
+                         const float *const *shadowmaps, uint32_t light_mask, float opacity,
+                         int storage_fp32, float *rad0, uint8_t *litmask, or_stats *stats) {
+    size_t nv = (size_t)dim * dim * dim;
+    inject_litmask(dim, center, vmin, vmax, lights, nlights, shadowmaps, light_mask, litmask);
     /* transferVoxels.comp:37-62 (+ normalizeVoxels.comp:101-105 for rgba16f
      * radiance) and injectRadiance.comp:186,212 per voxel */
     for (size_t vi = 0; vi < nv; ++vi) {
@@ -429,6 +436,50 @@ void or_normalize_inject(uint32_t *albedo, int dim, const float center[3], const
         if (!storage_fp32)
             for (int k = 0; k < 4; ++k) r[k] = q16(r[k]);
     }
+}
+
+/* The temporalFilterRadiance path in the reference's own two steps: level 0
+ * is not cleared (Application.cpp:762-764); transferVoxels.comp:37-62 with
+ * temporalFilter stores mix(previous, (0,0,0,a'), 1 - decay) into every voxel;
+ * then injectRadiance.comp:99-101 overwrites every lit voxel with
+ * packUnorm4x8(voxelColor[v]), i.e. (albedo, a') or 0 for an empty voxel.
+ * prev: last frame's stored level 0, float4 per voxel. */
+void or_normalize_inject_temporal(uint32_t *albedo, int dim, const float center[3], const float vmin[3],
+                                  const float vmax[3], const or_light *lights, int nlights,
+                                  const float *const *shadowmaps, uint32_t light_mask, float opacity,
+                                  float decay, int storage_fp32, const float *prev, float *rad0,
+                                  uint8_t *litmask, or_stats *stats) {
+    size_t nv = (size_t)dim * dim * dim;
+    inject_litmask(dim, center, vmin, vmax, lights, nlights, shadowmaps, light_mask, litmask);
+    const float a = 1.0f - decay; /* GLSL mix(x, y, a) = x * (1 - a) + y * a */
+    for (size_t vi = 0; vi < nv; ++vi) { /* transfer */
+        float c[4];
+        or_unpack_unorm4x8(albedo[vi], c);
+        if (c[3] > 0.0f) {
+            stats->unique_voxels += 1;
+            uint32_t mf = (uint32_t)(255.0f * c[3]);
+            if (mf > stats->max_fragments_per_voxel) stats->max_fragments_per_voxel = mf;
+            if (opacity > 0.0f) c[3] = opacity;
+            albedo[vi] = or_pack_unorm4x8(c);
+        }
+        c[0] = c[1] = c[2] = 0.0f;
+        for (int k = 0; k < 4; ++k) rad0[vi * 4 + k] = prev[vi * 4 + k] * (1.0f - a) + c[k] * a;
+    }
+    for (size_t vi = 0; vi < nv; ++vi) { /* inject: lit voxels take their (rewritten) albedo */
+        if (!litmask[vi]) continue;
+        float c[4];
+        or_unpack_unorm4x8(albedo[vi], c);
+        if (c[3] > 0.0f) {
+            stats->lit_voxels += 1;
+            /* the alpha is a' itself, as in or_normalize_inject: the fp16/fp32
+             * radiance keeps it, where the reference's rgba8 radiance holds
+             * round(255 a') / 255 */
+            if (opacity > 0.0f) c[3] = opacity;
+        }
+        for (int k = 0; k < 4; ++k) rad0[vi * 4 + k] = c[k];
+    }
+    if (!storage_fp32)
+        for (size_t i = 0; i < nv * 4; ++i) rad0[i] = q16(rad0[i]);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -22,6 +22,9 @@
  *       exact rational raster on tiny inputs (tests/test_oracle_raster.py)
  *   or_normalize_inject                                     per-voxel closed form,
  *       brute-force per-texel unprojection, idempotence
+ *   or_normalize_inject_temporal                            decay 0 = the plain pass,
+ *       decay 1 keeps unlit voxels, geometric decay on a static scene,
+ *       lit voxels = the plain pass
  *   or_fill_holes                                           brute force 3x3x3
  *   or_mip                                                  constant/ramp closed
  *       forms, mean preservation, numpy float16 rounding
@@ -120,6 +123,15 @@ void or_normalize_inject(uint32_t *albedo, int dim, const float center[3],
                          const or_light *lights, int nlights, const float *const *shadowmaps,
                          uint32_t light_mask, float opacity, int storage_fp32,
                          float *rad0, uint8_t *litmask, or_stats *stats);
+
+/* The same with temporalFilterRadiance (Application.h:75-76): mix with the
+ * previous level 0 `prev` (float4 per voxel, as stored), then lit voxels
+ * overwritten (transferVoxels.comp:55-62, injectRadiance.comp:99-101). */
+void or_normalize_inject_temporal(uint32_t *albedo, int dim, const float center[3],
+                                  const float vmin[3], const float vmax[3],
+                                  const or_light *lights, int nlights, const float *const *shadowmaps,
+                                  uint32_t light_mask, float opacity, float decay, int storage_fp32,
+                                  const float *prev, float *rad0, uint8_t *litmask, or_stats *stats);
 
 /* voxelFillHoles.comp (shaders/voxelFillHoles.comp:8-36) on a float4 grid whose
  * values are quantised by `quant` (0 = fp16 rounding, 1 = none). */

@@ -10,6 +10,12 @@
                alpha_n = 1 - (1 - a)^n, c_n = c (1 - (1 - a)^n) / a with n from
                the h_{k+1} = h_k (1 + tan(theta/2)) schedule; FP32 vs FP64
   G-buffer     one quad facing the camera: positions are ray-plane hits
+  temporal     decay 0 = the plain pass bit for bit; decay 1 keeps every unlit
+               voxel; lit voxels always equal the plain pass; on a static scene
+               the alpha of an unlit occupied voxel follows a (1 - decay^k)
+  camera snap  voxelTrackCamera: centre = floor(eye / cell) * cell with
+               cell = 2^levels (max - min) / dim, seen through the voxel
+               projections the application derives from it
 """
 import math
 
@@ -74,6 +80,68 @@ def test_inject_is_idempotent(oracle, scene1):
     a1, r1, l1, _ = oracle.normalize_inject(s["alb"], 64, *s["ext"], s["L"], [s["sm"]])
     a2, r2, l2, _ = oracle.normalize_inject(a1, 64, *s["ext"], s["L"], [s["sm"]])
     assert np.array_equal(r1, r2) and np.array_equal(l1, l2) and np.array_equal(a1, a2)
+
+
+def _inject(oracle, s, prev=None, decay=0.8):
+    return oracle.normalize_inject(s["alb"], 64, *s["ext"], s["L"], [s["sm"]], prev=prev, decay=decay)
+
+
+def test_temporal_decay_zero_is_the_plain_pass(oracle, scene1):
+    rng = np.random.default_rng(5)
+    prev = rng.uniform(0, 1, (64 ** 3, 4)).astype(np.float16).astype(np.float32)
+    a0, r0, l0, s0 = _inject(oracle, scene1)
+    a1, r1, l1, s1 = _inject(oracle, scene1, prev=prev, decay=0.0)
+    assert np.array_equal(r0.view(np.uint32), r1.view(np.uint32))
+    assert np.array_equal(a0, a1) and np.array_equal(l0, l1)
+    assert (s0.unique_voxels, s0.lit_voxels) == (s1.unique_voxels, s1.lit_voxels)
+
+
+def test_temporal_decay_one_keeps_unlit_voxels(oracle, scene1):
+    rng = np.random.default_rng(6)
+    prev = rng.uniform(0, 1, (64 ** 3, 4)).astype(np.float16).astype(np.float32)
+    _, plain, lm, _ = _inject(oracle, scene1)
+    _, r, lm2, _ = _inject(oracle, scene1, prev=prev, decay=1.0)
+    lit = lm.astype(bool)
+    r, plain = r.reshape(-1, 4), plain.reshape(-1, 4)
+    assert np.array_equal(r[~lit], prev[~lit])      # mix(prev, c, 0) = prev
+    assert np.array_equal(r[lit], plain[lit])       # lit voxels are overwritten
+    assert lit.sum() > 100
+
+
+def test_temporal_static_scene_converges_geometrically(oracle, scene1):
+    d = 0.8
+    _, plain, lm, _ = _inject(oracle, scene1)
+    plain = plain.reshape(-1, 4)
+    occ = (scene1["alb"].reshape(-1) >> 24) > 0
+    lit = lm.astype(bool)
+    prev = np.zeros((64 ** 3, 4), np.float32)
+    for k in range(1, 9):
+        _, r, _, _ = _inject(oracle, scene1, prev=prev, decay=d)
+        r = r.reshape(-1, 4)
+        a = r[occ & ~lit, 3].astype(np.float64)
+        # alpha_k = 0.5 (1 - d^k), each frame rounded to fp16 (relative 2^-11 per step)
+        assert np.allclose(a, 0.5 * (1 - d ** k), rtol=k * 2 ** -10, atol=0)
+        assert np.all(r[occ & ~lit, :3] == 0)       # never-lit voxels gain no colour
+        assert np.all(r[~occ & ~lit] == 0)
+        assert np.array_equal(r[lit], plain[lit])
+        prev = r
+
+
+def test_track_camera_snaps_the_grid_centre(oracle):
+    import vct_amd
+    a0 = vct_amd.App(1, device=-1)
+    a1 = vct_amd.App(1, device=-1, track_camera=1)
+    fs = a1.frame_state(0)
+    eye = np.asarray(list(fs["cam"].eye), np.float32)
+    e = np.float32(1 + 1 / 63)
+    cell = np.float32(2.0 ** 7) * (e - (-e)) / np.float32(64)
+    centre = np.floor(eye / cell) * cell
+    expect = oracle.voxel_projections(tuple(centre), (-e,) * 3, (e,) * 3)
+    got = [np.asarray(list(getattr(fs["vox"], m)), np.float32) for m in ("mvp_x", "mvp_y", "mvp_z")]
+    for g, x in zip(got, expect):
+        assert np.array_equal(g.reshape(-1), np.asarray(x, np.float32).reshape(-1))
+    base = [np.asarray(list(getattr(a0.frame_state(0)["vox"], m)), np.float32) for m in ("mvp_x",)]
+    assert np.any(centre != 0) == (not np.array_equal(base[0], got[0]))
 
 
 def test_fill_holes_brute_force(oracle):

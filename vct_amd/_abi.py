@@ -66,7 +66,8 @@ class VoxelizeParams(C.Structure):
 
 
 class InjectParams(C.Structure):
-    _fields_ = [("voxel_set_opacity", C.c_float), ("light_mask", C.c_uint32)]
+    _fields_ = [("voxel_set_opacity", C.c_float), ("light_mask", C.c_uint32), ("temporal_filter", C.c_int),
+                ("temporal_decay", C.c_float)]
 
 
 class MipParams(C.Structure):
@@ -111,7 +112,8 @@ class AppConfig(C.Structure):
     _fields_ = [("scene", C.c_int), ("seed", C.c_uint64), ("dim", C.c_int), ("levels", C.c_int),
                 ("storage", C.c_int), ("anisotropic", C.c_int), ("width", C.c_int), ("height", C.c_int),
                 ("shadow_size", C.c_int), ("lighting", C.c_int), ("msaa", C.c_int), ("reflections", C.c_int),
-                ("fill_holes", C.c_int), ("device", C.c_int)]
+                ("fill_holes", C.c_int), ("device", C.c_int), ("temporal_filter", C.c_int),
+                ("temporal_decay", C.c_float), ("track_camera", C.c_int)]
 
 
 class LearnedParams(C.Structure):

@@ -623,6 +623,8 @@ int vct_normalize_inject(vct_ctx *c, const vct_inject_params *p) {
     if (!c || !p) return fail(c, VCT_ERR_INVALID, "vct_normalize_inject: NULL argument");
     if (!(p->voxel_set_opacity >= 0.0f && p->voxel_set_opacity <= 1.0f))
         return fail(c, VCT_ERR_INVALID, "voxel_set_opacity %f out of [0, 1]", (double)p->voxel_set_opacity);
+    if (p->temporal_filter && !(p->temporal_decay >= 0.0f && p->temporal_decay <= 1.0f))
+        return fail(c, VCT_ERR_INVALID, "temporal_decay %f out of [0, 1]", (double)p->temporal_decay);
     if (!c->have_voxels) return fail(c, VCT_ERR_STATE, "vct_normalize_inject: voxelize first");
     PassTimer pt(c, P_RADIANCE);
     Extent ext;
@@ -631,8 +633,9 @@ int vct_normalize_inject(vct_ctx *c, const vct_inject_params *p) {
     memcpy(ext.vmax, c->desc.vmax, 12);
     LAUNCHCHK(c, launch_inject_scatter(c->hlights.data(), c->d_lights, (int)c->lights.size(), p->light_mask,
                                        c->desc.dim, ext, c->d_litmask, c->stream));
-    LAUNCHCHK(c, launch_normalize_inject(c->d_grid_albedo, c->d_litmask, c->desc.dim, p->voxel_set_opacity, c->pyr,
-                                         c->d_stats, c->d_partials, c->stream));
+    LAUNCHCHK(c, launch_normalize_inject(c->d_grid_albedo, c->d_litmask, c->desc.dim, p->voxel_set_opacity,
+                                         p->temporal_filter != 0, p->temporal_decay, c->pyr, c->d_stats,
+                                         c->d_partials, c->stream));
     c->have_radiance = true;
     c->have_mips = false;
     return VCT_OK;

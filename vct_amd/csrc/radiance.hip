@@ -70,10 +70,18 @@ hipError_t launch_inject_scatter(const DevLight *lights, const DevLight *dlights
 // writes are whole 512-byte (fp16) bricks.  The grid strides over the rows
 // with a fixed number of workgroups, each of which reduces its statistics in
 // LDS and writes one partial.
-template <int FMT>
+// TEMPORAL (temporalFilterRadiance): the row's previous level-0 values are read
+// before they are overwritten, and a voxel that is not lit this frame gets
+// mix(previous, (0,0,0,a), 1 - decay) = previous * (1 - w) + (0,0,0,a) * w with
+// w = 1 - decay (GLSL mix, transferVoxels.comp:55-58); a lit voxel is
+// overwritten with its albedo, (0,0,0,0) if it is empty (injectRadiance.comp:
+// 99-101 stores packUnorm4x8(voxelColor) there).
+template <int FMT, bool TEMPORAL>
 __global__ __launch_bounds__(256) void k_normalize_inject(uint32_t *albedo, const uint8_t *__restrict__ litmask,
-                                                          int dim, LevelInfo L0, float opacity, char *pyr,
-                                                          uint32_t *partials) {
+                                                          int dim, LevelInfo L0, float opacity, float decay,
+                                                          char *pyr, uint32_t *partials) {
+    const float mw = 1.0f - decay;  // the mix weight of this frame's value
+    const float pw = 1.0f - mw;     // and of the previous one, as GLSL mix computes it
     const int nb = L0.nb;
     const size_t nrows = (size_t)nb * nb * nb * 16;
     uint32_t uniq = 0, maxf = 0, lit_count = 0;
@@ -89,6 +97,20 @@ __global__ __launch_bounds__(256) void k_normalize_inject(uint32_t *albedo, cons
         uint32_t w[4] = {a.x, a.y, a.z, a.w};
         float r[4][4];
         bool changed = false;
+        char *row = pyr + L0.stage + (size_t)elem_index(L0, x0, y, z, 0) * 16;
+        float4 prev[4];
+        if (TEMPORAL) {  // the four texels of the row as stored last frame
+            if (FMT == 1) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) prev[v] = ((const float4 *)row)[v];
+            } else {  // fp16 pairs: element x0 holds texels x0, x0+1; element x0+2 texels x0+2, x0+3
+                const uint4 e0 = ((const uint4 *)row)[0], e2 = ((const uint4 *)row)[2];
+                prev[0] = unpack_h4(e0.x, e0.y);
+                prev[1] = unpack_h4(e0.z, e0.w);
+                prev[2] = unpack_h4(e2.x, e2.y);
+                prev[3] = unpack_h4(e2.z, e2.w);
+            }
+        }
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const uint32_t a32 = w[v];
@@ -108,13 +130,24 @@ __global__ __launch_bounds__(256) void k_normalize_inject(uint32_t *albedo, cons
                 r[v][1] = lit ? c1 : 0.0f;
                 r[v][2] = lit ? c2 : 0.0f;
                 r[v][3] = na;
+                if (TEMPORAL && !lit) {
+                    r[v][0] = prev[v].x * pw + 0.0f * mw;
+                    r[v][1] = prev[v].y * pw + 0.0f * mw;
+                    r[v][2] = prev[v].z * pw + 0.0f * mw;
+                    r[v][3] = prev[v].w * pw + na * mw;
+                }
             } else {
                 r[v][0] = r[v][1] = r[v][2] = r[v][3] = 0.0f;
+                if (TEMPORAL && ((lw >> (8 * v)) & 0xffu) == 0u) {
+                    r[v][0] = prev[v].x * pw + 0.0f * mw;
+                    r[v][1] = prev[v].y * pw + 0.0f * mw;
+                    r[v][2] = prev[v].z * pw + 0.0f * mw;
+                    r[v][3] = prev[v].w * pw + 0.0f * mw;
+                }
             }
         }
         if (changed) *(uint4 *)&albedo[gi] = make_uint4(w[0], w[1], w[2], w[3]);
         // the row's four elements are contiguous (level 0 is isotropic: one face)
-        char *row = pyr + L0.stage + (size_t)elem_index(L0, x0, y, z, 0) * 16;
         if (FMT == 1) {
             float4 *o = (float4 *)row;
 #pragma unroll
@@ -137,19 +170,26 @@ __global__ __launch_bounds__(256) void k_normalize_inject(uint32_t *albedo, cons
     block_partial(uniq, maxf, lit_count, 0u, partials);
 }
 
-hipError_t launch_normalize_inject(uint32_t *albedo, const uint8_t *litmask, int dim, float opacity,
-                                   const PyrDesc &pyr, vct_stats *stats, uint32_t *partials, hipStream_t s) {
+hipError_t launch_normalize_inject(uint32_t *albedo, const uint8_t *litmask, int dim, float opacity, bool temporal,
+                                   float decay, const PyrDesc &pyr, vct_stats *stats, uint32_t *partials,
+                                   hipStream_t s) {
     const LevelInfo &L0 = pyr.lv[0];
     const size_t nrows = (size_t)L0.nb * L0.nb * L0.nb * 16;
     size_t blocks = (nrows + 255) / 256;
     if (blocks > (size_t)kStatPartials) blocks = kStatPartials;
     dim3 grid((unsigned)blocks);
-    if (L0.fmt == 1)
-        hipLaunchKernelGGL(k_normalize_inject<1>, grid, dim3(256), 0, s, albedo, litmask, dim, L0, opacity, pyr.pyr,
-                           partials);
+    if (L0.fmt == 1 && temporal)
+        hipLaunchKernelGGL((k_normalize_inject<1, true>), grid, dim3(256), 0, s, albedo, litmask, dim, L0, opacity,
+                           decay, pyr.pyr, partials);
+    else if (L0.fmt == 1)
+        hipLaunchKernelGGL((k_normalize_inject<1, false>), grid, dim3(256), 0, s, albedo, litmask, dim, L0, opacity,
+                           decay, pyr.pyr, partials);
+    else if (temporal)
+        hipLaunchKernelGGL((k_normalize_inject<0, true>), grid, dim3(256), 0, s, albedo, litmask, dim, L0, opacity,
+                           decay, pyr.pyr, partials);
     else
-        hipLaunchKernelGGL(k_normalize_inject<0>, grid, dim3(256), 0, s, albedo, litmask, dim, L0, opacity, pyr.pyr,
-                           partials);
+        hipLaunchKernelGGL((k_normalize_inject<0, false>), grid, dim3(256), 0, s, albedo, litmask, dim, L0, opacity,
+                           decay, pyr.pyr, partials);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     // uniqueVoxels, maxFragmentsPerVoxel (transferVoxels.comp:39-45) and the lit count

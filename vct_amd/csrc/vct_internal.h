@@ -144,8 +144,9 @@ hipError_t launch_voxelize(const DevScene &sc, const DevLight *dlights, int nlig
 // radiance.hip
 hipError_t launch_inject_scatter(const DevLight *dlights_host, const DevLight *dlights_dev, int nlights,
                                  uint32_t light_mask, int dim, Extent ext, uint8_t *litmask, hipStream_t s);
-hipError_t launch_normalize_inject(uint32_t *albedo, const uint8_t *litmask, int dim, float opacity,
-                                   const PyrDesc &pyr, vct_stats *stats, uint32_t *partials, hipStream_t s);
+hipError_t launch_normalize_inject(uint32_t *albedo, const uint8_t *litmask, int dim, float opacity, bool temporal,
+                                   float decay, const PyrDesc &pyr, vct_stats *stats, uint32_t *partials,
+                                   hipStream_t s);
 hipError_t launch_fill_holes(const PyrDesc &pyr, void *tmp, hipStream_t s);
 
 // mip.hip

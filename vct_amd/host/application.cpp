@@ -340,6 +340,9 @@ int vct_app_default_config(int scene, vct_app_config *o) {
     o->msaa = 1;
     o->fill_holes = 0;
     o->device = 0;
+    o->temporal_filter = 0;
+    o->temporal_decay = 0.8f;
+    o->track_camera = 0;
     if (scene == 1) {
         o->dim = 64;
         o->levels = 7;
@@ -374,6 +377,21 @@ static void fill_defaults(vct_app_config &c) {
     if (c.msaa < 0) c.msaa = d.msaa;
     if (c.reflections < 0) c.reflections = d.reflections;
     if (c.fill_holes < 0) c.fill_holes = d.fill_holes;
+}
+
+// voxelTrackCamera (Application.cpp:187-191): the grid centre snaps to the
+// eye on cells of 2^levels voxels, so that the coarsest level's texels keep
+// their places between snaps.  Returns whether the centre changed.
+static bool track_camera(vct_app *a) {
+    bool changed = false;
+    for (int k = 0; k < 3; ++k) {
+        const float cell = std::pow(2.0f, (float)a->cfg.levels) * (a->extent_max[k] - a->extent_min[k]) /
+                           (float)a->cfg.dim;
+        const float c = std::floor(a->cam.eye[k] / cell) * cell;
+        changed = changed || c != a->center[k];
+        a->center[k] = c;
+    }
+    return changed;
 }
 
 int vct_app_create(vct_app **out, const vct_app_config *cfg_in) {
@@ -485,6 +503,7 @@ int vct_app_create(vct_app **out, const vct_app_config *cfg_in) {
     t.ambient_scale = 1.0f;
     t.reflect_scale = 1.0f;
     std::memcpy(t.eye, a->cam.eye, 12);
+    if (c.track_camera) track_camera(a);
     // GPU context
     vct_grid_desc d;
     std::memset(&d, 0, sizeof(d));
@@ -590,6 +609,8 @@ int vct_app_frame_state(vct_app *a, int frame, float *model, vct_light *lights, 
     if (inj) {
         inj->voxel_set_opacity = 0.5f;
         inj->light_mask = 0xffffffffu;
+        inj->temporal_filter = a->cfg.temporal_filter;
+        inj->temporal_decay = a->cfg.temporal_decay;
     }
     if (mip) {
         mip->kernel_mode = 0;
@@ -691,6 +712,10 @@ int vct_app_surface_points(vct_app *a, int frame, int n, uint64_t seed, vct_gbuf
 int vct_app_build_radiance(vct_app *a, int frame) {
     if (!a) return VCT_ERR_INVALID;
     if (!a->ctx) return app_fail(a, VCT_ERR_STATE, "vct_app_build_radiance: app was created without a device");
+    if (a->cfg.track_camera && track_camera(a)) {
+        const int rc = vct_set_extent(a->ctx, a->center, a->extent_min, a->extent_max);
+        if (rc) return rc;
+    }
     vct_voxelize_params vox;
     vct_inject_params inj;
     vct_mip_params mip;
