@@ -69,6 +69,9 @@ extern "C" {
 #define VCT_BUF_WEIGHTS 13   /* float [VCT_MLP_PARAMS + 1]: FP32 weights, out scale */
 #define VCT_BUF_GRADIENT 14  /* float [VCT_MLP_PARAMS + 1]: last step's minibatch
                                  gradient (sum of the workgroup partials) + loss   */
+#define VCT_BUF_WARPMAP 16   /* float4 [32^3] z-major: the warp texture (xyz warped
+                                texture coordinate, w packCellInfo), unorm16 values */
+#define VCT_BUF_WARP_OCCUPANCY 17 /* u32 [32^3] z-major: the occupancy grid behind it */
 #define VCT_BUF_COLOR 15     /* float4 [height][width]: the composed image (rgb, 1)
                                 before framebuffer quantization; 0 where uncovered */
 
@@ -127,6 +130,9 @@ typedef struct vct_voxelize_params {
     int msaa;               /* 1: 4x MSAA coverage (ConservativeRasterizeMode MSAA)
                                0: single sample at the pixel centre (OFF)          */
     float eye[3];
+    int warp_texture;       /* warpTexture (Application.h:89, off): the voxel of a
+                               fragment is dim * texture(warpmap, position)
+                               (voxelize.frag:103-105); needs vct_warpmap        */
 } vct_voxelize_params;
 
 /* transferVoxels + injectRadiance uniforms (src/Application.cpp:757-837). */
@@ -142,6 +148,9 @@ typedef struct vct_inject_params {
      * stored level 0 of the last frame (fp16 or fp32, per the storage format). */
     int temporal_filter;
     float temporal_decay;
+    int warp_texture;        /* warpTexture: each shadow texel lands in the voxel of
+                                its warped position (getVoxelPosition,
+                                common.glsl:48-50); needs vct_warpmap             */
 } vct_inject_params;
 
 /* Mip chain (src/Application.cpp:877-921, shaders/filterRadiance.comp). */
@@ -178,6 +187,9 @@ typedef struct vct_trace_params {
     float ambient_scale;          /* ambientScale                                  */
     float reflect_scale;          /* reflectScale                                  */
     float eye[3];
+    int warp_texture;             /* warpTexture: every cone sample position goes
+                                     through the warp texture (phong.frag:150-152);
+                                     needs vct_warpmap                             */
 } vct_trace_params;
 
 /* One G-buffer texel, 64 bytes (four float4). */
@@ -208,6 +220,7 @@ typedef struct vct_pass_times {
     double render_ms;     /* cone trace                                            */
     double learned_ms;    /* learned gather (when used instead of render)          */
     double compose_ms;    /* direct light + composition                            */
+    double warpmap_ms;    /* occupancy voxelization + warp texture                 */
 } vct_pass_times;
 
 typedef struct vct_ctx vct_ctx;
@@ -242,6 +255,16 @@ int vct_set_transforms(vct_ctx *ctx, const float *model, int num_objects);
 /* Light SSBO at binding 3 (src/Scene.cpp:38-76). */
 int vct_set_lights(vct_ctx *ctx, const vct_light *lights, int num_lights);
 
+/* Warp texture parameters (Settings, src/Application.h:80-92). */
+typedef struct vct_warpmap_params {
+    float high_resolution;   /* warpTextureHighResolution, 2.0: occupied cells     */
+    float low_resolution;    /* warpTextureLowResolution, 0.5: lower bound, empty   */
+    int axes[3];             /* warpTextureAxes, {1,1,1}: the axes that are warped */
+    int linear;              /* warpTextureLinear, 0: 1 = identity map             */
+    int use_weights_texture; /* useWarpmapWeightsTexture, 1: the weights pass
+                                through an rgba16f texture (fp16-rounded)          */
+} vct_warpmap_params;
+
 /* ---- passes (order of Application::render) -------------------------------- */
 /* Shadow map of one light: simple.vert + depth only (src/Application.cpp:212-233). */
 int vct_shadowmap(vct_ctx *ctx, int light_index);
@@ -252,6 +275,13 @@ int vct_voxelize(vct_ctx *ctx, const vct_voxelize_params *p);
  * fused: lit-mask scatter over shadow texels, then one per-voxel pass that
  * normalizes, rewrites opacity and writes radiance level 0. */
 int vct_normalize_inject(vct_ctx *ctx, const vct_inject_params *p);
+/* The warp texture (src/Application.cpp:235-575): the scene voxelized into a
+ * 32^3 occupancy grid with the projections and coverage mode of `vox`
+ * (voxelize.frag:187-193), per-axis prefix sums, the weight table and the
+ * 32^3 RGBA16 warpmap of generateWarpmap.frag, all on the GPU (the reference
+ * reads the occupancy back to the host).  Run it before the passes that set
+ * warp_texture; the reference builds it every frame whether or not it is used. */
+int vct_warpmap(vct_ctx *ctx, const vct_voxelize_params *vox, const vct_warpmap_params *p);
 /* voxelFillHoles.comp + glCopyImageSubData (src/Application.cpp:839-875). */
 int vct_fill_holes(vct_ctx *ctx);
 /* filterRadiance.comp loop over levels (src/Application.cpp:877-921). */

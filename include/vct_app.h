@@ -48,6 +48,9 @@ typedef struct vct_app_config {
     int track_camera;    /* voxelTrackCamera (Application.h:86), 0: the grid
                             centre follows the eye in steps of
                             2^levels * (max - min) / dim (Application.cpp:187-191) */
+    int warp_texture;    /* warpTexture (Application.h:89), 0: every frame builds the
+                            warp texture (vct_warpmap) and voxelize, inject and
+                            trace go through it                                */
 } vct_app_config;
 
 int vct_app_default_config(int scene, vct_app_config *out);
@@ -78,6 +81,8 @@ int vct_app_surface_points(vct_app *app, int frame, int n, uint64_t seed, vct_gb
  * src/Application.h:60-71: Cook-Torrance, diffuse, specular, indirect,
  * reflections as configured, post-processing, ambient and reflect scale 1). */
 int vct_app_compose_params(vct_app *app, int frame, vct_compose_params *out);
+/* The warp texture settings (Settings, src/Application.h:80-92). */
+int vct_app_warpmap_params(const vct_app *app, vct_warpmap_params *out);
 
 /* One frame in the order of Application::render: transforms, shadow maps,
  * voxelize, normalize+inject, [fill holes], mip, G-buffer, trace, compose. */

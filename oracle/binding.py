@@ -27,7 +27,12 @@ class Scene(C.Structure):
 
 class VoxParams(C.Structure):
     _fields_ = [("mvp_x", C.c_void_p), ("mvp_y", C.c_void_p), ("mvp_z", C.c_void_p), ("axis_override", C.c_int),
-                ("lighting", C.c_int), ("msaa", C.c_int), ("dim", C.c_int)]
+                ("lighting", C.c_int), ("msaa", C.c_int), ("dim", C.c_int), ("warpmap", C.c_void_p)]
+
+
+class WarpParams(C.Structure):
+    _fields_ = [("high_resolution", C.c_float), ("low_resolution", C.c_float), ("axes", C.c_int * 3),
+                ("linear", C.c_int), ("use_weights_texture", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -44,7 +49,8 @@ class TraceParams(C.Structure):
     _fields_ = [("diffuse", Cone), ("specular", Cone), ("enable_reflections", C.c_int),
                 ("angle_from_roughness", C.c_int), ("draw_occlusion", C.c_int), ("has_roughness", C.c_int),
                 ("ambient_scale", C.c_float), ("reflect_scale", C.c_float), ("eye", F3), ("dim", C.c_int),
-                ("nlevels", C.c_int), ("center", F3), ("vmin", F3), ("vmax", F3), ("anisotropic", C.c_int)]
+                ("nlevels", C.c_int), ("center", F3), ("vmin", F3), ("vmax", F3), ("anisotropic", C.c_int),
+                ("warpmap", C.c_void_p)]
 
 
 class Camera(C.Structure):
@@ -81,9 +87,12 @@ def lib() -> C.CDLL:
             "or_voxelize": (None, [C.POINTER(Scene), vp, C.c_int, vp, C.POINTER(VoxParams), vp, vp,
                                    C.POINTER(Stats)]),
             "or_normalize_inject": (None, [vp, C.c_int, vp, vp, vp, vp, C.c_int, vp, C.c_uint32, C.c_float,
-                                           C.c_int, vp, vp, C.POINTER(Stats)]),
+                                           C.c_int, vp, vp, vp, C.POINTER(Stats)]),
             "or_normalize_inject_temporal": (None, [vp, C.c_int, vp, vp, vp, vp, C.c_int, vp, C.c_uint32, C.c_float,
-                                                    C.c_float, C.c_int, vp, vp, vp, C.POINTER(Stats)]),
+                                                    C.c_float, C.c_int, vp, vp, vp, vp, C.POINTER(Stats)]),
+            "or_warp_occupancy": (None, [C.POINTER(Scene), C.POINTER(VoxParams), vp]),
+            "or_warpmap": (None, [vp, C.POINTER(WarpParams), vp]),
+            "or_warp_sample": (None, [vp, vp, vp]),
             "or_fill_holes": (None, [vp, vp, C.c_int, C.c_int]),
             "or_mip": (None, [vp, C.c_int, C.c_int, C.c_int]),
             "or_mip_aniso": (None, [vp, vp, C.c_int, C.c_int, C.c_int]),
@@ -203,12 +212,15 @@ def shadowmap(sc: SceneArrays, light) -> np.ndarray:
     return out
 
 
-def voxelize(sc: SceneArrays, lights, shadowmaps, vox, dim: int, msaa=None, lighting=None, axis_override=None):
+def voxelize(sc: SceneArrays, lights, shadowmaps, vox, dim: int, msaa=None, lighting=None, axis_override=None,
+             warpmap=None):
     mx = np.array(list(vox.mvp_x), np.float32)
     my = np.array(list(vox.mvp_y), np.float32)
     mz = np.array(list(vox.mvp_z), np.float32)
+    wm = None if warpmap is None else np.ascontiguousarray(warpmap, np.float32)
     p = VoxParams(_p(mx), _p(my), _p(mz), vox.axis_override if axis_override is None else axis_override,
-                  vox.lighting if lighting is None else lighting, vox.msaa if msaa is None else msaa, dim)
+                  vox.lighting if lighting is None else lighting, vox.msaa if msaa is None else msaa, dim,
+                  None if wm is None else _p(wm))
     la = _light_array(lights)
     sms = [np.ascontiguousarray(s, np.float32) if s is not None else None for s in shadowmaps]
     ptrs = (C.c_void_p * max(1, len(sms)))(*[(s.ctypes.data if s is not None else None) for s in sms])
@@ -221,7 +233,7 @@ def voxelize(sc: SceneArrays, lights, shadowmaps, vox, dim: int, msaa=None, ligh
 
 
 def normalize_inject(albedo: np.ndarray, dim, center, vmin, vmax, lights, shadowmaps, light_mask=0xffffffff,
-                     opacity=0.5, fp32=False, prev=None, decay=0.8):
+                     opacity=0.5, fp32=False, prev=None, decay=0.8, warpmap=None):
     """prev (float4 per voxel, last frame's stored level 0) selects the
     temporal filter with the given decay."""
     alb = np.ascontiguousarray(albedo, np.uint32).copy().reshape(-1)
@@ -232,16 +244,18 @@ def normalize_inject(albedo: np.ndarray, dim, center, vmin, vmax, lights, shadow
     rad = np.zeros((dim ** 3, 4), np.float32)
     lm = np.zeros(dim ** 3, np.uint8)
     st = Stats()
+    wm = None if warpmap is None else np.ascontiguousarray(warpmap, np.float32)
+    wp = None if wm is None else _p(wm)
     if prev is None:
         lib().or_normalize_inject(_p(alb), dim, _p(c), _p(a), _p(b), C.cast(la, C.c_void_p), len(lights),
-                                  C.cast(ptrs, C.c_void_p), light_mask, opacity, 1 if fp32 else 0, _p(rad), _p(lm),
-                                  C.byref(st))
+                                  C.cast(ptrs, C.c_void_p), light_mask, opacity, 1 if fp32 else 0, wp, _p(rad),
+                                  _p(lm), C.byref(st))
     else:
         pv = np.ascontiguousarray(prev, np.float32).reshape(-1, 4)
         assert pv.shape[0] == dim ** 3
         lib().or_normalize_inject_temporal(_p(alb), dim, _p(c), _p(a), _p(b), C.cast(la, C.c_void_p), len(lights),
                                            C.cast(ptrs, C.c_void_p), light_mask, opacity, decay,
-                                           1 if fp32 else 0, _p(pv), _p(rad), _p(lm), C.byref(st))
+                                           1 if fp32 else 0, wp, _p(pv), _p(rad), _p(lm), C.byref(st))
     return alb.reshape(dim, dim, dim), rad.reshape(dim, dim, dim, 4), lm, st
 
 
@@ -293,13 +307,14 @@ def gbuffer(sc: SceneArrays, cam) -> np.ndarray:
     return out
 
 
-def trace_params(tr, dim, nlevels, center, vmin, vmax, has_roughness, anisotropic=0) -> TraceParams:
+def trace_params(tr, dim, nlevels, center, vmin, vmax, has_roughness, anisotropic=0, warpmap=None) -> TraceParams:
+    """warpmap: a float32 [32^3 x 4] array the caller keeps alive while tracing."""
     d, s = tr.diffuse, tr.specular
     return TraceParams(Cone(d.steps, d.cone_angle, d.bias, d.cone_initial_height, d.lod_offset),
                        Cone(s.steps, s.cone_angle, s.bias, s.cone_initial_height, s.lod_offset),
                        tr.enable_reflections, tr.cone_angle_from_roughness, tr.draw_occlusion,
                        1 if has_roughness else 0, tr.ambient_scale, tr.reflect_scale, F3(*tr.eye), dim, nlevels,
-                       F3(*center), F3(*vmin), F3(*vmax), anisotropic)
+                       F3(*center), F3(*vmin), F3(*vmax), anisotropic, None if warpmap is None else _p(warpmap))
 
 
 def _pyramid_ptrs(levels, aniso_faces):
@@ -348,4 +363,34 @@ def compose(gbuf: np.ndarray, indirect, specular, lights, shadowmaps, params, ha
     out = np.zeros((npix, 4), np.float32)
     lib().or_compose(_p(g), npix, _p(ind), _p(spe), None if la is None else C.cast(la, C.c_void_p), len(lights),
                      C.cast(ptrs, C.c_void_p), 1 if has_roughness else 0, 1 if occluded else 0, C.byref(P), _p(out))
+    return out
+
+
+# ---------------------------------------------------------------- warp texture
+def warp_params(high=2.0, low=0.5, axes=(1, 1, 1), linear=0, use_weights_texture=1) -> WarpParams:
+    return WarpParams(high, low, (C.c_int * 3)(*axes), linear, use_weights_texture)
+
+
+def warp_occupancy(sc: SceneArrays, vox, msaa=None) -> np.ndarray:
+    """32^3 occupancy (z, y, x) of the scene, voxelize.frag:187-193."""
+    mx, my, mz = (np.array(list(getattr(vox, m)), np.float32) for m in ("mvp_x", "mvp_y", "mvp_z"))
+    p = VoxParams(_p(mx), _p(my), _p(mz), vox.axis_override, 0, vox.msaa if msaa is None else msaa, 32, None)
+    occ = np.zeros(32 ** 3, np.uint32)
+    lib().or_warp_occupancy(C.byref(sc.s), C.byref(p), _p(occ))
+    return occ.reshape(32, 32, 32)
+
+
+def warpmap(occ: np.ndarray, params: WarpParams | None = None) -> np.ndarray:
+    o = np.ascontiguousarray(occ, np.uint32).reshape(-1)
+    assert o.size == 32 ** 3
+    out = np.zeros((32, 32, 32, 4), np.float32)
+    lib().or_warpmap(_p(o), C.byref(params or warp_params()), _p(out))
+    return out
+
+
+def warp_sample(wm: np.ndarray, tc) -> np.ndarray:
+    w = np.ascontiguousarray(wm, np.float32)
+    t = np.asarray(tc, np.float32).reshape(3)
+    out = np.zeros(3, np.float32)
+    lib().or_warp_sample(_p(w), _p(t), _p(out))
     return out

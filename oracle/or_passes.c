@@ -267,8 +267,11 @@ static void light_fragment(const or_light *lights, int nlights, const float *con
     for (int k = 0; k < 3; ++k) out[k] = clampf(fin[k], 0.0f, 1.0f);
 }
 
-void or_voxelize(const or_scene *sc, const or_light *lights, int nlights, const float *const *shadowmaps,
-                 const or_voxelize_params *p, uint32_t *albedo, uint32_t *normal, or_stats *stats) {
+/* occ != NULL: the occupancy mode of voxelize.frag:187-193 (imageAtomicOr 1 at
+ * the unwarped position, nothing else). */
+static void voxelize_impl(const or_scene *sc, const or_light *lights, int nlights, const float *const *shadowmaps,
+                          const or_voxelize_params *p, uint32_t *albedo, uint32_t *normal, or_stats *stats,
+                          uint32_t *occ) {
     int dim = p->dim;
     float half = (float)dim * 0.5f;
     for (int t = 0; t < sc->ntri; ++t) {
@@ -329,13 +332,15 @@ void or_voxelize(const or_scene *sc, const or_light *lights, int nlights, const 
                     wp[k] = interp(l, W[0][k], W[1][k], W[2][k]);
                     nr[k] = interp(l, Nw[0][k], Nw[1][k], Nw[2][k]);
                 }
-                stats->total_voxel_fragments += 1; /* voxelize.frag:195 */
                 float color[3] = {col[0], col[1], col[2]};
                 float nn[3], nenc[4];
-                nrm3(nr, nn);
-                for (int k = 0; k < 3; ++k) nenc[k] = (nn[k] + 1.0f) * 0.5f;
-                nenc[3] = 1.0f;
-                if (p->lighting) light_fragment(lights, nlights, shadowmaps, color, nr, wp, color);
+                if (!occ) {
+                    stats->total_voxel_fragments += 1; /* voxelize.frag:195 */
+                    nrm3(nr, nn);
+                    for (int k = 0; k < 3; ++k) nenc[k] = (nn[k] + 1.0f) * 0.5f;
+                    nenc[3] = 1.0f;
+                    if (p->lighting) light_fragment(lights, nlights, shadowmaps, color, nr, wp, color);
+                }
                 /* getVoxelPosition (voxelize.frag:79-108) */
                 float u[3];
                 for (int k = 0; k < 3; ++k) u[k] = (pos[k] + 1.0f) * 0.5f;
@@ -354,11 +359,16 @@ void or_voxelize(const or_scene *sc, const or_light *lights, int nlights, const 
                     un[2] = u[2];
                 }
                 un[2] = 1.0f - un[2];
+                if (!occ && p->warpmap) or_warp_sample(p->warpmap, un, un); /* warpTexture (:103-105) */
                 int idx[3];
                 for (int k = 0; k < 3; ++k) idx[k] = (int)((float)dim * un[k]);
                 if (idx[0] < 0 || idx[1] < 0 || idx[2] < 0 || idx[0] >= dim || idx[1] >= dim || idx[2] >= dim)
                     continue; /* out-of-range image atomics are no-ops */
                 size_t vi = ((size_t)idx[2] * dim + idx[1]) * dim + idx[0];
+                if (occ) {
+                    occ[vi] |= 1u;
+                    continue;
+                }
                 float c4[4] = {color[0], color[1], color[2], 1.0f};
                 uint32_t pc = or_pack_unorm4x8(c4), pn = or_pack_unorm4x8(nenc);
                 if (pc > albedo[vi]) albedo[vi] = pc;
@@ -367,13 +377,116 @@ void or_voxelize(const or_scene *sc, const or_light *lights, int nlights, const 
     }
 }
 
+void or_voxelize(const or_scene *sc, const or_light *lights, int nlights, const float *const *shadowmaps,
+                 const or_voxelize_params *p, uint32_t *albedo, uint32_t *normal, or_stats *stats) {
+    voxelize_impl(sc, lights, nlights, shadowmaps, p, albedo, normal, stats, NULL);
+}
+
+/* ------------------------------------------------------------------------ */
+/* warp texture (src/Application.cpp:235-575)                                */
+void or_warp_occupancy(const or_scene *sc, const or_voxelize_params *p, uint32_t *occ) {
+    memset(occ, 0, sizeof(uint32_t) * (size_t)p->dim * p->dim * p->dim); /* glClearTexImage (:257) */
+    voxelize_impl(sc, NULL, 0, NULL, p, NULL, NULL, NULL, occ);
+}
+
+/* generateWarpmap.frag:49-99 at every texel centre, with the inclusive
+ * per-axis prefix sums (Application.cpp:312-344) and the low/high weights per
+ * occupied-cell count (:346-370), stored as RGBA16 unorm. */
+void or_warpmap(const uint32_t *occ, const or_warp_params *p, float *out) {
+    enum { D = OR_WARP_DIM };
+    static int part[D][D][D][3];
+    for (int z = 0; z < D; ++z)
+        for (int y = 0; y < D; ++y) {
+            int s = 0;
+            for (int x = 0; x < D; ++x) part[z][y][x][0] = (s += occ[(z * D + y) * D + x] > 0 ? 1 : 0);
+        }
+    for (int z = 0; z < D; ++z)
+        for (int x = 0; x < D; ++x) {
+            int s = 0;
+            for (int y = 0; y < D; ++y) part[z][y][x][1] = (s += occ[(z * D + y) * D + x] > 0 ? 1 : 0);
+        }
+    for (int y = 0; y < D; ++y)
+        for (int x = 0; x < D; ++x) {
+            int s = 0;
+            for (int z = 0; z < D; ++z) part[z][y][x][2] = (s += occ[(z * D + y) * D + x] > 0 ? 1 : 0);
+        }
+    float wt[2][D + 1];
+    for (int n = 0; n <= D; ++n) {
+        if (n == 0 || n == D) {
+            wt[0][n] = wt[1][n] = 1.0f;
+            continue;
+        }
+        int empty = D - n;
+        float h = p->high_resolution;
+        float l = ((float)D - h * (float)n) / (float)empty;
+        if (l < p->low_resolution) {
+            l = p->low_resolution;
+            h = ((float)D - l * (float)empty) / (float)n;
+        }
+        wt[0][n] = l;
+        wt[1][n] = h;
+    }
+    for (int z = 0; z < D; ++z)
+        for (int y = 0; y < D; ++y)
+            for (int x = 0; x < D; ++x) {
+                float tc[3] = {((float)x + 0.5f) / (float)D, ((float)y + 0.5f) / (float)D, ((float)z + 0.5f) / (float)D};
+                int o = occ[(z * D + y) * D + x] > 0;
+                const int total[3] = {part[z][y][D - 1][0], part[z][D - 1][x][1], part[D - 1][y][x][2]};
+                float *t = out + (((size_t)z * D + y) * D + x) * 4;
+                for (int k = 0; k < 3; ++k) {
+                    float lin = tc[k] * (float)D;
+                    float cell = floorf(lin), inner = lin - cell; /* modf */
+                    float lo = wt[0][total[k]], hi = wt[1][total[k]];
+                    if (p->use_weights_texture) { /* through the rgba16f weights texture (:485-488) */
+                        lo = or_half_to_float(or_float_to_half(lo));
+                        hi = or_half_to_float(or_float_to_half(hi));
+                    }
+                    float res = o ? hi : lo;
+                    float prev = (float)(o ? part[z][y][x][k] - 1 : part[z][y][x][k]);
+                    float off = lo * (cell - prev) + hi * prev;
+                    float inoff = inner * res;
+                    float w = (off + inoff) / (float)D;
+                    float v = (!p->linear && p->axes[k]) ? w : tc[k];
+                    t[k] = (float)(int)(fminf(fmaxf(v, 0.0f), 1.0f) * 65535.0f + 0.5f) / 65535.0f;
+                }
+                /* w: packCellInfo (occupied counts and flag), as stored */
+                uint32_t bits = ((uint32_t)total[0] & 0x1Fu) | (((uint32_t)total[1] & 0x1Fu) << 5) |
+                                (((uint32_t)total[2] & 0x1Fu) << 10) | ((uint32_t)o << 15);
+                t[3] = (float)bits / 65535.0f;
+            }
+}
+
+/* texture(warpmap, tc) with LINEAR filtering and CLAMP_TO_EDGE (xyz) */
+void or_warp_sample(const float *warpmap, const float tc[3], float out[3]) {
+    enum { D = OR_WARP_DIM };
+    int i0[3], i1[3];
+    float f[3];
+    for (int k = 0; k < 3; ++k) {
+        float u = tc[k] * (float)D - 0.5f;
+        float fl = floorf(u);
+        f[k] = u - fl;
+        int a = (int)fl, b = a + 1;
+        i0[k] = a < 0 ? 0 : (a > D - 1 ? D - 1 : a);
+        i1[k] = b < 0 ? 0 : (b > D - 1 ? D - 1 : b);
+    }
+    float acc[3] = {0.0f, 0.0f, 0.0f};
+    for (int c = 0; c < 8; ++c) {
+        int dx = c & 1, dy = (c >> 1) & 1, dz = c >> 2;
+        float w = ((dx ? f[0] : 1.0f - f[0]) * (dy ? f[1] : 1.0f - f[1])) * (dz ? f[2] : 1.0f - f[2]);
+        int x = dx ? i1[0] : i0[0], y = dy ? i1[1] : i0[1], z = dz ? i1[2] : i0[2];
+        const float *t = warpmap + (((size_t)z * D + y) * D + x) * 4;
+        for (int k = 0; k < 3; ++k) acc[k] = acc[k] + w * t[k];
+    }
+    for (int k = 0; k < 3; ++k) out[k] = acc[k];
+}
+
 /* ------------------------------------------------------------------------ */
 /* normalize (transferVoxels) + inject                                       */
 static float q16(float x) { return or_half_to_float(or_float_to_half(x)); }
 
 static void inject_litmask(int dim, const float center[3], const float vmin[3], const float vmax[3],
                            const or_light *lights, int nlights, const float *const *shadowmaps,
-                           uint32_t light_mask, uint8_t *litmask) {
+                           uint32_t light_mask, const float *warpmap, uint8_t *litmask) {
     size_t nv = (size_t)dim * dim * dim;
     memset(litmask, 0, nv);
     /* injectRadiance.comp:152-168 per shadow texel, as a lit-mask scatter */
@@ -393,10 +506,10 @@ static void inject_litmask(int dim, const float center[3], const float vmin[3], 
                     w[2] = w[2] / w[3];
                 }
                 int idx[3];
-                for (int k = 0; k < 3; ++k) {
-                    float lin = ((w[k] - center[k]) - vmin[k]) / (vmax[k] - vmin[k]);
-                    idx[k] = (int)((float)dim * lin);
-                }
+                float lin[3];
+                for (int k = 0; k < 3; ++k) lin[k] = ((w[k] - center[k]) - vmin[k]) / (vmax[k] - vmin[k]);
+                if (warpmap) or_warp_sample(warpmap, lin, lin); /* getVoxelPosition, common.glsl:48-50 */
+                for (int k = 0; k < 3; ++k) idx[k] = (int)((float)dim * lin[k]);
                 if (idx[0] < 0 || idx[1] < 0 || idx[2] < 0 || idx[0] >= dim || idx[1] >= dim || idx[2] >= dim)
                     continue;
                 size_t vi = ((size_t)idx[2] * dim + idx[1]) * dim + idx[0];
@@ -408,9 +521,9 @@ static void inject_litmask(int dim, const float center[3], const float vmin[3], 
 void or_normalize_inject(uint32_t *albedo, int dim, const float center[3], const float vmin[3],
                          const float vmax[3], const or_light *lights, int nlights,
                          const float *const *shadowmaps, uint32_t light_mask, float opacity,
-                         int storage_fp32, float *rad0, uint8_t *litmask, or_stats *stats) {
+                         int storage_fp32, const float *warpmap, float *rad0, uint8_t *litmask, or_stats *stats) {
     size_t nv = (size_t)dim * dim * dim;
-    inject_litmask(dim, center, vmin, vmax, lights, nlights, shadowmaps, light_mask, litmask);
+    inject_litmask(dim, center, vmin, vmax, lights, nlights, shadowmaps, light_mask, warpmap, litmask);
     /* transferVoxels.comp:37-62 (+ normalizeVoxels.comp:101-105 for rgba16f
      * radiance) and injectRadiance.comp:186,212 per voxel */
     for (size_t vi = 0; vi < nv; ++vi) {
@@ -447,10 +560,10 @@ void or_normalize_inject(uint32_t *albedo, int dim, const float center[3], const
 void or_normalize_inject_temporal(uint32_t *albedo, int dim, const float center[3], const float vmin[3],
                                   const float vmax[3], const or_light *lights, int nlights,
                                   const float *const *shadowmaps, uint32_t light_mask, float opacity,
-                                  float decay, int storage_fp32, const float *prev, float *rad0,
-                                  uint8_t *litmask, or_stats *stats) {
+                                  float decay, int storage_fp32, const float *warpmap, const float *prev,
+                                  float *rad0, uint8_t *litmask, or_stats *stats) {
     size_t nv = (size_t)dim * dim * dim;
-    inject_litmask(dim, center, vmin, vmax, lights, nlights, shadowmaps, light_mask, litmask);
+    inject_litmask(dim, center, vmin, vmax, lights, nlights, shadowmaps, light_mask, warpmap, litmask);
     const float a = 1.0f - decay; /* GLSL mix(x, y, a) = x * (1 - a) + y * a */
     for (size_t vi = 0; vi < nv; ++vi) { /* transfer */
         float c[4];

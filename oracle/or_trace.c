@@ -60,7 +60,7 @@ void or_trace64(const or_gbuf_texel *g, int npix, float *const *levels, float *c
 void or_trace_cone(float *const *levels, float *const *const *aniso, int dim, int nlevels, const float pos[3],
                    const float nrm[3], const float dir[3], const or_cone *c, float out[4]) {
     trace_cone_f(levels, aniso, dim, nlevels, pos, nrm, dir, c->steps, c->bias, tanf(c->angle / 2.0f), c->h0,
-                 c->lod_offset, out);
+                 c->lod_offset, NULL, out);
 }
 
 void or_sample(float *const *levels, int dim, int nlevels, const float p[3], float lambda, float out[4]) {

@@ -91,10 +91,35 @@ static void FN(sample_lod)(float *const *levels, float *const *const *aniso, int
     for (int c = 0; c < 4; ++c) out[c] = ((REAL)1 - fr) * t1[c] + fr * t2[c];
 }
 
-/* traceCone (phong.frag:135-180).  tan_half = tan(coneAngle / 2). */
+/* texture(warpmap, p).xyz on the 32^3 warp texture: LINEAR, CLAMP_TO_EDGE */
+static void FN(warp_sample)(const float *warpmap, const REAL tc[3], REAL out[3]) {
+    int i0[3], i1[3];
+    REAL f[3];
+    for (int k = 0; k < 3; ++k) {
+        REAL u = tc[k] * (REAL)OR_WARP_DIM - (REAL)0.5;
+        REAL fl = FLOOR(u);
+        f[k] = u - fl;
+        int a = (int)fl, b = a + 1;
+        i0[k] = a < 0 ? 0 : (a > OR_WARP_DIM - 1 ? OR_WARP_DIM - 1 : a);
+        i1[k] = b < 0 ? 0 : (b > OR_WARP_DIM - 1 ? OR_WARP_DIM - 1 : b);
+    }
+    REAL acc[3] = {0, 0, 0};
+    for (int c = 0; c < 8; ++c) {
+        int dx = c & 1, dy = (c >> 1) & 1, dz = c >> 2;
+        REAL w = ((dx ? f[0] : (REAL)1 - f[0]) * (dy ? f[1] : (REAL)1 - f[1])) * (dz ? f[2] : (REAL)1 - f[2]);
+        int x = dx ? i1[0] : i0[0], y = dy ? i1[1] : i0[1], z = dz ? i1[2] : i0[2];
+        const float *t = warpmap + (((size_t)z * OR_WARP_DIM + y) * OR_WARP_DIM + x) * 4;
+        for (int k = 0; k < 3; ++k) acc[k] = acc[k] + w * (REAL)t[k];
+    }
+    for (int k = 0; k < 3; ++k) out[k] = acc[k];
+}
+
+/* traceCone (phong.frag:135-180).  tan_half = tan(coneAngle / 2).  warpmap:
+ * warpTexture (:150-152), every sample position goes through the warp texture. */
 static void FN(trace_cone)(float *const *levels, float *const *const *aniso, int dim, int nlevels,
                             const REAL pos[3], const REAL nrm[3], const REAL dir_in[3], int steps,
-                            REAL bias, REAL tan_half, REAL h0, REAL lod_offset, REAL out[4]) {
+                            REAL bias, REAL tan_half, REAL h0, REAL lod_offset, const float *warpmap,
+                            REAL out[4]) {
     REAL dir[3];
     FN(nrm3)(dir_in, dir);
     REAL c[3] = {0, 0, 0}, a = 0;
@@ -113,6 +138,7 @@ static void FN(trace_cone)(float *const *levels, float *const *const *aniso, int
             if (sp[k] < 0 || sp[k] > 1) outside = 1;
         }
         if (outside) break;
+        if (warpmap) FN(warp_sample)(warpmap, sp, sp);
         REAL s[4];
         FN(sample_lod)(levels, aniso, dim, nlevels, sp, dir, lod + lod_offset, s);
         REAL aa = (REAL)1 - a;
@@ -160,7 +186,7 @@ static void FN(trace_pixel)(const or_gbuf_texel *g, float *const *levels, float 
         FN(nrm3)(d, dn);
         REAL r[4];
         FN(trace_cone)(levels, aniso, p->dim, p->nlevels, pos, N, dn, p->diffuse.steps, (REAL)p->diffuse.bias,
-                       tan_d, (REAL)p->diffuse.h0, (REAL)p->diffuse.lod_offset, r);
+                       tan_d, (REAL)p->diffuse.h0, (REAL)p->diffuse.lod_offset, p->warpmap, r);
         for (int q = 0; q < 4; ++q) ind[q] = ind[q] + (REAL)FN(CONE_W)[i] * r[q];
     }
     REAL ia = ind[3] < 0 ? 0 : (ind[3] > 1 ? 1 : ind[3]);
@@ -181,7 +207,8 @@ static void FN(trace_pixel)(const or_gbuf_texel *g, float *const *levels, float 
         for (int k = 0; k < 3; ++k) R[k] = I[k] - ((REAL)2 * dni) * N[k];
         REAL r[4];
         FN(trace_cone)(levels, aniso, p->dim, p->nlevels, pos, N, R, p->specular.steps, (REAL)p->specular.bias,
-                       (REAL)TAN(angle / (REAL)2), (REAL)p->specular.h0, (REAL)p->specular.lod_offset, r);
+                       (REAL)TAN(angle / (REAL)2), (REAL)p->specular.h0, (REAL)p->specular.lod_offset, p->warpmap,
+                       r);
         for (int q = 0; q < 4; ++q) out_spec[q] = r[q];
     }
 }

@@ -100,6 +100,7 @@ typedef struct or_voxelize_params {
     const float *mvp_x, *mvp_y, *mvp_z;
     int axis_override, lighting, msaa;
     int dim;
+    const float *warpmap; /* warpTexture: the 32^3 warp texture (or_warpmap), or NULL */
 } or_voxelize_params;
 
 typedef struct or_stats {
@@ -122,7 +123,7 @@ void or_normalize_inject(uint32_t *albedo, int dim, const float center[3],
                          const float vmin[3], const float vmax[3],
                          const or_light *lights, int nlights, const float *const *shadowmaps,
                          uint32_t light_mask, float opacity, int storage_fp32,
-                         float *rad0, uint8_t *litmask, or_stats *stats);
+                         const float *warpmap, float *rad0, uint8_t *litmask, or_stats *stats);
 
 /* The same with temporalFilterRadiance (Application.h:75-76): mix with the
  * previous level 0 `prev` (float4 per voxel, as stored), then lit voxels
@@ -131,7 +132,27 @@ void or_normalize_inject_temporal(uint32_t *albedo, int dim, const float center[
                                   const float vmin[3], const float vmax[3],
                                   const or_light *lights, int nlights, const float *const *shadowmaps,
                                   uint32_t light_mask, float opacity, float decay, int storage_fp32,
-                                  const float *prev, float *rad0, uint8_t *litmask, or_stats *stats);
+                                  const float *warpmap, const float *prev, float *rad0, uint8_t *litmask,
+                                  or_stats *stats);
+
+/* Warp texture (src/Application.cpp:235-575): occupancy voxelization at
+ * 32^3 (voxelize.frag:187-193: the voxelizer's fragments OR 1 into the cell of
+ * their unwarped position), then per texel centre the warped coordinate of
+ * generateWarpmap.frag:49-99 from the inclusive per-axis prefix sums
+ * (Application.cpp:312-344) and the low/high weights per occupied-cell count
+ * (:346-370; through the rgba16f weights texture when use_weights_texture),
+ * stored as RGBA16 unorm (k / 65535).  w holds packCellInfo. */
+#define OR_WARP_DIM 32
+typedef struct or_warp_params {
+    float high_resolution, low_resolution; /* warpTextureHigh/LowResolution 2.0, 0.5 */
+    int axes[3];                           /* warpTextureAxes                       */
+    int linear;                            /* warpTextureLinear                     */
+    int use_weights_texture;               /* useWarpmapWeightsTexture              */
+} or_warp_params;
+void or_warp_occupancy(const or_scene *sc, const or_voxelize_params *p /* dim 32 */, uint32_t *occ);
+void or_warpmap(const uint32_t *occ, const or_warp_params *p, float *out /* [32^3][4] */);
+/* texture(warpmap, tc).xyz: LINEAR, CLAMP_TO_EDGE */
+void or_warp_sample(const float *warpmap, const float tc[3], float out[3]);
 
 /* voxelFillHoles.comp (shaders/voxelFillHoles.comp:8-36) on a float4 grid whose
  * values are quantised by `quant` (0 = fp16 rounding, 1 = none). */
@@ -178,6 +199,7 @@ typedef struct or_trace_params {
     int dim, nlevels;
     float center[3], vmin[3], vmax[3];
     int anisotropic;
+    const float *warpmap; /* warpTexture: the 32^3 warp texture (or_warpmap), or NULL */
 } or_trace_params;
 
 /* Cone gather of phong.frag (shaders/phong.frag:135-180, 455-512).

@@ -47,7 +47,7 @@ def test_struct_layouts_match_headers(tmp_path):
                "vct_mip_params": abi.MipParams, "vct_camera": abi.Camera, "vct_cone_settings": abi.ConeSettings,
                "vct_trace_params": abi.TraceParams, "vct_stats": abi.Stats, "vct_pass_times": abi.PassTimes,
                "vct_app_config": abi.AppConfig, "vct_learned_params": abi.LearnedParams,
-               "vct_compose_params": abi.ComposeParams}
+               "vct_compose_params": abi.ComposeParams, "vct_warpmap_params": abi.WarpmapParams}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "vct_app.h"', "int main(void){"]
     for name, cls in structs.items():
         lines.append(f'printf("{name} %zu\\n", sizeof({name}));')

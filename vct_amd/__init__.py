@@ -145,6 +145,16 @@ class Context:
     def trace(self, p: abi.TraceParams):
         self.check(self.lib.vct_trace(self.h, C.byref(p)))
 
+    def warpmap(self, vox: abi.VoxelizeParams, p: abi.WarpmapParams | None = None):
+        self.check(self.lib.vct_warpmap(self.h, C.byref(vox), C.byref(p or abi.default_warpmap_params())))
+
+    def warp_texture(self) -> np.ndarray:
+        """The 32^3 warp texture (z, y, x, 4): warped coordinate xyz, packCellInfo w."""
+        return self.download(abi.VCT_BUF_WARPMAP, 0, np.float32).reshape(32, 32, 32, 4)
+
+    def warp_occupancy(self) -> np.ndarray:
+        return self.download(abi.VCT_BUF_WARP_OCCUPANCY, 0, np.uint32).reshape(32, 32, 32)
+
     def compose(self, p: abi.ComposeParams):
         self.check(self.lib.vct_compose(self.h, C.byref(p)))
 

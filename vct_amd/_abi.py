@@ -36,6 +36,8 @@ VCT_BUF_REFERENCE = 12
 VCT_BUF_WEIGHTS = 13
 VCT_BUF_GRADIENT = 14
 VCT_BUF_COLOR = 15
+VCT_BUF_WARPMAP = 16
+VCT_BUF_WARP_OCCUPANCY = 17
 VCT_MLP_PARAMS = 16835
 MLP_IN = 160
 
@@ -62,12 +64,13 @@ class Light(C.Structure):
 
 class VoxelizeParams(C.Structure):
     _fields_ = [("mvp_x", F16), ("mvp_y", F16), ("mvp_z", F16), ("axis_override", C.c_int),
-                ("lighting", C.c_int), ("atomic_max", C.c_int), ("msaa", C.c_int), ("eye", C.c_float * 3)]
+                ("lighting", C.c_int), ("atomic_max", C.c_int), ("msaa", C.c_int), ("eye", C.c_float * 3),
+                ("warp_texture", C.c_int)]
 
 
 class InjectParams(C.Structure):
     _fields_ = [("voxel_set_opacity", C.c_float), ("light_mask", C.c_uint32), ("temporal_filter", C.c_int),
-                ("temporal_decay", C.c_float)]
+                ("temporal_decay", C.c_float), ("warp_texture", C.c_int)]
 
 
 class MipParams(C.Structure):
@@ -87,7 +90,8 @@ class ConeSettings(C.Structure):
 class TraceParams(C.Structure):
     _fields_ = [("diffuse", ConeSettings), ("specular", ConeSettings), ("enable_reflections", C.c_int),
                 ("cone_angle_from_roughness", C.c_int), ("draw_occlusion", C.c_int),
-                ("ambient_scale", C.c_float), ("reflect_scale", C.c_float), ("eye", C.c_float * 3)]
+                ("ambient_scale", C.c_float), ("reflect_scale", C.c_float), ("eye", C.c_float * 3),
+                ("warp_texture", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -98,7 +102,18 @@ class Stats(C.Structure):
 class PassTimes(C.Structure):
     _fields_ = [("shadowmap_ms", C.c_double), ("voxelize_ms", C.c_double), ("radiance_ms", C.c_double),
                 ("fill_ms", C.c_double), ("mipmap_ms", C.c_double), ("gbuffer_ms", C.c_double),
-                ("render_ms", C.c_double), ("learned_ms", C.c_double), ("compose_ms", C.c_double)]
+                ("render_ms", C.c_double), ("learned_ms", C.c_double), ("compose_ms", C.c_double),
+                ("warpmap_ms", C.c_double)]
+
+
+class WarpmapParams(C.Structure):
+    _fields_ = [("high_resolution", C.c_float), ("low_resolution", C.c_float), ("axes", C.c_int * 3),
+                ("linear", C.c_int), ("use_weights_texture", C.c_int)]
+
+
+def default_warpmap_params() -> WarpmapParams:
+    """Settings defaults (src/Application.h:80-92)."""
+    return WarpmapParams(2.0, 0.5, (C.c_int * 3)(1, 1, 1), 0, 1)
 
 
 class ComposeParams(C.Structure):
@@ -113,7 +128,7 @@ class AppConfig(C.Structure):
                 ("storage", C.c_int), ("anisotropic", C.c_int), ("width", C.c_int), ("height", C.c_int),
                 ("shadow_size", C.c_int), ("lighting", C.c_int), ("msaa", C.c_int), ("reflections", C.c_int),
                 ("fill_holes", C.c_int), ("device", C.c_int), ("temporal_filter", C.c_int),
-                ("temporal_decay", C.c_float), ("track_camera", C.c_int)]
+                ("temporal_decay", C.c_float), ("track_camera", C.c_int), ("warp_texture", C.c_int)]
 
 
 class LearnedParams(C.Structure):
@@ -144,6 +159,7 @@ VCT_FUNCS = [
     ("vct_gbuffer", C.c_int, [C.c_void_p, C.POINTER(Camera)]),
     ("vct_trace", C.c_int, [C.c_void_p, C.POINTER(TraceParams)]),
     ("vct_compose", C.c_int, [C.c_void_p, C.POINTER(ComposeParams)]),
+    ("vct_warpmap", C.c_int, [C.c_void_p, C.POINTER(VoxelizeParams), C.POINTER(WarpmapParams)]),
     ("vct_sync", C.c_int, [C.c_void_p]),
     ("vct_get_stats", C.c_int, [C.c_void_p, C.POINTER(Stats)]),
     ("vct_get_pass_times", C.c_int, [C.c_void_p, C.POINTER(PassTimes)]),
@@ -172,6 +188,7 @@ APP_FUNCS = [
                                       C.POINTER(Camera), C.POINTER(TraceParams)]),
     ("vct_app_render", C.c_int, [C.c_void_p, C.c_int]),
     ("vct_app_compose_params", C.c_int, [C.c_void_p, C.c_int, C.POINTER(ComposeParams)]),
+    ("vct_app_warpmap_params", C.c_int, [C.c_void_p, C.POINTER(WarpmapParams)]),
     ("vct_app_build_radiance", C.c_int, [C.c_void_p, C.c_int]),
     ("vct_app_surface_points", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_uint64, C.c_void_p]),
     ("vct_math_ortho", None, [C.c_float] * 6 + [C.c_void_p]),

@@ -23,7 +23,7 @@
 
 using namespace vctk;
 
-enum PassId { P_SHADOW = 0, P_VOXELIZE, P_RADIANCE, P_FILL, P_MIP, P_GBUFFER, P_TRACE, P_LEARNED, P_COMPOSE, P_COUNT };
+enum PassId { P_SHADOW = 0, P_VOXELIZE, P_RADIANCE, P_FILL, P_MIP, P_GBUFFER, P_TRACE, P_LEARNED, P_COMPOSE, P_WARP, P_COUNT };
 
 struct TimedPair {
     int pass;
@@ -66,6 +66,11 @@ struct vct_ctx {
     float4 *d_ind = nullptr, *d_spec = nullptr, *d_color = nullptr;
     // what the last vct_trace of the current G-buffer left in d_ind / d_spec
     bool have_trace = false, trace_occluded = false, trace_spec = false;
+    // the warp texture: 32^3 occupancy, per-axis prefix sums, the float4 texels
+    uint32_t *d_warp_occ = nullptr;
+    int4 *d_warp_part = nullptr;
+    float4 *d_warpmap = nullptr;
+    bool have_warpmap = false;
     bool have_gbuf = false;
     // staging for downloads
     float4 *d_stage = nullptr;
@@ -327,6 +332,9 @@ void vct_destroy(vct_ctx *c) {
     dfree(c->d_ind);
     dfree(c->d_spec);
     dfree(c->d_color);
+    dfree(c->d_warp_occ);
+    dfree(c->d_warp_part);
+    dfree(c->d_warpmap);
     dfree(c->d_stage);
     dfree(c->d_points);
     dfree(c->d_labels);
@@ -611,11 +619,33 @@ int vct_voxelize(vct_ctx *c, const vct_voxelize_params *p) {
     if (p->axis_override < -1 || p->axis_override > 2)
         return fail(c, VCT_ERR_INVALID, "axis_override %d out of [-1, 2]", p->axis_override);
     if (!c->d_model) return fail(c, VCT_ERR_STATE, "vct_voxelize: no scene");
+    if (p->warp_texture && !c->have_warpmap) return fail(c, VCT_ERR_STATE, "vct_voxelize: warp_texture without vct_warpmap");
     PassTimer pt(c, P_VOXELIZE);
     LAUNCHCHK(c, launch_voxelize(dev_scene(c), c->d_lights, (int)c->lights.size(), *p, c->desc.dim,
-                                 c->d_grid_albedo, c->d_grid_normal, c->d_stats, c->d_partials, c->rs, c->stream));
+                                 c->d_grid_albedo, c->d_grid_normal, c->d_stats, c->d_partials,
+                                 p->warp_texture ? c->d_warpmap : nullptr, c->rs, c->stream));
     c->have_voxels = true;
     c->have_mips = false;
+    return VCT_OK;
+}
+
+int vct_warpmap(vct_ctx *c, const vct_voxelize_params *vox, const vct_warpmap_params *p) {
+    if (!c || !vox || !p) return fail(c, VCT_ERR_INVALID, "vct_warpmap: NULL argument");
+    if (vox->axis_override < -1 || vox->axis_override > 2)
+        return fail(c, VCT_ERR_INVALID, "axis_override %d out of [-1, 2]", vox->axis_override);
+    if (!(p->high_resolution > 0.0f) || !(p->low_resolution > 0.0f))
+        return fail(c, VCT_ERR_INVALID, "vct_warpmap: resolutions must be positive");
+    if (!c->d_model) return fail(c, VCT_ERR_STATE, "vct_warpmap: no scene");
+    const size_t n = (size_t)kWarpDim * kWarpDim * kWarpDim;
+    if (!c->d_warpmap) {
+        HIPCHK(c, dalloc(&c->d_warp_occ, n * 4));
+        HIPCHK(c, dalloc(&c->d_warp_part, n * sizeof(int4)));
+        HIPCHK(c, dalloc(&c->d_warpmap, n * sizeof(float4)));
+    }
+    PassTimer pt(c, P_WARP);
+    LAUNCHCHK(c, launch_warp_occupancy(dev_scene(c), *vox, c->d_warp_occ, c->rs, c->stream));
+    LAUNCHCHK(c, launch_warpmap(c->d_warp_occ, c->d_warp_part, *p, c->d_warpmap, c->stream));
+    c->have_warpmap = true;
     return VCT_OK;
 }
 
@@ -626,13 +656,16 @@ int vct_normalize_inject(vct_ctx *c, const vct_inject_params *p) {
     if (p->temporal_filter && !(p->temporal_decay >= 0.0f && p->temporal_decay <= 1.0f))
         return fail(c, VCT_ERR_INVALID, "temporal_decay %f out of [0, 1]", (double)p->temporal_decay);
     if (!c->have_voxels) return fail(c, VCT_ERR_STATE, "vct_normalize_inject: voxelize first");
+    if (p->warp_texture && !c->have_warpmap)
+        return fail(c, VCT_ERR_STATE, "vct_normalize_inject: warp_texture without vct_warpmap");
     PassTimer pt(c, P_RADIANCE);
     Extent ext;
     memcpy(ext.center, c->desc.center, 12);
     memcpy(ext.vmin, c->desc.vmin, 12);
     memcpy(ext.vmax, c->desc.vmax, 12);
     LAUNCHCHK(c, launch_inject_scatter(c->hlights.data(), c->d_lights, (int)c->lights.size(), p->light_mask,
-                                       c->desc.dim, ext, c->d_litmask, c->stream));
+                                       c->desc.dim, ext, p->warp_texture ? c->d_warpmap : nullptr, c->d_litmask,
+                                       c->stream));
     LAUNCHCHK(c, launch_normalize_inject(c->d_grid_albedo, c->d_litmask, c->desc.dim, p->voxel_set_opacity,
                                          p->temporal_filter != 0, p->temporal_decay, c->pyr, c->d_stats,
                                          c->d_partials, c->stream));
@@ -705,38 +738,16 @@ int vct_gbuffer(vct_ctx *c, const vct_camera *cam) {
     return VCT_OK;
 }
 
+static TraceLaunch make_trace_launch(vct_ctx *c, const vct_trace_params *p);
+
 int vct_trace(vct_ctx *c, const vct_trace_params *p) {
     if (!c || !p) return fail(c, VCT_ERR_INVALID, "vct_trace: NULL argument");
     if (!c->have_gbuf) return fail(c, VCT_ERR_STATE, "vct_trace: no G-buffer (call vct_gbuffer or upload one)");
     if (!c->have_mips) return fail(c, VCT_ERR_STATE, "vct_trace: mip chain not built");
     if (p->diffuse.steps < 0 || p->diffuse.steps > 4096 || p->specular.steps < 0 || p->specular.steps > 4096)
         return fail(c, VCT_ERR_INVALID, "cone steps out of [0, 4096]");
-    TraceLaunch t;
-    t.gbuf = c->d_gbuf;
-    t.width = c->gw;
-    t.height = c->gh;
-    t.out_ind = c->d_ind;
-    t.out_spec = c->d_spec;
-    t.enable_reflections = p->enable_reflections;
-    t.angle_from_roughness = p->cone_angle_from_roughness;
-    t.draw_occlusion = p->draw_occlusion;
-    t.has_roughness = c->has_rough;
-    t.ambient_scale = p->ambient_scale;
-    t.reflect_scale = p->reflect_scale;
-    memcpy(t.eye, p->eye, 12);
-    t.d_steps = p->diffuse.steps;
-    t.d_tan_half = tanf(p->diffuse.cone_angle / 2.0f);
-    t.d_bias = p->diffuse.bias;
-    t.d_h0 = p->diffuse.cone_initial_height;
-    t.d_lod_offset = p->diffuse.lod_offset;
-    t.s_steps = p->specular.steps;
-    t.s_angle = p->specular.cone_angle;
-    t.s_bias = p->specular.bias;
-    t.s_h0 = p->specular.cone_initial_height;
-    t.s_lod_offset = p->specular.lod_offset;
-    memcpy(t.ext.center, c->desc.center, 12);
-    memcpy(t.ext.vmin, c->desc.vmin, 12);
-    memcpy(t.ext.vmax, c->desc.vmax, 12);
+    if (p->warp_texture && !c->have_warpmap) return fail(c, VCT_ERR_STATE, "vct_trace: warp_texture without vct_warpmap");
+    const TraceLaunch t = make_trace_launch(c, p);
     PassTimer pt(c, P_TRACE);
     LAUNCHCHK(c, launch_trace(c->pyr, t, c->stream));
     c->have_trace = true;
@@ -798,6 +809,7 @@ int vct_get_pass_times(vct_ctx *c, vct_pass_times *out) {
     out->render_ms = ms[P_TRACE];
     out->learned_ms = ms[P_LEARNED];
     out->compose_ms = ms[P_COMPOSE];
+    out->warpmap_ms = ms[P_WARP];
     return VCT_OK;
 }
 
@@ -833,6 +845,8 @@ size_t vct_buffer_bytes(const vct_ctx *c, int what, int level) {
     case VCT_BUF_SPECULAR:
     case VCT_BUF_COLOR:
     case VCT_BUF_REFERENCE: return (size_t)c->gw * c->gh * 16;
+    case VCT_BUF_WARPMAP: return c->have_warpmap ? (size_t)kWarpDim * kWarpDim * kWarpDim * 16 : 0;
+    case VCT_BUF_WARP_OCCUPANCY: return c->have_warpmap ? (size_t)kWarpDim * kWarpDim * kWarpDim * 4 : 0;
     case VCT_BUF_LABELS: return (size_t)c->npoints * 16;
     case VCT_BUF_FEATURES: return (size_t)c->npoints * kMlpIn * 2;
     case VCT_BUF_WEIGHTS:
@@ -872,6 +886,8 @@ int vct_download(vct_ctx *c, int what, int level, void *host, size_t bytes) {
     case VCT_BUF_INDIRECT: src = c->d_ind; break;
     case VCT_BUF_SPECULAR: src = c->d_spec; break;
     case VCT_BUF_COLOR: src = c->d_color; break;
+    case VCT_BUF_WARPMAP: src = c->d_warpmap; break;
+    case VCT_BUF_WARP_OCCUPANCY: src = c->d_warp_occ; break;
     case VCT_BUF_LABELS: src = c->d_labels; break;
     case VCT_BUF_FEATURES: src = c->d_X; break;
     case VCT_BUF_REFERENCE:
@@ -1134,6 +1150,7 @@ static TraceLaunch make_trace_launch(vct_ctx *c, const vct_trace_params *p) {
     t.s_h0 = p->specular.cone_initial_height;
     t.s_lod_offset = p->specular.lod_offset;
     t.ext = ctx_extent(c);
+    t.warp = p->warp_texture ? c->d_warpmap : nullptr;
     return t;
 }
 
@@ -1143,6 +1160,8 @@ int vct_trace_learned(vct_ctx *c, const vct_trace_params *p, float thr, int *fal
     if (rc) return rc;
     if (!c->have_gbuf) return fail(c, VCT_ERR_STATE, "vct_trace_learned: no G-buffer");
     if (!c->have_weights) return fail(c, VCT_ERR_STATE, "vct_trace_learned: no trained weights");
+    if (p->warp_texture)
+        return fail(c, VCT_ERR_INVALID, "vct_trace_learned: the learned gather is trained on the unwarped grid");
     if (!c->d_tile_count) HIPCHK(c, dalloc(&c->d_tile_count, 4));
     TraceLaunch t = make_trace_launch(c, p);
     t.enable_reflections = 0;

@@ -22,7 +22,8 @@ struct InjectLight {
     int N, perspective;
 };
 
-__global__ __launch_bounds__(256) void k_inject_scatter(InjectLight L, int dim, Extent ext, uint8_t *litmask) {
+__global__ __launch_bounds__(256) void k_inject_scatter(InjectLight L, int dim, Extent ext, const float4 *warp,
+                                                        uint8_t *litmask) {
     const int i = blockIdx.x * 16 + (threadIdx.x & 15);
     const int j = blockIdx.y * 16 + (threadIdx.x >> 4);
     if (i >= L.N || j >= L.N) return;
@@ -36,16 +37,17 @@ __global__ __launch_bounds__(256) void k_inject_scatter(InjectLight L, int dim, 
         w[2] = w[2] / w[3];
     }
     int idx[3];
-    for (int k = 0; k < 3; ++k) {
-        const float lin = ((w[k] - ext.center[k]) - ext.vmin[k]) / (ext.vmax[k] - ext.vmin[k]);
-        idx[k] = (int)((float)dim * lin);
-    }
+    float lin[3];
+    for (int k = 0; k < 3; ++k) lin[k] = ((w[k] - ext.center[k]) - ext.vmin[k]) / (ext.vmax[k] - ext.vmin[k]);
+    if (warp) warp_sample(warp, lin, lin);  // getVoxelPosition with warpTexture (common.glsl:48-50)
+    for (int k = 0; k < 3; ++k) idx[k] = (int)((float)dim * lin[k]);
     if (idx[0] < 0 || idx[1] < 0 || idx[2] < 0 || idx[0] >= dim || idx[1] >= dim || idx[2] >= dim) return;
     litmask[((size_t)idx[2] * dim + idx[1]) * dim + idx[0]] = 1;
 }
 
 hipError_t launch_inject_scatter(const DevLight *lights, const DevLight *dlights_dev, int nlights,
-                                 uint32_t light_mask, int dim, Extent ext, uint8_t *litmask, hipStream_t s) {
+                                 uint32_t light_mask, int dim, Extent ext, const float4 *warp, uint8_t *litmask,
+                                 hipStream_t s) {
     (void)dlights_dev;
     const size_t nv = (size_t)dim * dim * dim;
     hipError_t e = launch_fill_u32((uint32_t *)litmask, 0u, nv / 4, s);
@@ -59,7 +61,7 @@ hipError_t launch_inject_scatter(const DevLight *lights, const DevLight *dlights
         il.N = (int)L.shadow_size;
         il.perspective = (int)L.perspective;
         dim3 grid((il.N + 15) / 16, (il.N + 15) / 16);
-        hipLaunchKernelGGL(k_inject_scatter, grid, dim3(256), 0, s, il, dim, ext, litmask);
+        hipLaunchKernelGGL(k_inject_scatter, grid, dim3(256), 0, s, il, dim, ext, warp, litmask);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     return hipSuccess;

@@ -265,10 +265,12 @@ __device__ __forceinline__ float4 sample_class(const PyrArgs &pa, __amdgpu_buffe
 // with it the sampled level) depends only on the step index and the level
 // class is read from the first active lane.  Otherwise the lanes' level
 // classes are visited one at a time (each pass with a uniform class).
+// warp: warpTexture (phong.frag:150-152), each sample position goes through the
+// warp texture before the pyramid is sampled (the level stays the unwarped one).
 template <int MODE, bool ANISO, bool ULOD>
 __device__ float4 trace_cone(const PyrArgs &pa, __amdgpu_buffer_rsrc_t rs, const float pos[3], const float nrm[3],
                              const float dir_in[3], int steps, float bias, float tan_half, float h0,
-                             float lod_offset) {
+                             float lod_offset, const float4 *warp) {
     float dir[3];
     {
         const float il = __builtin_amdgcn_rsqf(dot3(dir_in, dir_in));
@@ -299,6 +301,7 @@ __device__ float4 trace_cone(const PyrArgs &pa, __amdgpu_buffer_rsrc_t rs, const
 #pragma unroll
         for (int k = 0; k < 3; ++k) sp[k] = start[k] + (h * dir[k]) * scale;
         if (sp[0] < 0.f || sp[0] > 1.f || sp[1] < 0.f || sp[1] > 1.f || sp[2] < 0.f || sp[2] > 1.f) break;
+        if (warp) warp_sample(warp, sp, sp);
         float fr;
         const int D = level_class(lod + lod_offset, q, &fr);
         float4 smp;
@@ -365,7 +368,7 @@ __device__ __forceinline__ void trace_pixel(const TraceLaunch &t, const PyrArgs 
 #pragma unroll
         for (int k = 0; k < 3; ++k) d[k] = (T[k] * cx + B[k] * cy) + nraw[k] * cz;
         const float4 r = trace_cone<MODE, ANISO, true>(pa, rs, pos, N, d, t.d_steps, t.d_bias, t.d_tan_half, t.d_h0,
-                                                       t.d_lod_offset);
+                                                       t.d_lod_offset, t.warp);
         const float w = kConeW[i];
         i0 += w * r.x;
         i1 += w * r.y;
@@ -384,7 +387,7 @@ __device__ __forceinline__ void trace_pixel(const TraceLaunch &t, const PyrArgs 
 #pragma unroll
         for (int k = 0; k < 3; ++k) R[k] = I[k] - (2.0f * dni) * N[k];
         const float4 r = trace_cone<MODE, ANISO, false>(pa, rs, pos, N, R, t.s_steps, t.s_bias, tanf(angle / 2.0f),
-                                                        t.s_h0, t.s_lod_offset);
+                                                        t.s_h0, t.s_lod_offset, t.warp);
         t.out_spec[pi] = r;
     }
 }

@@ -62,6 +62,39 @@ __device__ __forceinline__ float interp3(const float l[3], float a0, float a1, f
 }
 __device__ __forceinline__ float clampf(float x, float a, float b) { return x < a ? a : (x > b ? b : x); }
 
+// ---- warp texture --------------------------------------------------------------
+// The 32^3 warp texture (warp.hip) as float4 texels, z-major: texture(warpmap,
+// tc).xyz with LINEAR filtering and CLAMP_TO_EDGE (common.glsl:48-50,
+// phong.frag:150-152, voxelize.frag:103-105).  Weights and sums in a fixed
+// order (the oracle's or_warp_sample).
+__device__ __forceinline__ void warp_sample(const float4 *__restrict__ wm, const float tc[3], float out[3]) {
+    int i0[3], i1[3];
+    float f[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float u = tc[k] * (float)kWarpDim - 0.5f;
+        const float fl = __builtin_floorf(u);
+        f[k] = u - fl;
+        const int a = (int)fl, b = a + 1;
+        i0[k] = a < 0 ? 0 : (a > kWarpDim - 1 ? kWarpDim - 1 : a);
+        i1[k] = b < 0 ? 0 : (b > kWarpDim - 1 ? kWarpDim - 1 : b);
+    }
+    float acc[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const int dx = c & 1, dy = (c >> 1) & 1, dz = c >> 2;
+        const float w = ((dx ? f[0] : 1.0f - f[0]) * (dy ? f[1] : 1.0f - f[1])) * (dz ? f[2] : 1.0f - f[2]);
+        const int x = dx ? i1[0] : i0[0], y = dy ? i1[1] : i0[1], z = dz ? i1[2] : i0[2];
+        const float4 t = wm[(z * kWarpDim + y) * kWarpDim + x];
+        acc[0] = acc[0] + w * t.x;
+        acc[1] = acc[1] + w * t.y;
+        acc[2] = acc[2] + w * t.z;
+    }
+    out[0] = acc[0];
+    out[1] = acc[1];
+    out[2] = acc[2];
+}
+
 // ---- storage formats -------------------------------------------------------
 __device__ __forceinline__ uint32_t pack_unorm4x8(float r, float g, float b, float a) {
     float c[4] = {r, g, b, a};

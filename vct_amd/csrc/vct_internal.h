@@ -11,6 +11,11 @@
 #define VCT_MAX_LEVELS 10
 
 namespace vctk {
+// edge of the warp texture and its occupancy grid (warpDim, src/Application.h:178)
+constexpr int kWarpDim = 32;
+}  // namespace vctk
+
+namespace vctk {
 
 // Light as the kernels see it: the ABI record plus its shadow map pointer.
 struct DevLight {
@@ -111,6 +116,7 @@ struct TraceLaunch {
     int s_steps;
     float s_angle, s_bias, s_h0, s_lod_offset;
     Extent ext;
+    const float4 *warp;  // warpTexture: the 32^3 warp texture, or null
 };
 
 // Statistics are reduced per workgroup into `partials` (kStatPartials slots of
@@ -139,11 +145,17 @@ hipError_t launch_gbuffer(const DevScene &sc, const vct_camera &cam, uint64_t *v
 // voxelize.hip
 hipError_t launch_voxelize(const DevScene &sc, const DevLight *dlights, int nlights, const vct_voxelize_params &p,
                            int dim, uint32_t *albedo, uint32_t *normal, vct_stats *stats, uint32_t *partials,
-                           RasterScratch &rs, hipStream_t s);
+                           const float4 *warp, RasterScratch &rs, hipStream_t s);
+// the 32^3 occupancy grid of the warp texture (the voxelizer in occupancy mode)
+hipError_t launch_warp_occupancy(const DevScene &sc, const vct_voxelize_params &p, uint32_t *occ, RasterScratch &rs,
+                                 hipStream_t s);
+// warp.hip: prefix sums + the 32^3 warp texture from the occupancy grid
+hipError_t launch_warpmap(const uint32_t *occ, int4 *part, const vct_warpmap_params &p, float4 *out, hipStream_t s);
 
 // radiance.hip
 hipError_t launch_inject_scatter(const DevLight *dlights_host, const DevLight *dlights_dev, int nlights,
-                                 uint32_t light_mask, int dim, Extent ext, uint8_t *litmask, hipStream_t s);
+                                 uint32_t light_mask, int dim, Extent ext, const float4 *warp, uint8_t *litmask,
+                                 hipStream_t s);
 hipError_t launch_normalize_inject(uint32_t *albedo, const uint8_t *litmask, int dim, float opacity, bool temporal,
                                    float decay, const PyrDesc &pyr, vct_stats *stats, uint32_t *partials,
                                    hipStream_t s);

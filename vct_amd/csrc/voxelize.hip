@@ -81,18 +81,11 @@ __global__ __launch_bounds__(256) void k_vox_setup(DevScene sc, VoxMats vm, int4
 }
 
 // calcShadowFactor (shaders/voxelize.frag:160-184)
-struct VoxFrag {
-    const float4 *attr;
-    const DevLight *lights;
-    int nlights, lighting, msaa, dim;
-    uint32_t *albedo, *normal;
-    uint32_t *partials;
-    uint32_t frags;
-
-    // Coverage of one (triangle, pixel) item: 4x MSAA with the per-sample
-    // near/far clip; on coverage, the pixel-centre barycentrics in l.
-    __device__ bool cover(int slot, int px, int py, const Edges &e, float l[3]) const {
-        const float4 *a = attr + (size_t)slot * 8;
+// Coverage of one (triangle, pixel) item: 4x MSAA with the per-sample
+// near/far clip; on coverage, the pixel-centre barycentrics in l.
+__device__ __forceinline__ bool vox_cover(const float4 *attr, int msaa, int slot, int px, int py, const Edges &e,
+                                          float l[3]) {
+    const float4 *a = attr + (size_t)slot * 8;
         const uint32_t flags = __float_as_uint(a[7].x);
         const bool zclip = (flags >> 4) & 1u;
         // ndc z of the three vertices: f[2], f[5], f[8]
@@ -112,6 +105,40 @@ struct VoxFrag {
         }
         if (cov) edge_bary(e, px, py, 128, 128, l);
         return cov;
+}
+
+// getVoxelPosition's swizzle (voxelize.frag:79-101): NDC of the fragment to
+// unit texture coordinates of the grid.
+__device__ __forceinline__ void vox_unit(const float pos[3], int axis, float un[3]) {
+    float u[3];
+    for (int k = 0; k < 3; ++k) u[k] = (pos[k] + 1.0f) * 0.5f;
+    if (axis == 0) {
+        un[0] = 1.0f - u[2];
+        un[1] = u[1];
+        un[2] = u[0];
+    } else if (axis == 1) {
+        un[0] = u[0];
+        un[1] = 1.0f - u[2];
+        un[2] = u[1];
+    } else {
+        un[0] = u[0];
+        un[1] = u[1];
+        un[2] = u[2];
+    }
+    un[2] = 1.0f - un[2];
+}
+
+struct VoxFrag {
+    const float4 *attr;
+    const DevLight *lights;
+    int nlights, lighting, msaa, dim;
+    uint32_t *albedo, *normal;
+    uint32_t *partials;
+    const float4 *warp;  // warpTexture: the warp texture, or null
+    uint32_t frags;
+
+    __device__ bool cover(int slot, int px, int py, const Edges &e, float l[3]) const {
+        return vox_cover(attr, msaa, slot, px, py, e, l);
     }
 
     // One covered fragment: varyings, lighting and the two atomicMax stores.
@@ -175,23 +202,9 @@ struct VoxFrag {
             }
             for (int k = 0; k < 3; ++k) color[k] = clampf(fin[k], 0.0f, 1.0f);
         }
-        float u[3];
-        for (int k = 0; k < 3; ++k) u[k] = (pos[k] + 1.0f) * 0.5f;
         float un[3];
-        if (axis == 0) {
-            un[0] = 1.0f - u[2];
-            un[1] = u[1];
-            un[2] = u[0];
-        } else if (axis == 1) {
-            un[0] = u[0];
-            un[1] = 1.0f - u[2];
-            un[2] = u[1];
-        } else {
-            un[0] = u[0];
-            un[1] = u[1];
-            un[2] = u[2];
-        }
-        un[2] = 1.0f - un[2];
+        vox_unit(pos, axis, un);
+        if (warp) warp_sample(warp, un, un);  // voxelize.frag:103-105
         int idx[3];
         for (int k = 0; k < 3; ++k) idx[k] = (int)((float)dim * un[k]);
         if (idx[0] < 0 || idx[1] < 0 || idx[2] < 0 || idx[0] >= dim || idx[1] >= dim || idx[2] >= dim) return;
@@ -202,9 +215,61 @@ struct VoxFrag {
     __device__ void finish() { block_partial(frags, 0u, 0u, 0u, partials); }
 };
 
+// The occupancy mode of the voxelizer (voxelize.frag:187-193): a covered
+// fragment ORs 1 into the 32^3 cell of its unwarped position.
+struct OccFrag {
+    const float4 *attr;
+    int msaa;
+    uint32_t *occ;
+
+    __device__ bool cover(int slot, int px, int py, const Edges &e, float l[3]) const {
+        return vox_cover(attr, msaa, slot, px, py, e, l);
+    }
+    __device__ void shade(int slot, const float l[3]) {
+        const float4 *a = attr + (size_t)slot * 8;
+        const int axis = (int)(__float_as_uint(a[7].x) & 3u);
+        const float4 a0 = a[0], a1 = a[1], a2 = a[2];
+        const float pos[3] = {interp3(l, a0.x, a0.w, a1.z), interp3(l, a0.y, a1.x, a1.w),
+                              interp3(l, a0.z, a1.y, a2.x)};
+        float un[3];
+        vox_unit(pos, axis, un);
+        int idx[3];
+        for (int k = 0; k < 3; ++k) idx[k] = (int)((float)kWarpDim * un[k]);
+        if (idx[0] < 0 || idx[1] < 0 || idx[2] < 0 || idx[0] >= kWarpDim || idx[1] >= kWarpDim ||
+            idx[2] >= kWarpDim)
+            return;
+        atomicOr(&occ[(idx[2] * kWarpDim + idx[1]) * kWarpDim + idx[0]], 1u);
+    }
+    __device__ void finish() {}
+};
+
+hipError_t launch_warp_occupancy(const DevScene &sc, const vct_voxelize_params &p, uint32_t *occ, RasterScratch &rs,
+                                 hipStream_t s) {
+    hipError_t e = launch_fill_u32(occ, 0u, (size_t)kWarpDim * kWarpDim * kWarpDim, s);  // Application.cpp:257
+    if (e != hipSuccess || sc.ntri == 0) return e;
+    VoxMats vm;
+    for (int i = 0; i < 16; ++i) {
+        vm.mvp[0][i] = p.mvp_x[i];
+        vm.mvp[1][i] = p.mvp_y[i];
+        vm.mvp[2][i] = p.mvp_z[i];
+    }
+    vm.axis_override = p.axis_override;
+    vm.dim = kWarpDim;  // glViewport(0, 0, voxelOccupancyDim, voxelOccupancyDim) (:239)
+    hipLaunchKernelGGL(k_vox_setup, dim3((sc.ntri + 255) / 256), dim3(256), 0, s, sc, vm, rs.rec, rs.attr, rs.counts);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = raster_scan(rs, (size_t)sc.ntri, s)) != hipSuccess) return e;
+    OccFrag f;
+    f.attr = rs.attr;
+    f.msaa = p.msaa;
+    f.occ = occ;
+    hipLaunchKernelGGL(k_raster_queued<OccFrag>, dim3(kRasterBlocks), dim3(256), 0, s, rs.rec, rs.offs, rs.chunk,
+                       rs.chunk_cap, sc.ntri, f);
+    return hipGetLastError();
+}
+
 hipError_t launch_voxelize(const DevScene &sc, const DevLight *dlights, int nlights, const vct_voxelize_params &p,
                            int dim, uint32_t *albedo, uint32_t *normal, vct_stats *stats, uint32_t *partials,
-                           RasterScratch &rs, hipStream_t s) {
+                           const float4 *warp, RasterScratch &rs, hipStream_t s) {
     const size_t nv = (size_t)dim * dim * dim;
     hipError_t e = launch_fill_u32(albedo, 0u, nv, s);  // glClearTexImage (Application.cpp:686-687)
     if (e != hipSuccess) return e;
@@ -231,6 +296,7 @@ hipError_t launch_voxelize(const DevScene &sc, const DevLight *dlights, int nlig
     f.albedo = albedo;
     f.normal = normal;
     f.partials = partials;
+    f.warp = warp;
     f.frags = 0;
     static_assert(kRasterBlocks <= kStatPartials, "one statistics partial per rasterizer workgroup");
     hipLaunchKernelGGL(k_raster_queued<VoxFrag>, dim3(kRasterBlocks), dim3(256), 0, s, rs.rec, rs.offs, rs.chunk,

@@ -343,6 +343,7 @@ int vct_app_default_config(int scene, vct_app_config *o) {
     o->temporal_filter = 0;
     o->temporal_decay = 0.8f;
     o->track_camera = 0;
+    o->warp_texture = 0;
     if (scene == 1) {
         o->dim = 64;
         o->levels = 7;
@@ -503,6 +504,7 @@ int vct_app_create(vct_app **out, const vct_app_config *cfg_in) {
     t.ambient_scale = 1.0f;
     t.reflect_scale = 1.0f;
     std::memcpy(t.eye, a->cam.eye, 12);
+    t.warp_texture = c.warp_texture;
     if (c.track_camera) track_camera(a);
     // GPU context
     vct_grid_desc d;
@@ -605,12 +607,14 @@ int vct_app_frame_state(vct_app *a, int frame, float *model, vct_light *lights, 
         vox->atomic_max = 1;
         vox->msaa = a->cfg.msaa;
         std::memcpy(vox->eye, a->cam.eye, 12);
+        vox->warp_texture = a->cfg.warp_texture;
     }
     if (inj) {
         inj->voxel_set_opacity = 0.5f;
         inj->light_mask = 0xffffffffu;
         inj->temporal_filter = a->cfg.temporal_filter;
         inj->temporal_decay = a->cfg.temporal_decay;
+        inj->warp_texture = a->cfg.warp_texture;
     }
     if (mip) {
         mip->kernel_mode = 0;
@@ -724,10 +728,25 @@ int vct_app_build_radiance(vct_app *a, int frame) {
     if ((rc = vct_set_transforms(a->ctx, a->model.data(), a->mesh.nobj))) return rc;
     for (size_t i = 0; i < a->lights.size(); ++i)
         if (a->lights[i].shadow_caster && (rc = vct_shadowmap(a->ctx, (int)i))) return rc;
+    if (a->cfg.warp_texture) {  // the occupancy grid and warp texture of this frame (Application.cpp:235-575)
+        vct_warpmap_params wp;
+        vct_app_warpmap_params(a, &wp);
+        if ((rc = vct_warpmap(a->ctx, &vox, &wp))) return rc;
+    }
     if ((rc = vct_voxelize(a->ctx, &vox))) return rc;
     if ((rc = vct_normalize_inject(a->ctx, &inj))) return rc;
     if (a->cfg.fill_holes && (rc = vct_fill_holes(a->ctx))) return rc;
     if ((rc = vct_mip(a->ctx, &mip))) return rc;
+    return VCT_OK;
+}
+
+int vct_app_warpmap_params(const vct_app *a, vct_warpmap_params *out) {
+    if (!a || !out) return VCT_ERR_INVALID;
+    out->high_resolution = 2.0f;  // Settings (src/Application.h:80-92)
+    out->low_resolution = 0.5f;
+    out->axes[0] = out->axes[1] = out->axes[2] = 1;
+    out->linear = 0;
+    out->use_weights_texture = 1;
     return VCT_OK;
 }
 
