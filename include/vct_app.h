@@ -53,7 +53,26 @@ typedef struct vct_app_config {
                             trace go through it                                */
 } vct_app_config;
 
+/* ---- OBJ/MTL assets: Mesh::loadMesh (src/Graphics/Mesh.cpp:42-266) ------------
+ * Faces triangulated (fans), texture v flipped, per-face tangents from the uv
+ * deltas (Mesh.cpp:174-189), albedo = the material's Kd as RGBA8 (white
+ * without a material), roughness = the material's Pr standing in for a
+ * constant roughness map.  Textures are not decoded; a face without normals
+ * gets its face normal, one without usable uvs a tangent orthogonal to it.
+ * The file is one object (one actor). */
+typedef struct vct_obj vct_obj;
+int vct_obj_load(const char *path, vct_obj **out, char *err, int errlen);
+/* views into the loaded arrays (valid until vct_obj_free) */
+int vct_obj_mesh(const vct_obj *obj, vct_mesh *out);
+int vct_obj_bounds(const vct_obj *obj, float bmin[3], float bmax[3], int *num_materials);
+void vct_obj_free(vct_obj *obj);
+
 int vct_app_default_config(int scene, vct_app_config *out);
+/* An application whose scene is an OBJ file (`cfg->scene` is ignored): the
+ * mesh as one static actor, the grid extent a cube around its bounds (5%
+ * margin), one shadow-casting directional light from above and a camera
+ * outside the bounds looking at their centre. */
+int vct_app_create_from_obj(vct_app **out, const vct_app_config *cfg, const char *path);
 int vct_app_create(vct_app **out, const vct_app_config *cfg);
 void vct_app_destroy(vct_app *app);
 const char *vct_app_last_error(const vct_app *app);

@@ -244,19 +244,53 @@ class Context:
         self.upload(abi.VCT_BUF_WEIGHTS, 0, np.ascontiguousarray(w, np.float32))
 
 
+def load_obj(path) -> dict:
+    """An OBJ/MTL file as the triangle-soup arrays of vct_mesh (one object),
+    plus its bounds and material count (vct_obj_load)."""
+    lib = load_app_library()
+    h = C.c_void_p()
+    err = C.create_string_buffer(256)
+    rc = lib.vct_obj_load(str(path).encode(), C.byref(h), err, 256)
+    if rc:
+        raise VCTError(rc, err.value.decode())
+    try:
+        m = abi.Mesh()
+        lib.vct_obj_mesh(h, C.byref(m))
+        n = m.num_tris
+
+        def arr(ptr, count, dt):
+            return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(np.ctypeslib.as_ctypes_type(dt))),
+                                         shape=(count,)).copy()
+
+        bmin, bmax = np.zeros(3, np.float32), np.zeros(3, np.float32)
+        nm = C.c_int()
+        lib.vct_obj_bounds(h, _ptr(bmin), _ptr(bmax), C.byref(nm))
+        return dict(pos=arr(m.positions, n * 9, np.float32).reshape(n, 3, 3),
+                    nrm=arr(m.normals, n * 9, np.float32).reshape(n, 3, 3),
+                    tan=arr(m.tangents, n * 3, np.float32).reshape(n, 3),
+                    albedo=arr(m.albedo, n, np.uint32), obj=arr(m.object, n, np.uint32), nobj=1,
+                    rough=None if not m.roughness else arr(m.roughness, 1, np.float32),
+                    bmin=bmin, bmax=bmax, materials=nm.value)
+    finally:
+        lib.vct_obj_free(h)
+
+
 class App:
     """Owns one vct_app (the headless Application of include/vct_app.h)."""
 
-    def __init__(self, scene=1, **overrides):
+    def __init__(self, scene=1, obj_path=None, **overrides):
         self.lib = load_app_library()
         cfg = abi.AppConfig()
-        rc = self.lib.vct_app_default_config(scene, C.byref(cfg))
+        rc = self.lib.vct_app_default_config(scene if obj_path is None else 1, C.byref(cfg))
         if rc:
             raise VCTError(rc, f"unknown scene {scene}")
         for k, v in overrides.items():
             setattr(cfg, k, v)
         h = C.c_void_p()
-        rc = self.lib.vct_app_create(C.byref(h), C.byref(cfg))
+        if obj_path is None:
+            rc = self.lib.vct_app_create(C.byref(h), C.byref(cfg))
+        else:
+            rc = self.lib.vct_app_create_from_obj(C.byref(h), C.byref(cfg), str(obj_path).encode())
         if rc:
             raise VCTError(rc, (self.lib.vct_app_last_error(None) or b"").decode())
         self.h = h

@@ -177,6 +177,11 @@ VCT_FUNCS = [
 APP_FUNCS = [
     ("vct_app_default_config", C.c_int, [C.c_int, C.POINTER(AppConfig)]),
     ("vct_app_create", C.c_int, [C.POINTER(C.c_void_p), C.POINTER(AppConfig)]),
+    ("vct_app_create_from_obj", C.c_int, [C.POINTER(C.c_void_p), C.POINTER(AppConfig), C.c_char_p]),
+    ("vct_obj_load", C.c_int, [C.c_char_p, C.POINTER(C.c_void_p), C.c_char_p, C.c_int]),
+    ("vct_obj_mesh", C.c_int, [C.c_void_p, C.POINTER(Mesh)]),
+    ("vct_obj_bounds", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_int)]),
+    ("vct_obj_free", None, [C.c_void_p]),
     ("vct_app_destroy", None, [C.c_void_p]),
     ("vct_app_last_error", C.c_char_p, [C.c_void_p]),
     ("vct_app_ctx", C.c_void_p, [C.c_void_p]),

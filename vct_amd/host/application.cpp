@@ -395,6 +395,8 @@ static bool track_camera(vct_app *a) {
     return changed;
 }
 
+static int app_finish(vct_app *a, vct_app **out);
+
 int vct_app_create(vct_app **out, const vct_app_config *cfg_in) {
     if (!out || !cfg_in) return VCT_ERR_INVALID;
     *out = nullptr;
@@ -493,6 +495,13 @@ int vct_app_create(vct_app **out, const vct_app_config *cfg_in) {
     }
     make_camera(r, boxes, a->mesh, c.scene == 1 ? 0.5 : 0.6, c.width, c.height, c.scene == 1 ? 0.05f : 0.02f,
                 a->cam);
+    return app_finish(a, out);
+}
+
+// trace settings, the camera snap, the GPU context and the scene upload,
+// common to every way of building the scene
+static int app_finish(vct_app *a, vct_app **out) {
+    const vct_app_config &c = a->cfg;
     // trace settings: VCTSettings defaults (Application.h:96-98)
     vct_trace_params &t = a->trace;
     std::memset(&t, 0, sizeof(t));
@@ -542,6 +551,81 @@ int vct_app_create(vct_app **out, const vct_app_config *cfg_in) {
     }
     *out = a;
     return VCT_OK;
+}
+
+int vct_app_create_from_obj(vct_app **out, const vct_app_config *cfg_in, const char *path) {
+    if (!out || !cfg_in || !path) return VCT_ERR_INVALID;
+    *out = nullptr;
+    vct_obj *obj = nullptr;
+    char err[256] = {0};
+    int rc = vct_obj_load(path, &obj, err, (int)sizeof(err));
+    if (rc) return rc;
+    vct_app *a = new vct_app();
+    a->cfg = *cfg_in;
+    a->cfg.scene = 1;  // defaults of the small static scene for unset fields
+    fill_defaults(a->cfg);
+    const vct_app_config &c = a->cfg;
+    vct_mesh m;
+    vct_obj_mesh(obj, &m);
+    a->mesh.pos.assign(m.positions, m.positions + (size_t)m.num_tris * 9);
+    a->mesh.nrm.assign(m.normals, m.normals + (size_t)m.num_tris * 9);
+    a->mesh.tan.assign(m.tangents, m.tangents + (size_t)m.num_tris * 3);
+    a->mesh.albedo.assign(m.albedo, m.albedo + m.num_tris);
+    a->mesh.obj.assign(m.object, m.object + m.num_tris);
+    if (m.roughness) a->mesh.rough.assign(m.roughness, m.roughness + 1);
+    a->mesh.nobj = 1;
+    float bmin[3], bmax[3];
+    vct_obj_bounds(obj, bmin, bmax, nullptr);
+    vct_obj_free(obj);
+    float half = 0.0f, cen[3];
+    for (int k = 0; k < 3; ++k) {
+        cen[k] = 0.5f * (bmin[k] + bmax[k]);
+        half = std::max(half, 0.5f * (bmax[k] - bmin[k]));
+    }
+    half = half > 0.0f ? 1.05f * half : 1.0f;
+    for (int k = 0; k < 3; ++k) {
+        a->center[k] = cen[k];
+        a->extent_min[k] = -half;
+        a->extent_max[k] = half;
+    }
+    a->motion.resize(1);
+    std::memset(&a->motion[0], 0, sizeof(ObjMotion));
+    a->motion[0].axis[1] = 1.0;
+    a->dynamic = false;
+    // one directional light from above, its shadow frustum around the bounds
+    a->lights.resize(1);
+    vct_light &L = a->lights[0];
+    std::memset(&L, 0, sizeof(L));
+    const float dir[3] = {-0.3f, -0.9f, -0.3f};
+    const float dl = std::sqrt((dir[0] * dir[0] + dir[1] * dir[1]) + dir[2] * dir[2]);
+    for (int k = 0; k < 3; ++k) L.direction[k] = dir[k] / dl;
+    const float dist = 2.5f * half;
+    for (int k = 0; k < 3; ++k) L.position[k] = cen[k] - L.direction[k] * dist;
+    L.color[0] = L.color[1] = L.color[2] = 1.0f;
+    L.intensity = 1.0f;
+    L.range = 5.0f * half;
+    L.type = VCT_LIGHT_DIRECTIONAL;
+    L.enabled = 1;
+    L.shadow_caster = 1;
+    L.shadow_size = (uint32_t)c.shadow_size;
+    float lp[16], lv[16];
+    ortho(-1.5f * half, 1.5f * half, -1.5f * half, 1.5f * half, 0.0f, 2.0f * dist, lp);
+    const float tgt[3] = {L.position[0] + L.direction[0], L.position[1] + L.direction[1],
+                          L.position[2] + L.direction[2]};
+    const float up[3] = {0.0f, 1.0f, 0.0f};
+    lookat(L.position, tgt, up, lv);
+    mul4(lp, lv, L.ls);
+    inverse4(L.ls, L.ls_inverse);
+    // camera outside the bounds, looking at the centre
+    std::memset(&a->cam, 0, sizeof(a->cam));
+    a->cam.width = c.width;
+    a->cam.height = c.height;
+    const float eye[3] = {cen[0] + 1.2f * half, cen[1] + 0.6f * half, cen[2] + 1.8f * half};
+    std::memcpy(a->cam.eye, eye, 12);
+    lookat(eye, cen, up, a->cam.view);
+    perspective(1.0471975512f, (float)c.width / (float)c.height, 0.02f * half, 10.0f * half, a->cam.projection);
+    a->cam.cull_back = 1;
+    return app_finish(a, out);
 }
 
 void vct_app_destroy(vct_app *a) {
