@@ -236,13 +236,45 @@ __device__ __forceinline__ void feat_setup(const float4 g0, const PyrView &pv, c
     F.fz = f[2];
     F.gz = 1.0f - f[2];
 }
-// z slice zs of the block, interpolated in x and y: ycur[oy * 3 + ox]
-__device__ __forceinline__ void feat_slice(const PyrView &pv, const FeatState &F, int zs, h4 ycur[9]) {
+// The same row when x0, y, z are the same in every active lane of the
+// wavefront: scalar loads (s_load_dwordx4 into SGPRs), uniform branches.  The
+// pyramid is not written while the gather runs and the addresses are the ones
+// row4_h would load, inside the allocation.
+__device__ __forceinline__ void row4_hu(const PyrView &pv, const LevelInfo &L, int x0, int y, int z, h4 t[4]) {
+    const int s = L.size;
+    const h2 z2 = {(_Float16)0.0f, (_Float16)0.0f};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) t[k].rg = t[k].ba = z2;
+    if ((unsigned)y >= (unsigned)s || (unsigned)z >= (unsigned)s) return;
+    if (x0 >= 0) {
+        const uint4 v = sload16(pv.pyr, L.base + elem_index(L, x0, y, z, 0) * 16u);
+        t[0] = as_h4(v.x, v.y);
+        t[1] = as_h4(v.z, v.w);
+    } else if (x0 == -1) {
+        const uint4 v = sload16(pv.pyr, L.base + elem_index(L, 0, y, z, 0) * 16u);
+        t[1] = as_h4(v.x, v.y);
+    }
+    const int xb = x0 + 2;
+    if (xb >= 0 && xb < s) {
+        const uint4 v = sload16(pv.pyr, L.base + elem_index(L, xb, y, z, 0) * 16u);
+        t[2] = as_h4(v.x, v.y);
+        t[3] = as_h4(v.z, v.w);
+    }
+}
+
+// z slice zs of the block, interpolated in x and y: ycur[oy * 3 + ox].
+// UNI: the block's first texel iu (= i0 - 1) is the same in every active lane.
+template <bool UNI = false>
+__device__ __forceinline__ void feat_slice(const PyrView &pv, const FeatState &F, int zs, h4 ycur[9],
+                                           const int *iu = nullptr) {
     const LevelInfo &L2 = pv.lv[2];
 #pragma unroll
     for (int ys = 0; ys < 4; ++ys) {
         h4 t[4];
-        row4_h(pv, L2, F.i0[0] - 1, F.i0[1] - 1 + ys, F.i0[2] - 1 + zs, t);
+        if (UNI)
+            row4_hu(pv, L2, iu[0], iu[1] + ys, iu[2] + zs, t);
+        else
+            row4_h(pv, L2, F.i0[0] - 1, F.i0[1] - 1 + ys, F.i0[2] - 1 + zs, t);
 #pragma unroll
         for (int ox = 0; ox < 3; ++ox) {
             h4 xv;
@@ -864,13 +896,34 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
     init_bias_t(acc, bias_ptr(W.master, 0));
     FeatState F;
     if (covered) feat_setup(g0, pv, t.ext, F);
+    // Usually every covered pixel of the tile reads the same 4x4x4 level-2
+    // block (an 8x8 tile covers about a quarter of a level-2 texel at 1080p):
+    // then its rows come through scalar loads and only the weights are per lane.
+    int iu[3] = {0, 0, 0};
+    bool uni = false;
+    {
+        const uint64_t cm = __ballot(covered);
+        if (cm) {
+            const int first = (int)__builtin_ctzll(cm);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) iu[k] = __builtin_amdgcn_readlane(covered ? F.i0[k] : 0, first) - 1;
+            const bool same = !covered || (F.i0[0] - 1 == iu[0] && F.i0[1] - 1 == iu[1] && F.i0[2] - 1 == iu[2]);
+            uni = __ballot(!same) == 0;
+        }
+    }
     h4 yprev[9], ycur[9];
     uint32_t d[18];
     // input dwords 0 .. 53 come from the z passes (18 per slice pair), 54 .. 64
     // from the tail, 65 .. 79 are zero; chunk c holds dwords 16 c .. 16 c + 15
+    // (unrolled: the slice registers rotate without copies and the chunk
+    // bookkeeping below is resolved at compile time)
+#pragma unroll
     for (int zs = 0; zs < 4; ++zs) {
         if (covered) {
-            feat_slice(pv, F, zs, ycur);
+            if (uni)
+                feat_slice<true>(pv, F, zs, ycur, iu);
+            else
+                feat_slice<false>(pv, F, zs, ycur);
             if (zs >= 1) feat_zpass(F, yprev, ycur, d);
 #pragma unroll
             for (int j = 0; j < 9; ++j) yprev[j] = ycur[j];
