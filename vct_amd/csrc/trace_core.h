@@ -68,6 +68,16 @@ __device__ __forceinline__ void axis_terms(float p, int s, uint32_t mul, uint32_
 }
 
 
+// All 2x2x2 texels of a uniform fetch are +0 (empty space): their weighted sum
+// adds exactly 0, so the weighting is skipped.  The test runs on the scalar
+// unit, beside the vector work of the other wavefronts.
+__device__ __forceinline__ bool zero16(const uint4 v[4]) {
+    uint32_t o = 0u;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) o |= (v[c].x | v[c].y) | (v[c].z | v[c].w);
+    return o == 0u;
+}
+
 __device__ __forceinline__ float hlo(uint32_t v) { return __half2float(__ushort_as_half((uint16_t)(v & 0xffffu))); }
 __device__ __forceinline__ float hhi(uint32_t v) { return __half2float(__ushort_as_half((uint16_t)(v >> 16))); }
 
@@ -92,11 +102,13 @@ __device__ __forceinline__ void acc_f4(float4 &acc, uint4 v, float w) {
 
 // acc += lw * (trilinear sample of level L at p): isotropic, or the
 // dir^2-weighted mix of the three faces that face the cone (fe: their element
-// offsets, 64 * face).  L is wave-uniform.
+// offsets, 64 * face).  L is wave-uniform.  fmask (wave-uniform) has bit a
+// set unless d2[a] is 0 in every lane: a face no lane weights adds exactly 0
+// and is not fetched.
 template <int MODE>
 __device__ __forceinline__ void level_sample(float4 &acc, const char *pyr, __amdgpu_buffer_rsrc_t rs,
                                              const float *__restrict__ scales, const LevelInfo &L, const float p[3],
-                                             float lw, const uint32_t fe[3], const float d2[3]) {
+                                             float lw, const uint32_t fe[3], const float d2[3], uint32_t fmask) {
     const int s = L.size;
     const bool aniso = L.nfaces == 6;
     uint32_t ty[2], tz[2];
@@ -119,6 +131,7 @@ __device__ __forceinline__ void level_sample(float4 &acc, const char *pyr, __amd
         }
         const int nf = aniso ? 3 : 1;
         for (int a = 0; a < nf; ++a) {
+            if (aniso && !((fmask >> a) & 1u)) continue;
             const uint32_t fo = aniso ? fe[a] : 0u;
             const float fw = aniso ? d2[a] : 1.0f;
             if (MODE == 1) {
@@ -174,16 +187,25 @@ __device__ __forceinline__ void level_sample(float4 &acc, const char *pyr, __amd
 #pragma unroll
         for (int c = 0; c < 4; ++c) so[c] = __builtin_amdgcn_readfirstlane(off[c]);
         if (!aniso) {
+            uint4 v[4];
 #pragma unroll
-            for (int c = 0; c < 4; ++c) acc_pair(acc, sload16(pyr, so[c]), wa[c], wb[c]);
+            for (int c = 0; c < 4; ++c) v[c] = sload16(pyr, so[c]);
+            if (zero16(v)) return;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) acc_pair(acc, v[c], wa[c], wb[c]);
             return;
         }
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
+            if (!((fmask >> a) & 1u)) continue;
             const uint32_t fo = __builtin_amdgcn_readfirstlane(fe[a]) * 16u;
+            uint4 v[4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) v[c] = sload16(pyr, so[c] + fo);
+            if (zero16(v)) continue;
             float4 t = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
-            for (int c = 0; c < 4; ++c) acc_pair(t, sload16(pyr, so[c] + fo), wa[c], wb[c]);
+            for (int c = 0; c < 4; ++c) acc_pair(t, v[c], wa[c], wb[c]);
             acc.x = fmaf(t.x, d2[a], acc.x);
             acc.y = fmaf(t.y, d2[a], acc.y);
             acc.z = fmaf(t.z, d2[a], acc.z);
@@ -201,6 +223,7 @@ __device__ __forceinline__ void level_sample(float4 &acc, const char *pyr, __amd
     }
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
+        if (!((fmask >> a) & 1u)) continue;
         uint4 v[4];
 #pragma unroll
         for (int c = 0; c < 4; ++c) v[c] = bload16(rs, off[c] + fe[a] * 16u);
@@ -237,7 +260,8 @@ __device__ __forceinline__ int level_class(float lambda, int q, float *fr) {
 
 template <int MODE>
 __device__ __forceinline__ float4 sample_class(const PyrArgs &pa, __amdgpu_buffer_rsrc_t rs, int D, float fr,
-                                               const float p[3], const uint32_t fe[3], const float d2[3]) {
+                                               const float p[3], const uint32_t fe[3], const float d2[3],
+                                               uint32_t fmask) {
     float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
     const int q = pa.nlevels - 1;
     if (D < 0) {
@@ -252,11 +276,11 @@ __device__ __forceinline__ float4 sample_class(const PyrArgs &pa, __amdgpu_buffe
         return unpack_h4(v.x, v.y);
     }
     if (D >= q) {
-        level_sample<MODE>(acc, pa.pyr, rs, pa.scales, pa.lv[q], p, 1.0f, fe, d2);
+        level_sample<MODE>(acc, pa.pyr, rs, pa.scales, pa.lv[q], p, 1.0f, fe, d2, fmask);
         return acc;
     }
-    level_sample<MODE>(acc, pa.pyr, rs, pa.scales, pa.lv[D], p, 1.0f - fr, fe, d2);
-    level_sample<MODE>(acc, pa.pyr, rs, pa.scales, pa.lv[D + 1], p, fr, fe, d2);
+    level_sample<MODE>(acc, pa.pyr, rs, pa.scales, pa.lv[D], p, 1.0f - fr, fe, d2, fmask);
+    level_sample<MODE>(acc, pa.pyr, rs, pa.scales, pa.lv[D + 1], p, fr, fe, d2, fmask);
     return acc;
 }
 
@@ -280,11 +304,15 @@ __device__ float4 trace_cone(const PyrArgs &pa, __amdgpu_buffer_rsrc_t rs, const
     }
     uint32_t fe[3] = {0u, 0u, 0u};
     float d2[3] = {0.f, 0.f, 0.f};
+    uint32_t fmask = 0u;
     if (ANISO) {
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             fe[k] = 64u * (uint32_t)(2 * k + (dir[k] >= 0.0f ? 0 : 1));
             d2[k] = dir[k] * dir[k];
+            // cones along a grid axis (faces of axis-aligned walls) weight only
+            // one or two of the three faces
+            if (__ballot(d2[k] != 0.0f) != 0) fmask |= 1u << k;
         }
     }
     const int q = pa.nlevels - 1;
@@ -308,7 +336,7 @@ __device__ float4 trace_cone(const PyrArgs &pa, __amdgpu_buffer_rsrc_t rs, const
         if (ULOD) {
             const int Du = __builtin_amdgcn_readfirstlane(D);
             const float fru = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(fr)));
-            smp = sample_class<MODE>(pa, rs, Du, fru, sp, fe, d2);
+            smp = sample_class<MODE>(pa, rs, Du, fru, sp, fe, d2, fmask);
         } else {
             smp = make_float4(0.f, 0.f, 0.f, 0.f);
             bool pending = true;
@@ -317,7 +345,7 @@ __device__ float4 trace_cone(const PyrArgs &pa, __amdgpu_buffer_rsrc_t rs, const
                 if (m == 0) break;
                 const int Du = __builtin_amdgcn_readlane(D, __builtin_ctzll(m));
                 if (pending && D == Du) {
-                    smp = sample_class<MODE>(pa, rs, Du, fr, sp, fe, d2);
+                    smp = sample_class<MODE>(pa, rs, Du, fr, sp, fe, d2, fmask);
                     pending = false;
                 }
             }
