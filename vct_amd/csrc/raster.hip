@@ -96,7 +96,7 @@ hipError_t launch_shadowmap(const DevScene &sc, const DevLight *L, float *depth,
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if ((e = raster_scan(rs, (size_t)sc.ntri, s)) != hipSuccess) return e;
     ShadowFrag f{rs.attr, (uint32_t *)depth, N};
-    hipLaunchKernelGGL(k_raster_items<ShadowFrag>, dim3(kRasterBlocks), dim3(256), 0, s, rs.rec, rs.offs, rs.chunk,
+    hipLaunchKernelGGL(k_raster_items<ShadowFrag>, dim3(raster_grid<ShadowFrag, false>()), dim3(256), 0, s, rs.rec, rs.offs, rs.chunk,
                        rs.chunk_cap, sc.ntri, f);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     return launch_shadow_quads(depth, N, const_cast<float4 *>(L->shadowquad), s);
@@ -289,7 +289,7 @@ hipError_t launch_gbuffer(const DevScene &sc, const vct_camera &cam, uint64_t *v
         if ((e = hipGetLastError()) != hipSuccess) return e;
         if ((e = raster_scan(rs, (size_t)sc.ntri * 2, s)) != hipSuccess) return e;
         GbufFrag f{rs.attr, (unsigned long long *)vis, cam.width};
-        hipLaunchKernelGGL(k_raster_items<GbufFrag>, dim3(kRasterBlocks), dim3(256), 0, s, rs.rec, rs.offs,
+        hipLaunchKernelGGL(k_raster_items<GbufFrag>, dim3(raster_grid<GbufFrag, false>()), dim3(256), 0, s, rs.rec, rs.offs,
                            rs.chunk, rs.chunk_cap, sc.ntri * 2, f);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }

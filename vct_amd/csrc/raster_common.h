@@ -301,4 +301,27 @@ __global__ __launch_bounds__(256) void k_raster_queued(const int4 *__restrict__ 
 // Grid for k_raster_items: enough resident wavefronts to fill 256 CUs.
 constexpr int kRasterBlocks = 2048;
 
+// Workgroups for a walk: as many as the device holds at once, at most
+// kRasterBlocks.  The walk splits the chunks evenly over its wavefronts, so
+// workgroups beyond what is resident would run as a second round after the
+// first, with most of the chip idle (the lit voxelizer holds 5 wavefronts per
+// SIMD, not 8).
+template <class F, bool QUEUED>
+inline int raster_grid() {
+    static int per_cu = 0;  // a property of the kernel: computed once
+    if (per_cu == 0) {
+        int n = 0;
+        hipError_t e;
+        if constexpr (QUEUED) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_raster_queued<F>, 256, 0);
+        else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_raster_items<F>, 256, 0);
+        per_cu = (e == hipSuccess && n > 0) ? n : 8;
+    }
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+        return kRasterBlocks;
+    const long g = (long)per_cu * cus;
+    return g > kRasterBlocks ? kRasterBlocks : (int)g;
+}
+
 }  // namespace vctk

@@ -262,7 +262,8 @@ hipError_t launch_warp_occupancy(const DevScene &sc, const vct_voxelize_params &
     f.attr = rs.attr;
     f.msaa = p.msaa;
     f.occ = occ;
-    hipLaunchKernelGGL(k_raster_queued<OccFrag>, dim3(kRasterBlocks), dim3(256), 0, s, rs.rec, rs.offs, rs.chunk,
+    hipLaunchKernelGGL(k_raster_queued<OccFrag>, dim3(raster_grid<OccFrag, true>()), dim3(256), 0, s, rs.rec, rs.offs,
+                       rs.chunk,
                        rs.chunk_cap, sc.ntri, f);
     return hipGetLastError();
 }
@@ -299,11 +300,12 @@ hipError_t launch_voxelize(const DevScene &sc, const DevLight *dlights, int nlig
     f.warp = warp;
     f.frags = 0;
     static_assert(kRasterBlocks <= kStatPartials, "one statistics partial per rasterizer workgroup");
-    hipLaunchKernelGGL(k_raster_queued<VoxFrag>, dim3(kRasterBlocks), dim3(256), 0, s, rs.rec, rs.offs, rs.chunk,
+    const int nblk = raster_grid<VoxFrag, true>();
+    hipLaunchKernelGGL(k_raster_queued<VoxFrag>, dim3(nblk), dim3(256), 0, s, rs.rec, rs.offs, rs.chunk,
                        rs.chunk_cap, sc.ntri, f);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     // totalVoxelFragments (voxelize.frag:195)
-    return launch_reduce_stats(partials, kRasterBlocks, &stats->total_voxel_fragments, nullptr, nullptr, s);
+    return launch_reduce_stats(partials, nblk, &stats->total_voxel_fragments, nullptr, nullptr, s);
 }
 
 }  // namespace vctk
