@@ -291,7 +291,11 @@ __device__ __forceinline__ float shadow_bilinear_quad(const float4 *__restrict__
 // that occlude a light-space position; 0 beyond the far plane.
 __device__ __forceinline__ float calc_shadow_factor(const float lsp[4], const float4 *sq, int N) {
     float sh[3];
-    for (int k = 0; k < 3; ++k) sh[k] = (lsp[k] / lsp[3] + 1.0f) * 0.5f;
+    if (__ballot(lsp[3] != 1.0f) == 0) {  // orthographic light space: w = 1, and x / 1 is x
+        for (int k = 0; k < 3; ++k) sh[k] = (lsp[k] + 1.0f) * 0.5f;
+    } else {
+        for (int k = 0; k < 3; ++k) sh[k] = (lsp[k] / lsp[3] + 1.0f) * 0.5f;
+    }
     const float frag = sh[2] - 0.01f;
     if (frag > 1.0f) return 0.0f;
     float f = 0.0f;
@@ -300,7 +304,7 @@ __device__ __forceinline__ float calc_shadow_factor(const float lsp[4], const fl
     if (frag > shadow_bilinear_quad(sq, N, sh[0], sh[1], 0, 1)) f += 1.0f;
     if (frag > shadow_bilinear_quad(sq, N, sh[0], sh[1], -1, 0)) f += 1.0f;
     if (frag > shadow_bilinear_quad(sq, N, sh[0], sh[1], 0, -1)) f += 1.0f;
-    f = f / 5.0f;
+    f = f * 0.2f;  // the same float as f / 5 for each of f = 0, 1, ..., 5
     return f;
 }
 
