@@ -57,6 +57,7 @@ struct vct_ctx {
     void *d_fill_tmp = nullptr;
     vct_stats *d_stats = nullptr;
     bool have_voxels = false, have_radiance = false, have_mips = false;
+    bool pyr_nonneg = true;  // level 0 was injected (not uploaded): no negative texels
     // raster scratch
     RasterScratch rs{};
     // camera images
@@ -671,6 +672,7 @@ int vct_normalize_inject(vct_ctx *c, const vct_inject_params *p) {
                                          c->d_partials, c->stream));
     c->have_radiance = true;
     c->have_mips = false;
+    c->pyr_nonneg = true;
     return VCT_OK;
 }
 
@@ -986,6 +988,7 @@ int vct_upload(vct_ctx *c, int what, int level, const void *host, size_t bytes) 
         HIPCHK(c, hipStreamSynchronize(c->stream));
         c->have_radiance = true;
         c->have_mips = false;
+        c->pyr_nonneg = false;  // the caller's values may be negative
         return VCT_OK;
     }
     default: return fail(c, VCT_ERR_INVALID, "vct_upload: buffer %d cannot be uploaded", what);
@@ -1151,6 +1154,7 @@ static TraceLaunch make_trace_launch(vct_ctx *c, const vct_trace_params *p) {
     t.s_lod_offset = p->specular.lod_offset;
     t.ext = ctx_extent(c);
     t.warp = p->warp_texture ? c->d_warpmap : nullptr;
+    t.nested = c->pyr_nonneg ? 1 : 0;
     return t;
 }
 

@@ -1075,7 +1075,7 @@ hipError_t launch_learned(const MlpBuffers &B, const PyrDesc &pyr, const TraceLa
     if (e != hipSuccess) return e;
     const int mode = pyr.storage == VCT_STORE_FP32 ? 1 : (pyr.storage == VCT_STORE_FP8 ? 2 : 0);
     const MlpDev W = mlp_dev(B);
-    const PyrArgs pa = pyr_args(pyr);
+    const PyrArgs pa = pyr_args(pyr, t.nested);
     const PyrView pv = pyr_view(pyr);
     if (mode == 0)
         hipLaunchKernelGGL(k_learned_stream<0>, dim3(tiles), dim3(64), 0, s, W, t, pa, pv, threshold,

@@ -34,7 +34,7 @@ hipError_t launch_trace(const PyrDesc &pyr, const TraceLaunch &t, hipStream_t s)
     dim3 grid((unsigned)(per_xcd * 8));
     const int mode = pyr.storage == VCT_STORE_FP32 ? 1 : (pyr.storage == VCT_STORE_FP8 ? 2 : 0);
 #define VCT_TRACE_LAUNCH(M, A)                                                                                      \
-    hipLaunchKernelGGL((k_trace<M, A>), grid, dim3(256), 0, s, t, pyr_args(pyr), tiles_x, tiles, per_xcd)
+    hipLaunchKernelGGL((k_trace<M, A>), grid, dim3(256), 0, s, t, pyr_args(pyr, t.nested), tiles_x, tiles, per_xcd)
     if (pyr.aniso) {
         if (mode == 0) VCT_TRACE_LAUNCH(0, true);
         else if (mode == 1) VCT_TRACE_LAUNCH(1, true);

@@ -33,6 +33,7 @@ struct PyrArgs {
     const float *scales;
     const LevelInfo *lv;
     int dim, nlevels;
+    int nested;  // TraceLaunch::nested
 };
 
 __device__ __forceinline__ uint4 bload16(__amdgpu_buffer_rsrc_t r, uint32_t off) {
@@ -104,9 +105,11 @@ __device__ __forceinline__ void acc_f4(float4 &acc, uint4 v, float w) {
 // dir^2-weighted mix of the three faces that face the cone (fe: their element
 // offsets, 64 * face).  L is wave-uniform.  fmask (wave-uniform) has bit a
 // set unless d2[a] is 0 in every lane: a face no lane weights adds exactly 0
-// and is not fetched.
-template <int MODE>
-__device__ __forceinline__ void level_sample(float4 &acc, const char *pyr, __amdgpu_buffer_rsrc_t rs,
+// and is not fetched.  Returns (wave-uniform) whether every texel fetched, in
+// every lane, was +0 -- reported by the fp16 paths only (ZQ: also test the
+// per-lane fetches, which costs vector instructions).
+template <int MODE, bool ZQ>
+__device__ __forceinline__ bool level_sample(float4 &acc, const char *pyr, __amdgpu_buffer_rsrc_t rs,
                                              const float *__restrict__ scales, const LevelInfo &L, const float p[3],
                                              float lw, const uint32_t fe[3], const float d2[3], uint32_t fmask) {
     const int s = L.size;
@@ -151,7 +154,7 @@ __device__ __forceinline__ void level_sample(float4 &acc, const char *pyr, __amd
                 }
             }
         }
-        return;
+        return false;
     }
     // fp16 pairs
     const float u = p[0] * (float)s - 0.5f;
@@ -190,11 +193,12 @@ __device__ __forceinline__ void level_sample(float4 &acc, const char *pyr, __amd
             uint4 v[4];
 #pragma unroll
             for (int c = 0; c < 4; ++c) v[c] = sload16(pyr, so[c]);
-            if (zero16(v)) return;
+            if (zero16(v)) return true;
 #pragma unroll
             for (int c = 0; c < 4; ++c) acc_pair(acc, v[c], wa[c], wb[c]);
-            return;
+            return false;
         }
+        bool z = true;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             if (!((fmask >> a) & 1u)) continue;
@@ -203,6 +207,7 @@ __device__ __forceinline__ void level_sample(float4 &acc, const char *pyr, __amd
 #pragma unroll
             for (int c = 0; c < 4; ++c) v[c] = sload16(pyr, so[c] + fo);
             if (zero16(v)) continue;
+            z = false;
             float4 t = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
             for (int c = 0; c < 4; ++c) acc_pair(t, v[c], wa[c], wb[c]);
@@ -211,22 +216,26 @@ __device__ __forceinline__ void level_sample(float4 &acc, const char *pyr, __amd
             acc.z = fmaf(t.z, d2[a], acc.z);
             acc.w = fmaf(t.w, d2[a], acc.w);
         }
-        return;
+        return z;
     }
     if (!aniso) {
         uint4 v[4];
 #pragma unroll
         for (int c = 0; c < 4; ++c) v[c] = bload16(rs, off[c]);
+        if (ZQ && __ballot(!zero16(v)) == 0) return true;
 #pragma unroll
         for (int c = 0; c < 4; ++c) acc_pair(acc, v[c], wa[c], wb[c]);
-        return;
+        return false;
     }
+    bool z = ZQ;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         if (!((fmask >> a) & 1u)) continue;
         uint4 v[4];
 #pragma unroll
         for (int c = 0; c < 4; ++c) v[c] = bload16(rs, off[c] + fe[a] * 16u);
+        if (ZQ && __ballot(!zero16(v)) == 0) continue;
+        z = false;
         float4 t = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
         for (int c = 0; c < 4; ++c) acc_pair(t, v[c], wa[c], wb[c]);
@@ -235,6 +244,7 @@ __device__ __forceinline__ void level_sample(float4 &acc, const char *pyr, __amd
         acc.z = fmaf(t.z, d2[a], acc.z);
         acc.w = fmaf(t.w, d2[a], acc.w);
     }
+    return z;
 }
 
 // textureLod with LINEAR_MIPMAP_LINEAR, the lambda clamp to [0, q] and the
@@ -276,11 +286,18 @@ __device__ __forceinline__ float4 sample_class(const PyrArgs &pa, __amdgpu_buffe
         return unpack_h4(v.x, v.y);
     }
     if (D >= q) {
-        level_sample<MODE>(acc, pa.pyr, rs, pa.scales, pa.lv[q], p, 1.0f, fe, d2, fmask);
+        level_sample<MODE, false>(acc, pa.pyr, rs, pa.scales, pa.lv[q], p, 1.0f, fe, d2, fmask);
         return acc;
     }
-    level_sample<MODE>(acc, pa.pyr, rs, pa.scales, pa.lv[D], p, 1.0f - fr, fe, d2, fmask);
-    level_sample<MODE>(acc, pa.pyr, rs, pa.scales, pa.lv[D + 1], p, fr, fe, d2, fmask);
+    // The coarser level first.  Its 2x2x2 cell covers the finer level's cell
+    // (texels j, j+1 filter texels 2j .. 2j+3, and the finer cell around the
+    // same point lies in that range), and in a nested pyramid a filtered texel
+    // is zero only where every texel it filters is.  So where the coarser
+    // fetch was all zero in every lane, the finer sample adds exactly 0 and is
+    // skipped: cones marching through empty space.
+    const bool z = level_sample<MODE, true>(acc, pa.pyr, rs, pa.scales, pa.lv[D + 1], p, fr, fe, d2, fmask);
+    if (!(z && pa.nested))
+        level_sample<MODE, false>(acc, pa.pyr, rs, pa.scales, pa.lv[D], p, 1.0f - fr, fe, d2, fmask);
     return acc;
 }
 
@@ -420,7 +437,7 @@ __device__ __forceinline__ void trace_pixel(const TraceLaunch &t, const PyrArgs 
     }
 }
 
-inline PyrArgs pyr_args(const PyrDesc &pyr) {
+inline PyrArgs pyr_args(const PyrDesc &pyr, int nested) {
     PyrArgs a;
     a.pyr = pyr.pyr;
     a.bytes = pyr.bytes;
@@ -428,6 +445,7 @@ inline PyrArgs pyr_args(const PyrDesc &pyr) {
     a.lv = pyr.d_lv;
     a.dim = pyr.dim;
     a.nlevels = pyr.nlevels;
+    a.nested = nested;
     return a;
 }
 

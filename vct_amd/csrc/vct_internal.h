@@ -117,6 +117,10 @@ struct TraceLaunch {
     float s_angle, s_bias, s_h0, s_lod_offset;
     Extent ext;
     const float4 *warp;  // warpTexture: the 32^3 warp texture, or null
+    // 1 when the pyramid was built by injection and filtering (not uploaded):
+    // every texel is non-negative, and a filtered texel is zero exactly when
+    // all the texels it filters are
+    int nested;
 };
 
 // Statistics are reduced per workgroup into `partials` (kStatPartials slots of
