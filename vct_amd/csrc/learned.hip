@@ -46,6 +46,10 @@ struct PyrView {
     const float *scales;
     const LevelInfo *lv;
     int dim, nlevels;
+    // level 2 (the stencil level) by value: its fields then come from the
+    // kernel arguments, not from memory that the compiler must reload after
+    // every store the gather makes
+    LevelInfo l2;
 };
 
 // textureLod with LINEAR_MIPMAP_LINEAR on an isotropic pyramid (border 0).
@@ -219,7 +223,7 @@ struct FeatState {
 __device__ __forceinline__ void feat_setup(const float4 g0, const PyrView &pv, const Extent &ext, FeatState &F) {
     const float gp[3] = {g0.x, g0.y, g0.z};
     for (int k = 0; k < 3; ++k) F.p[k] = ((gp[k] - ext.center[k]) - ext.vmin[k]) / (ext.vmax[k] - ext.vmin[k]);
-    const int s2 = pv.lv[2].size;
+    const int s2 = pv.l2.size;
     float f[3];
     for (int k = 0; k < 3; ++k) {
         const float u = F.p[k] * (float)s2 - 0.5f;
@@ -267,7 +271,7 @@ __device__ __forceinline__ void row4_hu(const PyrView &pv, const LevelInfo &L, i
 template <bool UNI = false>
 __device__ __forceinline__ void feat_slice(const PyrView &pv, const FeatState &F, int zs, h4 ycur[9],
                                            const int *iu = nullptr) {
-    const LevelInfo &L2 = pv.lv[2];
+    const LevelInfo &L2 = pv.l2;
 #pragma unroll
     for (int ys = 0; ys < 4; ++ys) {
         h4 t[4];
@@ -357,7 +361,7 @@ __device__ __forceinline__ void make_features(const float4 g0, const float4 g3, 
     float p[3];
     const float gp[3] = {g0.x, g0.y, g0.z};
     for (int k = 0; k < 3; ++k) p[k] = ((gp[k] - ext.center[k]) - ext.vmin[k]) / (ext.vmax[k] - ext.vmin[k]);
-    const LevelInfo &L2 = pv.lv[2];
+    const LevelInfo &L2 = pv.l2;
     const int s2 = L2.size;
     int i0[3];
     float f[3];
@@ -1013,6 +1017,7 @@ static PyrView pyr_view(const PyrDesc &pyr) {
     v.lv = pyr.d_lv;
     v.dim = pyr.dim;
     v.nlevels = pyr.nlevels;
+    v.l2 = pyr.lv[pyr.nlevels > 2 ? 2 : 0];
     return v;
 }
 
