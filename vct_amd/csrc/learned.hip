@@ -181,27 +181,43 @@ __device__ __forceinline__ h4 as_h4(uint32_t a, uint32_t b) {
     r.ba = __builtin_bit_cast(h2, b);
     return r;
 }
-__device__ __forceinline__ void row4_h(const PyrView &pv, const LevelInfo &L, int x0, int y, int z, h4 t[4]) {
+// Texels x0 .. x0+3 of row (y, z), zero outside the level: the two pair
+// elements at clamped (always valid) indices (row_fetch), then selects
+// (row_select).  Element e holds texels (e, e+1); with x0 = -1 texel 0 is the
+// first half of element 0, and likewise for x0 + 2 = -1.  SCALAR: x0, y, z
+// are the same in every active lane, and the elements come through scalar
+// loads (the pyramid is not written while the gather runs).
+struct RowRaw {
+    uint4 a, b;
+};
+template <bool SCALAR>
+__device__ __forceinline__ void row_fetch(const PyrView &pv, const LevelInfo &L, int x0, int y, int z, RowRaw &r) {
     const int s = L.size;
-    const h2 z2 = {(_Float16)0.0f, (_Float16)0.0f};
-#pragma unroll
-    for (int k = 0; k < 4; ++k) t[k].rg = t[k].ba = z2;
-    if ((unsigned)y >= (unsigned)s || (unsigned)z >= (unsigned)s) return;
-    const char *base = pv.pyr + L.base;
-    if (x0 >= 0) {
-        const uint4 v = *(const uint4 *)(base + (size_t)elem_index(L, x0, y, z, 0) * 16);
-        t[0] = as_h4(v.x, v.y);
-        t[1] = as_h4(v.z, v.w);
-    } else if (x0 == -1) {
-        const uint2 v = *(const uint2 *)(base + (size_t)elem_index(L, 0, y, z, 0) * 16);
-        t[1] = as_h4(v.x, v.y);
-    }
+    const int yc = y < 0 ? 0 : (y > s - 1 ? s - 1 : y), zc = z < 0 ? 0 : (z > s - 1 ? s - 1 : z);
+    const int ea = x0 < 0 ? 0 : (x0 > s - 1 ? s - 1 : x0);
     const int xb = x0 + 2;
-    if (xb >= 0 && xb < s) {
-        const uint4 v = *(const uint4 *)(base + (size_t)elem_index(L, xb, y, z, 0) * 16);
-        t[2] = as_h4(v.x, v.y);
-        t[3] = as_h4(v.z, v.w);
+    const int eb = xb < 0 ? 0 : (xb > s - 1 ? s - 1 : xb);
+    const uint32_t oa = L.base + elem_index(L, ea, yc, zc, 0) * 16u, ob = L.base + elem_index(L, eb, yc, zc, 0) * 16u;
+    if (SCALAR) {
+        r.a = sload16(pv.pyr, oa);
+        r.b = sload16(pv.pyr, ob);
+    } else {
+        r.a = *(const uint4 *)(pv.pyr + oa);
+        r.b = *(const uint4 *)(pv.pyr + ob);
     }
+}
+__device__ __forceinline__ void row_select(const RowRaw &r, int s, int x0, int y, int z, h4 t[4]) {
+    const bool yz = (unsigned)y < (unsigned)s && (unsigned)z < (unsigned)s;
+    const int xb = x0 + 2;
+    const h4 zero = as_h4(0u, 0u);
+    const h4 alo = as_h4(r.a.x, r.a.y), ahi = as_h4(r.a.z, r.a.w), blo = as_h4(r.b.x, r.b.y),
+             bhi = as_h4(r.b.z, r.b.w);
+    const bool v0 = yz && (unsigned)x0 < (unsigned)s, v1 = yz && (unsigned)(x0 + 1) < (unsigned)s;
+    const bool v2 = yz && (unsigned)xb < (unsigned)s, v3 = yz && (unsigned)(xb + 1) < (unsigned)s;
+    t[0] = v0 ? alo : zero;
+    t[1] = v1 ? (x0 >= 0 ? ahi : alo) : zero;
+    t[2] = v2 ? blo : zero;
+    t[3] = v3 ? (xb >= 0 ? bhi : blo) : zero;
 }
 
 __device__ __forceinline__ float4 lerp4(float4 a, float4 b, float f) {
@@ -240,45 +256,21 @@ __device__ __forceinline__ void feat_setup(const float4 g0, const PyrView &pv, c
     F.fz = f[2];
     F.gz = 1.0f - f[2];
 }
-// The same row when x0, y, z are the same in every active lane of the
-// wavefront: scalar loads (s_load_dwordx4 into SGPRs), uniform branches.  The
-// pyramid is not written while the gather runs and the addresses are the ones
-// row4_h would load, inside the allocation.
-__device__ __forceinline__ void row4_hu(const PyrView &pv, const LevelInfo &L, int x0, int y, int z, h4 t[4]) {
-    const int s = L.size;
-    const h2 z2 = {(_Float16)0.0f, (_Float16)0.0f};
-#pragma unroll
-    for (int k = 0; k < 4; ++k) t[k].rg = t[k].ba = z2;
-    if ((unsigned)y >= (unsigned)s || (unsigned)z >= (unsigned)s) return;
-    if (x0 >= 0) {
-        const uint4 v = sload16(pv.pyr, L.base + elem_index(L, x0, y, z, 0) * 16u);
-        t[0] = as_h4(v.x, v.y);
-        t[1] = as_h4(v.z, v.w);
-    } else if (x0 == -1) {
-        const uint4 v = sload16(pv.pyr, L.base + elem_index(L, 0, y, z, 0) * 16u);
-        t[1] = as_h4(v.x, v.y);
-    }
-    const int xb = x0 + 2;
-    if (xb >= 0 && xb < s) {
-        const uint4 v = sload16(pv.pyr, L.base + elem_index(L, xb, y, z, 0) * 16u);
-        t[2] = as_h4(v.x, v.y);
-        t[3] = as_h4(v.z, v.w);
-    }
-}
-
 // z slice zs of the block, interpolated in x and y: ycur[oy * 3 + ox].
 // UNI: the block's first texel iu (= i0 - 1) is the same in every active lane.
 template <bool UNI = false>
 __device__ __forceinline__ void feat_slice(const PyrView &pv, const FeatState &F, int zs, h4 ycur[9],
                                            const int *iu = nullptr) {
     const LevelInfo &L2 = pv.l2;
+    const int x0 = UNI ? iu[0] : F.i0[0] - 1, y0 = UNI ? iu[1] : F.i0[1] - 1, z = (UNI ? iu[2] : F.i0[2] - 1) + zs;
+    // the rows' loads carry no branches, so they issue ahead of the selects
+    RowRaw raw[4];
+#pragma unroll
+    for (int ys = 0; ys < 4; ++ys) row_fetch<UNI>(pv, L2, x0, y0 + ys, z, raw[ys]);
 #pragma unroll
     for (int ys = 0; ys < 4; ++ys) {
         h4 t[4];
-        if (UNI)
-            row4_hu(pv, L2, iu[0], iu[1] + ys, iu[2] + zs, t);
-        else
-            row4_h(pv, L2, F.i0[0] - 1, F.i0[1] - 1 + ys, F.i0[2] - 1 + zs, t);
+        row_select(raw[ys], L2.size, x0, y0 + ys, z, t);
 #pragma unroll
         for (int ox = 0; ox < 3; ++ox) {
             h4 xv;
