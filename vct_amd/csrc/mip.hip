@@ -33,6 +33,75 @@ __device__ __forceinline__ float4 round_stage(float4 v, int fp32) {
     return make_float4(q16(v.x), q16(v.y), q16(v.z), q16(v.w));
 }
 
+// One filtering step of the chain: the (cur/2)^3 texels per face of output
+// level k from the cur^3 texels per face in LDS (faces: 1 or 6 source faces).
+// CUR: cur as a compile-time constant (0: use the argument), which turns the
+// index arithmetic into shifts.
+template <bool ANISO, int CUR>
+__device__ __forceinline__ void mip_step(const MipArgs &a, int cur_rt, int faces, const float4 *in, int in_stride,
+                                         float4 *out, int out_stride, int k, int ox, int oy, int oz) {
+    constexpr int NF = ANISO ? 6 : 1;
+    const int cur = CUR ? CUR : cur_rt;
+    const int d = cur >> 1;
+    const LevelInfo &ol = a.out[k];
+    const int sh = k + 1;
+    const int gx = ox >> sh, gy = oy >> sh, gz = oz >> sh;
+    const int d3 = d * d * d;
+    for (int i = threadIdx.x; i < d3 * NF; i += 256) {
+        const int f = i / d3;
+        const int vi = i - f * d3;
+        const int x = vi % d, y = (vi / d) % d, z = vi / (d * d);
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (!ANISO) {
+            // BOX2 offsets in filterRadiance.comp order: (0,0,0),(0,0,1),(0,1,0),(0,1,1),(1,0,0),...
+#pragma unroll
+            for (int o = 0; o < 8; ++o) {
+                const int dx = (o >> 2) & 1, dy = (o >> 1) & 1, dz = o & 1;
+                const float4 t = in[((2 * z + dz) * cur + (2 * y + dy)) * cur + (2 * x + dx)];
+                v.x = v.x + t.x;
+                v.y = v.y + t.y;
+                v.z = v.z + t.z;
+                v.w = v.w + t.w;
+            }
+            v.x = v.x * 0.125f;
+            v.y = v.y * 0.125f;
+            v.z = v.z * 0.125f;
+            v.w = v.w * 0.125f;
+        } else {
+            const int ax = f >> 1, neg = f & 1;
+            const int sf = faces == 1 ? 0 : f;
+            const float4 *src = in + (size_t)sf * in_stride;
+            const int oa = ax == 0 ? 1 : 0, ob = ax == 2 ? 1 : 2;
+#pragma unroll
+            for (int pa = 0; pa < 2; ++pa)
+#pragma unroll
+                for (int pb = 0; pb < 2; ++pb) {
+                    int cn[3] = {2 * x, 2 * y, 2 * z}, cf[3] = {2 * x, 2 * y, 2 * z};
+                    cn[oa] += pa;
+                    cf[oa] += pa;
+                    cn[ob] += pb;
+                    cf[ob] += pb;
+                    cn[ax] += neg ? 1 : 0;
+                    cf[ax] += neg ? 0 : 1;
+                    const float4 pn = src[(cn[2] * cur + cn[1]) * cur + cn[0]];
+                    const float4 pf = src[(cf[2] * cur + cf[1]) * cur + cf[0]];
+                    const float t = 1.0f - pn.w;
+                    v.x = v.x + (pn.x + t * pf.x);
+                    v.y = v.y + (pn.y + t * pf.y);
+                    v.z = v.z + (pn.z + t * pf.z);
+                    v.w = v.w + (pn.w + t * pf.w);
+                }
+            v.x = v.x * 0.25f;
+            v.y = v.y * 0.25f;
+            v.z = v.z * 0.25f;
+            v.w = v.w * 0.25f;
+        }
+        v = round_stage(v, a.fp32);
+        out[(size_t)f * out_stride + vi] = v;
+        store_stage_texel(a.pyr, ol, gx + x, gy + y, gz + z, NF == 6 ? f : 0, v);
+    }
+}
+
 // ANISO: outputs have six faces; SRC6: the source level has six faces (any
 // anisotropic level >= 1).  LDS: the 8^3 source region (per face), then the
 // ping-pong buffers of the three output steps (4^3, 2^3, 1^3 per face).
@@ -86,6 +155,18 @@ __global__ __launch_bounds__(256) void k_mip_chain(MipArgs a) {
         }
     }
     __syncthreads();
+    if (R == 8) {  // the three steps with their sizes known at compile time
+        mip_step<ANISO, 8>(a, 8, SF, &bufA[0][0], 512, &bufB[0][0], 64, 0, ox, oy, oz);
+        if (a.nout > 1) {
+            __syncthreads();
+            mip_step<ANISO, 4>(a, 4, NF, &bufB[0][0], 64, &bufA[0][0], 8, 1, ox, oy, oz);
+        }
+        if (a.nout > 2) {
+            __syncthreads();
+            mip_step<ANISO, 2>(a, 2, NF, &bufA[0][0], 8, &bufB[0][0], 64, 2, ox, oy, oz);
+        }
+        return;
+    }
     int cur = R, faces = SF;
     float4 *in = &bufA[0][0];
     int in_stride = 512;
@@ -96,63 +177,7 @@ __global__ __launch_bounds__(256) void k_mip_chain(MipArgs a) {
         // bufA (k odd, <= 2^3 per face)
         float4 *out = (k & 1) ? &bufA[0][0] : &bufB[0][0];
         const int out_stride = (k & 1) ? 8 : 64;
-        const LevelInfo &ol = a.out[k];
-        const int sh = k + 1;
-        const int gx = ox >> sh, gy = oy >> sh, gz = oz >> sh;
-        const int d3 = d * d * d;
-        for (int i = tid; i < d3 * NF; i += 256) {
-            const int f = i / d3;
-            const int vi = i - f * d3;
-            const int x = vi % d, y = (vi / d) % d, z = vi / (d * d);
-            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (!ANISO) {
-                // BOX2 offsets in filterRadiance.comp order: (0,0,0),(0,0,1),(0,1,0),(0,1,1),(1,0,0),...
-#pragma unroll
-                for (int o = 0; o < 8; ++o) {
-                    const int dx = (o >> 2) & 1, dy = (o >> 1) & 1, dz = o & 1;
-                    const float4 t = in[((2 * z + dz) * cur + (2 * y + dy)) * cur + (2 * x + dx)];
-                    v.x = v.x + t.x;
-                    v.y = v.y + t.y;
-                    v.z = v.z + t.z;
-                    v.w = v.w + t.w;
-                }
-                v.x = v.x * 0.125f;
-                v.y = v.y * 0.125f;
-                v.z = v.z * 0.125f;
-                v.w = v.w * 0.125f;
-            } else {
-                const int ax = f >> 1, neg = f & 1;
-                const int sf = faces == 1 ? 0 : f;
-                const float4 *src = in + (size_t)sf * in_stride;
-                const int oa = ax == 0 ? 1 : 0, ob = ax == 2 ? 1 : 2;
-#pragma unroll
-                for (int pa = 0; pa < 2; ++pa)
-#pragma unroll
-                    for (int pb = 0; pb < 2; ++pb) {
-                        int cn[3] = {2 * x, 2 * y, 2 * z}, cf[3] = {2 * x, 2 * y, 2 * z};
-                        cn[oa] += pa;
-                        cf[oa] += pa;
-                        cn[ob] += pb;
-                        cf[ob] += pb;
-                        cn[ax] += neg ? 1 : 0;
-                        cf[ax] += neg ? 0 : 1;
-                        const float4 pn = src[(cn[2] * cur + cn[1]) * cur + cn[0]];
-                        const float4 pf = src[(cf[2] * cur + cf[1]) * cur + cf[0]];
-                        const float t = 1.0f - pn.w;
-                        v.x = v.x + (pn.x + t * pf.x);
-                        v.y = v.y + (pn.y + t * pf.y);
-                        v.z = v.z + (pn.z + t * pf.z);
-                        v.w = v.w + (pn.w + t * pf.w);
-                    }
-                v.x = v.x * 0.25f;
-                v.y = v.y * 0.25f;
-                v.z = v.z * 0.25f;
-                v.w = v.w * 0.25f;
-            }
-            v = round_stage(v, a.fp32);
-            out[(size_t)f * out_stride + vi] = v;
-            store_stage_texel(a.pyr, ol, gx + x, gy + y, gz + z, NF == 6 ? f : 0, v);
-        }
+        mip_step<ANISO, 0>(a, cur, faces, in, in_stride, out, out_stride, k, ox, oy, oz);
         __syncthreads();
         in = out;
         in_stride = out_stride;
