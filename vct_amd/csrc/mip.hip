@@ -102,6 +102,68 @@ __device__ __forceinline__ void mip_step(const MipArgs &a, int cur_rt, int faces
     }
 }
 
+// The first step of an 8^3 region (4^3 texels per face): wavefront w filters
+// faces w and w + 4, one texel per lane.  The face is wave-uniform, so the
+// child offsets of its near/far pairs are scalar values and only the texel's
+// base address is per lane.  Same taps, in the same order, as mip_step.
+template <bool ANISO, bool SRC6>
+__device__ __forceinline__ void mip_step8(const MipArgs &a, const float4 *in, float4 *out, int ox, int oy, int oz) {
+    constexpr int NF = ANISO ? 6 : 1;
+    constexpr int cur = 8;
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int lane = threadIdx.x & 63;
+    const int x = lane & 3, y = (lane >> 2) & 3, z = lane >> 4;
+    const int base = ((2 * z) * cur + 2 * y) * cur + 2 * x;
+    const LevelInfo &ol = a.out[0];
+    for (int f = w; f < NF; f += 4) {
+        const float4 *c = in + (SRC6 ? f * 512 : 0) + base;
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (!ANISO) {
+#pragma unroll
+            for (int o = 0; o < 8; ++o) {
+                const int dx = (o >> 2) & 1, dy = (o >> 1) & 1, dz = o & 1;
+                const float4 t = c[(dz * cur + dy) * cur + dx];
+                v.x = v.x + t.x;
+                v.y = v.y + t.y;
+                v.z = v.z + t.z;
+                v.w = v.w + t.w;
+            }
+            v.x = v.x * 0.125f;
+            v.y = v.y * 0.125f;
+            v.z = v.z * 0.125f;
+            v.w = v.w * 0.125f;
+        } else {
+            const int ax = f >> 1, neg = f & 1;
+            // strides of the face's axis and of the two axes across it (oa, ob
+            // as in mip_step: y, z for x; x, z for y; x, y for z)
+            const int s_ax = ax == 0 ? 1 : (ax == 1 ? cur : cur * cur);
+            const int s_oa = ax == 0 ? cur : 1;
+            const int s_ob = ax == 2 ? cur : cur * cur;
+            const int near = neg ? s_ax : 0, far = neg ? 0 : s_ax;
+#pragma unroll
+            for (int pa = 0; pa < 2; ++pa)
+#pragma unroll
+                for (int pb = 0; pb < 2; ++pb) {
+                    const int o = pa * s_oa + pb * s_ob;
+                    const float4 pn = c[o + near];
+                    const float4 pf = c[o + far];
+                    const float t = 1.0f - pn.w;
+                    v.x = v.x + (pn.x + t * pf.x);
+                    v.y = v.y + (pn.y + t * pf.y);
+                    v.z = v.z + (pn.z + t * pf.z);
+                    v.w = v.w + (pn.w + t * pf.w);
+                }
+            v.x = v.x * 0.25f;
+            v.y = v.y * 0.25f;
+            v.z = v.z * 0.25f;
+            v.w = v.w * 0.25f;
+        }
+        v = round_stage(v, a.fp32);
+        out[f * 64 + lane] = v;
+        store_stage_texel(a.pyr, ol, (ox >> 1) + x, (oy >> 1) + y, (oz >> 1) + z, NF == 6 ? f : 0, v);
+    }
+}
+
 // ANISO: outputs have six faces; SRC6: the source level has six faces (any
 // anisotropic level >= 1).  LDS: the 8^3 source region (per face), then the
 // ping-pong buffers of the three output steps (4^3, 2^3, 1^3 per face).
@@ -156,7 +218,7 @@ __global__ __launch_bounds__(256) void k_mip_chain(MipArgs a) {
     }
     __syncthreads();
     if (R == 8) {  // the three steps with their sizes known at compile time
-        mip_step<ANISO, 8>(a, 8, SF, &bufA[0][0], 512, &bufB[0][0], 64, 0, ox, oy, oz);
+        mip_step8<ANISO, SRC6>(a, &bufA[0][0], &bufB[0][0], ox, oy, oz);
         if (a.nout > 1) {
             __syncthreads();
             mip_step<ANISO, 4>(a, 4, NF, &bufB[0][0], 64, &bufA[0][0], 8, 1, ox, oy, oz);
