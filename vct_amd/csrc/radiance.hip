@@ -22,8 +22,15 @@ struct InjectLight {
     int N, perspective;
 };
 
-__global__ __launch_bounds__(256) void k_inject_scatter(InjectLight L, int dim, Extent ext, const float4 *warp,
+// Up to kScatterBatch shadow-casting lights per launch, one per grid z slice.
+constexpr int kScatterBatch = 8;
+struct InjectBatch {
+    InjectLight l[kScatterBatch];
+};
+
+__global__ __launch_bounds__(256) void k_inject_scatter(InjectBatch B, int dim, Extent ext, const float4 *warp,
                                                         uint8_t *litmask) {
+    const InjectLight &L = B.l[blockIdx.z];
     const int i = blockIdx.x * 16 + (threadIdx.x & 15);
     const int j = blockIdx.y * 16 + (threadIdx.x >> 4);
     if (i >= L.N || j >= L.N) return;
@@ -52,148 +59,169 @@ hipError_t launch_inject_scatter(const DevLight *lights, const DevLight *dlights
     const size_t nv = (size_t)dim * dim * dim;
     hipError_t e = launch_fill_u32((uint32_t *)litmask, 0u, nv / 4, s);
     if (e != hipSuccess) return e;
-    for (int li = 0; li < nlights; ++li) {
-        const DevLight &L = lights[li];
-        if (!((light_mask >> li) & 1u) || !L.shadow_caster || !L.shadowmap) continue;
-        InjectLight il;
-        for (int k = 0; k < 16; ++k) il.ls_inverse[k] = L.ls_inverse[k];
-        il.shadowmap = L.shadowmap;
-        il.N = (int)L.shadow_size;
-        il.perspective = (int)L.perspective;
-        dim3 grid((il.N + 15) / 16, (il.N + 15) / 16);
-        hipLaunchKernelGGL(k_inject_scatter, grid, dim3(256), 0, s, il, dim, ext, warp, litmask);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
+    InjectBatch B;
+    int nb = 0, nmax = 0;
+    for (int li = 0; li <= nlights; ++li) {
+        if (li < nlights) {
+            const DevLight &L = lights[li];
+            if (!((light_mask >> li) & 1u) || !L.shadow_caster || !L.shadowmap) continue;
+            InjectLight &il = B.l[nb++];
+            for (int k = 0; k < 16; ++k) il.ls_inverse[k] = L.ls_inverse[k];
+            il.shadowmap = L.shadowmap;
+            il.N = (int)L.shadow_size;
+            il.perspective = (int)L.perspective;
+            nmax = il.N > nmax ? il.N : nmax;
+        }
+        if (nb == kScatterBatch || (li == nlights && nb > 0)) {  // each stored texel writes 1: order-free
+            dim3 grid((nmax + 15) / 16, (nmax + 15) / 16, nb);
+            hipLaunchKernelGGL(k_inject_scatter, grid, dim3(256), 0, s, B, dim, ext, warp, litmask);
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+            nb = 0;
+            nmax = 0;
+        }
     }
     return hipSuccess;
 }
 
-// One thread per 4-voxel x-row of a brick (16 rows per brick): a wavefront
-// covers 4 consecutive bricks, so albedo reads are 64-byte runs and radiance
-// writes are whole 512-byte (fp16) bricks.  The grid strides over the rows
-// with a fixed number of workgroups, each of which reduces its statistics in
-// LDS and writes one partial.
-// TEMPORAL (temporalFilterRadiance): the row's previous level-0 values are read
-// before they are overwritten, and a voxel that is not lit this frame gets
+// Level-0 radiance of one voxel (transferVoxels + injectRadiance): albedo
+// word a32 with its opacity rewritten (nw), and the injected value.
+// TEMPORAL (temporalFilterRadiance): a voxel that is not lit this frame gets
 // mix(previous, (0,0,0,a), 1 - decay) = previous * (1 - w) + (0,0,0,a) * w with
 // w = 1 - decay (GLSL mix, transferVoxels.comp:55-58); a lit voxel is
 // overwritten with its albedo, (0,0,0,0) if it is empty (injectRadiance.comp:
 // 99-101 stores packUnorm4x8(voxelColor) there).
+template <bool TEMPORAL>
+__device__ __forceinline__ float4 inject_texel(uint32_t a32, bool lit, float opacity, float4 prev, float mw, float pw,
+                                               uint32_t *nw) {
+    const float ca = unorm8(a32, 3);
+    float4 r;
+    *nw = a32;
+    if (ca > 0.0f) {
+        const float c0 = unorm8(a32, 0), c1 = unorm8(a32, 1), c2 = unorm8(a32, 2);
+        const float na = opacity > 0.0f ? opacity : ca;
+        *nw = pack_unorm4x8(c0, c1, c2, na);
+        r = make_float4(lit ? c0 : 0.0f, lit ? c1 : 0.0f, lit ? c2 : 0.0f, na);
+        if (TEMPORAL && !lit)
+            r = make_float4(prev.x * pw + 0.0f * mw, prev.y * pw + 0.0f * mw, prev.z * pw + 0.0f * mw,
+                            prev.w * pw + na * mw);
+    } else {
+        r = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (TEMPORAL && !lit)
+            r = make_float4(prev.x * pw + 0.0f * mw, prev.y * pw + 0.0f * mw, prev.z * pw + 0.0f * mw,
+                            prev.w * pw + 0.0f * mw);
+    }
+    return r;
+}
+
+// One wavefront per 4^3 brick, one voxel per lane in the brick's element
+// order (lane = (x&3) | (y&3) << 2 | (z&3) << 4, the same for the bricked
+// albedo grid and for level 0), so albedo reads are one 256-byte run and each
+// level-0 store is one contiguous run of the brick's elements: 1 KB of fp16
+// pairs or fp32 texels.  An fp16 pair element x holds texels x and x+1: the
+// second comes from the next lane, or for the last voxel of a brick row from
+// that voxel's own inputs in the next brick (texel dim is the zero border).
+// The grid strides over the bricks with a fixed number of workgroups, each of
+// which reduces its statistics in LDS and writes one partial.
+// The opacity rewrite of the albedo word (rewrite = 1) races with the read of
+// that word by the neighbouring brick's wavefront; either value gives the same
+// texel, except when the opacity rounds to an alpha byte of 0 -- then the
+// launcher passes rewrite = 0 and rewrites in a pass of its own afterwards.
 template <int FMT, bool TEMPORAL>
 __global__ __launch_bounds__(256) void k_normalize_inject(uint32_t *albedo, const uint8_t *__restrict__ litmask,
                                                           int dim, LevelInfo L0, float opacity, float decay,
-                                                          char *pyr, uint32_t *partials) {
+                                                          int rewrite, char *pyr, uint32_t *partials) {
     const float mw = 1.0f - decay;  // the mix weight of this frame's value
     const float pw = 1.0f - mw;     // and of the previous one, as GLSL mix computes it
     const int nb = L0.nb;
-    const size_t nrows = (size_t)nb * nb * nb * 16;
+    const uint32_t nbricks = (uint32_t)nb * nb * nb;
+    const int lane = threadIdx.x & 63;
+    const int vx = lane & 3, vy = (lane >> 2) & 3, vz = lane >> 4;
     uint32_t uniq = 0, maxf = 0, lit_count = 0;
-    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < nrows; g += (size_t)gridDim.x * blockDim.x) {
-        const size_t b = g >> 4;
-        const int q = (int)(g & 15);
-        const int bx = (int)(b % nb), by = (int)((b / nb) % nb), bz = (int)(b / ((size_t)nb * nb));
-        const int x0 = bx * 4, y = by * 4 + (q & 3), z = bz * 4 + (q >> 2);
-        const size_t li = ((size_t)z * dim + y) * dim + x0;  // the lit mask is linear
-        const size_t gi = (b << 6) + (size_t)q * 4;           // the albedo grid is bricked (grid_index)
-        const uint4 a = *(const uint4 *)&albedo[gi];
-        const uint32_t lw = *(const uint32_t *)&litmask[li];  // 4 flag bytes
-        uint32_t w[4] = {a.x, a.y, a.z, a.w};
-        float r[4][4];
-        bool changed = false;
-        char *row = pyr + L0.stage + (size_t)elem_index(L0, x0, y, z, 0) * 16;
-        float4 prev[4];
-        if (TEMPORAL) {  // the four texels of the row as stored last frame
+    for (uint32_t b = blockIdx.x * 4u + (threadIdx.x >> 6); b < nbricks; b += gridDim.x * 4u) {
+        const int bx = (int)(b % (uint32_t)nb), by = (int)((b / (uint32_t)nb) % (uint32_t)nb),
+                  bz = (int)(b / ((uint32_t)nb * nb));
+        const int x = bx * 4 + vx, y = by * 4 + vy, z = bz * 4 + vz;
+        const size_t gi = ((size_t)b << 6) + (size_t)lane;  // grid_index(x, y, z) = elem_index(L0, x, y, z)
+        const size_t li = ((size_t)z * dim + y) * dim + x;   // the lit mask is linear
+        const uint32_t a32 = albedo[gi];
+        const bool lit = litmask[li] != 0;
+        char *elem = pyr + L0.stage + gi * 16;
+        float4 prev = make_float4(0.f, 0.f, 0.f, 0.f), prev1 = prev;
+        if (TEMPORAL) {  // this element as stored last frame (texels x, x+1 for fp16 pairs)
+            const uint4 e = *(const uint4 *)elem;
             if (FMT == 1) {
-#pragma unroll
-                for (int v = 0; v < 4; ++v) prev[v] = ((const float4 *)row)[v];
-            } else {  // fp16 pairs: element x0 holds texels x0, x0+1; element x0+2 texels x0+2, x0+3
-                const uint4 e0 = ((const uint4 *)row)[0], e2 = ((const uint4 *)row)[2];
-                prev[0] = unpack_h4(e0.x, e0.y);
-                prev[1] = unpack_h4(e0.z, e0.w);
-                prev[2] = unpack_h4(e2.x, e2.y);
-                prev[3] = unpack_h4(e2.z, e2.w);
-            }
-        }
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const uint32_t a32 = w[v];
-            const float ca = unorm8(a32, 3);
-            if (ca > 0.0f) {
-                uniq += 1;
-                const uint32_t mf = (uint32_t)(255.0f * ca);
-                maxf = mf > maxf ? mf : maxf;
-                const float c0 = unorm8(a32, 0), c1 = unorm8(a32, 1), c2 = unorm8(a32, 2);
-                const float na = opacity > 0.0f ? opacity : ca;
-                const uint32_t nw = pack_unorm4x8(c0, c1, c2, na);
-                if (nw != a32) changed = true;
-                w[v] = nw;
-                const bool lit = ((lw >> (8 * v)) & 0xffu) != 0u;
-                lit_count += lit ? 1u : 0u;
-                r[v][0] = lit ? c0 : 0.0f;
-                r[v][1] = lit ? c1 : 0.0f;
-                r[v][2] = lit ? c2 : 0.0f;
-                r[v][3] = na;
-                if (TEMPORAL && !lit) {
-                    r[v][0] = prev[v].x * pw + 0.0f * mw;
-                    r[v][1] = prev[v].y * pw + 0.0f * mw;
-                    r[v][2] = prev[v].z * pw + 0.0f * mw;
-                    r[v][3] = prev[v].w * pw + na * mw;
-                }
+                prev = make_float4(__uint_as_float(e.x), __uint_as_float(e.y), __uint_as_float(e.z),
+                                   __uint_as_float(e.w));
             } else {
-                r[v][0] = r[v][1] = r[v][2] = r[v][3] = 0.0f;
-                if (TEMPORAL && ((lw >> (8 * v)) & 0xffu) == 0u) {
-                    r[v][0] = prev[v].x * pw + 0.0f * mw;
-                    r[v][1] = prev[v].y * pw + 0.0f * mw;
-                    r[v][2] = prev[v].z * pw + 0.0f * mw;
-                    r[v][3] = prev[v].w * pw + 0.0f * mw;
-                }
+                prev = unpack_h4(e.x, e.y);
+                prev1 = unpack_h4(e.z, e.w);
             }
         }
-        if (changed) *(uint4 *)&albedo[gi] = make_uint4(w[0], w[1], w[2], w[3]);
-        // the row's four elements are contiguous (level 0 is isotropic: one face)
-        if (FMT == 1) {
-            float4 *o = (float4 *)row;
-#pragma unroll
-            for (int v = 0; v < 4; ++v) o[v] = make_float4(r[v][0], r[v][1], r[v][2], r[v][3]);
-        } else {
-            // fp16 pairs: elements x0 .. x0+2 complete, the first half of x0+3
-            // (its second half is the next row's texel x0+4, written by that
-            // row's thread, or the border), and the second half of x0-1
-            uint2 h[4];
-#pragma unroll
-            for (int v = 0; v < 4; ++v) h[v] = pack_h4(make_float4(r[v][0], r[v][1], r[v][2], r[v][3]));
-            uint4 *o = (uint4 *)row;
-            o[0] = make_uint4(h[0].x, h[0].y, h[1].x, h[1].y);
-            o[1] = make_uint4(h[1].x, h[1].y, h[2].x, h[2].y);
-            o[2] = make_uint4(h[2].x, h[2].y, h[3].x, h[3].y);
-            *(uint2 *)(o + 3) = h[3];
-            if (x0 > 0) *(uint2 *)(pyr + L0.stage + (size_t)elem_index(L0, x0 - 1, y, z, 0) * 16 + 8) = h[0];
+        uint32_t nw;
+        const float4 r = inject_texel<TEMPORAL>(a32, lit, opacity, prev, mw, pw, &nw);
+        if (unorm8(a32, 3) > 0.0f) {
+            uniq += 1;
+            const uint32_t mf = (uint32_t)(255.0f * unorm8(a32, 3));
+            maxf = mf > maxf ? mf : maxf;
+            lit_count += lit ? 1u : 0u;
         }
+        if (rewrite && nw != a32) albedo[gi] = nw;
+        if (FMT == 1) {
+            *(float4 *)elem = r;
+            continue;
+        }
+        const uint2 h = pack_h4(r);
+        uint2 h1 = make_uint2(__shfl_down(h.x, 1, 64), __shfl_down(h.y, 1, 64));  // texel x+1 in lane + 1
+        if (vx == 3) {
+            h1 = make_uint2(0u, 0u);  // texel dim: the border
+            if (x + 1 < dim) {        // voxel x+1 opens brick b+1 along x
+                const uint32_t a1 = albedo[gi + 64 - 3];
+                const bool lit1 = litmask[li + 1] != 0;
+                uint32_t nw1;
+                h1 = pack_h4(inject_texel<TEMPORAL>(a1, lit1, opacity, prev1, mw, pw, &nw1));
+            }
+        }
+        *(uint4 *)elem = make_uint4(h.x, h.y, h1.x, h1.y);
     }
     block_partial(uniq, maxf, lit_count, 0u, partials);
+}
+
+// The opacity rewrite on its own (transferVoxels.comp:47-50), after the
+// injection, for opacities whose alpha byte is 0.
+__global__ __launch_bounds__(256) void k_rewrite_opacity(uint32_t *albedo, size_t n, float opacity) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t a32 = albedo[i];
+        uint32_t nw;
+        inject_texel<false>(a32, false, opacity, make_float4(0.f, 0.f, 0.f, 0.f), 0.f, 0.f, &nw);
+        if (nw != a32) albedo[i] = nw;
+    }
 }
 
 hipError_t launch_normalize_inject(uint32_t *albedo, const uint8_t *litmask, int dim, float opacity, bool temporal,
                                    float decay, const PyrDesc &pyr, vct_stats *stats, uint32_t *partials,
                                    hipStream_t s) {
     const LevelInfo &L0 = pyr.lv[0];
-    const size_t nrows = (size_t)L0.nb * L0.nb * L0.nb * 16;
-    size_t blocks = (nrows + 255) / 256;
+    const size_t nbricks = (size_t)L0.nb * L0.nb * L0.nb;
+    size_t blocks = (nbricks + 3) / 4;
     if (blocks > (size_t)kStatPartials) blocks = kStatPartials;
     dim3 grid((unsigned)blocks);
-    if (L0.fmt == 1 && temporal)
-        hipLaunchKernelGGL((k_normalize_inject<1, true>), grid, dim3(256), 0, s, albedo, litmask, dim, L0, opacity,
-                           decay, pyr.pyr, partials);
-    else if (L0.fmt == 1)
-        hipLaunchKernelGGL((k_normalize_inject<1, false>), grid, dim3(256), 0, s, albedo, litmask, dim, L0, opacity,
-                           decay, pyr.pyr, partials);
-    else if (temporal)
-        hipLaunchKernelGGL((k_normalize_inject<0, true>), grid, dim3(256), 0, s, albedo, litmask, dim, L0, opacity,
-                           decay, pyr.pyr, partials);
-    else
-        hipLaunchKernelGGL((k_normalize_inject<0, false>), grid, dim3(256), 0, s, albedo, litmask, dim, L0, opacity,
-                           decay, pyr.pyr, partials);
+    // pack_unorm4x8 rounds opacity * 255: below 0.5 the rewritten alpha byte is 0
+    const int rewrite = (opacity > 0.0f && opacity * 255.0f < 0.5f) ? 0 : 1;
+#define VCT_INJECT_LAUNCH(F, T)                                                                                     \
+    hipLaunchKernelGGL((k_normalize_inject<F, T>), grid, dim3(256), 0, s, albedo, litmask, dim, L0, opacity, decay, \
+                       rewrite, pyr.pyr, partials)
+    if (L0.fmt == 1 && temporal) VCT_INJECT_LAUNCH(1, true);
+    else if (L0.fmt == 1) VCT_INJECT_LAUNCH(1, false);
+    else if (temporal) VCT_INJECT_LAUNCH(0, true);
+    else VCT_INJECT_LAUNCH(0, false);
+#undef VCT_INJECT_LAUNCH
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    if (!rewrite) {
+        const size_t nv = (size_t)dim * dim * dim;
+        hipLaunchKernelGGL(k_rewrite_opacity, dim3(2048), dim3(256), 0, s, albedo, nv, opacity);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
     // uniqueVoxels, maxFragmentsPerVoxel (transferVoxels.comp:39-45) and the lit count
     return launch_reduce_stats(partials, (int)blocks, &stats->unique_voxels, &stats->max_fragments_per_voxel,
                                &stats->lit_voxels, s);
