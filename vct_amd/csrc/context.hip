@@ -204,15 +204,27 @@ static int alloc_grids(vct_ctx *c) {
         L.nb = (L.size + 3) / 4;
         L.fmt = P.storage == VCT_STORE_FP32 ? 1 : ((P.storage == VCT_STORE_FP8 && l >= 3) ? 2 : 0);
         L.nfaces = (P.aniso && l >= 1) ? 6 : 1;
+        const uint32_t nbp = (uint32_t)L.nb + 2;  // a border brick on each side
         L.sx = 64u * (uint32_t)L.nfaces;
-        L.sy = L.sx * (uint32_t)L.nb;
-        L.sz = L.sy * (uint32_t)L.nb;
-        const size_t nelem = (size_t)L.sz * L.nb;
-        L.base = (uint32_t)off;
-        off += align256(nelem * (L.fmt == 2 ? 4 : 16));
+        L.sy = L.sx * nbp;
+        L.sz = L.sy * nbp;
+        L.pad = L.sx + L.sy + L.sz;
+        const size_t nelem = (size_t)L.sz * nbp;
+        if (nelem > 0xffffffffull)
+            return fail(c, VCT_ERR_INVALID, "radiance level %d of %zu elements exceeds the 32-bit range", l, nelem);
+        L.nelem = (uint32_t)nelem;
+        const size_t esz = L.fmt == 2 ? 4 : 16;
+        if (off + align256(nelem * esz) > 0xffffff00ull)
+            return fail(c, VCT_ERR_INVALID, "radiance pyramid exceeds the 32-bit offset range at level %d", l);
+        L.base = (uint32_t)(off + (size_t)L.pad * esz);
+        off += align256(nelem * esz);
         if (L.fmt == 2) {
-            L.stage = (uint32_t)off;
+            if (off + align256(nelem * 16) > 0xffffff00ull)
+                return fail(c, VCT_ERR_INVALID, "radiance pyramid exceeds the 32-bit offset range at level %d", l);
+            L.stage = (uint32_t)(off + (size_t)L.pad * 16);
             off += align256(nelem * 16);
+            // scales[scale + element / 64] for elements relative to base: the
+            // interior bricks (the border holds zero codes, read by nobody)
             L.scale = (uint32_t)soff;
             soff += nelem / 64;
         } else {
@@ -225,7 +237,7 @@ static int alloc_grids(vct_ctx *c) {
     c->pyr_bytes = off;
     c->scale_floats = soff;
     HIPCHK(c, dalloc(&c->d_pyr, off));
-    HIPCHK(c, hipMemsetAsync(c->d_pyr, 0, off, c->stream));  // the fp16-pair border halves stay 0
+    HIPCHK(c, hipMemsetAsync(c->d_pyr, 0, off, c->stream));  // the border bricks and halves stay 0
     if (soff) {
         HIPCHK(c, dalloc(&c->d_scales, soff * 4));
         HIPCHK(c, hipMemsetAsync(c->d_scales, 0, soff * 4, c->stream));
@@ -681,7 +693,7 @@ int vct_fill_holes(vct_ctx *c) {
     if (!c->have_radiance) return fail(c, VCT_ERR_STATE, "vct_fill_holes: inject first");
     const LevelInfo &L0 = c->pyr.lv[0];
     if (!c->d_fill_tmp) {  // level 0's layout; its fp16-pair border halves stay 0
-        const size_t bytes = (size_t)L0.sz * L0.nb * 16;
+        const size_t bytes = (size_t)L0.nelem * 16;
         HIPCHK(c, dalloc((char **)&c->d_fill_tmp, bytes));
         HIPCHK(c, hipMemsetAsync(c->d_fill_tmp, 0, bytes, c->stream));
     }
@@ -837,7 +849,7 @@ size_t vct_buffer_bytes(const vct_ctx *c, int what, int level) {
     case VCT_BUF_RADIANCE_RAW: {
         if (!decode_level(c, level, &l, &f)) return 0;
         const LevelInfo &L = c->pyr.lv[l];
-        return (size_t)L.sz * L.nb * (L.fmt == 2 ? 4 : 16);
+        return (size_t)L.nelem * (L.fmt == 2 ? 4 : 16);
     }
     case VCT_BUF_SHADOWMAP:
         if (level < 0 || level >= (int)c->lights.size() || !c->lights[level].shadow_caster) return 0;
@@ -927,7 +939,10 @@ int vct_download(vct_ctx *c, int what, int level, void *host, size_t bytes) {
     }
     case VCT_BUF_RADIANCE_RAW:
         decode_level(c, level, &l, &f);
-        src = c->pyr.pyr + c->pyr.lv[l].base;
+        {  // the stored level, border bricks included
+            const LevelInfo &L = c->pyr.lv[l];
+            src = c->pyr.pyr + L.base - (size_t)L.pad * (L.fmt == 2 ? 4 : 16);
+        }
         break;
     default: return fail(c, VCT_ERR_INVALID, "vct_download: unknown buffer %d", what);
     }

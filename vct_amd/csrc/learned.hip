@@ -181,23 +181,26 @@ __device__ __forceinline__ h4 as_h4(uint32_t a, uint32_t b) {
     r.ba = __builtin_bit_cast(h2, b);
     return r;
 }
-// Texels x0 .. x0+3 of row (y, z), zero outside the level: the two pair
-// elements at clamped (always valid) indices (row_fetch), then selects
-// (row_select).  Element e holds texels (e, e+1); with x0 = -1 texel 0 is the
-// first half of element 0, and likewise for x0 + 2 = -1.  SCALAR: x0, y, z
-// are the same in every active lane, and the elements come through scalar
-// loads (the pyramid is not written while the gather runs).
+// Texels x0 .. x0+3 of row (y, z), zero outside the level: pair elements x0
+// (texels x0, x0+1) and x0+2.  A block with first texel i0 - 1, -2 <= i0 <= s
+// on every axis (always so for a point in [0,1]^3), spans texels -3 .. s+2,
+// inside the zero border bricks around the level (element -1 holds
+// (0, texel 0)), so no index is clamped and no texel masked; element indices
+// below 0 wrap, and the byte offsets are formed in 32 bits.  A block further
+// out (in = false) is all zero and is not read.  SCALAR: x0, y, z are the same
+// in every active lane, and the elements come through scalar loads (the
+// pyramid is not written while the gather runs).
 struct RowRaw {
     uint4 a, b;
 };
 template <bool SCALAR>
-__device__ __forceinline__ void row_fetch(const PyrView &pv, const LevelInfo &L, int x0, int y, int z, RowRaw &r) {
-    const int s = L.size;
-    const int yc = y < 0 ? 0 : (y > s - 1 ? s - 1 : y), zc = z < 0 ? 0 : (z > s - 1 ? s - 1 : z);
-    const int ea = x0 < 0 ? 0 : (x0 > s - 1 ? s - 1 : x0);
-    const int xb = x0 + 2;
-    const int eb = xb < 0 ? 0 : (xb > s - 1 ? s - 1 : xb);
-    const uint32_t oa = L.base + elem_index(L, ea, yc, zc, 0) * 16u, ob = L.base + elem_index(L, eb, yc, zc, 0) * 16u;
+__device__ __forceinline__ void row_fetch(const PyrView &pv, const LevelInfo &L, int x0, int y, int z, bool in,
+                                          RowRaw &r) {
+    if (!in) {
+        r.a = r.b = make_uint4(0u, 0u, 0u, 0u);
+        return;
+    }
+    const uint32_t oa = L.base + elem_index(L, x0, y, z, 0) * 16u, ob = L.base + elem_index(L, x0 + 2, y, z, 0) * 16u;
     if (SCALAR) {
         r.a = sload16(pv.pyr, oa);
         r.b = sload16(pv.pyr, ob);
@@ -206,18 +209,11 @@ __device__ __forceinline__ void row_fetch(const PyrView &pv, const LevelInfo &L,
         r.b = *(const uint4 *)(pv.pyr + ob);
     }
 }
-__device__ __forceinline__ void row_select(const RowRaw &r, int s, int x0, int y, int z, h4 t[4]) {
-    const bool yz = (unsigned)y < (unsigned)s && (unsigned)z < (unsigned)s;
-    const int xb = x0 + 2;
-    const h4 zero = as_h4(0u, 0u);
-    const h4 alo = as_h4(r.a.x, r.a.y), ahi = as_h4(r.a.z, r.a.w), blo = as_h4(r.b.x, r.b.y),
-             bhi = as_h4(r.b.z, r.b.w);
-    const bool v0 = yz && (unsigned)x0 < (unsigned)s, v1 = yz && (unsigned)(x0 + 1) < (unsigned)s;
-    const bool v2 = yz && (unsigned)xb < (unsigned)s, v3 = yz && (unsigned)(xb + 1) < (unsigned)s;
-    t[0] = v0 ? alo : zero;
-    t[1] = v1 ? (x0 >= 0 ? ahi : alo) : zero;
-    t[2] = v2 ? blo : zero;
-    t[3] = v3 ? (xb >= 0 ? bhi : blo) : zero;
+__device__ __forceinline__ void row_select(const RowRaw &r, h4 t[4]) {
+    t[0] = as_h4(r.a.x, r.a.y);
+    t[1] = as_h4(r.a.z, r.a.w);
+    t[2] = as_h4(r.b.x, r.b.y);
+    t[3] = as_h4(r.b.z, r.b.w);
 }
 
 __device__ __forceinline__ float4 lerp4(float4 a, float4 b, float f) {
@@ -233,6 +229,7 @@ __device__ __forceinline__ float4 lerp4(float4 a, float4 b, float f) {
 struct FeatState {
     float p[3];     // texture-space position
     int i0[3];      // first texel of the 4x4x4 level-2 block is i0 - 1
+    bool in;        // -2 <= i0 <= size on every axis: the block reaches the level
     h2 fx, gx, fy, gy;
     float fz, gz;
 };
@@ -247,6 +244,8 @@ __device__ __forceinline__ void feat_setup(const float4 g0, const PyrView &pv, c
         F.i0[k] = (int)fl;
         f[k] = u - fl;
     }
+    F.in = true;
+    for (int k = 0; k < 3; ++k) F.in = F.in && F.i0[k] >= -2 && F.i0[k] <= s2;
     const _Float16 fx = (_Float16)f[0], gx = (_Float16)(1.0f - f[0]);
     const _Float16 fy = (_Float16)f[1], gy = (_Float16)(1.0f - f[1]);
     F.fx = h2{fx, fx};
@@ -263,14 +262,16 @@ __device__ __forceinline__ void feat_slice(const PyrView &pv, const FeatState &F
                                            const int *iu = nullptr) {
     const LevelInfo &L2 = pv.l2;
     const int x0 = UNI ? iu[0] : F.i0[0] - 1, y0 = UNI ? iu[1] : F.i0[1] - 1, z = (UNI ? iu[2] : F.i0[2] - 1) + zs;
-    // the rows' loads carry no branches, so they issue ahead of the selects
+    // the rows' loads carry no branches, so they issue ahead of their use
     RowRaw raw[4];
+    // UNI: every lane has the same block, so the first lane's `in` is everyone's
+    const bool in = UNI ? __builtin_amdgcn_readfirstlane((int)F.in) != 0 : F.in;
 #pragma unroll
-    for (int ys = 0; ys < 4; ++ys) row_fetch<UNI>(pv, L2, x0, y0 + ys, z, raw[ys]);
+    for (int ys = 0; ys < 4; ++ys) row_fetch<UNI>(pv, L2, x0, y0 + ys, z, in, raw[ys]);
 #pragma unroll
     for (int ys = 0; ys < 4; ++ys) {
         h4 t[4];
-        row_select(raw[ys], L2.size, x0, y0 + ys, z, t);
+        row_select(raw[ys], t);
 #pragma unroll
         for (int ox = 0; ox < 3; ++ox) {
             h4 xv;

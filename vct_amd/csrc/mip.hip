@@ -263,8 +263,8 @@ __global__ __launch_bounds__(64) void k_fp8_encode(char *pyr, float *scales, Lev
     float m = fmaxf(fmaxf(__builtin_fabsf(c.x), __builtin_fabsf(c.y)), fmaxf(__builtin_fabsf(c.z), __builtin_fabsf(c.w)));
     for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
     const float sc = m > 0.0f ? m / 448.0f : 1.0f;
-    const uint32_t e = (uint32_t)bf * 64u + (uint32_t)v;  // = elem_index(L, x, y, z, f)
-    if (v == 0) scales[L.scale + (uint32_t)bf] = sc;
+    const uint32_t e = elem_index(L, x, y, z, f);  // brick and face: e / 64 (every stride is a multiple of 64)
+    if (v == 0) scales[L.scale + (e >> 6)] = sc;
     const uint32_t code = f2fp8(c.x / sc) | (f2fp8(c.y / sc) << 8) | (f2fp8(c.z / sc) << 16) | (f2fp8(c.w / sc) << 24);
     *(uint32_t *)(pyr + L.base + (size_t)e * 4) = code;
 }

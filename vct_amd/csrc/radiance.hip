@@ -141,11 +141,11 @@ __global__ __launch_bounds__(256) void k_normalize_inject(uint32_t *albedo, cons
         const int bx = (int)(b % (uint32_t)nb), by = (int)((b / (uint32_t)nb) % (uint32_t)nb),
                   bz = (int)(b / ((uint32_t)nb * nb));
         const int x = bx * 4 + vx, y = by * 4 + vy, z = bz * 4 + vz;
-        const size_t gi = ((size_t)b << 6) + (size_t)lane;  // grid_index(x, y, z) = elem_index(L0, x, y, z)
+        const size_t gi = ((size_t)b << 6) + (size_t)lane;  // grid_index(x, y, z)
         const size_t li = ((size_t)z * dim + y) * dim + x;   // the lit mask is linear
         const uint32_t a32 = albedo[gi];
         const bool lit = litmask[li] != 0;
-        char *elem = pyr + L0.stage + gi * 16;
+        char *elem = pyr + L0.stage + (size_t)elem_index(L0, x, y, z, 0) * 16;
         float4 prev = make_float4(0.f, 0.f, 0.f, 0.f), prev1 = prev;
         if (TEMPORAL) {  // this element as stored last frame (texels x, x+1 for fp16 pairs)
             const uint4 e = *(const uint4 *)elem;
@@ -182,6 +182,8 @@ __global__ __launch_bounds__(256) void k_normalize_inject(uint32_t *albedo, cons
             }
         }
         *(uint4 *)elem = make_uint4(h.x, h.y, h1.x, h1.y);
+        if (x == 0)  // texel 0 is also the second half of element -1, in the border brick
+            *(uint2 *)(pyr + (uint32_t)(L0.stage + elem_index(L0, -1, y, z, 0) * 16u) + 8) = h;
     }
     block_partial(uniq, maxf, lit_count, 0u, partials);
 }
@@ -261,7 +263,7 @@ __global__ __launch_bounds__(256) void k_fill_holes(const char *pyr, char *tmp, 
         }
     }
     LevelInfo T = L0;
-    T.stage = 0;
+    T.stage = L0.pad * 16u;  // tmp holds the stored level, border bricks included
     store_stage_texel(tmp, T, x, y, z, 0, cur);
 }
 
@@ -272,8 +274,9 @@ hipError_t launch_fill_holes(const PyrDesc &pyr, void *tmp, hipStream_t s) {
                        pyr.dim);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    const size_t bytes = (size_t)L0.sz * L0.nb * 16;
-    return hipMemcpyAsync(pyr.pyr + L0.stage, tmp, bytes, hipMemcpyDeviceToDevice, s);  // glCopyImageSubData
+    const size_t bytes = (size_t)L0.nelem * 16;
+    return hipMemcpyAsync(pyr.pyr + L0.stage - (size_t)L0.pad * 16, tmp, bytes, hipMemcpyDeviceToDevice,
+                          s);  // glCopyImageSubData
 }
 
 // ---------------------------------------------------------------- conversions

@@ -19,8 +19,10 @@ namespace vctk {
 // element index is a sum of per-axis terms (vct_internal.h):
 //   X(x) = (x>>2)*sx + (x&3),  Y(y) = (y>>2)*sy + (y&3)*4,
 //   Z(z) = (z>>2)*sz + (z&3)*16, plus 64 per face.
-// Corners outside the level (the border) are clamped to a valid texel and get
-// weight 0, so all fetches are unconditional.  fp16 levels store texel pairs
+// fp16 levels read corners outside the level (the border) from the zero
+// border bricks around every level; the one-texel-per-element formats clamp
+// them to a valid texel with weight 0.  Either way all fetches are
+// unconditional.  fp16 levels store texel pairs
 // (x, x+1) per element: a trilinear sample is four 16-byte loads, one per
 // (y, z) corner pair, instead of eight 8-byte loads -- the vector memory
 // pipeline processes a wavefront's load in the same number of cycles for 8 or
@@ -79,6 +81,21 @@ __device__ __forceinline__ bool zero16(const uint4 v[4]) {
     return o == 0u;
 }
 
+// The fp16-pair sampler's axis terms: the border bricks hold texels -1 and s
+// as zeros, so neither index is clamped nor weight masked.  Indices are
+// shifted by one brick (+4, never negative for the 24-bit multiplies) and
+// offsets are taken from the level's storage start.
+__device__ __forceinline__ void axis_terms_pad(float p, int s, uint32_t mul, uint32_t sub, uint32_t t[2], float w[2]) {
+    const float u = p * (float)s - 0.5f;
+    const float fl = __builtin_floorf(u);
+    const int i0 = (int)fl + 4, i1 = i0 + 1;  // texel -1 .. s, shifted
+    const float f = u - fl;
+    w[0] = 1.0f - f;
+    w[1] = f;
+    t[0] = __umul24((uint32_t)(i0 >> 2), mul) + (uint32_t)(i0 & 3) * sub;
+    t[1] = __umul24((uint32_t)(i1 >> 2), mul) + (uint32_t)(i1 & 3) * sub;
+}
+
 __device__ __forceinline__ float hlo(uint32_t v) { return __half2float(__ushort_as_half((uint16_t)(v & 0xffffu))); }
 __device__ __forceinline__ float hhi(uint32_t v) { return __half2float(__ushort_as_half((uint16_t)(v >> 16))); }
 
@@ -116,11 +133,11 @@ __device__ __forceinline__ bool level_sample(float4 &acc, const char *pyr, __amd
     const bool aniso = L.nfaces == 6;
     uint32_t ty[2], tz[2];
     float wy[2], wz[2];
-    axis_terms(p[1], s, L.sy, 4u, ty, wy);
-    axis_terms(p[2], s, L.sz, 16u, tz, wz);
-    wz[0] *= lw;
-    wz[1] *= lw;
     if (MODE == 1 || (MODE == 2 && L.fmt == 2)) {  // one texel per element: 8 corners
+        axis_terms(p[1], s, L.sy, 4u, ty, wy);
+        axis_terms(p[2], s, L.sz, 16u, tz, wz);
+        wz[0] *= lw;
+        wz[1] *= lw;
         uint32_t tx[2];
         float wx[2];
         axis_terms(p[0], s, L.sx, 1u, tx, wx);
@@ -156,21 +173,25 @@ __device__ __forceinline__ bool level_sample(float4 &acc, const char *pyr, __amd
         }
         return false;
     }
-    // fp16 pairs
+    // fp16 pairs: element x holds texels x and x+1, so one load per (y, z)
+    // corner pair; element -1 (border brick) holds (0, texel 0)
+    axis_terms_pad(p[1], s, L.sy, 4u, ty, wy);
+    axis_terms_pad(p[2], s, L.sz, 16u, tz, wz);
+    wz[0] *= lw;
+    wz[1] *= lw;
     const float u = p[0] * (float)s - 0.5f;
     const float fl = __builtin_floorf(u);
-    const int i0 = (int)fl;
+    const int cx = (int)fl + 4;  // pair element of texel -1 .. s-1, shifted one brick
     const float f = u - fl;
-    const bool xn = i0 < 0;  // texel -1 is the border: read pair (0, 1), weight the first by f
-    const int cx = xn ? 0 : i0;
-    const float wlo = xn ? f : 1.0f - f, whi = xn ? 0.0f : f;
+    const float wlo = 1.0f - f, whi = f;
     const uint32_t ex = __umul24((uint32_t)(cx >> 2), L.sx) + (uint32_t)(cx & 3);
+    const uint32_t org = L.base - L.pad * 16u;  // the level's storage start
     uint32_t off[4];
     float wa[4], wb[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
         const int dy = c & 1, dz = c >> 1;
-        off[c] = L.base + (ex + ty[dy] + tz[dz]) * 16u;
+        off[c] = org + (ex + ty[dy] + tz[dz]) * 16u;
         const float wyz = wy[dy] * wz[dz];
         wa[c] = wlo * wyz;
         wb[c] = whi * wyz;

@@ -175,7 +175,8 @@ __device__ __forceinline__ float4 load_stage_texel(const char *pyr, const LevelI
 }
 // Writes texel (x,y,z) of face f into a level's stage: the fp32 element, or
 // both fp16 halves that hold it (the first half of its own element and the
-// second half of the element of x - 1).
+// second half of the element of x - 1, in the border brick for x = 0; its
+// element index wraps below 0, so the byte offset is formed in 32 bits).
 __device__ __forceinline__ void store_stage_texel(char *pyr, const LevelInfo &L, int x, int y, int z, int f,
                                                   float4 c) {
     char *p = pyr + L.stage + (size_t)elem_index(L, x, y, z, f) * 16;
@@ -185,7 +186,7 @@ __device__ __forceinline__ void store_stage_texel(char *pyr, const LevelInfo &L,
     }
     const uint2 h = pack_h4(c);
     *(uint2 *)p = h;
-    if (x > 0) *(uint2 *)(pyr + L.stage + (size_t)elem_index(L, x - 1, y, z, f) * 16 + 8) = h;
+    *(uint2 *)(pyr + (uint32_t)(L.stage + elem_index(L, x - 1, y, z, f) * 16u) + 8) = h;
 }
 // Texel (x,y,z) of face f in a level's storage, decoded to float.
 __device__ __forceinline__ float4 load_texel(const char *pyr, const float *scales, const LevelInfo &L, int x, int y,

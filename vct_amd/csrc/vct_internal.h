@@ -39,24 +39,32 @@ struct DevLight {
 // (so a sampler needs a single buffer descriptor for every level and face).
 // Level l has edge `size`, nb = ceil(size/4) bricks per axis and `nfaces`
 // volumes (1, or 6 for anisotropic levels >= 1: +x,-x,+y,-y,+z,-z).  A brick
-// holds 4x4x4 elements x-fastest and the faces of one brick are adjacent, so
-// the element of texel (x, y, z) in face f is
+// holds 4x4x4 elements x-fastest and the faces of one brick are adjacent.
+// Every level carries a border one brick thick on each side (nb + 2 bricks
+// per axis) that holds zeros, so that the fp16-pair sampler can fetch texel
+// -1 and texel size of any axis without clamping or masking.  With `base` at
+// the first interior brick, the element of texel (x, y, z) in face f is
 //   (x>>2)*sx + (x&3) + (y>>2)*sy + (y&3)*4 + (z>>2)*sz + (z&3)*16 + 64*f
-// with sx = 64*nfaces, sy = nb*sx, sz = nb*sy (vct_device.h elem_index).
+// with sx = 64*nfaces, sy = (nb+2)*sx, sz = (nb+2)*sy (vct_device.h
+// elem_index); the level's storage starts `pad` = sx + sy + sz elements
+// before `base`.
 // Element formats (fmt):
 //   0  fp16 pair, 16 B: texel (x,y,z) and texel (x+1,y,z) as fp16x4 each, so
-//      one aligned 16-byte load feeds both x-taps of a trilinear sample; the
-//      second half of the last texel in x is the border (0) and is never
-//      written.
+//      one aligned 16-byte load feeds both x-taps of a trilinear sample.  The
+//      writers of texel 0 also write the second half of element -1 (in the
+//      border); the second half of element size-1 is texel size, the border
+//      (0), and is never written.
 //   1  fp32x4, 16 B, one texel.
 //   2  fp8 E4M3 x4, 4 B, one texel, times the fp32 scale of its (brick, face):
 //      scales[scale + element / 64].
 // `stage` is where the mip chain reads and writes the level: `base` for fmt 0
 // and 1, an fp16-pair copy for fmt 2 (filtered in fp16, encoded after).
 struct LevelInfo {
-    uint32_t base, stage;  // byte offsets into the pyramid allocation
+    uint32_t base, stage;  // byte offsets into the pyramid allocation (first interior brick)
     uint32_t scale;        // float offset into the scale array (fmt 2)
     uint32_t sx, sy, sz;   // brick strides in elements along x, y, z
+    uint32_t pad;          // sx + sy + sz: elements from the storage start to `base`
+    uint32_t nelem;        // elements of the stored level, border included
     int size, nb, fmt, nfaces;
 };
 
