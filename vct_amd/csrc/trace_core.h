@@ -452,8 +452,18 @@ __device__ __forceinline__ void trace_pixel(const TraceLaunch &t, const PyrArgs 
         float R[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) R[k] = I[k] - (2.0f * dni) * N[k];
-        const float4 r = trace_cone<MODE, ANISO, false>(pa, rs, pos, N, R, t.s_steps, t.s_bias, tanf(angle / 2.0f),
-                                                        t.s_h0, t.s_lod_offset, t.warp);
+        const float th = tanf(angle / 2.0f);
+        // One material under the whole wavefront (the usual case): every lane's
+        // cone has the same aperture, so the step schedule and the level
+        // classes are the same in every lane.
+        const float th0 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(th)));
+        float4 r;
+        if (__ballot(__float_as_int(th) != __float_as_int(th0)) == 0)
+            r = trace_cone<MODE, ANISO, true>(pa, rs, pos, N, R, t.s_steps, t.s_bias, th, t.s_h0, t.s_lod_offset,
+                                              t.warp);
+        else
+            r = trace_cone<MODE, ANISO, false>(pa, rs, pos, N, R, t.s_steps, t.s_bias, th, t.s_h0, t.s_lod_offset,
+                                               t.warp);
         t.out_spec[pi] = r;
     }
 }
