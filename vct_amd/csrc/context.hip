@@ -531,7 +531,7 @@ int vct_set_transforms(vct_ctx *c, const float *model, int n) {
 
 int vct_set_lights(vct_ctx *c, const vct_light *lights, int n) {
     if (!c || (n > 0 && !lights)) return fail(c, VCT_ERR_INVALID, "vct_set_lights: NULL argument");
-    if (n < 0 || n > 32) return fail(c, VCT_ERR_INVALID, "vct_set_lights: %d lights (max 32)", n);
+    if (n < 0 || n > kMaxLights) return fail(c, VCT_ERR_INVALID, "vct_set_lights: %d lights (max %d)", n, kMaxLights);
     for (int i = 0; i < n; ++i) {
         if (lights[i].type > 1u) return fail(c, VCT_ERR_INVALID, "light %d: unknown type %u", i, lights[i].type);
         if (lights[i].shadow_caster && (lights[i].shadow_size < 16 || lights[i].shadow_size > 8192))

@@ -217,8 +217,9 @@ __global__ __launch_bounds__(256) void k_raster_items(const int4 *__restrict__ r
 }
 
 // The same walk for passes whose per-fragment work is heavy (voxelize with
-// lighting): F provides cover(slot, px, py, edges, l[3]) -> bool, shade(slot,
-// l) and finish().  Covered fragments are appended to a per-wavefront queue in
+// lighting): F provides begin() (called by every thread before the walk, for
+// workgroup-wide set-up; it ends with a barrier if it writes LDS),
+// cover(slot, px, py, edges, l[3]) -> bool, shade(slot, l) and finish().  Covered fragments are appended to a per-wavefront queue in
 // LDS and shaded 64 at a time, so the shading runs with every lane busy
 // instead of with the lanes of uncovered bbox pixels idle.  The queue holds up
 // to 127 entries (at most 63 left over plus one chunk of 64).
@@ -229,6 +230,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
                                                        int nslots, F f) {
     __shared__ int q_slot[4][128];
     __shared__ float q_l[4][3][128];
+    f.begin();
     const uint64_t total = offs[nslots];
     const uint64_t nchunks = (total + 63) >> 6;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;

@@ -18,6 +18,8 @@ constexpr int kWarpDim = 32;
 namespace vctk {
 
 // Light as the kernels see it: the ABI record plus its shadow map pointer.
+constexpr int kMaxLights = 32;  // vct_set_lights
+
 struct DevLight {
     float position[3];
     float range;

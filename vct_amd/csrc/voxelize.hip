@@ -128,6 +128,9 @@ __device__ __forceinline__ void vox_unit(const float pos[3], int axis, float un[
     un[2] = 1.0f - un[2];
 }
 
+// The scene's lights in LDS, for the lit voxelizer (VoxFrag::begin).
+__shared__ DevLight vox_lights[kMaxLights];
+
 struct VoxFrag {
     const float4 *attr;
     const DevLight *lights;
@@ -137,6 +140,15 @@ struct VoxFrag {
     const float4 *warp;  // warpTexture: the warp texture, or null
     uint32_t frags;
 
+    // The lights are read field by field inside the shading loop, behind
+    // branches on their type and flags: from LDS (vox_lights) each such read
+    // is a short round trip instead of one to the cache hierarchy.
+    __device__ void begin() {
+        const int nw = (lighting ? nlights : 0) * (int)(sizeof(DevLight) / 4);
+        for (int i = threadIdx.x; i < nw; i += blockDim.x)
+            ((uint32_t *)vox_lights)[i] = ((const uint32_t *)lights)[i];
+        __syncthreads();
+    }
     __device__ bool cover(int slot, int px, int py, const Edges &e, float l[3]) const {
         return vox_cover(attr, msaa, slot, px, py, e, l);
     }
@@ -171,7 +183,7 @@ struct VoxFrag {
         if (lighting) {
             float fin[3] = {0.0f, 0.0f, 0.0f};
             for (int i = 0; i < nlights; ++i) {
-                const DevLight &L = lights[i];
+                const DevLight &L = vox_lights[i];
                 if (!L.enabled) continue;
                 float lit[3];
                 if (L.type == 0u) {
@@ -222,6 +234,7 @@ struct OccFrag {
     int msaa;
     uint32_t *occ;
 
+    __device__ void begin() {}
     __device__ bool cover(int slot, int px, int py, const Edges &e, float l[3]) const {
         return vox_cover(attr, msaa, slot, px, py, e, l);
     }
