@@ -122,3 +122,36 @@ def test_compose_matches_oracle(scene1, variant):
     assert_compose_close(got, ref, f"compose {variant}")
     cov = cap["gbuf"][..., 11].view(np.uint32).reshape(-1) != 0
     assert ref[cov, :3].sum() > 0 and np.all(ref[~cov] == 0)
+
+
+def test_normalize_inject_alone_bit_exact():
+    """A one-level grid (no mip chain at all): the injection alone matches the
+    oracle, statistics included."""
+    app = make_app(1, levels=1)
+    try:
+        ctx = app.ctx
+        g = ctx.grid()
+        assert g.levels == 1
+        fs = app.frame_state(0)
+        ctx.set_transforms(fs["model"])
+        sms = []
+        for i, L in enumerate(fs["lights"]):
+            if L.shadow_caster:
+                ctx.shadowmap(i)
+                sms.append(ctx.shadow(i))
+            else:
+                sms.append(None)
+        ctx.voxelize(fs["vox"])
+        alb_vox = ctx.albedo()
+        ctx.normalize_inject(fs["inj"])
+        alb, rad0, lm, st = ob.normalize_inject(alb_vox, g.dim, tuple(g.center), tuple(g.vmin), tuple(g.vmax),
+                                               fs["lights"], sms)
+        assert np.array_equal(ctx.litmask(), lm)
+        assert np.array_equal(ctx.albedo(), alb)
+        assert np.array_equal(ctx.radiance(0).view(np.uint32), rad0.view(np.uint32))
+        s = ctx.stats()
+        assert (s.unique_voxels, s.max_fragments_per_voxel, s.lit_voxels) == \
+            (st.unique_voxels, st.max_fragments_per_voxel, st.lit_voxels)
+        assert st.lit_voxels > 0
+    finally:
+        app.close()
