@@ -62,7 +62,8 @@ extern "C" {
                                  occlusion) -- phong.frag:489 debugIndirect view  */
 #define VCT_BUF_SPECULAR 6   /* float4 [width*height]: specular cone rgba          */
 #define VCT_BUF_LITMASK 7    /* u8 [dim^3]: 1 where a voxel is lit by any light   */
-#define VCT_BUF_RADIANCE_RAW 8 /* raw stored bytes of mip `level` (layout-native) */
+#define VCT_BUF_RADIANCE_RAW 8 /* raw stored bytes of mip `level` (layout-native:   */
+                               /* 4^3 bricks with a zero border brick each side)    */
 #define VCT_BUF_LABELS 10    /* float4 [npoints]: 64-cone reference (rgb*occ, occ)  */
 #define VCT_BUF_FEATURES 11  /* u16 (bf16) [npoints][160]: learned-gather inputs   */
 #define VCT_BUF_REFERENCE 12 /* float4 [width*height]: 64-cone reference gather    */
