@@ -224,7 +224,7 @@ __global__ __launch_bounds__(256) void k_raster_items(const int4 *__restrict__ r
 // instead of with the lanes of uncovered bbox pixels idle.  The queue holds up
 // to 127 entries (at most 63 left over plus one chunk of 64).
 template <class F>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_raster_queued(const int4 *__restrict__ recs,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_raster_queued(const int4 *__restrict__ recs,
                                                        const uint64_t *__restrict__ offs,
                                                        const uint32_t *__restrict__ chunk, size_t chunk_cap,
                                                        int nslots, F f) {

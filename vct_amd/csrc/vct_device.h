@@ -268,45 +268,71 @@ __device__ __forceinline__ float shadow_bilinear(const float *__restrict__ sm, i
     return ((w00 * t00 + w10 * t10) + w01 * t01) + w11 * t11;
 }
 
-// shadow_bilinear on the footprint layout of DevLight::shadowquad: the same
-// coordinates, weights and sum order, the four texels from one aligned load.
-__device__ __forceinline__ float shadow_bilinear_quad(const float4 *__restrict__ q, int N, float s, float t, int ox,
-                                                      int oy) {
+// calcShadowFactor (shaders/voxelize.frag:159-180, phong.frag:183-207): the
+// fraction of the five PCF taps (offsets (0,0), (1,0), (0,1), (-1,0), (0,-1))
+// that occlude a light-space position; 0 beyond the far plane.  In two halves
+// so that a caller can have the taps of several lights in flight at once:
+// shadow_taps fetches the five footprints, shadow_count compares them (the
+// same operations, in the same order, as one shadow_bilinear_quad per tap).
+struct ShadowTaps {
+    float4 q[5];  // the footprints of the five taps (1.0 outside the map)
+    float s, t, frag;
+    bool far;  // beyond the far plane: factor 0, nothing fetched
+};
+__device__ __forceinline__ void shadow_tap_index(int N, float s, float t, int ox, int oy, int *i0, int *j0) {
     float u = s * (float)N;
     u = u + (float)ox;
     u = u - 0.5f;
     float v = t * (float)N;
     v = v + (float)oy;
     v = v - 0.5f;
-    const float fu = __builtin_floorf(u), fv = __builtin_floorf(v);
-    const int i0 = (int)fu, j0 = (int)fv;
-    const float a = u - fu, b = v - fv;
-    float4 T = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
-    if (i0 >= -1 && i0 <= N - 1 && j0 >= -1 && j0 <= N - 1) T = q[(size_t)(j0 + 1) * (N + 1) + (i0 + 1)];
-    const float w00 = (1.0f - a) * (1.0f - b), w10 = a * (1.0f - b), w01 = (1.0f - a) * b, w11 = a * b;
-    return ((w00 * T.x + w10 * T.y) + w01 * T.z) + w11 * T.w;
+    *i0 = (int)__builtin_floorf(u);
+    *j0 = (int)__builtin_floorf(v);
 }
-
-// calcShadowFactor (shaders/voxelize.frag:159-180, phong.frag:183-207): the
-// fraction of the five PCF taps (offsets (0,0), (1,0), (0,1), (-1,0), (0,-1))
-// that occlude a light-space position; 0 beyond the far plane.
-__device__ __forceinline__ float calc_shadow_factor(const float lsp[4], const float4 *sq, int N) {
+__device__ __forceinline__ void shadow_taps(const float lsp[4], const float4 *sq, int N, ShadowTaps &st) {
     float sh[3];
     if (__ballot(lsp[3] != 1.0f) == 0) {  // orthographic light space: w = 1, and x / 1 is x
         for (int k = 0; k < 3; ++k) sh[k] = (lsp[k] + 1.0f) * 0.5f;
     } else {
         for (int k = 0; k < 3; ++k) sh[k] = (lsp[k] / lsp[3] + 1.0f) * 0.5f;
     }
-    const float frag = sh[2] - 0.01f;
-    if (frag > 1.0f) return 0.0f;
+    st.s = sh[0];
+    st.t = sh[1];
+    st.frag = sh[2] - 0.01f;
+    st.far = st.frag > 1.0f;
+    const int ox[5] = {0, 1, 0, -1, 0}, oy[5] = {0, 0, 1, 0, -1};
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        int i0, j0;
+        shadow_tap_index(N, st.s, st.t, ox[k], oy[k], &i0, &j0);
+        st.q[k] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+        if (!st.far && i0 >= -1 && i0 <= N - 1 && j0 >= -1 && j0 <= N - 1)
+            st.q[k] = sq[(size_t)(j0 + 1) * (N + 1) + (i0 + 1)];
+    }
+}
+__device__ __forceinline__ float shadow_count(const ShadowTaps &st, int N) {
+    if (st.far) return 0.0f;
+    const int ox[5] = {0, 1, 0, -1, 0}, oy[5] = {0, 0, 1, 0, -1};
     float f = 0.0f;
-    if (frag > shadow_bilinear_quad(sq, N, sh[0], sh[1], 0, 0)) f += 1.0f;
-    if (frag > shadow_bilinear_quad(sq, N, sh[0], sh[1], 1, 0)) f += 1.0f;
-    if (frag > shadow_bilinear_quad(sq, N, sh[0], sh[1], 0, 1)) f += 1.0f;
-    if (frag > shadow_bilinear_quad(sq, N, sh[0], sh[1], -1, 0)) f += 1.0f;
-    if (frag > shadow_bilinear_quad(sq, N, sh[0], sh[1], 0, -1)) f += 1.0f;
-    f = f * 0.2f;  // the same float as f / 5 for each of f = 0, 1, ..., 5
-    return f;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        float u = st.s * (float)N;
+        u = u + (float)ox[k];
+        u = u - 0.5f;
+        float v = st.t * (float)N;
+        v = v + (float)oy[k];
+        v = v - 0.5f;
+        const float a = u - __builtin_floorf(u), b = v - __builtin_floorf(v);
+        const float4 T = st.q[k];
+        const float w00 = (1.0f - a) * (1.0f - b), w10 = a * (1.0f - b), w01 = (1.0f - a) * b, w11 = a * b;
+        if (st.frag > ((w00 * T.x + w10 * T.y) + w01 * T.z) + w11 * T.w) f += 1.0f;
+    }
+    return f * 0.2f;  // the same float as f / 5 for each of f = 0, 1, ..., 5
+}
+__device__ __forceinline__ float calc_shadow_factor(const float lsp[4], const float4 *sq, int N) {
+    ShadowTaps st;
+    shadow_taps(lsp, sq, N, st);
+    return shadow_count(st, N);
 }
 
 // ---- wave helpers ----------------------------------------------------------
