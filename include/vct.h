@@ -269,6 +269,11 @@ typedef struct vct_warpmap_params {
 /* ---- passes (order of Application::render) -------------------------------- */
 /* Shadow map of one light: simple.vert + depth only (src/Application.cpp:212-233). */
 int vct_shadowmap(vct_ctx *ctx, int light_index);
+/* Shadow maps of every shadow-casting light.  The reference draws one map,
+ * the main light's (src/Application.cpp:212-233); the multi-light maps repeat
+ * that pass per caster, here up to 8 lights per rasterizer walk.  Each map is
+ * bit-identical to what vct_shadowmap gives that light. */
+int vct_shadowmaps(vct_ctx *ctx);
 /* Voxelize, GS path: voxelize.vert/geom/frag drawn by scene->draw
  * (src/Application.cpp:666-754). Clears albedo/normal first (:686-687). */
 int vct_voxelize(vct_ctx *ctx, const vct_voxelize_params *p);

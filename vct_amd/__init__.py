@@ -126,6 +126,9 @@ class Context:
     def shadowmap(self, i):
         self.check(self.lib.vct_shadowmap(self.h, i))
 
+    def shadowmaps(self):
+        self.check(self.lib.vct_shadowmaps(self.h))
+
     def voxelize(self, p: abi.VoxelizeParams):
         self.check(self.lib.vct_voxelize(self.h, C.byref(p)))
 

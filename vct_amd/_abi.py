@@ -152,6 +152,7 @@ VCT_FUNCS = [
     ("vct_set_transforms", C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     ("vct_set_lights", C.c_int, [C.c_void_p, C.POINTER(Light), C.c_int]),
     ("vct_shadowmap", C.c_int, [C.c_void_p, C.c_int]),
+    ("vct_shadowmaps", C.c_int, [C.c_void_p]),
     ("vct_voxelize", C.c_int, [C.c_void_p, C.POINTER(VoxelizeParams)]),
     ("vct_normalize_inject", C.c_int, [C.c_void_p, C.POINTER(InjectParams)]),
     ("vct_fill_holes", C.c_int, [C.c_void_p]),

@@ -430,7 +430,7 @@ static int ensure_raster(vct_ctx *c, size_t slots) {
     c->rs.scan_tmp_bytes = raster_scan_tmp_bytes(slots);
     HIPCHK(c, hipMalloc(&c->rs.scan_tmp, std::max<size_t>(c->rs.scan_tmp_bytes, 256)));
     if (!c->rs.chunk) {
-        c->rs.chunk_cap = (size_t)1 << 20;  // chunk starts for 64M items; beyond that a full search is used
+        c->rs.chunk_cap = (size_t)1 << 22;  // chunk starts for 256M items; beyond that a full search is used
         HIPCHK(c, dalloc(&c->rs.chunk, c->rs.chunk_cap * sizeof(uint32_t)));
     }
     c->rs.cap_slots = slots;
@@ -538,7 +538,7 @@ int vct_set_lights(vct_ctx *c, const vct_light *lights, int n) {
             return fail(c, VCT_ERR_INVALID, "light %d: shadow_size %u out of [16, 8192]", i, lights[i].shadow_size);
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    // keep shadow maps whose size is unchanged (rebuilt per frame by vct_shadowmap)
+    // keep shadow maps whose size is unchanged (rebuilt per frame by vct_shadowmap(s))
     std::vector<float *> maps(n, nullptr);
     std::vector<float4 *> quads(n, nullptr);
     for (int i = 0; i < n; ++i) {
@@ -619,7 +619,23 @@ int vct_shadowmap(vct_ctx *c, int li) {
     if (!c->lights[li].shadow_caster) return fail(c, VCT_ERR_INVALID, "light %d casts no shadow", li);
     if (!c->d_model) return fail(c, VCT_ERR_STATE, "vct_shadowmap: no scene");
     PassTimer pt(c, P_SHADOW);
-    LAUNCHCHK(c, launch_shadowmap(dev_scene(c), &c->hlights[li], c->d_shadow[li], c->rs, c->stream));
+    LAUNCHCHK(c, launch_shadowmaps(dev_scene(c), c->hlights.data(), c->d_lights, &li, 1, c->rs, c->stream));
+    return VCT_OK;
+}
+
+int vct_shadowmaps(vct_ctx *c) {
+    if (!c) return fail(nullptr, VCT_ERR_INVALID, "ctx is NULL");
+    if (!c->d_model) return fail(c, VCT_ERR_STATE, "vct_shadowmaps: no scene");
+    std::vector<int> li;
+    for (int i = 0; i < (int)c->lights.size(); ++i)
+        if (c->lights[i].shadow_caster) li.push_back(i);
+    if (li.empty()) return VCT_OK;
+    const size_t walk = std::min<size_t>(li.size(), (size_t)shadow_batch(c->ntri));
+    int rc = ensure_raster(c, std::max<size_t>(walk * (size_t)c->ntri, 64));
+    if (rc) return rc;
+    PassTimer pt(c, P_SHADOW);
+    LAUNCHCHK(c, launch_shadowmaps(dev_scene(c), c->hlights.data(), c->d_lights, li.data(), (int)li.size(), c->rs,
+                                   c->stream));
     return VCT_OK;
 }
 

@@ -26,21 +26,30 @@ __device__ __forceinline__ void world_vertex(const DevScene &sc, int t, int v, f
 }
 
 // ---------------------------------------------------------------- shadow map
-struct LightMat {
-    float ls[16];
-};
+// One walk draws up to kShadowBatch lights' maps: slot j * ntri + t is
+// triangle t seen from the walk's j-th light, whose index in the light array
+// sits at bits 5j .. 5j+4 of `li` (light indices are below kMaxLights = 32).
+// Each map receives exactly the fragments a walk of its light alone gives it
+// (depth is an order-independent atomicMin), so the batching does not change
+// a texel.
+static_assert(kMaxLights <= 32, "light indices are packed in 5 bits");
+static_assert(kShadowBatch * 5 <= 64, "a walk's light indices fit one 64-bit argument");
 
-__global__ __launch_bounds__(256) void k_shadow_setup(DevScene sc, LightMat L, int N, int4 *rec, float4 *attr,
-                                                      uint64_t *counts) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_shadow_setup(DevScene sc, const DevLight *__restrict__ lights, uint64_t li,
+                                                      int4 *rec, float4 *attr, uint64_t *counts) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= sc.ntri) return;
+    const int l = (int)((li >> (5u * blockIdx.y)) & 31u);
+    const float *ls = lights[l].ls;
+    const int N = (int)lights[l].shadow_size;
+    const size_t slot = (size_t)blockIdx.y * (size_t)sc.ntri + (size_t)t;
     float clip[3][4];
     bool behind = false;
     for (int v = 0; v < 3; ++v) {
         float w[3];
         world_vertex(sc, t, v, w);
         float w4[4] = {w[0], w[1], w[2], 1.0f};
-        mat4_vec4(L.ls, w4, clip[v]);
+        mat4_vec4(ls, w4, clip[v]);
         if (clip[v][2] < -clip[v][3] || clip[v][3] <= 0.0f) behind = true;
     }
     uint64_t cnt = 0;
@@ -56,20 +65,19 @@ __global__ __launch_bounds__(256) void k_shadow_setup(DevScene sc, LightMat L, i
         int4 r[4];
         // back faces culled: GL_CULL_FACE stays enabled from the previous
         // frame's passes when the reference draws its shadow map
-        // (Application.cpp:737,942,979)
-        cnt = tri_setup(win, N, N, 1, t, r);
+        // (Application.cpp:737,942,979); the record's user field is the light
+        cnt = tri_setup(win, N, N, 1, l, r);
         if (cnt) {
-            for (int i = 0; i < 4; ++i) rec[(size_t)t * 4 + i] = r[i];
-            attr[t] = make_float4(zw[0], zw[1], zw[2], 0.0f);
+            for (int i = 0; i < 4; ++i) rec[slot * 4 + i] = r[i];
+            attr[slot] = make_float4(zw[0], zw[1], zw[2], 0.0f);
         }
     }
-    counts[t] = cnt;
+    counts[slot] = cnt;
 }
 
 struct ShadowFrag {
     const float4 *attr;
-    uint32_t *depth;
-    int N;
+    const DevLight *lights;
     __device__ void item(int slot, int px, int py, const Edges &e) {
         if (!edge_inside(e, px, py, 128, 128)) return;
         float l[3];
@@ -78,28 +86,47 @@ struct ShadowFrag {
         float z = interp3(l, a.x, a.y, a.z);
         z = z + 0.0f;
         if (!(z >= 0.0f && z <= 1.0f)) return;
-        atomicMin(&depth[(size_t)py * N + px], __float_as_uint(z));
+        const DevLight &L = lights[e.user];
+        uint32_t *depth = (uint32_t *)const_cast<float *>(L.shadowmap);
+        atomicMin(&depth[(size_t)py * L.shadow_size + px], __float_as_uint(z));
     }
     __device__ void finish() {}
 };
 
-hipError_t launch_shadowmap(const DevScene &sc, const DevLight *L, float *depth, RasterScratch &rs,
-                            hipStream_t s) {
-    const int N = (int)L->shadow_size;
-    hipError_t e = launch_fill_u32((uint32_t *)depth, 0x3f800000u, (size_t)N * N, s);  // clear depth 1.0
-    if (e != hipSuccess) return e;
-    if (sc.ntri == 0) return hipSuccess;
-    LightMat lm;
-    for (int i = 0; i < 16; ++i) lm.ls[i] = L->ls[i];
-    hipLaunchKernelGGL(k_shadow_setup, dim3((sc.ntri + 255) / 256), dim3(256), 0, s, sc, lm, N, rs.rec, rs.attr,
-                       rs.counts);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    if ((e = raster_scan(rs, (size_t)sc.ntri, s)) != hipSuccess) return e;
-    ShadowFrag f{rs.attr, (uint32_t *)depth, N};
-    hipLaunchKernelGGL(k_raster_items<ShadowFrag>, dim3(raster_grid<ShadowFrag, false>()), dim3(256), 0, s, rs.rec, rs.offs, rs.chunk,
-                       rs.chunk_cap, sc.ntri, f);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    return launch_shadow_quads(depth, N, const_cast<float4 *>(L->shadowquad), s);
+hipError_t launch_shadowmaps(const DevScene &sc, const DevLight *hlights, const DevLight *dlights, const int *li,
+                             int n, RasterScratch &rs, hipStream_t s) {
+    hipError_t e;
+    const int batch = shadow_batch(sc.ntri);
+    for (int j0 = 0; j0 < n; j0 += batch) {
+        const int nb = n - j0 < batch ? n - j0 : batch;
+        uint64_t packed = 0;
+        for (int j = 0; j < nb; ++j) {
+            const DevLight &L = hlights[li[j0 + j]];
+            const size_t N = L.shadow_size;  // clear depth 1.0
+            if ((e = launch_fill_u32((uint32_t *)const_cast<float *>(L.shadowmap), 0x3f800000u, N * N, s)) !=
+                hipSuccess)
+                return e;
+            packed |= (uint64_t)(uint32_t)li[j0 + j] << (5 * j);
+        }
+        if (sc.ntri > 0) {
+            const size_t nslots = (size_t)nb * (size_t)sc.ntri;
+            hipLaunchKernelGGL(k_shadow_setup, dim3((sc.ntri + 255) / 256, nb), dim3(256), 0, s, sc, dlights, packed,
+                               rs.rec, rs.attr, rs.counts);
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+            if ((e = raster_scan(rs, nslots, s)) != hipSuccess) return e;
+            ShadowFrag f{rs.attr, dlights};
+            hipLaunchKernelGGL(k_raster_items<ShadowFrag>, dim3(raster_grid<ShadowFrag, false>()), dim3(256), 0, s,
+                               rs.rec, rs.offs, rs.chunk, rs.chunk_cap, (int)nslots, f);
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+        }
+        for (int j = 0; j < nb; ++j) {
+            const DevLight &L = hlights[li[j0 + j]];
+            if ((e = launch_shadow_quads(L.shadowmap, (int)L.shadow_size, const_cast<float4 *>(L.shadowquad), s)) !=
+                hipSuccess)
+                return e;
+        }
+    }
+    return hipSuccess;
 }
 
 __global__ __launch_bounds__(256) void k_shadow_quads(const float *__restrict__ sm, int N, float4 *q) {

@@ -149,8 +149,17 @@ hipError_t raster_scan(RasterScratch &rs, size_t nslots, hipStream_t s);
 size_t raster_scan_tmp_bytes(size_t nslots);
 
 // raster.hip
-hipError_t launch_shadowmap(const DevScene &sc, const DevLight *dlight_host, float *depth, RasterScratch &rs,
-                            hipStream_t s);
+// Shadow maps of the lights li[0 .. n): hlights / dlights are the same light
+// array on the host / device (map pointers and sizes from there).  The lights
+// are drawn shadow_batch(ntri) per walk, so the raster scratch must hold
+// min(n, shadow_batch(ntri)) * ntri slots.
+constexpr int kShadowBatch = 8;
+inline int shadow_batch(int ntri) {  // slot indices stay below 2^31
+    const long m = ntri > 0 ? 0x7ffffffeL / ntri : kShadowBatch;
+    return m < kShadowBatch ? (int)m : kShadowBatch;
+}
+hipError_t launch_shadowmaps(const DevScene &sc, const DevLight *hlights, const DevLight *dlights, const int *li,
+                             int n, RasterScratch &rs, hipStream_t s);
 // rebuilds DevLight::shadowquad from the depth map
 hipError_t launch_shadow_quads(const float *depth, int N, float4 *quads, hipStream_t s);
 hipError_t launch_gbuffer(const DevScene &sc, const vct_camera &cam, uint64_t *vis, vct_gbuffer_texel *gbuf,

@@ -810,8 +810,7 @@ int vct_app_build_radiance(vct_app *a, int frame) {
     vct_app_frame_state(a, frame, a->model.data(), nullptr, &vox, &inj, &mip, nullptr, nullptr);
     int rc;
     if ((rc = vct_set_transforms(a->ctx, a->model.data(), a->mesh.nobj))) return rc;
-    for (size_t i = 0; i < a->lights.size(); ++i)
-        if (a->lights[i].shadow_caster && (rc = vct_shadowmap(a->ctx, (int)i))) return rc;
+    if ((rc = vct_shadowmaps(a->ctx))) return rc;  // every caster's map (Application.cpp:212-233 per light)
     if (a->cfg.warp_texture) {  // the occupancy grid and warp texture of this frame (Application.cpp:235-575)
         vct_warpmap_params wp;
         vct_app_warpmap_params(a, &wp);
