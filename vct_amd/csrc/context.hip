@@ -50,6 +50,7 @@ struct vct_ctx {
     uint8_t *d_litmask = nullptr;
     uint32_t *d_partials = nullptr;  // per-workgroup statistics partials
     char *d_pyr = nullptr;
+    uint8_t *d_zero0 = nullptr;  // PyrDesc::zero0
     float *d_scales = nullptr;
     size_t pyr_bytes = 0, scale_floats = 0;
     PyrDesc pyr{};
@@ -173,6 +174,7 @@ static void free_grids(vct_ctx *c) {
     dfree(c->d_grid_normal);
     dfree(c->d_litmask);
     dfree(c->d_pyr);
+    dfree(c->d_zero0);
     dfree(c->d_scales);
     dfree(c->d_lv);
     dfree(c->d_fill_tmp);
@@ -242,7 +244,14 @@ static int alloc_grids(vct_ctx *c) {
         HIPCHK(c, dalloc(&c->d_scales, soff * 4));
         HIPCHK(c, hipMemsetAsync(c->d_scales, 0, soff * 4, c->stream));
     }
+    {
+        const size_t nb0 = (size_t)P.lv[0].nb;
+        const size_t zbytes = (nb0 * nb0 * nb0 + 3) / 4 * 4;
+        HIPCHK(c, dalloc(&c->d_zero0, zbytes));
+        HIPCHK(c, hipMemsetAsync(c->d_zero0, 0, zbytes, c->stream));  // unknown until the first injection
+    }
     P.pyr = c->d_pyr;
+    P.zero0 = c->d_zero0;
     P.bytes = (uint32_t)off;
     P.scales = c->d_scales;
     HIPCHK(c, dalloc(&c->d_lv, sizeof(LevelInfo) * P.nlevels));

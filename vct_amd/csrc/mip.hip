@@ -26,7 +26,17 @@ struct MipArgs {
     LevelInfo out[3];
     int nout;
     int fp32;  // stage format: 1 = fp32x4, 0 = fp16 pairs
+    // source level 0 only: its bricks known to hold +0 (PyrDesc::zero0), or null
+    const uint8_t *zero0;
 };
+
+// Whether brick (x, y, z) >> 2 of the source is known to hold +0: its texels
+// are then not read.
+__device__ __forceinline__ bool src_zero(const MipArgs &a, int x, int y, int z) {
+    if (a.zero0 == nullptr) return false;
+    const uint32_t nb = (uint32_t)a.src.nb;
+    return a.zero0[((uint32_t)(z >> 2) * nb + (uint32_t)(y >> 2)) * nb + (uint32_t)(x >> 2)] != 0;
+}
 
 __device__ __forceinline__ float4 round_stage(float4 v, int fp32) {
     if (fp32) return v;
@@ -195,7 +205,8 @@ __global__ __launch_bounds__(256) void k_mip_chain(MipArgs a) {
             const int x = ox + (j & 1) * 4 + (v & 1) * 2;
             const int y = oy + ((j >> 1) & 1) * 4 + ((v >> 1) & 3);
             const int z = oz + (j >> 2) * 4 + (v >> 3);
-            const uint4 e = *(const uint4 *)(a.pyr + a.src.stage + (size_t)elem_index(a.src, x, y, z, f) * 16);
+            uint4 e = make_uint4(0u, 0u, 0u, 0u);
+            if (!src_zero(a, x, y, z)) e = *(const uint4 *)(a.pyr + a.src.stage + (size_t)elem_index(a.src, x, y, z, f) * 16);
             float4 *d = &bufA[f][((z - oz) * R + (y - oy)) * R + (x - ox)];
             d[0] = unpack_h4(e.x, e.y);
             d[1] = unpack_h4(e.z, e.w);
@@ -212,7 +223,8 @@ __global__ __launch_bounds__(256) void k_mip_chain(MipArgs a) {
                     y = (i / R) % R;
                     z = i / (R * R);
                 }
-                bufA[f][((z - oz) * R + (y - oy)) * R + (x - ox)] = load_stage_texel(a.pyr, a.src, x, y, z, f);
+                bufA[f][((z - oz) * R + (y - oy)) * R + (x - ox)] =
+                    src_zero(a, x, y, z) ? make_float4(0.f, 0.f, 0.f, 0.f) : load_stage_texel(a.pyr, a.src, x, y, z, f);
             }
         }
     }
@@ -285,6 +297,7 @@ hipError_t launch_mip(const PyrDesc &pyr, hipStream_t s) {
         for (int k = 0; k < 3; ++k) a.out[k] = pyr.lv[k < nout ? src + 1 + k : src + 1];
         a.nout = nout;
         a.fp32 = fp32;
+        a.zero0 = src == 0 ? pyr.zero0 : nullptr;
         const int nreg = S.size / R;
         dim3 grid((unsigned)(nreg * nreg * nreg));
         if (pyr.aniso && S.nfaces == 6) hipLaunchKernelGGL((k_mip_chain<true, true>), grid, dim3(256), 0, s, a);

@@ -129,7 +129,8 @@ __device__ __forceinline__ float4 inject_texel(uint32_t a32, bool lit, float opa
 template <int FMT, bool TEMPORAL>
 __global__ __launch_bounds__(256) void k_normalize_inject(uint32_t *albedo, const uint8_t *__restrict__ litmask,
                                                           int dim, LevelInfo L0, float opacity, float decay,
-                                                          int rewrite, char *pyr, uint32_t *partials) {
+                                                          int rewrite, char *pyr, uint8_t *zero0,
+                                                          uint32_t *partials) {
     const float mw = 1.0f - decay;  // the mix weight of this frame's value
     const float pw = 1.0f - mw;     // and of the previous one, as GLSL mix computes it
     const int nb = L0.nb;
@@ -146,8 +147,10 @@ __global__ __launch_bounds__(256) void k_normalize_inject(uint32_t *albedo, cons
         const uint32_t a32 = albedo[gi];
         const bool lit = litmask[li] != 0;
         char *elem = pyr + L0.stage + (size_t)elem_index(L0, x, y, z, 0) * 16;
+        // the brick's elements already hold +0 (wave-uniform)
+        const bool was0 = zero0 != nullptr && zero0[b] != 0;
         float4 prev = make_float4(0.f, 0.f, 0.f, 0.f), prev1 = prev;
-        if (TEMPORAL) {  // this element as stored last frame (texels x, x+1 for fp16 pairs)
+        if (TEMPORAL && !was0) {  // this element as stored last frame (texels x, x+1 for fp16 pairs)
             const uint4 e = *(const uint4 *)elem;
             if (FMT == 1) {
                 prev = make_float4(__uint_as_float(e.x), __uint_as_float(e.y), __uint_as_float(e.z),
@@ -167,7 +170,11 @@ __global__ __launch_bounds__(256) void k_normalize_inject(uint32_t *albedo, cons
         }
         if (rewrite && nw != a32) albedo[gi] = nw;
         if (FMT == 1) {
-            *(float4 *)elem = r;
+            const bool nz = (__float_as_uint(r.x) | __float_as_uint(r.y) | __float_as_uint(r.z) |
+                             __float_as_uint(r.w)) != 0u;
+            const bool now0 = __ballot(nz) == 0;
+            if (!(now0 && was0)) *(float4 *)elem = r;  // an all-zero brick over zeros is not rewritten
+            if (zero0 != nullptr && lane == 0 && now0 != was0) zero0[b] = now0 ? 1 : 0;
             continue;
         }
         const uint2 h = pack_h4(r);
@@ -181,9 +188,13 @@ __global__ __launch_bounds__(256) void k_normalize_inject(uint32_t *albedo, cons
                 h1 = pack_h4(inject_texel<TEMPORAL>(a1, lit1, opacity, prev1, mw, pw, &nw1));
             }
         }
-        *(uint4 *)elem = make_uint4(h.x, h.y, h1.x, h1.y);
-        if (x == 0)  // texel 0 is also the second half of element -1, in the border brick
-            *(uint2 *)(pyr + (uint32_t)(L0.stage + elem_index(L0, -1, y, z, 0) * 16u) + 8) = h;
+        const bool now0 = __ballot((h.x | h.y | h1.x | h1.y) != 0u) == 0;
+        if (!(now0 && was0)) {  // an all-zero brick over zeros is not rewritten
+            *(uint4 *)elem = make_uint4(h.x, h.y, h1.x, h1.y);
+            if (x == 0)  // texel 0 is also the second half of element -1, in the border brick
+                *(uint2 *)(pyr + (uint32_t)(L0.stage + elem_index(L0, -1, y, z, 0) * 16u) + 8) = h;
+        }
+        if (zero0 != nullptr && lane == 0 && now0 != was0) zero0[b] = now0 ? 1 : 0;
     }
     block_partial(uniq, maxf, lit_count, 0u, partials);
 }
@@ -211,7 +222,7 @@ hipError_t launch_normalize_inject(uint32_t *albedo, const uint8_t *litmask, int
     const int rewrite = (opacity > 0.0f && opacity * 255.0f < 0.5f) ? 0 : 1;
 #define VCT_INJECT_LAUNCH(F, T)                                                                                     \
     hipLaunchKernelGGL((k_normalize_inject<F, T>), grid, dim3(256), 0, s, albedo, litmask, dim, L0, opacity, decay, \
-                       rewrite, pyr.pyr, partials)
+                       rewrite, pyr.pyr, pyr.zero0, partials)
     if (L0.fmt == 1 && temporal) VCT_INJECT_LAUNCH(1, true);
     else if (L0.fmt == 1) VCT_INJECT_LAUNCH(1, false);
     else if (temporal) VCT_INJECT_LAUNCH(0, true);
@@ -275,8 +286,16 @@ hipError_t launch_fill_holes(const PyrDesc &pyr, void *tmp, hipStream_t s) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const size_t bytes = (size_t)L0.nelem * 16;
-    return hipMemcpyAsync(pyr.pyr + L0.stage - (size_t)L0.pad * 16, tmp, bytes, hipMemcpyDeviceToDevice,
-                          s);  // glCopyImageSubData
+    e = hipMemcpyAsync(pyr.pyr + L0.stage - (size_t)L0.pad * 16, tmp, bytes, hipMemcpyDeviceToDevice,
+                       s);  // glCopyImageSubData
+    if (e != hipSuccess) return e;
+    return launch_zero0_reset(pyr, s);
+}
+
+hipError_t launch_zero0_reset(const PyrDesc &pyr, hipStream_t s) {
+    if (!pyr.zero0) return hipSuccess;
+    const size_t nb = (size_t)pyr.lv[0].nb;
+    return launch_fill_u32((uint32_t *)pyr.zero0, 0u, (nb * nb * nb + 3) / 4, s);
 }
 
 // ---------------------------------------------------------------- conversions
@@ -310,7 +329,9 @@ hipError_t launch_linear_to_pyr0(const PyrDesc &pyr, const float4 *in, hipStream
     const LevelInfo &L = pyr.lv[0];
     const size_t n = (size_t)L.size * L.size * L.size;
     hipLaunchKernelGGL(k_linear_to_pyr0, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, pyr.pyr, L, in);
-    return hipGetLastError();
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_zero0_reset(pyr, s);
 }
 
 // bricked voxel grid <-> linear x-fastest (downloads and uploads)

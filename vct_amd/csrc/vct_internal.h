@@ -77,7 +77,15 @@ struct PyrDesc {
     float *scales;   // per-(brick, face) fp8 scales, or null
     LevelInfo lv[VCT_MAX_LEVELS];
     const LevelInfo *d_lv;  // device copy of lv[0..nlevels)
+    // One byte per level-0 brick (x fastest, nb^3 rounded up to whole words):
+    // 1 = every element the injection writes for the brick (its 64 elements,
+    // and for bricks at x = 0 the second halves of the border elements
+    // before them) holds +0 in memory.  The injection keeps the bytes; any
+    // other writer of level 0 resets them to 0 (unknown).
+    uint8_t *zero0;
 };
+// Resets PyrDesc::zero0 to "unknown" (after a write of level 0 other than the injection).
+hipError_t launch_zero0_reset(const PyrDesc &pyr, hipStream_t s);
 
 // Grid extent as the kernels use it (VCT::center/min/max).
 struct Extent {
