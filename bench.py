@@ -29,6 +29,13 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 PEAK_HBM_GBS = 8000.0
+# HBM traffic of one k_trace launch on the default workload (scene 2, 256^3 fp16
+# 6-direction mips, 1920x1080): rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE in KiB
+# per dispatch (profiles/counters_trace_scalar_path.md), scaled by the factors a
+# 1 GiB device copy gives the same counters, x2 and x1
+# (profiles/counters_copy_calibration.md).  The read factor comes from a copy's
+# 128-byte requests, so for smaller requests the read bytes are an upper bound.
+TRACE_FETCH_KIB, TRACE_WRITE_KIB = 68323, 64802
 METRIC = "ms/frame voxelize+inject+mip+trace @256^3,1080p; GB/s vs peak; learned vs plain"
 
 
@@ -291,6 +298,8 @@ def main() -> int:
                     for name, key in (("voxelize", "voxelize_ms"), ("radiance", "radiance_ms"),
                                       ("mip", "mipmap_ms"), ("trace", "render_ms")) if passes[key] > 0}
     trace_gbs = ab["trace"] / (passes["render_ms"] * 1e-3) / 1e9
+    default_workload = (args.scene == 2 and cfg.dim == 256 and cfg.width == 1920 and cfg.height == 1080 and
+                        cfg.storage == 0 and bool(cfg.anisotropic))
     out = {
         "metric": METRIC,
         "value": round(metric_ms, 4),
@@ -315,7 +324,10 @@ def main() -> int:
         "cones_per_s": round(npix * ncones / (passes["render_ms"] * 1e-3), 1) if passes["render_ms"] > 0 else None,
         "pass_algorithmic_gbs": per_pass_gbs,
         "roofline": {"kernel": "k_trace", "bound": "hbm", "achieved": round(trace_gbs, 1), "peak": PEAK_HBM_GBS,
-                     "unit": "GB/s", "frac": round(trace_gbs / PEAK_HBM_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(trace_gbs / PEAK_HBM_GBS, 4),
+                     "traffic": (TRACE_FETCH_KIB * 2 + TRACE_WRITE_KIB) * 1024 if default_workload else None,
+                     "traffic_source": ("rocprofv3 --pmc FETCH_SIZE x2 + WRITE_SIZE x1 per k_trace dispatch, "
+                                        "profiles/counters_trace_scalar_path.md") if default_workload else None,
                      "algorithmic_bytes": ab["trace"]},
     }
     app.close()
