@@ -94,7 +94,9 @@ hipError_t raster_scan(RasterScratch &rs, size_t nslots, hipStream_t s) {
     size_t bytes = rs.scan_tmp_bytes;
     e = hipcub::DeviceScan::ExclusiveSum(rs.scan_tmp, bytes, rs.counts, rs.offs, (int)(nslots + 1), s);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_chunk_starts, dim3(1024), dim3(256), 0, s, rs.offs, (int)nslots, rs.chunk,
+    // each chunk is a chain of dependent loads (the search): 8 wavefronts per
+    // SIMD of the 256 CUs keep the most of them in flight
+    hipLaunchKernelGGL(k_chunk_starts, dim3(2048), dim3(256), 0, s, rs.offs, (int)nslots, rs.chunk,
                        rs.chunk_cap);
     return hipGetLastError();
 }
