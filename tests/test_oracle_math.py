@@ -120,3 +120,37 @@ def test_axis_projection_plus_swizzle_is_linear_voxel_position(oracle, axis):
             u[2] = 1 - u[2]
             expect = (p - c - mn) / (mx - mn)
             assert np.allclose(u, expect, atol=2e-6), (axis, p, u, expect)
+
+
+def test_unorm8_correction_step_is_exact():
+    """The device's byte / 255 (vct_device.h unorm8): q = x * RN(1/255), then
+    RN(q + RN(x - q * 255) * RN(1/255)) with each fma rounded once, equals the
+    correctly rounded x / 255 for every byte; the plain product does not."""
+    from fractions import Fraction as F
+
+    def rn32(v):  # round a rational to the nearest float32, ties to even
+        if v == 0:
+            return F(0)
+        s, v = (-1 if v < 0 else 1), abs(v)
+        e = v.numerator.bit_length() - v.denominator.bit_length()
+        while F(2) ** e > v:
+            e -= 1
+        while F(2) ** (e + 1) <= v:
+            e += 1
+        m = v / F(2) ** (e - 23)
+        fl = m.numerator // m.denominator
+        rem = m - fl
+        if rem > F(1, 2) or (rem == F(1, 2) and fl % 2 == 1):
+            fl += 1
+        return s * F(fl) * F(2) ** (e - 23)
+
+    R = rn32(F(1, 255))
+    assert float(R) == float(np.float32(1.0) / np.float32(255.0))
+    plain_off = 0
+    for x in range(256):
+        want = rn32(F(x, 255))
+        assert float(want) == float(np.float32(x) / np.float32(255.0))  # numpy's IEEE division agrees
+        q = rn32(x * R)
+        plain_off += q != want
+        assert rn32(q + rn32(x - q * 255) * R) == want, x
+    assert plain_off == 126
