@@ -135,12 +135,14 @@ __global__ __launch_bounds__(256) void k_normalize_inject(uint32_t *albedo, cons
     const float pw = 1.0f - mw;     // and of the previous one, as GLSL mix computes it
     const int nb = L0.nb;
     const uint32_t nbricks = (uint32_t)nb * nb * nb;
+    const int lgnb = __builtin_ctz((unsigned)nb);
     const int lane = threadIdx.x & 63;
     const int vx = lane & 3, vy = (lane >> 2) & 3, vz = lane >> 4;
     uint32_t uniq = 0, maxf = 0, lit_count = 0;
     for (uint32_t b = blockIdx.x * 4u + (threadIdx.x >> 6); b < nbricks; b += gridDim.x * 4u) {
-        const int bx = (int)(b % (uint32_t)nb), by = (int)((b / (uint32_t)nb) % (uint32_t)nb),
-                  bz = (int)(b / ((uint32_t)nb * nb));
+        // nb = dim / 4 is a power of two (dim is one): shifts and masks
+        const int bx = (int)(b & (uint32_t)(nb - 1)), by = (int)((b >> lgnb) & (uint32_t)(nb - 1)),
+                  bz = (int)(b >> (2 * lgnb));
         const int x = bx * 4 + vx, y = by * 4 + vy, z = bz * 4 + vz;
         const size_t gi = ((size_t)b << 6) + (size_t)lane;  // grid_index(x, y, z)
         const size_t li = ((size_t)z * dim + y) * dim + x;   // the lit mask is linear
