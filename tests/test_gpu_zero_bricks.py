@@ -42,6 +42,26 @@ def _check_level0(app, fs, alb, sms, what, fp32=False):
     return got
 
 
+def _check_raw_pairs(ctx, rad0, what):
+    """Level 0 as stored: fp16 pair elements in 4^3 bricks with a zero border
+    brick on each side; element x holds texel x and texel x + 1 (zero past
+    the last texel), and the border element before texel 0 holds (0, texel 0).
+    The trace's sampler reads both halves, the texel download only the first."""
+    d = rad0.shape[0]
+    nbp = d // 4 + 2
+    raw = ctx.download(abi.VCT_BUF_RADIANCE_RAW, 0, np.float16)
+    assert raw.size == nbp ** 3 * 64 * 8
+    raw = raw.reshape(nbp, nbp, nbp, 4, 4, 4, 2, 4)  # brick z, y, x; voxel z, y, x; half; channel
+    h = rad0.astype(np.float16)  # [z, y, x, c], fp16 values already
+    e = np.zeros((d + 8, d + 8, d + 8, 2, 4), np.float16)  # padded texel coordinates, offset 4
+    e[4:d + 4, 4:d + 4, 4:d + 4, 0] = h
+    e[4:d + 4, 4:d + 4, 4:d + 3, 1] = h[:, :, 1:]
+    e[4:d + 4, 4:d + 4, 3, 1] = h[:, :, 0]
+    want = e.reshape(nbp, 4, nbp, 4, nbp, 4, 2, 4).transpose(0, 2, 4, 1, 3, 5, 6, 7)
+    bad = raw.view(np.uint16) != want.view(np.uint16)
+    assert not bad.any(), f"{what}: {int(bad.any(axis=(6, 7)).sum())} stored pair elements differ"
+
+
 def _zero_bricks(rad0):
     d = rad0.shape[0]
     b = rad0.reshape(d // 4, 4, d // 4, 4, d // 4, 4, 4)
@@ -59,6 +79,8 @@ def test_moving_scene_over_frames(storage):
         for frame in range(4):
             fs, alb, sms = _inject_frame(app, frame)
             got = _check_level0(app, fs, alb, sms, f"frame {frame}", fp32=(storage == 1))
+            if storage == 0:
+                _check_raw_pairs(ctx, got, f"frame {frame}")
             ctx.mip(fs["mip"])
             faces = ob.mip_aniso(got, g.levels, fp32=(storage == 1))
             for l in (1, 2):
@@ -82,11 +104,13 @@ def test_upload_and_fill_holes_reset_the_record():
         g = ctx.grid()
         fs, alb, sms = _inject_frame(app, 0)
         got = _check_level0(app, fs, alb, sms, "first injection")
+        _check_raw_pairs(ctx, got, "first injection")
         assert _zero_bricks(got).any()
         # an upload writes every brick: the next injection must rewrite the empty ones
         ctx.upload(abi.VCT_BUF_RADIANCE, 0, np.full((g.dim ** 3, 4), 0.25, np.float32))
         fs, alb, sms = _inject_frame(app, 0)
-        _check_level0(app, fs, alb, sms, "injection after an upload")
+        got = _check_level0(app, fs, alb, sms, "injection after an upload")
+        _check_raw_pairs(ctx, got, "injection after an upload")
         # an upload read by the mip chain: no brick is skipped
         up = np.full((g.dim, g.dim, g.dim, 4), 0.5, np.float32)
         ctx.upload(abi.VCT_BUF_RADIANCE, 0, up)
