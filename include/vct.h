@@ -51,7 +51,8 @@ extern "C" {
 #define VCT_LIGHT_DIRECTIONAL 1u
 
 /* ---- what vct_download / vct_upload address ------------------------------ */
-#define VCT_BUF_ALBEDO 0     /* u32 packed RGBA8 [dim^3], x fastest (voxelColor)  */
+#define VCT_BUF_ALBEDO 0     /* u32 packed RGBA8 [s^3], x fastest (voxelColor);
+                                level 0 the grid, levels >= 1 its mip chain      */
 #define VCT_BUF_NORMAL 1     /* u32 packed RGBA8 [dim^3] (voxelNormal)            */
 #define VCT_BUF_RADIANCE 2   /* float4 [s^3] of mip `level` (decoded to f32); for
                                  anisotropic levels >= 1 pass l | (face << 4),
@@ -157,7 +158,9 @@ typedef struct vct_inject_params {
 /* Mip chain (src/Application.cpp:877-921, shaders/filterRadiance.comp). */
 typedef struct vct_mip_params {
     int kernel_mode;   /* 0 = KERNEL_BOX2 (the only live mode)                     */
-    int albedo_chain;  /* reserved: the albedo (voxelColor) chain is not built     */
+    int albedo_chain;  /* 1: also filter the albedo (voxelColor) chain, as the
+                          reference's mipmap pass does (Application.cpp:904-917):
+                          RGBA8 levels, each packUnorm4x8 of the BOX2 mean      */
 } vct_mip_params;
 
 /* Camera for the G-buffer (depth pre-pass, src/Application.cpp:936-965). */

@@ -97,6 +97,7 @@ def lib() -> C.CDLL:
             "or_mip": (None, [vp, C.c_int, C.c_int, C.c_int]),
             "or_mip_aniso": (None, [vp, vp, C.c_int, C.c_int, C.c_int]),
             "or_fp8_quantize_level": (None, [vp, C.c_int]),
+            "or_mip_albedo": (None, [vp, C.c_int, C.c_int]),
             "or_gbuffer": (None, [C.POINTER(Scene), C.POINTER(Camera), vp]),
             "or_trace": (None, [vp, C.c_int, vp, vp, C.POINTER(TraceParams), vp, vp]),
             "or_trace64": (None, [vp, C.c_int, vp, vp, C.POINTER(TraceParams), vp, vp]),
@@ -292,6 +293,18 @@ def mip_aniso(rad0: np.ndarray, nlevels: int, fp32=False) -> list:
     top = (C.c_void_p * nlevels)(*([None] + [C.cast(r, C.c_void_p).value for r in rows[1:]]))
     lib().or_mip_aniso(_p(src), C.cast(top, C.c_void_p), dim, nlevels, 1 if fp32 else 0)
     return faces
+
+
+def mip_albedo(grid: np.ndarray, nlevels: int) -> list:
+    """The albedo (voxelColor) chain: [grid, level 1, ...] as u32 RGBA8 volumes."""
+    dim = grid.shape[0]
+    levels = [np.ascontiguousarray(grid, np.uint32).reshape(dim, dim, dim)]
+    for l in range(1, nlevels):
+        s = max(1, dim >> l)
+        levels.append(np.zeros((s, s, s), np.uint32))
+    ptrs = (C.c_void_p * nlevels)(*[a.ctypes.data for a in levels])
+    lib().or_mip_albedo(C.cast(ptrs, C.c_void_p), dim, nlevels)
+    return levels
 
 
 def fp8_quantize(level: np.ndarray) -> np.ndarray:

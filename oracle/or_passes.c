@@ -653,6 +653,33 @@ void or_mip(float **levels, int dim, int nlevels, int storage_fp32) {
     }
 }
 
+/* The voxelColor chain of the same mipmap pass (src/Application.cpp:904-917):
+ * filterRadiance.comp KERNEL_BOX2 with rgba8 image bindings, so each level is
+ * packUnorm4x8 of 0.125 * the sum of its children read back as k/255. */
+void or_mip_albedo(uint32_t **levels, int dim, int nlevels) {
+    for (int l = 0; l + 1 < nlevels; ++l) {
+        int s = dim >> l;
+        if (s < 1) s = 1;
+        int d = s >> 1;
+        if (d < 1) break;
+        const uint32_t *src = levels[l];
+        uint32_t *dst = levels[l + 1];
+        for (int z = 0; z < d; ++z)
+            for (int y = 0; y < d; ++y)
+                for (int x = 0; x < d; ++x) {
+                    float v[4] = {0, 0, 0, 0};
+                    for (int o = 0; o < 8; ++o) {
+                        int xx = 2 * x + BOX2[o][0], yy = 2 * y + BOX2[o][1], zz = 2 * z + BOX2[o][2];
+                        float t[4];
+                        or_unpack_unorm4x8(src[((size_t)zz * s + yy) * s + xx], t);
+                        for (int q = 0; q < 4; ++q) v[q] = v[q] + t[q];
+                    }
+                    for (int q = 0; q < 4; ++q) v[q] = v[q] * 0.125f;
+                    dst[((size_t)z * d + y) * d + x] = or_pack_unorm4x8(v);
+                }
+    }
+}
+
 /* 6-direction anisotropic chain.  Face f: axis f/2, sign + (f even) means the
  * light/ray travels toward +axis, so the near child is the lower coordinate.
  * Each output is the mean of the four front-to-back composites

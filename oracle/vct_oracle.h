@@ -28,8 +28,16 @@
  *   or_fill_holes                                           brute force 3x3x3
  *   or_mip                                                  constant/ramp closed
  *       forms, mean preservation, numpy float16 rounding
+ *   or_mip_albedo                                           round(255 * mean of the
+ *       eight children's bytes / 255) per level in float64 (exact ties listed)
  *   or_trace / or_trace64                                   empty volume -> 0,
- *       uniform volume closed form alpha_n = 1-(1-a)^n
+ *       uniform volume closed form alpha_n = 1-(1-a)^n; LINEAR_MIPMAP_LINEAR
+ *       level blend and the cone's lambda schedule on pyramids whose levels
+ *       (and anisotropic faces) are uniform with distinct values
+ *   voxelize lighting (inside or_voxelize)                  closed forms on a quad:
+ *       directional N.(-d), point-light smoothstep band, shadow * (1 - PCF)
+ *   or_fp8_quantize_level                                   scale = brick max / 448,
+ *       nearest E4M3 code by table search, per-brick independence
  *   or_half / or_fp8                                        numpy float16; OCP
  *       E4M3 code table round trip
  *   or_gbuffer                                              closed form plane hits
@@ -165,6 +173,10 @@ void or_fill_holes(const float *src, float *dst, int dim, int storage_fp32);
  * f = 0..5 for +x,-x,+y,-y,+z,-z) receive the six directional volumes. */
 void or_mip(float **levels, int dim, int nlevels, int storage_fp32);
 void or_mip_aniso(const float *level0, float ***aniso, int dim, int nlevels, int storage_fp32);
+/* The albedo (voxelColor) chain of the same pass (src/Application.cpp:904-917):
+ * levels[0] is the RGBA8 grid (linear, x fastest), levels[l] (l >= 1) receive
+ * packUnorm4x8 of the BOX2 mean of the unorm8 children. */
+void or_mip_albedo(uint32_t **levels, int dim, int nlevels);
 /* FP8 E4M3 + per-4^3-brick fp32 scale round trip of one level (values
  * replaced in place by their decoded storage). */
 void or_fp8_quantize_level(float *level, int size);

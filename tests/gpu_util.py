@@ -43,6 +43,10 @@ def run_frame_capture(app, frame=0, fill_holes=False):
         out["rad0_filled"] = ctx.radiance(0)
     ctx.mip(fs["mip"])
     out["levels"] = [ctx.radiance(l) for l in range(g.levels)]
+    if fs["mip"].albedo_chain:
+        out["albedo_levels"] = [None] + [
+            ctx.download(abi.VCT_BUF_ALBEDO, l, np.uint32).reshape((max(1, g.dim >> l),) * 3)
+            for l in range(1, g.levels)]
     if g.anisotropic:
         out["faces"] = [None] + [[ctx.radiance(l, f) for f in range(6)] for l in range(1, g.levels)]
     ctx.gbuffer(fs["cam"])

@@ -77,6 +77,14 @@ def test_mip_chain_bit_exact(scene1):
             f"level {l}: max abs {np.abs(got - ref[l]).max()}"
 
 
+def test_albedo_chain_bit_exact(scene1):
+    app, cap = scene1
+    g = cap["grid"]
+    ref = ob.mip_albedo(cap["albedo_norm"], g.levels)
+    for l in range(1, g.levels):
+        assert np.array_equal(cap["albedo_levels"][l], ref[l]), f"albedo level {l}"
+
+
 def test_gbuffer_matches_oracle(scene1):
     app, cap = scene1
     fs = cap["fs"]

@@ -1,7 +1,8 @@
 """GPU parity beyond the small scene: the dynamic scene at reduced resolution
 (5 shadow-casting lights including perspective point-light maps, 6-direction
 mips, the specular cone, the composed image with point lights and reflections),
-the FP32 and FP8 storage formats, and fill-holes."""
+the FP32 and FP8 storage formats on isotropic and 6-direction pyramids, and
+fill-holes."""
 import numpy as np
 import pytest
 
@@ -53,6 +54,18 @@ def test_scene2_aniso_mips_bit_exact(scene2_small):
         for f in range(6):
             got = cap["faces"][l][f]
             assert np.array_equal(got.view(np.uint32), faces[l][f].view(np.uint32)), (l, f)
+
+
+def test_scene2_albedo_chain_bit_exact(scene2_small):
+    """The voxelColor chain the mipmap pass filters beside the radiance
+    (Application.cpp:904-917), from the normalized albedo grid."""
+    app, cap = scene2_small
+    g = cap["grid"]
+    ref = ob.mip_albedo(cap["albedo_norm"], g.levels)
+    for l in range(1, g.levels):
+        got = cap["albedo_levels"][l]
+        assert np.array_equal(got, ref[l]), (l, int((got != ref[l]).sum()))
+    assert (ref[1] >> 24).any()
 
 
 def test_scene2_gbuffer_and_trace(scene2_small):
@@ -108,6 +121,43 @@ def test_storage_formats(storage):
     ind, _ = ob.trace(cap["gbuf"], cap["levels"], tp)
     assert rel_err(cap["indirect"][:, :3], ind[:, :3]) < 1e-3
     app.close()
+
+
+@pytest.mark.parametrize("storage", [1, 2])
+def test_scene2_storage_formats_aniso(storage):
+    """FP32 and FP8 (+ per-(brick, face) scale) storage of the 6-direction
+    pyramid: level 0, every face of every level bit for bit, and the traced
+    indirect and specular images against the oracle on the decoded faces."""
+    app = make_app(2, dim=128, levels=7, width=480, height=270, shadow_size=256, storage=storage)
+    try:
+        print(f"scene 2 @128^3, anisotropic, storage {storage}", flush=True)
+        cap = run_frame_capture(app, 5)
+        fs, g = cap["fs"], cap["grid"]
+        assert g.anisotropic
+        fp32 = storage == 1
+        _, rad0, _, _ = ob.normalize_inject(cap["albedo_vox"], g.dim, tuple(g.center), tuple(g.vmin),
+                                            tuple(g.vmax), fs["lights"], cap["shadow"], fp32=fp32)
+        assert np.array_equal(cap["rad0"].view(np.uint32), rad0.view(np.uint32))
+        faces = ob.mip_aniso(cap["rad0"], g.levels, fp32=fp32)
+        for l in range(1, g.levels):
+            for f in range(6):
+                r = faces[l][f]
+                if storage == 2 and l >= 3:
+                    r = ob.fp8_quantize(r)  # E4M3 with an fp32 scale per 4^3 brick of this face
+                got = cap["faces"][l][f]
+                assert np.array_equal(got.view(np.uint32), r.view(np.uint32)), (storage, l, f,
+                                                                                np.abs(got - r).max())
+        mesh = app.mesh()
+        tp = ob.trace_params(fs["trace"], g.dim, g.levels, tuple(g.center), tuple(g.vmin), tuple(g.vmax),
+                             mesh["rough"] is not None, 1)
+        ind, spec = ob.trace(cap["gbuf"], cap["levels"], tp, aniso_faces=cap["faces"])
+        ei, es = rel_err(cap["indirect"][:, :3], ind[:, :3]), rel_err(cap["specular"], spec)
+        print(f"storage {storage}: indirect {ei:.3e}, specular {es:.3e} vs the oracle", flush=True)
+        assert ei < 1e-3 and es < 1e-3
+        assert rel_err(cap["indirect"][:, 3], ind[:, 3]) < 1e-3
+        assert ind[:, :3].sum() > 0 and spec[:, :3].sum() > 0
+    finally:
+        app.close()
 
 
 def test_fill_holes_bit_exact():

@@ -1,0 +1,409 @@
+"""Pins the oracle functions that the per-pass tests leave open, each by a
+closed form that a plausible slip in the restatement fails.
+
+  trace LOD      textureLod with LINEAR_MIPMAP_LINEAR on a pyramid whose level l
+                 is uniform with its own value v_l: at the volume centre every
+                 level samples exactly v_l, so a sample at lambda is
+                 (1 - fr) v_D + fr v_{D+1} (D = floor(lambda), fr = lambda - D),
+                 v_0 by NEAREST for lambda <= 0 and v_q for lambda >= q (GL 4.5
+                 core spec, 8.14.3 "Mipmapping")
+  cone schedule  the same pyramid along a whole cone (phong.frag:143-177):
+                 lambda_k = log2(max(1, 2 h_k tan(theta/2))) + lodOffset,
+                 h_{k+1} = h_k (1 + tan(theta/2)), the zero border of every
+                 level through the per-axis in-range weight
+                 (1 - f)[0 <= i < s] + f[0 <= i + 1 < s], and, for anisotropic
+                 levels, the face mix sum_axis dir_axis^2 v_{l, face(axis, sign)}
+  lighting       voxelize.frag:196-225 on a quad in a plane z = z0 whose voxels
+                 each receive one fragment at the voxel centre: directional
+                 round(255 clamp(albedo C I cos)) with cos = N.(-d), point light
+                 I C albedo (N.L) (1 - smoothstep(0.75 r, r, dist)) across the
+                 smoothstep band, a fully occluding shadow map giving 0
+  fp8 bricks     E4M3 with one FP32 scale per 4^3 brick, scale = max|v| / 448
+                 (the largest E4M3 value), each value rounded to the nearest
+                 code (ties to even) by a search over the code table
+
+The expectations are computed here, in float64 from the formulas above, not
+by calling the oracle's own helpers.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+from vct_amd import _abi as abi
+
+
+# ------------------------------------------------------------------ trace LOD
+def _uniform_pyramid(dim, nlevels, values):
+    return [np.broadcast_to(np.asarray(values[l], np.float32), (max(1, dim >> l),) * 3 + (4,)).copy()
+            for l in range(nlevels)]
+
+
+LEVEL_VALUES = [(0.90, 0.05, 0.30, 0.11), (0.20, 0.70, 0.45, 0.23), (0.55, 0.33, 0.95, 0.37),
+                (0.05, 0.85, 0.10, 0.52), (0.62, 0.41, 0.73, 0.68), (0.13, 0.27, 0.58, 0.81),
+                (0.77, 0.66, 0.02, 0.94)]
+
+
+@pytest.mark.parametrize("lam", [-0.5, 0.0, 0.2, 0.75, 1.0, 1.3, 2.1, 2.9, 3.5, 3.99, 4.0, 4.7, 11.0])
+def test_sample_lod_blends_adjacent_levels(oracle, lam):
+    dim, L = 16, 5
+    q = L - 1
+    levels = _uniform_pyramid(dim, L, LEVEL_VALUES)
+    v = np.asarray(LEVEL_VALUES, np.float64)
+    ptrs = (C.c_void_p * L)(*[a.ctypes.data for a in levels])
+    p = np.array([0.5, 0.5, 0.5], np.float32)
+    out = np.zeros(4, np.float32)
+    oracle.lib().or_sample(C.cast(ptrs, C.c_void_p), dim, L, p.ctypes.data, float(lam), out.ctypes.data)
+    if lam <= 0:
+        want = v[0]
+    elif lam >= q:
+        want = v[q]
+    else:
+        d = math.floor(lam)
+        fr = lam - d
+        want = (1 - fr) * v[d] + fr * v[d + 1]
+    np.testing.assert_allclose(out, want, rtol=0, atol=2e-7)
+
+
+def _inrange_weight(p, s):
+    """Trilinear weight that lands inside [0, s) along one axis (the rest
+    falls on the zero border)."""
+    u = p * s - 0.5
+    i = math.floor(u)
+    f = u - i
+    return (1 - f) * (0 <= i < s) + f * (0 <= i + 1 < s)
+
+
+def _expected_cone(dim, nlevels, iso, faces, pos, nrm, dirn, steps, angle, bias, h0, lod_offset):
+    """The cone of phong.frag:135-180 over a pyramid of uniform levels, in
+    float64 from the closed form of each sample."""
+    d = np.asarray(dirn, np.float64)
+    d = d / np.linalg.norm(d)
+    t = math.tan(angle / 2)
+    start = np.asarray(pos, np.float64) + bias * np.asarray(nrm, np.float64) / dim
+    q = nlevels - 1
+
+    def level_value(l, x):
+        s = max(1, dim >> l)
+        w = _inrange_weight(x[0], s) * _inrange_weight(x[1], s) * _inrange_weight(x[2], s)
+        if faces is None or l == 0:
+            return w * iso[l]
+        mix = sum(d[a] ** 2 * faces[l][2 * a + (0 if d[a] >= 0 else 1)] for a in range(3))
+        return w * mix
+
+    c = np.zeros(3)
+    a = 0.0
+    h = h0
+    for _ in range(steps):
+        if a >= 0.95:
+            break
+        r = h * t
+        lam = math.log2(max(1.0, 2 * r)) + lod_offset
+        x = start + h * d / dim
+        if np.any(x < 0) or np.any(x > 1):
+            break
+        if lam <= 0:
+            raise AssertionError("the schedule below never magnifies")
+        if lam >= q:
+            s = level_value(q, x)
+        else:
+            D = math.floor(lam)
+            fr = lam - D
+            s = (1 - fr) * level_value(D, x) + fr * level_value(D + 1, x)
+        c += (1 - a) * s[:3]
+        a += (1 - a) * s[3]
+        h += r
+    return np.array([*c, a])
+
+
+def _oracle_cone(oracle, levels, faces, dim, nlevels, pos, nrm, dirn, steps, angle, bias, h0, lod_offset):
+    lp = (C.c_void_p * nlevels)(*[a.ctypes.data for a in levels])
+    ap = None
+    keep = []
+    if faces is not None:
+        rows = [None]
+        for l in range(1, nlevels):
+            arr = (C.c_void_p * 6)(*[f.ctypes.data for f in faces[l]])
+            keep.append(arr)
+            rows.append(C.cast(arr, C.c_void_p).value)
+        ap = (C.c_void_p * nlevels)(*rows)
+        ap = C.cast(ap, C.c_void_p)
+    out = np.zeros(4, np.float32)
+    p, n, dd = (np.asarray(v, np.float32) for v in (pos, nrm, dirn))
+    cone = oracle.Cone(steps, angle, bias, h0, lod_offset)
+    oracle.lib().or_trace_cone(C.cast(lp, C.c_void_p), ap, dim, nlevels, p.ctypes.data, n.ctypes.data,
+                               dd.ctypes.data, C.byref(cone), out.ctypes.data)
+    return out
+
+
+# low opacities so that a cone runs every step to the volume's edge; the
+# cone cases cover the diffuse and specular settings (VCTSettings defaults)
+CONE_VALUES = [(0.10, 0.40, 0.20, 0.020), (0.60, 0.15, 0.35, 0.045), (0.25, 0.55, 0.80, 0.075),
+               (0.85, 0.30, 0.05, 0.110), (0.40, 0.90, 0.60, 0.150), (0.05, 0.20, 0.95, 0.190),
+               (0.70, 0.65, 0.45, 0.240)]
+
+
+@pytest.mark.parametrize("case", [
+    dict(steps=16, angle=math.radians(60), bias=1.0, h0=1.0, lod=0.5, pos=(0.5, 0.5, 0.3), dirn=(0, 0, 1)),
+    dict(steps=32, angle=math.radians(30), bias=1.7, h0=0.5, lod=0.1, pos=(0.5, 0.5, 0.2), dirn=(0, 0, 1)),
+    dict(steps=16, angle=math.radians(60), bias=1.0, h0=1.0, lod=0.0, pos=(0.31, 0.42, 0.27), dirn=(0.3, 0.5, 0.8)),
+    dict(steps=24, angle=math.radians(20), bias=1.0, h0=1.0, lod=1.25, pos=(0.5, 0.5, 0.5), dirn=(-0.6, 0.2, 0.7)),
+])
+def test_cone_lod_schedule_on_distinct_levels(oracle, case):
+    dim, L = 64, 7
+    levels = _uniform_pyramid(dim, L, CONE_VALUES)
+    iso = [np.asarray(v, np.float64) for v in CONE_VALUES]
+    nrm = np.asarray(case["dirn"], np.float64)
+    nrm = nrm / np.linalg.norm(nrm)
+    got = _oracle_cone(oracle, levels, None, dim, L, case["pos"], nrm, case["dirn"], case["steps"], case["angle"],
+                       case["bias"], case["h0"], case["lod"])
+    want = _expected_cone(dim, L, iso, None, case["pos"], nrm, case["dirn"], case["steps"], case["angle"],
+                          case["bias"], case["h0"], case["lod"])
+    assert want[3] > 0.3  # the cone crosses several levels
+    np.testing.assert_allclose(got, want, rtol=0, atol=2e-5)
+
+
+def test_cone_anisotropic_face_mix(oracle):
+    """Anisotropic levels: each of the six faces of level l uniform with its
+    own value; a cone along a direction with three non-zero components mixes
+    the faces it faces by dir_axis^2."""
+    dim, L = 32, 6
+    rng = np.random.default_rng(12)
+    levels = _uniform_pyramid(dim, L, CONE_VALUES)
+    iso = [np.asarray(v, np.float64) for v in CONE_VALUES]
+    faces = [None]
+    face_vals = [None]
+    for l in range(1, L):
+        s = max(1, dim >> l)
+        vals = [np.array([*rng.uniform(0.05, 0.95, 3), rng.uniform(0.01, 0.12)]) for _ in range(6)]
+        face_vals.append(vals)
+        faces.append([np.broadcast_to(v.astype(np.float32), (s, s, s, 4)).copy() for v in vals])
+    for dirn in [(0.48, 0.6, 0.64), (-0.48, 0.6, -0.64), (0.36, -0.48, 0.8)]:
+        nrm = np.asarray(dirn)
+        got = _oracle_cone(oracle, levels, faces, dim, L, (0.45, 0.5, 0.55), nrm, dirn, 16, math.radians(60),
+                           1.0, 1.0, 0.5)
+        want = _expected_cone(dim, L, iso, face_vals, (0.45, 0.5, 0.55), nrm, dirn, 16, math.radians(60), 1.0, 1.0,
+                              0.5)
+        np.testing.assert_allclose(got, want, rtol=0, atol=2e-5)
+
+
+# ------------------------------------------------------------------ lighting
+DIM = 16
+Z0 = 0.03  # the quad's plane: inside voxel layer z = 8, off every boundary
+
+
+def _quad_scene(oracle, albedo):
+    q = np.array([[-0.5, -0.5, Z0], [0.5, -0.5, Z0], [0.5, 0.5, Z0], [-0.5, 0.5, Z0]], np.float32)
+    pos = np.stack([q[[0, 1, 2]], q[[0, 2, 3]]])
+    nrm = np.zeros_like(pos)
+    nrm[..., 2] = 1
+    mesh = dict(pos=pos, nrm=nrm, tan=np.array([[1, 0, 0]] * 2, np.float32),
+                albedo=np.array([albedo, albedo], np.uint32), obj=np.zeros(2, np.uint32), nobj=1)
+    return oracle.SceneArrays(mesh, np.eye(4, dtype=np.float32).reshape(1, 16))
+
+
+def _vox_params(oracle):
+    mx, my, mz = oracle.voxel_projections((0, 0, 0), (-1, -1, -1), (1, 1, 1))
+    v = abi.VoxelizeParams()
+    v.mvp_x[:], v.mvp_y[:], v.mvp_z[:] = list(mx), list(my), list(mz)
+    v.axis_override, v.lighting, v.atomic_max, v.msaa = -1, 1, 1, 1
+    return v
+
+
+def _light(kind, **kw):
+    L = abi.Light()
+    L.type = kind
+    L.enabled = 1
+    L.color[:] = kw.get("color", (1, 1, 1))
+    L.intensity = kw.get("intensity", 1.0)
+    L.direction[:] = kw.get("direction", (0, 0, -1))
+    L.position[:] = kw.get("position", (0, 0, 0))
+    L.range = kw.get("range", 0.0)
+    L.shadow_caster = 0
+    return L
+
+
+def _voxel_centres():
+    c = -1 + (np.arange(DIM) + 0.5) * (2.0 / DIM)
+    return c
+
+
+def _bytes(alb):
+    return np.stack([(alb >> (8 * k)) & 255 for k in range(4)], -1).astype(np.int64)
+
+
+def _check_quad(alb, expect_rgb):
+    """Every voxel of the quad's layer (x, y in 4..11, z = 8) holds
+    round(255 clamp(expect_rgb[y, x])) with alpha 255; nothing else is set."""
+    b = _bytes(alb)
+    occ = b[..., 3] > 0
+    want_occ = np.zeros_like(occ)
+    want_occ[8, 4:12, 4:12] = True
+    assert np.array_equal(occ, want_occ)
+    e = np.clip(expect_rgb, 0, 1) * 255
+    got = b[8, 4:12, 4:12, :3]
+    want = np.rint(e[4:12, 4:12])
+    tie = np.abs(e[4:12, 4:12] - np.floor(e[4:12, 4:12]) - 0.5) < 2e-3
+    assert np.all(b[8, 4:12, 4:12, 3] == 255)
+    assert np.all((got == want) | (tie & (np.abs(got - want) <= 1))), (got, want)
+
+
+ALB = (200, 140, 60)
+ALB_U32 = ALB[0] | (ALB[1] << 8) | (ALB[2] << 16) | (255 << 24)
+
+
+def test_voxelize_directional_light_closed_form(oracle):
+    sc = _quad_scene(oracle, ALB_U32)
+    d = np.array([0.3, -0.2, -0.9])
+    C3, I = np.array([0.9, 0.8, 0.7]), 1.6
+    L = _light(abi.VCT_LIGHT_DIRECTIONAL, direction=tuple(d), color=tuple(C3), intensity=I)
+    alb, _, _ = oracle.voxelize(sc, [L], [None], _vox_params(oracle), DIM)
+    cos = -d[2] / np.linalg.norm(d)  # N = +z, light travels along d: N . (-d / |d|)
+    rgb = np.asarray(ALB) / 255.0 * C3 * I * cos
+    assert rgb.max() > 1 and rgb.min() < 1  # the clamp and the scaling both show
+    _check_quad(alb, np.broadcast_to(rgb, (DIM, DIM, 3)))
+
+
+def test_voxelize_light_from_behind_is_black(oracle):
+    sc = _quad_scene(oracle, ALB_U32)
+    L = _light(abi.VCT_LIGHT_DIRECTIONAL, direction=(0.3, -0.2, 0.9))
+    alb, _, _ = oracle.voxelize(sc, [L], [None], _vox_params(oracle), DIM)
+    _check_quad(alb, np.zeros((DIM, DIM, 3)))
+
+
+def test_voxelize_point_light_smoothstep_band(oracle):
+    sc = _quad_scene(oracle, ALB_U32)
+    hgt, rng_, I = 0.2, 0.5, 2.5
+    C3 = np.array([1.0, 0.9, 0.6])
+    lp = np.array([0.02, -0.03, Z0 + hgt])
+    L = _light(abi.VCT_LIGHT_POINT, position=tuple(lp), range=rng_, color=tuple(C3), intensity=I)
+    alb, _, _ = oracle.voxelize(sc, [L], [None], _vox_params(oracle), DIM)
+    c = _voxel_centres()
+    X, Y = np.meshgrid(c, c)  # [y, x]
+    P = np.stack([X, Y, np.full_like(X, Z0)], -1)
+    dv = lp - P
+    dist = np.linalg.norm(dv, axis=-1)
+    ndl = np.maximum(0, dv[..., 2] / dist)
+    t = np.clip((dist - 0.75 * rng_) / (rng_ - 0.75 * rng_), 0, 1)
+    att = 1 - t * t * (3 - 2 * t)
+    rgb = (np.asarray(ALB) / 255.0 * I * C3)[None, None, :] * (ndl * att)[..., None]
+    inner = dist[4:12, 4:12] < 0.75 * rng_
+    band = (dist[4:12, 4:12] > 0.75 * rng_) & (dist[4:12, 4:12] < rng_)
+    outer = dist[4:12, 4:12] > rng_
+    assert inner.sum() >= 4 and band.sum() >= 8 and outer.sum() >= 4  # all three regimes are exercised
+    _check_quad(alb, rgb)
+
+
+def _ortho_light_matrix(oracle, d):
+    """A light frustum looking along d onto the quad (depth of the quad about 0.5)."""
+    d = np.asarray(d, np.float64) / np.linalg.norm(d)
+    eye = -10 * d
+    proj = oracle.ortho(-2, 2, -2, 2, 0.1, 20.0)
+    view = oracle.lookat(tuple(eye), (0, 0, 0), (0, 1, 0))
+    ls = np.asarray(oracle.mul4(proj, view), np.float32).reshape(-1)
+    return ls, np.asarray(oracle.inverse4(ls), np.float32).reshape(-1)
+
+
+@pytest.mark.parametrize("depth,lit", [(0.0, 0.0), (1.0, 1.0)])
+def test_voxelize_shadow_scales_light_by_one_minus_pcf(oracle, depth, lit):
+    sc = _quad_scene(oracle, ALB_U32)
+    d = (0.2, 0.1, -0.95)
+    L = _light(abi.VCT_LIGHT_DIRECTIONAL, direction=d, intensity=1.2)
+    L.shadow_caster, L.shadow_size = 1, 64
+    ls, lsi = _ortho_light_matrix(oracle, d)
+    L.ls[:], L.ls_inverse[:] = list(ls), list(lsi)
+    sm = np.full((64, 64), depth, np.float32)
+    alb, _, _ = oracle.voxelize(sc, [L], [sm], _vox_params(oracle), DIM)
+    dn = np.asarray(d) / np.linalg.norm(d)
+    rgb = np.asarray(ALB) / 255.0 * 1.2 * (-dn[2]) * lit
+    _check_quad(alb, np.broadcast_to(rgb, (DIM, DIM, 3)))
+
+
+def test_voxelize_lights_add_before_the_clamp(oracle):
+    sc = _quad_scene(oracle, ALB_U32)
+    d1, d2 = np.array([0.0, 0.0, -1.0]), np.array([0.6, 0.0, -0.8])
+    L1 = _light(abi.VCT_LIGHT_DIRECTIONAL, direction=tuple(d1), intensity=0.3)
+    L2 = _light(abi.VCT_LIGHT_DIRECTIONAL, direction=tuple(d2), intensity=0.45, color=(0.5, 1.0, 1.0))
+    alb, _, _ = oracle.voxelize(sc, [L1, L2], [None, None], _vox_params(oracle), DIM)
+    a = np.asarray(ALB) / 255.0
+    rgb = a * 0.3 * 1.0 + a * np.array([0.5, 1.0, 1.0]) * 0.45 * 0.8
+    _check_quad(alb, np.broadcast_to(rgb, (DIM, DIM, 3)))
+
+
+# ------------------------------------------------------------------ fp8 bricks
+def _e4m3_values():
+    vals = []
+    for code in range(0x7f):
+        e, m = code >> 3, code & 7
+        vals.append(m * 2.0 ** -9 if e == 0 else (1 + m / 8) * 2.0 ** (e - 7))
+    return np.array(vals)
+
+
+def _e4m3_round(x):
+    """Nearest E4M3 value to x >= 0 (ties to the even code), saturating at 448."""
+    table = _e4m3_values()
+    if x >= 448.0:
+        return 448.0
+    dd = np.abs(table - x)
+    best = np.flatnonzero(dd == dd.min())
+    code = best[0] if len(best) == 1 else [b for b in best if b % 2 == 0][0]
+    return table[code]
+
+
+def test_fp8_scale_is_brick_max_over_448(oracle):
+    size = 8  # 2 x 2 x 2 bricks of 4^3
+    rng = np.random.default_rng(4)
+    mags = [0.013, 0.9, 7.5, 0.0, 0.25, 3.0e-3, 1.0, 42.0]  # per-brick magnitudes, one brick all zero
+    lv = np.zeros((size, size, size, 4), np.float32)
+    for b, m in enumerate(mags):
+        bx, by, bz = b & 1, (b >> 1) & 1, b >> 2
+        blk = rng.uniform(-m, m, (4, 4, 4, 4)).astype(np.float32)
+        lv[bz * 4:bz * 4 + 4, by * 4:by * 4 + 4, bx * 4:bx * 4 + 4] = blk
+    got = oracle.fp8_quantize(lv)
+    for b, m in enumerate(mags):
+        bx, by, bz = b & 1, (b >> 1) & 1, b >> 2
+        blk = lv[bz * 4:bz * 4 + 4, by * 4:by * 4 + 4, bx * 4:bx * 4 + 4]
+        gblk = got[bz * 4:bz * 4 + 4, by * 4:by * 4 + 4, bx * 4:bx * 4 + 4]
+        mx = float(np.abs(blk).max())
+        if mx == 0:
+            assert np.all(gblk == 0)
+            continue
+        scale = np.float32(np.float32(mx) / np.float32(448.0))
+        for v, g in zip(blk.reshape(-1), gblk.reshape(-1)):
+            x = float(np.float32(abs(v)) / scale)
+            want = math.copysign(_e4m3_round(x) * float(scale), float(v))
+            assert abs(float(g) - want) <= 1e-7 * abs(want), (b, v, g, want)
+        # the brick's largest value uses the top code and comes back to itself
+        i = int(np.argmax(np.abs(blk)))
+        assert abs(abs(float(gblk.reshape(-1)[i])) - mx) <= 2e-7 * mx
+
+
+# ------------------------------------------------------------------ albedo chain
+def test_albedo_chain_rounds_the_byte_mean(oracle):
+    """Each albedo level is packUnorm4x8(0.125 * sum(child bytes / 255)): in
+    exact arithmetic round(sum(bytes) / 8), whose only ties are sums = 4 mod 8
+    (where float rounding may land either side)."""
+    dim, L = 16, 5
+    rng = np.random.default_rng(9)
+    grid = rng.integers(0, 256, size=(dim, dim, dim, 4)).astype(np.uint32)
+    g32 = grid[..., 0] | (grid[..., 1] << 8) | (grid[..., 2] << 16) | (grid[..., 3] << 24)
+    lv = oracle.mip_albedo(g32, L)
+    for l in range(1, L):
+        prev = _bytes(lv[l - 1])
+        s = prev.shape[0] // 2
+        sums = prev.reshape(s, 2, s, 2, s, 2, 4).sum(axis=(1, 3, 5))
+        got = _bytes(lv[l])
+        lo, tie = sums // 8, (sums % 8) == 4
+        want = lo + ((sums % 8) > 4)
+        ok = (got == want) | (tie & ((got == lo) | (got == lo + 1)))
+        assert ok.all(), (l, np.argwhere(~ok)[:4])
+        if l == 1:
+            assert tie.any() and (~tie).any()
+
+
+def test_albedo_chain_of_a_constant_grid_is_constant(oracle):
+    v = 0x80336699
+    lv = oracle.mip_albedo(np.full((8, 8, 8), v, np.uint32), 4)
+    for l in range(1, 4):
+        assert np.all(lv[l] == v)

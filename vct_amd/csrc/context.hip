@@ -39,6 +39,15 @@ struct vct_ctx {
     float *d_pos = nullptr, *d_nrm = nullptr, *d_tan = nullptr, *d_rough = nullptr;
     uint32_t *d_albedo = nullptr, *d_obj = nullptr;
     float *d_model = nullptr, *d_nmat = nullptr;
+    // pinned staging ring of the per-frame transform upload: slot k holds the
+    // model and normal matrices of one vct_set_transforms call until the copy
+    // out of it has run on the stream (xf_done[k]), so the host never waits for
+    // the frame in flight
+    static constexpr int kXfSlots = 3;
+    float *h_xf[kXfSlots] = {};
+    hipEvent_t xf_done[kXfSlots] = {};
+    bool xf_recorded[kXfSlots] = {};
+    int xf_next = 0;
     // lights
     std::vector<vct_light> lights;
     std::vector<float *> d_shadow;
@@ -47,6 +56,9 @@ struct vct_ctx {
     DevLight *d_lights = nullptr;
     // voxel grids
     uint32_t *d_grid_albedo = nullptr, *d_grid_normal = nullptr;
+    // the albedo (voxelColor) chain: levels 1..levels-1, RGBA8 linear x-fastest
+    uint32_t *d_albedo_mips = nullptr;
+    uint32_t *albedo_lv[VCT_MAX_LEVELS] = {};
     uint8_t *d_litmask = nullptr;
     uint32_t *d_partials = nullptr;  // per-workgroup statistics partials
     char *d_pyr = nullptr;
@@ -128,6 +140,17 @@ static void dfree(T *&p) {
     p = nullptr;
 }
 
+static void free_xf_ring(vct_ctx *c) {
+    for (int k = 0; k < vct_ctx::kXfSlots; ++k) {
+        if (c->xf_done[k]) (void)hipEventDestroy(c->xf_done[k]);
+        if (c->h_xf[k]) (void)hipHostFree(c->h_xf[k]);
+        c->xf_done[k] = nullptr;
+        c->h_xf[k] = nullptr;
+        c->xf_recorded[k] = false;
+    }
+    c->xf_next = 0;
+}
+
 static bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
 static int ilog2(int v) {
     int l = 0;
@@ -172,6 +195,8 @@ struct PassTimer {
 static void free_grids(vct_ctx *c) {
     dfree(c->d_grid_albedo);
     dfree(c->d_grid_normal);
+    dfree(c->d_albedo_mips);
+    for (auto &p : c->albedo_lv) p = nullptr;
     dfree(c->d_litmask);
     dfree(c->d_pyr);
     dfree(c->d_zero0);
@@ -190,6 +215,23 @@ static int alloc_grids(vct_ctx *c) {
     HIPCHK(c, hipMemsetAsync(c->d_grid_albedo, 0, nv * 4, c->stream));
     HIPCHK(c, hipMemsetAsync(c->d_grid_normal, 0, nv * 4, c->stream));
     HIPCHK(c, hipMemsetAsync(c->d_litmask, 0, nv, c->stream));
+    {
+        size_t words = 0;
+        for (int l = 1; l < c->desc.levels; ++l) {
+            const size_t sz = (size_t)std::max(1, dim >> l);
+            words += sz * sz * sz;
+        }
+        if (words) {
+            HIPCHK(c, dalloc(&c->d_albedo_mips, words * 4));
+            HIPCHK(c, hipMemsetAsync(c->d_albedo_mips, 0, words * 4, c->stream));
+            size_t off = 0;
+            for (int l = 1; l < c->desc.levels; ++l) {
+                const size_t sz = (size_t)std::max(1, dim >> l);
+                c->albedo_lv[l] = c->d_albedo_mips + off;
+                off += sz * sz * sz;
+            }
+        }
+    }
     PyrDesc &P = c->pyr;
     memset(&P, 0, sizeof(P));
     P.dim = dim;
@@ -340,6 +382,7 @@ void vct_destroy(vct_ctx *c) {
     dfree(c->d_obj);
     dfree(c->d_model);
     dfree(c->d_nmat);
+    free_xf_ring(c);
     for (auto &p : c->d_shadow) dfree(p);
     for (auto &p : c->d_shadowq) dfree(p);
     dfree(c->d_lights);
@@ -467,6 +510,7 @@ int vct_set_scene(vct_ctx *c, const vct_mesh *m) {
     dfree(c->d_rough);
     dfree(c->d_model);
     dfree(c->d_nmat);
+    free_xf_ring(c);  // the stream is idle: no copy reads the ring
     const size_t T = (size_t)m->num_tris;
     HIPCHK(c, dalloc(&c->d_pos, T * 9 * 4));
     HIPCHK(c, dalloc(&c->d_nrm, T * 9 * 4));
@@ -475,6 +519,10 @@ int vct_set_scene(vct_ctx *c, const vct_mesh *m) {
     HIPCHK(c, dalloc(&c->d_obj, T * 4));
     HIPCHK(c, dalloc(&c->d_model, (size_t)m->num_objects * 16 * 4));
     HIPCHK(c, dalloc(&c->d_nmat, (size_t)m->num_objects * 9 * 4));
+    for (int k = 0; k < vct_ctx::kXfSlots; ++k) {
+        HIPCHK(c, hipHostMalloc((void **)&c->h_xf[k], (size_t)m->num_objects * 25 * 4, hipHostMallocDefault));
+        HIPCHK(c, hipEventCreateWithFlags(&c->xf_done[k], hipEventDisableTiming));
+    }
     if (T) {
         HIPCHK(c, hipMemcpy(c->d_pos, m->positions, T * 9 * 4, hipMemcpyHostToDevice));
         HIPCHK(c, hipMemcpy(c->d_nrm, m->normals, T * 9 * 4, hipMemcpyHostToDevice));
@@ -528,12 +576,19 @@ static void normal_matrix(const float *m, float *out) {
 int vct_set_transforms(vct_ctx *c, const float *model, int n) {
     if (!c || !model) return fail(c, VCT_ERR_INVALID, "vct_set_transforms: NULL argument");
     if (n != c->nobj) return fail(c, VCT_ERR_INVALID, "vct_set_transforms: %d matrices for %d objects", n, c->nobj);
-    std::vector<float> nm((size_t)n * 9);
-    for (int o = 0; o < n; ++o) normal_matrix(model + (size_t)o * 16, nm.data() + (size_t)o * 9);
-    // synchronous copies: the host arrays are borrowed for the call only
-    HIPCHK(c, hipMemcpyAsync(c->d_model, model, (size_t)n * 64, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->d_nmat, nm.data(), (size_t)n * 36, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    // the host arrays are borrowed for the call only: they are copied into a
+    // pinned ring slot whose previous copy has completed, and the device copy
+    // runs from there in stream order (after the passes of earlier frames)
+    const int k = c->xf_next;
+    c->xf_next = (k + 1) % vct_ctx::kXfSlots;
+    if (c->xf_recorded[k]) HIPCHK(c, hipEventSynchronize(c->xf_done[k]));
+    float *h = c->h_xf[k];
+    memcpy(h, model, (size_t)n * 64);
+    for (int o = 0; o < n; ++o) normal_matrix(model + (size_t)o * 16, h + (size_t)n * 16 + (size_t)o * 9);
+    HIPCHK(c, hipMemcpyAsync(c->d_model, h, (size_t)n * 64, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_nmat, h + (size_t)n * 16, (size_t)n * 36, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipEventRecord(c->xf_done[k], c->stream));
+    c->xf_recorded[k] = true;
     c->have_transforms = 1;
     return VCT_OK;
 }
@@ -709,7 +764,10 @@ int vct_normalize_inject(vct_ctx *c, const vct_inject_params *p) {
                                          c->d_partials, c->stream));
     c->have_radiance = true;
     c->have_mips = false;
-    c->pyr_nonneg = true;
+    // a plain injection rewrites every texel of level 0 from unorm8 albedo (no
+    // negative values); the temporal filter keeps decay * the previous value of
+    // every unlit voxel, so a signed level 0 from vct_upload stays signed
+    if (!p->temporal_filter) c->pyr_nonneg = true;
     return VCT_OK;
 }
 
@@ -736,6 +794,9 @@ int vct_mip(vct_ctx *c, const vct_mip_params *p) {
     if (!c->have_radiance) return fail(c, VCT_ERR_STATE, "vct_mip: inject first");
     PassTimer pt(c, P_MIP);
     LAUNCHCHK(c, launch_mip(c->pyr, c->stream));
+    // the voxelColor chain, filtered by the same mipmap pass (Application.cpp:904-917)
+    if (p->albedo_chain)
+        LAUNCHCHK(c, launch_albedo_chain(c->d_grid_albedo, c->albedo_lv, c->desc.dim, c->desc.levels, c->stream));
     c->have_mips = true;
     return VCT_OK;
 }
@@ -866,7 +927,13 @@ size_t vct_buffer_bytes(const vct_ctx *c, int what, int level) {
     int l, f;
     switch (what) {
     case VCT_BUF_ALBEDO:
-    case VCT_BUF_NORMAL: return nv * 4;
+        if (level == 0) return nv * 4;
+        if (level < 0 || level >= c->desc.levels) return 0;
+        {
+            const size_t sz = (size_t)c->pyr.lv[level].size;
+            return sz * sz * sz * 4;
+        }
+    case VCT_BUF_NORMAL: return level == 0 ? nv * 4 : 0;
     case VCT_BUF_LITMASK: return nv;
     case VCT_BUF_RADIANCE:
         if (!decode_level(c, level, &l, &f)) return 0;
@@ -912,6 +979,10 @@ int vct_download(vct_ctx *c, int what, int level, void *host, size_t bytes) {
     switch (what) {
     case VCT_BUF_ALBEDO:
     case VCT_BUF_NORMAL: {  // bricked on the device, linear x-fastest for the caller
+        if (what == VCT_BUF_ALBEDO && level > 0) {  // the albedo chain is linear
+            src = c->albedo_lv[level];
+            break;
+        }
         int rc = ensure_stage(c, need);
         if (rc) return rc;
         LAUNCHCHK(c, launch_grid_convert(what == VCT_BUF_ALBEDO ? c->d_grid_albedo : c->d_grid_normal,
@@ -1000,6 +1071,7 @@ int vct_upload(vct_ctx *c, int what, int level, const void *host, size_t bytes) 
     }
     case VCT_BUF_ALBEDO:
     case VCT_BUF_NORMAL: {
+        if (level != 0) return fail(c, VCT_ERR_INVALID, "vct_upload: only level 0 of the albedo grid can be uploaded");
         int rc = ensure_stage(c, need);
         if (rc) return rc;
         HIPCHK(c, hipMemcpyAsync(c->d_stage, host, bytes, hipMemcpyHostToDevice, c->stream));

@@ -281,6 +281,105 @@ __global__ __launch_bounds__(64) void k_fp8_encode(char *pyr, float *scales, Lev
     *(uint32_t *)(pyr + L.base + (size_t)e * 4) = code;
 }
 
+// ---------------------------------------------------------------- albedo chain
+// The voxelColor chain of the same mipmap pass (Application.cpp:904-917): BOX2
+// on an RGBA8 image, so every level is stored as packUnorm4x8 of the mean of
+// its eight children read back as k/255 (filterRadiance.comp:14-65 with
+// rgba8 image bindings).  Same region scheme as k_mip_chain: a workgroup loads
+// an 8^3 region of the source (two 4^3 bricks per axis for the bricked level
+// 0, rows otherwise) and writes up to three levels from LDS.  Levels >= 1 are
+// linear, x fastest.
+struct AlbedoMipArgs {
+    const uint32_t *src;
+    int src_size, src_bricked;
+    uint32_t *out[3];
+    int nout;
+};
+
+__global__ __launch_bounds__(256) void k_albedo_chain(AlbedoMipArgs a) {
+    __shared__ float4 bufA[512];
+    __shared__ float4 bufB[64];
+    const int tid = threadIdx.x;
+    const int s = a.src_size;
+    const int R = s < 8 ? s : 8;
+    const int nreg = s / R;
+    const int b = (int)blockIdx.x;
+    const int ox = (b % nreg) * R, oy = ((b / nreg) % nreg) * R, oz = (b / (nreg * nreg)) * R;
+    const int R3 = R * R * R;
+    for (int i = tid; i < R3; i += 256) {
+        int x, y, z;
+        uint32_t w;
+        if (a.src_bricked) {  // brick-major walk: 8 bricks of 64 contiguous voxels
+            const int j = i >> 6, v = i & 63;
+            x = ox + (j & 1) * 4 + (v & 3);
+            y = oy + ((j >> 1) & 1) * 4 + ((v >> 2) & 3);
+            z = oz + (j >> 2) * 4 + (v >> 4);
+            w = a.src[grid_index(x, y, z, s >> 2)];
+        } else {
+            x = ox + i % R;
+            y = oy + (i / R) % R;
+            z = oz + i / (R * R);
+            w = a.src[((size_t)z * s + y) * s + x];
+        }
+        bufA[((z - oz) * R + (y - oy)) * R + (x - ox)] = make_float4(unorm8(w, 0), unorm8(w, 1), unorm8(w, 2),
+                                                                     unorm8(w, 3));
+    }
+    __syncthreads();
+    int cur = R;
+    float4 *in = bufA;
+    for (int k = 0; k < a.nout; ++k) {
+        const int d = cur >> 1;
+        if (d < 1) break;
+        float4 *out = (k & 1) ? bufA : bufB;
+        const int os = s >> (k + 1);  // output level size
+        const int gx = ox >> (k + 1), gy = oy >> (k + 1), gz = oz >> (k + 1);
+        for (int i = tid; i < d * d * d; i += 256) {
+            const int x = i % d, y = (i / d) % d, z = i / (d * d);
+            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+            // BOX2 offsets in filterRadiance.comp order: (0,0,0),(0,0,1),(0,1,0),(0,1,1),(1,0,0),...
+#pragma unroll
+            for (int o = 0; o < 8; ++o) {
+                const int dx = (o >> 2) & 1, dy = (o >> 1) & 1, dz = o & 1;
+                const float4 t = in[((2 * z + dz) * cur + (2 * y + dy)) * cur + (2 * x + dx)];
+                v.x = v.x + t.x;
+                v.y = v.y + t.y;
+                v.z = v.z + t.z;
+                v.w = v.w + t.w;
+            }
+            const uint32_t pw = pack_unorm4x8(v.x * 0.125f, v.y * 0.125f, v.z * 0.125f, v.w * 0.125f);
+            a.out[k][((size_t)(gz + z) * os + (gy + y)) * os + (gx + x)] = pw;
+            out[i] = make_float4(unorm8(pw, 0), unorm8(pw, 1), unorm8(pw, 2), unorm8(pw, 3));
+        }
+        __syncthreads();
+        in = out;
+        cur = d;
+    }
+}
+
+hipError_t launch_albedo_chain(const uint32_t *grid, uint32_t *const *levels, int dim, int nlevels, hipStream_t s) {
+    int src = 0;
+    while (src + 1 < nlevels) {
+        const int size = dim >> src;
+        const int R = size < 8 ? size : 8;
+        if (R < 2) break;
+        const int lg = R == 8 ? 3 : (R == 4 ? 2 : 1);
+        int nout = nlevels - 1 - src;
+        if (nout > lg) nout = lg;
+        AlbedoMipArgs a;
+        a.src = src == 0 ? grid : levels[src];
+        a.src_size = size;
+        a.src_bricked = src == 0 ? 1 : 0;
+        for (int k = 0; k < 3; ++k) a.out[k] = levels[k < nout ? src + 1 + k : src + 1];
+        a.nout = nout;
+        const int nreg = size / R;
+        hipLaunchKernelGGL(k_albedo_chain, dim3((unsigned)(nreg * nreg * nreg)), dim3(256), 0, s, a);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        src += nout;
+    }
+    return hipSuccess;
+}
+
 hipError_t launch_mip(const PyrDesc &pyr, hipStream_t s) {
     const int fp32 = pyr.storage == VCT_STORE_FP32 ? 1 : 0;
     int src = 0;

@@ -145,6 +145,7 @@ struct TraceLaunch {
 // 4 words) and summed by one small kernel: no contended same-address atomics.
 constexpr int kStatPartials = 2048;
 
+
 // ---- launch wrappers (return hipError_t of the last launch) ----------------
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
 hipError_t launch_fill_u64(uint64_t *p, uint64_t v, size_t n, hipStream_t s);
@@ -194,6 +195,8 @@ hipError_t launch_fill_holes(const PyrDesc &pyr, void *tmp, hipStream_t s);
 
 // mip.hip
 hipError_t launch_mip(const PyrDesc &pyr, hipStream_t s);
+// the albedo chain: levels[1 .. nlevels) (linear RGBA8) from the bricked grid
+hipError_t launch_albedo_chain(const uint32_t *grid, uint32_t *const *levels, int dim, int nlevels, hipStream_t s);
 
 // trace.hip
 hipError_t launch_trace(const PyrDesc &pyr, const TraceLaunch &t, hipStream_t s);

@@ -702,7 +702,7 @@ int vct_app_frame_state(vct_app *a, int frame, float *model, vct_light *lights, 
     }
     if (mip) {
         mip->kernel_mode = 0;
-        mip->albedo_chain = 0;
+        mip->albedo_chain = 1;  // the mipmap pass filters both chains (Application.cpp:877-919)
     }
     if (cam) *cam = a->cam;
     if (trace) *trace = a->trace;
