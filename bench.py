@@ -9,12 +9,18 @@ re-voxelized, re-injected, re-filtered and re-traced.
 
 One JSON line.  `value` = ms per frame of the four metric passes (voxelize +
 normalize/inject + mip + trace) from HIP events recorded on the context
-stream around each pass; the shadow-map, G-buffer and final composition
-(direct light + indirect + post-process) passes each frame also needs are
-reported beside it (`passes`, `frame_ms_all_passes`).
+stream around each pass; the mip pass filters both chains (radiance and
+albedo), as the reference's mipmap timer brackets both.  `value_ref_boundaries`
+adds the composition to the trace, as the reference's render timer brackets
+the whole phong pass.  The shadow-map and G-buffer passes each frame also needs
+are reported beside it (`passes`, `frame_ms_all_passes`).
 `roofline` is for the dominant kernel (the cone trace): algorithmic bytes per
-launch / its average event-timed duration.  `cpu_baseline` times the oracle
-(the CPU restatement, one thread) on a bounded sample of the same frame.
+launch / its average event-timed duration, with the kernel's instruction-issue
+figures beside it.  `precision_sweep` runs the small scene with FP32, FP16 and
+FP8 storage (ms/frame, GB/s) and, in the CPU leg, scores each format's traced
+indirect light at 4096 seeded pixels against the FP64 oracle of the cone
+integral.  `cpu_baseline` times the oracle (the CPU restatement, OpenMP over
+the host cores it may use) on one full frame of the same workload.
 """
 from __future__ import annotations
 
@@ -29,13 +35,20 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 PEAK_HBM_GBS = 8000.0
-# HBM traffic of one k_trace launch on the default workload (scene 2, 256^3 fp16
-# 6-direction mips, 1920x1080): rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE in KiB
-# per dispatch (profiles/counters_trace_scalar_path.md), scaled by the factors a
-# 1 GiB device copy gives the same counters, x2 and x1
-# (profiles/counters_copy_calibration.md).  The read factor comes from a copy's
-# 128-byte requests, so for smaller requests the read bytes are an upper bound.
+# Profile-derived figures of one k_trace launch on the default workload (scene
+# 2, 256^3 fp16 6-direction mips, 1920x1080), copied from the committed profile
+# named in TRACE_PROFILE (they cannot be measured inside this process):
+#  - HBM traffic: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE in KiB per dispatch,
+#    scaled by the factors a 1 GiB device copy gives the same counters, x2 and
+#    x1 (profiles/counters_copy_calibration.md).  The read factor comes from a
+#    copy's 128-byte requests, so for smaller requests the read bytes are an
+#    upper bound.
+#  - instruction issue: SQ_INSTS_VALU per wavefront and SQ_ACTIVE_INST_VALU x 4
+#    / (1024 SIMDs x kernel cycles) -- the fraction of SIMD cycles issuing VALU,
+#    the ceiling this kernel actually runs against.
+TRACE_PROFILE = "profiles/counters_trace_scalar_path.md"
 TRACE_FETCH_KIB, TRACE_WRITE_KIB = 68323, 64802
+TRACE_VALU_PER_WAVE, TRACE_VALU_BUSY = 23550, 0.95
 METRIC = "ms/frame voxelize+inject+mip+trace @256^3,1080p; GB/s vs peak; learned vs plain"
 
 
@@ -54,58 +67,152 @@ def algorithmic_bytes(cfg, ntri: int, nlights_sm: int, sm: int) -> dict:
     upper = sum(s ** 3 for s in sizes[1:])
     pyramid = nv * bpv + faces * upper * bpv
     npix = cfg.width * cfg.height
-    out_px = 16 * (2 if cfg.reflections else 1)
+    albedo_upper = 4 * upper
     return {
         # triangles (positions + normals + albedo + object id) read once, albedo + normal grids written once
         "voxelize": ntri * (72 + 4 + 4) + 2 * 4 * nv,
-        # albedo read, lit mask written + read, shadow maps read, radiance level 0 written
-        "radiance": 4 * nv + 2 * nv + nlights_sm * sm * sm * 4 + bpv * nv,
-        # level 0 read, every level >= 1 (all faces) written
-        "mip": bpv * nv + faces * upper * bpv,
-        # G-buffer read, outputs written, the whole pyramid read once
-        "trace": npix * 64 + npix * out_px + pyramid,
+        # albedo read, lit mask (1 bit per voxel) written + read, shadow maps read, radiance level 0 written
+        "radiance": 4 * nv + nv // 8 * 2 + nlights_sm * sm * sm * 4 + bpv * nv,
+        # radiance level 0 read, every level >= 1 (all faces) written; the albedo
+        # chain: the RGBA8 grid read, its levels >= 1 written
+        "mip": bpv * nv + faces * upper * bpv + 4 * nv + albedo_upper,
+        "mip_radiance": bpv * nv + faces * upper * bpv,
+        # the plan's G-buffer (32 B/px) read, 8 B/px written (rgba16f), the whole pyramid read once
+        "trace": npix * 32 + npix * 8 + pyramid,
     }
 
 
-def cpu_baseline(app, frame: int, pixel_stride: int = 64) -> dict:
-    """Times the oracle (single thread) on one frame: shadow maps, voxelize,
-    normalize+inject and the mip chain in full, the trace on every
-    `pixel_stride`-th G-buffer pixel; reports the frame's
-    voxelize+inject+mip+trace time with the trace scaled to all pixels."""
-    os.environ["OMP_NUM_THREADS"] = "1"
+def host_cores() -> int:
+    """The host threads this process may use: its CPU affinity, capped by
+    OMP_NUM_THREADS when that is set (the GPU box sets it to its CPU share)."""
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
+
+
+def cpu_baseline(app, frame: int, cores: int, extent) -> dict:
+    """Times the oracle (the C restatement, OpenMP over `cores` threads) on one
+    full frame of the workload: voxelize, normalize+inject, the mip chains
+    (radiance, its six faces when anisotropic, and albedo) and the cone trace
+    of every G-buffer pixel.  The shadow maps and the G-buffer are rendered
+    first and not timed (they are not metric passes)."""
     import numpy as np
 
     from oracle import binding as ob
 
     cfg = app.cfg
     mesh, fs = app.mesh(), app.frame_state(frame)
-    g = app.ctx.grid()
+    center, vmin, vmax = extent
     sc = ob.SceneArrays(mesh, fs["model"])
     L = fs["lights"]
     sms = [ob.shadowmap(sc, l) if l.shadow_caster else None for l in L]
+    gb = ob.gbuffer(sc, fs["cam"]).reshape(-1, 16)
     t0 = time.perf_counter()
     alb, _, _ = ob.voxelize(sc, L, sms, fs["vox"], cfg.dim)
     t1 = time.perf_counter()
-    _, rad0, _, _ = ob.normalize_inject(alb, cfg.dim, tuple(g.center), tuple(g.vmin), tuple(g.vmax), L, sms)
+    alb2, rad0, _, _ = ob.normalize_inject(alb, cfg.dim, center, vmin, vmax, L, sms)
     t2 = time.perf_counter()
     levels = ob.mip(rad0, cfg.levels)
     faces = ob.mip_aniso(rad0, cfg.levels) if cfg.anisotropic else None
+    ob.mip_albedo(alb2, cfg.levels)
     t3 = time.perf_counter()
-    gb = ob.gbuffer(sc, fs["cam"]).reshape(-1, 16)
-    sample = np.ascontiguousarray(gb[::pixel_stride])
-    tp = ob.trace_params(fs["trace"], cfg.dim, cfg.levels, tuple(g.center), tuple(g.vmin), tuple(g.vmax),
-                         mesh["rough"] is not None, cfg.anisotropic)
+    tp = ob.trace_params(fs["trace"], cfg.dim, cfg.levels, center, vmin, vmax, mesh["rough"] is not None,
+                         cfg.anisotropic)
     t4 = time.perf_counter()
-    ob.trace(sample, levels, tp, aniso_faces=faces)
+    ob.trace(np.ascontiguousarray(gb), levels, tp, aniso_faces=faces)
     t5 = time.perf_counter()
-    trace_ms = (t5 - t4) * 1e3 * (gb.shape[0] / sample.shape[0])
-    total = (t1 - t0) * 1e3 + (t2 - t1) * 1e3 + (t3 - t2) * 1e3 + trace_ms
-    return {"value": round(total, 1), "unit": "ms/frame", "cores": 1, "kind": "port",
-            "sample": f"oracle (CPU restatement, 1 thread) on frame {frame}: voxelize, normalize+inject and the "
-                      f"mip chain in full, the cone trace on 1/{pixel_stride} of the {gb.shape[0]} pixels "
-                      f"scaled to the full frame (not llvmpipe: no GL context on this host)",
-            "passes_ms": {"voxelize": round((t1 - t0) * 1e3, 1), "radiance": round((t2 - t1) * 1e3, 1),
-                          "mipmap": round((t3 - t2) * 1e3, 1), "render": round(trace_ms, 1)}}
+    ms = [(t1 - t0) * 1e3, (t2 - t1) * 1e3, (t3 - t2) * 1e3, (t5 - t4) * 1e3]
+    return {"value": round(sum(ms), 1), "unit": "ms/frame", "cores": cores, "kind": "port",
+            "sample": f"oracle (CPU restatement in C, OpenMP, {cores} threads) on one full frame (frame {frame}): "
+                      f"voxelize, normalize+inject, the radiance and albedo mip chains and the cone trace of all "
+                      f"{gb.shape[0]} pixels (not llvmpipe: no GL context on this host)",
+            "passes_ms": {"voxelize": round(ms[0], 1), "radiance": round(ms[1], 1), "mipmap": round(ms[2], 1),
+                          "render": round(ms[3], 1)}}
+
+
+SWEEP_FORMATS = ((1, "fp32"), (0, "fp16"), (2, "fp8_e4m3_brick_scale"))
+
+
+def precision_sweep_gpu(steps: int, warmup: int, npix: int = 4096) -> dict:
+    """GPU leg of the storage-format sweep on the small scene (64^3 x 7
+    levels, 256x256, 6 diffuse cones): per format, ms/frame of the four metric
+    passes and the trace's algorithmic GB/s over `steps` frames, and the traced
+    indirect light of `npix` seeded covered pixels (scored by the CPU leg)."""
+    import numpy as np
+
+    from vct_amd import _abi as abi
+
+    res = {"workload": None, "pixels": npix, "formats": {}, "_samples": {}}
+    for storage, name in SWEEP_FORMATS:
+        app = vct_amd_app(1, storage=storage)
+        ctx, cfg = app.ctx, app.cfg
+        for f in range(warmup):
+            app.render(0)
+        ctx.sync()
+        ctx.pass_times()
+        for f in range(steps):
+            app.render(0)
+        ctx.sync()
+        pt = ctx.pass_times()
+        k = float(steps)
+        ms = {"voxelize_ms": pt.voxelize_ms / k, "radiance_ms": pt.radiance_ms / k, "mipmap_ms": pt.mipmap_ms / k,
+              "render_ms": pt.render_ms / k}
+        ntri, _, _ = app.sizes()
+        fs = app.frame_state(0)
+        n_sm = sum(1 for L in fs["lights"] if L.shadow_caster)
+        ab = algorithmic_bytes(cfg, ntri, n_sm, cfg.shadow_size)
+        gb = ctx.download(abi.VCT_BUF_GBUFFER, 0, np.float32).reshape(-1, 16)
+        ind = ctx.download(abi.VCT_BUF_INDIRECT, 0, np.float32).reshape(-1, 4)
+        cov = np.flatnonzero(gb[:, 11].view(np.uint32) == 1)
+        pick = np.sort(np.random.default_rng(3).choice(cov, min(npix, len(cov)), replace=False))
+        res["workload"] = (f"scene 1: {ntri} tris, {cfg.dim}^3 x {cfg.levels} levels isotropic, "
+                           f"{cfg.width}x{cfg.height}, 6 diffuse cones")
+        res["formats"][name] = {"ms_per_frame": round(sum(ms.values()), 4),
+                                "passes": {k2: round(v, 4) for k2, v in ms.items()},
+                                "trace_gbs": round(ab["trace"] / (ms["render_ms"] * 1e-3) / 1e9, 1),
+                                "mip_gbs": round(ab["mip"] / (ms["mipmap_ms"] * 1e-3) / 1e9, 1)}
+        res["_samples"][name] = (np.ascontiguousarray(gb[pick]), np.ascontiguousarray(ind[pick]))
+        g = ctx.grid()
+        res["_extent"] = (tuple(g.center), tuple(g.vmin), tuple(g.vmax))
+        app.close()
+    return res
+
+
+def precision_sweep_score(sweep: dict) -> None:
+    """CPU leg: the FP64 oracle of the cone integral -- level 0 from the
+    oracle's own voxelization and injection in FP32 (no storage rounding), the
+    mip chain filtered in FP32, every cone traced with FP64 sampling weights
+    and FP64 accumulation (or_trace64) -- at the sampled pixels, against which
+    each format's GPU result is scored: mean and max absolute error per pixel
+    (rgb), and the mean relative error sum|d| / sum|ref|."""
+    import numpy as np
+
+    import vct_amd
+    from oracle import binding as ob
+
+    app = vct_amd.App(1, device=-1)  # the scene and its uniforms only
+    cfg, mesh, fs = app.cfg, app.mesh(), app.frame_state(0)
+    center, vmin, vmax = sweep.pop("_extent")
+    sc = ob.SceneArrays(mesh, fs["model"])
+    L = fs["lights"]
+    sms = [ob.shadowmap(sc, l) if l.shadow_caster else None for l in L]
+    alb, _, _ = ob.voxelize(sc, L, sms, fs["vox"], cfg.dim)
+    _, rad0, _, _ = ob.normalize_inject(alb, cfg.dim, center, vmin, vmax, L, sms, fp32=True)
+    levels = ob.mip(rad0, cfg.levels, fp32=True)
+    tp = ob.trace_params(fs["trace"], cfg.dim, cfg.levels, center, vmin, vmax, mesh["rough"] is not None)
+    app.close()
+    samples = sweep.pop("_samples")
+    for name, (gb, ind) in samples.items():
+        ref, _ = ob.trace(gb, levels, tp, fp64=True)
+        d = np.abs(ind[:, :3].astype(np.float64) - ref[:, :3])
+        sweep["formats"][name]["error_vs_fp64_oracle"] = {
+            "mean_abs": float(d.sum(1).mean() / 3), "max_abs": float(d.max()),
+            "mean_rel": float(d.sum() / max(np.abs(ref[:, :3]).sum(), 1e-30)),
+            "occlusion_max_abs": float(np.abs(ind[:, 3] - ref[:, 3]).max())}
+    sweep["reference"] = ("FP64 oracle of the cone integral: level 0 and the mip chain in FP32 without storage "
+                          "rounding, traced with FP64 weights and accumulation (oracle/or_trace.c or_trace64)")
 
 
 def learned_leg(args) -> dict:
@@ -242,7 +349,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--scene", type=int, default=2)
     ap.add_argument("--storage", type=int, default=-1, help="0 fp16, 1 fp32, 2 fp8 (levels >= 3)")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true",
+                    help="skip the CPU leg (the oracle's baseline timing and the precision sweep's scores)")
+    ap.add_argument("--no-sweep", action="store_true", help="skip the storage-format precision sweep")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-plain gather leg")
     ap.add_argument("--only-learned", action="store_true", help="run only the learned-vs-plain gather leg")
     ap.add_argument("--learned-lr-final", type=float, default=1e-3,
@@ -265,6 +374,9 @@ def main() -> int:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the pipeline runs on one GPU per context)", file=sys.stderr)
         return 2
+
+    cores = host_cores()
+    os.environ.setdefault("OMP_NUM_THREADS", str(cores))  # the oracle's OpenMP threads (CPU leg)
 
     import vct_amd
 
@@ -291,8 +403,13 @@ def main() -> int:
               "radiance_ms": pt.radiance_ms / k, "mipmap_ms": pt.mipmap_ms / k, "gbuffer_ms": pt.gbuffer_ms / k,
               "render_ms": pt.render_ms / k, "compose_ms": pt.compose_ms / k}
     metric_ms = passes["voxelize_ms"] + passes["radiance_ms"] + passes["mipmap_ms"] + passes["render_ms"]
+    # the reference's timers: renderTimer brackets the whole phong pass, whose
+    # cone gather and composition are vct_trace + vct_compose here
+    ref_ms = metric_ms + passes["compose_ms"]
     npix = cfg.width * cfg.height
     ncones = 6 + (1 if cfg.reflections else 0)
+    g = ctx.grid()
+    extent = (tuple(g.center), tuple(g.vmin), tuple(g.vmax))
     ab = algorithmic_bytes(cfg, ntri, n_sm, cfg.shadow_size)
     per_pass_gbs = {name: round(ab[name] / (passes[key] * 1e-3) / 1e9, 1)
                     for name, key in (("voxelize", "voxelize_ms"), ("radiance", "radiance_ms"),
@@ -308,6 +425,7 @@ def main() -> int:
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(wall_ms, 4),
+        "value_ref_boundaries": round(ref_ms, 4),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
@@ -326,20 +444,39 @@ def main() -> int:
         "roofline": {"kernel": "k_trace", "bound": "hbm", "achieved": round(trace_gbs, 1), "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": round(trace_gbs / PEAK_HBM_GBS, 4),
                      "traffic": (TRACE_FETCH_KIB * 2 + TRACE_WRITE_KIB) * 1024 if default_workload else None,
-                     "traffic_source": ("rocprofv3 --pmc FETCH_SIZE x2 + WRITE_SIZE x1 per k_trace dispatch, "
-                                        "profiles/counters_trace_scalar_path.md") if default_workload else None,
-                     "algorithmic_bytes": ab["trace"]},
+                     "traffic_source": (f"profile-derived, not measured in this run: rocprofv3 --pmc FETCH_SIZE x2 + "
+                                        f"WRITE_SIZE x1 per k_trace dispatch, {TRACE_PROFILE}")
+                     if default_workload else None,
+                     "algorithmic_bytes": ab["trace"],
+                     "launch_ms": round(passes["render_ms"], 4),
+                     "issue": {"bound": "valu", "valu_busy_frac": TRACE_VALU_BUSY if default_workload else None,
+                               "valu_insts_per_wave": TRACE_VALU_PER_WAVE if default_workload else None,
+                               "source": f"profile-derived, not measured in this run: {TRACE_PROFILE}"}},
     }
     app.close()
+    print("bench: metric frames done", file=sys.stderr, flush=True)
+    sweep = None
+    if not args.no_sweep:
+        sweep = precision_sweep_gpu(args.steps, args.warmup)
+        print("bench: precision sweep (GPU) done", file=sys.stderr, flush=True)
     if not args.no_learned:
         out["learned_vs_plain"] = learned_leg(args)
+        print("bench: learned leg done", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
-        app = vct_amd_app(args.scene, storage=args.storage)
+        # the CPU leg: the oracle as the baseline, and as the checker of the sweep
+        app = vct_amd.App(args.scene, device=-1, storage=args.storage)  # the scene and its uniforms only
         try:
-            out["cpu_baseline"] = cpu_baseline(app, args.warmup + args.steps)
+            out["cpu_baseline"] = cpu_baseline(app, args.warmup + args.steps, cores, extent)
         except Exception as e:  # the baseline is reported, never required
             out["cpu_baseline"] = {"value": None, "error": repr(e)}
         app.close()
+        if sweep is not None:
+            precision_sweep_score(sweep)
+        print("bench: CPU leg done", file=sys.stderr, flush=True)
+    if sweep is not None:
+        sweep.pop("_samples", None)
+        sweep.pop("_extent", None)
+        out["precision_sweep"] = sweep
     print(json.dumps(out), flush=True)
     return 0
 

@@ -228,6 +228,16 @@ void or_shadowmap(const or_scene *sc, const or_light *L, float *depth) {
 
 /* ------------------------------------------------------------------------ */
 /* voxelize (GS path)                                                         */
+/* The voxelize loop runs over triangles on all host threads (OpenMP); the
+ * stores are the order-independent imageAtomicMax / imageAtomicOr of the
+ * shader, done here with compare-and-swap, so the result does not depend on
+ * the thread count or schedule. */
+static void atomic_max_u32(uint32_t *p, uint32_t v) {
+    uint32_t cur = __atomic_load_n(p, __ATOMIC_RELAXED);
+    while (v > cur && !__atomic_compare_exchange_n(p, &cur, v, 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {
+    }
+}
+
 static void light_fragment(const or_light *lights, int nlights, const float *const *shadowmaps,
                            const float color[3], const float nrm_interp[3], const float wpos[3],
                            float out[3]) {
@@ -274,6 +284,8 @@ static void voxelize_impl(const or_scene *sc, const or_light *lights, int nlight
                           uint32_t *occ) {
     int dim = p->dim;
     float half = (float)dim * 0.5f;
+    uint32_t nfrag = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : nfrag)
     for (int t = 0; t < sc->ntri; ++t) {
         const float *Nm = sc->nmat + (size_t)sc->obj[t] * 9;
         float W[3][3], Nw[3][3];
@@ -335,7 +347,7 @@ static void voxelize_impl(const or_scene *sc, const or_light *lights, int nlight
                 float color[3] = {col[0], col[1], col[2]};
                 float nn[3], nenc[4];
                 if (!occ) {
-                    stats->total_voxel_fragments += 1; /* voxelize.frag:195 */
+                    nfrag += 1; /* voxelize.frag:195 */
                     nrm3(nr, nn);
                     for (int k = 0; k < 3; ++k) nenc[k] = (nn[k] + 1.0f) * 0.5f;
                     nenc[3] = 1.0f;
@@ -366,15 +378,16 @@ static void voxelize_impl(const or_scene *sc, const or_light *lights, int nlight
                     continue; /* out-of-range image atomics are no-ops */
                 size_t vi = ((size_t)idx[2] * dim + idx[1]) * dim + idx[0];
                 if (occ) {
-                    occ[vi] |= 1u;
+                    __atomic_fetch_or(&occ[vi], 1u, __ATOMIC_RELAXED);
                     continue;
                 }
                 float c4[4] = {color[0], color[1], color[2], 1.0f};
                 uint32_t pc = or_pack_unorm4x8(c4), pn = or_pack_unorm4x8(nenc);
-                if (pc > albedo[vi]) albedo[vi] = pc;
-                if (pn > normal[vi]) normal[vi] = pn;
+                atomic_max_u32(&albedo[vi], pc);
+                atomic_max_u32(&normal[vi], pn);
             }
     }
+    if (stats) stats->total_voxel_fragments += nfrag;
 }
 
 void or_voxelize(const or_scene *sc, const or_light *lights, int nlights, const float *const *shadowmaps,
@@ -494,6 +507,7 @@ static void inject_litmask(int dim, const float center[3], const float vmin[3], 
         const or_light *L = &lights[li];
         if (!((light_mask >> li) & 1u) || !L->shadow_caster || !shadowmaps || !shadowmaps[li]) continue;
         int N = (int)L->shadow_size;
+#pragma omp parallel for schedule(static)
         for (int j = 0; j < N; ++j)
             for (int i = 0; i < N; ++i) {
                 float tc[2] = {(float)i / (float)N, (float)j / (float)N};
@@ -513,7 +527,7 @@ static void inject_litmask(int dim, const float center[3], const float vmin[3], 
                 if (idx[0] < 0 || idx[1] < 0 || idx[2] < 0 || idx[0] >= dim || idx[1] >= dim || idx[2] >= dim)
                     continue;
                 size_t vi = ((size_t)idx[2] * dim + idx[1]) * dim + idx[0];
-                litmask[vi] = 1;
+                __atomic_store_n(&litmask[vi], (uint8_t)1, __ATOMIC_RELAXED); /* every writer stores 1 */
             }
     }
 }
@@ -526,19 +540,21 @@ void or_normalize_inject(uint32_t *albedo, int dim, const float center[3], const
     inject_litmask(dim, center, vmin, vmax, lights, nlights, shadowmaps, light_mask, warpmap, litmask);
     /* transferVoxels.comp:37-62 (+ normalizeVoxels.comp:101-105 for rgba16f
      * radiance) and injectRadiance.comp:186,212 per voxel */
+    uint32_t uniq = 0, maxf = 0, nlit = 0;
+#pragma omp parallel for schedule(static) reduction(+ : uniq, nlit) reduction(max : maxf)
     for (size_t vi = 0; vi < nv; ++vi) {
         uint32_t a32 = albedo[vi];
         float c[4];
         or_unpack_unorm4x8(a32, c);
         float *r = rad0 + vi * 4;
         if (c[3] > 0.0f) {
-            stats->unique_voxels += 1;
+            uniq += 1;
             uint32_t mf = (uint32_t)(255.0f * c[3]);
-            if (mf > stats->max_fragments_per_voxel) stats->max_fragments_per_voxel = mf;
+            if (mf > maxf) maxf = mf;
             if (opacity > 0.0f) c[3] = opacity;
             albedo[vi] = or_pack_unorm4x8(c);
             int lit = litmask[vi] != 0;
-            if (lit) stats->lit_voxels += 1;
+            if (lit) nlit += 1;
             r[0] = lit ? c[0] : 0.0f;
             r[1] = lit ? c[1] : 0.0f;
             r[2] = lit ? c[2] : 0.0f;
@@ -549,6 +565,9 @@ void or_normalize_inject(uint32_t *albedo, int dim, const float center[3], const
         if (!storage_fp32)
             for (int k = 0; k < 4; ++k) r[k] = q16(r[k]);
     }
+    stats->unique_voxels += uniq;
+    if (maxf > stats->max_fragments_per_voxel) stats->max_fragments_per_voxel = maxf;
+    stats->lit_voxels += nlit;
 }
 
 /* The temporalFilterRadiance path in the reference's own two steps: level 0
@@ -635,6 +654,7 @@ void or_mip(float **levels, int dim, int nlevels, int storage_fp32) {
         if (d < 1) break;
         const float *src = levels[l];
         float *dst = levels[l + 1];
+#pragma omp parallel for schedule(static)
         for (int z = 0; z < d; ++z)
             for (int y = 0; y < d; ++y)
                 for (int x = 0; x < d; ++x) {
@@ -664,6 +684,7 @@ void or_mip_albedo(uint32_t **levels, int dim, int nlevels) {
         if (d < 1) break;
         const uint32_t *src = levels[l];
         uint32_t *dst = levels[l + 1];
+#pragma omp parallel for schedule(static)
         for (int z = 0; z < d; ++z)
             for (int y = 0; y < d; ++y)
                 for (int x = 0; x < d; ++x) {
@@ -694,6 +715,7 @@ void or_mip_aniso(const float *level0, float ***aniso, int dim, int nlevels, int
             const float *src = (l == 0) ? level0 : aniso[l][f];
             float *dst = aniso[l + 1][f];
             int ax = f >> 1, neg = f & 1;
+#pragma omp parallel for schedule(static)
             for (int z = 0; z < d; ++z)
                 for (int y = 0; y < d; ++y)
                     for (int x = 0; x < d; ++x) {

@@ -313,3 +313,24 @@ def test_gbuffer_quad_positions_are_ray_plane_hits(oracle):
     cam.view[:] = list(oracle.lookat((0, 0, -4), (0, 0, 0), (0, 1, 0)))
     g = oracle.gbuffer(sc, cam)
     assert np.all(g[..., 11].view(np.uint32) == 0)
+
+
+def test_parallel_oracle_is_thread_count_independent(oracle, scene1):
+    """The oracle's voxelize, inject and mip loops run on OpenMP threads; the
+    voxel stores are atomic max/or, so one thread and many give identical bits."""
+    import ctypes
+    gomp = ctypes.CDLL("libgomp.so.1")
+    s = scene1
+    runs = []
+    for n in (1, 4):
+        gomp.omp_set_num_threads(n)
+        alb, nrm, st = oracle.voxelize(s["sc"], s["L"], [s["sm"]], s["fs"]["vox"], 64)
+        a2, rad0, lm, st2 = oracle.normalize_inject(alb, 64, *s["ext"], s["L"], [s["sm"]])
+        lv = oracle.mip(rad0, 7)
+        runs.append((alb, nrm, st.total_voxel_fragments, a2, rad0, lm, st2.unique_voxels, st2.lit_voxels, lv))
+    gomp.omp_set_num_threads(max(1, __import__("os").cpu_count() or 1))
+    a, b = runs
+    for x, y in zip(a[:8], b[:8]):
+        assert np.array_equal(np.asarray(x), np.asarray(y))
+    for x, y in zip(a[8], b[8]):
+        assert np.array_equal(x.view(np.uint32), y.view(np.uint32))
