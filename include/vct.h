@@ -284,6 +284,16 @@ int vct_voxelize(vct_ctx *ctx, const vct_voxelize_params *p);
  * fused: lit-mask scatter over shadow texels, then one per-voxel pass that
  * normalizes, rewrites opacity and writes radiance level 0. */
 int vct_normalize_inject(vct_ctx *ctx, const vct_inject_params *p);
+/* vct_normalize_inject without any device allocation of its own.  No
+ * counterpart in the reference: it is for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops).  The same passes
+ * and results as vct_normalize_inject; its scratch (the per-voxel lit mask and
+ * the statistics partials) is a static device array of the library, one per
+ * GPU, 16 MiB + 32 KiB, so the grid's dim must be <= 256 (VCT_ERR_INVALID
+ * otherwise).  Every call on one GPU shares that array: calls that overlap on
+ * one GPU (contexts on different streams) race.  The lit mask of such a call
+ * stays in the scratch (VCT_BUF_LITMASK keeps the last vct_normalize_inject's). */
+int vct_normalize_inject_noalloc(vct_ctx *ctx, const vct_inject_params *p);
 /* The warp texture (src/Application.cpp:235-575): the scene voxelized into a
  * 32^3 occupancy grid with the projections and coverage mode of `vox`
  * (voxelize.frag:187-193), per-axis prefix sums, the weight table and the

@@ -135,6 +135,11 @@ class Context:
     def normalize_inject(self, p: abi.InjectParams):
         self.check(self.lib.vct_normalize_inject(self.h, C.byref(p)))
 
+    def normalize_inject_noalloc(self, p: abi.InjectParams):
+        """The same pass with its scratch in the library's static device array
+        (no allocation; grids up to 256^3)."""
+        self.check(self.lib.vct_normalize_inject_noalloc(self.h, C.byref(p)))
+
     def fill_holes(self):
         self.check(self.lib.vct_fill_holes(self.h))
 

@@ -155,6 +155,7 @@ VCT_FUNCS = [
     ("vct_shadowmaps", C.c_int, [C.c_void_p]),
     ("vct_voxelize", C.c_int, [C.c_void_p, C.POINTER(VoxelizeParams)]),
     ("vct_normalize_inject", C.c_int, [C.c_void_p, C.POINTER(InjectParams)]),
+    ("vct_normalize_inject_noalloc", C.c_int, [C.c_void_p, C.POINTER(InjectParams)]),
     ("vct_fill_holes", C.c_int, [C.c_void_p]),
     ("vct_mip", C.c_int, [C.c_void_p, C.POINTER(MipParams)]),
     ("vct_gbuffer", C.c_int, [C.c_void_p, C.POINTER(Camera)]),

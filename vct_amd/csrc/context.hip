@@ -365,6 +365,12 @@ int vct_create(vct_ctx **out, const vct_grid_desc *desc) {
         vct_destroy(c);
         return rc;
     }
+    // this device's address of the static scratch of vct_normalize_inject_noalloc
+    // (looked up once per device, with the device current)
+    if (noalloc_scratch_setup(d.device) != hipSuccess) {
+        vct_destroy(c);
+        return fail(nullptr, VCT_ERR_HIP, "hipGetSymbolAddress of the no-alloc scratch failed");
+    }
     c->err = warn;
     *out = c;
     return VCT_OK;
@@ -767,6 +773,40 @@ int vct_normalize_inject(vct_ctx *c, const vct_inject_params *p) {
     // a plain injection rewrites every texel of level 0 from unorm8 albedo (no
     // negative values); the temporal filter keeps decay * the previous value of
     // every unlit voxel, so a signed level 0 from vct_upload stays signed
+    if (!p->temporal_filter) c->pyr_nonneg = true;
+    return VCT_OK;
+}
+
+int vct_normalize_inject_noalloc(vct_ctx *c, const vct_inject_params *p) {
+    if (!c || !p) return fail(c, VCT_ERR_INVALID, "vct_normalize_inject_noalloc: NULL argument");
+    if (c->desc.dim > kNoAllocMaxDim)
+        return fail(c, VCT_ERR_INVALID,
+                    "vct_normalize_inject_noalloc: voxel dim %d exceeds the static scratch (dim <= %d)",
+                    c->desc.dim, kNoAllocMaxDim);
+    if (!(p->voxel_set_opacity >= 0.0f && p->voxel_set_opacity <= 1.0f))
+        return fail(c, VCT_ERR_INVALID, "voxel_set_opacity %f out of [0, 1]", (double)p->voxel_set_opacity);
+    if (p->temporal_filter && !(p->temporal_decay >= 0.0f && p->temporal_decay <= 1.0f))
+        return fail(c, VCT_ERR_INVALID, "temporal_decay %f out of [0, 1]", (double)p->temporal_decay);
+    if (!c->have_voxels) return fail(c, VCT_ERR_STATE, "vct_normalize_inject_noalloc: voxelize first");
+    if (p->warp_texture && !c->have_warpmap)
+        return fail(c, VCT_ERR_STATE, "vct_normalize_inject_noalloc: warp_texture without vct_warpmap");
+    const NoAllocScratch sc = noalloc_scratch(c->desc.device);
+    if (!sc.litmask) return fail(c, VCT_ERR_STATE, "vct_normalize_inject_noalloc: scratch not set up");
+    PassTimer pt(c, P_RADIANCE);
+    Extent ext;
+    memcpy(ext.center, c->desc.center, 12);
+    memcpy(ext.vmin, c->desc.vmin, 12);
+    memcpy(ext.vmax, c->desc.vmax, 12);
+    // the same kernels as vct_normalize_inject, with the lit mask (cleared by a
+    // fill kernel inside the scatter launch) and the partials in the scratch
+    LAUNCHCHK(c, launch_inject_scatter(c->hlights.data(), c->d_lights, (int)c->lights.size(), p->light_mask,
+                                       c->desc.dim, ext, p->warp_texture ? c->d_warpmap : nullptr, sc.litmask,
+                                       c->stream));
+    LAUNCHCHK(c, launch_normalize_inject(c->d_grid_albedo, sc.litmask, c->desc.dim, p->voxel_set_opacity,
+                                         p->temporal_filter != 0, p->temporal_decay, c->pyr, c->d_stats,
+                                         sc.partials, c->stream));
+    c->have_radiance = true;
+    c->have_mips = false;
     if (!p->temporal_filter) c->pyr_nonneg = true;
     return VCT_OK;
 }

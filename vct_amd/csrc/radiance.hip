@@ -12,9 +12,48 @@
 // the injection together, writing radiance level 0 once in the storage format.
 #pragma clang fp contract(off)
 
+#include <mutex>
+
 #include "vct_device.h"
 
 namespace vctk {
+
+// ---------------------------------------------------------------- no-alloc scratch
+// The scratch of vct_normalize_inject_noalloc: the lit mask (one byte per voxel,
+// grids up to kNoAllocMaxDim^3) followed by the statistics partials.  Every GPU
+// the library runs on holds its own copy until the process ends; every call on
+// one GPU shares it, so calls that overlap on one GPU race.
+constexpr size_t kNoAllocLitBytes = (size_t)kNoAllocMaxDim * kNoAllocMaxDim * kNoAllocMaxDim;
+constexpr size_t kNoAllocScratchBytes = kNoAllocLitBytes + (size_t)kStatPartials * 4 * sizeof(uint32_t);
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
+constexpr int kNoAllocMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[kNoAllocMaxDevices];  // device address of `scratch` on each device
+
+// Called by vct_create with `device` current: looks up this device's address of
+// the scratch array once (the array's host-side name is not a device address).
+hipError_t noalloc_scratch_setup(int device) {
+    if (device < 0 || device >= kNoAllocMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return hipSuccess;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = ptr;
+    return hipSuccess;
+}
+
+NoAllocScratch noalloc_scratch(int device) {
+    NoAllocScratch s{};
+    if (device < 0 || device >= kNoAllocMaxDevices) return s;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    uint8_t *base = (uint8_t *)g_scratch_ptr[device];
+    if (!base) return s;
+    s.litmask = base;
+    s.partials = (uint32_t *)(base + kNoAllocLitBytes);
+    return s;
+}
 
 struct InjectLight {
     float ls_inverse[16];

@@ -145,6 +145,15 @@ struct TraceLaunch {
 // 4 words) and summed by one small kernel: no contended same-address atomics.
 constexpr int kStatPartials = 2048;
 
+// radiance.hip: the static scratch of vct_normalize_inject_noalloc (lit mask of
+// grids up to kNoAllocMaxDim^3 + statistics partials), one copy per device.
+constexpr int kNoAllocMaxDim = 256;
+struct NoAllocScratch {
+    uint8_t *litmask;    // kNoAllocMaxDim^3 bytes
+    uint32_t *partials;  // kStatPartials * 4 words
+};
+hipError_t noalloc_scratch_setup(int device);  // with `device` current
+NoAllocScratch noalloc_scratch(int device);    // nulls before the setup
 
 // ---- launch wrappers (return hipError_t of the last launch) ----------------
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
