@@ -122,13 +122,26 @@ typedef struct vct_light {
                                injectRadiance.comp (src/Application.cpp:804-805)   */
 } vct_light;
 
+/* How fragments accumulate into a voxel (vct_voxelize_params.atomic_max). */
+#define VCT_ACCUM_AVG 0 /* voxelizeAtomicMax off, the reference's RGBA8 build:
+                           imageAtomicRGBA8Avg, a compare-and-swap running
+                           average with the fragment count in the alpha byte
+                           (voxelize.frag:111-139); the colour depends on the
+                           order in which fragments land, the count does not */
+#define VCT_ACCUM_MAX 1 /* voxelizeAtomicMax (default): atomicMax of the packed
+                           RGBA8 word, order independent (voxelize.frag:271-273) */
+#define VCT_ACCUM_SUM 2 /* the fp16 imageAtomicAdd of (c, 1) (voxelize.frag:
+                           257-260, an NV extension), here exact 32-bit fixed
+                           point sums, order independent; vct_normalize_voxels
+                           turns them into the averages                        */
+
 /* Voxelize pass uniforms (src/Application.cpp:689-733). */
 typedef struct vct_voxelize_params {
     float mvp_x[16], mvp_y[16], mvp_z[16]; /* dominant-axis projections            */
     int axis_override;      /* -1 = dominant axis, 0/1/2 forces x/y/z             */
     int lighting;           /* voxelizeLighting                                    */
-    int atomic_max;         /* voxelizeAtomicMax: must be 1 (the order-independent
-                               mode); the CAS running average is order dependent  */
+    int atomic_max;         /* voxelizeAtomicMax: VCT_ACCUM_MAX (1, the default),
+                               VCT_ACCUM_AVG (0) or VCT_ACCUM_SUM (2)             */
     int msaa;               /* 1: 4x MSAA coverage (ConservativeRasterizeMode MSAA)
                                0: single sample at the pixel centre (OFF)          */
     float eye[3];
@@ -284,6 +297,14 @@ int vct_voxelize(vct_ctx *ctx, const vct_voxelize_params *p);
  * fused: lit-mask scatter over shadow texels, then one per-voxel pass that
  * normalizes, rewrites opacity and writes radiance level 0. */
 int vct_normalize_inject(vct_ctx *ctx, const vct_inject_params *p);
+/* normalizeVoxels.comp (shaders/normalizeVoxels.comp:19-42), which the
+ * reference links but never dispatches: after a VCT_ACCUM_SUM voxelization,
+ * every voxel with fragments gets albedo = packUnorm4x8(sum / count, alpha 1
+ * or voxel_set_opacity when > 0), normal = packUnorm4x8(normal sum / count),
+ * radiance level 0 = (0, 0, 0, alpha); uniqueVoxels and maxFragmentsPerVoxel
+ * (the largest count) are updated.  VCT_ERR_STATE unless the last
+ * vct_voxelize accumulated sums. */
+int vct_normalize_voxels(vct_ctx *ctx, float voxel_set_opacity);
 /* vct_normalize_inject without any device allocation of its own.  No
  * counterpart in the reference: it is for callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops).  The same passes

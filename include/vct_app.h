@@ -28,9 +28,16 @@ typedef struct vct_app vct_app;
  *  scene 2: 2000 boxes + 64 icospheres (subdivision 4), 351,680 tris, moving
  *           rigidly every frame, 256^3 x 8 levels with 6-direction mips,
  *           1 directional + 4 point lights with 1024^2 shadow maps,
- *           1920x1080 camera, 6 diffuse + 1 specular cone. */
+ *           1920x1080 camera, 6 diffuse + 1 specular cone.
+ *  scene 5: scene 2's distributions at 4x the extent: 8000 boxes + 512
+ *           icospheres, 2,717,440 tris in [-4,4]^3 (positions x4, sizes x2.5:
+ *           the same volume fraction as scene 2), moving rigidly, 256^3 x 8
+ *           levels with 6-direction mips, FP8 E4M3 storage on levels >= 3,
+ *           the warp texture at high resolution 4.0 (voxelize, inject and
+ *           trace through it), 1 directional + 7 point lights with 1024^2
+ *           shadow maps, 3840x2160 camera, 6 diffuse + 1 specular cone. */
 typedef struct vct_app_config {
-    int scene;          /* 1 or 2                                               */
+    int scene;          /* 1, 2 or 5                                            */
     uint64_t seed;      /* 0 = scene index                                      */
     int dim, levels;    /* voxel grid                                           */
     int storage;        /* VCT_STORE_*, -1 = default (FP16)                     */
@@ -51,6 +58,9 @@ typedef struct vct_app_config {
     int warp_texture;    /* warpTexture (Application.h:89), 0: every frame builds the
                             warp texture (vct_warpmap) and voxelize, inject and
                             trace go through it                                */
+    float warp_high_resolution; /* warpTextureHighResolution (Application.h:87):
+                            2.0 (0 = default); 4.0, the overlay's maximum
+                            (Overlay.cpp:289), for scene 5                     */
 } vct_app_config;
 
 /* ---- OBJ/MTL assets: Mesh::loadMesh (src/Graphics/Mesh.cpp:42-266) ------------
@@ -95,6 +105,12 @@ int vct_app_frame_state(vct_app *app, int frame, float *model, vct_light *lights
  * proportional to their world-space area, uniform barycentrics, attributes
  * interpolated exactly as the G-buffer resolve interpolates them. */
 int vct_app_surface_points(vct_app *app, int frame, int n, uint64_t seed, vct_gbuffer_texel *out);
+
+/* Another seeded camera of the scene's camera distribution (the rule that
+ * placed the scene's own camera: an eye outside every object, a random yaw
+ * and a small pitch) at width x height.  The learned gather trains on the
+ * views of some seeds and is scored on the views of others. */
+int vct_app_camera(const vct_app *app, uint64_t seed, int width, int height, vct_camera *out);
 
 /* The composition uniforms of frame `frame` (Settings defaults of
  * src/Application.h:60-71: Cook-Torrance, diffuse, specular, indirect,

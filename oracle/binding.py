@@ -27,7 +27,8 @@ class Scene(C.Structure):
 
 class VoxParams(C.Structure):
     _fields_ = [("mvp_x", C.c_void_p), ("mvp_y", C.c_void_p), ("mvp_z", C.c_void_p), ("axis_override", C.c_int),
-                ("lighting", C.c_int), ("msaa", C.c_int), ("dim", C.c_int), ("warpmap", C.c_void_p)]
+                ("lighting", C.c_int), ("msaa", C.c_int), ("dim", C.c_int), ("warpmap", C.c_void_p),
+                ("accum", C.c_int), ("sums", C.c_void_p)]
 
 
 class WarpParams(C.Structure):
@@ -98,6 +99,7 @@ def lib() -> C.CDLL:
             "or_mip_aniso": (None, [vp, vp, C.c_int, C.c_int, C.c_int]),
             "or_fp8_quantize_level": (None, [vp, C.c_int]),
             "or_mip_albedo": (None, [vp, C.c_int, C.c_int]),
+            "or_normalize_voxels": (None, [vp, C.c_int, C.c_float, vp, vp, vp, C.POINTER(Stats)]),
             "or_gbuffer": (None, [C.POINTER(Scene), C.POINTER(Camera), vp]),
             "or_trace": (None, [vp, C.c_int, vp, vp, C.POINTER(TraceParams), vp, vp]),
             "or_trace64": (None, [vp, C.c_int, vp, vp, C.POINTER(TraceParams), vp, vp]),
@@ -214,14 +216,19 @@ def shadowmap(sc: SceneArrays, light) -> np.ndarray:
 
 
 def voxelize(sc: SceneArrays, lights, shadowmaps, vox, dim: int, msaa=None, lighting=None, axis_override=None,
-             warpmap=None):
+             warpmap=None, accum=1, sums=None):
+    """accum: 1 atomicMax (default), 0 the RGBA8 running average, 2 fixed-point
+    sums into `sums` (uint32 [dim^3][8], linear x-fastest, zeroed here)."""
     mx = np.array(list(vox.mvp_x), np.float32)
     my = np.array(list(vox.mvp_y), np.float32)
     mz = np.array(list(vox.mvp_z), np.float32)
     wm = None if warpmap is None else np.ascontiguousarray(warpmap, np.float32)
+    if accum == 2:
+        assert sums is not None and sums.dtype == np.uint32 and sums.size == dim ** 3 * 8 and sums.flags.c_contiguous
+        sums[...] = 0
     p = VoxParams(_p(mx), _p(my), _p(mz), vox.axis_override if axis_override is None else axis_override,
                   vox.lighting if lighting is None else lighting, vox.msaa if msaa is None else msaa, dim,
-                  None if wm is None else _p(wm))
+                  None if wm is None else _p(wm), int(accum), None if sums is None else _p(sums))
     la = _light_array(lights)
     sms = [np.ascontiguousarray(s, np.float32) if s is not None else None for s in shadowmaps]
     ptrs = (C.c_void_p * max(1, len(sms)))(*[(s.ctypes.data if s is not None else None) for s in sms])
@@ -258,6 +265,17 @@ def normalize_inject(albedo: np.ndarray, dim, center, vmin, vmax, lights, shadow
                                            C.cast(ptrs, C.c_void_p), light_mask, opacity, decay,
                                            1 if fp32 else 0, wp, _p(pv), _p(rad), _p(lm), C.byref(st))
     return alb.reshape(dim, dim, dim), rad.reshape(dim, dim, dim, 4), lm, st
+
+
+def normalize_voxels(sums: np.ndarray, dim: int, opacity: float = 0.5):
+    """normalizeVoxels.comp on fixed-point sums: (albedo, normal, rad0, stats)."""
+    alb = np.zeros(dim ** 3, np.uint32)
+    nrm = np.zeros(dim ** 3, np.uint32)
+    rad = np.zeros((dim ** 3, 4), np.float32)
+    st = Stats()
+    lib().or_normalize_voxels(_p(np.ascontiguousarray(sums, np.uint32)), dim, opacity, _p(alb), _p(nrm), _p(rad),
+                              C.byref(st))
+    return alb.reshape(dim, dim, dim), nrm.reshape(dim, dim, dim), rad.reshape(dim, dim, dim, 4), st
 
 
 def fill_holes(rad0: np.ndarray, fp32=False) -> np.ndarray:

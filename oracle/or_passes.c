@@ -189,6 +189,11 @@ void or_shadowmap(const or_scene *sc, const or_light *L, float *depth) {
     int N = (int)L->shadow_size;
     for (size_t i = 0; i < (size_t)N * N; ++i) depth[i] = 1.0f;
     float half = (float)N * 0.5f;
+    /* triangles on all host threads: depth z in [0, 1] (+0 for -0), so the
+     * order of the bit patterns is the order of the depths and the depth test
+     * is an atomic minimum on them -- independent of the thread schedule */
+    uint32_t *dbits = (uint32_t *)depth;
+#pragma omp parallel for schedule(dynamic, 256)
     for (int t = 0; t < sc->ntri; ++t) {
         float clip[3][4];
         int behind = 0;
@@ -220,8 +225,12 @@ void or_shadowmap(const or_scene *sc, const or_light *L, float *depth) {
                 float z = interp(l, zw[0], zw[1], zw[2]);
                 z = z + 0.0f; /* -0 -> +0 */
                 if (!(z >= 0.0f && z <= 1.0f)) continue;
-                float *d = &depth[(size_t)py * N + px];
-                if (z < *d) *d = z;
+                uint32_t zb;
+                memcpy(&zb, &z, 4);
+                uint32_t *d = &dbits[(size_t)py * N + px];
+                uint32_t cur = __atomic_load_n(d, __ATOMIC_RELAXED);
+                while (zb < cur && !__atomic_compare_exchange_n(d, &cur, zb, 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {
+                }
             }
     }
 }
@@ -237,6 +246,34 @@ static void atomic_max_u32(uint32_t *p, uint32_t v) {
     while (v > cur && !__atomic_compare_exchange_n(p, &cur, v, 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {
     }
 }
+
+/* imageAtomicRGBA8Avg (voxelize.frag:111-139, the live `#if 1` branch): the
+ * word's rgb bytes hold the running average of 255 c, its alpha byte the
+ * count; convVec4ToRGBA8 truncates each component to an integer and keeps its
+ * low byte.  Applied in fragment order (the oracle runs this mode on one
+ * thread). */
+static uint32_t rgba8_from_floats(float x, float y, float z, float w) {
+    return (((uint32_t)w & 0xffu) << 24) | (((uint32_t)z & 0xffu) << 16) | (((uint32_t)y & 0xffu) << 8) |
+           ((uint32_t)x & 0xffu);
+}
+static void rgba8_avg(uint32_t *p, const float c[3]) {
+    const float v[4] = {c[0] * 255.0f, c[1] * 255.0f, c[2] * 255.0f, 1.0f};
+    const uint32_t cur = *p;
+    if (cur == 0u) { /* the first compare-and-swap against 0 succeeds */
+        *p = rgba8_from_floats(v[0], v[1], v[2], v[3]);
+        return;
+    }
+    float r[4] = {(float)(cur & 0xffu), (float)((cur >> 8) & 0xffu), (float)((cur >> 16) & 0xffu),
+                  (float)(cur >> 24)};
+    for (int k = 0; k < 3; ++k) r[k] = r[k] * r[3];
+    float sm[4];
+    for (int k = 0; k < 4; ++k) sm[k] = r[k] + v[k];
+    for (int k = 0; k < 3; ++k) sm[k] = sm[k] / sm[3];
+    *p = rgba8_from_floats(sm[0], sm[1], sm[2], sm[3]);
+}
+
+/* the sum-accumulate mode's fixed point: round(65535 clamp(c, 0, 1)) */
+static uint32_t fix16(float c) { return (uint32_t)rintf(clampf(c, 0.0f, 1.0f) * 65535.0f); }
 
 static void light_fragment(const or_light *lights, int nlights, const float *const *shadowmaps,
                            const float color[3], const float nrm_interp[3], const float wpos[3],
@@ -285,7 +322,8 @@ static void voxelize_impl(const or_scene *sc, const or_light *lights, int nlight
     int dim = p->dim;
     float half = (float)dim * 0.5f;
     uint32_t nfrag = 0;
-#pragma omp parallel for schedule(dynamic, 64) reduction(+ : nfrag)
+    const int serial = !occ && p->accum == 0; /* the running average depends on fragment order */
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : nfrag) if (!serial)
     for (int t = 0; t < sc->ntri; ++t) {
         const float *Nm = sc->nmat + (size_t)sc->obj[t] * 9;
         float W[3][3], Nw[3][3];
@@ -379,6 +417,18 @@ static void voxelize_impl(const or_scene *sc, const or_light *lights, int nlight
                 size_t vi = ((size_t)idx[2] * dim + idx[1]) * dim + idx[0];
                 if (occ) {
                     __atomic_fetch_or(&occ[vi], 1u, __ATOMIC_RELAXED);
+                    continue;
+                }
+                if (p->accum == 0) {
+                    rgba8_avg(&albedo[vi], color);
+                    rgba8_avg(&normal[vi], nenc);
+                    continue;
+                }
+                if (p->accum == 2) {
+                    uint32_t *w = p->sums + vi * 8;
+                    const uint32_t add[8] = {fix16(color[0]), fix16(color[1]), fix16(color[2]), 1u,
+                                             fix16(nenc[0]), fix16(nenc[1]), fix16(nenc[2]), 1u};
+                    for (int k = 0; k < 8; ++k) __atomic_fetch_add(&w[k], add[k], __ATOMIC_RELAXED);
                     continue;
                 }
                 float c4[4] = {color[0], color[1], color[2], 1.0f};
@@ -671,6 +721,32 @@ void or_mip(float **levels, int dim, int nlevels, int storage_fp32) {
                     }
                 }
     }
+}
+
+void or_normalize_voxels(const uint32_t *sums, int dim, float opacity, uint32_t *albedo, uint32_t *normal,
+                         float *rad0, or_stats *stats) {
+    const size_t nv = (size_t)dim * dim * dim;
+    uint32_t uniq = 0, maxc = 0;
+    for (size_t v = 0; v < nv; ++v) {
+        const uint32_t *w = sums + v * 8;
+        float *r = rad0 + v * 4;
+        r[0] = r[1] = r[2] = r[3] = 0.0f;
+        if (w[3] > 0u) { /* normalizeVoxels.comp:26-35 */
+            uniq += 1;
+            if (w[3] > maxc) maxc = w[3];
+            const float den = 65535.0f * (float)w[3];
+            const float c[4] = {(float)w[0] / den, (float)w[1] / den, (float)w[2] / den, opacity > 0.0f ? opacity : 1.0f};
+            albedo[v] = or_pack_unorm4x8(c);
+            r[3] = c[3];
+        }
+        if (w[7] > 0u) { /* :38-41 */
+            const float den = 65535.0f * (float)w[7];
+            const float n[4] = {(float)w[4] / den, (float)w[5] / den, (float)w[6] / den, 1.0f};
+            normal[v] = or_pack_unorm4x8(n);
+        }
+    }
+    stats->unique_voxels += uniq;
+    if (maxc > stats->max_fragments_per_voxel) stats->max_fragments_per_voxel = maxc;
 }
 
 /* The voxelColor chain of the same mipmap pass (src/Application.cpp:904-917):

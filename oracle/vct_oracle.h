@@ -34,6 +34,10 @@
  *       uniform volume closed form alpha_n = 1-(1-a)^n; LINEAR_MIPMAP_LINEAR
  *       level blend and the cone's lambda schedule on pyramids whose levels
  *       (and anisotropic faces) are uniform with distinct values
+ *   or_voxelize accum 0 / 2, or_normalize_voxels            a quad whose voxels get
+ *       one fragment each (average = that fragment), two coincident quads of
+ *       known colours (running average and sum closed forms), the alpha byte
+ *       = fragment count
  *   voxelize lighting (inside or_voxelize)                  closed forms on a quad:
  *       directional N.(-d), point-light smoothstep band, shadow * (1 - PCF)
  *   or_fp8_quantize_level                                   scale = brick max / 448,
@@ -109,6 +113,11 @@ typedef struct or_voxelize_params {
     int axis_override, lighting, msaa;
     int dim;
     const float *warpmap; /* warpTexture: the 32^3 warp texture (or_warpmap), or NULL */
+    int accum;            /* VCT_ACCUM_*: 1 atomicMax, 0 the RGBA8 running average
+                             (fragments in triangle order, pixel rows in order),
+                             2 fixed-point sums into `sums`                       */
+    uint32_t *sums;       /* accum 2: 8 words per voxel (linear x-fastest), zeroed
+                             by the caller                                        */
 } or_voxelize_params;
 
 typedef struct or_stats {
@@ -177,6 +186,13 @@ void or_mip_aniso(const float *level0, float ***aniso, int dim, int nlevels, int
  * levels[0] is the RGBA8 grid (linear, x fastest), levels[l] (l >= 1) receive
  * packUnorm4x8 of the BOX2 mean of the unorm8 children. */
 void or_mip_albedo(uint32_t **levels, int dim, int nlevels);
+/* normalizeVoxels.comp:19-42 on the fixed-point sums of an accum-2
+ * voxelization: albedo/normal (linear, zeroed by the caller) receive
+ * packUnorm4x8 of the averages (colour alpha 1, or opacity when > 0), rad0
+ * (float4 per voxel) (0, 0, 0, alpha) where there are fragments, 0 elsewhere;
+ * stats->unique_voxels and max_fragments_per_voxel (the largest count). */
+void or_normalize_voxels(const uint32_t *sums, int dim, float opacity, uint32_t *albedo, uint32_t *normal,
+                         float *rad0, or_stats *stats);
 /* FP8 E4M3 + per-4^3-brick fp32 scale round trip of one level (values
  * replaced in place by their decoded storage). */
 void or_fp8_quantize_level(float *level, int size);

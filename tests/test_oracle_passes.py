@@ -316,14 +316,16 @@ def test_gbuffer_quad_positions_are_ray_plane_hits(oracle):
 
 
 def test_parallel_oracle_is_thread_count_independent(oracle, scene1):
-    """The oracle's voxelize, inject and mip loops run on OpenMP threads; the
-    voxel stores are atomic max/or, so one thread and many give identical bits."""
+    """The oracle's shadow map, voxelize, inject and mip loops run on OpenMP
+    threads; the depth and voxel stores are atomic min/max/or, so one thread
+    and many give identical bits."""
     import ctypes
     gomp = ctypes.CDLL("libgomp.so.1")
     s = scene1
     runs = []
     for n in (1, 4):
         gomp.omp_set_num_threads(n)
+        assert np.array_equal(oracle.shadowmap(s["sc"], s["L"][0]).view(np.uint32), s["sm"].view(np.uint32))
         alb, nrm, st = oracle.voxelize(s["sc"], s["L"], [s["sm"]], s["fs"]["vox"], 64)
         a2, rad0, lm, st2 = oracle.normalize_inject(alb, 64, *s["ext"], s["L"], [s["sm"]])
         lv = oracle.mip(rad0, 7)

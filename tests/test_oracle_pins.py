@@ -407,3 +407,84 @@ def test_albedo_chain_of_a_constant_grid_is_constant(oracle):
     lv = oracle.mip_albedo(np.full((8, 8, 8), v, np.uint32), 4)
     for l in range(1, 4):
         assert np.all(lv[l] == v)
+
+
+# ------------------------------------------------------------------ accumulation modes
+def _quad_at(z0, albedo):
+    q = np.array([[-0.5, -0.5, z0], [0.5, -0.5, z0], [0.5, 0.5, z0], [-0.5, 0.5, z0]], np.float32)
+    pos = np.stack([q[[0, 1, 2]], q[[0, 2, 3]]])
+    nrm = np.zeros_like(pos)
+    nrm[..., 2] = 1
+    return pos, nrm, np.array([albedo, albedo], np.uint32)
+
+
+def _two_quads(oracle, alb_a, alb_b):
+    """Two quads in the same voxel layer (z0 = 0.03 and 0.05, both inside
+    voxel z = 8 at 16^3): every voxel of the layer gets one fragment of each,
+    the first quad's first (triangle order) -- two of each on the diagonal the
+    quads' triangles share, whose pixels both triangles cover (4x MSAA)."""
+    pa, na, aa = _quad_at(0.03, alb_a)
+    pb, nb, ab = _quad_at(0.05, alb_b)
+    mesh = dict(pos=np.concatenate([pa, pb]), nrm=np.concatenate([na, nb]),
+                tan=np.array([[1, 0, 0]] * 4, np.float32), albedo=np.concatenate([aa, ab]),
+                obj=np.zeros(4, np.uint32), nobj=1)
+    return oracle.SceneArrays(mesh, np.eye(4, dtype=np.float32).reshape(1, 16))
+
+
+def _unlit(oracle):
+    v = _vox_params(oracle)
+    v.lighting = 0
+    return v
+
+
+def test_running_average_closed_form(oracle):
+    """imageAtomicRGBA8Avg: the first fragment stores (trunc(255 c), 1); the
+    second (trunc((255 c1 * 1 + 255 c2) / 2), 2) -- bytes of the first read
+    back as integers, every component truncated."""
+    ca, cb = (200, 17, 90), (61, 250, 131)
+    ua = ca[0] | (ca[1] << 8) | (ca[2] << 16) | (255 << 24)
+    ub = cb[0] | (cb[1] << 8) | (cb[2] << 16) | (255 << 24)
+    sc = _two_quads(oracle, ua, ub)
+    alb, nrm, st = oracle.voxelize(sc, [], [], _unlit(oracle), DIM, accum=0)
+    b = _bytes(alb)
+    diag = np.eye(8, dtype=bool)
+    first = [math.floor(np.float32(np.float32(k) / np.float32(255)) * np.float32(255)) for k in ca]
+    want = [math.floor((f + np.float32(np.float32(k) / np.float32(255)) * np.float32(255)) / 2)
+            for f, k in zip(first, cb)]
+    cnt = b[8, 4:12, 4:12, 3]
+    assert np.all(cnt[~diag] == 2) and np.all(cnt[diag] == 4)  # the alpha byte counts the fragments
+    for ch in range(3):
+        got = b[8, 4:12, 4:12, ch][~diag]
+        assert np.all(got == want[ch]), (ch, np.unique(got), want[ch])
+    # the normals (n + 1) / 2 = (0.5, 0.5, 1): 127, 127, 255 for both -> unchanged by the average
+    nb_ = _bytes(nrm)
+    assert np.all(nb_[8, 4:12, 4:12, 2] == 255) and np.array_equal(nb_[8, 4:12, 4:12, 3], cnt)
+    assert st.total_voxel_fragments == int(cnt.sum())
+
+
+def test_sum_mode_and_normalize_voxels_closed_form(oracle):
+    """Fixed-point sums round(65535 c) per fragment and the count; normalize:
+    packUnorm4x8(sum / (65535 count)) with alpha = voxelSetOpacity, radiance
+    (0, 0, 0, alpha), uniqueVoxels and maxFragmentsPerVoxel = the count."""
+    ca, cb = (200, 17, 90), (61, 250, 131)
+    ua = ca[0] | (ca[1] << 8) | (ca[2] << 16) | (255 << 24)
+    ub = cb[0] | (cb[1] << 8) | (cb[2] << 16) | (255 << 24)
+    sc = _two_quads(oracle, ua, ub)
+    sums = np.zeros((DIM ** 3, 8), np.uint32)
+    oracle.voxelize(sc, [], [], _unlit(oracle), DIM, accum=2, sums=sums)
+    s = sums.reshape(DIM, DIM, DIM, 8)
+    diag = np.eye(8, dtype=bool)
+    fx = [round(k / 255 * 65535) for k in ca], [round(k / 255 * 65535) for k in cb]
+    cnt = s[8, 4:12, 4:12, 3]
+    assert np.all(cnt[~diag] == 2) and np.all(cnt[diag] == 4)
+    for ch in range(3):
+        assert np.array_equal(s[8, 4:12, 4:12, ch], (fx[0][ch] + fx[1][ch]) * cnt // 2)
+    assert np.array_equal(s[8, 4:12, 4:12, 7], cnt) and s[..., 3].sum() == cnt.sum()
+    alb, nrm, rad0, st = oracle.normalize_voxels(sums, DIM, 0.5)
+    b = _bytes(alb)
+    for ch in range(3):
+        mean = (ca[ch] + cb[ch]) / 2  # exact in the fixed point up to 1/65535
+        assert np.all(np.abs(b[8, 4:12, 4:12, ch] - mean) <= 0.5 + 1e-9)
+    assert np.all(b[8, 4:12, 4:12, 3] == 128)
+    assert np.all(rad0[8, 4:12, 4:12, 3] == 0.5) and np.all(rad0[8, 4:12, 4:12, :3] == 0)
+    assert (rad0[..., 3] > 0).sum() == 64 and st.unique_voxels == 64 and st.max_fragments_per_voxel == 4

@@ -113,3 +113,53 @@ def test_scene2_motion_is_rigid_and_bounded(s2):
     ang = np.degrees(np.arccos(np.clip((np.trace(rel, axis1=1, axis2=2) - 1) / 2, -1, 1)))
     assert ang.max() <= 5.0 + 1e-3
     assert np.abs(M1[:, :3, 3] - M0[:, :3, 3]).max() <= 0.01 + 1e-6
+
+
+@pytest.fixture(scope="module")
+def s5():
+    return vct_amd.App(5, device=-1)
+
+
+def test_scene5_counts_and_config(s5):
+    """Scene 5: scene 2's distributions at 4x the extent -- 8000 boxes + 512
+    subdivision-4 icospheres (2,717,440 triangles) in [-4,4]^3 with sizes x2.5,
+    8 lights, a 3840x2160 camera, FP8 coarse levels and the warp texture at
+    4.0."""
+    nt, no, nl = s5.sizes()
+    assert nt == 8000 * 12 + 512 * 20 * 4 ** 4 == 2_717_440
+    assert no == 8512 and nl == 8
+    c = s5.cfg
+    assert (c.dim, c.levels, c.anisotropic, c.storage) == (256, 8, 1, 2)
+    assert (c.width, c.height, c.reflections) == (3840, 2160, 1)
+    assert c.warp_texture == 1 and c.warp_high_resolution == 4.0
+    m = s5.mesh()
+    sph = m["obj"] >= 8000
+    assert sph.sum() == 512 * 5120
+    r = np.linalg.norm(m["pos"][sph].astype(np.float64), axis=2)
+    rad = np.array([r[m["obj"][sph] == o].mean() for o in range(8000, 8512, 16)])
+    assert rad.min() >= 0.05 - 1e-5 and rad.max() <= 0.25 + 1e-5
+    box = ~sph
+    half = np.abs(m["pos"][box].astype(np.float64)).max(axis=1)  # per-triangle max |coordinate|
+    # half-extents 2.5 lognormal(-3, 0.5): median 2.5 e^-3 ~ 0.12
+    assert 0.06 < np.median(half) < 0.25
+    # the boxes fill about the same fraction of the extent as in scene 2: the
+    # mean box volume is 8 Z^3 E[e^X]^3 with X ~ N(-3, 0.5^2)
+    frac5 = 8 * 2.5 ** 3 * np.exp(3 * (-3.0 + 0.25 / 2)) * 8000 / 8 ** 3
+    frac2 = 8 * np.exp(3 * (-3.0 + 0.25 / 2)) * 2000 / 2 ** 3
+    assert abs(frac5 / frac2 - 1) < 0.05 and 0.2 < frac5 < 0.5
+    fs = s5.frame_state(0)
+    assert [L.type for L in fs["lights"]] == [1] + [0] * 7
+    assert all(L.shadow_caster and L.shadow_size == 1024 for L in fs["lights"])
+    assert all(np.abs(np.asarray(L.position[:])).max() <= 3.2 + 1e-5 for L in fs["lights"][1:])
+    assert (fs["cam"].width, fs["cam"].height) == (3840, 2160)
+    assert fs["vox"].warp_texture == 1 and fs["inj"].warp_texture == 1 and fs["trace"].warp_texture == 1
+
+
+def test_scene5_motion_is_bounded_by_the_extent(s5):
+    f0, f1, f80 = (s5.frame_state(f)["model"] for f in (0, 1, 80))
+    for m in (f0, f80):
+        M = m.reshape(-1, 4, 4).transpose(0, 2, 1).astype(np.float64)
+        assert np.all(np.abs(M[:, :3, 3]) <= 3.8 + 1e-5)
+    M0 = f0.reshape(-1, 4, 4).transpose(0, 2, 1).astype(np.float64)
+    M1 = f1.reshape(-1, 4, 4).transpose(0, 2, 1).astype(np.float64)
+    assert np.abs(M1[:, :3, 3] - M0[:, :3, 3]).max() <= 0.04 + 1e-5

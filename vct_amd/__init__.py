@@ -132,6 +132,10 @@ class Context:
     def voxelize(self, p: abi.VoxelizeParams):
         self.check(self.lib.vct_voxelize(self.h, C.byref(p)))
 
+    def normalize_voxels(self, opacity: float = 0.5):
+        """normalizeVoxels.comp after a VCT_ACCUM_SUM voxelization."""
+        self.check(self.lib.vct_normalize_voxels(self.h, opacity))
+
     def normalize_inject(self, p: abi.InjectParams):
         self.check(self.lib.vct_normalize_inject(self.h, C.byref(p)))
 
@@ -359,6 +363,14 @@ class App:
         out = np.zeros((n, 16), np.float32)
         self.check(self.lib.vct_app_surface_points(self.h, frame, n, seed, _ptr(out)))
         return out
+
+    def camera(self, seed: int, width: int | None = None, height: int | None = None) -> abi.Camera:
+        """A seeded camera of the scene's camera distribution."""
+        cam = abi.Camera()
+        w = width or self.cfg.width
+        h = height or self.cfg.height
+        self.check(self.lib.vct_app_camera(self.h, seed, w, h, C.byref(cam)))
+        return cam
 
     def render(self, frame: int = 0):
         self.check(self.lib.vct_app_render(self.h, frame))

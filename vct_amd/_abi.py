@@ -18,6 +18,7 @@ VCT_STORE_FP16 = 0
 VCT_STORE_FP32 = 1
 VCT_STORE_FP8 = 2
 
+VCT_ACCUM_AVG, VCT_ACCUM_MAX, VCT_ACCUM_SUM = 0, 1, 2
 VCT_LIGHT_POINT = 0
 VCT_LIGHT_DIRECTIONAL = 1
 
@@ -128,7 +129,8 @@ class AppConfig(C.Structure):
                 ("storage", C.c_int), ("anisotropic", C.c_int), ("width", C.c_int), ("height", C.c_int),
                 ("shadow_size", C.c_int), ("lighting", C.c_int), ("msaa", C.c_int), ("reflections", C.c_int),
                 ("fill_holes", C.c_int), ("device", C.c_int), ("temporal_filter", C.c_int),
-                ("temporal_decay", C.c_float), ("track_camera", C.c_int), ("warp_texture", C.c_int)]
+                ("temporal_decay", C.c_float), ("track_camera", C.c_int), ("warp_texture", C.c_int),
+                ("warp_high_resolution", C.c_float)]
 
 
 class LearnedParams(C.Structure):
@@ -155,6 +157,7 @@ VCT_FUNCS = [
     ("vct_shadowmaps", C.c_int, [C.c_void_p]),
     ("vct_voxelize", C.c_int, [C.c_void_p, C.POINTER(VoxelizeParams)]),
     ("vct_normalize_inject", C.c_int, [C.c_void_p, C.POINTER(InjectParams)]),
+    ("vct_normalize_voxels", C.c_int, [C.c_void_p, C.c_float]),
     ("vct_normalize_inject_noalloc", C.c_int, [C.c_void_p, C.POINTER(InjectParams)]),
     ("vct_fill_holes", C.c_int, [C.c_void_p]),
     ("vct_mip", C.c_int, [C.c_void_p, C.POINTER(MipParams)]),
@@ -198,6 +201,7 @@ APP_FUNCS = [
     ("vct_app_warpmap_params", C.c_int, [C.c_void_p, C.POINTER(WarpmapParams)]),
     ("vct_app_build_radiance", C.c_int, [C.c_void_p, C.c_int]),
     ("vct_app_surface_points", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_uint64, C.c_void_p]),
+    ("vct_app_camera", C.c_int, [C.c_void_p, C.c_uint64, C.c_int, C.c_int, C.POINTER(Camera)]),
     ("vct_math_ortho", None, [C.c_float] * 6 + [C.c_void_p]),
     ("vct_math_lookat", None, [C.c_void_p] * 4),
     ("vct_math_perspective", None, [C.c_float] * 4 + [C.c_void_p]),

@@ -56,6 +56,8 @@ struct vct_ctx {
     DevLight *d_lights = nullptr;
     // voxel grids
     uint32_t *d_grid_albedo = nullptr, *d_grid_normal = nullptr;
+    uint32_t *d_sums = nullptr;  // VCT_ACCUM_SUM: 8 words per voxel (allocated on first use)
+    bool have_sums = false;      // the last vct_voxelize accumulated sums
     // the albedo (voxelColor) chain: levels 1..levels-1, RGBA8 linear x-fastest
     uint32_t *d_albedo_mips = nullptr;
     uint32_t *albedo_lv[VCT_MAX_LEVELS] = {};
@@ -195,6 +197,8 @@ struct PassTimer {
 static void free_grids(vct_ctx *c) {
     dfree(c->d_grid_albedo);
     dfree(c->d_grid_normal);
+    dfree(c->d_sums);
+    c->have_sums = false;
     dfree(c->d_albedo_mips);
     for (auto &p : c->albedo_lv) p = nullptr;
     dfree(c->d_litmask);
@@ -711,20 +715,38 @@ int vct_shadowmaps(vct_ctx *c) {
 
 int vct_voxelize(vct_ctx *c, const vct_voxelize_params *p) {
     if (!c || !p) return fail(c, VCT_ERR_INVALID, "vct_voxelize: NULL argument");
-    if (!p->atomic_max)
-        return fail(c, VCT_ERR_INVALID,
-                    "vct_voxelize: only the atomicMax accumulation (voxelizeAtomicMax) is supported; the CAS "
-                    "running average of voxelize.frag:127-139 is order dependent");
+    if (p->atomic_max < VCT_ACCUM_AVG || p->atomic_max > VCT_ACCUM_SUM)
+        return fail(c, VCT_ERR_INVALID, "vct_voxelize: atomic_max %d is not a VCT_ACCUM_* mode", p->atomic_max);
     if (p->axis_override < -1 || p->axis_override > 2)
         return fail(c, VCT_ERR_INVALID, "axis_override %d out of [-1, 2]", p->axis_override);
     if (!c->d_model) return fail(c, VCT_ERR_STATE, "vct_voxelize: no scene");
     if (p->warp_texture && !c->have_warpmap) return fail(c, VCT_ERR_STATE, "vct_voxelize: warp_texture without vct_warpmap");
+    if (p->atomic_max == VCT_ACCUM_SUM && !c->d_sums) {
+        const size_t nv = (size_t)c->desc.dim * c->desc.dim * c->desc.dim;
+        HIPCHK(c, dalloc(&c->d_sums, nv * 8 * sizeof(uint32_t)));
+    }
     PassTimer pt(c, P_VOXELIZE);
     LAUNCHCHK(c, launch_voxelize(dev_scene(c), c->d_lights, (int)c->lights.size(), *p, c->desc.dim,
-                                 c->d_grid_albedo, c->d_grid_normal, c->d_stats, c->d_partials,
+                                 c->d_grid_albedo, c->d_grid_normal, c->d_sums, c->d_stats, c->d_partials,
                                  p->warp_texture ? c->d_warpmap : nullptr, c->rs, c->stream));
     c->have_voxels = true;
     c->have_mips = false;
+    c->have_sums = p->atomic_max == VCT_ACCUM_SUM;
+    return VCT_OK;
+}
+
+int vct_normalize_voxels(vct_ctx *c, float opacity) {
+    if (!c) return fail(nullptr, VCT_ERR_INVALID, "ctx is NULL");
+    if (!(opacity >= 0.0f && opacity <= 1.0f))
+        return fail(c, VCT_ERR_INVALID, "voxel_set_opacity %f out of [0, 1]", (double)opacity);
+    if (!c->have_sums)
+        return fail(c, VCT_ERR_STATE, "vct_normalize_voxels: the last vct_voxelize did not accumulate sums");
+    PassTimer pt(c, P_RADIANCE);
+    LAUNCHCHK(c, launch_normalize_voxels(c->d_sums, c->d_grid_albedo, c->d_grid_normal, c->desc.dim, opacity, c->pyr,
+                                         c->d_stats, c->d_partials, c->stream));
+    c->have_radiance = true;
+    c->have_mips = false;
+    c->pyr_nonneg = true;
     return VCT_OK;
 }
 
@@ -1307,6 +1329,34 @@ static TraceLaunch make_trace_launch(vct_ctx *c, const vct_trace_params *p) {
     t.ext = ctx_extent(c);
     t.warp = p->warp_texture ? c->d_warpmap : nullptr;
     t.nested = c->pyr_nonneg ? 1 : 0;
+    // the diffuse schedule, step by step as traceCone computes it
+    // (phong.frag:145-168): r = h tan(theta/2), lambda = log2(max(1, 2r)) +
+    // lodOffset, the level class of lambda (LINEAR_MIPMAP_LINEAR with the clamp
+    // to [0, q]), h += r
+    memset(&t.d_sched, 0, sizeof(t.d_sched));
+    if (t.d_steps > 0 && t.d_steps <= kMaxSchedule) {
+        const int q = c->desc.levels - 1;
+        float h = t.d_h0;
+        for (int i = 0; i < t.d_steps; ++i) {
+            const float r = h * t.d_tan_half;
+            const float m = 2.0f * r;
+            const float lambda = log2f(m > 1.0f ? m : 1.0f) + t.d_lod_offset;
+            int cls;
+            float fr = 0.0f;
+            if (lambda <= 0.0f) cls = -1;
+            else if (!(lambda < (float)q)) cls = q;
+            else {
+                const float fl = floorf(lambda);
+                cls = std::min(std::max((int)fl, 0), q - 1);
+                fr = lambda - fl;
+            }
+            t.d_sched.h[i] = h;
+            t.d_sched.cls[i] = cls;
+            t.d_sched.fr[i] = fr;
+            h = h + r;
+        }
+        t.d_sched.n = t.d_steps;
+    }
     return t;
 }
 

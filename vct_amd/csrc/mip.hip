@@ -285,13 +285,13 @@ __global__ __launch_bounds__(64) void k_fp8_encode(char *pyr, float *scales, Lev
 // The voxelColor chain of the same mipmap pass (Application.cpp:904-917): BOX2
 // on an RGBA8 image, so every level is stored as packUnorm4x8 of the mean of
 // its eight children read back as k/255 (filterRadiance.comp:14-65 with
-// rgba8 image bindings).  Same region scheme as k_mip_chain: a workgroup loads
-// an 8^3 region of the source (two 4^3 bricks per axis for the bricked level
-// 0, rows otherwise) and writes up to three levels from LDS.  Levels >= 1 are
-// linear, x fastest.
+// rgba8 image bindings).  Levels >= 1 are linear, x fastest.  Levels 1 and 2
+// come from k_albedo_bricks (below); the coarser ones from k_albedo_chain,
+// the region scheme of k_mip_chain: a workgroup loads an 8^3 region of a
+// linear source level and writes up to three levels from LDS.
 struct AlbedoMipArgs {
     const uint32_t *src;
-    int src_size, src_bricked;
+    int src_size;
     uint32_t *out[3];
     int nout;
 };
@@ -307,20 +307,8 @@ __global__ __launch_bounds__(256) void k_albedo_chain(AlbedoMipArgs a) {
     const int ox = (b % nreg) * R, oy = ((b / nreg) % nreg) * R, oz = (b / (nreg * nreg)) * R;
     const int R3 = R * R * R;
     for (int i = tid; i < R3; i += 256) {
-        int x, y, z;
-        uint32_t w;
-        if (a.src_bricked) {  // brick-major walk: 8 bricks of 64 contiguous voxels
-            const int j = i >> 6, v = i & 63;
-            x = ox + (j & 1) * 4 + (v & 3);
-            y = oy + ((j >> 1) & 1) * 4 + ((v >> 2) & 3);
-            z = oz + (j >> 2) * 4 + (v >> 4);
-            w = a.src[grid_index(x, y, z, s >> 2)];
-        } else {
-            x = ox + i % R;
-            y = oy + (i / R) % R;
-            z = oz + i / (R * R);
-            w = a.src[((size_t)z * s + y) * s + x];
-        }
+        const int x = ox + i % R, y = oy + (i / R) % R, z = oz + i / (R * R);
+        const uint32_t w = a.src[((size_t)z * s + y) * s + x];
         bufA[((z - oz) * R + (y - oy)) * R + (x - ox)] = make_float4(unorm8(w, 0), unorm8(w, 1), unorm8(w, 2),
                                                                      unorm8(w, 3));
     }
@@ -356,8 +344,76 @@ __global__ __launch_bounds__(256) void k_albedo_chain(AlbedoMipArgs a) {
     }
 }
 
+// Levels 1 and 2 of the albedo chain straight from the bricked grid: one
+// thread per 4^3 brick, which holds exactly the 2^3 level-1 texels and the one
+// level-2 texel it filters to.  The brick is read a z-slab pair at a time
+// (eight 16-byte loads of 32 words), every level-1 texel is rounded to RGBA8
+// before level 2 is filtered from it, and the taps are summed in the BOX2
+// order of filterRadiance.comp -- the same values as filtering level by level.
+__global__ __launch_bounds__(256) void k_albedo_bricks(const uint32_t *__restrict__ grid, int dim, uint32_t *out1,
+                                                       uint32_t *out2) {
+    const int nb = dim >> 2;
+    const uint32_t b = blockIdx.x * 256u + threadIdx.x;
+    if (b >= (uint32_t)nb * nb * nb) return;
+    const int lg = __builtin_ctz((unsigned)nb);
+    const int bx = (int)(b & (uint32_t)(nb - 1)), by = (int)((b >> lg) & (uint32_t)(nb - 1)), bz = (int)(b >> (2 * lg));
+    const uint4 *src = (const uint4 *)(grid + ((size_t)b << 6));
+    const int s1 = dim >> 1;
+    float4 l1[8];  // the rounded level-1 texels, (cx, cy, cz) = bits (2, 1, 0) in BOX2 order
+#pragma unroll
+    for (int cz = 0; cz < 2; ++cz) {
+        uint32_t w[32];  // z rows 2cz, 2cz+1 of the brick: w[(z & 1) * 16 + y * 4 + x]
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const uint4 v = src[cz * 8 + k];
+            w[k * 4 + 0] = v.x;
+            w[k * 4 + 1] = v.y;
+            w[k * 4 + 2] = v.z;
+            w[k * 4 + 3] = v.w;
+        }
+#pragma unroll
+        for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+            for (int cx = 0; cx < 2; ++cx) {
+                float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+                for (int o = 0; o < 8; ++o) {  // (0,0,0),(0,0,1),(0,1,0),(0,1,1),(1,0,0),...
+                    const int dx = (o >> 2) & 1, dy = (o >> 1) & 1, dz = o & 1;
+                    const uint32_t t = w[dz * 16 + (2 * cy + dy) * 4 + (2 * cx + dx)];
+                    v.x = v.x + unorm8(t, 0);
+                    v.y = v.y + unorm8(t, 1);
+                    v.z = v.z + unorm8(t, 2);
+                    v.w = v.w + unorm8(t, 3);
+                }
+                const uint32_t pw = pack_unorm4x8(v.x * 0.125f, v.y * 0.125f, v.z * 0.125f, v.w * 0.125f);
+                const int x = bx * 2 + cx, y = by * 2 + cy, z = bz * 2 + cz;
+                out1[((size_t)z * s1 + y) * s1 + x] = pw;
+                l1[(cx << 2) | (cy << 1) | cz] = make_float4(unorm8(pw, 0), unorm8(pw, 1), unorm8(pw, 2), unorm8(pw, 3));
+            }
+    }
+    if (out2 == nullptr) return;
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int o = 0; o < 8; ++o) {
+        v.x = v.x + l1[o].x;
+        v.y = v.y + l1[o].y;
+        v.z = v.z + l1[o].z;
+        v.w = v.w + l1[o].w;
+    }
+    const int s2 = dim >> 2;
+    out2[((size_t)bz * s2 + by) * s2 + bx] = pack_unorm4x8(v.x * 0.125f, v.y * 0.125f, v.z * 0.125f, v.w * 0.125f);
+}
+
 hipError_t launch_albedo_chain(const uint32_t *grid, uint32_t *const *levels, int dim, int nlevels, hipStream_t s) {
-    int src = 0;
+    if (nlevels < 2) return hipSuccess;
+    {
+        const unsigned nbricks = (unsigned)((dim >> 2) * (dim >> 2) * (dim >> 2));
+        hipLaunchKernelGGL(k_albedo_bricks, dim3((nbricks + 255) / 256), dim3(256), 0, s, grid, dim, levels[1],
+                           nlevels > 2 ? levels[2] : nullptr);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    int src = nlevels > 2 ? 2 : 1;
     while (src + 1 < nlevels) {
         const int size = dim >> src;
         const int R = size < 8 ? size : 8;
@@ -366,9 +422,8 @@ hipError_t launch_albedo_chain(const uint32_t *grid, uint32_t *const *levels, in
         int nout = nlevels - 1 - src;
         if (nout > lg) nout = lg;
         AlbedoMipArgs a;
-        a.src = src == 0 ? grid : levels[src];
+        a.src = levels[src];
         a.src_size = size;
-        a.src_bricked = src == 0 ? 1 : 0;
         for (int k = 0; k < 3; ++k) a.out[k] = levels[k < nout ? src + 1 + k : src + 1];
         a.nout = nout;
         const int nreg = size / R;

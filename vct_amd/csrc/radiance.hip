@@ -152,6 +152,26 @@ __device__ __forceinline__ float4 inject_texel(uint32_t a32, bool lit, float opa
     return r;
 }
 
+// The same texel in the fp16 storage format without the temporal filter,
+// from tables instead of arithmetic: h16[k] holds the fp16 bits of k/255 (the
+// unorm8 value, correctly rounded to float, then to fp16, as pack_h4 of
+// inject_texel gives them).  The rewritten albedo word is the old word with
+// its alpha byte replaced by the opacity's: packUnorm4x8 of k/255 gives back
+// k for every byte (checked exhaustively), so the colour bytes are unchanged.
+// op_b / op_h: the opacity as an unorm8 byte and as fp16 bits (opacity > 0).
+__device__ __forceinline__ uint2 inject_texel_h(uint32_t a32, bool lit, float opacity, uint32_t op_b, uint32_t op_h,
+                                                const uint16_t *h16, uint32_t *nw) {
+    const uint32_t ab = a32 >> 24;
+    *nw = a32;
+    if (ab == 0u) return make_uint2(0u, 0u);
+    uint32_t na_h = op_h;
+    if (opacity > 0.0f) *nw = (a32 & 0x00ffffffu) | (op_b << 24);
+    else na_h = h16[ab];
+    if (!lit) return make_uint2(0u, na_h << 16);
+    return make_uint2((uint32_t)h16[a32 & 0xffu] | ((uint32_t)h16[(a32 >> 8) & 0xffu] << 16),
+                      (uint32_t)h16[(a32 >> 16) & 0xffu] | (na_h << 16));
+}
+
 // One wavefront per 4^3 brick, one voxel per lane in the brick's element
 // order (lane = (x&3) | (y&3) << 2 | (z&3) << 4, the same for the bricked
 // albedo grid and for level 0), so albedo reads are one 256-byte run and each
@@ -172,6 +192,18 @@ __global__ __launch_bounds__(256) void k_normalize_inject(uint32_t *albedo, cons
                                                           uint32_t *partials) {
     const float mw = 1.0f - decay;  // the mix weight of this frame's value
     const float pw = 1.0f - mw;     // and of the previous one, as GLSL mix computes it
+    // per byte k: the fp16 bits of k/255, and maxFragmentsPerVoxel's
+    // uint(255 * (k/255)) (transferVoxels.comp:41) as the float product gives it
+    __shared__ uint16_t h16[256];
+    __shared__ uint8_t mf8[256];
+    {
+        const float v = unorm8((uint32_t)threadIdx.x, 0);  // 256 threads, one byte each
+        h16[threadIdx.x] = f2h(v);
+        mf8[threadIdx.x] = (uint8_t)(uint32_t)(255.0f * v);
+    }
+    __syncthreads();
+    const uint32_t op_b = pack_unorm4x8(0.0f, 0.0f, 0.0f, opacity) >> 24;
+    const uint32_t op_h = (uint32_t)f2h(opacity);
     const int nb = L0.nb;
     const uint32_t nbricks = (uint32_t)nb * nb * nb;
     const int lgnb = __builtin_ctz((unsigned)nb);
@@ -202,10 +234,14 @@ __global__ __launch_bounds__(256) void k_normalize_inject(uint32_t *albedo, cons
             }
         }
         uint32_t nw;
-        const float4 r = inject_texel<TEMPORAL>(a32, lit, opacity, prev, mw, pw, &nw);
-        if (unorm8(a32, 3) > 0.0f) {
+        constexpr bool LUT = FMT == 0 && !TEMPORAL;
+        float4 r = make_float4(0.f, 0.f, 0.f, 0.f);
+        uint2 h = make_uint2(0u, 0u);
+        if (LUT) h = inject_texel_h(a32, lit, opacity, op_b, op_h, h16, &nw);
+        else r = inject_texel<TEMPORAL>(a32, lit, opacity, prev, mw, pw, &nw);
+        if ((a32 >> 24) != 0u) {
             uniq += 1;
-            const uint32_t mf = (uint32_t)(255.0f * unorm8(a32, 3));
+            const uint32_t mf = mf8[a32 >> 24];
             maxf = mf > maxf ? mf : maxf;
             lit_count += lit ? 1u : 0u;
         }
@@ -218,7 +254,7 @@ __global__ __launch_bounds__(256) void k_normalize_inject(uint32_t *albedo, cons
             if (zero0 != nullptr && lane == 0 && now0 != was0) zero0[b] = now0 ? 1 : 0;
             continue;
         }
-        const uint2 h = pack_h4(r);
+        if (!LUT) h = pack_h4(r);
         uint2 h1 = make_uint2(__shfl_down(h.x, 1, 64), __shfl_down(h.y, 1, 64));  // texel x+1 in lane + 1
         if (vx == 3) {
             h1 = make_uint2(0u, 0u);  // texel dim: the border
@@ -226,7 +262,8 @@ __global__ __launch_bounds__(256) void k_normalize_inject(uint32_t *albedo, cons
                 const uint32_t a1 = albedo[gi + 64 - 3];
                 const bool lit1 = litmask[li + 1] != 0;
                 uint32_t nw1;
-                h1 = pack_h4(inject_texel<TEMPORAL>(a1, lit1, opacity, prev1, mw, pw, &nw1));
+                if (LUT) h1 = inject_texel_h(a1, lit1, opacity, op_b, op_h, h16, &nw1);
+                else h1 = pack_h4(inject_texel<TEMPORAL>(a1, lit1, opacity, prev1, mw, pw, &nw1));
             }
         }
         const bool now0 = __ballot((h.x | h.y | h1.x | h1.y) != 0u) == 0;

@@ -128,7 +128,8 @@ __device__ __forceinline__ void acc_f4(float4 &acc, uint4 v, float w) {
 template <int MODE, bool ZQ>
 __device__ __forceinline__ bool level_sample(float4 &acc, const char *pyr, __amdgpu_buffer_rsrc_t rs,
                                              const float *__restrict__ scales, const LevelInfo &L, const float p[3],
-                                             float lw, const uint32_t fe[3], const float d2[3], uint32_t fmask) {
+                                             float lw, const uint32_t fe[3], const float d2[3], uint32_t fmask,
+                                             bool fsame) {
     const int s = L.size;
     const bool aniso = L.nfaces == 6;
     uint32_t ty[2], tz[2];
@@ -202,10 +203,11 @@ __device__ __forceinline__ bool level_sample(float4 &acc, const char *pyr, __amd
     // loads and every lane weights them itself: the same operations on the same
     // values as the per-lane path, without its vector memory traffic.  Corners
     // 0 and 3 together fix all four (the element encoding of a texel index is
-    // unique, and corner 3 carries the clamped upper indices).
-    const uint32_t fk = aniso ? (fe[0] | (fe[1] << 10) | (fe[2] << 20)) : 0u;
-    const bool same = __builtin_amdgcn_readfirstlane(off[0]) == off[0] &&
-                      __builtin_amdgcn_readfirstlane(off[3]) == off[3] && __builtin_amdgcn_readfirstlane(fk) == fk;
+    // unique, and corner 3 carries the clamped upper indices).  fsame: the
+    // faces of every lane's cone are the first lane's (wave-uniform, decided
+    // once per cone).
+    const bool same = (!aniso || fsame) && __builtin_amdgcn_readfirstlane(off[0]) == off[0] &&
+                      __builtin_amdgcn_readfirstlane(off[3]) == off[3];
     if (__ballot(same) == __ballot(true)) {
         uint32_t so[4];
 #pragma unroll
@@ -292,7 +294,7 @@ __device__ __forceinline__ int level_class(float lambda, int q, float *fr) {
 template <int MODE>
 __device__ __forceinline__ float4 sample_class(const PyrArgs &pa, __amdgpu_buffer_rsrc_t rs, int D, float fr,
                                                const float p[3], const uint32_t fe[3], const float d2[3],
-                                               uint32_t fmask) {
+                                               uint32_t fmask, bool fsame) {
     float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
     const int q = pa.nlevels - 1;
     if (D < 0) {
@@ -307,7 +309,7 @@ __device__ __forceinline__ float4 sample_class(const PyrArgs &pa, __amdgpu_buffe
         return unpack_h4(v.x, v.y);
     }
     if (D >= q) {
-        level_sample<MODE, false>(acc, pa.pyr, rs, pa.scales, pa.lv[q], p, 1.0f, fe, d2, fmask);
+        level_sample<MODE, false>(acc, pa.pyr, rs, pa.scales, pa.lv[q], p, 1.0f, fe, d2, fmask, fsame);
         return acc;
     }
     // The coarser level first.  Its 2x2x2 cell covers the finer level's cell
@@ -316,23 +318,25 @@ __device__ __forceinline__ float4 sample_class(const PyrArgs &pa, __amdgpu_buffe
     // is zero only where every texel it filters is.  So where the coarser
     // fetch was all zero in every lane, the finer sample adds exactly 0 and is
     // skipped: cones marching through empty space.
-    const bool z = level_sample<MODE, true>(acc, pa.pyr, rs, pa.scales, pa.lv[D + 1], p, fr, fe, d2, fmask);
+    const bool z = level_sample<MODE, true>(acc, pa.pyr, rs, pa.scales, pa.lv[D + 1], p, fr, fe, d2, fmask, fsame);
     if (!(z && pa.nested))
-        level_sample<MODE, false>(acc, pa.pyr, rs, pa.scales, pa.lv[D], p, 1.0f - fr, fe, d2, fmask);
+        level_sample<MODE, false>(acc, pa.pyr, rs, pa.scales, pa.lv[D], p, 1.0f - fr, fe, d2, fmask, fsame);
     return acc;
 }
 
 // traceCone (phong.frag:135-180).  ULOD: every lane of the wavefront traces a
 // cone of the same aperture and initial height, so the step schedule h_i (and
 // with it the sampled level) depends only on the step index and the level
-// class is read from the first active lane.  Otherwise the lanes' level
-// classes are visited one at a time (each pass with a uniform class).
+// class is read from the first active lane -- or, with `sched` (the diffuse
+// cones, whose schedule the host computed for the launch), read from the
+// kernel arguments.  Otherwise the lanes' level classes are visited one at a
+// time (each pass with a uniform class).
 // warp: warpTexture (phong.frag:150-152), each sample position goes through the
 // warp texture before the pyramid is sampled (the level stays the unwarped one).
 template <int MODE, bool ANISO, bool ULOD>
 __device__ float4 trace_cone(const PyrArgs &pa, __amdgpu_buffer_rsrc_t rs, const float pos[3], const float nrm[3],
                              const float dir_in[3], int steps, float bias, float tan_half, float h0,
-                             float lod_offset, const float4 *warp) {
+                             float lod_offset, const float4 *warp, const ConeSchedule *sched = nullptr) {
     float dir[3];
     {
         const float il = __builtin_amdgcn_rsqf(dot3(dir_in, dir_in));
@@ -343,6 +347,7 @@ __device__ float4 trace_cone(const PyrArgs &pa, __amdgpu_buffer_rsrc_t rs, const
     uint32_t fe[3] = {0u, 0u, 0u};
     float d2[3] = {0.f, 0.f, 0.f};
     uint32_t fmask = 0u;
+    bool fsame = true;
     if (ANISO) {
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
@@ -352,6 +357,8 @@ __device__ float4 trace_cone(const PyrArgs &pa, __amdgpu_buffer_rsrc_t rs, const
             // one or two of the three faces
             if (__ballot(d2[k] != 0.0f) != 0) fmask |= 1u << k;
         }
+        const uint32_t fk = fe[0] | (fe[1] << 10) | (fe[2] << 20);
+        fsame = __ballot(__builtin_amdgcn_readfirstlane(fk) != fk) == 0;
     }
     const int q = pa.nlevels - 1;
     float c0 = 0.f, c1 = 0.f, c2 = 0.f, a = 0.f;
@@ -359,6 +366,27 @@ __device__ float4 trace_cone(const PyrArgs &pa, __amdgpu_buffer_rsrc_t rs, const
     float start[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) start[k] = pos[k] + (bias * nrm[k]) * scale;
+    if (ULOD && sched != nullptr && sched->n == steps) {
+        // dim is a power of two, so (h * dir) * scale == h * (dir * scale) exactly
+        float ds[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) ds[k] = dir[k] * scale;
+        for (int i = 0; i < steps && a < 0.95f; ++i) {
+            const float h = sched->h[i];
+            float sp[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) sp[k] = start[k] + h * ds[k];
+            if (sp[0] < 0.f || sp[0] > 1.f || sp[1] < 0.f || sp[1] > 1.f || sp[2] < 0.f || sp[2] > 1.f) break;
+            if (warp) warp_sample(warp, sp, sp);
+            const float4 smp = sample_class<MODE>(pa, rs, sched->cls[i], sched->fr[i], sp, fe, d2, fmask, fsame);
+            const float aa = 1.0f - a;
+            c0 += smp.x * aa;
+            c1 += smp.y * aa;
+            c2 += smp.z * aa;
+            a += aa * smp.w;
+        }
+        return make_float4(c0, c1, c2, a);
+    }
     float h = h0;
     for (int i = 0; i < steps && a < 0.95f; ++i) {
         const float r = h * tan_half;
@@ -374,7 +402,7 @@ __device__ float4 trace_cone(const PyrArgs &pa, __amdgpu_buffer_rsrc_t rs, const
         if (ULOD) {
             const int Du = __builtin_amdgcn_readfirstlane(D);
             const float fru = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(fr)));
-            smp = sample_class<MODE>(pa, rs, Du, fru, sp, fe, d2, fmask);
+            smp = sample_class<MODE>(pa, rs, Du, fru, sp, fe, d2, fmask, fsame);
         } else {
             smp = make_float4(0.f, 0.f, 0.f, 0.f);
             bool pending = true;
@@ -383,7 +411,7 @@ __device__ float4 trace_cone(const PyrArgs &pa, __amdgpu_buffer_rsrc_t rs, const
                 if (m == 0) break;
                 const int Du = __builtin_amdgcn_readlane(D, __builtin_ctzll(m));
                 if (pending && D == Du) {
-                    smp = sample_class<MODE>(pa, rs, Du, fr, sp, fe, d2, fmask);
+                    smp = sample_class<MODE>(pa, rs, Du, fr, sp, fe, d2, fmask, fsame);
                     pending = false;
                 }
             }
@@ -434,7 +462,7 @@ __device__ __forceinline__ void trace_pixel(const TraceLaunch &t, const PyrArgs 
 #pragma unroll
         for (int k = 0; k < 3; ++k) d[k] = (T[k] * cx + B[k] * cy) + nraw[k] * cz;
         const float4 r = trace_cone<MODE, ANISO, true>(pa, rs, pos, N, d, t.d_steps, t.d_bias, t.d_tan_half, t.d_h0,
-                                                       t.d_lod_offset, t.warp);
+                                                       t.d_lod_offset, t.warp, &t.d_sched);
         const float w = kConeW[i];
         i0 += w * r.x;
         i1 += w * r.y;

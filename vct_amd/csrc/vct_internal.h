@@ -121,6 +121,21 @@ struct RasterScratch {
     size_t cap_slots;
 };
 
+// The diffuse cones' step schedule.  Every diffuse cone of every pixel starts
+// at the same initial height with the same aperture, so the sample distance
+// h_i of step i and its level class (D_i, fr_i: levels D_i and D_i + 1 blended
+// with weights 1 - fr_i, fr_i; D_i = -1 NEAREST on level 0, D_i = q the last
+// level alone) depend on i only.  The host computes them once per launch in
+// traceCone's own order (phong.frag:145-168), so the kernel reads them from
+// its arguments instead of deriving them for every cone of every pixel.
+constexpr int kMaxSchedule = 32;
+struct ConeSchedule {
+    int n;                    // steps covered: the cone's step count, or 0 (compute in the kernel)
+    float h[kMaxSchedule];    // h_i, in voxels
+    int cls[kMaxSchedule];    // D_i
+    float fr[kMaxSchedule];   // fr_i
+};
+
 struct TraceLaunch {
     const vct_gbuffer_texel *gbuf;
     int width, height;
@@ -139,6 +154,7 @@ struct TraceLaunch {
     // every texel is non-negative, and a filtered texel is zero exactly when
     // all the texels it filters are
     int nested;
+    ConeSchedule d_sched;  // the diffuse cones' schedule (n = 0: none)
 };
 
 // Statistics are reduced per workgroup into `partials` (kStatPartials slots of
@@ -185,8 +201,11 @@ hipError_t launch_gbuffer(const DevScene &sc, const vct_camera &cam, uint64_t *v
 
 // voxelize.hip
 hipError_t launch_voxelize(const DevScene &sc, const DevLight *dlights, int nlights, const vct_voxelize_params &p,
-                           int dim, uint32_t *albedo, uint32_t *normal, vct_stats *stats, uint32_t *partials,
-                           const float4 *warp, RasterScratch &rs, hipStream_t s);
+                           int dim, uint32_t *albedo, uint32_t *normal, uint32_t *sums, vct_stats *stats,
+                           uint32_t *partials, const float4 *warp, RasterScratch &rs, hipStream_t s);
+// normalizeVoxels.comp for the sum-accumulate mode (sums: 8 words per voxel)
+hipError_t launch_normalize_voxels(const uint32_t *sums, uint32_t *albedo, uint32_t *normal, int dim, float opacity,
+                                   const PyrDesc &pyr, vct_stats *stats, uint32_t *partials, hipStream_t s);
 // the 32^3 occupancy grid of the warp texture (the voxelizer in occupancy mode)
 hipError_t launch_warp_occupancy(const DevScene &sc, const vct_voxelize_params &p, uint32_t *occ, RasterScratch &rs,
                                  hipStream_t s);

@@ -128,6 +128,46 @@ __device__ __forceinline__ void vox_unit(const float pos[3], int axis, float un[
     un[2] = 1.0f - un[2];
 }
 
+// imageAtomicRGBA8Avg, the running average the reference's RGBA8 build uses
+// when voxelizeAtomicMax is off (voxelize.frag:111-139, the live `#if 1`
+// branch): the word holds the average colour in its rgb bytes (0..255) and
+// the fragment count in its alpha byte; each fragment adds (255 c, 1) by
+// compare-and-swap, rgb = (rgb * count + 255 c) / (count + 1) truncated to
+// bytes.  The result depends on the order in which fragments land (the count
+// does not).  A lane whose swap fails retries with the value it found, so
+// every attempt by some lane succeeds and the loop ends.
+__device__ __forceinline__ uint32_t rgba8_from_floats(float x, float y, float z, float w) {
+    return (((uint32_t)w & 0xffu) << 24) | (((uint32_t)z & 0xffu) << 16) | (((uint32_t)y & 0xffu) << 8) |
+           ((uint32_t)x & 0xffu);
+}
+__device__ __forceinline__ void rgba8_avg(uint32_t *p, float r, float g, float b) {
+    const float v0 = r * 255.0f, v1 = g * 255.0f, v2 = b * 255.0f, v3 = 1.0f;
+    uint32_t nv = rgba8_from_floats(v0, v1, v2, v3);
+    uint32_t prev = 0u, cur;
+    while ((cur = atomicCAS(p, prev, nv)) != prev) {
+        prev = cur;
+        float c0 = (float)(cur & 0xffu), c1 = (float)((cur >> 8) & 0xffu), c2 = (float)((cur >> 16) & 0xffu),
+              c3 = (float)(cur >> 24);
+        c0 = c0 * c3;
+        c1 = c1 * c3;
+        c2 = c2 * c3;
+        float s0 = c0 + v0, s1 = c1 + v1, s2 = c2 + v2;
+        const float s3 = c3 + v3;
+        s0 = s0 / s3;
+        s1 = s1 / s3;
+        s2 = s2 / s3;
+        nv = rgba8_from_floats(s0, s1, s2, s3);
+    }
+}
+
+// The sum-accumulate mode (the reference's fp16 imageAtomicAdd of (c, 1),
+// voxelize.frag:257-260, normalized by normalizeVoxels.comp): here the sums
+// are 32-bit fixed point, round(65535 clamp(c, 0, 1)) per channel plus the
+// count, so they are exact and independent of fragment order.  Eight words
+// per voxel: colour rgb + count, normal rgb + count (voxel index v at
+// sums[8 v ..]).
+__device__ __forceinline__ uint32_t fix16(float c) { return (uint32_t)__builtin_rintf(clampf(c, 0.0f, 1.0f) * 65535.0f); }
+
 // The scene's lights in LDS, for the lit voxelizer (VoxFrag::begin).
 __shared__ DevLight vox_lights[kMaxLights];
 
@@ -135,6 +175,8 @@ struct VoxFrag {
     const float4 *attr;
     const DevLight *lights;
     int nlights, lighting, msaa, dim;
+    int accum;        // VCT_ACCUM_* (voxelize params atomic_max)
+    uint32_t *sums;   // VCT_ACCUM_SUM: 8 words per voxel
     uint32_t *albedo, *normal;
     uint32_t *partials;
     const float4 *warp;  // warpTexture: the warp texture, or null
@@ -241,8 +283,23 @@ struct VoxFrag {
         for (int k = 0; k < 3; ++k) idx[k] = (int)((float)dim * un[k]);
         if (idx[0] < 0 || idx[1] < 0 || idx[2] < 0 || idx[0] >= dim || idx[1] >= dim || idx[2] >= dim) return;
         const uint32_t vi = grid_index(idx[0], idx[1], idx[2], dim >> 2);
-        atomicMax(&albedo[vi], pack_unorm4x8(color[0], color[1], color[2], 1.0f));
-        atomicMax(&normal[vi], pack_unorm4x8(ne0, ne1, ne2, 1.0f));
+        if (accum == VCT_ACCUM_MAX) {
+            atomicMax(&albedo[vi], pack_unorm4x8(color[0], color[1], color[2], 1.0f));
+            atomicMax(&normal[vi], pack_unorm4x8(ne0, ne1, ne2, 1.0f));
+        } else if (accum == VCT_ACCUM_AVG) {
+            rgba8_avg(&albedo[vi], color[0], color[1], color[2]);
+            rgba8_avg(&normal[vi], ne0, ne1, ne2);
+        } else {
+            uint32_t *w = sums + (size_t)vi * 8;
+            atomicAdd(&w[0], fix16(color[0]));
+            atomicAdd(&w[1], fix16(color[1]));
+            atomicAdd(&w[2], fix16(color[2]));
+            atomicAdd(&w[3], 1u);
+            atomicAdd(&w[4], fix16(ne0));
+            atomicAdd(&w[5], fix16(ne1));
+            atomicAdd(&w[6], fix16(ne2));
+            atomicAdd(&w[7], 1u);
+        }
     }
     __device__ void finish() { block_partial(frags, 0u, 0u, 0u, partials); }
 };
@@ -302,12 +359,13 @@ hipError_t launch_warp_occupancy(const DevScene &sc, const vct_voxelize_params &
 }
 
 hipError_t launch_voxelize(const DevScene &sc, const DevLight *dlights, int nlights, const vct_voxelize_params &p,
-                           int dim, uint32_t *albedo, uint32_t *normal, vct_stats *stats, uint32_t *partials,
-                           const float4 *warp, RasterScratch &rs, hipStream_t s) {
+                           int dim, uint32_t *albedo, uint32_t *normal, uint32_t *sums, vct_stats *stats,
+                           uint32_t *partials, const float4 *warp, RasterScratch &rs, hipStream_t s) {
     const size_t nv = (size_t)dim * dim * dim;
     hipError_t e = launch_fill_u32(albedo, 0u, nv, s);  // glClearTexImage (Application.cpp:686-687)
     if (e != hipSuccess) return e;
     if ((e = launch_fill_u32(normal, 0u, nv, s)) != hipSuccess) return e;
+    if (p.atomic_max == VCT_ACCUM_SUM && (e = launch_fill_u32(sums, 0u, nv * 8, s)) != hipSuccess) return e;
     if (sc.ntri == 0) return launch_fill_u32(&stats->total_voxel_fragments, 0u, 1, s);
     VoxMats vm;
     for (int i = 0; i < 16; ++i) {
@@ -327,6 +385,8 @@ hipError_t launch_voxelize(const DevScene &sc, const DevLight *dlights, int nlig
     f.lighting = p.lighting;
     f.msaa = p.msaa;
     f.dim = dim;
+    f.accum = p.atomic_max;
+    f.sums = sums;
     f.albedo = albedo;
     f.normal = normal;
     f.partials = partials;
@@ -339,6 +399,51 @@ hipError_t launch_voxelize(const DevScene &sc, const DevLight *dlights, int nlig
     if ((e = hipGetLastError()) != hipSuccess) return e;
     // totalVoxelFragments (voxelize.frag:195)
     return launch_reduce_stats(partials, nblk, &stats->total_voxel_fragments, nullptr, nullptr, s);
+}
+
+// normalizeVoxels.comp:19-42 for the sum-accumulate mode: per voxel with
+// fragments, the averages colour = sum / count (alpha 1, then voxelSetOpacity
+// when it is > 0) and normal = sum / count, written to the RGBA8 grids with
+// packUnorm4x8, radiance level 0 := (0, 0, 0, alpha), uniqueVoxels and
+// maxFragmentsPerVoxel (= the largest count).  Voxels without fragments keep
+// the cleared grids and a zero level 0.
+__global__ __launch_bounds__(256) void k_normalize_voxels(const uint32_t *__restrict__ sums, uint32_t *albedo,
+                                                          uint32_t *normal, int dim, float opacity, LevelInfo L0,
+                                                          char *pyr, uint32_t *partials) {
+    const size_t nv = (size_t)dim * dim * dim;
+    uint32_t uniq = 0, maxc = 0;
+    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < nv; g += (size_t)gridDim.x * blockDim.x) {
+        const int x = (int)(g % dim), y = (int)((g / dim) % dim), z = (int)(g / ((size_t)dim * dim));
+        const uint32_t vi = grid_index(x, y, z, dim >> 2);
+        const uint4 c = *(const uint4 *)(sums + (size_t)vi * 8);
+        const uint4 n = *(const uint4 *)(sums + (size_t)vi * 8 + 4);
+        float4 r = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (c.w > 0u) {
+            uniq += 1;
+            maxc = c.w > maxc ? c.w : maxc;
+            const float den = 65535.0f * (float)c.w;
+            const float a = opacity > 0.0f ? opacity : 1.0f;
+            albedo[vi] = pack_unorm4x8((float)c.x / den, (float)c.y / den, (float)c.z / den, a);
+            r.w = a;
+        }
+        if (n.w > 0u) {
+            const float den = 65535.0f * (float)n.w;
+            normal[vi] = pack_unorm4x8((float)n.x / den, (float)n.y / den, (float)n.z / den, 1.0f);
+        }
+        store_stage_texel(pyr, L0, x, y, z, 0, r);
+    }
+    block_partial(uniq, maxc, 0u, 0u, partials);
+}
+
+hipError_t launch_normalize_voxels(const uint32_t *sums, uint32_t *albedo, uint32_t *normal, int dim, float opacity,
+                                   const PyrDesc &pyr, vct_stats *stats, uint32_t *partials, hipStream_t s) {
+    const int nblk = kStatPartials;
+    hipLaunchKernelGGL(k_normalize_voxels, dim3(nblk), dim3(256), 0, s, sums, albedo, normal, dim, opacity, pyr.lv[0],
+                       pyr.pyr, partials);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if ((e = launch_zero0_reset(pyr, s)) != hipSuccess) return e;  // level 0 written by another pass than the injection
+    return launch_reduce_stats(partials, nblk, &stats->unique_voxels, &stats->max_fragments_per_voxel, nullptr, s);
 }
 
 }  // namespace vctk

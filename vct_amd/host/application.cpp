@@ -228,6 +228,10 @@ struct vct_app {
     vct_camera cam{};
     vct_trace_params trace{};
     std::vector<float> model;  // scratch
+    // where seeded cameras may stand (vct_app_camera): not inside these boxes
+    std::vector<std::array<double, 6>> boxes;
+    double cam_range = 0.0;
+    float cam_znear = 0.05f, cam_zfar = 10.0f;
 };
 
 static int app_fail(vct_app *a, int code, const std::string &msg) {
@@ -263,14 +267,17 @@ static void make_dir_light(Rng &r, double bound, int sm, vct_light &L) {
     inverse4(L.ls, L.ls_inverse);
 }
 
-static void make_point_light(Rng &r, int sm, vct_light &L) {
+// scale: the scene's extent over the unit extent of scene 2 (positions,
+// ranges and the shadow frustum's depth range scale with it)
+static void make_point_light(Rng &r, int sm, vct_light &L, double scale = 1.0) {
     std::memset(&L, 0, sizeof(L));
-    const float pos[3] = {(float)r.uni(-0.8, 0.8), (float)r.uni(-0.8, 0.8), (float)r.uni(-0.8, 0.8)};
+    const float pos[3] = {(float)(scale * r.uni(-0.8, 0.8)), (float)(scale * r.uni(-0.8, 0.8)),
+                          (float)(scale * r.uni(-0.8, 0.8))};
     std::memcpy(L.position, pos, 12);
     L.direction[2] = -1.0f;
     for (int k = 0; k < 3; ++k) L.color[k] = (float)r.uni(0.5, 1.0);
     L.intensity = 1.0f;
-    L.range = (float)r.uni(0.8, 1.6);
+    L.range = (float)(scale * r.uni(0.8, 1.6));
     L.type = VCT_LIGHT_POINT;
     L.enabled = 1;
     L.shadow_caster = 1;
@@ -278,10 +285,10 @@ static void make_point_light(Rng &r, int sm, vct_light &L) {
     L.perspective = 1;
     // one 120-degree perspective shadow map aimed at the scene centre
     float lp[16], lv[16];
-    perspective(2.0943951f, 1.0f, 0.02f, 4.0f, lp);
+    perspective(2.0943951f, 1.0f, (float)(0.02 * scale), (float)(4.0 * scale), lp);
     float tgt[3] = {0.0f, 0.0f, 0.0f};
     const float d2 = (pos[0] * pos[0] + pos[1] * pos[1]) + pos[2] * pos[2];
-    if (d2 < 0.0025f) tgt[1] = pos[1] - 1.0f;
+    if (d2 < (float)(0.0025 * scale * scale)) tgt[1] = pos[1] - 1.0f;
     float up[3] = {0.0f, 1.0f, 0.0f};
     const float f[3] = {tgt[0] - pos[0], tgt[1] - pos[1], tgt[2] - pos[2]};
     if (std::fabs(f[1]) > 0.99f * std::sqrt((f[0] * f[0] + f[1] * f[1]) + f[2] * f[2])) {
@@ -305,7 +312,7 @@ static bool inside_any_box(const Mesh &m, const std::vector<std::array<double, 6
 }
 
 static void make_camera(Rng &r, const std::vector<std::array<double, 6>> &boxes, const Mesh &m, double range, int W,
-                        int H, float znear, vct_camera &cam) {
+                        int H, float znear, vct_camera &cam, float zfar = 10.0f) {
     double eye[3];
     for (int tries = 0; tries < 1000; ++tries) {
         for (int k = 0; k < 3; ++k) eye[k] = r.uni(-range, range);
@@ -324,14 +331,14 @@ static void make_camera(Rng &r, const std::vector<std::array<double, 6>> &boxes,
     lookat(e, tgt, up, cam.view);
     // explicit 60-degree vertical field of view (the reference passes 45 as
     // radians to glm::perspective, src/Camera.h:18 + Application.cpp:200)
-    perspective(1.0471975512f, (float)W / (float)H, znear, 10.0f, cam.projection);
+    perspective(1.0471975512f, (float)W / (float)H, znear, zfar, cam.projection);
     cam.cull_back = 1;
 }
 
 extern "C" {
 
 int vct_app_default_config(int scene, vct_app_config *o) {
-    if (!o || (scene != 1 && scene != 2)) return VCT_ERR_INVALID;
+    if (!o || (scene != 1 && scene != 2 && scene != 5)) return VCT_ERR_INVALID;
     std::memset(o, 0, sizeof(*o));
     o->scene = scene;
     o->seed = (uint64_t)scene;
@@ -351,7 +358,7 @@ int vct_app_default_config(int scene, vct_app_config *o) {
         o->width = o->height = 256;
         o->shadow_size = 512;
         o->reflections = 0;
-    } else {
+    } else if (scene == 2) {
         o->dim = 256;
         o->levels = 8;
         o->anisotropic = 1;
@@ -359,6 +366,17 @@ int vct_app_default_config(int scene, vct_app_config *o) {
         o->height = 1080;
         o->shadow_size = 1024;
         o->reflections = 1;
+    } else {  // the warped 4K workload: FP8 coarse levels, warp texture at high resolution 4
+        o->dim = 256;
+        o->levels = 8;
+        o->anisotropic = 1;
+        o->storage = VCT_STORE_FP8;
+        o->width = 3840;
+        o->height = 2160;
+        o->shadow_size = 1024;
+        o->reflections = 1;
+        o->warp_texture = 1;
+        o->warp_high_resolution = 4.0f;
     }
     return VCT_OK;
 }
@@ -378,6 +396,7 @@ static void fill_defaults(vct_app_config &c) {
     if (c.msaa < 0) c.msaa = d.msaa;
     if (c.reflections < 0) c.reflections = d.reflections;
     if (c.fill_holes < 0) c.fill_holes = d.fill_holes;
+    if (!(c.warp_high_resolution > 0.0f)) c.warp_high_resolution = 2.0f;  // Settings (Application.h:87)
 }
 
 // voxelTrackCamera (Application.cpp:187-191): the grid centre snaps to the
@@ -400,7 +419,7 @@ static int app_finish(vct_app *a, vct_app **out);
 int vct_app_create(vct_app **out, const vct_app_config *cfg_in) {
     if (!out || !cfg_in) return VCT_ERR_INVALID;
     *out = nullptr;
-    if (cfg_in->scene != 1 && cfg_in->scene != 2) return VCT_ERR_INVALID;
+    if (cfg_in->scene != 1 && cfg_in->scene != 2 && cfg_in->scene != 5) return VCT_ERR_INVALID;
     vct_app *a = new vct_app();
     a->cfg = *cfg_in;
     fill_defaults(a->cfg);
@@ -433,23 +452,31 @@ int vct_app_create(vct_app **out, const vct_app_config *cfg_in) {
         }
         bound = 1.8;
     } else {
-        // 2000 boxes (half-extents lognormal(mu=-3, sigma=0.5)), 64 icospheres
-        // (subdivision 4, radius U[0.02,0.1]); each object moves rigidly
-        const int nbox = 2000, nsph = 64;
+        // scene 2: 2000 boxes (half-extents lognormal(mu=-3, sigma=0.5)), 64
+        // icospheres (subdivision 4, radius U[0.02,0.1]) in [-1,1]^3.  Scene 5:
+        // 8000 boxes and 512 icospheres in [-4,4]^3, positions scaled by 4 and
+        // sizes by 2.5 -- the cube root of the volume each object has to
+        // itself (64 x the volume for 4.12 x the objects), so the objects fill
+        // the same quarter of the volume as in scene 2 (sizes x4 would fill it
+        // completely).  Centres are drawn from 0.9 of the extent, so that
+        // objects of the largest sizes start inside the grid.  Each object
+        // moves rigidly.
+        const double S = c.scene == 5 ? 4.0 : 1.0, Z = c.scene == 5 ? 2.5 : 1.0;
+        const int nbox = c.scene == 5 ? 8000 : 2000, nsph = c.scene == 5 ? 512 : 64;
         for (int b = 0; b < nbox; ++b) {
             double h[3];
-            for (int k = 0; k < 3; ++k) h[k] = std::exp(-3.0 + 0.5 * r.normal());
+            for (int k = 0; k < 3; ++k) h[k] = Z * std::exp(-3.0 + 0.5 * r.normal());
             add_box(a->mesh, h, false, (uint32_t)b, r);
             double cen[3];
-            for (int k = 0; k < 3; ++k) cen[k] = r.uni(-0.9, 0.9);
+            for (int k = 0; k < 3; ++k) cen[k] = r.uni(-0.9 * S, 0.9 * S);
             centers.insert(centers.end(), {cen[0], cen[1], cen[2]});
             boxes.push_back({cen[0], cen[1], cen[2], h[0], h[1], h[2]});
         }
         for (int s = 0; s < nsph; ++s) {
-            double rad = r.uni(0.02, 0.1);
+            double rad = r.uni(0.02 * Z, 0.1 * Z);
             add_icosphere(a->mesh, rad, 4, (uint32_t)(nbox + s), r);
             double cen[3];
-            for (int k = 0; k < 3; ++k) cen[k] = r.uni(-0.9, 0.9);
+            for (int k = 0; k < 3; ++k) cen[k] = r.uni(-0.9 * S, 0.9 * S);
             centers.insert(centers.end(), {cen[0], cen[1], cen[2]});
             boxes.push_back({cen[0], cen[1], cen[2], rad, rad, rad});
         }
@@ -457,11 +484,11 @@ int vct_app_create(vct_app **out, const vct_app_config *cfg_in) {
         a->mesh.rough.resize(a->mesh.nobj);
         for (auto &v : a->mesh.rough) v = (float)r.uni(0.1, 0.8);
         for (int k = 0; k < 3; ++k) {
-            a->extent_min[k] = -1.0f;
-            a->extent_max[k] = 1.0f;
+            a->extent_min[k] = (float)-S;
+            a->extent_max[k] = (float)S;
             a->center[k] = 0.0f;
         }
-        bound = 1.8;
+        bound = 1.8 * S;
         a->dynamic = true;
     }
     // per-object motion (static scenes keep omega = vel = 0)
@@ -477,8 +504,9 @@ int vct_app_create(vct_app **out, const vct_app_config *cfg_in) {
         }
         for (int k = 0; k < 3; ++k) mo.axis[k] = ax[k] / l;
         if (a->dynamic) {
+            const double S = c.scene == 5 ? 4.0 : 1.0;  // translation scales with the extent
             mo.omega = r.uni(0.0, 5.0 * 3.141592653589793 / 180.0);
-            for (int k = 0; k < 3; ++k) mo.vel[k] = r.uni(-0.01, 0.01);
+            for (int k = 0; k < 3; ++k) mo.vel[k] = r.uni(-0.01 * S, 0.01 * S);
         } else {
             mo.omega = 0.0;
             for (int k = 0; k < 3; ++k) mo.vel[k] = 0.0;
@@ -488,13 +516,20 @@ int vct_app_create(vct_app **out, const vct_app_config *cfg_in) {
     if (c.scene == 1) {
         a->lights.resize(1);
         make_dir_light(r, bound, c.shadow_size, a->lights[0]);
-    } else {
+    } else if (c.scene == 2) {
         a->lights.resize(5);
         make_dir_light(r, bound, c.shadow_size, a->lights[0]);
         for (int i = 1; i < 5; ++i) make_point_light(r, c.shadow_size, a->lights[i]);
+    } else {  // 8 lights: 1 directional + 7 point lights, each with its shadow map
+        a->lights.resize(8);
+        make_dir_light(r, bound, c.shadow_size, a->lights[0]);
+        for (int i = 1; i < 8; ++i) make_point_light(r, c.shadow_size, a->lights[i], 4.0);
     }
-    make_camera(r, boxes, a->mesh, c.scene == 1 ? 0.5 : 0.6, c.width, c.height, c.scene == 1 ? 0.05f : 0.02f,
-                a->cam);
+    a->boxes = boxes;
+    a->cam_range = c.scene == 5 ? 2.4 : (c.scene == 1 ? 0.5 : 0.6);
+    a->cam_znear = c.scene == 5 ? 0.08f : (c.scene == 1 ? 0.05f : 0.02f);
+    a->cam_zfar = c.scene == 5 ? 40.0f : 10.0f;
+    make_camera(r, boxes, a->mesh, a->cam_range, c.width, c.height, a->cam_znear, a->cam, a->cam_zfar);
     return app_finish(a, out);
 }
 
@@ -673,7 +708,9 @@ static void frame_models(const vct_app *a, int frame, float *out) {
     for (int o = 0; o < a->mesh.nobj; ++o) {
         const ObjMotion &mo = a->motion[o];
         double c[3];
-        for (int k = 0; k < 3; ++k) c[k] = a->dynamic ? reflect_into(mo.c0[k] + mo.vel[k] * frame, 0.95) : mo.c0[k];
+        // centres bounce inside 0.95 of the extent (0.95 in scene 2, 3.8 in scene 5)
+        const double B = 0.95 * (double)a->extent_max[0];
+        for (int k = 0; k < 3; ++k) c[k] = a->dynamic ? reflect_into(mo.c0[k] + mo.vel[k] * frame, B) : mo.c0[k];
         rigid(c, mo.axis, mo.omega * frame, out + (size_t)o * 16);
     }
 }
@@ -797,6 +834,14 @@ int vct_app_surface_points(vct_app *a, int frame, int n, uint64_t seed, vct_gbuf
     return VCT_OK;
 }
 
+int vct_app_camera(const vct_app *a, uint64_t seed, int width, int height, vct_camera *out) {
+    if (!a || !out || width < 1 || height < 1) return VCT_ERR_INVALID;
+    if (a->cam_range <= 0.0) return VCT_ERR_STATE;  // an OBJ scene: one fitted camera only
+    Rng r(seed * 0x9E3779B97F4A7C15ull + 4242);
+    make_camera(r, a->boxes, a->mesh, a->cam_range, width, height, a->cam_znear, *out, a->cam_zfar);
+    return VCT_OK;
+}
+
 int vct_app_build_radiance(vct_app *a, int frame) {
     if (!a) return VCT_ERR_INVALID;
     if (!a->ctx) return app_fail(a, VCT_ERR_STATE, "vct_app_build_radiance: app was created without a device");
@@ -825,7 +870,7 @@ int vct_app_build_radiance(vct_app *a, int frame) {
 
 int vct_app_warpmap_params(const vct_app *a, vct_warpmap_params *out) {
     if (!a || !out) return VCT_ERR_INVALID;
-    out->high_resolution = 2.0f;  // Settings (src/Application.h:80-92)
+    out->high_resolution = a->cfg.warp_high_resolution;  // Settings (src/Application.h:80-92), 2.0 by default
     out->low_resolution = 0.5f;
     out->axes[0] = out->axes[1] = out->axes[2] = 1;
     out->linear = 0;
