@@ -101,6 +101,10 @@ typedef struct vct_mesh {
     const uint32_t *albedo;    /* [num_tris] RGBA8 (R in low byte) = diffuseMap texel */
     const uint32_t *object;    /* [num_tris] actor index in [0, num_objects)       */
     const float *roughness;    /* [num_objects] roughness map value, or NULL (0.5)  */
+    const float *vertex_tangents; /* [num_tris][3][3] object-space tangent of each
+                                  corner's vertex (Mesh.cpp:190-197: the normalized
+                                  sum over the faces sharing it), or NULL: every
+                                  corner takes its triangle's `tangents` entry    */
 } vct_mesh;
 
 /* One light, replacing the std140 SSBO record of Light::writeSSBO

@@ -94,6 +94,9 @@ int vct_app_scene_sizes(const vct_app *app, int *num_tris, int *num_objects, int
 /* Any pointer may be NULL.  Returns 1 if the scene has per-object roughness. */
 int vct_app_get_mesh(const vct_app *app, float *pos, float *nrm, float *tan, uint32_t *albedo, uint32_t *obj,
                      float *rough);
+/* The scene's per-corner (vertex) tangents, [num_tris][3][3], when it has
+ * them (OBJ scenes): returns 1 and fills vtan (may be NULL), else 0. */
+int vct_app_get_vertex_tangents(const vct_app *app, float *vtan);
 
 /* Everything the passes of frame `frame` receive (uniforms and matrices);
  * any pointer may be NULL. model: [num_objects][16]; lights: [num_lights]. */

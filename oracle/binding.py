@@ -22,7 +22,7 @@ F3 = C.c_float * 3
 class Scene(C.Structure):
     _fields_ = [("ntri", C.c_int), ("nobj", C.c_int), ("pos", C.c_void_p), ("nrm", C.c_void_p),
                 ("tan", C.c_void_p), ("albedo", C.c_void_p), ("obj", C.c_void_p), ("rough", C.c_void_p),
-                ("model", C.c_void_p), ("nmat", C.c_void_p)]
+                ("model", C.c_void_p), ("nmat", C.c_void_p), ("vtan", C.c_void_p)]
 
 
 class VoxParams(C.Structure):
@@ -196,9 +196,10 @@ class SceneArrays:
         self.rough = None if mesh.get("rough") is None else np.ascontiguousarray(mesh["rough"], np.float32)
         self.model = np.ascontiguousarray(model, np.float32).reshape(-1, 16)
         self.nmat = normal_matrices(self.model)
+        self.vtan = None if mesh.get("vtan") is None else np.ascontiguousarray(mesh["vtan"], np.float32)
         self.s = Scene(int(self.albedo.size), int(self.model.shape[0]), _p(self.pos), _p(self.nrm), _p(self.tan),
                        _p(self.albedo), _p(self.obj), None if self.rough is None else _p(self.rough),
-                       _p(self.model), _p(self.nmat))
+                       _p(self.model), _p(self.nmat), None if self.vtan is None else _p(self.vtan))
 
 
 def _light_array(lights):

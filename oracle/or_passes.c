@@ -953,6 +953,8 @@ void or_gbuffer(const or_scene *sc, const or_camera *cam, or_gbuf_texel *out) {
             mat3_vec3(Nm, sc->tan + (size_t)t * 3, Tw);
             for (int v = 0; v < 3; ++v) {
                 mat3_vec3(Nm, sc->nrm + (size_t)t * 9 + v * 3, Nv[v]);
+                /* phong.vert:48-52 per vertex, with the vertex's tangent when there is one */
+                if (sc->vtan) mat3_vec3(Nm, sc->vtan + (size_t)t * 9 + v * 3, Tw);
                 float dtn = dot3(Tw, Nv[v]);
                 float tt[3] = {Tw[0] - dtn * Nv[v][0], Tw[1] - dtn * Nv[v][1], Tw[2] - dtn * Nv[v][2]};
                 nrm3(tt, Tv[v]);

@@ -93,6 +93,8 @@ typedef struct or_scene {
     const float *rough;     /* [nobj] or NULL                     */
     const float *model;     /* [nobj][16]                         */
     const float *nmat;      /* [nobj][9] normal matrices          */
+    const float *vtan;      /* [ntri][3][3] per-corner (vertex) tangents
+                               (Mesh.cpp:190-197), or NULL: `tan` */
 } or_scene;
 
 typedef struct or_light {

@@ -1,5 +1,5 @@
 """Small OBJ/MTL files written by the tests (a cube with two materials, uvs
-and normals; a pentagon; files with faults)."""
+and normals; a pentagon; a smooth UV sphere; files with faults)."""
 from pathlib import Path
 
 CUBE_MTL = """# two materials
@@ -55,3 +55,27 @@ def write_cube(d: Path) -> Path:
     p = d / "cube.obj"
     p.write_text(CUBE_OBJ)
     return p
+
+
+def uv_sphere_obj(n_lat: int = 12, n_lon: int = 24, r: float = 1.0) -> str:
+    """A smooth UV sphere (y up): vertex (i, j) at latitude theta_i = pi i / n_lat
+    and longitude phi_j = 2 pi j / n_lon, shared by the faces around it, with
+    texture (u, v) = (j / n_lon, i / n_lat) and the unit position as normal.
+    The seam column j = n_lon repeats j = 0's positions with u = 1."""
+    import math
+    lines = []
+    for i in range(n_lat + 1):
+        th = math.pi * i / n_lat
+        for j in range(n_lon + 1):
+            ph = 2 * math.pi * j / n_lon
+            x, y, z = math.cos(ph) * math.sin(th), math.cos(th), math.sin(ph) * math.sin(th)
+            lines.append(f"v {r * x:.9f} {r * y:.9f} {r * z:.9f}")
+            lines.append(f"vn {x:.9f} {y:.9f} {z:.9f}")
+            lines.append(f"vt {j / n_lon:.9f} {i / n_lat:.9f}")
+    w = n_lon + 1
+    for i in range(n_lat):
+        for j in range(n_lon):
+            a, b, c, d = i * w + j + 1, i * w + j + 2, (i + 1) * w + j + 2, (i + 1) * w + j + 1
+            # counter-clockwise seen from outside
+            lines.append(f"f {a}/{a}/{a} {d}/{d}/{d} {c}/{c}/{c} {b}/{b}/{b}")
+    return "\n".join(lines) + "\n"

@@ -39,3 +39,31 @@ def test_obj_scene_passes_match_oracle(tmp_path):
         assert cov.mean() > 0.05 and np.isfinite(col).all() and col[cov, :3].sum() > 0
     finally:
         app.close()
+
+
+def test_obj_vertex_tangents_reach_the_gbuffer(tmp_path):
+    """A smooth UV sphere whose vertices are shared by their faces: the
+    G-buffer interpolates the per-vertex tangent sums (Mesh.cpp:190-197,
+    phong.vert:48-52), bit-exact against the oracle given the same tangents,
+    and differs from the image the per-face tangents would give."""
+    from obj_fixtures import uv_sphere_obj
+    p = tmp_path / "sphere.obj"
+    p.write_text(uv_sphere_obj(16, 32))
+    app = vct_amd.App(obj_path=p, device=0, width=256, height=192)
+    try:
+        print("obj sphere: G-buffer with vertex tangents", flush=True)
+        cap = run_frame_capture(app, 0)
+        fs = cap["fs"]
+        mesh = app.mesh()
+        assert mesh["vtan"] is not None
+        ref = ob.gbuffer(ob.SceneArrays(mesh, fs["model"]), fs["cam"])
+        got = cap["gbuf"]
+        cov = got[..., 11].view(np.uint32) != 0
+        assert cov.mean() > 0.1
+        assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), \
+            f"attributes max abs diff {np.abs(got - ref).max()}"
+        flat = dict(mesh, vtan=None)
+        ref_face = ob.gbuffer(ob.SceneArrays(flat, fs["model"]), fs["cam"])
+        assert np.abs(ref_face[cov] - got[cov]).max() > 1e-3
+    finally:
+        app.close()

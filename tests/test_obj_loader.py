@@ -80,3 +80,37 @@ def test_app_frames_the_loaded_mesh(oracle, tmp_path):
     assert st.total_voxel_fragments > 0
     assert np.array_equal(occ, occ[::-1, :, :]) and np.array_equal(occ, occ[:, :, ::-1])  # symmetric cube
     assert not occ[32, 32, 32]                                          # hollow
+
+
+def test_sphere_vertex_tangents_are_face_sums(tmp_path):
+    """Mesh.cpp:138-197: a vertex shared by several faces (same v/vt/vn
+    triple) gets the normalized sum of their tangents.  On a UV sphere whose u
+    runs with the longitude, that is the east direction (-sin phi, 0, cos phi)
+    up to the faces' curvature, the same for every corner of the vertex, and
+    different from its faces' own tangents."""
+    import math
+    from obj_fixtures import uv_sphere_obj
+    p = tmp_path / "sphere.obj"
+    p.write_text(uv_sphere_obj(12, 24))
+    m = vct_amd.load_obj(p)
+    vt, ft, pos = m["vtan"], m["tan"], m["pos"].astype(np.float64)
+    assert vt.shape == (m["pos"].shape[0], 3, 3)
+    assert np.allclose(np.linalg.norm(vt, axis=2), 1, atol=1e-6)
+    # the corners of one vertex (same position, away from the seam) share its tangent
+    keys = {}
+    for t in range(pos.shape[0]):
+        for c in range(3):
+            k = tuple(np.round(pos[t, c], 5))
+            keys.setdefault(k, []).append(vt[t, c])
+    shared = [np.asarray(v) for k, v in keys.items() if len(v) > 3 and abs(k[1]) < 0.95]
+    assert len(shared) > 100
+    # the seam column (u = 0 and u = 1) splits vertices: compare away from x = 1, z = 0
+    inner = [(k, v) for k, v in keys.items() if abs(k[1]) < 0.95 and not (k[0] > 0.2 and abs(k[2]) < 1e-4)]
+    for k, v in inner:
+        v = np.asarray(v)
+        assert np.allclose(v, v[0], atol=1e-6), k
+        phi = math.atan2(k[2], k[0])
+        east = np.array([-math.sin(phi), 0.0, math.cos(phi)])
+        assert np.dot(v[0], east) > 0.995, (k, v[0], east)
+    # per-face tangents of the faces around a vertex differ from its sum
+    assert np.abs(vt - ft[:, None, :]).max() > 0.05

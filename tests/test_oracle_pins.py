@@ -21,6 +21,11 @@ closed form that a plausible slip in the restatement fails.
   fp8 bricks     E4M3 with one FP32 scale per 4^3 brick, scale = max|v| / 448
                  (the largest E4M3 value), each value rounded to the nearest
                  code (ties to even) by a search over the code table
+  albedo chain   BOX2 of packUnorm4x8 bytes: round(255 mean(k/255)) per channel
+  accumulation   imageAtomicRGBA8Avg's running average, the fixed-point sums
+                 and normalizeVoxels on voxels with known fragment counts
+  G-buffer TBN   the per-vertex tangents blended by the hit's barycentrics,
+                 recovered from the interpolated position
 
 The expectations are computed here, in float64 from the formulas above, not
 by calling the oracle's own helpers.
@@ -488,3 +493,52 @@ def test_sum_mode_and_normalize_voxels_closed_form(oracle):
     assert np.all(b[8, 4:12, 4:12, 3] == 128)
     assert np.all(rad0[8, 4:12, 4:12, 3] == 0.5) and np.all(rad0[8, 4:12, 4:12, :3] == 0)
     assert (rad0[..., 3] > 0).sum() == 64 and st.unique_voxels == 64 and st.max_fragments_per_voxel == 4
+
+
+# ------------------------------------------------------------------ vertex tangents
+def test_gbuffer_interpolates_the_vertex_tangents(oracle, tmp_path):
+    """phong.vert:48-52 per vertex, then the rasterizer's interpolation: the
+    G-buffer tangent at a hit on triangle t is sum_v b_v T_v, where
+    T_v = normalize(t_v - (t_v . n_v) n_v) from corner v's tangent t_v (the
+    vertex tangent sum when the mesh has one) and normal n_v, and b are the
+    barycentrics of the hit, recovered here from the G-buffer's position
+    (sum_v b_v P_v with sum_v b_v = 1) in float64.  On a smooth UV sphere the
+    corners of a triangle carry different tangents, so a face tangent or a
+    rotated corner order is off by ~1e-1."""
+    import vct_amd
+    from obj_fixtures import uv_sphere_obj
+    p = tmp_path / "sphere.obj"
+    p.write_text(uv_sphere_obj(12, 24))
+    app = vct_amd.App(obj_path=p, device=-1, width=160, height=120)
+    try:
+        fs = app.frame_state(0)
+        mesh = app.mesh()
+    finally:
+        app.close()
+    assert mesh["vtan"] is not None
+    assert np.array_equal(fs["model"].reshape(-1), np.eye(4, dtype=np.float32).reshape(-1))
+    gb = oracle.gbuffer(oracle.SceneArrays(mesh, fs["model"]), fs["cam"])
+    cov = gb[..., 11].view(np.uint32) != 0
+    assert cov.sum() > 2000
+    g = gb[cov].astype(np.float64)
+    tri = gb[cov][:, 15].view(np.uint32).astype(np.int64)
+    P = mesh["pos"][tri].astype(np.float64)            # [n][3 corners][3]
+    A = np.concatenate([P.transpose(0, 2, 1), np.ones((len(tri), 1, 3))], 1)  # [n][4][3]
+    rhs = np.concatenate([g[:, 0:3], np.ones((len(tri), 1))], 1)[..., None]
+    bary = np.stack([np.linalg.lstsq(A[k], rhs[k], rcond=None)[0][:, 0] for k in range(len(tri))])
+    assert np.all(bary > -1e-3)
+
+    def expected(tv):
+        n = mesh["nrm"][tri].astype(np.float64)
+        t = tv.astype(np.float64)
+        o = t - np.sum(t * n, 2, keepdims=True) * n
+        o /= np.linalg.norm(o, axis=2, keepdims=True)
+        return np.sum(bary[..., None] * o, 1)
+
+    got = g[:, 8:11]
+    assert np.abs(got - expected(mesh["vtan"][tri])).max() < 2e-4
+    face = np.repeat(mesh["tan"][tri][:, None, :], 3, 1)
+    assert np.abs(got - expected(face)).max() > 3e-2
+    assert np.abs(got - expected(mesh["vtan"][tri][:, [1, 2, 0]])).max() > 3e-2
+    flat = oracle.gbuffer(oracle.SceneArrays(dict(mesh, vtan=None), fs["model"]), fs["cam"])
+    assert np.abs(flat[cov][:, 8:11] - expected(face)).max() < 2e-4

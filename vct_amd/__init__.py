@@ -103,15 +103,19 @@ class Context:
         self.check(self.lib.vct_get_grid(self.h, C.byref(d)))
         return d
 
-    def set_scene(self, pos, nrm, tan, albedo, obj, nobj, rough=None):
+    def set_scene(self, pos, nrm, tan, albedo, obj, nobj, rough=None, vtan=None):
+        """vct_set_scene; `vtan` [num_tris][3][3] per-corner tangents or None."""
         pos = np.ascontiguousarray(pos, np.float32)
         nrm = np.ascontiguousarray(nrm, np.float32)
         tan = np.ascontiguousarray(tan, np.float32)
         albedo = np.ascontiguousarray(albedo, np.uint32)
         obj = np.ascontiguousarray(obj, np.uint32)
         rough_a = None if rough is None else np.ascontiguousarray(rough, np.float32)
+        vtan_a = None if vtan is None else np.ascontiguousarray(vtan, np.float32)
+        if vtan_a is not None and vtan_a.size != albedo.size * 9:
+            raise ValueError("vtan must be [num_tris][3][3]")
         m = abi.Mesh(int(albedo.size), int(nobj), _ptr(pos), _ptr(nrm), _ptr(tan), _ptr(albedo), _ptr(obj),
-                     None if rough_a is None else _ptr(rough_a))
+                     None if rough_a is None else _ptr(rough_a), None if vtan_a is None else _ptr(vtan_a))
         self.check(self.lib.vct_set_scene(self.h, C.byref(m)))
 
     def set_transforms(self, model: np.ndarray):
@@ -280,6 +284,7 @@ def load_obj(path) -> dict:
         return dict(pos=arr(m.positions, n * 9, np.float32).reshape(n, 3, 3),
                     nrm=arr(m.normals, n * 9, np.float32).reshape(n, 3, 3),
                     tan=arr(m.tangents, n * 3, np.float32).reshape(n, 3),
+                    vtan=None if not m.vertex_tangents else arr(m.vertex_tangents, n * 9, np.float32).reshape(n, 3, 3),
                     albedo=arr(m.albedo, n, np.uint32), obj=arr(m.object, n, np.uint32), nobj=1,
                     rough=None if not m.roughness else arr(m.roughness, 1, np.float32),
                     bmin=bmin, bmax=bmax, materials=nm.value)
@@ -342,7 +347,10 @@ class App:
         obj = np.empty(nt, np.uint32)
         rough = np.empty(no, np.float32)
         has = self.lib.vct_app_get_mesh(self.h, _ptr(pos), _ptr(nrm), _ptr(tan), _ptr(alb), _ptr(obj), _ptr(rough))
-        return dict(pos=pos, nrm=nrm, tan=tan, albedo=alb, obj=obj, nobj=no, rough=rough if has == 1 else None)
+        vtan = np.empty((nt, 3, 3), np.float32)
+        has_vtan = self.lib.vct_app_get_vertex_tangents(self.h, _ptr(vtan))
+        return dict(pos=pos, nrm=nrm, tan=tan, albedo=alb, obj=obj, nobj=no, rough=rough if has == 1 else None,
+                    vtan=vtan if has_vtan == 1 else None)
 
     def frame_state(self, frame: int = 0) -> dict:
         _, no, nl = self.sizes()

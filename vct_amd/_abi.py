@@ -53,7 +53,7 @@ class GridDesc(C.Structure):
 class Mesh(C.Structure):
     _fields_ = [("num_tris", C.c_int), ("num_objects", C.c_int), ("positions", C.c_void_p),
                 ("normals", C.c_void_p), ("tangents", C.c_void_p), ("albedo", C.c_void_p),
-                ("object", C.c_void_p), ("roughness", C.c_void_p)]
+                ("object", C.c_void_p), ("roughness", C.c_void_p), ("vertex_tangents", C.c_void_p)]
 
 
 class Light(C.Structure):
@@ -193,6 +193,7 @@ APP_FUNCS = [
     ("vct_app_get_config", C.c_int, [C.c_void_p, C.POINTER(AppConfig)]),
     ("vct_app_scene_sizes", C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     ("vct_app_get_mesh", C.c_int, [C.c_void_p] + [C.c_void_p] * 6),
+    ("vct_app_get_vertex_tangents", C.c_int, [C.c_void_p, C.c_void_p]),
     ("vct_app_frame_state", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.POINTER(Light),
                                       C.POINTER(VoxelizeParams), C.POINTER(InjectParams), C.POINTER(MipParams),
                                       C.POINTER(Camera), C.POINTER(TraceParams)]),

@@ -33,10 +33,16 @@ struct TimedPair {
 struct vct_ctx {
     vct_grid_desc desc{};
     hipStream_t stream = nullptr;
+    // a second stream for work inside one pass that does not depend on the
+    // rest of that pass (the albedo chain beside the radiance chain); forked
+    // from and joined back to `stream` by events, so stream order holds
+    hipStream_t aux = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     std::string err;
     // scene
     int ntri = 0, nobj = 0, has_rough = 0, have_transforms = 0;
     float *d_pos = nullptr, *d_nrm = nullptr, *d_tan = nullptr, *d_rough = nullptr;
+    float *d_vtan = nullptr;  // per-corner tangents (vct_mesh.vertex_tangents), or null
     uint32_t *d_albedo = nullptr, *d_obj = nullptr;
     float *d_model = nullptr, *d_nmat = nullptr;
     // pinned staging ring of the per-frame transform upload: slot k holds the
@@ -357,6 +363,12 @@ int vct_create(vct_ctx **out, const vct_grid_desc *desc) {
         delete c;
         return fail(nullptr, VCT_ERR_HIP, "hipStreamCreate failed");
     }
+    if (hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->join, hipEventDisableTiming) != hipSuccess) {
+        vct_destroy(c);
+        return fail(nullptr, VCT_ERR_HIP, "hipStreamCreate (aux) failed");
+    }
     if (hipMalloc((void **)&c->d_partials, kStatPartials * 4 * sizeof(uint32_t)) != hipSuccess ||
         hipMalloc((void **)&c->d_stats, sizeof(vct_stats)) != hipSuccess ||
         hipMemset(c->d_stats, 0, sizeof(vct_stats)) != hipSuccess) {
@@ -383,11 +395,13 @@ int vct_create(vct_ctx **out, const vct_grid_desc *desc) {
 void vct_destroy(vct_ctx *c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->aux) (void)hipStreamSynchronize(c->aux);
     free_grids(c);
     dfree(c->d_pos);
     dfree(c->d_nrm);
     dfree(c->d_tan);
     dfree(c->d_rough);
+    dfree(c->d_vtan);
     dfree(c->d_albedo);
     dfree(c->d_obj);
     dfree(c->d_model);
@@ -435,6 +449,10 @@ void vct_destroy(vct_ctx *c) {
         (void)hipEventDestroy(tp.b);
     }
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
+    if (c->aux) (void)hipStreamSynchronize(c->aux);
+    if (c->fork) (void)hipEventDestroy(c->fork);
+    if (c->join) (void)hipEventDestroy(c->join);
+    if (c->aux) (void)hipStreamDestroy(c->aux);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -451,6 +469,7 @@ int vct_remake(vct_ctx *c, int dim, int levels) {
         return rc;
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->aux));
     free_grids(c);
     c->desc = nd;
     rc = alloc_grids(c);
@@ -518,6 +537,7 @@ int vct_set_scene(vct_ctx *c, const vct_mesh *m) {
     dfree(c->d_albedo);
     dfree(c->d_obj);
     dfree(c->d_rough);
+    dfree(c->d_vtan);
     dfree(c->d_model);
     dfree(c->d_nmat);
     free_xf_ring(c);  // the stream is idle: no copy reads the ring
@@ -539,6 +559,10 @@ int vct_set_scene(vct_ctx *c, const vct_mesh *m) {
         HIPCHK(c, hipMemcpy(c->d_tan, m->tangents, T * 3 * 4, hipMemcpyHostToDevice));
         HIPCHK(c, hipMemcpy(c->d_albedo, m->albedo, T * 4, hipMemcpyHostToDevice));
         HIPCHK(c, hipMemcpy(c->d_obj, m->object, T * 4, hipMemcpyHostToDevice));
+    }
+    if (T && m->vertex_tangents) {
+        HIPCHK(c, dalloc(&c->d_vtan, T * 9 * 4));
+        HIPCHK(c, hipMemcpy(c->d_vtan, m->vertex_tangents, T * 9 * 4, hipMemcpyHostToDevice));
     }
     c->has_rough = m->roughness != nullptr;
     if (c->has_rough) {
@@ -676,6 +700,7 @@ static DevScene dev_scene(const vct_ctx *c) {
     s.albedo = c->d_albedo;
     s.obj = c->d_obj;
     s.rough = c->has_rough ? c->d_rough : nullptr;
+    s.vtan = c->d_vtan;
     s.model = c->d_model;
     s.nmat = c->d_nmat;
     return s;
@@ -855,10 +880,18 @@ int vct_mip(vct_ctx *c, const vct_mip_params *p) {
                     p->kernel_mode);
     if (!c->have_radiance) return fail(c, VCT_ERR_STATE, "vct_mip: inject first");
     PassTimer pt(c, P_MIP);
+    // the voxelColor chain, filtered by the same mipmap pass (Application.cpp:
+    // 904-917), depends only on the albedo grid: it runs on the aux stream
+    // beside the radiance chain, from a fork after everything enqueued so far
+    // to a join before anything enqueued later
+    if (p->albedo_chain) {
+        HIPCHK(c, hipEventRecord(c->fork, c->stream));
+        HIPCHK(c, hipStreamWaitEvent(c->aux, c->fork, 0));
+        LAUNCHCHK(c, launch_albedo_chain(c->d_grid_albedo, c->albedo_lv, c->desc.dim, c->desc.levels, c->aux));
+        HIPCHK(c, hipEventRecord(c->join, c->aux));
+    }
     LAUNCHCHK(c, launch_mip(c->pyr, c->stream));
-    // the voxelColor chain, filtered by the same mipmap pass (Application.cpp:904-917)
-    if (p->albedo_chain)
-        LAUNCHCHK(c, launch_albedo_chain(c->d_grid_albedo, c->albedo_lv, c->desc.dim, c->desc.levels, c->stream));
+    if (p->albedo_chain) HIPCHK(c, hipStreamWaitEvent(c->stream, c->join, 0));
     c->have_mips = true;
     return VCT_OK;
 }

@@ -272,6 +272,8 @@ __global__ __launch_bounds__(256) void k_gbuf_resolve(DevScene sc, CamMats cm, c
     float Nv[3][3], Tv[3][3], Bv[3][3];
     for (int v = 0; v < 3; ++v) {
         mat3_vec3(Nm, sc.nrm + (size_t)t * 9 + v * 3, Nv[v]);
+        // phong.vert:48-52 per vertex: T = normalMatrix * tangent, re-orthogonalized against N
+        if (sc.vtan) mat3_vec3(Nm, sc.vtan + (size_t)t * 9 + v * 3, Tw);
         float dtn = dot3(Tw, Nv[v]);
         float tt[3] = {Tw[0] - dtn * Nv[v][0], Tw[1] - dtn * Nv[v][1], Tw[2] - dtn * Nv[v][2]};
         nrm3(tt, Tv[v]);

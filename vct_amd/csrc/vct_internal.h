@@ -98,6 +98,7 @@ struct DevScene {
     const float *pos;       // [ntri][9]
     const float *nrm;       // [ntri][9]
     const float *tan;       // [ntri][3]
+    const float *vtan;      // [ntri][3][3] per-corner (vertex) tangents, or null
     const uint32_t *albedo; // [ntri]
     const uint32_t *obj;    // [ntri]
     const float *rough;     // [nobj] or null

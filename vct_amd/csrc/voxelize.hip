@@ -223,55 +223,64 @@ struct VoxFrag {
         nrm3(nr, nn);
         const float ne0 = (nn[0] + 1.0f) * 0.5f, ne1 = (nn[1] + 1.0f) * 0.5f, ne2 = (nn[2] + 1.0f) * 0.5f;
         if (lighting) {
-            // Two lights per pass: both lights' shadow taps are fetched before
-            // either is compared, so their memory latency overlaps.  Each
-            // light's term and the sum run in light order, as one light at a
-            // time would.
+            // Two lights per pass: each light's term first, then the shadow
+            // taps of both lights fetched before either is compared, so their
+            // memory latency overlaps -- and only for a light whose term is
+            // non-zero in some lane of the wavefront: a zero term (facing
+            // away, beyond a point light's range) stays exactly zero whatever
+            // the shadow factor (finite, in [0, 1]).  Each light's term and
+            // the sum run in light order, as one light at a time would.
             float fin[3] = {0.0f, 0.0f, 0.0f};
             for (int i = 0; i < nlights; i += 2) {
                 ShadowTaps st[2];
-                bool sc[2];
+                bool fetch[2];
+                float lit[2][3];
 #pragma unroll
                 for (int h = 0; h < 2; ++h) {
-                    sc[h] = false;
+                    fetch[h] = false;
+                    lit[h][0] = lit[h][1] = lit[h][2] = 0.0f;
                     if (i + h >= nlights) continue;
                     const DevLight &L = vox_lights[i + h];
-                    sc[h] = L.enabled && L.shadow_caster && L.shadowquad;
-                    if (sc[h]) {
-                        float w4[4] = {wp[0], wp[1], wp[2], 1.0f}, lsp[4];
-                        mat4_vec4(L.ls, w4, lsp);
-                        shadow_taps(lsp, L.shadowquad, (int)L.shadow_size, st[h]);
+                    if (!L.enabled) continue;
+                    if (L.type == 0u) {
+                        float d[3] = {L.position[0] - wp[0], L.position[1] - wp[1], L.position[2] - wp[2]};
+                        float dn[3];
+                        nrm3(d, dn);
+                        const float ndl = __builtin_fmaxf(0.0f, dot3(nn, dn));
+                        for (int k = 0; k < 3; ++k) lit[h][k] = ((color[k] * L.intensity) * L.color[k]) * ndl;
+                        const float dist = __builtin_sqrtf(dot3(d, d));
+                        const float e0 = 0.75f * L.range, e1 = L.range;
+                        const float tt = clampf((dist - e0) / (e1 - e0), 0.0f, 1.0f);
+                        const float ss = (tt * tt) * (3.0f - 2.0f * tt);
+                        const float att = 1.0f - ss;
+                        for (int k = 0; k < 3; ++k) lit[h][k] = lit[h][k] * att;
+                    } else {
+                        float nd[3] = {-L.direction[0], -L.direction[1], -L.direction[2]}, ndn[3];
+                        nrm3(nd, ndn);
+                        const float ndl = __builtin_fmaxf(0.0f, dot3(nn, ndn));
+                        for (int k = 0; k < 3; ++k) lit[h][k] = ((color[k] * L.color[k]) * L.intensity) * ndl;
                     }
+                    const bool nz = lit[h][0] != 0.0f || lit[h][1] != 0.0f || lit[h][2] != 0.0f;
+                    fetch[h] = L.shadow_caster && L.shadowquad && __ballot(nz) != 0;  // wave-uniform
+                }
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    if (!fetch[h]) continue;
+                    const DevLight &L = vox_lights[i + h];
+                    float w4[4] = {wp[0], wp[1], wp[2], 1.0f}, lsp[4];
+                    mat4_vec4(L.ls, w4, lsp);
+                    shadow_taps(lsp, L.shadowquad, (int)L.shadow_size, st[h]);
                 }
 #pragma unroll
                 for (int h = 0; h < 2; ++h) {
                     if (i + h >= nlights) continue;
                     const DevLight &L = vox_lights[i + h];
                     if (!L.enabled) continue;
-                    float lit[3];
-                    if (L.type == 0u) {
-                        float d[3] = {L.position[0] - wp[0], L.position[1] - wp[1], L.position[2] - wp[2]};
-                        float dn[3];
-                        nrm3(d, dn);
-                        const float ndl = __builtin_fmaxf(0.0f, dot3(nn, dn));
-                        for (int k = 0; k < 3; ++k) lit[k] = ((color[k] * L.intensity) * L.color[k]) * ndl;
-                        const float dist = __builtin_sqrtf(dot3(d, d));
-                        const float e0 = 0.75f * L.range, e1 = L.range;
-                        const float tt = clampf((dist - e0) / (e1 - e0), 0.0f, 1.0f);
-                        const float ss = (tt * tt) * (3.0f - 2.0f * tt);
-                        const float att = 1.0f - ss;
-                        for (int k = 0; k < 3; ++k) lit[k] = lit[k] * att;
-                    } else {
-                        float nd[3] = {-L.direction[0], -L.direction[1], -L.direction[2]}, ndn[3];
-                        nrm3(nd, ndn);
-                        const float ndl = __builtin_fmaxf(0.0f, dot3(nn, ndn));
-                        for (int k = 0; k < 3; ++k) lit[k] = ((color[k] * L.color[k]) * L.intensity) * ndl;
-                    }
-                    if (sc[h]) {
+                    if (fetch[h]) {
                         const float sf = 1.0f - shadow_count(st[h], (int)L.shadow_size);
-                        for (int k = 0; k < 3; ++k) lit[k] = lit[k] * sf;
+                        for (int k = 0; k < 3; ++k) lit[h][k] = lit[h][k] * sf;
                     }
-                    for (int k = 0; k < 3; ++k) fin[k] = fin[k] + lit[k];
+                    for (int k = 0; k < 3; ++k) fin[k] = fin[k] + lit[h][k];
                 }
             }
             for (int k = 0; k < 3; ++k) color[k] = clampf(fin[k], 0.0f, 1.0f);

@@ -47,6 +47,7 @@ struct Rng {
 
 struct Mesh {
     std::vector<float> pos, nrm, tan, rough;
+    std::vector<float> vtan;  // per-corner (vertex) tangents, [ntri][3][3], or empty
     std::vector<uint32_t> albedo, obj;
     int nobj = 0;
     int ntri() const { return (int)albedo.size(); }
@@ -580,6 +581,7 @@ static int app_finish(vct_app *a, vct_app **out) {
     m.albedo = a->mesh.albedo.data();
     m.object = a->mesh.obj.data();
     m.roughness = a->mesh.rough.empty() ? nullptr : a->mesh.rough.data();
+    m.vertex_tangents = a->mesh.vtan.empty() ? nullptr : a->mesh.vtan.data();
     if ((rc = vct_set_scene(a->ctx, &m)) || (rc = vct_set_lights(a->ctx, a->lights.data(), (int)a->lights.size()))) {
         vct_app_destroy(a);
         return rc;
@@ -605,6 +607,7 @@ int vct_app_create_from_obj(vct_app **out, const vct_app_config *cfg_in, const c
     a->mesh.pos.assign(m.positions, m.positions + (size_t)m.num_tris * 9);
     a->mesh.nrm.assign(m.normals, m.normals + (size_t)m.num_tris * 9);
     a->mesh.tan.assign(m.tangents, m.tangents + (size_t)m.num_tris * 3);
+    if (m.vertex_tangents) a->mesh.vtan.assign(m.vertex_tangents, m.vertex_tangents + (size_t)m.num_tris * 9);
     a->mesh.albedo.assign(m.albedo, m.albedo + m.num_tris);
     a->mesh.obj.assign(m.object, m.object + m.num_tris);
     if (m.roughness) a->mesh.rough.assign(m.roughness, m.roughness + 1);
@@ -702,6 +705,13 @@ int vct_app_get_mesh(const vct_app *a, float *pos, float *nrm, float *tan, uint3
     if (obj) std::memcpy(obj, m.obj.data(), m.obj.size() * 4);
     if (rough && !m.rough.empty()) std::memcpy(rough, m.rough.data(), m.rough.size() * 4);
     return m.rough.empty() ? 0 : 1;
+}
+
+int vct_app_get_vertex_tangents(const vct_app *a, float *vtan) {
+    if (!a) return VCT_ERR_INVALID;
+    if (a->mesh.vtan.empty()) return 0;
+    if (vtan) std::memcpy(vtan, a->mesh.vtan.data(), a->mesh.vtan.size() * 4);
+    return 1;
 }
 
 static void frame_models(const vct_app *a, int frame, float *out) {
@@ -814,6 +824,7 @@ int vct_app_surface_points(vct_app *a, int frame, int n, uint64_t seed, vct_gbuf
         for (int v = 0; v < 3; ++v) {
             world(t, v, W[v]);
             m3(Nm, m.nrm.data() + (size_t)t * 9 + v * 3, Nv[v]);
+            if (!m.vtan.empty()) m3(Nm, m.vtan.data() + (size_t)t * 9 + v * 3, Tw);
             const float dtn = (Tw[0] * Nv[v][0] + Tw[1] * Nv[v][1]) + Tw[2] * Nv[v][2];
             float tt[3] = {Tw[0] - dtn * Nv[v][0], Tw[1] - dtn * Nv[v][1], Tw[2] - dtn * Nv[v][2]};
             nrm3(tt, Tv[v]);

@@ -5,17 +5,23 @@
 //     1-based indices, negative ones relative to the end of the list so far;
 //   * texture v flipped to 1 - v (Mesh.cpp:150);
 //   * the per-face tangent of learnopengl's normal-mapping rule from the
-//     position and uv deltas (Mesh.cpp:174-189), normalized;
+//     position and uv deltas (Mesh.cpp:174-189), normalized, and the
+//     per-vertex tangents the reference draws with: vertices are shared by
+//     their (v, vt, vn) index triple (Mesh.cpp:138-167), every face adds its
+//     unnormalized tangent to its three vertices, and each vertex's sum is
+//     normalized (Mesh.cpp:190-197);
 //   * the diffuse colour Kd of the face's material (usemtl/newmtl) as RGBA8,
 //     and the PBR roughness Pr per material.
 // Differences, on purpose: polygons become triangle fans (tinyobjloader's
 // default triangulation for convex faces); a face without vn gets its face
 // normal and one without usable uvs an orthonormal tangent from the normal
-// (the reference would produce NaN there); textures (map_Kd, map_Pr, ...) are
+// (the reference would produce NaN there) and adds nothing to its vertices'
+// tangent sums (a vertex whose sum is zero keeps its face's tangent); textures (map_Kd, map_Pr, ...) are
 // not decoded -- the image has no image decoder -- so a face's albedo is its
 // material's Kd, white without a material, and its roughness Pr (0.5 if the
 // material has none).  One object: the file is one actor, as in Scene.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -31,6 +37,7 @@
 
 struct vct_obj {
     std::vector<float> pos, nrm, tan;
+    std::vector<float> vtan;  // [ntri][3][3]: the normalized per-vertex tangent sums
     std::vector<uint32_t> albedo, object;
     std::vector<float> rough;  // one value: the file is one object
     int nmaterials = 0;
@@ -125,6 +132,11 @@ int vct_obj_load(const char *path, vct_obj **out, char *err, int errlen) {
     bool any_pr = false;
     float pr = 0.5f;
     vct_obj *o = new vct_obj();
+    // shared vertices (Mesh.cpp:138-167): index triple -> slot, and each
+    // slot's tangent sum in float, as glm::vec3 accumulates it
+    std::map<std::array<int, 3>, int> vmap;
+    std::vector<float> vacc;
+    std::vector<int> corner_slot;  // [ntri][3]
     std::string line;
     int lineno = 0;
     while (std::getline(f, line)) {
@@ -233,6 +245,17 @@ int vct_obj_load(const char *path, vct_obj **out, char *err, int errlen) {
                     tg[0] = ax[1] * fn[2] - ax[2] * fn[1];
                     tg[1] = ax[2] * fn[0] - ax[0] * fn[2];
                     tg[2] = ax[0] * fn[1] - ax[1] * fn[0];
+                    if (tg[0] == 0.0f && tg[1] == 0.0f && tg[2] == 0.0f) tg[0] = 1.0f;  // zero-area face: no normal
+                }
+                for (int j = 0; j < 3; ++j) {
+                    const size_t q = c3[j];
+                    const std::array<int, 3> key = {iv[q], it_[q], in[q]};
+                    auto ins = vmap.emplace(key, (int)(vacc.size() / 3));
+                    if (ins.second) vacc.insert(vacc.end(), {0.0f, 0.0f, 0.0f});
+                    const int slot = ins.first->second;
+                    corner_slot.push_back(slot);
+                    if (tan_ok)  // Mesh.cpp:190-193: the unnormalized face tangent
+                        for (int d = 0; d < 3; ++d) vacc[(size_t)slot * 3 + d] += tg[d];
                 }
                 normalize3(tg);
                 for (int j = 0; j < 3; ++j) {
@@ -251,6 +274,18 @@ int vct_obj_load(const char *path, vct_obj **out, char *err, int errlen) {
         delete o;
         return VCT_ERR_INVALID;
     }
+    // Mesh.cpp:195-197: every vertex's tangent sum normalized
+    const size_t ntri = o->albedo.size();
+    o->vtan.resize(ntri * 9);
+    for (size_t t = 0; t < ntri; ++t)
+        for (int j = 0; j < 3; ++j) {
+            const float *a = &vacc[(size_t)corner_slot[t * 3 + j] * 3];
+            float tv[3] = {a[0], a[1], a[2]};
+            const float l2 = (tv[0] * tv[0] + tv[1] * tv[1]) + tv[2] * tv[2];
+            if (l2 > 0.0f && std::isfinite(l2)) normalize3(tv);
+            else std::memcpy(tv, &o->tan[t * 3], 12);
+            std::memcpy(&o->vtan[t * 9 + (size_t)j * 3], tv, 12);
+        }
     o->nmaterials = (int)mats.size();
     if (any_pr) o->rough.push_back(pr);
     for (int k = 0; k < 3; ++k) {
@@ -276,6 +311,7 @@ int vct_obj_mesh(const vct_obj *o, vct_mesh *m) {
     m->albedo = o->albedo.data();
     m->object = o->object.data();
     m->roughness = o->rough.empty() ? nullptr : o->rough.data();
+    m->vertex_tangents = o->vtan.data();
     return VCT_OK;
 }
 
