@@ -4,10 +4,14 @@
   features   k_features vs a numpy restatement of the input definition
   gradients  one k_mlp_train minibatch vs the FP64 reference network with BF16
              rounding at the same operand points (mlp_ref, bf16=True)
-  training   the loss falls; the trained network beats the plain 6-cone
-             gather against the 64-cone reference
+  training   the loss falls
+  held out   the network at its configured size (2^20 points, 20,000 steps)
+             within 2% pooled mean relative error of the 64-cone reference on
+             16 cameras that gave no training point, and at least ten times
+             closer to it than the plain 6-cone gather
   inference  k_learned output vs the reference network on the same inputs;
-             fallback tiles carry the plain tracer's result
+             fallback tiles carry the plain tracer's result (k_learned_fallback)
+  reference  vct_reference_gather follows the G-buffer size
 """
 import ctypes as C
 
@@ -73,6 +77,25 @@ def test_labels_match_fp64_oracle(prepared):
     assert out[:, :3].sum() > 0
     assert rel_err(lab[:256, :3], out[:, :3]) < 1e-4
     assert np.abs(lab[:256, 3] - out[:, 3]).max() < 1e-3
+
+
+def test_reference_gather_follows_the_image_size(prepared):
+    """vct_reference_gather on uploaded G-buffers of growing size (16x16, then
+    32x32): its buffer follows the image, and each result is the 64-cone
+    label of the same points from vct_learned_prepare."""
+    app, ctx, pts = prepared["app"], prepared["ctx"], prepared["pts"]
+    lab = ctx.download(abi.VCT_BUF_LABELS, 0, np.float32).reshape(-1, 4)
+    cam = abi.Camera.from_buffer_copy(app.frame_state(0)["cam"])
+    for side in (16, 32):
+        n = side * side
+        cam.width = cam.height = side
+        print(f"reference gather on a {side}x{side} G-buffer", flush=True)
+        ctx.gbuffer(cam)
+        ctx.upload(abi.VCT_BUF_GBUFFER, 0, np.ascontiguousarray(pts[:n]))
+        ctx.reference_gather()
+        ref = ctx.download(abi.VCT_BUF_REFERENCE, 0, np.float32).reshape(-1, 4)
+        assert ref.shape == (n, 4)
+        assert np.array_equal(ref, lab[:n])
 
 
 def _features_ref(pts, levels, g):
@@ -159,26 +182,45 @@ def test_gradient_matches_reference(prepared):
     assert np.abs(w1[:-1] - want).max() < 1e-6
 
 
-def test_training_reduces_loss_and_beats_plain_gather(prepared):
-    app, ctx = prepared["app"], prepared["ctx"]
+def test_training_reduces_loss(prepared):
+    ctx = prepared["ctx"]
     loss = ctx.learned_train(steps=600, batch=1024, lr=2e-3, seed=3, reset=True)
     assert np.isfinite(loss).all()
     assert loss[-50:].mean() < 0.5 * loss[:10].mean()
-    # learned vs plain 6-cone gather, both measured against the 64-cone reference
-    fs = app.frame_state(0)
-    ctx.gbuffer(fs["cam"])
-    tr = fs["trace"]
-    tr.enable_reflections = 0
-    ctx.trace(tr)
-    plain = ctx.download(abi.VCT_BUF_INDIRECT, 0, np.float32).reshape(-1, 4)
-    ctx.reference_gather()
-    ref = ctx.download(abi.VCT_BUF_REFERENCE, 0, np.float32).reshape(-1, 4)
-    ctx.trace_learned(tr, 1e9)  # no fallback
-    learned = ctx.download(abi.VCT_BUF_INDIRECT, 0, np.float32).reshape(-1, 4)
-    e_plain = rel_err(plain[:, :3], ref[:, :3])
-    e_learned = rel_err(learned[:, :3], ref[:, :3])
-    print(f"mean relative error vs 64-cone reference: plain {e_plain:.4f}, learned {e_learned:.4f}")
-    assert e_learned < e_plain
+
+
+def test_full_size_network_meets_the_bar_on_held_out_views():
+    """The gain of the trained network, at its configured size: 2^20 training
+    points (three quarters from the G-buffers of 32 seeded training cameras),
+    Adam at lr 1e-3 for 20,000 steps of 4096; scored on 16 seeded cameras that
+    gave no training point (480x270 each) against the 64-cone FP64-accumulated
+    reference.  The pooled mean relative error is at most 2%, and at least
+    ten times below that of the plain 6-cone gather on the same views."""
+    from vct_amd import learned as L
+
+    app = make_app(1, width=1920, height=1080)
+    try:
+        ctx = app.ctx
+        app.build_radiance(0)
+        print("learned: training points and labels", flush=True)
+        pts = L.training_points(app, 1 << 20, 0.75, 32)
+        ctx.learned_set_points(pts)
+        ctx.learned_prepare()
+        print("learned: 20000 Adam steps", flush=True)
+        loss = ctx.learned_train(steps=20000, batch=4096, lr=1e-3, seed=1, reset=True)
+        assert np.isfinite(loss).all()
+        tr = app.frame_state(0)["trace"]
+        tr.enable_reflections = 0
+        views = L.held_out_views(app, L.TEST_SEEDS)
+        refs = {}
+        print("learned: held-out scoring", flush=True)
+        e_learned = L.pooled_error(app, tr, views, refs=refs)["pooled"]
+        e_plain = L.pooled_error(app, tr, views, plain=True, refs=refs)["pooled"]
+        print(f"held-out pooled error vs the 64-cone reference: learned {e_learned:.4f}, plain {e_plain:.4f}")
+        assert e_learned <= 0.02
+        assert e_learned * 10 <= e_plain
+    finally:
+        app.close()
 
 
 def test_inference_matches_reference_network_and_fallback(prepared):

@@ -64,6 +64,6 @@ def test_obj_vertex_tangents_reach_the_gbuffer(tmp_path):
             f"attributes max abs diff {np.abs(got - ref).max()}"
         flat = dict(mesh, vtan=None)
         ref_face = ob.gbuffer(ob.SceneArrays(flat, fs["model"]), fs["cam"])
-        assert np.abs(ref_face[cov] - got[cov]).max() > 1e-3
+        assert np.abs(ref_face[cov][:, 8:11] - got[cov][:, 8:11]).max() > 1e-3  # the tangent columns
     finally:
         app.close()

@@ -107,7 +107,7 @@ struct vct_ctx {
     float *d_loss = nullptr;
     int loss_cap = 0, last_train_blocks = 0;
     int adam_t = 0;  // Adam steps taken since the weights were initialised or uploaded
-    uint32_t *d_tile_count = nullptr;  // fallback tiles of the last learned gather
+    uint32_t *d_tile_count = nullptr;  // fallback tiles of the last learned gather: count, then the tiles
     float4 *d_reference = nullptr;
     // timing
     bool timing = true;
@@ -904,6 +904,8 @@ static int ensure_camera(vct_ctx *c, int w, int h) {
     dfree(c->d_ind);
     dfree(c->d_spec);
     dfree(c->d_color);
+    dfree(c->d_reference);  // sized for the old image: the next vct_reference_gather allocates it anew
+    dfree(c->d_tile_count);  // likewise for the next vct_trace_learned
     const size_t np = (size_t)w * h;
     HIPCHK(c, dalloc(&c->d_gbuf, np * sizeof(vct_gbuffer_texel)));
     HIPCHK(c, dalloc(&c->d_vis, np * 8));
@@ -1401,7 +1403,8 @@ int vct_trace_learned(vct_ctx *c, const vct_trace_params *p, float thr, int *fal
     if (!c->have_weights) return fail(c, VCT_ERR_STATE, "vct_trace_learned: no trained weights");
     if (p->warp_texture)
         return fail(c, VCT_ERR_INVALID, "vct_trace_learned: the learned gather is trained on the unwarped grid");
-    if (!c->d_tile_count) HIPCHK(c, dalloc(&c->d_tile_count, 4));
+    if (!c->d_tile_count)  // sized for the image: ensure_camera frees it
+        HIPCHK(c, dalloc(&c->d_tile_count, 4 * (1 + (size_t)((c->gw + 7) / 8) * ((c->gh + 7) / 8))));
     TraceLaunch t = make_trace_launch(c, p);
     t.enable_reflections = 0;
     PassTimer pt(c, P_LEARNED);

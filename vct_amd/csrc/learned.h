@@ -35,9 +35,10 @@ hipError_t launch_mlp_train_step(MlpBuffers &B, const uint16_t *X, const float4 
 size_t mlp_train_smem_bytes();
 // Learned diffuse gather over the G-buffer of t into t.out_ind; tiles whose
 // predicted-luminance variance exceeds threshold are traced with the 6-cone
-// gather instead and counted in *fallback_count.
+// gather instead.  fallback_list holds 1 + (number of 8x8 tiles) words: the
+// count of such tiles, then their indices.
 hipError_t launch_learned(const MlpBuffers &B, const PyrDesc &pyr, const TraceLaunch &t, float threshold,
-                          uint32_t *fallback_count, hipStream_t s);
+                          uint32_t *fallback_list, hipStream_t s);
 hipError_t launch_emit_operands(MlpBuffers &B, hipStream_t s);
 
 }  // namespace vctk

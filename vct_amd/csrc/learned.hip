@@ -751,7 +751,7 @@ __device__ __forceinline__ void store_hidden_t(const f32x4 (&acc)[4][4], const f
 
 template <int MODE>
 __global__ __launch_bounds__(64) void k_learned(MlpDev W, TraceLaunch t, PyrArgs pa, PyrView pv, float threshold,
-                                                uint32_t *fallback_count) {
+                                                uint32_t *fallback_list) {
     constexpr int FMT = MODE == 1 ? 1 : 0;  // level 2 is fp32 in fp32 storage, fp16 pairs otherwise
     __shared__ __align__(16) InferSmem S;
     const vct_gbuffer_texel *__restrict__ gbuf = t.gbuf;
@@ -819,9 +819,8 @@ __global__ __launch_bounds__(64) void k_learned(MlpDev W, TraceLaunch t, PyrArgs
     const float mean = cnt > 0.f ? s1 / cnt : 0.f;
     const float var = cnt > 0.f ? s2 / cnt - mean * mean : 0.f;
     // the butterfly sums are identical on every lane, so the choice is wave-uniform
-    if (var > threshold) {
-        if (lane == 0) atomicAdd(fallback_count, 1u);
-        if (inside) trace_pixel<MODE, false>(t, pa, pyr_rsrc(pa), px, py, false);
+    if (var > threshold) {  // the tile goes on the list that k_learned_fallback traces
+        if (lane == 0) fallback_list[1 + atomicAdd(fallback_list, 1u)] = (uint32_t)tile;
     } else if (inside) {
         t.out_ind[(size_t)py * width + px] = make_float4(r, gch, bch, covered ? 1.0f : 0.0f);
     }
@@ -871,7 +870,7 @@ __device__ __forceinline__ void chunk_put(uint32_t *row, int o, const uint32_t *
 
 template <int MODE>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_learned_stream(MlpDev W, TraceLaunch t, PyrArgs pa, PyrView pv,
-                                                       float threshold, uint32_t *fallback_count) {
+                                                       float threshold, uint32_t *fallback_list) {
     __shared__ __align__(16) StreamSmem S;
     const vct_gbuffer_texel *__restrict__ gbuf = t.gbuf;
     const int width = t.width, height = t.height;
@@ -994,9 +993,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
     }
     const float mean = cnt > 0.f ? s1 / cnt : 0.f;
     const float var = cnt > 0.f ? s2 / cnt - mean * mean : 0.f;
-    if (var > threshold) {
-        if (lane == 0) atomicAdd(fallback_count, 1u);
-        if (inside) trace_pixel<MODE, false>(t, pa, pyr_rsrc(pa), px, py, false);
+    if (var > threshold) {  // the tile goes on the list that k_learned_fallback traces
+        if (lane == 0) fallback_list[1 + atomicAdd(fallback_list, 1u)] = (uint32_t)tile;
     } else if (inside) {
         t.out_ind[(size_t)py * width + px] = make_float4(r, gch, bch, covered ? 1.0f : 0.0f);
     }
@@ -1066,23 +1064,44 @@ hipError_t launch_mlp_train_step(MlpBuffers &B, const uint16_t *X, const float4 
 
 size_t mlp_train_smem_bytes() { return sizeof(TrainSmem); }
 
+// The 6-cone tracer on the tiles the confidence check rejected: list[0] is
+// their number, list[1 ..] their indices (8x8 tiles, row-major).  Each
+// workgroup takes list entries blockIdx.x, blockIdx.x + gridDim.x, ...
+template <int MODE>
+__global__ __launch_bounds__(64) void k_learned_fallback(TraceLaunch t, PyrArgs pa, const uint32_t *__restrict__ list) {
+    const uint32_t n = list[0];
+    const int tiles_x = (t.width + 7) / 8;
+    for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+        const int tile = (int)list[1 + i];
+        const int px = (tile % tiles_x) * 8 + (threadIdx.x & 7), py = (tile / tiles_x) * 8 + (threadIdx.x >> 3);
+        if (px < t.width && py < t.height) trace_pixel<MODE, false>(t, pa, pyr_rsrc(pa), px, py, false);
+    }
+}
+
+// Workgroups of the fallback launch: enough for a full chip when many tiles
+// fall back, few enough that an empty list costs only the launch.
+constexpr int kFallbackBlocks = 1024;
+
 hipError_t launch_learned(const MlpBuffers &B, const PyrDesc &pyr, const TraceLaunch &t, float threshold,
-                          uint32_t *fallback_count, hipStream_t s) {
+                          uint32_t *fallback_list, hipStream_t s) {
     const int tiles = ((t.width + 7) / 8) * ((t.height + 7) / 8);
-    hipError_t e = launch_fill_u32(fallback_count, 0u, 1, s);
+    hipError_t e = launch_fill_u32(fallback_list, 0u, 1, s);
     if (e != hipSuccess) return e;
     const int mode = pyr.storage == VCT_STORE_FP32 ? 1 : (pyr.storage == VCT_STORE_FP8 ? 2 : 0);
     const MlpDev W = mlp_dev(B);
     const PyrArgs pa = pyr_args(pyr, t.nested);
     const PyrView pv = pyr_view(pyr);
-    if (mode == 0)
-        hipLaunchKernelGGL(k_learned_stream<0>, dim3(tiles), dim3(64), 0, s, W, t, pa, pv, threshold,
-                           fallback_count);
-    else if (mode == 1)
-        hipLaunchKernelGGL(k_learned<1>, dim3(tiles), dim3(64), 0, s, W, t, pa, pv, threshold, fallback_count);
-    else
-        hipLaunchKernelGGL(k_learned_stream<2>, dim3(tiles), dim3(64), 0, s, W, t, pa, pv, threshold,
-                           fallback_count);
+    const dim3 fb(tiles < kFallbackBlocks ? tiles : kFallbackBlocks);
+    if (mode == 0) {
+        hipLaunchKernelGGL(k_learned_stream<0>, dim3(tiles), dim3(64), 0, s, W, t, pa, pv, threshold, fallback_list);
+        hipLaunchKernelGGL(k_learned_fallback<0>, fb, dim3(64), 0, s, t, pa, fallback_list);
+    } else if (mode == 1) {
+        hipLaunchKernelGGL(k_learned<1>, dim3(tiles), dim3(64), 0, s, W, t, pa, pv, threshold, fallback_list);
+        hipLaunchKernelGGL(k_learned_fallback<1>, fb, dim3(64), 0, s, t, pa, fallback_list);
+    } else {
+        hipLaunchKernelGGL(k_learned_stream<2>, dim3(tiles), dim3(64), 0, s, W, t, pa, pv, threshold, fallback_list);
+        hipLaunchKernelGGL(k_learned_fallback<2>, fb, dim3(64), 0, s, t, pa, fallback_list);
+    }
     return hipGetLastError();
 }
 
