@@ -273,18 +273,23 @@ def learned_leg(args) -> dict:
         if len(loss):
             loss_last = float(loss[-100:].mean())
         done = total
-        thr, cal_frac = L.calibrate_threshold(app, tr, args.learned_fallback_fraction)
+        # the tile variance grows with the tile's footprint: the threshold is
+        # calibrated at the resolution of the views it is applied to
+        thr, cal_frac = L.calibrate_threshold(app, tr, args.learned_fallback_fraction, width=480, height=270)
+        thr_1080, cal_frac_1080 = L.calibrate_threshold(app, tr, args.learned_fallback_fraction)
         runs.append({"steps": total, "train_s": round(t_train, 3), "loss_last": loss_last,
                      "in_sample": L.in_sample_error(app, tr, pts),
                      "validation": L.pooled_error(app, tr, val_views, refs=refs),
                      "test": L.pooled_error(app, tr, test_views, refs=refs),
-                     "fallback": {"threshold": thr, "calibration_tile_fraction": round(cal_frac, 4),
+                     "fallback": {"threshold_480x270": thr, "calibration_tile_fraction": round(cal_frac, 4),
                                   "validation": L.pooled_error(app, tr, val_views, thr, refs=refs),
-                                  "test": L.pooled_error(app, tr, test_views, thr, refs=refs)}})
+                                  "test": L.pooled_error(app, tr, test_views, thr, refs=refs),
+                                  "threshold_1920x1080": thr_1080,
+                                  "calibration_tile_fraction_1080": round(cal_frac_1080, 4)}})
         print(f"bench: learned, {total} steps: test {runs[-1]['test']['pooled']}", file=sys.stderr, flush=True)
     plain_err = {"validation": L.pooled_error(app, tr, val_views, plain=True, refs=refs)["pooled"],
                  "test": L.pooled_error(app, tr, test_views, plain=True, refs=refs)["pooled"]}
-    thr = runs[-1]["fallback"]["threshold"]
+    thr = runs[-1]["fallback"]["threshold_1920x1080"]
     ctx.gbuffer(cam)
     reps = 10
     for _ in range(2):

@@ -1067,19 +1067,21 @@ size_t mlp_train_smem_bytes() { return sizeof(TrainSmem); }
 // The 6-cone tracer on the tiles the confidence check rejected: list[0] is
 // their number, list[1 ..] their indices (8x8 tiles, row-major).  Each
 // workgroup takes list entries blockIdx.x, blockIdx.x + gridDim.x, ...
+// wavefront w of workgroup b takes list entries 4 b + w, 4 (b + gridDim.x) + w, ...
 template <int MODE>
-__global__ __launch_bounds__(64) void k_learned_fallback(TraceLaunch t, PyrArgs pa, const uint32_t *__restrict__ list) {
+__global__ __launch_bounds__(256) void k_learned_fallback(TraceLaunch t, PyrArgs pa, const uint32_t *__restrict__ list) {
     const uint32_t n = list[0];
     const int tiles_x = (t.width + 7) / 8;
-    for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+    const int lane = threadIdx.x & 63;
+    for (uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6); i < n; i += gridDim.x * 4) {
         const int tile = (int)list[1 + i];
-        const int px = (tile % tiles_x) * 8 + (threadIdx.x & 7), py = (tile / tiles_x) * 8 + (threadIdx.x >> 3);
+        const int px = (tile % tiles_x) * 8 + (lane & 7), py = (tile / tiles_x) * 8 + (lane >> 3);
         if (px < t.width && py < t.height) trace_pixel<MODE, false>(t, pa, pyr_rsrc(pa), px, py, false);
     }
 }
 
-// Workgroups of the fallback launch: enough for a full chip when many tiles
-// fall back, few enough that an empty list costs only the launch.
+// Workgroups (of 4 wavefronts) of the fallback launch: 16 wavefronts per CU
+// when many tiles fall back, and an empty list costs only the launch.
 constexpr int kFallbackBlocks = 1024;
 
 hipError_t launch_learned(const MlpBuffers &B, const PyrDesc &pyr, const TraceLaunch &t, float threshold,
@@ -1091,16 +1093,17 @@ hipError_t launch_learned(const MlpBuffers &B, const PyrDesc &pyr, const TraceLa
     const MlpDev W = mlp_dev(B);
     const PyrArgs pa = pyr_args(pyr, t.nested);
     const PyrView pv = pyr_view(pyr);
-    const dim3 fb(tiles < kFallbackBlocks ? tiles : kFallbackBlocks);
+    const int fb_blocks = (tiles + 3) / 4;
+    const dim3 fb(fb_blocks < kFallbackBlocks ? fb_blocks : kFallbackBlocks);
     if (mode == 0) {
         hipLaunchKernelGGL(k_learned_stream<0>, dim3(tiles), dim3(64), 0, s, W, t, pa, pv, threshold, fallback_list);
-        hipLaunchKernelGGL(k_learned_fallback<0>, fb, dim3(64), 0, s, t, pa, fallback_list);
+        hipLaunchKernelGGL(k_learned_fallback<0>, fb, dim3(256), 0, s, t, pa, fallback_list);
     } else if (mode == 1) {
         hipLaunchKernelGGL(k_learned<1>, dim3(tiles), dim3(64), 0, s, W, t, pa, pv, threshold, fallback_list);
-        hipLaunchKernelGGL(k_learned_fallback<1>, fb, dim3(64), 0, s, t, pa, fallback_list);
+        hipLaunchKernelGGL(k_learned_fallback<1>, fb, dim3(256), 0, s, t, pa, fallback_list);
     } else {
         hipLaunchKernelGGL(k_learned_stream<2>, dim3(tiles), dim3(64), 0, s, W, t, pa, pv, threshold, fallback_list);
-        hipLaunchKernelGGL(k_learned_fallback<2>, fb, dim3(64), 0, s, t, pa, fallback_list);
+        hipLaunchKernelGGL(k_learned_fallback<2>, fb, dim3(256), 0, s, t, pa, fallback_list);
     }
     return hipGetLastError();
 }
