@@ -415,6 +415,10 @@ void vct_destroy(vct_ctx *c) {
     dfree(c->rs.counts);
     dfree(c->rs.offs);
     dfree(c->rs.chunk);
+    dfree(c->rs.bcounts);
+    dfree(c->rs.boffs);
+    dfree(c->rs.bchunk);
+    dfree(c->rs.bslots);
     if (c->rs.scan_tmp) (void)hipFree(c->rs.scan_tmp);
     dfree(c->d_gbuf);
     dfree(c->d_vis);
@@ -502,17 +506,24 @@ static int ensure_raster(vct_ctx *c, size_t slots) {
     dfree(c->rs.attr);
     dfree(c->rs.counts);
     dfree(c->rs.offs);
+    dfree(c->rs.bcounts);
+    dfree(c->rs.boffs);
     if (c->rs.scan_tmp) (void)hipFree(c->rs.scan_tmp);
     c->rs.scan_tmp = nullptr;
     HIPCHK(c, dalloc(&c->rs.rec, slots * 4 * sizeof(int4)));
     HIPCHK(c, dalloc(&c->rs.attr, slots * 8 * sizeof(float4)));
     HIPCHK(c, dalloc(&c->rs.counts, (slots + 1) * sizeof(uint64_t)));
     HIPCHK(c, dalloc(&c->rs.offs, (slots + 1) * sizeof(uint64_t)));
+    HIPCHK(c, dalloc(&c->rs.bcounts, (slots + 1) * sizeof(uint64_t)));
+    HIPCHK(c, dalloc(&c->rs.boffs, (slots + 1) * sizeof(uint64_t)));
     c->rs.scan_tmp_bytes = raster_scan_tmp_bytes(slots);
     HIPCHK(c, hipMalloc(&c->rs.scan_tmp, std::max<size_t>(c->rs.scan_tmp_bytes, 256)));
     if (!c->rs.chunk) {
         c->rs.chunk_cap = (size_t)1 << 22;  // chunk starts for 256M items; beyond that a full search is used
         HIPCHK(c, dalloc(&c->rs.chunk, c->rs.chunk_cap * sizeof(uint32_t)));
+        HIPCHK(c, dalloc(&c->rs.bchunk, c->rs.chunk_cap * sizeof(uint32_t)));
+        c->rs.bslots_cap = (size_t)1 << 24;  // 64 MiB: blocks past this use the walk's own search
+        HIPCHK(c, dalloc(&c->rs.bslots, c->rs.bslots_cap * sizeof(uint32_t)));
     }
     c->rs.cap_slots = slots;
     return VCT_OK;

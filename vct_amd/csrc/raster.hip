@@ -35,8 +35,12 @@ __device__ __forceinline__ void world_vertex(const DevScene &sc, int t, int v, f
 static_assert(kMaxLights <= 32, "light indices are packed in 5 bits");
 static_assert(kShadowBatch * 5 <= 64, "a walk's light indices fit one 64-bit argument");
 
+// A triangle whose bounding box holds more than this many pixels is drawn by
+// the block walk (k_shadow_blocks), the others by the pixel walk.
+constexpr uint64_t kShadowBlockPixels = 1024;
+
 __global__ __launch_bounds__(256) void k_shadow_setup(DevScene sc, const DevLight *__restrict__ lights, uint64_t li,
-                                                      int4 *rec, float4 *attr, uint64_t *counts) {
+                                                      int4 *rec, float4 *attr, uint64_t *counts, uint64_t *bcounts) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= sc.ntri) return;
     const int l = (int)((li >> (5u * blockIdx.y)) & 31u);
@@ -71,8 +75,130 @@ __global__ __launch_bounds__(256) void k_shadow_setup(DevScene sc, const DevLigh
             for (int i = 0; i < 4; ++i) rec[slot * 4 + i] = r[i];
             attr[slot] = make_float4(zw[0], zw[1], zw[2], 0.0f);
         }
+        if (cnt > kShadowBlockPixels) {  // 8x8 blocks of the bounding box instead of its pixels
+            const int x0 = r[1].z & 0xffff, y0 = (int)((uint32_t)r[1].z >> 16), bw = r[1].w & 0xffff;
+            const int bh = (int)(cnt / (uint64_t)bw);
+            const uint64_t nbx = (uint64_t)(((x0 + bw - 1) >> 3) - (x0 >> 3) + 1);
+            const uint64_t nby = (uint64_t)(((y0 + bh - 1) >> 3) - (y0 >> 3) + 1);
+            counts[slot] = 0;
+            bcounts[slot] = nbx * nby;
+            return;
+        }
     }
     counts[slot] = cnt;
+    bcounts[slot] = 0;
+}
+
+// The triangle slot of every block of the block walk, one thread per block
+// (the search of k_raster_items, run for 64 blocks per wavefront so that the
+// dependent loads of many searches are in flight together).  Blocks beyond
+// `cap` are left to the walk's own search.
+__global__ __launch_bounds__(256) void k_block_slots(const uint64_t *__restrict__ offs,
+                                                     const uint32_t *__restrict__ chunk, size_t chunk_cap, int nslots,
+                                                     uint32_t *slots, size_t cap) {
+    const uint64_t total = offs[nslots];
+    const uint64_t n = total < cap ? total : cap;
+    const uint64_t nchunks = (total + 63) >> 6;
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t c = k >> 6;
+        int lo = c < chunk_cap ? (int)chunk[c] : 0;
+        int hi = (c + 1 < nchunks && c + 1 < chunk_cap) ? (int)chunk[c + 1] : nslots - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (offs[mid] <= k) lo = mid;
+            else hi = mid - 1;
+        }
+        slots[k] = (uint32_t)lo;
+    }
+}
+
+// The large triangles of a shadow walk: one wavefront per 8x8-pixel block of
+// a triangle's bounding box, its lanes the block's pixels (row-major).  Each
+// wavefront takes a contiguous run of blocks, so consecutive blocks mostly
+// belong to one triangle and its record (wave-uniform) is read again only when
+// the run enters the next triangle (its slot from k_block_slots, or the same
+// search past that table).  For the 64-bit edge form each lane steps the edge
+// functions from the block corner (E_i = E_i(corner) + A_i dx + B_i dy, the
+// same integers as edge_value), so every fragment and its depth are those the
+// pixel walk (ShadowFrag) produces.
+__global__ __launch_bounds__(256) void k_shadow_blocks(const int4 *__restrict__ recs, const uint64_t *__restrict__ offs,
+                                                       const uint32_t *__restrict__ chunk, size_t chunk_cap, int nslots,
+                                                       const uint32_t *__restrict__ bslots, size_t bslots_cap,
+                                                       const float4 *__restrict__ attr,
+                                                       const DevLight *__restrict__ lights) {
+    const uint64_t total = offs[nslots];
+    const uint64_t nchunks = (total + 63) >> 6;
+    const int lane = threadIdx.x & 63;
+    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    const uint64_t wave = __builtin_amdgcn_readfirstlane((uint32_t)(((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+    const uint64_t per = (total + nwaves - 1) / nwaves;
+    const uint64_t k0 = wave * per, k1 = k0 + per < total ? k0 + per : total;
+    uint64_t run_lo = 0, run_hi = 0;  // the blocks [run_lo, run_hi) of the current slot
+    int slot = 0;
+    Edges e;
+    float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
+    uint32_t *map = nullptr;
+    int N = 0;
+    uint32_t nbx = 1, bxi = 0, byi = 0;  // the block's column and row in the triangle's bounding box
+    for (uint64_t k = k0; k < k1; ++k, ++bxi) {
+        if (bxi == nbx) {
+            bxi = 0;
+            ++byi;
+        }
+        if (k >= run_hi) {  // the run enters the next triangle
+            int lo;
+            if (k < bslots_cap) {
+                lo = (int)bslots[k];
+            } else {
+                const uint64_t c = k >> 6;  // block k lies in chunk c (64 blocks per chunk)
+                lo = c < chunk_cap ? (int)chunk[c] : 0;
+                int hi = (c + 1 < nchunks && c + 1 < chunk_cap) ? (int)chunk[c + 1] : nslots - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (offs[mid] <= k) lo = mid;
+                    else hi = mid - 1;
+                }
+            }
+            slot = __builtin_amdgcn_readfirstlane(lo);
+            run_lo = offs[slot];
+            run_hi = offs[slot + 1];
+            load_edges(recs, (size_t)slot, e);
+            a = attr[slot];
+            const DevLight &L = lights[e.user];
+            map = (uint32_t *)const_cast<float *>(L.shadowmap);
+            N = (int)L.shadow_size;
+            nbx = (uint32_t)(((e.x0 + e.bw - 1) >> 3) - (e.x0 >> 3) + 1);
+            const uint32_t local = (uint32_t)(k - run_lo);  // the run may enter a triangle mid-way
+            bxi = local % nbx;
+            byi = local / nbx;
+        }
+        const int bx = ((e.x0 >> 3) + (int)bxi) * 8, by = ((e.y0 >> 3) + (int)byi) * 8;
+        const int px = bx + (lane & 7), py = by + (lane >> 3);
+        if (px >= N || py >= N) continue;  // the block overhangs the map
+        float l[3];
+        if (e.small) {
+            if (!edge_inside(e, px, py, 128, 128)) continue;
+            edge_bary(e, px, py, 128, 128, l);
+        } else {
+            int64_t E[3];
+            bool in = true;
+            const int64_t sx = (int64_t)bx * 256 + 128, sy = (int64_t)by * 256 + 128;
+            const int dx = (lane & 7) * 256, dy = (lane >> 3) * 256;
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                const int64_t corner = (int64_t)e.A[i] * sx + (int64_t)e.B[i] * sy + e.C[i];
+                E[i] = corner + (int64_t)e.A[i] * dx + (int64_t)e.B[i] * dy;
+                in = in && (E[i] + e.bias[i] >= 0);
+            }
+            if (!in) continue;
+#pragma unroll
+            for (int i = 0; i < 3; ++i) l[i] = (float)E[i] / e.area;
+        }
+        float z = interp3(l, a.x, a.y, a.z);
+        z = z + 0.0f;
+        if (!(z >= 0.0f && z <= 1.0f)) continue;
+        atomicMin(&map[(size_t)py * N + px], __float_as_uint(z));
+    }
 }
 
 struct ShadowFrag {
@@ -111,12 +237,19 @@ hipError_t launch_shadowmaps(const DevScene &sc, const DevLight *hlights, const 
         if (sc.ntri > 0) {
             const size_t nslots = (size_t)nb * (size_t)sc.ntri;
             hipLaunchKernelGGL(k_shadow_setup, dim3((sc.ntri + 255) / 256, nb), dim3(256), 0, s, sc, dlights, packed,
-                               rs.rec, rs.attr, rs.counts);
+                               rs.rec, rs.attr, rs.counts, rs.bcounts);
             if ((e = hipGetLastError()) != hipSuccess) return e;
             if ((e = raster_scan(rs, nslots, s)) != hipSuccess) return e;
+            if ((e = raster_scan_blocks(rs, nslots, s)) != hipSuccess) return e;
             ShadowFrag f{rs.attr, dlights};
             hipLaunchKernelGGL(k_raster_items<ShadowFrag>, dim3(raster_grid<ShadowFrag, false>()), dim3(256), 0, s,
                                rs.rec, rs.offs, rs.chunk, rs.chunk_cap, (int)nslots, f);
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+            hipLaunchKernelGGL(k_block_slots, dim3(kRasterBlocks), dim3(256), 0, s, rs.boffs, rs.bchunk, rs.chunk_cap,
+                               (int)nslots, rs.bslots, rs.bslots_cap);
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+            hipLaunchKernelGGL(k_shadow_blocks, dim3(kRasterBlocks), dim3(256), 0, s, rs.rec, rs.boffs, rs.bchunk,
+                               rs.chunk_cap, (int)nslots, rs.bslots, rs.bslots_cap, rs.attr, dlights);
             if ((e = hipGetLastError()) != hipSuccess) return e;
         }
         for (int j = 0; j < nb; ++j) {

@@ -87,18 +87,26 @@ size_t raster_scan_tmp_bytes(size_t nslots) {
     return bytes;
 }
 
-hipError_t raster_scan(RasterScratch &rs, size_t nslots, hipStream_t s) {
+static hipError_t scan_counts(RasterScratch &rs, uint64_t *counts, uint64_t *offs, uint32_t *chunk, size_t nslots,
+                              hipStream_t s) {
     // counts[nslots] is kept at 0 so offs[nslots] is the total
-    hipError_t e = launch_fill_u64(rs.counts + nslots, 0, 1, s);
+    hipError_t e = launch_fill_u64(counts + nslots, 0, 1, s);
     if (e != hipSuccess) return e;
     size_t bytes = rs.scan_tmp_bytes;
-    e = hipcub::DeviceScan::ExclusiveSum(rs.scan_tmp, bytes, rs.counts, rs.offs, (int)(nslots + 1), s);
+    e = hipcub::DeviceScan::ExclusiveSum(rs.scan_tmp, bytes, counts, offs, (int)(nslots + 1), s);
     if (e != hipSuccess) return e;
     // each chunk is a chain of dependent loads (the search): 8 wavefronts per
     // SIMD of the 256 CUs keep the most of them in flight
-    hipLaunchKernelGGL(k_chunk_starts, dim3(2048), dim3(256), 0, s, rs.offs, (int)nslots, rs.chunk,
-                       rs.chunk_cap);
+    hipLaunchKernelGGL(k_chunk_starts, dim3(2048), dim3(256), 0, s, offs, (int)nslots, chunk, rs.chunk_cap);
     return hipGetLastError();
+}
+
+hipError_t raster_scan(RasterScratch &rs, size_t nslots, hipStream_t s) {
+    return scan_counts(rs, rs.counts, rs.offs, rs.chunk, nslots, s);
+}
+
+hipError_t raster_scan_blocks(RasterScratch &rs, size_t nslots, hipStream_t s) {
+    return scan_counts(rs, rs.bcounts, rs.boffs, rs.bchunk, nslots, s);
 }
 
 }  // namespace vctk

@@ -120,6 +120,13 @@ struct RasterScratch {
     void *scan_tmp;
     size_t scan_tmp_bytes;
     size_t cap_slots;
+    // the shadow pass's 8x8-block walk of its large triangles: per slot block
+    // counts, their scan and its chunk starts (same capacities as above)
+    uint64_t *bcounts;
+    uint64_t *boffs;
+    uint32_t *bchunk;
+    uint32_t *bslots;   // the slot of each block, for the first bslots_cap blocks
+    size_t bslots_cap;
 };
 
 // The diffuse cones' step schedule.  Every diffuse cone of every pixel starts
@@ -181,6 +188,8 @@ hipError_t launch_reduce_stats(const uint32_t *partials, int n, uint32_t *sum0, 
 
 // scan.hip: counts[0..n) -> offs[0..n] exclusive (offs[n] = total) and chunk starts
 hipError_t raster_scan(RasterScratch &rs, size_t nslots, hipStream_t s);
+// the same for the block counts: bcounts -> boffs, bchunk
+hipError_t raster_scan_blocks(RasterScratch &rs, size_t nslots, hipStream_t s);
 size_t raster_scan_tmp_bytes(size_t nslots);
 
 // raster.hip
