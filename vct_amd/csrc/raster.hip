@@ -37,7 +37,7 @@ static_assert(kShadowBatch * 5 <= 64, "a walk's light indices fit one 64-bit arg
 
 // A triangle whose bounding box holds more than this many pixels is drawn by
 // the block walk (k_shadow_blocks), the others by the pixel walk.
-constexpr uint64_t kShadowBlockPixels = 1024;
+constexpr uint64_t kShadowBlockPixels = 256;
 
 __global__ __launch_bounds__(256) void k_shadow_setup(DevScene sc, const DevLight *__restrict__ lights, uint64_t li,
                                                       int4 *rec, float4 *attr, uint64_t *counts, uint64_t *bcounts) {
