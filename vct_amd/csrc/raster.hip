@@ -36,8 +36,18 @@ static_assert(kMaxLights <= 32, "light indices are packed in 5 bits");
 static_assert(kShadowBatch * 5 <= 64, "a walk's light indices fit one 64-bit argument");
 
 // A triangle whose bounding box holds more than this many pixels is drawn by
-// the block walk (k_shadow_blocks), the others by the pixel walk.
-constexpr uint64_t kShadowBlockPixels = 256;
+// the block walk (k_raster_blocks), the others by the pixel walk.
+constexpr uint64_t kBlockPixels = 256;
+
+// The 8x8 pixel blocks (aligned to multiples of 8) that cover the bounding box
+// of a setup record r with cnt bounding-box pixels.
+__device__ __forceinline__ uint64_t bbox_blocks(const int4 r[4], uint64_t cnt) {
+    const int x0 = r[1].z & 0xffff, y0 = (int)((uint32_t)r[1].z >> 16), bw = r[1].w & 0xffff;
+    const int bh = (int)(cnt / (uint64_t)bw);
+    const uint64_t nbx = (uint64_t)(((x0 + bw - 1) >> 3) - (x0 >> 3) + 1);
+    const uint64_t nby = (uint64_t)(((y0 + bh - 1) >> 3) - (y0 >> 3) + 1);
+    return nbx * nby;
+}
 
 __global__ __launch_bounds__(256) void k_shadow_setup(DevScene sc, const DevLight *__restrict__ lights, uint64_t li,
                                                       int4 *rec, float4 *attr, uint64_t *counts, uint64_t *bcounts) {
@@ -75,13 +85,9 @@ __global__ __launch_bounds__(256) void k_shadow_setup(DevScene sc, const DevLigh
             for (int i = 0; i < 4; ++i) rec[slot * 4 + i] = r[i];
             attr[slot] = make_float4(zw[0], zw[1], zw[2], 0.0f);
         }
-        if (cnt > kShadowBlockPixels) {  // 8x8 blocks of the bounding box instead of its pixels
-            const int x0 = r[1].z & 0xffff, y0 = (int)((uint32_t)r[1].z >> 16), bw = r[1].w & 0xffff;
-            const int bh = (int)(cnt / (uint64_t)bw);
-            const uint64_t nbx = (uint64_t)(((x0 + bw - 1) >> 3) - (x0 >> 3) + 1);
-            const uint64_t nby = (uint64_t)(((y0 + bh - 1) >> 3) - (y0 >> 3) + 1);
+        if (cnt > kBlockPixels) {  // 8x8 blocks of the bounding box instead of its pixels
             counts[slot] = 0;
-            bcounts[slot] = nbx * nby;
+            bcounts[slot] = bbox_blocks(r, cnt);
             return;
         }
     }
@@ -112,20 +118,21 @@ __global__ __launch_bounds__(256) void k_block_slots(const uint64_t *__restrict_
     }
 }
 
-// The large triangles of a shadow walk: one wavefront per 8x8-pixel block of
-// a triangle's bounding box, its lanes the block's pixels (row-major).  Each
+// The large triangles of a walk: one wavefront per 8x8-pixel block of a
+// triangle's bounding box, its lanes the block's pixels (row-major).  Each
 // wavefront takes a contiguous run of blocks, so consecutive blocks mostly
 // belong to one triangle and its record (wave-uniform) is read again only when
 // the run enters the next triangle (its slot from k_block_slots, or the same
 // search past that table).  For the 64-bit edge form each lane steps the edge
 // functions from the block corner (E_i = E_i(corner) + A_i dx + B_i dy, the
-// same integers as edge_value), so every fragment and its depth are those the
-// pixel walk (ShadowFrag) produces.
-__global__ __launch_bounds__(256) void k_shadow_blocks(const int4 *__restrict__ recs, const uint64_t *__restrict__ offs,
+// same integers as edge_value), so every fragment and its barycentrics are
+// those the pixel walk produces.  F provides enter(slot, edges) (the slot's
+// wave-uniform data, and the image size W x H) and frag(edges, px, py, l).
+template <class F>
+__global__ __launch_bounds__(256) void k_raster_blocks(const int4 *__restrict__ recs, const uint64_t *__restrict__ offs,
                                                        const uint32_t *__restrict__ chunk, size_t chunk_cap, int nslots,
-                                                       const uint32_t *__restrict__ bslots, size_t bslots_cap,
-                                                       const float4 *__restrict__ attr,
-                                                       const DevLight *__restrict__ lights) {
+                                                       const uint32_t *__restrict__ bslots, size_t bslots_cap, F fr) {
+    F f = fr;
     const uint64_t total = offs[nslots];
     const uint64_t nchunks = (total + 63) >> 6;
     const int lane = threadIdx.x & 63;
@@ -136,9 +143,6 @@ __global__ __launch_bounds__(256) void k_shadow_blocks(const int4 *__restrict__ 
     uint64_t run_lo = 0, run_hi = 0;  // the blocks [run_lo, run_hi) of the current slot
     int slot = 0;
     Edges e;
-    float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
-    uint32_t *map = nullptr;
-    int N = 0;
     uint32_t nbx = 1, bxi = 0, byi = 0;  // the block's column and row in the triangle's bounding box
     for (uint64_t k = k0; k < k1; ++k, ++bxi) {
         if (bxi == nbx) {
@@ -163,10 +167,7 @@ __global__ __launch_bounds__(256) void k_shadow_blocks(const int4 *__restrict__ 
             run_lo = offs[slot];
             run_hi = offs[slot + 1];
             load_edges(recs, (size_t)slot, e);
-            a = attr[slot];
-            const DevLight &L = lights[e.user];
-            map = (uint32_t *)const_cast<float *>(L.shadowmap);
-            N = (int)L.shadow_size;
+            f.enter(slot, e);
             nbx = (uint32_t)(((e.x0 + e.bw - 1) >> 3) - (e.x0 >> 3) + 1);
             const uint32_t local = (uint32_t)(k - run_lo);  // the run may enter a triangle mid-way
             bxi = local % nbx;
@@ -174,7 +175,7 @@ __global__ __launch_bounds__(256) void k_shadow_blocks(const int4 *__restrict__ 
         }
         const int bx = ((e.x0 >> 3) + (int)bxi) * 8, by = ((e.y0 >> 3) + (int)byi) * 8;
         const int px = bx + (lane & 7), py = by + (lane >> 3);
-        if (px >= N || py >= N) continue;  // the block overhangs the map
+        if (px >= f.W || py >= f.H) continue;  // the block overhangs the image
         float l[3];
         if (e.small) {
             if (!edge_inside(e, px, py, 128, 128)) continue;
@@ -194,12 +195,30 @@ __global__ __launch_bounds__(256) void k_shadow_blocks(const int4 *__restrict__ 
 #pragma unroll
             for (int i = 0; i < 3; ++i) l[i] = (float)E[i] / e.area;
         }
-        float z = interp3(l, a.x, a.y, a.z);
-        z = z + 0.0f;
-        if (!(z >= 0.0f && z <= 1.0f)) continue;
-        atomicMin(&map[(size_t)py * N + px], __float_as_uint(z));
+        f.frag(e, px, py, l);
     }
 }
+
+// ShadowFrag for the block walk.
+struct ShadowBlockFrag {
+    const float4 *attr;
+    const DevLight *lights;
+    float4 a;
+    uint32_t *map;
+    int W, H;
+    __device__ void enter(int slot, const Edges &e) {
+        a = attr[slot];
+        const DevLight &L = lights[e.user];
+        map = (uint32_t *)const_cast<float *>(L.shadowmap);
+        W = H = (int)L.shadow_size;
+    }
+    __device__ void frag(const Edges &, int px, int py, const float l[3]) {
+        float z = interp3(l, a.x, a.y, a.z);
+        z = z + 0.0f;
+        if (!(z >= 0.0f && z <= 1.0f)) return;
+        atomicMin(&map[(size_t)py * W + px], __float_as_uint(z));
+    }
+};
 
 struct ShadowFrag {
     const float4 *attr;
@@ -248,8 +267,9 @@ hipError_t launch_shadowmaps(const DevScene &sc, const DevLight *hlights, const 
             hipLaunchKernelGGL(k_block_slots, dim3(kRasterBlocks), dim3(256), 0, s, rs.boffs, rs.bchunk, rs.chunk_cap,
                                (int)nslots, rs.bslots, rs.bslots_cap);
             if ((e = hipGetLastError()) != hipSuccess) return e;
-            hipLaunchKernelGGL(k_shadow_blocks, dim3(kRasterBlocks), dim3(256), 0, s, rs.rec, rs.boffs, rs.bchunk,
-                               rs.chunk_cap, (int)nslots, rs.bslots, rs.bslots_cap, rs.attr, dlights);
+            ShadowBlockFrag fb{rs.attr, dlights, make_float4(0.f, 0.f, 0.f, 0.f), nullptr, 0, 0};
+            hipLaunchKernelGGL(k_raster_blocks<ShadowBlockFrag>, dim3(kRasterBlocks), dim3(256), 0, s, rs.rec,
+                               rs.boffs, rs.bchunk, rs.chunk_cap, (int)nslots, rs.bslots, rs.bslots_cap, fb);
             if ((e = hipGetLastError()) != hipSuccess) return e;
         }
         for (int j = 0; j < nb; ++j) {
@@ -287,7 +307,7 @@ struct CamMats {
 };
 
 __global__ __launch_bounds__(256) void k_gbuf_setup(DevScene sc, CamMats cm, int4 *rec, float4 *attr,
-                                                    uint64_t *counts) {
+                                                    uint64_t *counts, uint64_t *bcounts) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= sc.ntri) return;
     float clip[3][4];
@@ -339,9 +359,15 @@ __global__ __launch_bounds__(256) void k_gbuf_setup(DevScene sc, CamMats cm, int
                     for (int i = 0; i < 4; ++i) rec[slot * 4 + i] = r[i];
                     attr[slot] = make_float4(zw[0], zw[1], zw[2], 0.0f);
                 }
+                if (cnt > kBlockPixels) {  // 8x8 blocks of the bounding box instead of its pixels
+                    counts[slot] = 0;
+                    bcounts[slot] = bbox_blocks(r, cnt);
+                    continue;
+                }
             }
         }
         counts[slot] = cnt;
+        bcounts[slot] = 0;
     }
 }
 
@@ -362,6 +388,24 @@ struct GbufFrag {
         atomicMin(&vis[(size_t)py * W + px], key);
     }
     __device__ void finish() {}
+};
+
+// GbufFrag for the block walk.
+struct GbufBlockFrag {
+    const float4 *attr;
+    unsigned long long *vis;
+    int W, H;
+    float4 a;
+    __device__ void enter(int slot, const Edges &) { a = attr[slot]; }
+    __device__ void frag(const Edges &e, int px, int py, const float l[3]) {
+        float z = interp3(l, a.x, a.y, a.z);
+        z = z + 0.0f;
+        if (!(z >= 0.0f && z <= 1.0f)) return;
+        const uint32_t tri = (uint32_t)e.user;
+        const unsigned long long key =
+            ((unsigned long long)__float_as_uint(z) << 32) | (unsigned long long)(0xffffffffu - tri);
+        atomicMin(&vis[(size_t)py * W + px], key);
+    }
 };
 
 __global__ __launch_bounds__(256) void k_gbuf_resolve(DevScene sc, CamMats cm, const unsigned long long *vis,
@@ -446,13 +490,22 @@ hipError_t launch_gbuffer(const DevScene &sc, const vct_camera &cam, uint64_t *v
     cm.H = cam.height;
     cm.cull = cam.cull_back;
     if (sc.ntri > 0) {
+        const size_t nslots = (size_t)sc.ntri * 2;
         hipLaunchKernelGGL(k_gbuf_setup, dim3((sc.ntri + 255) / 256), dim3(256), 0, s, sc, cm, rs.rec, rs.attr,
-                           rs.counts);
+                           rs.counts, rs.bcounts);
         if ((e = hipGetLastError()) != hipSuccess) return e;
-        if ((e = raster_scan(rs, (size_t)sc.ntri * 2, s)) != hipSuccess) return e;
+        if ((e = raster_scan(rs, nslots, s)) != hipSuccess) return e;
+        if ((e = raster_scan_blocks(rs, nslots, s)) != hipSuccess) return e;
         GbufFrag f{rs.attr, (unsigned long long *)vis, cam.width};
         hipLaunchKernelGGL(k_raster_items<GbufFrag>, dim3(raster_grid<GbufFrag, false>()), dim3(256), 0, s, rs.rec, rs.offs,
-                           rs.chunk, rs.chunk_cap, sc.ntri * 2, f);
+                           rs.chunk, rs.chunk_cap, (int)nslots, f);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        hipLaunchKernelGGL(k_block_slots, dim3(kRasterBlocks), dim3(256), 0, s, rs.boffs, rs.bchunk, rs.chunk_cap,
+                           (int)nslots, rs.bslots, rs.bslots_cap);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        GbufBlockFrag fb{rs.attr, (unsigned long long *)vis, cam.width, cam.height, make_float4(0.f, 0.f, 0.f, 0.f)};
+        hipLaunchKernelGGL(k_raster_blocks<GbufBlockFrag>, dim3(kRasterBlocks), dim3(256), 0, s, rs.rec, rs.boffs,
+                           rs.bchunk, rs.chunk_cap, (int)nslots, rs.bslots, rs.bslots_cap, fb);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     dim3 grid((cam.width + 15) / 16, (cam.height + 15) / 16);
