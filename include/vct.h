@@ -297,6 +297,17 @@ int vct_shadowmaps(vct_ctx *ctx);
 /* Voxelize, GS path: voxelize.vert/geom/frag drawn by scene->draw
  * (src/Application.cpp:666-754). Clears albedo/normal first (:686-687). */
 int vct_voxelize(vct_ctx *ctx, const vct_voxelize_params *p);
+/* Voxelize, tessellation path: simpleTesselated.vert + testTesselation.tesc /
+ * .tese in point mode with rasterizer discard (src/Application.cpp:585-654,
+ * the reference's choice when voxelizeTesselation is set).  Clears albedo and
+ * normal (:627-628); patches whose three vertices share a voxel are dropped
+ * (.tesc:40-49); every tessellation point stores the triangle's albedo and its
+ * normal normalize(n) * 0.5 + 0.5 in the voxel under it.  Of p only
+ * atomic_max is read: VCT_ACCUM_MAX (voxelizeAtomicMax) or VCT_ACCUM_AVG (the
+ * RGBA8 running average).  No lighting and no fragment count (the stats'
+ * total_voxel_fragments is 0); the voxelizeTesselationWarp projection is not
+ * provided. */
+int vct_voxelize_tessellated(vct_ctx *ctx, const vct_voxelize_params *p);
 /* transferVoxels.comp then injectRadiance.comp (src/Application.cpp:757-837),
  * fused: lit-mask scatter over shadow texels, then one per-voxel pass that
  * normalizes, rewrites opacity and writes radiance level 0. */

@@ -61,6 +61,10 @@ typedef struct vct_app_config {
     float warp_high_resolution; /* warpTextureHighResolution (Application.h:87):
                             2.0 (0 = default); 4.0, the overlay's maximum
                             (Overlay.cpp:289), for scene 5                     */
+    int tessellation;    /* voxelizeTesselation (Application.h:100), 0: 1 draws
+                            the voxels with the tessellation voxelizer
+                            (vct_voxelize_tessellated, Application.cpp:585-654)
+                            instead of the GS path                             */
 } vct_app_config;
 
 /* ---- OBJ/MTL assets: Mesh::loadMesh (src/Graphics/Mesh.cpp:42-266) ------------

@@ -92,6 +92,10 @@ def lib() -> C.CDLL:
             "or_normalize_inject_temporal": (None, [vp, C.c_int, vp, vp, vp, vp, C.c_int, vp, C.c_uint32, C.c_float,
                                                     C.c_float, C.c_int, vp, vp, vp, vp, C.POINTER(Stats)]),
             "or_warp_occupancy": (None, [C.POINTER(Scene), C.POINTER(VoxParams), vp]),
+            "or_tess_levels": (C.c_int, [vp, C.c_int, vp, vp, vp, vp]),
+            "or_tess_count": (C.c_int, [vp]),
+            "or_tess_point": (None, [vp, C.c_int, vp]),
+            "or_voxelize_tess": (None, [C.POINTER(Scene), C.c_int, vp, vp, vp, C.c_int, vp, vp]),
             "or_warpmap": (None, [vp, C.POINTER(WarpParams), vp]),
             "or_warp_sample": (None, [vp, vp, vp]),
             "or_fill_holes": (None, [vp, vp, C.c_int, C.c_int]),
@@ -396,6 +400,35 @@ def compose(gbuf: np.ndarray, indirect, specular, lights, shadowmaps, params, ha
     lib().or_compose(_p(g), npix, _p(ind), _p(spe), None if la is None else C.cast(la, C.c_void_p), len(lights),
                      C.cast(ptrs, C.c_void_p), 1 if has_roughness else 0, 1 if occluded else 0, C.byref(P), _p(out))
     return out
+
+
+# ---------------------------------------------------------------- tessellation voxelizer
+def tess_levels(P, dim, center, vmin, vmax):
+    """(point count, [inner, outer0, outer1, outer2]) of one patch (world positions P[3][3])."""
+    Pa = np.ascontiguousarray(P, np.float32).reshape(3, 3)
+    c, mn, mx = (np.ascontiguousarray(v, np.float32) for v in (center, vmin, vmax))
+    lv = np.zeros(4, np.int32)
+    n = lib().or_tess_levels(_p(Pa), int(dim), _p(c), _p(mn), _p(mx), _p(lv))
+    return int(n), [int(x) for x in lv]
+
+
+def tess_points(lv) -> np.ndarray:
+    """Barycentric (u, v, w) of every point of a patch with levels lv, [count][3]."""
+    lv = np.ascontiguousarray(lv, np.int32)
+    n = lib().or_tess_count(_p(lv))
+    out = np.zeros((n, 3), np.float32)
+    for j in range(n):
+        lib().or_tess_point(_p(lv), j, out[j].ctypes.data)
+    return out
+
+
+def voxelize_tess(sc: SceneArrays, dim, center, vmin, vmax, accum=1):
+    """Albedo and normal grids (z, y, x) of the tessellation voxelizer."""
+    c, mn, mx = (np.ascontiguousarray(v, np.float32) for v in (center, vmin, vmax))
+    alb = np.zeros(dim ** 3, np.uint32)
+    nrm = np.zeros(dim ** 3, np.uint32)
+    lib().or_voxelize_tess(C.byref(sc.s), int(dim), _p(c), _p(mn), _p(mx), int(accum), _p(alb), _p(nrm))
+    return alb.reshape(dim, dim, dim), nrm.reshape(dim, dim, dim)
 
 
 # ---------------------------------------------------------------- warp texture

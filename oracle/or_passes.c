@@ -973,3 +973,173 @@ void or_gbuffer(const or_scene *sc, const or_camera *cam, or_gbuf_texel *out) {
         }
     free(vis);
 }
+
+/* ------------------------------------------------------------------------ */
+/* tessellation voxelizer: simpleTesselated.vert + testTesselation.tesc/.tese
+ * (Application.cpp:585-654), the alternative the reference runs when
+ * voxelizeTesselation is set.  Per triangle (a GL_PATCHES patch of 3):
+ *   - world positions and normals as simpleTesselated.vert:16-21;
+ *   - .tesc:40-49: the patch is discarded (levels 0) when all three vertices
+ *     fall in one voxel, ivec3(dim * (p - center - min) / (max - min));
+ *   - .tesc:51-75: IL = max(1, max altitude / voxelSize.x), outer level of an
+ *     edge = max(1, length / length(normalize(edge) * voxelSize)), assigned
+ *     OL0 = edge b - a, OL1 = edge c - b, OL2 = edge c - a (as written there);
+ *   - point-mode triangle tessellation with equal_spacing (GL 4.5 core,
+ *     11.2.2.1): each level clamped to [1, 64] and rounded up; an inner level
+ *     of 1 with any outer level above 1 becomes 2; levels all 1 give the three
+ *     corners; otherwise the corners, the OL_e - 1 interior points of outer
+ *     edge e (edge 0: u = 0, 1: v = 0, 2: w = 0), and the concentric inner
+ *     triangles k = 1 .. n/2 whose corners are the intersections of the
+ *     perpendiculars through the k-th subdivision points of the domain's edges
+ *     (in barycentric coordinates (1 - 4k/3n, 2k/3n, 2k/3n) and rotations),
+ *     each edge cut into n - 2k segments (n - 2k = 0: the centre point);
+ *   - .tese:121-140: position, normal (and texcoord) interpolated with the
+ *     tessellation coordinate, voxelStore at ivec3(dim * linear position)
+ *     with the triangle's albedo (the diffuse map stands for a constant) and
+ *     normalize(n) * 0.5 + 0.5; atomicMax (voxelizeAtomicMax) or the RGBA8
+ *     running average; no lighting, no fragment counter.
+ * Fixed evaluation order, no contraction. */
+int or_tess_levels(const float P[3][3], int dim, const float center[3], const float vmin[3], const float vmax[3],
+                   int lv[4]) {
+    int vox[3][3];
+    for (int v = 0; v < 3; ++v)
+        for (int k = 0; k < 3; ++k) {
+            const float tc = ((P[v][k] - center[k]) - vmin[k]) / (vmax[k] - vmin[k]);
+            vox[v][k] = (int)((float)dim * tc);
+        }
+    int same = 1;
+    for (int k = 0; k < 3; ++k) same = same && vox[0][k] == vox[1][k] && vox[0][k] == vox[2][k];
+    if (same) return 0;
+    const float *a = P[0], *b = P[1], *c = P[2];
+    float E[3][3]; /* A = c - b, B = c - a, C = b - a */
+    for (int k = 0; k < 3; ++k) {
+        E[0][k] = c[k] - b[k];
+        E[1][k] = c[k] - a[k];
+        E[2][k] = b[k] - a[k];
+    }
+    float len[3];
+    for (int i = 0; i < 3; ++i) len[i] = sqrtf(dot3(E[i], E[i]));
+    const float s = ((len[0] + len[1]) + len[2]) * 0.5f;
+    const float area = sqrtf(((s * (s - len[0])) * (s - len[1])) * (s - len[2]));
+    float maxalt = 0.0f;
+    for (int i = 0; i < 3; ++i) {
+        const float alt = (2.0f * area) / len[i];
+        maxalt = i == 0 ? alt : fmaxf(maxalt, alt);
+    }
+    float vs[3];
+    for (int k = 0; k < 3; ++k) vs[k] = (vmax[k] - vmin[k]) / (float)dim;
+    float outer[3];
+    for (int i = 0; i < 3; ++i) {
+        float n[3];
+        nrm3(E[i], n);
+        const float d[3] = {n[0] * vs[0], n[1] * vs[1], n[2] * vs[2]};
+        const float div = fabsf(sqrtf(dot3(d, d)));
+        outer[i] = fmaxf(1.0f, len[i] / div);
+    }
+    const float lev[4] = {fmaxf(1.0f, maxalt / vs[0]), outer[2], outer[0], outer[1]};
+    for (int i = 0; i < 4; ++i) {
+        const float x = lev[i] > 64.0f ? 64.0f : lev[i]; /* >= 1 already (NaN became 1) */
+        lv[i] = (int)ceilf(x);
+    }
+    const int any_outer = lv[1] > 1 || lv[2] > 1 || lv[3] > 1;
+    if (lv[0] == 1 && any_outer) lv[0] = 2;
+    return or_tess_count(lv);
+}
+
+int or_tess_count(const int lv[4]) {
+    const int n = lv[0];
+    if (n == 1 && lv[1] == 1 && lv[2] == 1 && lv[3] == 1) return 3;
+    int cnt = 3 + (lv[1] - 1) + (lv[2] - 1) + (lv[3] - 1);
+    for (int k = 1; 2 * k <= n; ++k) {
+        const int m = n - 2 * k;
+        cnt += m == 0 ? 1 : 3 * m;
+    }
+    return cnt;
+}
+
+void or_tess_point(const int lv[4], int j, float uvw[3]) {
+    static const float corner[3][3] = {{1.0f, 0.0f, 0.0f}, {0.0f, 1.0f, 0.0f}, {0.0f, 0.0f, 1.0f}};
+    if (j < 3) {
+        for (int k = 0; k < 3; ++k) uvw[k] = corner[j][k];
+        return;
+    }
+    j -= 3;
+    for (int e = 0; e < 3; ++e) {
+        const int m = lv[1 + e];
+        if (j < m - 1) {
+            const float f = (float)(j + 1) / (float)m;
+            const float g = 1.0f - f;
+            if (e == 0) { uvw[0] = 0.0f; uvw[1] = g; uvw[2] = f; }
+            else if (e == 1) { uvw[0] = f; uvw[1] = 0.0f; uvw[2] = g; }
+            else { uvw[0] = g; uvw[1] = f; uvw[2] = 0.0f; }
+            return;
+        }
+        j -= m - 1;
+    }
+    const int n = lv[0];
+    for (int k = 1; 2 * k <= n; ++k) {
+        const int m = n - 2 * k;
+        const float cc = (2.0f * (float)k) / (3.0f * (float)n);
+        const float oo = 1.0f - 2.0f * cc;
+        const float R[3][3] = {{oo, cc, cc}, {cc, oo, cc}, {cc, cc, oo}};
+        const int cnt = m == 0 ? 1 : 3 * m;
+        if (j < cnt) {
+            if (m == 0) {
+                for (int q = 0; q < 3; ++q) uvw[q] = R[0][q];
+                return;
+            }
+            const int e = j / m, s = j % m;
+            const float f = (float)s / (float)m;
+            const float *ra = R[e], *rb = R[(e + 1) % 3];
+            for (int q = 0; q < 3; ++q) uvw[q] = ra[q] + f * (rb[q] - ra[q]);
+            return;
+        }
+        j -= cnt;
+    }
+    uvw[0] = uvw[1] = uvw[2] = 0.0f; /* beyond the count: not reached */
+}
+
+void or_voxelize_tess(const or_scene *sc, int dim, const float center[3], const float vmin[3], const float vmax[3],
+                      int accum, uint32_t *albedo, uint32_t *normal) {
+    /* accum 1 (atomicMax) on all host threads; accum 0 (the running average)
+     * in patch order, points in the order of or_tess_point */
+#pragma omp parallel for schedule(dynamic, 16) if (accum == 1)
+    for (int t = 0; t < sc->ntri; ++t) {
+        float P[3][3], N[3][3];
+        const float *Nm = sc->nmat + (size_t)sc->obj[t] * 9;
+        for (int v = 0; v < 3; ++v) {
+            world_vertex(sc, t, v, P[v]);
+            mat3_vec3(Nm, sc->nrm + (size_t)t * 9 + v * 3, N[v]);
+        }
+        int lv[4];
+        const int cnt = or_tess_levels(P, dim, center, vmin, vmax, lv);
+        float col[4];
+        or_unpack_unorm4x8(sc->albedo[t], col);
+        for (int j = 0; j < cnt; ++j) {
+            float w[3];
+            or_tess_point(lv, j, w);
+            float p[3], nr[3];
+            for (int k = 0; k < 3; ++k) {
+                p[k] = (w[0] * P[0][k] + w[1] * P[1][k]) + w[2] * P[2][k];
+                nr[k] = (w[0] * N[0][k] + w[1] * N[1][k]) + w[2] * N[2][k];
+            }
+            int idx[3];
+            for (int k = 0; k < 3; ++k)
+                idx[k] = (int)((float)dim * (((p[k] - center[k]) - vmin[k]) / (vmax[k] - vmin[k])));
+            if (idx[0] < 0 || idx[1] < 0 || idx[2] < 0 || idx[0] >= dim || idx[1] >= dim || idx[2] >= dim) continue;
+            float nn[3], nenc[4];
+            nrm3(nr, nn);
+            for (int k = 0; k < 3; ++k) nenc[k] = nn[k] * 0.5f + 0.5f;
+            nenc[3] = 1.0f;
+            const size_t vi = ((size_t)idx[2] * dim + idx[1]) * dim + idx[0];
+            if (accum == 1) {
+                atomic_max_u32(&albedo[vi], or_pack_unorm4x8(col));
+                atomic_max_u32(&normal[vi], or_pack_unorm4x8(nenc));
+            } else {
+                const float c3[3] = {col[0], col[1], col[2]};
+                rgba8_avg(&albedo[vi], c3);
+                rgba8_avg(&normal[vi], nenc);
+            }
+        }
+    }
+}

@@ -169,6 +169,18 @@ typedef struct or_warp_params {
     int use_weights_texture;               /* useWarpmapWeightsTexture              */
 } or_warp_params;
 void or_warp_occupancy(const or_scene *sc, const or_voxelize_params *p /* dim 32 */, uint32_t *occ);
+
+/* tessellation voxelizer (simpleTesselated.vert, testTesselation.tesc/.tese):
+ * or_tess_levels -> the patch's point count (0: discarded) and lv = (inner,
+ * outer 0, 1, 2) after clamping and rounding; or_tess_point -> barycentric
+ * (u, v, w) of point j < count; or_voxelize_tess -> albedo/normal grids
+ * (linear x-fastest, zeroed by the caller), accum 1 atomicMax, 0 running avg */
+int or_tess_levels(const float P[3][3], int dim, const float center[3], const float vmin[3], const float vmax[3],
+                   int lv[4]);
+int or_tess_count(const int lv[4]);
+void or_tess_point(const int lv[4], int j, float uvw[3]);
+void or_voxelize_tess(const or_scene *sc, int dim, const float center[3], const float vmin[3], const float vmax[3],
+                      int accum, uint32_t *albedo, uint32_t *normal);
 void or_warpmap(const uint32_t *occ, const or_warp_params *p, float *out /* [32^3][4] */);
 /* texture(warpmap, tc).xyz: LINEAR, CLAMP_TO_EDGE */
 void or_warp_sample(const float *warpmap, const float tc[3], float out[3]);

@@ -136,6 +136,9 @@ class Context:
     def voxelize(self, p: abi.VoxelizeParams):
         self.check(self.lib.vct_voxelize(self.h, C.byref(p)))
 
+    def voxelize_tessellated(self, p: abi.VoxelizeParams):
+        self.check(self.lib.vct_voxelize_tessellated(self.h, C.byref(p)))
+
     def normalize_voxels(self, opacity: float = 0.5):
         """normalizeVoxels.comp after a VCT_ACCUM_SUM voxelization."""
         self.check(self.lib.vct_normalize_voxels(self.h, opacity))

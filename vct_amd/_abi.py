@@ -130,7 +130,7 @@ class AppConfig(C.Structure):
                 ("shadow_size", C.c_int), ("lighting", C.c_int), ("msaa", C.c_int), ("reflections", C.c_int),
                 ("fill_holes", C.c_int), ("device", C.c_int), ("temporal_filter", C.c_int),
                 ("temporal_decay", C.c_float), ("track_camera", C.c_int), ("warp_texture", C.c_int),
-                ("warp_high_resolution", C.c_float)]
+                ("warp_high_resolution", C.c_float), ("tessellation", C.c_int)]
 
 
 class LearnedParams(C.Structure):
@@ -156,6 +156,7 @@ VCT_FUNCS = [
     ("vct_shadowmap", C.c_int, [C.c_void_p, C.c_int]),
     ("vct_shadowmaps", C.c_int, [C.c_void_p]),
     ("vct_voxelize", C.c_int, [C.c_void_p, C.POINTER(VoxelizeParams)]),
+    ("vct_voxelize_tessellated", C.c_int, [C.c_void_p, C.POINTER(VoxelizeParams)]),
     ("vct_normalize_inject", C.c_int, [C.c_void_p, C.POINTER(InjectParams)]),
     ("vct_normalize_voxels", C.c_int, [C.c_void_p, C.c_float]),
     ("vct_normalize_inject_noalloc", C.c_int, [C.c_void_p, C.POINTER(InjectParams)]),

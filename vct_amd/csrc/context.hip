@@ -771,6 +771,22 @@ int vct_voxelize(vct_ctx *c, const vct_voxelize_params *p) {
     return VCT_OK;
 }
 
+int vct_voxelize_tessellated(vct_ctx *c, const vct_voxelize_params *p) {
+    if (!c || !p) return fail(c, VCT_ERR_INVALID, "vct_voxelize_tessellated: NULL argument");
+    if (p->atomic_max != VCT_ACCUM_AVG && p->atomic_max != VCT_ACCUM_MAX)
+        return fail(c, VCT_ERR_INVALID, "vct_voxelize_tessellated: atomic_max %d is not VCT_ACCUM_AVG or _MAX",
+                    p->atomic_max);
+    if (!c->d_model) return fail(c, VCT_ERR_STATE, "vct_voxelize_tessellated: no scene");
+    PassTimer pt(c, P_VOXELIZE);
+    LAUNCHCHK(c, launch_voxelize_tess(dev_scene(c), c->desc.center, c->desc.vmin, c->desc.vmax, c->desc.dim,
+                                      p->atomic_max, c->d_grid_albedo, c->d_grid_normal, c->d_stats, c->rs,
+                                      c->stream));
+    c->have_voxels = true;
+    c->have_mips = false;
+    c->have_sums = false;
+    return VCT_OK;
+}
+
 int vct_normalize_voxels(vct_ctx *c, float opacity) {
     if (!c) return fail(nullptr, VCT_ERR_INVALID, "ctx is NULL");
     if (!(opacity >= 0.0f && opacity <= 1.0f))

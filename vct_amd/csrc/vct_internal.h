@@ -213,6 +213,11 @@ hipError_t launch_gbuffer(const DevScene &sc, const vct_camera &cam, uint64_t *v
 hipError_t launch_voxelize(const DevScene &sc, const DevLight *dlights, int nlights, const vct_voxelize_params &p,
                            int dim, uint32_t *albedo, uint32_t *normal, uint32_t *sums, vct_stats *stats,
                            uint32_t *partials, const float4 *warp, RasterScratch &rs, hipStream_t s);
+// the tessellation voxelizer (accum: VCT_ACCUM_MAX or VCT_ACCUM_AVG) into the
+// brick-ordered albedo / normal grids of a dim^3 volume with the given extent
+hipError_t launch_voxelize_tess(const DevScene &sc, const float center[3], const float vmin[3], const float vmax[3],
+                                int dim, int accum, uint32_t *albedo, uint32_t *normal, vct_stats *stats,
+                                RasterScratch &rs, hipStream_t s);
 // normalizeVoxels.comp for the sum-accumulate mode (sums: 8 words per voxel)
 hipError_t launch_normalize_voxels(const uint32_t *sums, uint32_t *albedo, uint32_t *normal, int dim, float opacity,
                                    const PyrDesc &pyr, vct_stats *stats, uint32_t *partials, hipStream_t s);

@@ -410,6 +410,255 @@ hipError_t launch_voxelize(const DevScene &sc, const DevLight *dlights, int nlig
     return launch_reduce_stats(partials, nblk, &stats->total_voxel_fragments, nullptr, nullptr, s);
 }
 
+// ---------------------------------------------------------------- tessellation voxelizer
+// The voxelizer the reference runs when voxelizeTesselation is set
+// (Application.cpp:585-654): simpleTesselated.vert -> testTesselation.tesc ->
+// testTesselation.tese in point mode with the rasterizer discarded.  Every
+// patch (triangle) becomes a set of tessellation points and each point stores
+// the triangle's albedo and its interpolated normal in the voxel under it.
+//  - .tesc:40-49: a patch whose three vertices fall in one voxel,
+//    ivec3(dim * linear position), gets levels 0 and is discarded;
+//  - .tesc:51-75: inner level max(1, max altitude / voxelSize.x) (Heron's
+//    area), outer level of an edge max(1, length / length(normalize(edge) *
+//    voxelSize)), assigned OL0 = b - a, OL1 = c - b, OL2 = c - a as written;
+//  - point-mode equal_spacing tessellation of the triangle domain (GL 4.5
+//    core, 11.2.2.1): levels clamped to [1, 64] and rounded up, an inner level
+//    1 with an outer level above 1 taken as 2, all levels 1 giving just the
+//    corners; else the corners, OL_e - 1 points inside outer edge e (u = 0,
+//    v = 0, w = 0 for e = 0, 1, 2), and inner triangles k = 1 .. n/2 with
+//    corners (1 - 4k/3n, 2k/3n, 2k/3n) and rotations (the perpendiculars
+//    through the k-th subdivision points of the edges), each edge cut into
+//    n - 2k segments (0: the centre);
+//  - .tese:121-140: position and normal interpolated with (u, v, w), voxel
+//    ivec3(dim * linear position), colour the triangle's albedo (standing for
+//    its diffuse map), normal normalize(n) * 0.5 + 0.5, atomicMax or the RGBA8
+//    running average; no lighting, no fragment count.
+// A setup pass counts every patch's points (attribute block: world positions
+// and normals, the levels in the setup record); the scan spreads the points
+// over the threads, one point each.
+struct TessArgs {
+    int dim;
+    float center[3], vmin[3], vmax[3];
+};
+
+__device__ __forceinline__ int tess_count(const int lv[4]) {
+    const int n = lv[0];
+    if (n == 1 && lv[1] == 1 && lv[2] == 1 && lv[3] == 1) return 3;
+    int cnt = 3 + (lv[1] - 1) + (lv[2] - 1) + (lv[3] - 1);
+    for (int k = 1; 2 * k <= n; ++k) {
+        const int m = n - 2 * k;
+        cnt += m == 0 ? 1 : 3 * m;
+    }
+    return cnt;
+}
+
+__global__ __launch_bounds__(256) void k_tess_setup(DevScene sc, TessArgs ta, int4 *rec, float4 *attr,
+                                                    uint64_t *counts) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= sc.ntri) return;
+    const int o = (int)sc.obj[t];
+    const float *M = sc.model + (size_t)o * 16;
+    const float *Nm = sc.nmat + (size_t)o * 9;
+    float P[3][3], N[3][3];
+    for (int v = 0; v < 3; ++v) {
+        const float *p = sc.pos + (size_t)t * 9 + v * 3;
+        float p4[4] = {p[0], p[1], p[2], 1.0f}, w4[4];
+        mat4_vec4(M, p4, w4);
+        P[v][0] = w4[0];
+        P[v][1] = w4[1];
+        P[v][2] = w4[2];
+        mat3_vec3(Nm, sc.nrm + (size_t)t * 9 + v * 3, N[v]);
+    }
+    float4 *a = attr + (size_t)t * 8;  // P0 P1 P2 N0 N1 N2 as 18 floats
+    a[0] = make_float4(P[0][0], P[0][1], P[0][2], P[1][0]);
+    a[1] = make_float4(P[1][1], P[1][2], P[2][0], P[2][1]);
+    a[2] = make_float4(P[2][2], N[0][0], N[0][1], N[0][2]);
+    a[3] = make_float4(N[1][0], N[1][1], N[1][2], N[2][0]);
+    a[4] = make_float4(N[2][1], N[2][2], 0.0f, 0.0f);
+    int vox[3][3];
+    for (int v = 0; v < 3; ++v)
+        for (int k = 0; k < 3; ++k) {
+            const float tc = ((P[v][k] - ta.center[k]) - ta.vmin[k]) / (ta.vmax[k] - ta.vmin[k]);
+            vox[v][k] = (int)((float)ta.dim * tc);
+        }
+    bool same = true;
+    for (int k = 0; k < 3; ++k) same = same && vox[0][k] == vox[1][k] && vox[0][k] == vox[2][k];
+    if (same) {
+        counts[t] = 0;
+        return;
+    }
+    float E[3][3];  // A = c - b, B = c - a, C = b - a
+    for (int k = 0; k < 3; ++k) {
+        E[0][k] = P[2][k] - P[1][k];
+        E[1][k] = P[2][k] - P[0][k];
+        E[2][k] = P[1][k] - P[0][k];
+    }
+    float len[3];
+    for (int i = 0; i < 3; ++i) len[i] = __builtin_sqrtf(dot3(E[i], E[i]));
+    const float sp = ((len[0] + len[1]) + len[2]) * 0.5f;
+    const float area = __builtin_sqrtf(((sp * (sp - len[0])) * (sp - len[1])) * (sp - len[2]));
+    float maxalt = 0.0f;
+    for (int i = 0; i < 3; ++i) {
+        const float alt = (2.0f * area) / len[i];
+        maxalt = i == 0 ? alt : __builtin_fmaxf(maxalt, alt);
+    }
+    float vs[3];
+    for (int k = 0; k < 3; ++k) vs[k] = (ta.vmax[k] - ta.vmin[k]) / (float)ta.dim;
+    float outer[3];
+    for (int i = 0; i < 3; ++i) {
+        float n[3];
+        nrm3(E[i], n);
+        const float d[3] = {n[0] * vs[0], n[1] * vs[1], n[2] * vs[2]};
+        const float div = __builtin_fabsf(__builtin_sqrtf(dot3(d, d)));
+        outer[i] = __builtin_fmaxf(1.0f, len[i] / div);
+    }
+    const float lev[4] = {__builtin_fmaxf(1.0f, maxalt / vs[0]), outer[2], outer[0], outer[1]};
+    int lv[4];
+    for (int i = 0; i < 4; ++i) {
+        const float x = lev[i] > 64.0f ? 64.0f : lev[i];
+        lv[i] = (int)__builtin_ceilf(x);
+    }
+    if (lv[0] == 1 && (lv[1] > 1 || lv[2] > 1 || lv[3] > 1)) lv[0] = 2;
+    rec[(size_t)t * 4] = make_int4(lv[0], lv[1], lv[2], lv[3]);
+    counts[t] = (uint64_t)tess_count(lv);
+}
+
+// barycentric (u, v, w) of point j of a patch with levels lv (the order of
+// the oracle's or_tess_point: corners, outer edges 0..2, inner rings outward in)
+__device__ __forceinline__ void tess_point(const int lv[4], int j, float w[3]) {
+    if (j < 3) {
+        w[0] = j == 0 ? 1.0f : 0.0f;
+        w[1] = j == 1 ? 1.0f : 0.0f;
+        w[2] = j == 2 ? 1.0f : 0.0f;
+        return;
+    }
+    j -= 3;
+    for (int e = 0; e < 3; ++e) {
+        const int m = lv[1 + e];
+        if (j < m - 1) {
+            const float f = (float)(j + 1) / (float)m;
+            const float g = 1.0f - f;
+            if (e == 0) {
+                w[0] = 0.0f;
+                w[1] = g;
+                w[2] = f;
+            } else if (e == 1) {
+                w[0] = f;
+                w[1] = 0.0f;
+                w[2] = g;
+            } else {
+                w[0] = g;
+                w[1] = f;
+                w[2] = 0.0f;
+            }
+            return;
+        }
+        j -= m - 1;
+    }
+    const int n = lv[0];
+    for (int k = 1; 2 * k <= n; ++k) {
+        const int m = n - 2 * k;
+        const float cc = (2.0f * (float)k) / (3.0f * (float)n);
+        const float oo = 1.0f - 2.0f * cc;
+        const int cnt = m == 0 ? 1 : 3 * m;
+        if (j < cnt) {
+            if (m == 0) {
+                w[0] = oo;
+                w[1] = cc;
+                w[2] = cc;
+                return;
+            }
+            const int e = j / m, s = j % m;
+            const float f = (float)s / (float)m;
+            // corners R0 = (o, c, c), R1 = (c, o, c), R2 = (c, c, o); edge e: R_e -> R_(e+1)%3
+            float ra[3] = {cc, cc, cc}, rb[3] = {cc, cc, cc};
+            ra[e] = oo;
+            rb[(e + 1) % 3] = oo;
+            for (int q = 0; q < 3; ++q) w[q] = ra[q] + f * (rb[q] - ra[q]);
+            return;
+        }
+        j -= cnt;
+    }
+    w[0] = w[1] = w[2] = 0.0f;
+}
+
+__global__ __launch_bounds__(256) void k_tess_points(DevScene sc, TessArgs ta, const int4 *__restrict__ rec,
+                                                     const float4 *__restrict__ attr, const uint64_t *__restrict__ offs,
+                                                     const uint32_t *__restrict__ chunk, size_t chunk_cap, int accum,
+                                                     uint32_t *albedo, uint32_t *normal) {
+    const int nslots = sc.ntri;
+    const uint64_t total = offs[nslots];
+    const uint64_t nchunks = (total + 63) >> 6;
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total;
+         k += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t c = k >> 6;
+        int lo = c < chunk_cap ? (int)chunk[c] : 0;
+        int hi = (c + 1 < nchunks && c + 1 < chunk_cap) ? (int)chunk[c + 1] : nslots - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (offs[mid] <= k) lo = mid;
+            else hi = mid - 1;
+        }
+        const int t = lo;
+        const int j = (int)(k - offs[t]);
+        const int4 r = rec[(size_t)t * 4];
+        const int lv[4] = {r.x, r.y, r.z, r.w};
+        float w[3];
+        tess_point(lv, j, w);
+        const float4 *a = attr + (size_t)t * 8;
+        const float4 a0 = a[0], a1 = a[1], a2 = a[2], a3 = a[3], a4 = a[4];
+        const float P[3][3] = {{a0.x, a0.y, a0.z}, {a0.w, a1.x, a1.y}, {a1.z, a1.w, a2.x}};
+        const float N[3][3] = {{a2.y, a2.z, a2.w}, {a3.x, a3.y, a3.z}, {a3.w, a4.x, a4.y}};
+        float p[3], nr[3];
+        for (int q = 0; q < 3; ++q) {
+            p[q] = (w[0] * P[0][q] + w[1] * P[1][q]) + w[2] * P[2][q];
+            nr[q] = (w[0] * N[0][q] + w[1] * N[1][q]) + w[2] * N[2][q];
+        }
+        int idx[3];
+        for (int q = 0; q < 3; ++q)
+            idx[q] = (int)((float)ta.dim * (((p[q] - ta.center[q]) - ta.vmin[q]) / (ta.vmax[q] - ta.vmin[q])));
+        if (idx[0] < 0 || idx[1] < 0 || idx[2] < 0 || idx[0] >= ta.dim || idx[1] >= ta.dim || idx[2] >= ta.dim)
+            continue;
+        float nn[3];
+        nrm3(nr, nn);
+        const float ne0 = nn[0] * 0.5f + 0.5f, ne1 = nn[1] * 0.5f + 0.5f, ne2 = nn[2] * 0.5f + 0.5f;
+        const uint32_t alb = sc.albedo[t];
+        const uint32_t vi = grid_index(idx[0], idx[1], idx[2], ta.dim >> 2);
+        if (accum == VCT_ACCUM_MAX) {
+            atomicMax(&albedo[vi], pack_unorm4x8(unorm8(alb, 0), unorm8(alb, 1), unorm8(alb, 2), unorm8(alb, 3)));
+            atomicMax(&normal[vi], pack_unorm4x8(ne0, ne1, ne2, 1.0f));
+        } else {
+            rgba8_avg(&albedo[vi], unorm8(alb, 0), unorm8(alb, 1), unorm8(alb, 2));
+            rgba8_avg(&normal[vi], ne0, ne1, ne2);
+        }
+    }
+}
+
+hipError_t launch_voxelize_tess(const DevScene &sc, const float center[3], const float vmin[3], const float vmax[3],
+                                int dim, int accum, uint32_t *albedo, uint32_t *normal, vct_stats *stats,
+                                RasterScratch &rs, hipStream_t s) {
+    const size_t nv = (size_t)dim * dim * dim;
+    hipError_t e = launch_fill_u32(albedo, 0u, nv, s);  // glClearTexImage (Application.cpp:627-628)
+    if (e != hipSuccess) return e;
+    if ((e = launch_fill_u32(normal, 0u, nv, s)) != hipSuccess) return e;
+    // the voxelizeInfo buffer is cleared (Application.cpp:581) and this path never counts
+    if ((e = launch_fill_u32(&stats->total_voxel_fragments, 0u, 1, s)) != hipSuccess) return e;
+    if (sc.ntri == 0) return hipSuccess;
+    TessArgs ta;
+    ta.dim = dim;
+    for (int k = 0; k < 3; ++k) {
+        ta.center[k] = center[k];
+        ta.vmin[k] = vmin[k];
+        ta.vmax[k] = vmax[k];
+    }
+    hipLaunchKernelGGL(k_tess_setup, dim3((sc.ntri + 255) / 256), dim3(256), 0, s, sc, ta, rs.rec, rs.attr, rs.counts);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = raster_scan(rs, (size_t)sc.ntri, s)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_tess_points, dim3(kRasterBlocks), dim3(256), 0, s, sc, ta, rs.rec, rs.attr, rs.offs, rs.chunk,
+                       rs.chunk_cap, accum, albedo, normal);
+    return hipGetLastError();
+}
+
 // normalizeVoxels.comp:19-42 for the sum-accumulate mode: per voxel with
 // fragments, the averages colour = sum / count (alpha 1, then voxelSetOpacity
 // when it is > 0) and normal = sum / count, written to the RGBA8 grids with

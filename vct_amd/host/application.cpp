@@ -872,7 +872,7 @@ int vct_app_build_radiance(vct_app *a, int frame) {
         vct_app_warpmap_params(a, &wp);
         if ((rc = vct_warpmap(a->ctx, &vox, &wp))) return rc;
     }
-    if ((rc = vct_voxelize(a->ctx, &vox))) return rc;
+    if ((rc = a->cfg.tessellation ? vct_voxelize_tessellated(a->ctx, &vox) : vct_voxelize(a->ctx, &vox))) return rc;
     if ((rc = vct_normalize_inject(a->ctx, &inj))) return rc;
     if (a->cfg.fill_holes && (rc = vct_fill_holes(a->ctx))) return rc;
     if ((rc = vct_mip(a->ctx, &mip))) return rc;
