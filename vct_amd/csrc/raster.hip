@@ -123,9 +123,12 @@ __global__ __launch_bounds__(256) void k_block_slots(const uint64_t *__restrict_
 // wavefront takes a contiguous run of blocks, so consecutive blocks mostly
 // belong to one triangle and its record (wave-uniform) is read again only when
 // the run enters the next triangle (its slot from k_block_slots, or the same
-// search past that table).  For the 64-bit edge form each lane steps the edge
-// functions from the block corner (E_i = E_i(corner) + A_i dx + B_i dy, the
-// same integers as edge_value), so every fragment and its barycentrics are
+// search past that table).  Each lane's edge values are the block corner's
+// (wave-uniform) plus the lane's offset A_i dx + B_i dy, set once per triangle:
+// in 32-bit wrapping arithmetic for the small form (the same value modulo 2^32
+// as edge_value's sum), in FP64 for the 64-bit form (every term is an integer
+// below 2^53, so each sum is exact).  The barycentrics are quot_f32(E, area)
+// with the triangle's reciprocal, so every fragment and its barycentrics are
 // those the pixel walk produces.  F provides enter(slot, edges) (the slot's
 // wave-uniform data, and the image size W x H) and frag(edges, px, py, l).
 template <class F>
@@ -144,6 +147,10 @@ __global__ __launch_bounds__(256) void k_raster_blocks(const int4 *__restrict__ 
     int slot = 0;
     Edges e;
     uint32_t nbx = 1, bxi = 0, byi = 0;  // the block's column and row in the triangle's bounding box
+    double rcp = 0.0;                    // RN64(1 / area)
+    uint32_t off32[3] = {0u, 0u, 0u};    // small form: A_i dx + B_i dy of this lane (mod 2^32)
+    double offd[3] = {0.0, 0.0, 0.0}, Ad[3] = {0.0, 0.0, 0.0}, Bd[3] = {0.0, 0.0, 0.0}, Cd[3] = {0.0, 0.0, 0.0};
+    const int dxl = (lane & 7) * 256, dyl = (lane >> 3) * 256;
     for (uint64_t k = k0; k < k1; ++k, ++bxi) {
         if (bxi == nbx) {
             bxi = 0;
@@ -172,28 +179,48 @@ __global__ __launch_bounds__(256) void k_raster_blocks(const int4 *__restrict__ 
             const uint32_t local = (uint32_t)(k - run_lo);  // the run may enter a triangle mid-way
             bxi = local % nbx;
             byi = local / nbx;
+            rcp = 1.0 / (double)e.area;
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                if (e.small) {
+                    off32[i] = (uint32_t)e.A[i] * (uint32_t)dxl + (uint32_t)e.B[i] * (uint32_t)dyl;
+                } else {
+                    Ad[i] = (double)e.A[i];
+                    Bd[i] = (double)e.B[i];
+                    Cd[i] = (double)e.C[i];
+                    offd[i] = Ad[i] * (double)dxl + Bd[i] * (double)dyl;
+                }
+            }
         }
         const int bx = ((e.x0 >> 3) + (int)bxi) * 8, by = ((e.y0 >> 3) + (int)byi) * 8;
         const int px = bx + (lane & 7), py = by + (lane >> 3);
         if (px >= f.W || py >= f.H) continue;  // the block overhangs the image
         float l[3];
+        bool in = true;
         if (e.small) {
-            if (!edge_inside(e, px, py, 128, 128)) continue;
-            edge_bary(e, px, py, 128, 128, l);
-        } else {
-            int64_t E[3];
-            bool in = true;
-            const int64_t sx = (int64_t)bx * 256 + 128, sy = (int64_t)by * 256 + 128;
-            const int dx = (lane & 7) * 256, dy = (lane >> 3) * 256;
+            // edge_value's dx = ((px - x0) << 8) + 128 = cx + dxl, likewise dy
+            const uint32_t cx = (uint32_t)(((bx - e.x0) << 8) + 128), cy = (uint32_t)(((by - e.y0) << 8) + 128);
+            int32_t E[3];
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
-                const int64_t corner = (int64_t)e.A[i] * sx + (int64_t)e.B[i] * sy + e.C[i];
-                E[i] = corner + (int64_t)e.A[i] * dx + (int64_t)e.B[i] * dy;
-                in = in && (E[i] + e.bias[i] >= 0);
+                const uint32_t corner = (uint32_t)e.E0[i] + (uint32_t)e.A[i] * cx + (uint32_t)e.B[i] * cy;
+                E[i] = (int32_t)(corner + off32[i]);
+                in = in && ((int32_t)((uint32_t)E[i] + (uint32_t)e.bias[i]) >= 0);
             }
             if (!in) continue;
 #pragma unroll
-            for (int i = 0; i < 3; ++i) l[i] = (float)E[i] / e.area;
+            for (int i = 0; i < 3; ++i) l[i] = quot_f32((float)E[i], rcp);
+        } else {
+            const double sx = (double)(bx * 256 + 128), sy = (double)(by * 256 + 128);
+            double E[3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                E[i] = fma(Ad[i], sx, fma(Bd[i], sy, Cd[i])) + offd[i];
+                in = in && (E[i] >= (double)(-e.bias[i]));
+            }
+            if (!in) continue;
+#pragma unroll
+            for (int i = 0; i < 3; ++i) l[i] = quot_f32((float)E[i], rcp);
         }
         f.frag(e, px, py, l);
     }
