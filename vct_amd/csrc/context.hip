@@ -419,7 +419,7 @@ void vct_destroy(vct_ctx *c) {
     dfree(c->rs.boffs);
     dfree(c->rs.bchunk);
     dfree(c->rs.bslots);
-    if (c->rs.scan_tmp) (void)hipFree(c->rs.scan_tmp);
+    dfree(c->rs.tsum);
     dfree(c->d_gbuf);
     dfree(c->d_vis);
     dfree(c->d_ind);
@@ -508,16 +508,15 @@ static int ensure_raster(vct_ctx *c, size_t slots) {
     dfree(c->rs.offs);
     dfree(c->rs.bcounts);
     dfree(c->rs.boffs);
-    if (c->rs.scan_tmp) (void)hipFree(c->rs.scan_tmp);
-    c->rs.scan_tmp = nullptr;
+    dfree(c->rs.tsum);
     HIPCHK(c, dalloc(&c->rs.rec, slots * 4 * sizeof(int4)));
     HIPCHK(c, dalloc(&c->rs.attr, slots * 8 * sizeof(float4)));
-    HIPCHK(c, dalloc(&c->rs.counts, (slots + 1) * sizeof(uint64_t)));
+    HIPCHK(c, dalloc(&c->rs.counts, (slots + 1) * sizeof(uint32_t)));
     HIPCHK(c, dalloc(&c->rs.offs, (slots + 1) * sizeof(uint64_t)));
-    HIPCHK(c, dalloc(&c->rs.bcounts, (slots + 1) * sizeof(uint64_t)));
+    HIPCHK(c, dalloc(&c->rs.bcounts, (slots + 1) * sizeof(uint32_t)));
     HIPCHK(c, dalloc(&c->rs.boffs, (slots + 1) * sizeof(uint64_t)));
-    c->rs.scan_tmp_bytes = raster_scan_tmp_bytes(slots);
-    HIPCHK(c, hipMalloc(&c->rs.scan_tmp, std::max<size_t>(c->rs.scan_tmp_bytes, 256)));
+    c->rs.scan_tiles_cap = raster_scan_tiles(slots);
+    HIPCHK(c, dalloc(&c->rs.tsum, (4 * c->rs.scan_tiles_cap + 4) * sizeof(uint64_t)));
     if (!c->rs.chunk) {
         c->rs.chunk_cap = (size_t)1 << 22;  // chunk starts for 256M items; beyond that a full search is used
         HIPCHK(c, dalloc(&c->rs.chunk, c->rs.chunk_cap * sizeof(uint32_t)));

@@ -50,7 +50,7 @@ __device__ __forceinline__ uint64_t bbox_blocks(const int4 r[4], uint64_t cnt) {
 }
 
 __global__ __launch_bounds__(256) void k_shadow_setup(DevScene sc, const DevLight *__restrict__ lights, uint64_t li,
-                                                      int4 *rec, float4 *attr, uint64_t *counts, uint64_t *bcounts) {
+                                                      int4 *rec, float4 *attr, uint32_t *counts, uint32_t *bcounts) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= sc.ntri) return;
     const int l = (int)((li >> (5u * blockIdx.y)) & 31u);
@@ -87,11 +87,11 @@ __global__ __launch_bounds__(256) void k_shadow_setup(DevScene sc, const DevLigh
         }
         if (cnt > kBlockPixels) {  // 8x8 blocks of the bounding box instead of its pixels
             counts[slot] = 0;
-            bcounts[slot] = bbox_blocks(r, cnt);
+            bcounts[slot] = (uint32_t)bbox_blocks(r, cnt);
             return;
         }
     }
-    counts[slot] = cnt;
+    counts[slot] = (uint32_t)cnt;
     bcounts[slot] = 0;
 }
 
@@ -265,6 +265,8 @@ struct ShadowFrag {
     __device__ void finish() {}
 };
 
+__global__ void k_shadow_quads_batch(const DevLight *__restrict__ lights, uint64_t li);
+
 hipError_t launch_shadowmaps(const DevScene &sc, const DevLight *hlights, const DevLight *dlights, const int *li,
                              int n, RasterScratch &rs, hipStream_t s) {
     hipError_t e;
@@ -285,8 +287,7 @@ hipError_t launch_shadowmaps(const DevScene &sc, const DevLight *hlights, const 
             hipLaunchKernelGGL(k_shadow_setup, dim3((sc.ntri + 255) / 256, nb), dim3(256), 0, s, sc, dlights, packed,
                                rs.rec, rs.attr, rs.counts, rs.bcounts);
             if ((e = hipGetLastError()) != hipSuccess) return e;
-            if ((e = raster_scan(rs, nslots, s)) != hipSuccess) return e;
-            if ((e = raster_scan_blocks(rs, nslots, s)) != hipSuccess) return e;
+            if ((e = raster_scan(rs, nslots, true, s)) != hipSuccess) return e;
             ShadowFrag f{rs.attr, dlights};
             hipLaunchKernelGGL(k_raster_items<ShadowFrag>, dim3(raster_grid<ShadowFrag, false>()), dim3(256), 0, s,
                                rs.rec, rs.offs, rs.chunk, rs.chunk_cap, (int)nslots, f);
@@ -299,25 +300,45 @@ hipError_t launch_shadowmaps(const DevScene &sc, const DevLight *hlights, const 
                                rs.boffs, rs.bchunk, rs.chunk_cap, (int)nslots, rs.bslots, rs.bslots_cap, fb);
             if ((e = hipGetLastError()) != hipSuccess) return e;
         }
+        // the footprint layout of every map of the walk, one launch
+        int n1max = 0;
         for (int j = 0; j < nb; ++j) {
             const DevLight &L = hlights[li[j0 + j]];
-            if ((e = launch_shadow_quads(L.shadowmap, (int)L.shadow_size, const_cast<float4 *>(L.shadowquad), s)) !=
-                hipSuccess)
-                return e;
+            const int n1 = (int)L.shadow_size + 1;
+            if (L.shadowquad && n1 * n1 > n1max) n1max = n1 * n1;
+        }
+        if (n1max > 0) {
+            hipLaunchKernelGGL(k_shadow_quads_batch, dim3((n1max + 255) / 256, nb), dim3(256), 0, s, dlights, packed);
+            if ((e = hipGetLastError()) != hipSuccess) return e;
         }
     }
     return hipSuccess;
 }
 
-__global__ __launch_bounds__(256) void k_shadow_quads(const float *__restrict__ sm, int N, float4 *q) {
+__device__ __forceinline__ void shadow_quad(const float *__restrict__ sm, int N, float4 *q, int idx) {
     const int n1 = N + 1;
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= n1 * n1) return;
     const int i = idx % n1 - 1, j = idx / n1 - 1;
     auto tex = [&](int x, int y) -> float {
         return (x >= 0 && x < N && y >= 0 && y < N) ? sm[(size_t)y * N + x] : 1.0f;
     };
     q[idx] = make_float4(tex(i, j), tex(i + 1, j), tex(i, j + 1), tex(i + 1, j + 1));
+}
+
+__global__ __launch_bounds__(256) void k_shadow_quads(const float *__restrict__ sm, int N, float4 *q) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (N + 1) * (N + 1)) return;
+    shadow_quad(sm, N, q, idx);
+}
+
+// The footprint layouts of a shadow walk's maps: blockIdx.y is the walk's
+// j-th light (its index at bits 5j .. 5j+4 of li); lights without a layout
+// buffer are skipped.
+__global__ __launch_bounds__(256) void k_shadow_quads_batch(const DevLight *__restrict__ lights, uint64_t li) {
+    const DevLight &L = lights[(li >> (5u * blockIdx.y)) & 31u];
+    const int N = (int)L.shadow_size;
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!L.shadowquad || idx >= (N + 1) * (N + 1)) return;
+    shadow_quad(L.shadowmap, N, const_cast<float4 *>(L.shadowquad), idx);
 }
 
 hipError_t launch_shadow_quads(const float *depth, int N, float4 *quads, hipStream_t s) {
@@ -334,7 +355,7 @@ struct CamMats {
 };
 
 __global__ __launch_bounds__(256) void k_gbuf_setup(DevScene sc, CamMats cm, int4 *rec, float4 *attr,
-                                                    uint64_t *counts, uint64_t *bcounts) {
+                                                    uint32_t *counts, uint32_t *bcounts) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= sc.ntri) return;
     float clip[3][4];
@@ -388,12 +409,12 @@ __global__ __launch_bounds__(256) void k_gbuf_setup(DevScene sc, CamMats cm, int
                 }
                 if (cnt > kBlockPixels) {  // 8x8 blocks of the bounding box instead of its pixels
                     counts[slot] = 0;
-                    bcounts[slot] = bbox_blocks(r, cnt);
+                    bcounts[slot] = (uint32_t)bbox_blocks(r, cnt);
                     continue;
                 }
             }
         }
-        counts[slot] = cnt;
+        counts[slot] = (uint32_t)cnt;
         bcounts[slot] = 0;
     }
 }
@@ -521,8 +542,7 @@ hipError_t launch_gbuffer(const DevScene &sc, const vct_camera &cam, uint64_t *v
         hipLaunchKernelGGL(k_gbuf_setup, dim3((sc.ntri + 255) / 256), dim3(256), 0, s, sc, cm, rs.rec, rs.attr,
                            rs.counts, rs.bcounts);
         if ((e = hipGetLastError()) != hipSuccess) return e;
-        if ((e = raster_scan(rs, nslots, s)) != hipSuccess) return e;
-        if ((e = raster_scan_blocks(rs, nslots, s)) != hipSuccess) return e;
+        if ((e = raster_scan(rs, nslots, true, s)) != hipSuccess) return e;
         GbufFrag f{rs.attr, (unsigned long long *)vis, cam.width};
         hipLaunchKernelGGL(k_raster_items<GbufFrag>, dim3(raster_grid<GbufFrag, false>()), dim3(256), 0, s, rs.rec, rs.offs,
                            rs.chunk, rs.chunk_cap, (int)nslots, f);

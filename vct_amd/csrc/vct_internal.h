@@ -108,21 +108,22 @@ struct DevScene {
 
 // Scratch for the binned rasterizer: per-slot setup records (4 x int4), an
 // attribute block of `attr_f4` float4 per slot, per-slot fragment-candidate
-// counts, their exclusive scan (nslots + 1 entries, last = total) and the
-// first slot of every 64-item chunk.
+// counts (below 2^32: a bounding box is at most 65535 x 65535 pixels), their
+// exclusive scan (nslots + 1 entries, last = total) and the first slot of
+// every 64-item chunk.
 struct RasterScratch {
     int4 *rec;
     float4 *attr;
-    uint64_t *counts;
+    uint32_t *counts;
     uint64_t *offs;
     uint32_t *chunk;
     size_t chunk_cap;
-    void *scan_tmp;
-    size_t scan_tmp_bytes;
+    uint64_t *tsum;        // the scan's tile sums and tile prefixes, 2 x 2 x scan_tiles_cap
+    size_t scan_tiles_cap;
     size_t cap_slots;
-    // the shadow pass's 8x8-block walk of its large triangles: per slot block
-    // counts, their scan and its chunk starts (same capacities as above)
-    uint64_t *bcounts;
+    // the 8x8-block walk of the large triangles (shadow maps, G-buffer): per
+    // slot block counts, their scan and its chunk starts (same capacities as above)
+    uint32_t *bcounts;
     uint64_t *boffs;
     uint32_t *bchunk;
     uint32_t *bslots;   // the slot of each block, for the first bslots_cap blocks
@@ -186,11 +187,11 @@ hipError_t launch_fill_u64(uint64_t *p, uint64_t v, size_t n, hipStream_t s);
 hipError_t launch_reduce_stats(const uint32_t *partials, int n, uint32_t *sum0, uint32_t *max1, uint32_t *sum2,
                                hipStream_t s);
 
-// scan.hip: counts[0..n) -> offs[0..n] exclusive (offs[n] = total) and chunk starts
-hipError_t raster_scan(RasterScratch &rs, size_t nslots, hipStream_t s);
-// the same for the block counts: bcounts -> boffs, bchunk
-hipError_t raster_scan_blocks(RasterScratch &rs, size_t nslots, hipStream_t s);
-size_t raster_scan_tmp_bytes(size_t nslots);
+// scan.hip: counts[0..n) -> offs[0..n] exclusive (offs[n] = total) and chunk
+// starts; with `blocks` the same for the block counts (bcounts -> boffs, bchunk)
+hipError_t raster_scan(RasterScratch &rs, size_t nslots, bool blocks, hipStream_t s);
+// the scan's tiles for nslots slots (rs.tsum holds 4 words per tile)
+size_t raster_scan_tiles(size_t nslots);
 
 // raster.hip
 // Shadow maps of the lights li[0 .. n): hlights / dlights are the same light

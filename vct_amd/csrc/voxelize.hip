@@ -23,7 +23,7 @@ struct VoxMats {
 //  f[0..8]  ndc xyz of v0,v1,v2        f[9..17]  world positions
 //  f[18..26] world normals             f[27] albedo bits  f[28] axis | zclip<<4
 __global__ __launch_bounds__(256) void k_vox_setup(DevScene sc, VoxMats vm, int4 *rec, float4 *attr,
-                                                   uint64_t *counts) {
+                                                   uint32_t *counts) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= sc.ntri) return;
     const int o = (int)sc.obj[t];
@@ -77,7 +77,7 @@ __global__ __launch_bounds__(256) void k_vox_setup(DevScene sc, VoxMats vm, int4
         float4 *a = attr + (size_t)t * 8;
         for (int i = 0; i < 8; ++i) a[i] = make_float4(f[i * 4], f[i * 4 + 1], f[i * 4 + 2], f[i * 4 + 3]);
     }
-    counts[t] = cnt;
+    counts[t] = (uint32_t)cnt;
 }
 
 // calcShadowFactor (shaders/voxelize.frag:160-184)
@@ -356,7 +356,7 @@ hipError_t launch_warp_occupancy(const DevScene &sc, const vct_voxelize_params &
     vm.dim = kWarpDim;  // glViewport(0, 0, voxelOccupancyDim, voxelOccupancyDim) (:239)
     hipLaunchKernelGGL(k_vox_setup, dim3((sc.ntri + 255) / 256), dim3(256), 0, s, sc, vm, rs.rec, rs.attr, rs.counts);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    if ((e = raster_scan(rs, (size_t)sc.ntri, s)) != hipSuccess) return e;
+    if ((e = raster_scan(rs, (size_t)sc.ntri, false, s)) != hipSuccess) return e;
     OccFrag f;
     f.attr = rs.attr;
     f.msaa = p.msaa;
@@ -386,7 +386,7 @@ hipError_t launch_voxelize(const DevScene &sc, const DevLight *dlights, int nlig
     vm.dim = dim;
     hipLaunchKernelGGL(k_vox_setup, dim3((sc.ntri + 255) / 256), dim3(256), 0, s, sc, vm, rs.rec, rs.attr, rs.counts);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    if ((e = raster_scan(rs, (size_t)sc.ntri, s)) != hipSuccess) return e;
+    if ((e = raster_scan(rs, (size_t)sc.ntri, false, s)) != hipSuccess) return e;
     VoxFrag f;
     f.attr = rs.attr;
     f.lights = dlights;
@@ -453,7 +453,7 @@ __device__ __forceinline__ int tess_count(const int lv[4]) {
 }
 
 __global__ __launch_bounds__(256) void k_tess_setup(DevScene sc, TessArgs ta, int4 *rec, float4 *attr,
-                                                    uint64_t *counts) {
+                                                    uint32_t *counts) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= sc.ntri) return;
     const int o = (int)sc.obj[t];
@@ -520,7 +520,7 @@ __global__ __launch_bounds__(256) void k_tess_setup(DevScene sc, TessArgs ta, in
     }
     if (lv[0] == 1 && (lv[1] > 1 || lv[2] > 1 || lv[3] > 1)) lv[0] = 2;
     rec[(size_t)t * 4] = make_int4(lv[0], lv[1], lv[2], lv[3]);
-    counts[t] = (uint64_t)tess_count(lv);
+    counts[t] = (uint32_t)tess_count(lv);
 }
 
 // barycentric (u, v, w) of point j of a patch with levels lv (the order of
@@ -653,7 +653,7 @@ hipError_t launch_voxelize_tess(const DevScene &sc, const float center[3], const
     }
     hipLaunchKernelGGL(k_tess_setup, dim3((sc.ntri + 255) / 256), dim3(256), 0, s, sc, ta, rs.rec, rs.attr, rs.counts);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    if ((e = raster_scan(rs, (size_t)sc.ntri, s)) != hipSuccess) return e;
+    if ((e = raster_scan(rs, (size_t)sc.ntri, false, s)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_tess_points, dim3(kRasterBlocks), dim3(256), 0, s, sc, ta, rs.rec, rs.attr, rs.offs, rs.chunk,
                        rs.chunk_cap, accum, albedo, normal);
     return hipGetLastError();
