@@ -105,7 +105,29 @@ typedef struct vct_mesh {
                                   corner's vertex (Mesh.cpp:190-197: the normalized
                                   sum over the faces sharing it), or NULL: every
                                   corner takes its triangle's `tangents` entry    */
+    const float *texcoords;    /* [num_tris][3][2] texture coordinates (s, t) of
+                                  each corner (Mesh.cpp:160-162), or NULL         */
+    const int32_t *texture;    /* [num_tris] the triangle's diffuse map, an index
+                                  into the vct_set_textures table, or -1: its
+                                  `albedo` is its colour (material.diffuse, or a
+                                  1x1 map); NULL: -1 for every triangle         */
 } vct_mesh;
+
+/* One diffuse map (Mesh.cpp:60-81 loads them; ResourceLoader.h:26-108 for DDS),
+ * sampled as the reference's sampler state sets it (ResourceLoader.h:79-82,
+ * GLHelper.cpp:180-183): LINEAR_MIPMAP_NEAREST minification, NEAREST
+ * magnification, REPEAT in s and t; isotropic (the reference asks for the
+ * driver's maximum anisotropy, whose footprint is implementation-defined).
+ * A texture with fewer than floor(log2(max(width, height))) + 1 levels is
+ * mipmap-incomplete and samples as (0, 0, 0, 1), as in GL. */
+typedef struct vct_texture {
+    int width, height;        /* level 0, each in [1, 32768]                       */
+    int levels;               /* levels in `texels`; level l is max(1, width >> l)
+                                 x max(1, height >> l)                             */
+    const uint32_t *texels;   /* RGBA8 (R in the low byte), the levels one after
+                                 another, each row-major from row t = 0 (the first
+                                 row glCompressedTexImage2D / glTexSubImage2D take) */
+} vct_texture;
 
 /* One light, replacing the std140 SSBO record of Light::writeSSBO
  * (src/Scene.h:13-34, src/Scene.cpp:64-76) plus its light-space matrix
@@ -268,6 +290,10 @@ int vct_set_extent(vct_ctx *ctx, const float center[3], const float vmin[3],
 /* ---- scene: Scene::addActor / Mesh upload / Scene::addLight --------------- */
 /* Uploads geometry (src/Scene.cpp:31-36 draws every actor; Mesh.cpp:268-374). */
 int vct_set_scene(vct_ctx *ctx, const vct_mesh *mesh);
+/* The diffuse maps the scene's triangles index (Mesh::draw binds each
+ * drawable's map to unit 0, Mesh.cpp:340): replaces the table.  A triangle
+ * whose index is not in the table samples (0, 0, 0, 1), as an unbound unit. */
+int vct_set_textures(vct_ctx *ctx, const vct_texture *textures, int count);
 /* Per-actor model matrices, the `model` uniform set per actor by Scene::draw
  * (src/Scene.cpp:33; Transform::getMatrix, src/Transform.h:98-104).  The
  * normal matrix mat3(transpose(inverse(model))) of voxelize.vert:18 and

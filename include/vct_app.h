@@ -67,18 +67,40 @@ typedef struct vct_app_config {
                             instead of the GS path                             */
 } vct_app_config;
 
+/* ---- DDS textures: ResourceLoader::loadDDS (src/ResourceLoader.h:26-108) ----------
+ * "DDS " + the 124-byte header, then DXT1 / DXT3 / DXT5 (S3TC) blocks, level
+ * after level: level l is (width >> l) x (height >> l), ((w + 3) / 4) *
+ * ((h + 3) / 4) blocks, for l < dwMipMapCount while a dimension is non-zero.
+ * Decoded by EXT_texture_compression_s3tc into RGBA8 (each channel the exact
+ * value rounded to the nearest byte; DXT1 opaque, as COMPRESSED_RGB_S3TC_DXT1;
+ * DXT3/DXT5 colour blocks always in four-colour mode).  Other four-character
+ * codes are rejected, as the reference returns 0 for them.  The reference's
+ * loop uploads a level with a zero dimension as an empty image, which leaves
+ * the texture incomplete; here the level list stops there and the shorter
+ * chain is incomplete in the same way. */
+typedef struct vct_image vct_image;
+int vct_dds_load(const char *path, vct_image **out, char *err, int errlen);
+/* a view of the decoded levels (valid until vct_dds_free) */
+int vct_dds_texture(const vct_image *img, vct_texture *out);
+void vct_dds_free(vct_image *img);
+
 /* ---- OBJ/MTL assets: Mesh::loadMesh (src/Graphics/Mesh.cpp:42-266) ------------
  * Faces triangulated (fans), texture v flipped, per-face tangents from the uv
  * deltas (Mesh.cpp:174-189), albedo = the material's Kd as RGBA8 (white
  * without a material), roughness = the material's Pr standing in for a
- * constant roughness map.  Textures are not decoded; a face without normals
- * gets its face normal, one without usable uvs a tangent orthogonal to it.
- * The file is one object (one actor). */
+ * constant roughness map.  A material's map_Kd that names a .dds file is
+ * loaded (vct_dds_load, path relative to the .obj) and its faces carry their
+ * corner uvs and that texture; other image formats (stb_image's) are not
+ * decoded and those faces keep Kd.  A face without normals gets its face
+ * normal, one without usable uvs a tangent orthogonal to it.  The file is one
+ * object (one actor). */
 typedef struct vct_obj vct_obj;
 int vct_obj_load(const char *path, vct_obj **out, char *err, int errlen);
 /* views into the loaded arrays (valid until vct_obj_free) */
 int vct_obj_mesh(const vct_obj *obj, vct_mesh *out);
 int vct_obj_bounds(const vct_obj *obj, float bmin[3], float bmax[3], int *num_materials);
+/* the diffuse maps vct_obj_mesh's texture indices refer to */
+int vct_obj_textures(const vct_obj *obj, const vct_texture **textures, int *count);
 void vct_obj_free(vct_obj *obj);
 
 int vct_app_default_config(int scene, vct_app_config *out);

@@ -22,7 +22,12 @@ F3 = C.c_float * 3
 class Scene(C.Structure):
     _fields_ = [("ntri", C.c_int), ("nobj", C.c_int), ("pos", C.c_void_p), ("nrm", C.c_void_p),
                 ("tan", C.c_void_p), ("albedo", C.c_void_p), ("obj", C.c_void_p), ("rough", C.c_void_p),
-                ("model", C.c_void_p), ("nmat", C.c_void_p), ("vtan", C.c_void_p)]
+                ("model", C.c_void_p), ("nmat", C.c_void_p), ("vtan", C.c_void_p), ("uv", C.c_void_p),
+                ("texid", C.c_void_p), ("tex", C.c_void_p), ("ntex", C.c_int)]
+
+
+class Texture(C.Structure):
+    _fields_ = [("width", C.c_int), ("height", C.c_int), ("levels", C.c_int), ("texels", C.c_void_p)]
 
 
 class VoxParams(C.Structure):
@@ -111,6 +116,8 @@ def lib() -> C.CDLL:
             "or_sample": (None, [vp, C.c_int, C.c_int, vp, C.c_float, vp]),
             "or_calc_shadow_factor": (C.c_float, [vp, vp, C.c_int]),
             "or_compose": (None, [vp, C.c_int, vp, vp, vp, C.c_int, vp, C.c_int, C.c_int, vp, vp]),
+            "or_tex_sample": (None, [vp, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, vp]),
+            "or_tex_rho2_affine": (C.c_float, [vp, vp, C.c_float, C.c_float]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -201,9 +208,35 @@ class SceneArrays:
         self.model = np.ascontiguousarray(model, np.float32).reshape(-1, 16)
         self.nmat = normal_matrices(self.model)
         self.vtan = None if mesh.get("vtan") is None else np.ascontiguousarray(mesh["vtan"], np.float32)
+        # diffuse maps: mesh["uv"] [ntri][3][2], mesh["texid"] [ntri], mesh["textures"] a list of
+        # (width, height, levels, texels uint32 array of every level)
+        self.uv = None if mesh.get("uv") is None else np.ascontiguousarray(mesh["uv"], np.float32)
+        self.texid = None if mesh.get("texid") is None else np.ascontiguousarray(mesh["texid"], np.int32)
+        texs = mesh.get("textures") or []
+        self.texels = [np.ascontiguousarray(t[3], np.uint32) for t in texs]
+        self.tex = (Texture * max(1, len(texs)))(*[Texture(int(t[0]), int(t[1]), int(t[2]), _p(a))
+                                                  for t, a in zip(texs, self.texels)])
         self.s = Scene(int(self.albedo.size), int(self.model.shape[0]), _p(self.pos), _p(self.nrm), _p(self.tan),
                        _p(self.albedo), _p(self.obj), None if self.rough is None else _p(self.rough),
-                       _p(self.model), _p(self.nmat), None if self.vtan is None else _p(self.vtan))
+                       _p(self.model), _p(self.nmat), None if self.vtan is None else _p(self.vtan),
+                       None if self.uv is None else _p(self.uv), None if self.texid is None else _p(self.texid),
+                       C.cast(self.tex, C.c_void_p), len(texs))
+
+
+def tex_sample(textures, tid, s, t, rho2):
+    """or_tex_sample: textures as in SceneArrays (mesh["textures"]); returns 4 floats."""
+    texels = [np.ascontiguousarray(x[3], np.uint32) for x in textures]
+    arr = (Texture * max(1, len(textures)))(*[Texture(int(x[0]), int(x[1]), int(x[2]), _p(a))
+                                              for x, a in zip(textures, texels)])
+    out = (C.c_float * 4)()
+    lib().or_tex_sample(C.cast(arr, C.c_void_p), len(textures), int(tid), s, t, rho2, out)
+    return np.array(out[:], np.float32)
+
+
+def tex_rho2_affine(win, st, W, H) -> float:
+    w = np.ascontiguousarray(win, np.float32).reshape(3, 2)
+    a = np.ascontiguousarray(st, np.float32).reshape(6)
+    return float(lib().or_tex_rho2_affine(_p(w), _p(a), W, H))
 
 
 def _light_array(lights):

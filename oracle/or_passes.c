@@ -173,6 +173,79 @@ float or_calc_shadow_factor(const float lsp[4], const float *sm, int N) {
     return f;
 }
 
+/* ------------------------------------------------------------------------ */
+/* diffuse maps: GL 4.5 core 8.14 with LINEAR_MIPMAP_NEAREST / NEAREST / REPEAT
+ * (ResourceLoader.h:79-82, GLHelper.cpp:180-183) */
+static int tex_wrap(int i, int n) {
+    int r = i % n;
+    return r < 0 ? r + n : r;
+}
+static int tex_floor(float x) { /* clamped to +-2^30: whole numbers there */
+    float f = floorf(x);
+    f = f < -1073741824.0f ? -1073741824.0f : (f > 1073741824.0f ? 1073741824.0f : f);
+    return (int)f;
+}
+static void tex_unorm(uint32_t v, float o[4]) {
+    for (int k = 0; k < 4; ++k) o[k] = (float)((v >> (8 * k)) & 0xffu) / 255.0f;
+}
+
+void or_tex_sample(const or_texture *tex, int ntex, int id, float s, float t, float rho2, float out[4]) {
+    out[0] = out[1] = out[2] = 0.0f;
+    out[3] = 1.0f;
+    if (id < 0 || id >= ntex) return; /* an unbound unit */
+    const or_texture *T = &tex[id];
+    int q = 0;
+    int mx = T->width > T->height ? T->width : T->height;
+    while ((mx >> (q + 1)) > 0) ++q;
+    if (T->levels != q + 1) return; /* mipmap-incomplete (8.17) */
+    if (!(rho2 > 1.0f)) {           /* lambda <= 0: magnification, NEAREST on level 0 */
+        int i = tex_wrap(tex_floor(s * (float)T->width), T->width);
+        int j = tex_wrap(tex_floor(t * (float)T->height), T->height);
+        tex_unorm(T->texels[(size_t)j * T->width + i], out);
+        return;
+    }
+    /* level d = ceil(lambda + 1/2) - 1 (0 for lambda <= 1/2), clamped to q:
+     * lambda > k - 1/2 is rho^2 > 2^(2k-1) */
+    int d = 0;
+    for (int k = 1; k <= q; ++k)
+        if (rho2 > ldexpf(1.0f, 2 * k - 1)) d = k;
+    size_t off = 0;
+    for (int l = 0; l < d; ++l) {
+        int w = T->width >> l, h = T->height >> l;
+        off += (size_t)(w < 1 ? 1 : w) * (size_t)(h < 1 ? 1 : h);
+    }
+    int W = T->width >> d, H = T->height >> d;
+    W = W < 1 ? 1 : W;
+    H = H < 1 ? 1 : H;
+    const uint32_t *L = T->texels + off;
+    float u = s * (float)W - 0.5f, v = t * (float)H - 0.5f;
+    float a = u - floorf(u), b = v - floorf(v);
+    int i0 = tex_floor(u), j0 = tex_floor(v);
+    int ia = tex_wrap(i0, W), ib = tex_wrap(i0 + 1, W), ja = tex_wrap(j0, H), jb = tex_wrap(j0 + 1, H);
+    float t00[4], t10[4], t01[4], t11[4];
+    tex_unorm(L[(size_t)ja * W + ia], t00);
+    tex_unorm(L[(size_t)ja * W + ib], t10);
+    tex_unorm(L[(size_t)jb * W + ia], t01);
+    tex_unorm(L[(size_t)jb * W + ib], t11);
+    float w00 = (1.0f - a) * (1.0f - b), w10 = a * (1.0f - b), w01 = (1.0f - a) * b, w11 = a * b;
+    for (int k = 0; k < 4; ++k) out[k] = ((w00 * t00[k] + w10 * t10[k]) + w01 * t01[k]) + w11 * t11[k];
+}
+
+float or_tex_rho2_affine(const float win[3][2], const float st[6], float W, float H) {
+    float e1x = win[1][0] - win[0][0], e1y = win[1][1] - win[0][1];
+    float e2x = win[2][0] - win[0][0], e2y = win[2][1] - win[0][1];
+    float den = e1x * e2y - e2x * e1y;
+    float du1 = (st[2] - st[0]) * W, du2 = (st[4] - st[0]) * W;
+    float dv1 = (st[3] - st[1]) * H, dv2 = (st[5] - st[1]) * H;
+    float dudx = (du1 * e2y - du2 * e1y) / den, dvdx = (dv1 * e2y - dv2 * e1y) / den;
+    float dudy = (du2 * e1x - du1 * e2x) / den, dvdy = (dv2 * e1x - dv1 * e2x) / den;
+    float x = dudx * dudx + dvdx * dvdx, y = dudy * dudy + dvdy * dvdy;
+    return fmaxf(x, y);
+}
+
+/* the triangle's diffuse map index, -1 for none */
+static int tri_tex(const or_scene *sc, int t) { return sc->texid ? sc->texid[t] : -1; }
+
 static void world_vertex(const or_scene *sc, int t, int v, float w[3]) {
     const float *M = sc->model + (size_t)sc->obj[t] * 16;
     const float *p = sc->pos + (size_t)t * 9 + v * 3;
@@ -356,6 +429,11 @@ static void voxelize_impl(const or_scene *sc, const or_light *lights, int nlight
         if (!tri_setup_fn(win, dim, dim, 0, &s)) continue;
         float col[4];
         or_unpack_unorm4x8(sc->albedo[t], col);
+        const int tid = tri_tex(sc, t);
+        float rho2 = 0.0f; /* the orthographic views: one footprint per triangle */
+        if (tid >= 0 && tid < sc->ntex)
+            rho2 = or_tex_rho2_affine(win, sc->uv + (size_t)t * 6, (float)sc->tex[tid].width,
+                                      (float)sc->tex[tid].height);
         int nsamp = p->msaa ? 4 : 1;
         for (int py = s.y0; py <= s.y1; ++py)
             for (int px = s.x0; px <= s.x1; ++px) {
@@ -383,6 +461,15 @@ static void voxelize_impl(const or_scene *sc, const or_light *lights, int nlight
                     nr[k] = interp(l, Nw[0][k], Nw[1][k], Nw[2][k]);
                 }
                 float color[3] = {col[0], col[1], col[2]};
+                if (!occ && tid >= 0) { /* texture(diffuseMap, texcoord).rgb (voxelize.frag:197) */
+                    const float *st = sc->uv + (size_t)t * 6;
+                    float tc[4];
+                    or_tex_sample(sc->tex, sc->ntex, tid, interp(l, st[0], st[2], st[4]), interp(l, st[1], st[3], st[5]),
+                                  rho2, tc);
+                    color[0] = tc[0];
+                    color[1] = tc[1];
+                    color[2] = tc[2];
+                }
                 float nn[3], nenc[4];
                 if (!occ) {
                     nfrag += 1; /* voxelize.frag:195 */
@@ -967,6 +1054,35 @@ void or_gbuffer(const or_scene *sc, const or_camera *cam, or_gbuf_texel *out) {
                 g->bitangent[k] = interp(b, Bv[0][k], Bv[1][k], Bv[2][k]);
             }
             g->albedo = sc->albedo[t];
+            const int tid = tri_tex(sc, t);
+            if (tid >= 0) { /* texture(diffuseMap, fragTexcoord) (phong.frag:449), kept as RGBA8 */
+                const float *st = sc->uv + (size_t)t * 6;
+                float ts = interp(b, st[0], st[2], st[4]), tt = interp(b, st[1], st[3], st[5]);
+                float rho2 = 0.0f;
+                if (tid < sc->ntex) {
+                    /* the unnormalized barycentrics are affine in NDC: their
+                     * derivatives give ds/dxn = sum_i db_i/dxn (s_i - s) / sum */
+                    float gx[3], gy[3];
+                    for (int i = 0; i < 3; ++i) {
+                        int j = (i + 1) % 3, k = (i + 2) % 3;
+                        gx[i] = clip[j][1] * clip[k][3] - clip[j][3] * clip[k][1];
+                        gy[i] = clip[j][3] * clip[k][0] - clip[j][0] * clip[k][3];
+                    }
+                    float TW = (float)sc->tex[tid].width, TH = (float)sc->tex[tid].height;
+                    float kx = 2.0f / (float)W, ky = 2.0f / (float)H;
+                    float sx = (gx[0] * (st[0] - ts) + gx[1] * (st[2] - ts)) + gx[2] * (st[4] - ts);
+                    float tx = (gx[0] * (st[1] - tt) + gx[1] * (st[3] - tt)) + gx[2] * (st[5] - tt);
+                    float sy = (gy[0] * (st[0] - ts) + gy[1] * (st[2] - ts)) + gy[2] * (st[4] - ts);
+                    float ty = (gy[0] * (st[1] - tt) + gy[1] * (st[3] - tt)) + gy[2] * (st[5] - tt);
+                    float dudx = ((sx / sum) * kx) * TW, dvdx = ((tx / sum) * kx) * TH;
+                    float dudy = ((sy / sum) * ky) * TW, dvdy = ((ty / sum) * ky) * TH;
+                    float x = dudx * dudx + dvdx * dvdx, y = dudy * dudy + dvdy * dvdy;
+                    rho2 = fmaxf(x, y);
+                }
+                float c[4];
+                or_tex_sample(sc->tex, sc->ntex, tid, ts, tt, rho2, c);
+                g->albedo = or_pack_unorm4x8(c);
+            }
             g->roughness = sc->rough ? sc->rough[sc->obj[t]] : 0.5f;
             g->covered = 1;
             g->triangle = (uint32_t)t;
@@ -1115,6 +1231,7 @@ void or_voxelize_tess(const or_scene *sc, int dim, const float center[3], const 
         const int cnt = or_tess_levels(P, dim, center, vmin, vmax, lv);
         float col[4];
         or_unpack_unorm4x8(sc->albedo[t], col);
+        const int tid = tri_tex(sc, t);
         for (int j = 0; j < cnt; ++j) {
             float w[3];
             or_tess_point(lv, j, w);
@@ -1122,6 +1239,11 @@ void or_voxelize_tess(const or_scene *sc, int dim, const float center[3], const 
             for (int k = 0; k < 3; ++k) {
                 p[k] = (w[0] * P[0][k] + w[1] * P[1][k]) + w[2] * P[2][k];
                 nr[k] = (w[0] * N[0][k] + w[1] * N[1][k]) + w[2] * N[2][k];
+            }
+            if (tid >= 0) { /* texture() in the evaluation shader: zero footprint, level 0 NEAREST */
+                const float *st = sc->uv + (size_t)t * 6;
+                or_tex_sample(sc->tex, sc->ntex, tid, (w[0] * st[0] + w[1] * st[2]) + w[2] * st[4],
+                              (w[0] * st[1] + w[1] * st[3]) + w[2] * st[5], 0.0f, col);
             }
             int idx[3];
             for (int k = 0; k < 3; ++k)

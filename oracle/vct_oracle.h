@@ -45,6 +45,12 @@
  *   or_half / or_fp8                                        numpy float16; OCP
  *       E4M3 code table round trip
  *   or_gbuffer                                              closed form plane hits
+ *   or_tex_sample (diffuse maps, inside or_voxelize, or_gbuffer and
+ *   or_voxelize_tess)                                       NEAREST / LINEAR texel
+ *       and weight closed forms, REPEAT wrap, the LINEAR_MIPMAP_NEAREST level at
+ *       footprints on both sides of each threshold rho = 2^(k-1/2), and the
+ *       footprint of affine and perspective uv maps against finite differences;
+ *       incomplete textures (0,0,0,1)
  *   or_compose                                              closed forms: N=L=V
  *       Cook-Torrance and Blinn-Phong values, point-light range cut and
  *       smoothstep, shadow on/off, Reinhard+gamma, indirect/occlusion algebra
@@ -95,7 +101,29 @@ typedef struct or_scene {
     const float *nmat;      /* [nobj][9] normal matrices          */
     const float *vtan;      /* [ntri][3][3] per-corner (vertex) tangents
                                (Mesh.cpp:190-197), or NULL: `tan` */
+    const float *uv;        /* [ntri][3][2] texture coordinates, or NULL     */
+    const int32_t *texid;   /* [ntri] diffuse map, -1 none; or NULL          */
+    const struct or_texture *tex; /* the diffuse maps                        */
+    int ntex;
 } or_scene;
+
+/* A diffuse map: RGBA8 levels back to back, level l max(1, w >> l) x
+ * max(1, h >> l), row-major from row t = 0. */
+typedef struct or_texture {
+    int width, height, levels;
+    const uint32_t *texels;
+} or_texture;
+
+/* texture(diffuseMap, (s, t)) with the reference's sampler state
+ * (ResourceLoader.h:79-82, GLHelper.cpp:180-183) by GL 4.5 core 8.14:
+ * rho2 = rho^2 of the footprint in level-0 texels; lambda <= 0: NEAREST on
+ * level 0; else LINEAR_MIPMAP_NEAREST (level ceil(lambda + 1/2) - 1, 0 for
+ * lambda <= 1/2, clamped); REPEAT; byte / 255; an incomplete texture or an
+ * index outside the table gives (0, 0, 0, 1). */
+void or_tex_sample(const or_texture *tex, int ntex, int id, float s, float t, float rho2, float out[4]);
+/* rho^2 of an affine (s, t) over a window-space triangle (orthographic views):
+ * st = (s0, t0, s1, t1, s2, t2), W x H level 0's size. */
+float or_tex_rho2_affine(const float win[3][2], const float st[6], float W, float H);
 
 typedef struct or_light {
     float position[3]; float range;

@@ -103,8 +103,10 @@ class Context:
         self.check(self.lib.vct_get_grid(self.h, C.byref(d)))
         return d
 
-    def set_scene(self, pos, nrm, tan, albedo, obj, nobj, rough=None, vtan=None):
-        """vct_set_scene; `vtan` [num_tris][3][3] per-corner tangents or None."""
+    def set_scene(self, pos, nrm, tan, albedo, obj, nobj, rough=None, vtan=None, uv=None, texid=None):
+        """vct_set_scene; `vtan` [num_tris][3][3] per-corner tangents or None;
+        `uv` [num_tris][3][2] texture coordinates and `texid` [num_tris]
+        diffuse-map indices (-1: none) or None."""
         pos = np.ascontiguousarray(pos, np.float32)
         nrm = np.ascontiguousarray(nrm, np.float32)
         tan = np.ascontiguousarray(tan, np.float32)
@@ -114,9 +116,22 @@ class Context:
         vtan_a = None if vtan is None else np.ascontiguousarray(vtan, np.float32)
         if vtan_a is not None and vtan_a.size != albedo.size * 9:
             raise ValueError("vtan must be [num_tris][3][3]")
+        uv_a = None if uv is None else np.ascontiguousarray(uv, np.float32)
+        tid_a = None if texid is None else np.ascontiguousarray(texid, np.int32)
+        if tid_a is not None and (uv_a is None or uv_a.size != albedo.size * 6 or tid_a.size != albedo.size):
+            raise ValueError("texid needs uv [num_tris][3][2] and one index per triangle")
         m = abi.Mesh(int(albedo.size), int(nobj), _ptr(pos), _ptr(nrm), _ptr(tan), _ptr(albedo), _ptr(obj),
-                     None if rough_a is None else _ptr(rough_a), None if vtan_a is None else _ptr(vtan_a))
+                     None if rough_a is None else _ptr(rough_a), None if vtan_a is None else _ptr(vtan_a),
+                     None if uv_a is None else _ptr(uv_a), None if tid_a is None else _ptr(tid_a))
         self.check(self.lib.vct_set_scene(self.h, C.byref(m)))
+
+    def set_textures(self, textures):
+        """vct_set_textures: a list of (width, height, levels, texels) with texels
+        the uint32 RGBA8 words of every level back to back."""
+        keep = [np.ascontiguousarray(t[3], np.uint32) for t in textures]
+        arr = (abi.Texture * max(1, len(textures)))(*[abi.Texture(int(t[0]), int(t[1]), int(t[2]), _ptr(a))
+                                                      for t, a in zip(textures, keep)])
+        self.check(self.lib.vct_set_textures(self.h, C.cast(arr, C.c_void_p) if textures else None, len(textures)))
 
     def set_transforms(self, model: np.ndarray):
         model = np.ascontiguousarray(model, np.float32)
@@ -284,15 +299,49 @@ def load_obj(path) -> dict:
         bmin, bmax = np.zeros(3, np.float32), np.zeros(3, np.float32)
         nm = C.c_int()
         lib.vct_obj_bounds(h, _ptr(bmin), _ptr(bmax), C.byref(nm))
+        tp, nt = C.c_void_p(), C.c_int()
+        lib.vct_obj_textures(h, C.byref(tp), C.byref(nt))
+        textures = []
+        for i in range(nt.value):
+            t = abi.Texture.from_address(tp.value + i * C.sizeof(abi.Texture))
+            textures.append(_texture_tuple(t))
         return dict(pos=arr(m.positions, n * 9, np.float32).reshape(n, 3, 3),
                     nrm=arr(m.normals, n * 9, np.float32).reshape(n, 3, 3),
                     tan=arr(m.tangents, n * 3, np.float32).reshape(n, 3),
                     vtan=None if not m.vertex_tangents else arr(m.vertex_tangents, n * 9, np.float32).reshape(n, 3, 3),
                     albedo=arr(m.albedo, n, np.uint32), obj=arr(m.object, n, np.uint32), nobj=1,
                     rough=None if not m.roughness else arr(m.roughness, 1, np.float32),
-                    bmin=bmin, bmax=bmax, materials=nm.value)
+                    bmin=bmin, bmax=bmax, materials=nm.value,
+                    uv=arr(m.texcoords, n * 6, np.float32).reshape(n, 3, 2),
+                    texid=arr(m.texture, n, np.int32), textures=textures)
     finally:
         lib.vct_obj_free(h)
+
+
+def _texture_tuple(t) -> tuple:
+    """(width, height, levels, texels) of a vct_texture view, texels copied."""
+    n = sum(max(1, t.width >> l) * max(1, t.height >> l) for l in range(t.levels))
+    if n == 0:
+        return (t.width, t.height, t.levels, np.zeros(0, np.uint32))
+    a = np.ctypeslib.as_array(C.cast(t.texels, C.POINTER(C.c_uint32)), shape=(n,)).copy()
+    return (t.width, t.height, t.levels, a)
+
+
+def load_dds(path) -> tuple:
+    """A DDS file (DXT1/3/5) decoded to RGBA8 levels (vct_dds_load):
+    (width, height, levels, texels of every level back to back)."""
+    lib = load_app_library()
+    h = C.c_void_p()
+    err = C.create_string_buffer(256)
+    rc = lib.vct_dds_load(str(path).encode(), C.byref(h), err, 256)
+    if rc:
+        raise VCTError(rc, err.value.decode())
+    try:
+        t = abi.Texture()
+        lib.vct_dds_texture(h, C.byref(t))
+        return _texture_tuple(t)
+    finally:
+        lib.vct_dds_free(h)
 
 
 class App:

@@ -53,7 +53,12 @@ class GridDesc(C.Structure):
 class Mesh(C.Structure):
     _fields_ = [("num_tris", C.c_int), ("num_objects", C.c_int), ("positions", C.c_void_p),
                 ("normals", C.c_void_p), ("tangents", C.c_void_p), ("albedo", C.c_void_p),
-                ("object", C.c_void_p), ("roughness", C.c_void_p), ("vertex_tangents", C.c_void_p)]
+                ("object", C.c_void_p), ("roughness", C.c_void_p), ("vertex_tangents", C.c_void_p),
+                ("texcoords", C.c_void_p), ("texture", C.c_void_p)]
+
+
+class Texture(C.Structure):
+    _fields_ = [("width", C.c_int), ("height", C.c_int), ("levels", C.c_int), ("texels", C.c_void_p)]
 
 
 class Light(C.Structure):
@@ -151,6 +156,7 @@ VCT_FUNCS = [
     ("vct_get_grid", C.c_int, [C.c_void_p, C.POINTER(GridDesc)]),
     ("vct_set_extent", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("vct_set_scene", C.c_int, [C.c_void_p, C.POINTER(Mesh)]),
+    ("vct_set_textures", C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     ("vct_set_transforms", C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     ("vct_set_lights", C.c_int, [C.c_void_p, C.POINTER(Light), C.c_int]),
     ("vct_shadowmap", C.c_int, [C.c_void_p, C.c_int]),
@@ -187,7 +193,11 @@ APP_FUNCS = [
     ("vct_obj_load", C.c_int, [C.c_char_p, C.POINTER(C.c_void_p), C.c_char_p, C.c_int]),
     ("vct_obj_mesh", C.c_int, [C.c_void_p, C.POINTER(Mesh)]),
     ("vct_obj_bounds", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_int)]),
+    ("vct_obj_textures", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_int)]),
     ("vct_obj_free", None, [C.c_void_p]),
+    ("vct_dds_load", C.c_int, [C.c_char_p, C.POINTER(C.c_void_p), C.c_char_p, C.c_int]),
+    ("vct_dds_texture", C.c_int, [C.c_void_p, C.POINTER(Texture)]),
+    ("vct_dds_free", None, [C.c_void_p]),
     ("vct_app_destroy", None, [C.c_void_p]),
     ("vct_app_last_error", C.c_char_p, [C.c_void_p]),
     ("vct_app_ctx", C.c_void_p, [C.c_void_p]),

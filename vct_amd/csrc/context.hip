@@ -43,6 +43,11 @@ struct vct_ctx {
     int ntri = 0, nobj = 0, has_rough = 0, have_transforms = 0;
     float *d_pos = nullptr, *d_nrm = nullptr, *d_tan = nullptr, *d_rough = nullptr;
     float *d_vtan = nullptr;  // per-corner tangents (vct_mesh.vertex_tangents), or null
+    float *d_uv = nullptr;      // per-corner texture coordinates (vct_mesh.texcoords), or null
+    int32_t *d_texid = nullptr;  // per-triangle diffuse map (vct_mesh.texture), or null
+    uint32_t *d_tex_texels = nullptr;  // vct_set_textures: every level of every map
+    int4 *d_tex_info = nullptr, *d_tex_levels = nullptr;
+    int ntex = 0;
     uint32_t *d_albedo = nullptr, *d_obj = nullptr;
     float *d_model = nullptr, *d_nmat = nullptr;
     // pinned staging ring of the per-frame transform upload: slot k holds the
@@ -402,6 +407,11 @@ void vct_destroy(vct_ctx *c) {
     dfree(c->d_tan);
     dfree(c->d_rough);
     dfree(c->d_vtan);
+    dfree(c->d_uv);
+    dfree(c->d_texid);
+    dfree(c->d_tex_texels);
+    dfree(c->d_tex_info);
+    dfree(c->d_tex_levels);
     dfree(c->d_albedo);
     dfree(c->d_obj);
     dfree(c->d_model);
@@ -536,6 +546,8 @@ int vct_set_scene(vct_ctx *c, const vct_mesh *m) {
     if (m->num_tris > (1 << 28)) return fail(c, VCT_ERR_INVALID, "vct_set_scene: too many triangles");
     if (m->num_tris > 0 && (!m->positions || !m->normals || !m->tangents || !m->albedo || !m->object))
         return fail(c, VCT_ERR_INVALID, "vct_set_scene: NULL geometry array");
+    if (m->num_tris > 0 && m->texture && !m->texcoords)
+        return fail(c, VCT_ERR_INVALID, "vct_set_scene: texture indices without texcoords");
     for (int t = 0; t < m->num_tris; ++t)
         if (m->object[t] >= (uint32_t)m->num_objects)
             return fail(c, VCT_ERR_INVALID, "vct_set_scene: triangle %d references object %u >= %d", t,
@@ -548,6 +560,8 @@ int vct_set_scene(vct_ctx *c, const vct_mesh *m) {
     dfree(c->d_obj);
     dfree(c->d_rough);
     dfree(c->d_vtan);
+    dfree(c->d_uv);
+    dfree(c->d_texid);
     dfree(c->d_model);
     dfree(c->d_nmat);
     free_xf_ring(c);  // the stream is idle: no copy reads the ring
@@ -574,6 +588,12 @@ int vct_set_scene(vct_ctx *c, const vct_mesh *m) {
         HIPCHK(c, dalloc(&c->d_vtan, T * 9 * 4));
         HIPCHK(c, hipMemcpy(c->d_vtan, m->vertex_tangents, T * 9 * 4, hipMemcpyHostToDevice));
     }
+    if (T && m->texture && m->texcoords) {  // the diffuse maps' indices and coordinates
+        HIPCHK(c, dalloc(&c->d_texid, T * 4));
+        HIPCHK(c, dalloc(&c->d_uv, T * 6 * 4));
+        HIPCHK(c, hipMemcpy(c->d_texid, m->texture, T * 4, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(c->d_uv, m->texcoords, T * 6 * 4, hipMemcpyHostToDevice));
+    }
     c->has_rough = m->roughness != nullptr;
     if (c->has_rough) {
         HIPCHK(c, dalloc(&c->d_rough, (size_t)m->num_objects * 4));
@@ -589,6 +609,51 @@ int vct_set_scene(vct_ctx *c, const vct_mesh *m) {
     for (int o = 0; o < c->nobj; ++o) id[(size_t)o * 16 + 0] = id[(size_t)o * 16 + 5] = id[(size_t)o * 16 + 10] =
         id[(size_t)o * 16 + 15] = 1.0f;
     return vct_set_transforms(c, id.data(), c->nobj);
+}
+
+int vct_set_textures(vct_ctx *c, const vct_texture *tex, int n) {
+    if (!c) return fail(nullptr, VCT_ERR_INVALID, "ctx is NULL");
+    if (n < 0 || (n > 0 && !tex)) return fail(c, VCT_ERR_INVALID, "vct_set_textures: %d textures at %p", n, (const void *)tex);
+    std::vector<int4> info((size_t)n), levels;
+    size_t total = 0;
+    for (int i = 0; i < n; ++i) {
+        const vct_texture &t = tex[i];
+        if (t.width < 1 || t.height < 1 || t.width > 32768 || t.height > 32768)
+            return fail(c, VCT_ERR_INVALID, "vct_set_textures: texture %d is %d x %d", i, t.width, t.height);
+        int q = 0;
+        while ((std::max(t.width, t.height) >> (q + 1)) > 0) ++q;
+        if (t.levels < 0 || t.levels > q + 1 || (t.levels > 0 && !t.texels))
+            return fail(c, VCT_ERR_INVALID, "vct_set_textures: texture %d has %d levels (at most %d)", i, t.levels, q + 1);
+        info[i] = make_int4((int)levels.size(), t.width, t.height, (t.levels == q + 1 ? 1 : 0) | (q << 8));
+        for (int l = 0; l < t.levels; ++l) {
+            const int w = std::max(1, t.width >> l), h = std::max(1, t.height >> l);
+            levels.push_back(make_int4((int)(uint32_t)total, w, h, 0));
+            total += (size_t)w * h;
+        }
+        if (total >= ((size_t)1 << 32)) return fail(c, VCT_ERR_INVALID, "vct_set_textures: more than 2^32 texels");
+        if (t.levels == 0) levels.push_back(make_int4(0, t.width, t.height, 0));  // never read: incomplete
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    dfree(c->d_tex_texels);
+    dfree(c->d_tex_info);
+    dfree(c->d_tex_levels);
+    c->ntex = 0;
+    if (n == 0) return VCT_OK;
+    HIPCHK(c, dalloc(&c->d_tex_texels, std::max<size_t>(total, 1) * 4));
+    HIPCHK(c, dalloc(&c->d_tex_info, info.size() * sizeof(int4)));
+    HIPCHK(c, dalloc(&c->d_tex_levels, levels.size() * sizeof(int4)));
+    size_t off = 0;
+    for (int i = 0; i < n; ++i) {
+        size_t cnt = 0;
+        for (int l = 0; l < tex[i].levels; ++l)
+            cnt += (size_t)std::max(1, tex[i].width >> l) * (size_t)std::max(1, tex[i].height >> l);
+        if (cnt) HIPCHK(c, hipMemcpy(c->d_tex_texels + off, tex[i].texels, cnt * 4, hipMemcpyHostToDevice));
+        off += cnt;
+    }
+    HIPCHK(c, hipMemcpy(c->d_tex_info, info.data(), info.size() * sizeof(int4), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_tex_levels, levels.data(), levels.size() * sizeof(int4), hipMemcpyHostToDevice));
+    c->ntex = n;
+    return VCT_OK;
 }
 
 // mat3(transpose(inverse(model))) as cofactor(A) / det(A) of the upper-left
@@ -711,6 +776,12 @@ static DevScene dev_scene(const vct_ctx *c) {
     s.obj = c->d_obj;
     s.rough = c->has_rough ? c->d_rough : nullptr;
     s.vtan = c->d_vtan;
+    s.uv = c->d_texid ? c->d_uv : nullptr;
+    s.texid = c->d_texid;
+    s.tex.texels = c->d_tex_texels;
+    s.tex.info = c->d_tex_info;
+    s.tex.levels = c->d_tex_levels;
+    s.tex.count = c->ntex;
     s.model = c->d_model;
     s.nmat = c->d_nmat;
     return s;

@@ -12,6 +12,7 @@
 #pragma clang fp contract(off)
 
 #include "raster_common.h"
+#include "texture.h"
 
 namespace vctk {
 
@@ -511,7 +512,37 @@ __global__ __launch_bounds__(256) void k_gbuf_resolve(DevScene sc, CamMats cm, c
         T[k] = interp3(b, Tv[0][k], Tv[1][k], Tv[2][k]);
         B[k] = interp3(b, Bv[0][k], Bv[1][k], Bv[2][k]);
     }
-    const uint32_t alb = sc.albedo[t];
+    uint32_t alb = sc.albedo[t];
+    const int tid = sc.texid ? sc.texid[t] : -1;
+    if (tid >= 0) {  // texture(diffuseMap, fragTexcoord) (phong.frag:449), stored as RGBA8
+        const float *st = sc.uv + (size_t)t * 6;
+        const float ts = interp3(b, st[0], st[2], st[4]), tt = interp3(b, st[1], st[3], st[5]);
+        float rho2 = 0.0f;
+        if (tid < sc.tex.count) {
+            // The unnormalized barycentrics (before the division by sum) are
+            // affine in NDC: d/dxn of b[i] is cy_j cw_k - cw_j cy_k and d/dyn
+            // is cw_j cx_k - cx_j cw_k, (i, j, k) cyclic, so ds/dxn =
+            // sum_i d b[i]/dxn (s_i - s) / sum exactly; one pixel is 2/W (2/H) in NDC.
+            float gx[3], gy[3];
+            for (int i = 0; i < 3; ++i) {
+                const int j = (i + 1) % 3, k = (i + 2) % 3;
+                gx[i] = clip[j][1] * clip[k][3] - clip[j][3] * clip[k][1];
+                gy[i] = clip[j][3] * clip[k][0] - clip[j][0] * clip[k][3];
+            }
+            const int4 inf = sc.tex.info[tid];
+            const float TW = (float)inf.y, TH = (float)inf.z;
+            const float kx = 2.0f / (float)cm.W, ky = 2.0f / (float)cm.H;
+            const float sx = (gx[0] * (st[0] - ts) + gx[1] * (st[2] - ts)) + gx[2] * (st[4] - ts);
+            const float tx = (gx[0] * (st[1] - tt) + gx[1] * (st[3] - tt)) + gx[2] * (st[5] - tt);
+            const float sy = (gy[0] * (st[0] - ts) + gy[1] * (st[2] - ts)) + gy[2] * (st[4] - ts);
+            const float ty = (gy[0] * (st[1] - tt) + gy[1] * (st[3] - tt)) + gy[2] * (st[5] - tt);
+            const float dudx = ((sx / sum) * kx) * TW, dvdx = ((tx / sum) * kx) * TH;
+            const float dudy = ((sy / sum) * ky) * TW, dvdy = ((ty / sum) * ky) * TH;
+            rho2 = fmaxf(dudx * dudx + dvdx * dvdx, dudy * dudy + dvdy * dvdy);
+        }
+        const float4 c = tex_sample(sc.tex, tid, ts, tt, rho2);
+        alb = pack_unorm4x8(c.x, c.y, c.z, c.w);
+    }
     const float rough = sc.rough ? sc.rough[sc.obj[t]] : 0.5f;
     o[0] = make_float4(P[0], P[1], P[2], __uint_as_float(alb));
     o[1] = make_float4(N[0], N[1], N[2], rough);

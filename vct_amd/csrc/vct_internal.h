@@ -92,6 +92,16 @@ struct Extent {
     float center[3], vmin[3], vmax[3];
 };
 
+// The scene's diffuse maps (vct_set_textures): info[i] = (first level entry,
+// width, height, complete | q << 8) with q = floor(log2(max(width, height)));
+// levels[e] = (first texel, width, height, 0).
+struct DevTex {
+    const uint32_t *texels;
+    const int4 *info;
+    const int4 *levels;
+    int count;
+};
+
 // Scene geometry on the device (object space, triangle soup).
 struct DevScene {
     int ntri, nobj;
@@ -104,6 +114,9 @@ struct DevScene {
     const float *rough;     // [nobj] or null
     const float *model;     // [nobj][16]
     const float *nmat;      // [nobj][9]
+    const float *uv;        // [ntri][3][2] texture coordinates, or null
+    const int32_t *texid;   // [ntri] diffuse map (-1: none), or null
+    DevTex tex;
 };
 
 // Scratch for the binned rasterizer: per-slot setup records (4 x int4), an

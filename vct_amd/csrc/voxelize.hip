@@ -11,6 +11,7 @@
 #pragma clang fp contract(off)
 
 #include "raster_common.h"
+#include "texture.h"
 
 namespace vctk {
 
@@ -22,6 +23,8 @@ struct VoxMats {
 // attribute block per triangle: 8 float4
 //  f[0..8]  ndc xyz of v0,v1,v2        f[9..17]  world positions
 //  f[18..26] world normals             f[27] albedo bits  f[28] axis | zclip<<4
+//  f[29] rho^2 of the diffuse map's footprint (constant over the triangle: the
+//  dominant-axis views are orthographic)  f[30] the map's index, -1 for none
 __global__ __launch_bounds__(256) void k_vox_setup(DevScene sc, VoxMats vm, int4 *rec, float4 *attr,
                                                    uint32_t *counts) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -73,7 +76,15 @@ __global__ __launch_bounds__(256) void k_vox_setup(DevScene sc, VoxMats vm, int4
             }
         f[27] = __uint_as_float(sc.albedo[t]);
         f[28] = __uint_as_float((uint32_t)axis | ((uint32_t)zclip << 4));
-        f[29] = f[30] = f[31] = 0.0f;
+        const int tid = sc.texid ? sc.texid[t] : -1;
+        float rho2 = 0.0f;
+        if (tid >= 0 && tid < sc.tex.count) {
+            const int4 inf = sc.tex.info[tid];
+            rho2 = tex_rho2_affine(win, sc.uv + (size_t)t * 6, (float)inf.y, (float)inf.z);
+        }
+        f[29] = rho2;
+        f[30] = __int_as_float(tid);
+        f[31] = 0.0f;
         float4 *a = attr + (size_t)t * 8;
         for (int i = 0; i < 8; ++i) a[i] = make_float4(f[i * 4], f[i * 4 + 1], f[i * 4 + 2], f[i * 4 + 3]);
     }
@@ -180,6 +191,8 @@ struct VoxFrag {
     uint32_t *albedo, *normal;
     uint32_t *partials;
     const float4 *warp;  // warpTexture: the warp texture, or null
+    const float *uv;     // the scene's texture coordinates and diffuse maps
+    DevTex tex;
     uint32_t frags;
 
     // The lights are read field by field inside the shading loop, behind
@@ -219,6 +232,16 @@ struct VoxFrag {
         frags += 1;
         const uint32_t alb = __float_as_uint(f[27]);
         float color[3] = {unorm8(alb, 0), unorm8(alb, 1), unorm8(alb, 2)};
+        const float4 a7 = a[7];
+        const int tid = __float_as_int(a7.z);
+        if (tid >= 0) {  // texture(diffuseMap, texcoord).rgb (voxelize.frag:197)
+            const float *st = uv + (size_t)slot * 6;
+            const float ts = interp3(l, st[0], st[2], st[4]), tt = interp3(l, st[1], st[3], st[5]);
+            const float4 c = tex_sample(tex, tid, ts, tt, a7.y);
+            color[0] = c.x;
+            color[1] = c.y;
+            color[2] = c.z;
+        }
         float nn[3];
         nrm3(nr, nn);
         const float ne0 = (nn[0] + 1.0f) * 0.5f, ne1 = (nn[1] + 1.0f) * 0.5f, ne2 = (nn[2] + 1.0f) * 0.5f;
@@ -400,6 +423,8 @@ hipError_t launch_voxelize(const DevScene &sc, const DevLight *dlights, int nlig
     f.normal = normal;
     f.partials = partials;
     f.warp = warp;
+    f.uv = sc.uv;
+    f.tex = sc.tex;
     f.frags = 0;
     static_assert(kRasterBlocks <= kStatPartials, "one statistics partial per rasterizer workgroup");
     const int nblk = raster_grid<VoxFrag, true>();
@@ -623,12 +648,20 @@ __global__ __launch_bounds__(256) void k_tess_points(DevScene sc, TessArgs ta, c
         nrm3(nr, nn);
         const float ne0 = nn[0] * 0.5f + 0.5f, ne1 = nn[1] * 0.5f + 0.5f, ne2 = nn[2] * 0.5f + 0.5f;
         const uint32_t alb = sc.albedo[t];
+        float4 col = make_float4(unorm8(alb, 0), unorm8(alb, 1), unorm8(alb, 2), unorm8(alb, 3));
+        const int tid = sc.texid ? sc.texid[t] : -1;
+        if (tid >= 0) {  // texture(diffuseMap, texcoord) in the evaluation shader: zero footprint
+            const float *st = sc.uv + (size_t)t * 6;
+            const float ts = (w[0] * st[0] + w[1] * st[2]) + w[2] * st[4];
+            const float tt = (w[0] * st[1] + w[1] * st[3]) + w[2] * st[5];
+            col = tex_sample(sc.tex, tid, ts, tt, 0.0f);
+        }
         const uint32_t vi = grid_index(idx[0], idx[1], idx[2], ta.dim >> 2);
         if (accum == VCT_ACCUM_MAX) {
-            atomicMax(&albedo[vi], pack_unorm4x8(unorm8(alb, 0), unorm8(alb, 1), unorm8(alb, 2), unorm8(alb, 3)));
+            atomicMax(&albedo[vi], pack_unorm4x8(col.x, col.y, col.z, col.w));
             atomicMax(&normal[vi], pack_unorm4x8(ne0, ne1, ne2, 1.0f));
         } else {
-            rgba8_avg(&albedo[vi], unorm8(alb, 0), unorm8(alb, 1), unorm8(alb, 2));
+            rgba8_avg(&albedo[vi], col.x, col.y, col.z);
             rgba8_avg(&normal[vi], ne0, ne1, ne2);
         }
     }

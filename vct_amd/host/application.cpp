@@ -48,6 +48,8 @@ struct Rng {
 struct Mesh {
     std::vector<float> pos, nrm, tan, rough;
     std::vector<float> vtan;  // per-corner (vertex) tangents, [ntri][3][3], or empty
+    std::vector<float> uv;    // per-corner texture coordinates, [ntri][3][2], or empty
+    std::vector<int32_t> texid;  // per-triangle diffuse map, or empty
     std::vector<uint32_t> albedo, obj;
     int nobj = 0;
     int ntri() const { return (int)albedo.size(); }
@@ -229,6 +231,8 @@ struct vct_app {
     vct_camera cam{};
     vct_trace_params trace{};
     std::vector<float> model;  // scratch
+    std::vector<vct_texture> textures;  // the scene's diffuse maps
+    std::vector<std::vector<uint32_t>> tex_texels;
     // where seeded cameras may stand (vct_app_camera): not inside these boxes
     std::vector<std::array<double, 6>> boxes;
     double cam_range = 0.0;
@@ -582,7 +586,11 @@ static int app_finish(vct_app *a, vct_app **out) {
     m.object = a->mesh.obj.data();
     m.roughness = a->mesh.rough.empty() ? nullptr : a->mesh.rough.data();
     m.vertex_tangents = a->mesh.vtan.empty() ? nullptr : a->mesh.vtan.data();
-    if ((rc = vct_set_scene(a->ctx, &m)) || (rc = vct_set_lights(a->ctx, a->lights.data(), (int)a->lights.size()))) {
+    m.texcoords = a->mesh.uv.empty() ? nullptr : a->mesh.uv.data();
+    m.texture = a->mesh.texid.empty() ? nullptr : a->mesh.texid.data();
+    if ((rc = vct_set_scene(a->ctx, &m)) ||
+        (rc = vct_set_textures(a->ctx, a->textures.empty() ? nullptr : a->textures.data(), (int)a->textures.size())) ||
+        (rc = vct_set_lights(a->ctx, a->lights.data(), (int)a->lights.size()))) {
         vct_app_destroy(a);
         return rc;
     }
@@ -612,6 +620,21 @@ int vct_app_create_from_obj(vct_app **out, const vct_app_config *cfg_in, const c
     a->mesh.obj.assign(m.object, m.object + m.num_tris);
     if (m.roughness) a->mesh.rough.assign(m.roughness, m.roughness + 1);
     a->mesh.nobj = 1;
+    const vct_texture *tex = nullptr;
+    int ntex = 0;
+    vct_obj_textures(obj, &tex, &ntex);
+    if (ntex > 0) {  // the diffuse maps, copied: the obj is freed below
+        a->mesh.uv.assign(m.texcoords, m.texcoords + (size_t)m.num_tris * 6);
+        a->mesh.texid.assign(m.texture, m.texture + m.num_tris);
+        for (int i = 0; i < ntex; ++i) {
+            size_t n = 0;
+            for (int l = 0; l < tex[i].levels; ++l)
+                n += (size_t)std::max(1, tex[i].width >> l) * (size_t)std::max(1, tex[i].height >> l);
+            a->tex_texels.emplace_back(tex[i].texels, tex[i].texels + n);
+            a->textures.push_back(tex[i]);
+        }
+        for (int i = 0; i < ntex; ++i) a->textures[i].texels = a->tex_texels[i].data();
+    }
     float bmin[3], bmax[3];
     vct_obj_bounds(obj, bmin, bmax, nullptr);
     vct_obj_free(obj);

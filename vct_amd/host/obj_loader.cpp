@@ -11,15 +11,22 @@
 //     unnormalized tangent to its three vertices, and each vertex's sum is
 //     normalized (Mesh.cpp:190-197);
 //   * the diffuse colour Kd of the face's material (usemtl/newmtl) as RGBA8,
-//     and the PBR roughness Pr per material.
+//     and the PBR roughness Pr per material;
+//   * the material's diffuse map (map_Kd, path relative to the .obj with
+//     backslashes made slashes, Mesh.cpp:62-81) when it is a .dds file: the
+//     faces of that material carry their corners' uvs and the map's index
+//     (maps are loaded once per path); a corner without vt has uv (0, 0).
 // Differences, on purpose: polygons become triangle fans (tinyobjloader's
 // default triangulation for convex faces); a face without vn gets its face
 // normal and one without usable uvs an orthonormal tangent from the normal
 // (the reference would produce NaN there) and adds nothing to its vertices'
-// tangent sums (a vertex whose sum is zero keeps its face's tangent); textures (map_Kd, map_Pr, ...) are
-// not decoded -- the image has no image decoder -- so a face's albedo is its
-// material's Kd, white without a material, and its roughness Pr (0.5 if the
-// material has none).  One object: the file is one actor, as in Scene.
+// tangent sums (a vertex whose sum is zero keeps its face's tangent); maps in
+// other formats (stb_image's PNG, JPEG, TGA ...) and the other maps (map_Pr,
+// map_Ks, ...) are not decoded -- there is no image decoder here -- so such a
+// face's albedo is its material's Kd, white without a material, and its
+// roughness Pr (0.5 if the material has none); a .dds map that cannot be read
+// is skipped the same way (the reference logs it and draws on).  One object:
+// the file is one actor, as in Scene.
 #include <algorithm>
 #include <array>
 #include <cmath>
@@ -38,10 +45,17 @@
 struct vct_obj {
     std::vector<float> pos, nrm, tan;
     std::vector<float> vtan;  // [ntri][3][3]: the normalized per-vertex tangent sums
+    std::vector<float> uv;    // [ntri][3][2]
+    std::vector<int32_t> texid;
+    std::vector<vct_image *> images;
+    std::vector<vct_texture> textures;
     std::vector<uint32_t> albedo, object;
     std::vector<float> rough;  // one value: the file is one object
     int nmaterials = 0;
     float bmin[3], bmax[3];
+    ~vct_obj() {
+        for (vct_image *i : images) vct_dds_free(i);
+    }
 };
 
 namespace {
@@ -49,6 +63,8 @@ namespace {
 struct Material {
     float kd[3] = {1.0f, 1.0f, 1.0f};
     float pr = -1.0f;  // no roughness given
+    std::string map_kd;
+    int tex = -2;  // index into vct_obj::textures; -1: no usable map; -2: not looked up yet
 };
 
 uint32_t pack_rgba8(const float c[3]) {
@@ -100,6 +116,11 @@ bool load_mtl(const std::string &path, std::map<std::string, Material> &mats, st
             ss >> m.kd[0] >> m.kd[1] >> m.kd[2];
         } else if (key == "Pr" && !cur.empty()) {
             ss >> mats[cur].pr;
+        } else if (key == "map_Kd" && !cur.empty()) {
+            std::string rest, tok;
+            while (ss >> tok) rest = tok;  // the file name ends the statement
+            std::replace(rest.begin(), rest.end(), '\\', '/');
+            mats[cur].map_kd = rest;
         }
     }
     return true;
@@ -127,8 +148,10 @@ int vct_obj_load(const char *path, vct_obj **out, char *err, int errlen) {
     const std::string dir = sp.substr(0, sp.find_last_of('/') + 1);
     std::vector<float> v, vn, vt;
     std::map<std::string, Material> mats;
-    const Material none;
-    const Material *cur = &none;
+    Material none;
+    none.tex = -1;
+    Material *cur = &none;
+    std::map<std::string, int> tex_by_path;
     bool any_pr = false;
     float pr = 0.5f;
     vct_obj *o = new vct_obj();
@@ -170,6 +193,28 @@ int vct_obj_load(const char *path, vct_obj **out, char *err, int errlen) {
             ss >> name;
             auto it = mats.find(name);
             cur = it == mats.end() ? &none : &it->second;
+            if (cur->tex == -2) {  // its diffuse map, loaded once per path
+                cur->tex = -1;
+                const std::string &mk = cur->map_kd;
+                const bool dds = mk.size() > 4 && (mk.compare(mk.size() - 4, 4, ".dds") == 0 ||
+                                                   mk.compare(mk.size() - 4, 4, ".DDS") == 0);
+                if (dds) {
+                    auto tp = tex_by_path.find(mk);
+                    if (tp != tex_by_path.end()) {
+                        cur->tex = tp->second;
+                    } else {
+                        vct_image *img = nullptr;
+                        if (vct_dds_load((dir + mk).c_str(), &img, nullptr, 0) == VCT_OK) {
+                            vct_texture t;
+                            vct_dds_texture(img, &t);
+                            o->images.push_back(img);
+                            o->textures.push_back(t);
+                            cur->tex = (int)o->textures.size() - 1;
+                        }
+                        tex_by_path[mk] = cur->tex;
+                    }
+                }
+            }
             if (cur->pr >= 0.0f) {
                 any_pr = true;
                 pr = cur->pr;
@@ -263,6 +308,11 @@ int vct_obj_load(const char *path, vct_obj **out, char *err, int errlen) {
                     o->nrm.insert(o->nrm.end(), N[j], N[j] + 3);
                 }
                 o->tan.insert(o->tan.end(), tg, tg + 3);
+                for (int j = 0; j < 3; ++j) {
+                    o->uv.push_back(it_[c3[j]] >= 0 ? T[j][0] : 0.0f);
+                    o->uv.push_back(it_[c3[j]] >= 0 ? T[j][1] : 0.0f);
+                }
+                o->texid.push_back(cur->tex >= 0 ? cur->tex : -1);
                 o->albedo.push_back(pack_rgba8(cur->kd));
                 o->object.push_back(0u);
             }
@@ -312,6 +362,15 @@ int vct_obj_mesh(const vct_obj *o, vct_mesh *m) {
     m->object = o->object.data();
     m->roughness = o->rough.empty() ? nullptr : o->rough.data();
     m->vertex_tangents = o->vtan.data();
+    m->texcoords = o->uv.data();
+    m->texture = o->texid.data();
+    return VCT_OK;
+}
+
+int vct_obj_textures(const vct_obj *o, const vct_texture **tex, int *count) {
+    if (!o) return VCT_ERR_INVALID;
+    if (tex) *tex = o->textures.empty() ? nullptr : o->textures.data();
+    if (count) *count = (int)o->textures.size();
     return VCT_OK;
 }
 
