@@ -59,12 +59,14 @@ struct InjectLight {
     float ls_inverse[16];
     const float *shadowmap;
     int N, perspective;
+    double rcp_n;  // RN64(1 / N)
 };
 
 // Up to kScatterBatch shadow-casting lights per launch, one per grid z slice.
 constexpr int kScatterBatch = 8;
 struct InjectBatch {
     InjectLight l[kScatterBatch];
+    double rcp_ext[3];  // RN64(1 / (vmax - vmin)) per axis, the difference in float
 };
 
 __global__ __launch_bounds__(256) void k_inject_scatter(InjectBatch B, int dim, Extent ext, const float4 *warp,
@@ -73,18 +75,21 @@ __global__ __launch_bounds__(256) void k_inject_scatter(InjectBatch B, int dim, 
     const int i = blockIdx.x * 16 + (threadIdx.x & 15);
     const int j = blockIdx.y * 16 + (threadIdx.x >> 4);
     if (i >= L.N || j >= L.N) return;
-    const float tcx = (float)i / (float)L.N, tcy = (float)j / (float)L.N;
+    // every division is quot_f32 with the divisor's FP64 reciprocal: the IEEE
+    // quotient, without a float division sequence per texel
+    const float tcx = quot_f32((float)i, L.rcp_n), tcy = quot_f32((float)j, L.rcp_n);
     const float d = shadow_bilinear(L.shadowmap, L.N, tcx, tcy, 0, 0);
     float ndc4[4] = {tcx * 2.0f - 1.0f, tcy * 2.0f - 1.0f, d * 2.0f - 1.0f, 1.0f}, w[4];
     mat4_vec4(L.ls_inverse, ndc4, w);
     if (L.perspective) {
-        w[0] = w[0] / w[3];
-        w[1] = w[1] / w[3];
-        w[2] = w[2] / w[3];
+        const double rw = 1.0 / (double)w[3];
+        w[0] = quot_f32(w[0], rw);
+        w[1] = quot_f32(w[1], rw);
+        w[2] = quot_f32(w[2], rw);
     }
     int idx[3];
     float lin[3];
-    for (int k = 0; k < 3; ++k) lin[k] = ((w[k] - ext.center[k]) - ext.vmin[k]) / (ext.vmax[k] - ext.vmin[k]);
+    for (int k = 0; k < 3; ++k) lin[k] = quot_f32((w[k] - ext.center[k]) - ext.vmin[k], B.rcp_ext[k]);
     if (warp) warp_sample(warp, lin, lin);  // getVoxelPosition with warpTexture (common.glsl:48-50)
     for (int k = 0; k < 3; ++k) idx[k] = (int)((float)dim * lin[k]);
     if (idx[0] < 0 || idx[1] < 0 || idx[2] < 0 || idx[0] >= dim || idx[1] >= dim || idx[2] >= dim) return;
@@ -99,6 +104,7 @@ hipError_t launch_inject_scatter(const DevLight *lights, const DevLight *dlights
     hipError_t e = launch_fill_u32((uint32_t *)litmask, 0u, nv / 4, s);
     if (e != hipSuccess) return e;
     InjectBatch B;
+    for (int k = 0; k < 3; ++k) B.rcp_ext[k] = 1.0 / (double)(ext.vmax[k] - ext.vmin[k]);
     int nb = 0, nmax = 0;
     for (int li = 0; li <= nlights; ++li) {
         if (li < nlights) {
@@ -109,6 +115,7 @@ hipError_t launch_inject_scatter(const DevLight *lights, const DevLight *dlights
             il.shadowmap = L.shadowmap;
             il.N = (int)L.shadow_size;
             il.perspective = (int)L.perspective;
+            il.rcp_n = 1.0 / (double)(float)il.N;
             nmax = il.N > nmax ? il.N : nmax;
         }
         if (nb == kScatterBatch || (li == nlights && nb > 0)) {  // each stored texel writes 1: order-free

@@ -143,16 +143,6 @@ __device__ __forceinline__ void load_edges(const int4 *__restrict__ recs, size_t
     e.user = r3.w;
 }
 
-// a / b correctly rounded to float (the IEEE division a / b), for a finite
-// float a >= 0 and a float b > 0 whose quotient is a normal float or zero,
-// given rcp = RN64(1 / b).  q = RN64(a * rcp) is within (1 + 2^-53)^2 - 1 <
-// 2^-52 of a / b, relatively.  A rounding boundary of float is a midpoint
-// m = M 2^k, M odd with 25 bits; with a = A 2^i, b = B 2^j (A, B < 2^24),
-// a - m b is a nonzero multiple of 2^min(i, k + j) (it is never zero: A would
-// need the odd factor M), so |a / b - m| >= 2^-50 |a / b|.  q therefore rounds
-// to float on the same side of every midpoint as a / b: RN32(q) = RN32(a / b).
-__device__ __forceinline__ float quot_f32(float a, double rcp) { return (float)((double)a * rcp); }
-
 // Edge i at sample (px * 256 + ox, py * 256 + oy), ox, oy in [0, 256).
 __device__ __forceinline__ int64_t edge_value(const Edges &e, int i, int px, int py, int ox, int oy) {
     if (e.small) {
