@@ -2,7 +2,8 @@
 // 455-520): direct light per light (Cook-Torrance or Blinn-Phong) scaled by
 // its PCF shadow factor, the traced indirect light and reflection, occlusion,
 // and the Reinhard + gamma post-process.  One thread per pixel; the operation
-// order is fixed (no FP contraction) and matches the oracle's or_compose.
+// order is fixed (no FP contraction) and matches the oracle's or_compose, whose
+// powf(x, 5) in the Fresnel term is x^4 * x here.
 #pragma clang fp contract(off)
 
 #include "vct_device.h"
@@ -15,8 +16,11 @@ constexpr float kPi = 3.1415982f;  // PI of shaders/common.glsl:1, as the shader
 
 __device__ __forceinline__ float pow2(float x) { return x * x; }
 
+// albedo / PI: the IEEE quotient by the constant's FP64 reciprocal
+__device__ __forceinline__ float div_pi(float x) { return quot_f32(x, 1.0 / (double)kPi); }
+
 // cook_torrance (phong.frag:229-253) with metallic 0: diffuse and specular
-__device__ __forceinline__ void cook_torrance(const float alb[3], const float col[3], const float N[3],
+__device__ __forceinline__ void cook_torrance(const float lambert[3], const float col[3], const float N[3],
                                               const float V[3], const float L[3], const float H[3], float rough,
                                               float dif[3], float spe[3]) {
     const float n_dot_l = fmaxf(0.0f, dot3(N, L));
@@ -31,12 +35,14 @@ __device__ __forceinline__ void cook_torrance(const float alb[3], const float co
     const float G = (n_dot_v / (n_dot_v * (1.0f - k) + k)) * (n_dot_l / (n_dot_l * (1.0f - k) + k));
     // F0 = mix(0.04, albedo, metallic = 0); schlicks (phong.frag:547-549)
     const float F0 = 0.04f;
-    const float F = F0 + (1.0f - F0) * powf(1.0f - h_dot_v, 5.0f);
+    // pow(x, 5) as x^4 * x: within 2 ulp of the power (the C library's powf
+    // and a GPU's pow are themselves approximations)
+    const float u = 1.0f - h_dot_v;
+    const float F = F0 + (1.0f - F0) * (pow2(pow2(u)) * u);
     const float ct = ((D * G) * F) / fmaxf((4.0f * n_dot_l) * n_dot_v, 0.001f);
     const float kd = 1.0f - F;
     for (int c = 0; c < 3; ++c) {
-        const float lambert = alb[c] / kPi;
-        dif[c] = ((col[c] * n_dot_l) * kd) * lambert;
+        dif[c] = ((col[c] * n_dot_l) * kd) * lambert[c];
         spe[c] = (col[c] * n_dot_l) * ct;
     }
 }
@@ -80,6 +86,7 @@ __global__ __launch_bounds__(256) void k_compose(ComposeArgs a) {
     float V[3];
     nrm3(ve, V);
     float dif[3] = {0.f, 0.f, 0.f}, spe[3] = {0.f, 0.f, 0.f};
+    const float lambert[3] = {div_pi(alb[0]), div_pi(alb[1]), div_pi(alb[2])};  // the same in every light
     for (int l = 0; l < a.nlights; ++l) {
         const DevLight &Lt = a.lights[l];
         if (!Lt.enabled) continue;
@@ -99,7 +106,7 @@ __global__ __launch_bounds__(256) void k_compose(ComposeArgs a) {
         nrm3(vl, H);
         float ld[3], ls[3];
         if (a.p.cook_torrance) {
-            cook_torrance(alb, Lt.color, N, V, Ld, H, rough, ld, ls);
+            cook_torrance(lambert, Lt.color, N, V, Ld, H, rough, ld, ls);
             if (Lt.type == 0u)
                 for (int c = 0; c < 3; ++c) {
                     ld[c] = ld[c] * att;

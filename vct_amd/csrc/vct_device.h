@@ -19,11 +19,25 @@ namespace vctk {
 __device__ __forceinline__ float dot3(const float a[3], const float b[3]) {
     return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
 }
+// a / b correctly rounded to float (the IEEE division a / b), for finite
+// floats a and b != 0 whose quotient is a normal float or zero, given rcp =
+// RN64(1 / b).  q = RN64(a * rcp) is within (1 + 2^-53)^2 - 1 < 2^-52 of a / b,
+// relatively.  A rounding boundary of float is a midpoint m = M 2^k, M odd
+// with 25 bits; with |a| = A 2^i, |b| = B 2^j (A, B < 2^24), |a| - m |b| is a
+// nonzero multiple of 2^min(i, k + j) (it is never zero: A would need the odd
+// factor M), so ||a / b| - m| >= 2^-50 |a / b|.  q therefore rounds to float
+// on the same side of every midpoint as a / b (rounding to nearest is
+// symmetric in sign): RN32(q) = RN32(a / b).
+__device__ __forceinline__ float quot_f32(float a, double rcp) { return (float)((double)a * rcp); }
+
+// v / length(v), each component the IEEE quotient v[k] / l (quot_f32 with one
+// FP64 reciprocal instead of three float divisions)
 __device__ __forceinline__ void nrm3(const float v[3], float o[3]) {
-    float l = __builtin_sqrtf(dot3(v, v));
-    o[0] = v[0] / l;
-    o[1] = v[1] / l;
-    o[2] = v[2] / l;
+    const float l = __builtin_sqrtf(dot3(v, v));
+    const double r = 1.0 / (double)l;
+    o[0] = quot_f32(v[0], r);
+    o[1] = quot_f32(v[1], r);
+    o[2] = quot_f32(v[2], r);
 }
 __device__ __forceinline__ void cross3(const float a[3], const float b[3], float o[3]) {
     float t0 = a[1] * b[2] - a[2] * b[1];
@@ -112,16 +126,6 @@ __device__ __forceinline__ uint32_t pack_unorm4x8(float r, float g, float b, flo
 // the rounded reciprocal is one ulp off for 126 of the 256 bytes, and one
 // correction step on the exact residual (two fmas) gives the correctly rounded
 // quotient for every byte 0..255 (checked exhaustively in exact arithmetic).
-// a / b correctly rounded to float (the IEEE division a / b), for finite
-// floats a and b != 0 whose quotient is a normal float or zero, given rcp =
-// RN64(1 / b).  q = RN64(a * rcp) is within (1 + 2^-53)^2 - 1 < 2^-52 of a / b,
-// relatively.  A rounding boundary of float is a midpoint m = M 2^k, M odd
-// with 25 bits; with |a| = A 2^i, |b| = B 2^j (A, B < 2^24), |a| - m |b| is a
-// nonzero multiple of 2^min(i, k + j) (it is never zero: A would need the odd
-// factor M), so ||a / b| - m| >= 2^-50 |a / b|.  q therefore rounds to float
-// on the same side of every midpoint as a / b (rounding to nearest is
-// symmetric in sign): RN32(q) = RN32(a / b).
-__device__ __forceinline__ float quot_f32(float a, double rcp) { return (float)((double)a * rcp); }
 
 __device__ __forceinline__ float unorm8(uint32_t v, int i) {
     const float x = (float)((v >> (8 * i)) & 0xffu);
