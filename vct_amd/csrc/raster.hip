@@ -491,7 +491,8 @@ __global__ __launch_bounds__(256) void k_gbuf_resolve(DevScene sc, CamMats cm, c
     b[1] = ax[2] * ay[0] - ay[2] * ax[0];
     b[2] = ax[0] * ay[1] - ay[0] * ax[1];
     const float sum = (b[0] + b[1]) + b[2];
-    for (int v = 0; v < 3; ++v) b[v] = b[v] / sum;
+    const double rsum = 1.0 / (double)sum;  // the three IEEE quotients b[v] / sum (quot_f32)
+    for (int v = 0; v < 3; ++v) b[v] = quot_f32(b[v], rsum);
     const float *Nm = sc.nmat + (size_t)sc.obj[t] * 9;
     float Tw[3];
     mat3_vec3(Nm, sc.tan + (size_t)t * 3, Tw);

@@ -167,15 +167,18 @@ __device__ __forceinline__ bool edge_inside(const Edges &e, int px, int py, int 
     return in;
 }
 
+// lambda_i = E_i / area, each the IEEE quotient (quot_f32 with one FP64
+// reciprocal of the area for the three)
 __device__ __forceinline__ void edge_bary(const Edges &e, int px, int py, int ox, int oy, float l[3]) {
+    const double r = 1.0 / (double)e.area;
     if (e.small) {
         const int dx = ((px - e.x0) << 8) + ox, dy = ((py - e.y0) << 8) + oy;
 #pragma unroll
-        for (int i = 0; i < 3; ++i) l[i] = (float)(e.E0[i] + __mul24(e.A[i], dx) + __mul24(e.B[i], dy)) / e.area;
+        for (int i = 0; i < 3; ++i) l[i] = quot_f32((float)(e.E0[i] + __mul24(e.A[i], dx) + __mul24(e.B[i], dy)), r);
         return;
     }
 #pragma unroll
-    for (int i = 0; i < 3; ++i) l[i] = (float)edge_value(e, i, px, py, ox, oy) / e.area;
+    for (int i = 0; i < 3; ++i) l[i] = quot_f32((float)edge_value(e, i, px, py, ox, oy), r);
 }
 
 // 4x MSAA sample offsets in 1/256 px: (0.375,0.125) (0.875,0.375) (0.125,0.625) (0.625,0.875)

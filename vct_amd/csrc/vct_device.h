@@ -319,7 +319,8 @@ __device__ __forceinline__ void shadow_taps(const float lsp[4], const float4 *sq
     if (__ballot(lsp[3] != 1.0f) == 0) {  // orthographic light space: w = 1, and x / 1 is x
         for (int k = 0; k < 3; ++k) sh[k] = (lsp[k] + 1.0f) * 0.5f;
     } else {
-        for (int k = 0; k < 3; ++k) sh[k] = (lsp[k] / lsp[3] + 1.0f) * 0.5f;
+        const double rw = 1.0 / (double)lsp[3];  // the three IEEE quotients by w (quot_f32)
+        for (int k = 0; k < 3; ++k) sh[k] = (quot_f32(lsp[k], rw) + 1.0f) * 0.5f;
     }
     st.s = sh[0];
     st.t = sh[1];

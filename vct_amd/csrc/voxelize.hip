@@ -164,9 +164,10 @@ __device__ __forceinline__ void rgba8_avg(uint32_t *p, float r, float g, float b
         c2 = c2 * c3;
         float s0 = c0 + v0, s1 = c1 + v1, s2 = c2 + v2;
         const float s3 = c3 + v3;
-        s0 = s0 / s3;
-        s1 = s1 / s3;
-        s2 = s2 / s3;
+        const double r3 = 1.0 / (double)s3;  // the three IEEE quotients by s3 (quot_f32)
+        s0 = quot_f32(s0, r3);
+        s1 = quot_f32(s1, r3);
+        s2 = quot_f32(s2, r3);
         nv = rgba8_from_floats(s0, s1, s2, s3);
     }
 }
