@@ -453,8 +453,10 @@ int vct_learned_prepare(vct_ctx *ctx);
 int vct_learned_train(vct_ctx *ctx, const vct_learned_params *p, float *loss);
 /* Learned gather over the current G-buffer into VCT_BUF_INDIRECT: (rgb, 1)
  * from the network, or the 6-cone tracer's (rgb*occ, occ) on tiles whose
- * predicted-luminance variance exceeds variance_threshold.  fallback_tiles
- * (may be NULL; non-NULL blocks) receives the number of such tiles. */
+ * predicted-luminance variance exceeds variance_threshold (an infinite or NaN
+ * threshold sends no tile to the tracer, which is then not launched).
+ * fallback_tiles (may be NULL; non-NULL blocks) receives the number of such
+ * tiles. */
 int vct_trace_learned(vct_ctx *ctx, const vct_trace_params *p, float variance_threshold, int *fallback_tiles);
 /* The 64-cone reference of the current G-buffer into VCT_BUF_REFERENCE. */
 int vct_reference_gather(vct_ctx *ctx);

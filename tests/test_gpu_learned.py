@@ -250,3 +250,12 @@ def test_inference_matches_reference_network_and_fallback(prepared):
     assert nfb == ((cam.width + 7) // 8) * ((cam.height + 7) // 8)
     out = ctx.download(abi.VCT_BUF_INDIRECT, 0, np.float32).reshape(-1, 4)
     assert np.abs(out - plain).max() <= 1e-6 * max(np.abs(plain).max(), 1e-30)
+    # an infinite threshold lists no tile (and launches no tracer): the image is
+    # the network's alone, bit for bit the finite no-fallback one, and the count
+    # is 0 even right after a call that listed every tile
+    ctx.trace_learned(tr, 1e9)
+    finite = ctx.download(abi.VCT_BUF_INDIRECT, 0, np.float32).reshape(-1, 4)
+    ctx.trace_learned(tr, -1.0)
+    assert ctx.trace_learned(tr, float("inf")) == 0
+    inf_out = ctx.download(abi.VCT_BUF_INDIRECT, 0, np.float32).reshape(-1, 4)
+    assert np.array_equal(inf_out.view(np.uint32), finite.view(np.uint32))

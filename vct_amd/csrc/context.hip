@@ -1507,7 +1507,9 @@ int vct_trace_learned(vct_ctx *c, const vct_trace_params *p, float thr, int *fal
     PassTimer pt(c, P_LEARNED);
     LAUNCHCHK(c, launch_learned(c->mlp, c->pyr, t, thr, c->d_tile_count, c->stream));
     pt.stop();
-    if (fallback) {
+    if (fallback && !(thr < __builtin_huge_valf())) {
+        *fallback = 0;  // no tile can exceed an infinite threshold: nothing was listed
+    } else if (fallback) {
         uint32_t n = 0;
         HIPCHK(c, hipMemcpyAsync(&n, c->d_tile_count, 4, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));

@@ -1087,8 +1087,13 @@ constexpr int kFallbackBlocks = 1024;
 hipError_t launch_learned(const MlpBuffers &B, const PyrDesc &pyr, const TraceLaunch &t, float threshold,
                           uint32_t *fallback_list, hipStream_t s) {
     const int tiles = ((t.width + 7) / 8) * ((t.height + 7) / 8);
-    hipError_t e = launch_fill_u32(fallback_list, 0u, 1, s);
-    if (e != hipSuccess) return e;
+    // var > threshold never holds for an infinite (or NaN) threshold: no tile
+    // can be listed, so neither the list nor the tracer is needed
+    const bool fallback = threshold < __builtin_huge_valf();
+    if (fallback) {
+        hipError_t e = launch_fill_u32(fallback_list, 0u, 1, s);
+        if (e != hipSuccess) return e;
+    }
     const int mode = pyr.storage == VCT_STORE_FP32 ? 1 : (pyr.storage == VCT_STORE_FP8 ? 2 : 0);
     const MlpDev W = mlp_dev(B);
     const PyrArgs pa = pyr_args(pyr, t.nested);
@@ -1097,13 +1102,13 @@ hipError_t launch_learned(const MlpBuffers &B, const PyrDesc &pyr, const TraceLa
     const dim3 fb(fb_blocks < kFallbackBlocks ? fb_blocks : kFallbackBlocks);
     if (mode == 0) {
         hipLaunchKernelGGL(k_learned_stream<0>, dim3(tiles), dim3(64), 0, s, W, t, pa, pv, threshold, fallback_list);
-        hipLaunchKernelGGL(k_learned_fallback<0>, fb, dim3(256), 0, s, t, pa, fallback_list);
+        if (fallback) hipLaunchKernelGGL(k_learned_fallback<0>, fb, dim3(256), 0, s, t, pa, fallback_list);
     } else if (mode == 1) {
         hipLaunchKernelGGL(k_learned<1>, dim3(tiles), dim3(64), 0, s, W, t, pa, pv, threshold, fallback_list);
-        hipLaunchKernelGGL(k_learned_fallback<1>, fb, dim3(256), 0, s, t, pa, fallback_list);
+        if (fallback) hipLaunchKernelGGL(k_learned_fallback<1>, fb, dim3(256), 0, s, t, pa, fallback_list);
     } else {
         hipLaunchKernelGGL(k_learned_stream<2>, dim3(tiles), dim3(64), 0, s, W, t, pa, pv, threshold, fallback_list);
-        hipLaunchKernelGGL(k_learned_fallback<2>, fb, dim3(256), 0, s, t, pa, fallback_list);
+        if (fallback) hipLaunchKernelGGL(k_learned_fallback<2>, fb, dim3(256), 0, s, t, pa, fallback_list);
     }
     return hipGetLastError();
 }
