@@ -84,13 +84,24 @@ int vct_dds_load(const char *path, vct_image **out, char *err, int errlen);
 int vct_dds_texture(const vct_image *img, vct_texture *out);
 void vct_dds_free(vct_image *img);
 
+/* ---- other images: GLHelper::createTextureFromImage (GLHelper.cpp:165-210) ---------
+ * A PNG file (every PNG colour type and depth, interlaced or not) as stb_image
+ * returns it with its own channel count, uploaded at GL's default unpack
+ * alignment of 4 as GL_R8 (1 channel: samples (r, 0, 0, 1)), GL_RGB8 or
+ * GL_RGBA8 (2 channels get no storage: an incomplete texture, no levels), and
+ * floor(log2(max(w, h))) + 1 levels by a 2x2 box (glGenerateTextureMipmap's
+ * filter is the implementation's).  The image is viewed and freed with
+ * vct_dds_texture and vct_dds_free.  JPEG, TGA and the other stb_image
+ * formats are not decoded. */
+int vct_png_load(const char *path, vct_image **out, char *err, int errlen);
+
 /* ---- OBJ/MTL assets: Mesh::loadMesh (src/Graphics/Mesh.cpp:42-266) ------------
  * Faces triangulated (fans), texture v flipped, per-face tangents from the uv
  * deltas (Mesh.cpp:174-189), albedo = the material's Kd as RGBA8 (white
  * without a material), roughness = the material's Pr standing in for a
- * constant roughness map.  A material's map_Kd that names a .dds file is
- * loaded (vct_dds_load, path relative to the .obj) and its faces carry their
- * corner uvs and that texture; other image formats (stb_image's) are not
+ * constant roughness map.  A material's map_Kd that names a .dds or .png file
+ * is loaded (vct_dds_load / vct_png_load, path relative to the .obj) and its
+ * faces carry their corner uvs and that texture; other image formats are not
  * decoded and those faces keep Kd.  A face without normals gets its face
  * normal, one without usable uvs a tangent orthogonal to it.  The file is one
  * object (one actor). */

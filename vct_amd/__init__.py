@@ -330,10 +330,20 @@ def _texture_tuple(t) -> tuple:
 def load_dds(path) -> tuple:
     """A DDS file (DXT1/3/5) decoded to RGBA8 levels (vct_dds_load):
     (width, height, levels, texels of every level back to back)."""
+    return _load_image(path, "vct_dds_load")
+
+
+def load_png(path) -> tuple:
+    """A PNG file as the reference uploads it (vct_png_load): (width, height,
+    levels, texels of every level back to back); 0 levels for two channels."""
+    return _load_image(path, "vct_png_load")
+
+
+def _load_image(path, fn) -> tuple:
     lib = load_app_library()
     h = C.c_void_p()
     err = C.create_string_buffer(256)
-    rc = lib.vct_dds_load(str(path).encode(), C.byref(h), err, 256)
+    rc = getattr(lib, fn)(str(path).encode(), C.byref(h), err, 256)
     if rc:
         raise VCTError(rc, err.value.decode())
     try:

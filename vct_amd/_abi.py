@@ -196,6 +196,7 @@ APP_FUNCS = [
     ("vct_obj_textures", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_int)]),
     ("vct_obj_free", None, [C.c_void_p]),
     ("vct_dds_load", C.c_int, [C.c_char_p, C.POINTER(C.c_void_p), C.c_char_p, C.c_int]),
+    ("vct_png_load", C.c_int, [C.c_char_p, C.POINTER(C.c_void_p), C.c_char_p, C.c_int]),
     ("vct_dds_texture", C.c_int, [C.c_void_p, C.POINTER(Texture)]),
     ("vct_dds_free", None, [C.c_void_p]),
     ("vct_app_destroy", None, [C.c_void_p]),

@@ -80,7 +80,7 @@ def build(jobs: int | None = None, verbose: bool = False) -> dict:
         _run([CXX, "-O2", "-fPIC", "-shared", "-std=c++17", "-Wall", "-ffp-contract=off",
               "-I", str(REPO / "include"), "-I", str(HOST),
               "-o", str(tmp), *map(str, host_sources),
-              "-L", str(LIB), "-lvct", "-Wl,-rpath,$ORIGIN"])
+              "-L", str(LIB), "-lvct", "-lz", "-Wl,-rpath,$ORIGIN"])
         os.replace(tmp, libapp)
         out["libvct_app"] = str(libapp)
     if verbose:

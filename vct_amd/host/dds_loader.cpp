@@ -29,12 +29,8 @@
 #include <string>
 #include <vector>
 
+#include "image.h"
 #include "vct_app.h"
-
-struct vct_image {
-    int width = 0, height = 0, levels = 0;
-    std::vector<uint32_t> texels;
-};
 
 namespace {
 

@@ -13,21 +13,22 @@
 //   * the diffuse colour Kd of the face's material (usemtl/newmtl) as RGBA8,
 //     and the PBR roughness Pr per material;
 //   * the material's diffuse map (map_Kd, path relative to the .obj with
-//     backslashes made slashes, Mesh.cpp:62-81) when it is a .dds file: the
-//     faces of that material carry their corners' uvs and the map's index
-//     (maps are loaded once per path); a corner without vt has uv (0, 0).
+//     backslashes made slashes, Mesh.cpp:62-81) when it is a .dds or .png
+//     file: the faces of that material carry their corners' uvs and the map's
+//     index (maps are loaded once per path); a corner without vt has uv (0, 0).
 // Differences, on purpose: polygons become triangle fans (tinyobjloader's
 // default triangulation for convex faces); a face without vn gets its face
 // normal and one without usable uvs an orthonormal tangent from the normal
 // (the reference would produce NaN there) and adds nothing to its vertices'
 // tangent sums (a vertex whose sum is zero keeps its face's tangent); maps in
-// other formats (stb_image's PNG, JPEG, TGA ...) and the other maps (map_Pr,
+// other formats (stb_image's JPEG, TGA, BMP ...) and the other maps (map_Pr,
 // map_Ks, ...) are not decoded -- there is no image decoder here -- so such a
 // face's albedo is its material's Kd, white without a material, and its
 // roughness Pr (0.5 if the material has none); a .dds map that cannot be read
 // is skipped the same way (the reference logs it and draws on).  One object:
 // the file is one actor, as in Scene.
 #include <algorithm>
+#include <cctype>
 #include <array>
 #include <cmath>
 #include <cstdint>
@@ -196,15 +197,19 @@ int vct_obj_load(const char *path, vct_obj **out, char *err, int errlen) {
             if (cur->tex == -2) {  // its diffuse map, loaded once per path
                 cur->tex = -1;
                 const std::string &mk = cur->map_kd;
-                const bool dds = mk.size() > 4 && (mk.compare(mk.size() - 4, 4, ".dds") == 0 ||
-                                                   mk.compare(mk.size() - 4, 4, ".DDS") == 0);
-                if (dds) {
+                std::string ext = mk.size() > 4 ? mk.substr(mk.size() - 4) : std::string();
+                for (char &ch : ext) ch = (char)std::tolower((unsigned char)ch);
+                const bool dds = ext == ".dds", png = ext == ".png";  // GLHelper.cpp:166-171
+                if (dds || png) {
                     auto tp = tex_by_path.find(mk);
                     if (tp != tex_by_path.end()) {
                         cur->tex = tp->second;
                     } else {
                         vct_image *img = nullptr;
-                        if (vct_dds_load((dir + mk).c_str(), &img, nullptr, 0) == VCT_OK) {
+                        const std::string full = dir + mk;
+                        const int rc = dds ? vct_dds_load(full.c_str(), &img, nullptr, 0)
+                                           : vct_png_load(full.c_str(), &img, nullptr, 0);
+                        if (rc == VCT_OK) {
                             vct_texture t;
                             vct_dds_texture(img, &t);
                             o->images.push_back(img);
