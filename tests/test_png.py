@@ -213,6 +213,13 @@ def test_bad_files_are_rejected(tmp_path):
     p.write_bytes(data[:i + 8])  # cut inside the image data
     with pytest.raises(vct_amd.VCTError):
         vct_amd.load_png(p)
+    # a 30000 x 30000 RGBA16 header over 20 bytes of data: deflate cannot
+    # expand that far, so it is refused without allocating the 7 GB it names
+    body = struct.pack(">IIBBBBB", 30000, 30000, 16, 6, 0, 0, 0)
+    p.write_bytes(b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", body) + chunk(b"IDAT", zlib.compress(b"\0" * 64))
+                  + chunk(b"IEND", b""))
+    with pytest.raises(vct_amd.VCTError, match="too short"):
+        vct_amd.load_png(p)
 
 
 def test_obj_material_map_kd_png(tmp_path):

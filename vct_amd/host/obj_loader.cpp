@@ -22,9 +22,9 @@
 // (the reference would produce NaN there) and adds nothing to its vertices'
 // tangent sums (a vertex whose sum is zero keeps its face's tangent); maps in
 // other formats (stb_image's JPEG, TGA, BMP ...) and the other maps (map_Pr,
-// map_Ks, ...) are not decoded -- there is no image decoder here -- so such a
+// map_Ks, ...) are not decoded -- only DDS and PNG decoders are built -- so such a
 // face's albedo is its material's Kd, white without a material, and its
-// roughness Pr (0.5 if the material has none); a .dds map that cannot be read
+// roughness Pr (0.5 if the material has none); a map that cannot be read
 // is skipped the same way (the reference logs it and draws on).  One object:
 // the file is one actor, as in Scene.
 #include <algorithm>

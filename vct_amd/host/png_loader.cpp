@@ -202,6 +202,13 @@ int vct_png_load(const char *path, vct_image **out, char *err, int errlen) {
         const uint32_t pw = p.w > x0 ? (p.w - x0 + dx - 1) / dx : 0, ph = p.h > y0 ? (p.h - y0 + dy - 1) / dy : 0;
         if (pw && ph) total += (size_t)ph * (rowbytes_of(pw) + 1);
     }
+    // deflate expands at most 1032:1 (a 258-byte match per two-bit code): a
+    // header announcing more than the data can hold is rejected before any
+    // buffer of its size is allocated
+    if (total / 1032 > p.idat.size() + 64) {
+        set_err(err, errlen, std::string("PNG image data too short for its size: ") + path);
+        return VCT_ERR_INVALID;
+    }
     std::vector<unsigned char> raw(total);
     uLongf got = (uLongf)total;
     if (uncompress(raw.data(), &got, p.idat.data(), (uLong)p.idat.size()) != Z_OK || got != total) {
