@@ -5,15 +5,28 @@ The chain does not read level-0 bricks that the injection recorded as +0, a
 record carried from frame to frame.  The scene alternates between its own
 placement and one shifted by a third of the grid, so 8^3 regions go from
 empty to filled and back while others stay empty; every frame's levels and
-faces are bit-exact against the oracle's chain of that frame's level 0.
+faces are bit-exact against the oracle's chain of that frame's level 0, and
+the albedo chain against the oracle's chain of that frame's albedo grid (the
+injection records the grid's all-zero bricks, which that chain does not read;
+a voxelization without an injection after it voids the record).
 """
 import numpy as np
 import pytest
 
 from gpu_util import make_app
 from oracle import binding as ob
+from vct_amd import _abi as abi
 
 pytestmark = pytest.mark.gpu
+
+
+def check_albedo_chain(ctx, g, alb, label):
+    """The voxelColor chain of the same mipmap pass (Application.cpp:904-917)
+    against the oracle's, from the albedo grid as the chain read it."""
+    ref = ob.mip_albedo(alb, g.levels)
+    for l in range(1, g.levels):
+        got = ctx.download(abi.VCT_BUF_ALBEDO, l, np.uint32).reshape((max(1, g.dim >> l),) * 3)
+        assert np.array_equal(got, ref[l]), (label, l, int((got != ref[l]).sum()))
 
 
 @pytest.mark.parametrize("aniso", [1, 0], ids=["aniso", "iso"])
@@ -37,7 +50,9 @@ def test_mip_chain_bit_exact_over_frames(aniso):
             ctx.voxelize(fs["vox"])
             ctx.normalize_inject(fs["inj"])
             rad0 = ctx.radiance(0)
+            alb = ctx.albedo()
             ctx.mip(fs["mip"])
+            check_albedo_chain(ctx, g, alb, frame)
             occupied = np.abs(rad0.reshape(16, 8, 16, 8, 16, 8, 4)).sum(axis=(1, 3, 5, 6)) > 0
             empty.append(~occupied)
             assert occupied.any()
@@ -52,6 +67,16 @@ def test_mip_chain_bit_exact_over_frames(aniso):
                 for l in range(1, g.levels):
                     got = ctx.radiance(l)
                     assert np.array_equal(got.view(np.uint32), ref[l].view(np.uint32)), (frame, l)
+        # a voxelization followed by the chain without an injection: the
+        # chain reads the new grid whole (the injection's record of empty
+        # bricks describes the grid it saw, not this one)
+        fs = app.frame_state(0)
+        model = fs["model"].copy()
+        model[:, 12] -= shift
+        ctx.set_transforms(model)
+        ctx.voxelize(fs["vox"])
+        ctx.mip(fs["mip"])
+        check_albedo_chain(ctx, g, ctx.albedo(), "voxelize + mip")
         # the frames exercised every transition of a region
         e = np.stack(empty)
         assert (e[:-1] & e[1:]).any()      # empty in two frames in a row

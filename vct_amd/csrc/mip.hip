@@ -35,7 +35,7 @@ struct MipArgs {
 __device__ __forceinline__ bool src_zero(const MipArgs &a, int x, int y, int z) {
     if (a.zero0 == nullptr) return false;
     const uint32_t nb = (uint32_t)a.src.nb;
-    return a.zero0[((uint32_t)(z >> 2) * nb + (uint32_t)(y >> 2)) * nb + (uint32_t)(x >> 2)] != 0;
+    return (a.zero0[((uint32_t)(z >> 2) * nb + (uint32_t)(y >> 2)) * nb + (uint32_t)(x >> 2)] & 1u) != 0;
 }
 
 __device__ __forceinline__ float4 round_stage(float4 v, int fp32) {
@@ -350,13 +350,29 @@ __global__ __launch_bounds__(256) void k_albedo_chain(AlbedoMipArgs a) {
 // (eight 16-byte loads of 32 words), every level-1 texel is rounded to RGBA8
 // before level 2 is filtered from it, and the taps are summed in the BOX2
 // order of filterRadiance.comp -- the same values as filtering level by level.
-__global__ __launch_bounds__(256) void k_albedo_bricks(const uint32_t *__restrict__ grid, int dim, uint32_t *out1,
+__global__ __launch_bounds__(256) void k_albedo_bricks(const uint32_t *__restrict__ grid,
+                                                       const uint8_t *__restrict__ nz, int dim, uint32_t *out1,
                                                        uint32_t *out2) {
     const int nb = dim >> 2;
     const uint32_t b = blockIdx.x * 256u + threadIdx.x;
     if (b >= (uint32_t)nb * nb * nb) return;
     const int lg = __builtin_ctz((unsigned)nb);
     const int bx = (int)(b & (uint32_t)(nb - 1)), by = (int)((b >> lg) & (uint32_t)(nb - 1)), bz = (int)(b >> (2 * lg));
+    if (nz != nullptr && (nz[b] & 2u) != 0) {
+        // every word of the brick is 0: the texels it filters to are
+        // packUnorm4x8 of sums of zeros, 0, and the brick is not read
+        const int s1 = dim >> 1;
+#pragma unroll
+        for (int cz = 0; cz < 2; ++cz)
+#pragma unroll
+            for (int cy = 0; cy < 2; ++cy)
+                *(uint2 *)&out1[((size_t)(bz * 2 + cz) * s1 + (by * 2 + cy)) * s1 + bx * 2] = make_uint2(0u, 0u);
+        if (out2 != nullptr) {
+            const int s2 = dim >> 2;
+            out2[((size_t)bz * s2 + by) * s2 + bx] = 0u;
+        }
+        return;
+    }
     const uint4 *src = (const uint4 *)(grid + ((size_t)b << 6));
     const int s1 = dim >> 1;
     float4 l1[8];  // the rounded level-1 texels, (cx, cy, cz) = bits (2, 1, 0) in BOX2 order
@@ -404,11 +420,12 @@ __global__ __launch_bounds__(256) void k_albedo_bricks(const uint32_t *__restric
     out2[((size_t)bz * s2 + by) * s2 + bx] = pack_unorm4x8(v.x * 0.125f, v.y * 0.125f, v.z * 0.125f, v.w * 0.125f);
 }
 
-hipError_t launch_albedo_chain(const uint32_t *grid, uint32_t *const *levels, int dim, int nlevels, hipStream_t s) {
+hipError_t launch_albedo_chain(const uint32_t *grid, const uint8_t *nz, uint32_t *const *levels, int dim, int nlevels,
+                               hipStream_t s) {
     if (nlevels < 2) return hipSuccess;
     {
         const unsigned nbricks = (unsigned)((dim >> 2) * (dim >> 2) * (dim >> 2));
-        hipLaunchKernelGGL(k_albedo_bricks, dim3((nbricks + 255) / 256), dim3(256), 0, s, grid, dim, levels[1],
+        hipLaunchKernelGGL(k_albedo_bricks, dim3((nbricks + 255) / 256), dim3(256), 0, s, grid, nz, dim, levels[1],
                            nlevels > 2 ? levels[2] : nullptr);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;

@@ -228,7 +228,8 @@ __global__ __launch_bounds__(256) void k_normalize_inject(uint32_t *albedo, cons
         const bool lit = litmask[li] != 0;
         char *elem = pyr + L0.stage + (size_t)elem_index(L0, x, y, z, 0) * 16;
         // the brick's elements already hold +0 (wave-uniform)
-        const bool was0 = zero0 != nullptr && zero0[b] != 0;
+        const uint32_t z0 = zero0 != nullptr ? (uint32_t)zero0[b] : 0u;  // PyrDesc::zero0 bits
+        const bool was0 = (z0 & 1u) != 0u;
         float4 prev = make_float4(0.f, 0.f, 0.f, 0.f), prev1 = prev;
         if (TEMPORAL && !was0) {  // this element as stored last frame (texels x, x+1 for fp16 pairs)
             const uint4 e = *(const uint4 *)elem;
@@ -253,12 +254,16 @@ __global__ __launch_bounds__(256) void k_normalize_inject(uint32_t *albedo, cons
             lit_count += lit ? 1u : 0u;
         }
         if (rewrite && nw != a32) albedo[gi] = nw;
+        // every albedo word of the brick is 0 (a word that is 0 stays 0 under
+        // any later opacity rewrite)
+        const uint32_t za = __ballot(nw != 0u) == 0 ? 2u : 0u;
         if (FMT == 1) {
             const bool nz = (__float_as_uint(r.x) | __float_as_uint(r.y) | __float_as_uint(r.z) |
                              __float_as_uint(r.w)) != 0u;
             const bool now0 = __ballot(nz) == 0;
             if (!(now0 && was0)) *(float4 *)elem = r;  // an all-zero brick over zeros is not rewritten
-            if (zero0 != nullptr && lane == 0 && now0 != was0) zero0[b] = now0 ? 1 : 0;
+            const uint32_t z1 = (now0 ? 1u : 0u) | za;
+            if (zero0 != nullptr && lane == 0 && z1 != z0) zero0[b] = (uint8_t)z1;
             continue;
         }
         if (!LUT) h = pack_h4(r);
@@ -279,7 +284,8 @@ __global__ __launch_bounds__(256) void k_normalize_inject(uint32_t *albedo, cons
             if (x == 0)  // texel 0 is also the second half of element -1, in the border brick
                 *(uint2 *)(pyr + (uint32_t)(L0.stage + elem_index(L0, -1, y, z, 0) * 16u) + 8) = h;
         }
-        if (zero0 != nullptr && lane == 0 && now0 != was0) zero0[b] = now0 ? 1 : 0;
+        const uint32_t z1 = (now0 ? 1u : 0u) | za;
+        if (zero0 != nullptr && lane == 0 && z1 != z0) zero0[b] = (uint8_t)z1;
     }
     block_partial(uniq, maxf, lit_count, 0u, partials);
 }

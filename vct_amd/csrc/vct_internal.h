@@ -78,10 +78,12 @@ struct PyrDesc {
     LevelInfo lv[VCT_MAX_LEVELS];
     const LevelInfo *d_lv;  // device copy of lv[0..nlevels)
     // One byte per level-0 brick (x fastest, nb^3 rounded up to whole words):
-    // 1 = every element the injection writes for the brick (its 64 elements,
-    // and for bricks at x = 0 the second halves of the border elements
-    // before them) holds +0 in memory.  The injection keeps the bytes; any
-    // other writer of level 0 resets them to 0 (unknown).
+    // bit 0 = every element the injection writes for the brick (its 64
+    // elements, and for bricks at x = 0 the second halves of the border
+    // elements before them) holds +0 in memory; bit 1 = every word of the
+    // same brick of the albedo grid was 0 after the injection.  The
+    // injection keeps the bytes; any other writer of level 0 resets them to 0
+    // (unknown).
     uint8_t *zero0;
 };
 // Resets PyrDesc::zero0 to "unknown" (after a write of level 0 other than the injection).
@@ -253,7 +255,10 @@ hipError_t launch_fill_holes(const PyrDesc &pyr, void *tmp, hipStream_t s);
 // mip.hip
 hipError_t launch_mip(const PyrDesc &pyr, hipStream_t s);
 // the albedo chain: levels[1 .. nlevels) (linear RGBA8) from the bricked grid
-hipError_t launch_albedo_chain(const uint32_t *grid, uint32_t *const *levels, int dim, int nlevels, hipStream_t s);
+// nz: PyrDesc::zero0 when the albedo grid has not been written since the last
+// injection (its bit 1 then marks the grid's all-zero bricks), else null
+hipError_t launch_albedo_chain(const uint32_t *grid, const uint8_t *nz, uint32_t *const *levels, int dim, int nlevels,
+                               hipStream_t s);
 
 // trace.hip
 hipError_t launch_trace(const PyrDesc &pyr, const TraceLaunch &t, hipStream_t s);
