@@ -88,12 +88,13 @@ __device__ __forceinline__ bool zero16(const uint4 v[4]) {
 __device__ __forceinline__ void axis_terms_pad(float p, int s, uint32_t mul, uint32_t sub, uint32_t t[2], float w[2]) {
     const float u = p * (float)s - 0.5f;
     const float fl = __builtin_floorf(u);
-    const int i0 = (int)fl + 4, i1 = i0 + 1;  // texel -1 .. s, shifted
+    const int i0 = (int)fl + 4;  // texel -1 .. s-1, shifted
     const float f = u - fl;
     w[0] = 1.0f - f;
     w[1] = f;
     t[0] = __umul24((uint32_t)(i0 >> 2), mul) + (uint32_t)(i0 & 3) * sub;
-    t[1] = __umul24((uint32_t)(i1 >> 2), mul) + (uint32_t)(i1 & 3) * sub;
+    // texel i0 + 1: the next one in the brick, or the first of the next brick
+    t[1] = t[0] + ((i0 & 3) == 3 ? mul - 3u * sub : sub);
 }
 
 __device__ __forceinline__ float hlo(uint32_t v) { return __half2float(__ushort_as_half((uint16_t)(v & 0xffffu))); }
