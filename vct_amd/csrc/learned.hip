@@ -690,6 +690,36 @@ __device__ __forceinline__ void mlp_layer_t(f32x4 (&acc)[NT][4], const uint16_t 
     }
 }
 
+// The A fragments of k-step ks of W (rows 16 nt + lane % 16), loaded ahead of
+// the layer that uses them.
+template <int NT>
+__device__ __forceinline__ void load_wfrag(bf16x8 (&a)[NT], const uint16_t *W, int ldw, int ks) {
+    const int lane = threadIdx.x & 63;
+    const int r = lane & 15, kq = (lane >> 4) * 8;
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) a[nt] = *(const bf16x8 *)(W + (size_t)(nt * 16 + r) * ldw + ks * 32 + kq);
+}
+
+// mlp_layer_t with the fragments of k-step 0 already loaded (a0)
+template <int NT>
+__device__ __forceinline__ void mlp_layer_pre(f32x4 (&acc)[NT][4], const bf16x8 (&a0)[NT], const uint16_t *W, int ldw,
+                                              const uint16_t *X, int ldx, int nks) {
+    const int lane = threadIdx.x & 63;
+    const int r = lane & 15, kq = (lane >> 4) * 8;
+    for (int ks = 0; ks < nks; ++ks) {
+        bf16x8 b[4];
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) b[mt] = *(const bf16x8 *)(X + (mt * 16 + r) * ldx + ks * 32 + kq);
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+            const bf16x8 a = ks == 0 ? a0[nt] : *(const bf16x8 *)(W + (size_t)(nt * 16 + r) * ldw + ks * 32 + kq);
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt)
+                acc[nt][mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[mt], acc[nt][mt], 0, 0, 0);
+        }
+    }
+}
+
 // acc[nt][mt] := the bias of rows 16 nt + 4 (lane / 16) .. +3 (the rows this
 // lane's part of the 16x16 FP32 tile holds), so the layer's MFMAs add the
 // products onto it.
@@ -957,13 +987,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
 #pragma unroll
     for (int k = 1; k < 16; ++k) row[k] = 0u;
     __syncthreads();
+    // each layer's first weight fragments are loaded before the barriers and
+    // activation stores that precede the layer, so they arrive meanwhile
+    bf16x8 wn[4], w4n[1];
+    load_wfrag<4>(wn, W.w2, 64, 0);
     mlp_kstep(acc, W.w1, kMlpIn, 128, S.chunk, kCs);
     __syncthreads();  // every lane's chunk reads are done before the activations overwrite them
     store_relu_t(acc, S.hid);
     __syncthreads();
+#pragma unroll
     for (int layer = 1; layer <= 2; ++layer) {
         init_bias_t(acc, bias_ptr(W.master, layer));
-        mlp_layer_t<4>(acc, layer == 1 ? W.w2 : W.w3, 64, S.hid, kAs, 2);
+        mlp_layer_pre<4>(acc, wn, layer == 1 ? W.w2 : W.w3, 64, S.hid, kAs, 2);
+        if (layer == 1) load_wfrag<4>(wn, W.w3, 64, 0);
+        else load_wfrag<1>(w4n, W.w4, 64, 0);
         __syncthreads();
         store_relu_t(acc, S.hid);
         __syncthreads();
@@ -971,7 +1008,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
     f32x4 o[1][4];
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt) o[0][mt] = f32x4{0.f, 0.f, 0.f, 0.f};
-    mlp_layer_t<1>(o, W.w4, 64, S.hid, kAs, 2);
+    mlp_layer_pre<1>(o, w4n, W.w4, 64, S.hid, kAs, 2);
     if (lane < 16) {
         const float *b4 = bias_ptr(W.master, 3);
 #pragma unroll
