@@ -85,7 +85,7 @@ struct vct_ctx {
     bool have_voxels = false, have_radiance = false, have_mips = false;
     // bit 1 of PyrDesc::zero0 describes the albedo grid as it is (set by the
     // injection, cleared by every other writer of the grid)
-    bool albedo_nz_valid = false;
+    bool albedo_zero_valid = false;
     bool pyr_nonneg = true;  // level 0 was injected (not uploaded): no negative texels
     // raster scratch
     RasterScratch rs{};
@@ -230,7 +230,7 @@ static int alloc_grids(vct_ctx *c) {
     HIPCHK(c, dalloc(&c->d_grid_albedo, nv * 4));
     HIPCHK(c, dalloc(&c->d_grid_normal, nv * 4));
     HIPCHK(c, dalloc(&c->d_litmask, nv));
-    c->albedo_nz_valid = false;
+    c->albedo_zero_valid = false;
     HIPCHK(c, hipMemsetAsync(c->d_grid_albedo, 0, nv * 4, c->stream));
     HIPCHK(c, hipMemsetAsync(c->d_grid_normal, 0, nv * 4, c->stream));
     HIPCHK(c, hipMemsetAsync(c->d_litmask, 0, nv, c->stream));
@@ -840,7 +840,7 @@ int vct_voxelize(vct_ctx *c, const vct_voxelize_params *p) {
                                  c->d_grid_albedo, c->d_grid_normal, c->d_sums, c->d_stats, c->d_partials,
                                  p->warp_texture ? c->d_warpmap : nullptr, c->rs, c->stream));
     c->have_voxels = true;
-    c->albedo_nz_valid = false;
+    c->albedo_zero_valid = false;
     c->have_mips = false;
     c->have_sums = p->atomic_max == VCT_ACCUM_SUM;
     return VCT_OK;
@@ -857,7 +857,7 @@ int vct_voxelize_tessellated(vct_ctx *c, const vct_voxelize_params *p) {
                                       p->atomic_max, c->d_grid_albedo, c->d_grid_normal, c->d_stats, c->rs,
                                       c->stream));
     c->have_voxels = true;
-    c->albedo_nz_valid = false;
+    c->albedo_zero_valid = false;
     c->have_mips = false;
     c->have_sums = false;
     return VCT_OK;
@@ -870,7 +870,7 @@ int vct_normalize_voxels(vct_ctx *c, float opacity) {
     if (!c->have_sums)
         return fail(c, VCT_ERR_STATE, "vct_normalize_voxels: the last vct_voxelize did not accumulate sums");
     PassTimer pt(c, P_RADIANCE);
-    c->albedo_nz_valid = false;
+    c->albedo_zero_valid = false;
     LAUNCHCHK(c, launch_normalize_voxels(c->d_sums, c->d_grid_albedo, c->d_grid_normal, c->desc.dim, opacity, c->pyr,
                                          c->d_stats, c->d_partials, c->stream));
     c->have_radiance = true;
@@ -916,11 +916,11 @@ int vct_normalize_inject(vct_ctx *c, const vct_inject_params *p) {
     LAUNCHCHK(c, launch_inject_scatter(c->hlights.data(), c->d_lights, (int)c->lights.size(), p->light_mask,
                                        c->desc.dim, ext, p->warp_texture ? c->d_warpmap : nullptr, c->d_litmask,
                                        c->stream));
-    c->albedo_nz_valid = false;
+    c->albedo_zero_valid = false;
     LAUNCHCHK(c, launch_normalize_inject(c->d_grid_albedo, c->d_litmask, c->desc.dim, p->voxel_set_opacity,
                                          p->temporal_filter != 0, p->temporal_decay, c->pyr, c->d_stats,
                                          c->d_partials, c->stream));
-    c->albedo_nz_valid = true;
+    c->albedo_zero_valid = true;
     c->have_radiance = true;
     c->have_mips = false;
     // a plain injection rewrites every texel of level 0 from unorm8 albedo (no
@@ -955,11 +955,11 @@ int vct_normalize_inject_noalloc(vct_ctx *c, const vct_inject_params *p) {
     LAUNCHCHK(c, launch_inject_scatter(c->hlights.data(), c->d_lights, (int)c->lights.size(), p->light_mask,
                                        c->desc.dim, ext, p->warp_texture ? c->d_warpmap : nullptr, sc.litmask,
                                        c->stream));
-    c->albedo_nz_valid = false;
+    c->albedo_zero_valid = false;
     LAUNCHCHK(c, launch_normalize_inject(c->d_grid_albedo, sc.litmask, c->desc.dim, p->voxel_set_opacity,
                                          p->temporal_filter != 0, p->temporal_decay, c->pyr, c->d_stats,
                                          sc.partials, c->stream));
-    c->albedo_nz_valid = true;
+    c->albedo_zero_valid = true;
     c->have_radiance = true;
     c->have_mips = false;
     if (!p->temporal_filter) c->pyr_nonneg = true;
@@ -995,7 +995,7 @@ int vct_mip(vct_ctx *c, const vct_mip_params *p) {
     if (p->albedo_chain) {
         HIPCHK(c, hipEventRecord(c->fork, c->stream));
         HIPCHK(c, hipStreamWaitEvent(c->aux, c->fork, 0));
-        LAUNCHCHK(c, launch_albedo_chain(c->d_grid_albedo, c->albedo_nz_valid ? c->pyr.zero0 : nullptr, c->albedo_lv,
+        LAUNCHCHK(c, launch_albedo_chain(c->d_grid_albedo, c->albedo_zero_valid ? c->pyr.zero0 : nullptr, c->albedo_lv,
                                          c->desc.dim, c->desc.levels, c->aux));
         HIPCHK(c, hipEventRecord(c->join, c->aux));
     }
@@ -1287,7 +1287,7 @@ int vct_upload(vct_ctx *c, int what, int level, const void *host, size_t bytes) 
         HIPCHK(c, hipStreamSynchronize(c->stream));
         if (what == VCT_BUF_ALBEDO) {
             c->have_voxels = true;
-            c->albedo_nz_valid = false;
+            c->albedo_zero_valid = false;
         }
         return VCT_OK;
     }

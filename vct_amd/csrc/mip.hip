@@ -351,14 +351,14 @@ __global__ __launch_bounds__(256) void k_albedo_chain(AlbedoMipArgs a) {
 // before level 2 is filtered from it, and the taps are summed in the BOX2
 // order of filterRadiance.comp -- the same values as filtering level by level.
 __global__ __launch_bounds__(256) void k_albedo_bricks(const uint32_t *__restrict__ grid,
-                                                       const uint8_t *__restrict__ nz, int dim, uint32_t *out1,
+                                                       const uint8_t *__restrict__ zero0, int dim, uint32_t *out1,
                                                        uint32_t *out2) {
     const int nb = dim >> 2;
     const uint32_t b = blockIdx.x * 256u + threadIdx.x;
     if (b >= (uint32_t)nb * nb * nb) return;
     const int lg = __builtin_ctz((unsigned)nb);
     const int bx = (int)(b & (uint32_t)(nb - 1)), by = (int)((b >> lg) & (uint32_t)(nb - 1)), bz = (int)(b >> (2 * lg));
-    if (nz != nullptr && (nz[b] & 2u) != 0) {
+    if (zero0 != nullptr && (zero0[b] & 2u) != 0) {  // bit 1: every word of the brick is 0
         // every word of the brick is 0: the texels it filters to are
         // packUnorm4x8 of sums of zeros, 0, and the brick is not read
         const int s1 = dim >> 1;
@@ -420,12 +420,12 @@ __global__ __launch_bounds__(256) void k_albedo_bricks(const uint32_t *__restric
     out2[((size_t)bz * s2 + by) * s2 + bx] = pack_unorm4x8(v.x * 0.125f, v.y * 0.125f, v.z * 0.125f, v.w * 0.125f);
 }
 
-hipError_t launch_albedo_chain(const uint32_t *grid, const uint8_t *nz, uint32_t *const *levels, int dim, int nlevels,
+hipError_t launch_albedo_chain(const uint32_t *grid, const uint8_t *zero0, uint32_t *const *levels, int dim, int nlevels,
                                hipStream_t s) {
     if (nlevels < 2) return hipSuccess;
     {
         const unsigned nbricks = (unsigned)((dim >> 2) * (dim >> 2) * (dim >> 2));
-        hipLaunchKernelGGL(k_albedo_bricks, dim3((nbricks + 255) / 256), dim3(256), 0, s, grid, nz, dim, levels[1],
+        hipLaunchKernelGGL(k_albedo_bricks, dim3((nbricks + 255) / 256), dim3(256), 0, s, grid, zero0, dim, levels[1],
                            nlevels > 2 ? levels[2] : nullptr);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;

@@ -255,9 +255,9 @@ hipError_t launch_fill_holes(const PyrDesc &pyr, void *tmp, hipStream_t s);
 // mip.hip
 hipError_t launch_mip(const PyrDesc &pyr, hipStream_t s);
 // the albedo chain: levels[1 .. nlevels) (linear RGBA8) from the bricked grid
-// nz: PyrDesc::zero0 when the albedo grid has not been written since the last
+// zero0: PyrDesc::zero0 when the albedo grid has not been written since the last
 // injection (its bit 1 then marks the grid's all-zero bricks), else null
-hipError_t launch_albedo_chain(const uint32_t *grid, const uint8_t *nz, uint32_t *const *levels, int dim, int nlevels,
+hipError_t launch_albedo_chain(const uint32_t *grid, const uint8_t *zero0, uint32_t *const *levels, int dim, int nlevels,
                                hipStream_t s);
 
 // trace.hip
