@@ -84,3 +84,58 @@ def test_mip_chain_bit_exact_over_frames(aniso):
         assert (~e[:-1] & e[1:]).any()     # emptied after being filled
     finally:
         app.close()
+
+
+def test_voxel_grids_bit_exact_over_frames():
+    """The voxelize pass clears both voxel grids (Application.cpp:686-687)
+    except the bricks the last injection found empty behind the standard
+    voxelizer; the grids of every frame are bit-exact against the oracle's
+    voxelization, with bricks emptied and refilled between frames, after a
+    tessellated voxelization (whose albedo words can be 0 where it wrote a
+    normal, so the next clear is whole) and after a second injection of the
+    same grids."""
+    app = make_app(2, dim=128, levels=7, width=256, height=144, shadow_size=256)
+    try:
+        ctx = app.ctx
+        g = ctx.grid()
+        shift = (g.vmax[0] - g.vmin[0]) / 3.0
+        mesh = app.mesh()
+
+        def frame(k, dx, tess=False, injections=1):
+            fs = app.frame_state(k)
+            model = fs["model"].copy()
+            model[:, 12] += dx
+            ctx.set_transforms(model)
+            sms = []
+            for i, L in enumerate(fs["lights"]):
+                if L.shadow_caster:
+                    ctx.shadowmap(i)
+                    sms.append(ctx.shadow(i))
+                else:
+                    sms.append(None)
+            sc = ob.SceneArrays(mesh, model)
+            vox = fs["vox"]
+            vox.atomic_max = abi.VCT_ACCUM_MAX
+            print(f"voxel grids: frame {k}, shift {dx:.3f}, tessellated {tess}", flush=True)
+            if tess:
+                ctx.voxelize_tessellated(vox)
+                ralb, rnrm = ob.voxelize_tess(sc, g.dim, tuple(g.center), tuple(g.vmin), tuple(g.vmax), accum=1)
+            else:
+                ctx.voxelize(vox)
+                ralb, rnrm, _ = ob.voxelize(sc, fs["lights"], sms, vox, g.dim)
+            alb, nrm = ctx.albedo(), ctx.normal()
+            assert np.array_equal(alb, ralb), (k, int((alb != ralb).sum()))
+            assert np.array_equal(nrm, rnrm), (k, int((nrm != rnrm).sum()))
+            for _ in range(injections):
+                ctx.normalize_inject(fs["inj"])
+            return (alb != 0).reshape(16, 8, 16, 8, 16, 8).any(axis=(1, 3, 5))
+
+        occ = [frame(k, shift * (k % 2)) for k in range(4)]
+        occ.append(frame(4, 0.0, tess=True))
+        occ.append(frame(5, shift))
+        occ.append(frame(6, 0.0, injections=2))
+        occ.append(frame(7, shift))
+        e = ~np.stack(occ)
+        assert (e[:-1] & ~e[1:]).any() and (~e[:-1] & e[1:]).any()  # regions filled and emptied
+    finally:
+        app.close()

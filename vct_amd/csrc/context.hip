@@ -86,6 +86,11 @@ struct vct_ctx {
     // bit 1 of PyrDesc::zero0 describes the albedo grid as it is (set by the
     // injection, cleared by every other writer of the grid)
     bool albedo_zero_valid = false;
+    // bit 2 describes both voxel grids as they are: set by an injection that
+    // follows the standard voxelizer in the max or average mode (every voxel it
+    // writes gets a nonzero albedo word, alpha 1 or a count), cleared by every
+    // other writer of the grids
+    bool grids_std_vox = false, grid_zero_valid = false;
     bool pyr_nonneg = true;  // level 0 was injected (not uploaded): no negative texels
     // raster scratch
     RasterScratch rs{};
@@ -231,6 +236,7 @@ static int alloc_grids(vct_ctx *c) {
     HIPCHK(c, dalloc(&c->d_grid_normal, nv * 4));
     HIPCHK(c, dalloc(&c->d_litmask, nv));
     c->albedo_zero_valid = false;
+    c->grids_std_vox = c->grid_zero_valid = false;
     HIPCHK(c, hipMemsetAsync(c->d_grid_albedo, 0, nv * 4, c->stream));
     HIPCHK(c, hipMemsetAsync(c->d_grid_normal, 0, nv * 4, c->stream));
     HIPCHK(c, hipMemsetAsync(c->d_litmask, 0, nv, c->stream));
@@ -836,9 +842,12 @@ int vct_voxelize(vct_ctx *c, const vct_voxelize_params *p) {
         HIPCHK(c, dalloc(&c->d_sums, nv * 8 * sizeof(uint32_t)));
     }
     PassTimer pt(c, P_VOXELIZE);
+    const uint8_t *zg = c->grid_zero_valid ? c->pyr.zero0 : nullptr;
+    c->grids_std_vox = c->grid_zero_valid = false;
     LAUNCHCHK(c, launch_voxelize(dev_scene(c), c->d_lights, (int)c->lights.size(), *p, c->desc.dim,
-                                 c->d_grid_albedo, c->d_grid_normal, c->d_sums, c->d_stats, c->d_partials,
+                                 c->d_grid_albedo, c->d_grid_normal, zg, c->d_sums, c->d_stats, c->d_partials,
                                  p->warp_texture ? c->d_warpmap : nullptr, c->rs, c->stream));
+    c->grids_std_vox = p->atomic_max != VCT_ACCUM_SUM;
     c->have_voxels = true;
     c->albedo_zero_valid = false;
     c->have_mips = false;
@@ -853,8 +862,10 @@ int vct_voxelize_tessellated(vct_ctx *c, const vct_voxelize_params *p) {
                     p->atomic_max);
     if (!c->d_model) return fail(c, VCT_ERR_STATE, "vct_voxelize_tessellated: no scene");
     PassTimer pt(c, P_VOXELIZE);
+    const uint8_t *zg = c->grid_zero_valid ? c->pyr.zero0 : nullptr;
+    c->grids_std_vox = c->grid_zero_valid = false;  // its albedo alpha is the map's: a written voxel may read 0
     LAUNCHCHK(c, launch_voxelize_tess(dev_scene(c), c->desc.center, c->desc.vmin, c->desc.vmax, c->desc.dim,
-                                      p->atomic_max, c->d_grid_albedo, c->d_grid_normal, c->d_stats, c->rs,
+                                      p->atomic_max, c->d_grid_albedo, c->d_grid_normal, zg, c->d_stats, c->rs,
                                       c->stream));
     c->have_voxels = true;
     c->albedo_zero_valid = false;
@@ -871,6 +882,7 @@ int vct_normalize_voxels(vct_ctx *c, float opacity) {
         return fail(c, VCT_ERR_STATE, "vct_normalize_voxels: the last vct_voxelize did not accumulate sums");
     PassTimer pt(c, P_RADIANCE);
     c->albedo_zero_valid = false;
+    c->grids_std_vox = c->grid_zero_valid = false;
     LAUNCHCHK(c, launch_normalize_voxels(c->d_sums, c->d_grid_albedo, c->d_grid_normal, c->desc.dim, opacity, c->pyr,
                                          c->d_stats, c->d_partials, c->stream));
     c->have_radiance = true;
@@ -921,6 +933,10 @@ int vct_normalize_inject(vct_ctx *c, const vct_inject_params *p) {
                                          p->temporal_filter != 0, p->temporal_decay, c->pyr, c->d_stats,
                                          c->d_partials, c->stream));
     c->albedo_zero_valid = true;
+    // only the first injection after the voxelizer sees its words (a later one
+    // reads them after the opacity rewrite)
+    c->grid_zero_valid = c->grids_std_vox;
+    c->grids_std_vox = false;
     c->have_radiance = true;
     c->have_mips = false;
     // a plain injection rewrites every texel of level 0 from unorm8 albedo (no
@@ -960,6 +976,10 @@ int vct_normalize_inject_noalloc(vct_ctx *c, const vct_inject_params *p) {
                                          p->temporal_filter != 0, p->temporal_decay, c->pyr, c->d_stats,
                                          sc.partials, c->stream));
     c->albedo_zero_valid = true;
+    // only the first injection after the voxelizer sees its words (a later one
+    // reads them after the opacity rewrite)
+    c->grid_zero_valid = c->grids_std_vox;
+    c->grids_std_vox = false;
     c->have_radiance = true;
     c->have_mips = false;
     if (!p->temporal_filter) c->pyr_nonneg = true;
@@ -1285,6 +1305,7 @@ int vct_upload(vct_ctx *c, int what, int level, const void *host, size_t bytes) 
                                          what == VCT_BUF_ALBEDO ? c->d_grid_albedo : c->d_grid_normal, c->desc.dim,
                                          false, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
+        c->grids_std_vox = c->grid_zero_valid = false;
         if (what == VCT_BUF_ALBEDO) {
             c->have_voxels = true;
             c->albedo_zero_valid = false;

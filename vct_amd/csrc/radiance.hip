@@ -255,8 +255,8 @@ __global__ __launch_bounds__(256) void k_normalize_inject(uint32_t *albedo, cons
         }
         if (rewrite && nw != a32) albedo[gi] = nw;
         // every albedo word of the brick is 0 (a word that is 0 stays 0 under
-        // any later opacity rewrite)
-        const uint32_t za = __ballot(nw != 0u) == 0 ? 2u : 0u;
+        // any later opacity rewrite); and was 0 as the voxelizer left it
+        const uint32_t za = (__ballot(nw != 0u) == 0 ? 2u : 0u) | (__ballot(a32 != 0u) == 0 ? 4u : 0u);
         if (FMT == 1) {
             const bool nz = (__float_as_uint(r.x) | __float_as_uint(r.y) | __float_as_uint(r.z) |
                              __float_as_uint(r.w)) != 0u;

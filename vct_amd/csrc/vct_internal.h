@@ -81,7 +81,9 @@ struct PyrDesc {
     // bit 0 = every element the injection writes for the brick (its 64
     // elements, and for bricks at x = 0 the second halves of the border
     // elements before them) holds +0 in memory; bit 1 = every word of the
-    // same brick of the albedo grid was 0 after the injection.  The
+    // same brick of the albedo grid was 0 after the injection; bit 2 = it was
+    // 0 before the injection's opacity rewrite, as the voxelizer left it
+    // (where the standard voxelizer wrote nothing to either grid).  The
     // injection keeps the bytes; any other writer of level 0 resets them to 0
     // (unknown).
     uint8_t *zero0;
@@ -226,14 +228,18 @@ hipError_t launch_gbuffer(const DevScene &sc, const vct_camera &cam, uint64_t *v
                           RasterScratch &rs, hipStream_t s);
 
 // voxelize.hip
+// zero_grids (here and for the tessellation voxelizer): PyrDesc::zero0 when its
+// bit 2 describes both grids as they are (the grids' clear then skips those
+// bricks), else null
 hipError_t launch_voxelize(const DevScene &sc, const DevLight *dlights, int nlights, const vct_voxelize_params &p,
-                           int dim, uint32_t *albedo, uint32_t *normal, uint32_t *sums, vct_stats *stats,
-                           uint32_t *partials, const float4 *warp, RasterScratch &rs, hipStream_t s);
+                           int dim, uint32_t *albedo, uint32_t *normal, const uint8_t *zero_grids, uint32_t *sums,
+                           vct_stats *stats, uint32_t *partials, const float4 *warp, RasterScratch &rs,
+                           hipStream_t s);
 // the tessellation voxelizer (accum: VCT_ACCUM_MAX or VCT_ACCUM_AVG) into the
 // brick-ordered albedo / normal grids of a dim^3 volume with the given extent
 hipError_t launch_voxelize_tess(const DevScene &sc, const float center[3], const float vmin[3], const float vmax[3],
-                                int dim, int accum, uint32_t *albedo, uint32_t *normal, vct_stats *stats,
-                                RasterScratch &rs, hipStream_t s);
+                                int dim, int accum, uint32_t *albedo, uint32_t *normal, const uint8_t *zero_grids,
+                                vct_stats *stats, RasterScratch &rs, hipStream_t s);
 // normalizeVoxels.comp for the sum-accumulate mode (sums: 8 words per voxel)
 hipError_t launch_normalize_voxels(const uint32_t *sums, uint32_t *albedo, uint32_t *normal, int dim, float opacity,
                                    const PyrDesc &pyr, vct_stats *stats, uint32_t *partials, hipStream_t s);

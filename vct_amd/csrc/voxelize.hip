@@ -391,13 +391,33 @@ hipError_t launch_warp_occupancy(const DevScene &sc, const vct_voxelize_params &
     return hipGetLastError();
 }
 
+// glClearTexImage of both voxel grids (Application.cpp:627-628, 686-687): 16
+// bytes of each grid per thread.  zero (or null): the zero record, whose bit 2
+// marks the bricks that hold 0 in both grids already; they are not rewritten.
+__global__ __launch_bounds__(256) void k_clear_grids(uint32_t *albedo, uint32_t *normal, size_t nchunks,
+                                                     const uint8_t *__restrict__ zero) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nchunks; i += (size_t)gridDim.x * blockDim.x) {
+        if (zero != nullptr && (zero[i >> 4] & 4u) != 0) continue;  // 16 chunks of 16 bytes per 4^3 brick
+        ((uint4 *)albedo)[i] = make_uint4(0u, 0u, 0u, 0u);
+        ((uint4 *)normal)[i] = make_uint4(0u, 0u, 0u, 0u);
+    }
+}
+
+hipError_t launch_clear_grids(uint32_t *albedo, uint32_t *normal, int dim, const uint8_t *zero, hipStream_t s) {
+    const size_t nchunks = (size_t)dim * dim * dim / 4;
+    const size_t want = (nchunks + 255) / 256;
+    const unsigned blocks = (unsigned)(want < 8192 ? want : 8192);
+    hipLaunchKernelGGL(k_clear_grids, dim3(blocks), dim3(256), 0, s, albedo, normal, nchunks, zero);
+    return hipGetLastError();
+}
+
 hipError_t launch_voxelize(const DevScene &sc, const DevLight *dlights, int nlights, const vct_voxelize_params &p,
-                           int dim, uint32_t *albedo, uint32_t *normal, uint32_t *sums, vct_stats *stats,
-                           uint32_t *partials, const float4 *warp, RasterScratch &rs, hipStream_t s) {
+                           int dim, uint32_t *albedo, uint32_t *normal, const uint8_t *zero_grids, uint32_t *sums,
+                           vct_stats *stats, uint32_t *partials, const float4 *warp, RasterScratch &rs,
+                           hipStream_t s) {
     const size_t nv = (size_t)dim * dim * dim;
-    hipError_t e = launch_fill_u32(albedo, 0u, nv, s);  // glClearTexImage (Application.cpp:686-687)
+    hipError_t e = launch_clear_grids(albedo, normal, dim, zero_grids, s);
     if (e != hipSuccess) return e;
-    if ((e = launch_fill_u32(normal, 0u, nv, s)) != hipSuccess) return e;
     if (p.atomic_max == VCT_ACCUM_SUM && (e = launch_fill_u32(sums, 0u, nv * 8, s)) != hipSuccess) return e;
     if (sc.ntri == 0) return launch_fill_u32(&stats->total_voxel_fragments, 0u, 1, s);
     VoxMats vm;
@@ -669,12 +689,11 @@ __global__ __launch_bounds__(256) void k_tess_points(DevScene sc, TessArgs ta, c
 }
 
 hipError_t launch_voxelize_tess(const DevScene &sc, const float center[3], const float vmin[3], const float vmax[3],
-                                int dim, int accum, uint32_t *albedo, uint32_t *normal, vct_stats *stats,
-                                RasterScratch &rs, hipStream_t s) {
+                                int dim, int accum, uint32_t *albedo, uint32_t *normal, const uint8_t *zero_grids,
+                                vct_stats *stats, RasterScratch &rs, hipStream_t s) {
     const size_t nv = (size_t)dim * dim * dim;
-    hipError_t e = launch_fill_u32(albedo, 0u, nv, s);  // glClearTexImage (Application.cpp:627-628)
+    hipError_t e = launch_clear_grids(albedo, normal, dim, zero_grids, s);
     if (e != hipSuccess) return e;
-    if ((e = launch_fill_u32(normal, 0u, nv, s)) != hipSuccess) return e;
     // the voxelizeInfo buffer is cleared (Application.cpp:581) and this path never counts
     if ((e = launch_fill_u32(&stats->total_voxel_fragments, 0u, 1, s)) != hipSuccess) return e;
     if (sc.ntri == 0) return hipSuccess;
