@@ -47,8 +47,8 @@ PEAK_HBM_GBS = 8000.0
 #    / (1024 SIMDs x kernel cycles) -- the fraction of SIMD cycles issuing VALU,
 #    the ceiling this kernel actually runs against.
 TRACE_PROFILE = "profiles/counters_trace_scalar_path.md"
-TRACE_FETCH_KIB, TRACE_WRITE_KIB = 67551, 64801
-TRACE_VALU_PER_WAVE, TRACE_VALU_BUSY = 23806, 0.96
+TRACE_FETCH_KIB, TRACE_WRITE_KIB = 67552, 64801
+TRACE_VALU_PER_WAVE, TRACE_VALU_BUSY = 23788, 0.95
 METRIC = "ms/frame voxelize+inject+mip+trace @256^3,1080p; GB/s vs peak; learned vs plain"
 
 
