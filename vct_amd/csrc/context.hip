@@ -844,8 +844,15 @@ int vct_voxelize(vct_ctx *c, const vct_voxelize_params *p) {
     PassTimer pt(c, P_VOXELIZE);
     const uint8_t *zg = c->grid_zero_valid ? c->pyr.zero0 : nullptr;
     c->grids_std_vox = c->grid_zero_valid = false;
+    // the grids' clear runs on the second stream beside the triangle setup and
+    // the scan, from a fork after everything enqueued so far; the fragment
+    // walk waits for it
+    HIPCHK(c, hipEventRecord(c->fork, c->stream));
+    HIPCHK(c, hipStreamWaitEvent(c->aux, c->fork, 0));
+    LAUNCHCHK(c, launch_clear_grids(c->d_grid_albedo, c->d_grid_normal, c->desc.dim, zg, c->aux));
+    HIPCHK(c, hipEventRecord(c->join, c->aux));
     LAUNCHCHK(c, launch_voxelize(dev_scene(c), c->d_lights, (int)c->lights.size(), *p, c->desc.dim,
-                                 c->d_grid_albedo, c->d_grid_normal, zg, c->d_sums, c->d_stats, c->d_partials,
+                                 c->d_grid_albedo, c->d_grid_normal, c->join, c->d_sums, c->d_stats, c->d_partials,
                                  p->warp_texture ? c->d_warpmap : nullptr, c->rs, c->stream));
     c->grids_std_vox = p->atomic_max != VCT_ACCUM_SUM;
     c->have_voxels = true;

@@ -228,13 +228,17 @@ hipError_t launch_gbuffer(const DevScene &sc, const vct_camera &cam, uint64_t *v
                           RasterScratch &rs, hipStream_t s);
 
 // voxelize.hip
-// zero_grids (here and for the tessellation voxelizer): PyrDesc::zero0 when its
-// bit 2 describes both grids as they are (the grids' clear then skips those
-// bricks), else null
+// The clear of both voxel grids (glClearTexImage); zero: PyrDesc::zero0 when
+// its bit 2 describes both grids as they are (those bricks are then skipped),
+// else null.
+hipError_t launch_clear_grids(uint32_t *albedo, uint32_t *normal, int dim, const uint8_t *zero, hipStream_t s);
+// The voxelizer; the grids are cleared by the caller, and grids_ready is
+// recorded after that clear (the fragment walk waits for it).
 hipError_t launch_voxelize(const DevScene &sc, const DevLight *dlights, int nlights, const vct_voxelize_params &p,
-                           int dim, uint32_t *albedo, uint32_t *normal, const uint8_t *zero_grids, uint32_t *sums,
+                           int dim, uint32_t *albedo, uint32_t *normal, hipEvent_t grids_ready, uint32_t *sums,
                            vct_stats *stats, uint32_t *partials, const float4 *warp, RasterScratch &rs,
                            hipStream_t s);
+// zero_grids: as launch_clear_grids' zero
 // the tessellation voxelizer (accum: VCT_ACCUM_MAX or VCT_ACCUM_AVG) into the
 // brick-ordered albedo / normal grids of a dim^3 volume with the given extent
 hipError_t launch_voxelize_tess(const DevScene &sc, const float center[3], const float vmin[3], const float vmax[3],

@@ -412,14 +412,16 @@ hipError_t launch_clear_grids(uint32_t *albedo, uint32_t *normal, int dim, const
 }
 
 hipError_t launch_voxelize(const DevScene &sc, const DevLight *dlights, int nlights, const vct_voxelize_params &p,
-                           int dim, uint32_t *albedo, uint32_t *normal, const uint8_t *zero_grids, uint32_t *sums,
+                           int dim, uint32_t *albedo, uint32_t *normal, hipEvent_t grids_ready, uint32_t *sums,
                            vct_stats *stats, uint32_t *partials, const float4 *warp, RasterScratch &rs,
                            hipStream_t s) {
     const size_t nv = (size_t)dim * dim * dim;
-    hipError_t e = launch_clear_grids(albedo, normal, dim, zero_grids, s);
-    if (e != hipSuccess) return e;
+    hipError_t e = hipSuccess;
     if (p.atomic_max == VCT_ACCUM_SUM && (e = launch_fill_u32(sums, 0u, nv * 8, s)) != hipSuccess) return e;
-    if (sc.ntri == 0) return launch_fill_u32(&stats->total_voxel_fragments, 0u, 1, s);
+    if (sc.ntri == 0) {
+        if ((e = hipStreamWaitEvent(s, grids_ready, 0)) != hipSuccess) return e;
+        return launch_fill_u32(&stats->total_voxel_fragments, 0u, 1, s);
+    }
     VoxMats vm;
     for (int i = 0; i < 16; ++i) {
         vm.mvp[0][i] = p.mvp_x[i];
@@ -449,6 +451,7 @@ hipError_t launch_voxelize(const DevScene &sc, const DevLight *dlights, int nlig
     f.frags = 0;
     static_assert(kRasterBlocks <= kStatPartials, "one statistics partial per rasterizer workgroup");
     const int nblk = raster_grid<VoxFrag, true>();
+    if ((e = hipStreamWaitEvent(s, grids_ready, 0)) != hipSuccess) return e;  // the grids are clear
     hipLaunchKernelGGL(k_raster_queued<VoxFrag>, dim3(nblk), dim3(256), 0, s, rs.rec, rs.offs, rs.chunk,
                        rs.chunk_cap, sc.ntri, f);
     if ((e = hipGetLastError()) != hipSuccess) return e;
